@@ -1,0 +1,412 @@
+"""Python mirror of B12's ImplicitBoxTree / CooMatrix API over the b12x C ABI.
+
+Names, argument meaning and return values follow the reference's C++ API
+(B12c/src/include/ImplicitBoxTree.h, CooMatrix.h, Grids.h, Map_*.h). Everything runs in the
+HIP library libb12x.so; there is no CPU fallback — a missing library or GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+NONE, HIT, INS, EXPD, SD, EQU, ENTR, EXTR, ALL = 0x00, 0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0xFF
+LEAF = 255
+INVALID = 0xFFFFFFFF
+
+GRID_INNER, GRID_UNIFORM = 0, 1
+MAP_IDENTITY, MAP_HENON, MAP_LORENZ, MAP_HENON_HEILES, MAP_DUFFING = 0, 1, 2, 3, 4
+
+_P = ctypes.c_void_p
+_U8 = ctypes.c_uint8
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+# name -> argtypes (every entry point returns int status)
+SIGNATURES = {
+    "b12x_device_name": [ctypes.c_char_p, _I],
+    "b12x_tree_create": [_I, _P, _P, _P, _I, _P],
+    "b12x_tree_destroy": [_P],
+    "b12x_size": [_P, _P],
+    "b12x_count": [_P, _U8, _P],
+    "b12x_depth": [_P, _P],
+    "b12x_tree_download": [_P, _P, _P, _P],
+    "b12x_tree_upload": [_P, _U32, _P, _P, _P],
+    "b12x_subdivide": [_P, _U8, _P],
+    "b12x_unsubdivide": [_P, _U8, _P],
+    "b12x_remove": [_P, _U8, _P],
+    "b12x_set_flags": [_P, _U8, _U8, _P],
+    "b12x_unset_flags": [_P, _U8, _U8, _P],
+    "b12x_change_flags": [_P, _U8, _U8, _U8, _P],
+    "b12x_flags_by_stencil": [_P, _I, _P, _U8, _U8, _U8, _P],
+    "b12x_flags_by_image": [_P, _P, _U8, _U8, _I, _U8, _U8, _U8, _P],
+    "b12x_num_points": [_P, _P, _U8, _U8, _P],
+    "b12x_search_by_image": [_P, _P, _U8, _U8, _U8, _P],
+    "b12x_search_compacted": [_P, _P, _U8, _U8, _U8, _P, _P],
+    "b12x_search_by_points": [_P, _P, _U64, _U8, _P],
+    "b12x_subdivision_steps": [_P, _P, _U8, _I, _P],
+    "b12x_tree_image_timing": [_P, _I, _P, _P, _P],
+    "b12x_scheme_create": [_I, _I, _P, _I, _I, _P, _I, _P],
+    "b12x_scheme_destroy": [_P],
+    "b12x_scheme_points_per_box": [_P, _P],
+    "b12x_map_points": [_P, _P, _P, _U64],
+    "b12x_transition_matrix": [_P, _P, _U8, _U8, _U8, _I, _I, _I, _P],
+    "b12x_coo_destroy": [_P],
+    "b12x_coo_info": [_P, _P, _P, _P, _P],
+    "b12x_coo_download": [_P, _P, _P, _P],
+    "b12x_coo_upload": [_P, _P, _P, _U64, _U32, _U32, _P],
+    "b12x_compress_index_vector": [_P, _U64, _U32, _P],
+    "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
+}
+
+_lib = None
+
+
+def library_path(flavour="parity"):
+    name = "libb12x.so" if flavour == "parity" else "libb12x_fast.so"
+    return os.path.join(HERE, name)
+
+
+def load(flavour="parity"):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and flavour == "parity":
+        return _lib
+    path = library_path(flavour)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m b12_amd.build` (no CPU fallback exists)")
+    lib = ctypes.CDLL(path)
+    lib.b12x_last_error.restype = ctypes.c_char_p
+    lib.b12x_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = _I
+    if flavour == "parity":
+        _lib = lib
+    return lib
+
+
+class B12xError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+class _Api:
+    def __init__(self, flavour="parity"):
+        self.lib = load(flavour)
+
+    def __call__(self, name, *args):
+        rc = getattr(self.lib, name)(*args)
+        if rc != 0:
+            raise B12xError(f"{name}: {self.lib.b12x_last_error().decode()} (status {rc})")
+
+
+def device_name(flavour="parity"):
+    buf = ctypes.create_string_buffer(256)
+    _Api(flavour)("b12x_device_name", buf, 256)
+    return buf.value.decode()
+
+
+# ---- functors (Grids.h, Map_*.h): kind + parameters, compiled into the library --------------
+class InnerGrid:
+    """InnerGrid<DIM>(nPointsPerDim) — Grids.hpp:99-134"""
+
+    def __init__(self, n=1):
+        self.kind, self.params = GRID_INNER, [int(n)]
+        self.n = int(n)
+
+    def points_per_box(self, dim):
+        return self.n ** dim
+
+
+class UniformGrid:
+    """UniformGrid<DIM>(nPoints, seed), host (std::mt19937) semantics — Grids.hpp:136-176"""
+
+    def __init__(self, k=1, seed=0):
+        self.kind, self.params = GRID_UNIFORM, [int(k), int(seed)]
+
+    def points_per_box(self, dim):
+        return self.params[0]
+
+
+class _Map:
+    def __init__(self, kind, dim, params):
+        self.kind, self.dim, self.params = kind, dim, [float(p) for p in params]
+
+
+def Identity(dim):
+    return _Map(MAP_IDENTITY, dim, [])
+
+
+def Henon(a=1.4, b=0.3, steps=1):
+    return _Map(MAP_HENON, 2, [a, b, steps])
+
+
+def Lorenz(sigma=10.0, rho=28.0, beta=8.0 / 3.0, h=0.01, steps=20):
+    return _Map(MAP_LORENZ, 3, [sigma, rho, beta, h, steps])
+
+
+def HenonHeiles(order=2, h=0.02, steps=50):
+    return _Map(MAP_HENON_HEILES, 4, [order, h, steps])
+
+
+def Duffing(delta=0.25, alpha=-1.0, beta=1.0, gamma=0.3, omega=1.0, t0=0.0, t_end=2 * np.pi, h=2 * np.pi / 100):
+    return _Map(MAP_DUFFING, 2, [delta, alpha, beta, gamma, omega, t0, t_end, h])
+
+
+class Scheme:
+    """A (Grid, Map) pair on the device."""
+
+    def __init__(self, dim, grid, mp, flavour="parity"):
+        self.api = _Api(flavour)
+        gp = np.asarray(grid.params, dtype=np.uint64)
+        mpar = np.asarray(mp.params if mp.params else [0.0], dtype=np.float64)
+        h = _P()
+        self.api("b12x_scheme_create", dim, grid.kind, _ptr(gp), len(grid.params), mp.kind, _ptr(mpar),
+                 len(mp.params), ctypes.byref(h))
+        self.h = h
+        self.dim = dim
+        k = _U64()
+        self.api("b12x_scheme_points_per_box", self.h, ctypes.byref(k))
+        self.k = k.value
+
+    def map_points(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, self.dim)
+        y = np.empty_like(x)
+        self.api("b12x_map_points", self.h, _ptr(x), _ptr(y), x.shape[0])
+        return y
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.api.lib.b12x_scheme_destroy(self.h)
+            self.h = None
+
+
+class PointIterator:
+    """The lazy point range of makePointIteratorPair* (ImplicitBoxTree.h:246-430)."""
+
+    def __init__(self, scheme, depth, flags_to_check=NONE):
+        self.scheme, self.depth, self.flags_to_check = scheme, depth & 0xFF, flags_to_check
+
+
+class CooMatrix:
+    """CooMatrix<A, double> (CooMatrix.h:56-121) with its arrays resident on the GPU."""
+
+    def __init__(self, handle, api):
+        self.h, self.api = handle, api
+        nnz, nr, nc, rm = _U64(), _U32(), _U32(), _I()
+        self.api("b12x_coo_info", self.h, ctypes.byref(nnz), ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(rm))
+        self.nnz, self.nRows, self.nColumns, self.row_major = nnz.value, nr.value, nc.value, bool(rm.value)
+
+    @classmethod
+    def from_arrays(cls, rows, cols, vals, n_rows, n_cols, flavour="parity"):
+        api = _Api(flavour)
+        rows = np.ascontiguousarray(rows, dtype=np.uint32)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        h = _P()
+        api("b12x_coo_upload", _ptr(rows), _ptr(cols), _ptr(vals), len(rows), n_rows, n_cols, ctypes.byref(h))
+        return cls(h, api)
+
+    def download(self):
+        r = np.empty(self.nnz, np.uint32)
+        c = np.empty(self.nnz, np.uint32)
+        v = np.empty(self.nnz, np.float64)
+        self.api("b12x_coo_download", self.h, _ptr(r), _ptr(c), _ptr(v))
+        return r, c, v
+
+    def power_iteration(self, max_iter=200, tol=-1.0, mu=None):
+        mu_arr = np.zeros(self.nRows, np.float64) if mu is None else np.ascontiguousarray(mu, np.float64).copy()
+        it, res = _I(), _D()
+        self.api("b12x_power_iteration", self.h, int(max_iter), float(tol), 0 if mu is None else 1, _ptr(mu_arr),
+                 ctypes.byref(it), ctypes.byref(res))
+        return mu_arr, it.value, res.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.api.lib.b12x_coo_destroy(self.h)
+            self.h = None
+
+
+def compressIndexVector(indices, dim, flavour="parity"):
+    """compressIndexVector<A>(indices, dim) — CooMatrix.hpp:14-56"""
+    api = _Api(flavour)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    out = np.empty(dim + 1, np.uint32)
+    api("b12x_compress_index_vector", _ptr(idx), len(idx), dim, _ptr(out))
+    return out
+
+
+def _depth(d):
+    return d & 0xFF
+
+
+class ImplicitBoxTree:
+    """ImplicitBoxTree<MI355X, DIM, double, 64> — ImplicitBoxTree.h:32-640."""
+
+    def __init__(self, center, radius, sd_scheme=None, flavour="parity"):
+        self.api = _Api(flavour)
+        self.flavour = flavour
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        r = np.ascontiguousarray(radius, dtype=np.float64)
+        self.dim = len(c)
+        sd = None if sd_scheme is None else np.ascontiguousarray(sd_scheme, dtype=np.uint8)
+        h = _P()
+        self.api("b12x_tree_create", self.dim, _ptr(c), _ptr(r), _ptr(sd), 0 if sd is None else len(sd),
+                 ctypes.byref(h))
+        self.h = h
+        self._schemes = {}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.api.lib.b12x_tree_destroy(self.h)
+            self.h = None
+
+    def _ret(self, name, *args):
+        r = _U32()
+        self.api(name, self.h, *args, ctypes.byref(r))
+        return r.value
+
+    # -- queries
+    def size(self):
+        return self._ret("b12x_size")
+
+    def count(self, depth=-1):
+        return self._ret("b12x_count", _depth(depth))
+
+    def depth(self):
+        r = _U8()
+        self.api("b12x_depth", self.h, ctypes.byref(r))
+        return r.value
+
+    def download(self):
+        n = self.size()
+        b = np.empty(n, np.uint64)
+        d = np.empty(n, np.uint8)
+        f = np.empty(n, np.uint8)
+        self.api("b12x_tree_download", self.h, _ptr(b), _ptr(d), _ptr(f))
+        return b, d, f
+
+    def upload(self, bits, depth, flags):
+        b = np.ascontiguousarray(bits, np.uint64)
+        d = np.ascontiguousarray(depth, np.uint8)
+        f = np.ascontiguousarray(flags, np.uint8)
+        self.api("b12x_tree_upload", self.h, len(b), _ptr(b), _ptr(d), _ptr(f))
+
+    def image_timing(self, enable):
+        """(ms, launches, points) of image-kernel launches since timing was last enabled."""
+        ms, nl, npt = _D(), _U64(), _U64()
+        self.api("b12x_tree_image_timing", self.h, int(enable), ctypes.byref(ms), ctypes.byref(nl),
+                 ctypes.byref(npt))
+        return ms.value, nl.value, npt.value
+
+    # -- structural operations
+    def subdivide(self, flags=SD):
+        return self._ret("b12x_subdivide", flags)
+
+    def unsubdivide(self, flags=SD):
+        return self._ret("b12x_unsubdivide", flags)
+
+    def remove(self, flags):
+        return self._ret("b12x_remove", flags)
+
+    # -- flags
+    def setFlags(self, flags, depth=-1):
+        return self._ret("b12x_set_flags", flags, _depth(depth))
+
+    def unsetFlags(self, flags, depth=-1):
+        return self._ret("b12x_unset_flags", flags, _depth(depth))
+
+    def changeFlags(self, flags_to_unset, flags_to_set, depth=-1):
+        return self._ret("b12x_change_flags", flags_to_unset, flags_to_set, _depth(depth))
+
+    def _stencil(self, op, stencil, f, f2, depth):
+        st = np.ascontiguousarray(np.asarray(stencil).astype(np.uint8))
+        return self._ret("b12x_flags_by_stencil", op, _ptr(st), f, f2, _depth(depth))
+
+    def setFlagsByStencil(self, stencil, flags, depth=-1):
+        return self._stencil(0, stencil, flags, 0, depth)
+
+    def unsetFlagsByStencil(self, stencil, flags, depth=-1):
+        return self._stencil(1, stencil, flags, 0, depth)
+
+    def changeFlagsByStencil(self, stencil, flags_to_unset, flags_to_set, depth=-1):
+        return self._stencil(2, stencil, flags_to_unset, flags_to_set, depth)
+
+    # -- point iterators
+    def scheme(self, grid, mp):
+        key = (grid.kind, tuple(grid.params), mp.kind, tuple(mp.params))
+        if key not in self._schemes:
+            self._schemes[key] = Scheme(self.dim, grid, mp, self.flavour)
+        return self._schemes[key]
+
+    def makePointIteratorPairOverLeaves(self, grid, mp):
+        return PointIterator(self.scheme(grid, mp), LEAF)
+
+    def makePointIteratorPairOverRoot(self, grid, mp):
+        return PointIterator(self.scheme(grid, mp), 0)
+
+    def makePointIteratorPairByFlags(self, depth, flags_to_check, grid, mp):
+        return PointIterator(self.scheme(grid, mp), depth, flags_to_check)
+
+    def num_points(self, it):
+        r = _U64()
+        self.api("b12x_num_points", self.h, it.scheme.h, it.depth, it.flags_to_check, ctypes.byref(r))
+        return r.value
+
+    def _by_image(self, it, op, f, f2, depth):
+        return self._ret("b12x_flags_by_image", it.scheme.h, it.depth, it.flags_to_check, op, f, f2, _depth(depth))
+
+    def setFlagsByIterators(self, it, flags, depth=-1):
+        return self._by_image(it, 0, flags, 0, depth)
+
+    def unsetFlagsByIterators(self, it, flags, depth=-1):
+        return self._by_image(it, 1, flags, 0, depth)
+
+    def changeFlagsByIterators(self, it, flags_to_unset, flags_to_set, depth=-1):
+        return self._by_image(it, 2, flags_to_unset, flags_to_set, depth)
+
+    def searchByIterators(self, it, depth=-1):
+        out = np.empty(self.num_points(it), np.uint32)
+        self.api("b12x_search_by_image", self.h, it.scheme.h, it.depth, it.flags_to_check, _depth(depth), _ptr(out))
+        return out
+
+    def searchCompacted(self, it, depth=-1):
+        n = _U64()
+        out = np.empty(max(self.size(), 1), np.uint32)
+        self.api("b12x_search_compacted", self.h, it.scheme.h, it.depth, it.flags_to_check, _depth(depth), _ptr(out),
+                 ctypes.byref(n))
+        return out[: n.value].copy()
+
+    def searchByPoints(self, points, depth=-1):
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, self.dim)
+        out = np.empty(p.shape[0], np.uint32)
+        self.api("b12x_search_by_points", self.h, _ptr(p), p.shape[0], _depth(depth), _ptr(out))
+        return out
+
+    def subdivision_steps(self, grid, mp, steps, sd_flag=SD):
+        """steps x {subdivide(SD); setFlagsByIterators(OverLeaves, HIT); remove(HIT); unsetFlags(HIT)}"""
+        counts = np.zeros(4 * max(steps, 1), np.uint32)
+        self.api("b12x_subdivision_steps", self.h, self.scheme(grid, mp).h, sd_flag, int(steps), _ptr(counts))
+        return counts[: 4 * steps].reshape(steps, 4)
+
+    # -- transition matrix
+    def transitionMatrix(self, it, depth=-1, use_abs=False, row_major=False, use_invalid=False):
+        h = _P()
+        self.api("b12x_transition_matrix", self.h, it.scheme.h, it.depth, it.flags_to_check, _depth(depth),
+                 int(use_abs), int(row_major), int(use_invalid), ctypes.byref(h))
+        return CooMatrix(h, self.api)
+
+    def transitionMatrixForLeaves(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):
+        return self.transitionMatrix(self.makePointIteratorPairOverLeaves(grid, mp), LEAF, use_abs, row_major,
+                                     use_invalid)
+
+    def transitionMatrixForRoot(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):
+        return self.transitionMatrix(self.makePointIteratorPairOverRoot(grid, mp), 0, use_abs, row_major, use_invalid)

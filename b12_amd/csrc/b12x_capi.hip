@@ -1,0 +1,1393 @@
+// Host implementation of the b12x C ABI (include/b12x_c_api.h): tree, scheme and matrix handles,
+// device buffers, and the launch sequences of every operation. Reference citations are relative
+// to B12c/src/include/.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/b12x_c_api.h"
+#include "b12x_kernels.hpp"
+
+using namespace b12x;
+
+namespace {
+
+thread_local std::string g_err;
+
+int setErr(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      throw HipFail(e_ == hipErrorOutOfMemory ? B12X_E_NOMEM : B12X_E_HIP,                      \
+                    std::string(#call) + " failed: " + hipGetErrorString(e_));                  \
+    }                                                                                           \
+  } while (0)
+
+struct HipFail {
+  int code;
+  std::string msg;
+  HipFail(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+template <class F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (const HipFail& e) {
+    return setErr(e.code, e.msg);
+  } catch (const std::exception& e) {
+    return setErr(B12X_E_ARG, e.what());
+  }
+}
+
+int ensureDevice() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return setErr(B12X_E_NODEVICE, "no HIP device available");
+  return B12X_OK;
+}
+
+template <class T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+uint32_t ceilDiv(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Handles
+// ------------------------------------------------------------------------------------------------
+struct b12x_scheme_s {
+  int dim = 0;
+  GridParams grid{};
+  MapParams map{};
+  double* dTable = nullptr;
+  ForcingStep* dForcing = nullptr;
+  ~b12x_scheme_s() {
+    dfree(dTable);
+    dfree(dForcing);
+  }
+};
+
+struct b12x_tree_s {
+  int dim = 0;
+  Domain dom{};
+  hipStream_t stream = nullptr;
+  uint32_t n = 0;      // number of leaves
+  uint32_t cap = 0;    // capacity of every per-leaf buffer
+  uint64_t* key[2] = {nullptr, nullptr};
+  uint8_t* depth[2] = {nullptr, nullptr};
+  uint8_t* flags[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint8_t* hit = nullptr;      // one byte per leaf, all zero between operations
+  uint32_t* list = nullptr;    // source-box indices for flag-filtered point iterators
+  uint32_t* totals = nullptr;  // per-tile totals of the scans
+  uint32_t* scal = nullptr;    // device scalars
+  uint32_t* hscal = nullptr;   // pinned host mirror of scal
+  uint64_t* rootKey = nullptr; // the surrounding box as a one-element source
+  bool timing = false;          // HIP-event timing of image launches
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<hipEvent_t> eventPool;
+  double imageMs = 0.0;
+  uint64_t imageLaunches = 0, imagePoints = 0;
+  uint8_t* rootDepth = nullptr;
+
+  TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
+  ~b12x_tree_s() {
+    for (int b = 0; b < 2; ++b) {
+      dfree(key[b]);
+      dfree(depth[b]);
+      dfree(flags[b]);
+    }
+    dfree(hit);
+    dfree(list);
+    dfree(totals);
+    dfree(scal);
+    dfree(rootKey);
+    dfree(rootDepth);
+    if (hscal) (void)hipHostFree(hscal);
+    for (auto& p : pending) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+    for (auto e : eventPool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct b12x_coo_s {
+  uint32_t nRows = 0, nCols = 0;
+  uint64_t nnz = 0;
+  bool rowMajor = false;
+  uint32_t* rows = nullptr;
+  uint32_t* cols = nullptr;
+  double* vals = nullptr;
+  hipStream_t stream = nullptr;
+  ~b12x_coo_s() {
+    dfree(rows);
+    dfree(cols);
+    dfree(vals);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+constexpr int kScal = 32;
+constexpr int kMaxScanTiles = 1 << 24;
+
+void syncScal(b12x_tree_s& T, int count = kScal) {
+  HIPCHK(hipMemcpyAsync(T.hscal, T.scal, count * sizeof(uint32_t), hipMemcpyDeviceToHost, T.stream));
+  HIPCHK(hipStreamSynchronize(T.stream));
+}
+
+void zeroScal(b12x_tree_s& T) { HIPCHK(hipMemsetAsync(T.scal, 0, kScal * sizeof(uint32_t), T.stream)); }
+
+// Grow every per-leaf buffer to hold at least `need` leaves, keeping the current leaves.
+void ensureCap(b12x_tree_s& T, uint64_t need) {
+  if (need > 0xFFFFFFFEull) throw HipFail(B12X_E_ARG, "number of leaves exceeds NrBoxes range");
+  if (need <= T.cap) return;
+  uint64_t nc = std::max<uint64_t>(need, std::min<uint64_t>(uint64_t(T.cap) * 2, 0xFFFFFFFEull));
+  nc = std::max<uint64_t>(nc, 1024);
+  uint64_t* k[2];
+  uint8_t* d[2];
+  uint8_t* f[2];
+  for (int b = 0; b < 2; ++b) {
+    k[b] = dalloc<uint64_t>(nc);
+    d[b] = dalloc<uint8_t>(nc);
+    f[b] = dalloc<uint8_t>(nc);
+  }
+  if (T.n) {
+    HIPCHK(hipMemcpyAsync(k[0], T.key[T.cur], size_t(T.n) * 8, hipMemcpyDeviceToDevice, T.stream));
+    HIPCHK(hipMemcpyAsync(d[0], T.depth[T.cur], T.n, hipMemcpyDeviceToDevice, T.stream));
+    HIPCHK(hipMemcpyAsync(f[0], T.flags[T.cur], T.n, hipMemcpyDeviceToDevice, T.stream));
+  }
+  uint8_t* h = dalloc<uint8_t>(nc);
+  HIPCHK(hipMemsetAsync(h, 0, nc, T.stream));
+  uint32_t* l = dalloc<uint32_t>(nc);
+  uint32_t* tt = dalloc<uint32_t>(uint64_t(nc) * 2 / kTile + 16);
+  HIPCHK(hipStreamSynchronize(T.stream));
+  for (int b = 0; b < 2; ++b) {
+    dfree(T.key[b]);
+    dfree(T.depth[b]);
+    dfree(T.flags[b]);
+    T.key[b] = k[b];
+    T.depth[b] = d[b];
+    T.flags[b] = f[b];
+  }
+  dfree(T.hit);
+  dfree(T.list);
+  dfree(T.totals);
+  T.hit = h;
+  T.list = l;
+  T.totals = tt;
+  T.cur = 0;
+  T.cap = uint32_t(nc);
+}
+
+// Runs a three-phase scan over n elements; returns the total number of outputs (synchronises).
+template <class Op>
+uint32_t runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int scalSlot) {
+  if (n == 0) return 0;
+  const uint32_t nb = ceilDiv(n, kTile);
+  if (nb > kMaxScanTiles) throw HipFail(B12X_E_ARG, "scan too large");
+  hipLaunchKernelGGL(scanCountKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, totals);
+  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, T.stream, totals, nb, T.scal + scalSlot);
+  hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, totals);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Exclusive scan of a device uint32 array in place on a stream; returns the total (synchronises).
+uint32_t scanArray(hipStream_t st, uint32_t* v, uint64_t n, uint32_t* scratchTotals, uint32_t* dTotal,
+                   uint32_t* hTotal) {
+  if (n == 0) return 0;
+  if (n > 0xFFFFFFFFull) throw HipFail(B12X_E_ARG, "scan too large");
+  const uint32_t nb = ceilDiv(n, kTile);
+  ScanArrayOp op{v};
+  hipLaunchKernelGGL(scanCountKernel<ScanArrayOp>, dim3(nb), dim3(kBlock), 0, st, op, uint32_t(n), scratchTotals);
+  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, st, scratchTotals, nb, dTotal);
+  hipLaunchKernelGGL(scanArrayScatterKernel, dim3(nb), dim3(kBlock), 0, st, v, uint32_t(n), scratchTotals);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(hTotal, dTotal, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return *hTotal;
+}
+
+struct IndexOp {  // indices i with sel(i) — flag-filtered source lists and searchCompacted
+  const uint8_t* flags;
+  const uint8_t* hit;
+  uint8_t check;
+  uint32_t* out;
+  __device__ uint32_t count(uint32_t i) const {
+    return hit ? (hit[i] ? 1u : 0u) : (areFlagsSet(flags[i], check) ? 1u : 0u);
+  }
+  __device__ void emit(uint32_t i, uint32_t, uint32_t pos) const { out[pos] = i; }
+};
+
+__global__ void fillU8Kernel(uint8_t* p, uint64_t n, uint8_t v) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void countSelKernel(const uint8_t* flags, const uint8_t* hit, uint32_t n, uint8_t a, uint32_t* counter) {
+  // counts leaves with a set hit byte that lack flags a (setFlagsBySearch's return value)
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t c = (i < n && hit[i] && !areFlagsSet(flags[i], a)) ? 1u : 0u;
+  c = blockReduceSum(c);
+  if (threadIdx.x == 0 && c) atomicAdd(counter, c);
+}
+
+__global__ void countFlagsKernel(const uint8_t* flags, uint32_t n, uint8_t a, uint32_t* counter) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t c = (i < n && areFlagsSet(flags[i], a)) ? 1u : 0u;
+  c = blockReduceSum(c);
+  if (threadIdx.x == 0 && c) atomicAdd(counter, c);
+}
+
+template <int DIM, int MAP>
+__global__ void mapPointsKernel(MapParams m, const double* in, double* out, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double x[DIM], y[DIM];
+#pragma unroll
+  for (int j = 0; j < DIM; ++j) x[j] = in[i * DIM + j];
+  applyMap<DIM, MAP>(m, x, y);
+#pragma unroll
+  for (int j = 0; j < DIM; ++j) out[i * DIM + j] = y[j];
+}
+
+// -- reductions over the leaves -------------------------------------------------------------
+struct LeafStats {
+  uint8_t flagsAnd, flagsOr, maxDepth, minDepth;
+};
+
+__global__ void minDepthKernel(const uint8_t* depth, uint32_t n, uint32_t* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t m = i < n ? depth[i] : 255u;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor(m, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicMin(out, m);
+}
+
+LeafStats leafStats(b12x_tree_s& T) {
+  const uint32_t init[4] = {0xFFu, 0u, 0u, 255u};
+  HIPCHK(hipMemcpyAsync(T.scal + 8, init, sizeof(init), hipMemcpyHostToDevice, T.stream));
+  const uint32_t nb = ceilDiv(T.n, kBlock);
+  hipLaunchKernelGGL(flagsReduceKernel, dim3(nb), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.depth[T.cur], T.n,
+                     T.scal + 8);
+  hipLaunchKernelGGL(minDepthKernel, dim3(nb), dim3(kBlock), 0, T.stream, T.depth[T.cur], T.n, T.scal + 11);
+  HIPCHK(hipGetLastError());
+  syncScal(T, 12);
+  return {uint8_t(T.hscal[8]), uint8_t(T.hscal[9]), uint8_t(T.hscal[10]), uint8_t(T.hscal[11])};
+}
+
+// Depth classes of the B12 API: 0 = root, 255 = leaves; a depth on which all leaves lie
+// behaves as the leaf depth (ImplicitBoxTree.hpp areAllLeavesOnDepth branches).
+enum DepthClass { DC_ROOT, DC_LEAF };
+
+DepthClass classifyDepth(b12x_tree_s& T, uint8_t d) {
+  if (d == 0) return DC_ROOT;
+  if (d == kLeafDepth) return DC_LEAF;
+  if (d != kUnsubDepth) {
+    LeafStats s = leafStats(T);
+    if (s.minDepth == d && s.maxDepth == d) return DC_LEAF;
+  }
+  throw HipFail(B12X_E_UNSUPPORTED, "depth must be 0, 255 (leaves) or the common depth of all leaves");
+}
+
+void insertRootIfEmpty(b12x_tree_s& T) {  // ImplicitBoxTree.hpp:1959-1970
+  if (T.n) return;
+  ensureCap(T, 1);
+  const uint64_t k = keyOf(0, 0);
+  const uint8_t z = 0;
+  HIPCHK(hipMemcpyAsync(T.key[T.cur], &k, 8, hipMemcpyHostToDevice, T.stream));
+  HIPCHK(hipMemcpyAsync(T.depth[T.cur], &z, 1, hipMemcpyHostToDevice, T.stream));
+  HIPCHK(hipMemcpyAsync(T.flags[T.cur], &z, 1, hipMemcpyHostToDevice, T.stream));
+  HIPCHK(hipStreamSynchronize(T.stream));
+  T.n = 1;
+}
+
+// -- point iterators --------------------------------------------------------------------------
+struct Source {
+  const uint64_t* key;
+  const uint8_t* depth;
+  const uint32_t* list;
+  uint64_t nBoxes;
+};
+
+Source makeSource(b12x_tree_s& T, uint8_t dIt, uint8_t check) {
+  if (dIt == 0) {  // ImplicitBoxTree.hpp:2082-2120 and :1483-1492 (root only if all leaves carry check)
+    uint64_t nb = 1;
+    if (check != 0) nb = areFlagsSet(leafStats(T).flagsAnd, check) ? 1 : 0;
+    return {T.rootKey, T.rootDepth, nullptr, nb};
+  }
+  if (classifyDepth(T, dIt) != DC_LEAF) throw HipFail(B12X_E_UNSUPPORTED, "point iterator depth");
+  if (check == 0) return {T.key[T.cur], T.depth[T.cur], nullptr, T.n};
+  IndexOp op{T.flags[T.cur], nullptr, check, T.list};
+  runScan(T, op, T.n, T.totals, 0);
+  syncScal(T, 1);
+  return {T.key[T.cur], T.depth[T.cur], T.list, T.hscal[0]};
+}
+
+template <int DIM, int MAP>
+void launchImage2(const ImageArgs& a, int gridKind, hipStream_t st) {
+  const uint64_t blocks64 = (a.nPoints + kBlock - 1) / kBlock;
+  const uint32_t blocks = uint32_t(std::min<uint64_t>(blocks64, 1u << 22));
+  if (blocks == 0) return;
+  if (gridKind == GRID_INNER)
+    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_INNER>), dim3(blocks), dim3(kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_UNIFORM>), dim3(blocks), dim3(kBlock), 0, st, a);
+}
+
+void launchImage(int dim, const ImageArgs& a, int gridKind, hipStream_t st) {
+  switch (a.map.kind) {
+    case MAP_IDENTITY:
+      if (dim == 1) launchImage2<1, MAP_IDENTITY>(a, gridKind, st);
+      else if (dim == 2) launchImage2<2, MAP_IDENTITY>(a, gridKind, st);
+      else if (dim == 3) launchImage2<3, MAP_IDENTITY>(a, gridKind, st);
+      else launchImage2<4, MAP_IDENTITY>(a, gridKind, st);
+      break;
+    case MAP_HENON: launchImage2<2, MAP_HENON>(a, gridKind, st); break;
+    case MAP_LORENZ: launchImage2<3, MAP_LORENZ>(a, gridKind, st); break;
+    case MAP_HENON_HEILES: launchImage2<4, MAP_HENON_HEILES>(a, gridKind, st); break;
+    case MAP_DUFFING: launchImage2<2, MAP_DUFFING>(a, gridKind, st); break;
+    default: throw HipFail(B12X_E_UNSUPPORTED, "map kind");
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// Image of every point of `src` located on the leaves (or on the root when depth0).
+// mode 0 sets hit bytes, mode 1 writes targets to out.
+void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, int mode, uint32_t* out) {
+  ImageArgs a{};
+  a.dom = T.dom;
+  a.map = S.map;
+  a.grid = S.grid;
+  a.key = T.key[T.cur];
+  a.depth = T.depth[T.cur];
+  a.n = T.n;
+  a.srcKey = src.key;
+  a.srcDepth = src.depth;
+  a.srcList = src.list;
+  a.nPoints = src.nBoxes * S.grid.k;
+  a.pointBase = 0;
+  a.mode = mode;
+  a.depth0 = depth0 ? 1 : 0;
+  a.hit = T.hit;
+  a.out = out;
+  if (a.nPoints == 0) return;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (T.timing) {
+    for (hipEvent_t* e : {&e0, &e1}) {
+      if (!T.eventPool.empty()) {
+        *e = T.eventPool.back();
+        T.eventPool.pop_back();
+      } else {
+        HIPCHK(hipEventCreate(e));
+      }
+    }
+    HIPCHK(hipEventRecord(e0, T.stream));
+  }
+  launchImage(T.dim, a, S.grid.kind, T.stream);
+  if (T.timing) {
+    HIPCHK(hipEventRecord(e1, T.stream));
+    T.pending.push_back({e0, e1});
+    T.imageLaunches += 1;
+    T.imagePoints += a.nPoints;
+  }
+}
+
+// Fold completed image timings into the totals (called after the stream has synchronised).
+void collectTimings(b12x_tree_s& T) {
+  for (auto& p : T.pending) {
+    HIPCHK(hipEventSynchronize(p.second));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, p.first, p.second));
+    T.imageMs += ms;
+    T.eventPool.push_back(p.first);
+    T.eventPool.push_back(p.second);
+  }
+  T.pending.clear();
+}
+
+void checkScheme(b12x_tree_s& T, b12x_scheme_s* S) {
+  if (!S) throw HipFail(B12X_E_ARG, "null scheme");
+  if (S->dim != T.dim) throw HipFail(B12X_E_ARG, "scheme dimension differs from tree dimension");
+}
+
+// Flag operation over a selector (nullptr = all leaves); returns the reference's count.
+uint32_t applyFlags(b12x_tree_s& T, uint8_t* sel, bool clearSel, int op, uint8_t a, uint8_t b) {
+  zeroScal(T);
+  hipLaunchKernelGGL(flagsKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.n, sel,
+                     clearSel ? 1 : 0, op, a, b, T.scal);
+  HIPCHK(hipGetLastError());
+  syncScal(T, 1);
+  return T.hscal[0];
+}
+
+bool rootPred(int op, const LeafStats& s, uint8_t a) {  // countIfFlags at depth 0
+  return op == 0 ? !areFlagsSet(s.flagsAnd, a) : (op == 1 ? isAnyFlagSet(s.flagsOr, a) : areFlagsSet(s.flagsAnd, a));
+}
+
+uint32_t stencilOp(b12x_tree_s& T, int op, const uint8_t* hostStencil, uint8_t a, uint8_t b, uint8_t d) {
+  const DepthClass dc = classifyDepth(T, d);
+  if (dc == DC_ROOT) {
+    const bool on = hostStencil ? hostStencil[0] != 0 : true;
+    const LeafStats s = leafStats(T);
+    const uint32_t r = (on && rootPred(op, s, a)) ? 1u : 0u;
+    if (on) applyFlags(T, nullptr, false, op, a, b);
+    return r;
+  }
+  uint8_t* sel = nullptr;
+  if (hostStencil) {
+    sel = T.hit;  // hit bytes are all zero between operations; reuse them for the stencil
+    HIPCHK(hipMemcpyAsync(sel, hostStencil, T.n, hipMemcpyHostToDevice, T.stream));
+  }
+  const uint32_t r = applyFlags(T, sel, sel != nullptr, op, a, b);
+  if (sel) {
+    hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, sel, uint64_t(T.n),
+                       uint8_t(0));
+    HIPCHK(hipStreamSynchronize(T.stream));
+  }
+  return r;
+}
+
+}  // namespace
+
+// ================================================================================================
+// C ABI
+// ================================================================================================
+extern "C" {
+
+const char* b12x_last_error(void) { return g_err.c_str(); }
+
+int b12x_device_name(char* buf, int len) {
+  return guarded([&]() -> int {
+    if (int e = ensureDevice()) return e;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    std::snprintf(buf, size_t(len), "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return B12X_OK;
+  });
+}
+
+int b12x_tree_create(int dim, const double* center, const double* radius, const uint8_t* sd, int nsd,
+                     b12x_tree* out) {
+  return guarded([&]() -> int {
+    if (!out || !center || !radius) return setErr(B12X_E_ARG, "null argument");
+    if (dim < 1 || dim > kMaxDim) return setErr(B12X_E_ARG, "dim must be 1..4");
+    if (int e = ensureDevice()) return e;
+    auto* T = new b12x_tree_s();
+    T->dim = dim;
+    for (int j = 0; j < dim; ++j) {
+      T->dom.center[j] = center[j];
+      T->dom.radius[j] = radius[j];
+    }
+    // ImplicitBoxTree.hpp:27-45 default scheme; :3512-3552 setSdScheme; :574-587 adaptSdCount
+    std::vector<uint8_t> scheme;
+    if (sd) {
+      for (int i = 0; i < nsd; ++i)
+        if (sd[i] >= dim) {
+          delete T;
+          return setErr(B12X_E_ARG, "sdScheme has values >= dim");
+        }
+      scheme.resize(kMaxDepth + 1);
+      for (int i = 0; i < kMaxDepth + 1; ++i) scheme[i] = nsd > 0 ? sd[i % nsd] : uint8_t(i % dim);
+    } else {
+      scheme.resize(kMaxDepth);
+      for (int i = 0; i < kMaxDepth; ++i) scheme[i] = uint8_t(i % dim);
+    }
+    std::memset(T->dom.sdScheme, 0, sizeof(T->dom.sdScheme));
+    std::memset(T->dom.sdCount, 0, sizeof(T->dom.sdCount));
+    T->dom.isDefault = 1;
+    for (size_t i = 0; i < scheme.size(); ++i) {
+      T->dom.sdScheme[i] = scheme[i];
+      if (scheme[i] != uint8_t(uint8_t(i) % dim)) T->dom.isDefault = 0;
+      for (int j = 0; j < dim; ++j) T->dom.sdCount[(i + 1) * dim + j] = T->dom.sdCount[i * dim + j];
+      ++T->dom.sdCount[(i + 1) * dim + scheme[i]];
+    }
+    try {
+      HIPCHK(hipStreamCreate(&T->stream));
+      T->scal = dalloc<uint32_t>(kScal);
+      HIPCHK(hipHostMalloc((void**)&T->hscal, kScal * sizeof(uint32_t), hipHostMallocDefault));
+      T->rootKey = dalloc<uint64_t>(1);
+      T->rootDepth = dalloc<uint8_t>(1);
+      const uint64_t rk = keyOf(0, 0);
+      const uint8_t z = 0;
+      HIPCHK(hipMemcpy(T->rootKey, &rk, 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(T->rootDepth, &z, 1, hipMemcpyHostToDevice));
+      insertRootIfEmpty(*T);
+    } catch (...) {
+      delete T;
+      throw;
+    }
+    *out = T;
+    return B12X_OK;
+  });
+}
+
+int b12x_tree_destroy(b12x_tree t) {
+  delete t;
+  return B12X_OK;
+}
+
+int b12x_size(b12x_tree t, uint32_t* out) {
+  if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+  *out = t->n;
+  return B12X_OK;
+}
+
+int b12x_count(b12x_tree t, uint8_t d, uint32_t* out) {
+  return guarded([&]() -> int {
+    if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+    // ImplicitBoxTree.hpp:1060-1095 for the root, the leaves and a depth shared by all leaves
+    if (d == 0) *out = 1;
+    else if (d == kLeafDepth) *out = t->n;
+    else {
+      LeafStats s = leafStats(*t);
+      if (d != kUnsubDepth && s.minDepth == d && s.maxDepth == d) *out = t->n;
+      else if (d != kUnsubDepth && d > s.maxDepth) *out = 0;
+      else return setErr(B12X_E_UNSUPPORTED, "count at a depth above the leaves is not implemented");
+    }
+    return B12X_OK;
+  });
+}
+
+int b12x_depth(b12x_tree t, uint8_t* out) {
+  return guarded([&]() -> int {
+    if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+    *out = leafStats(*t).maxDepth;
+    return B12X_OK;
+  });
+}
+
+int b12x_tree_download(b12x_tree t, uint64_t* bits, uint8_t* depth, uint8_t* flags) {
+  return guarded([&]() -> int {
+    if (!t) return setErr(B12X_E_ARG, "null tree");
+    std::vector<uint64_t> k(t->n);
+    std::vector<uint8_t> d(t->n);
+    HIPCHK(hipMemcpyAsync(k.data(), t->key[t->cur], size_t(t->n) * 8, hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipMemcpyAsync(d.data(), t->depth[t->cur], t->n, hipMemcpyDeviceToHost, t->stream));
+    if (flags) HIPCHK(hipMemcpyAsync(flags, t->flags[t->cur], t->n, hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    for (uint32_t i = 0; i < t->n; ++i) {
+      if (bits) bits[i] = bitsOf(k[i], d[i]);
+      if (depth) depth[i] = d[i];
+    }
+    return B12X_OK;
+  });
+}
+
+int b12x_tree_upload(b12x_tree t, uint32_t n, const uint64_t* bits, const uint8_t* depth, const uint8_t* flags) {
+  return guarded([&]() -> int {
+    if (!t || (n && (!bits || !depth || !flags))) return setErr(B12X_E_ARG, "null argument");
+    struct E {
+      uint64_t k;
+      uint8_t d, f;
+    };
+    std::vector<E> v(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      if (depth[i] > kMaxDepth) return setErr(B12X_E_ARG, "leaf depth exceeds 64");
+      v[i] = {keyOf(bits[i], depth[i]), depth[i], flags[i]};
+    }
+    // isStrictlyPreceding order (Box.hpp:122-135): ascending key, deeper first on ties
+    std::stable_sort(v.begin(), v.end(), [](const E& a, const E& b) { return a.k < b.k || (a.k == b.k && b.d < a.d); });
+    std::vector<uint64_t> k(n);
+    std::vector<uint8_t> d(n), f(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      k[i] = v[i].k;
+      d[i] = v[i].d;
+      f[i] = v[i].f;
+    }
+    t->n = 0;
+    ensureCap(*t, std::max<uint32_t>(n, 1));
+    HIPCHK(hipMemcpyAsync(t->key[t->cur], k.data(), size_t(n) * 8, hipMemcpyHostToDevice, t->stream));
+    HIPCHK(hipMemcpyAsync(t->depth[t->cur], d.data(), n, hipMemcpyHostToDevice, t->stream));
+    HIPCHK(hipMemcpyAsync(t->flags[t->cur], f.data(), n, hipMemcpyHostToDevice, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    t->n = n;
+    insertRootIfEmpty(*t);
+    return B12X_OK;
+  });
+}
+
+int b12x_subdivide(b12x_tree t, uint8_t f, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    ensureCap(T, uint64_t(T.n) * 2);
+    zeroScal(T);
+    hipLaunchKernelGGL(countFlagsKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.n,
+                       f, T.scal + 1);
+    SubdivideOp op{T.view(T.cur), T.view(1 - T.cur), f};
+    runScan(T, op, T.n, T.totals, 0);
+    syncScal(T, 2);
+    T.n = T.hscal[0];
+    T.cur = 1 - T.cur;
+    *ret = T.hscal[1];
+    return B12X_OK;
+  });
+}
+
+int b12x_unsubdivide(b12x_tree t, uint8_t f, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    zeroScal(T);
+    const int o = 1 - T.cur;
+    UnsubMarkKernelArgs a{T.view(T.cur), T.n, f, T.key[o], T.depth[o], T.scal + 1};
+    hipLaunchKernelGGL(unsubMarkKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, a);
+    // lifted keys/depths sit in the other buffer; merge back into the current one
+    UnsubMergeOp op{T.key[o], T.depth[o], T.flags[T.cur], T.n, T.view(o)};
+    // the merge writes keys/depths in place over the lifted arrays: positions never exceed i,
+    // but other blocks may still read them, so go through the hit-sized staging instead
+    uint64_t* k2 = dalloc<uint64_t>(T.n);
+    uint8_t* d2 = dalloc<uint8_t>(T.n);
+    uint8_t* f2 = dalloc<uint8_t>(T.n);
+    op.out = {k2, d2, f2};
+    runScan(T, op, T.n, T.totals, 0);
+    syncScal(T, 2);
+    const uint32_t m = T.hscal[0];
+    HIPCHK(hipMemcpyAsync(T.key[T.cur], k2, size_t(m) * 8, hipMemcpyDeviceToDevice, T.stream));
+    HIPCHK(hipMemcpyAsync(T.depth[T.cur], d2, m, hipMemcpyDeviceToDevice, T.stream));
+    HIPCHK(hipMemcpyAsync(T.flags[T.cur], f2, m, hipMemcpyDeviceToDevice, T.stream));
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(k2);
+    dfree(d2);
+    dfree(f2);
+    *ret = T.hscal[1] - (T.n - m);
+    T.n = m;
+    return B12X_OK;
+  });
+}
+
+int b12x_remove(b12x_tree t, uint8_t f, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    zeroScal(T);
+    KeepOp op{T.view(T.cur), T.view(1 - T.cur), nullptr, f, 0, 0};
+    runScan(T, op, T.n, T.totals, 0);
+    syncScal(T, 1);
+    const uint32_t m = T.hscal[0];
+    *ret = T.n - m;
+    T.n = m;
+    T.cur = 1 - T.cur;
+    insertRootIfEmpty(T);
+    return B12X_OK;
+  });
+}
+
+int b12x_set_flags(b12x_tree t, uint8_t f, uint8_t d, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
+    *ret = stencilOp(*t, 0, nullptr, f, 0, d);
+    return B12X_OK;
+  });
+}
+
+int b12x_unset_flags(b12x_tree t, uint8_t f, uint8_t d, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
+    *ret = stencilOp(*t, 1, nullptr, f, 0, d);
+    return B12X_OK;
+  });
+}
+
+int b12x_change_flags(b12x_tree t, uint8_t fu, uint8_t fs, uint8_t d, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
+    *ret = stencilOp(*t, 2, nullptr, fu, fs, d);
+    return B12X_OK;
+  });
+}
+
+int b12x_flags_by_stencil(b12x_tree t, int op, const uint8_t* st, uint8_t f, uint8_t f2, uint8_t d, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret || !st || op < 0 || op > 2) return setErr(B12X_E_ARG, "bad argument");
+    *ret = stencilOp(*t, op, st, f, f2, d);
+    return B12X_OK;
+  });
+}
+
+int b12x_num_points(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, uint64_t* out) {
+  return guarded([&]() -> int {
+    if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+    checkScheme(*t, s);
+    *out = makeSource(*t, dIt, check).nBoxes * s->grid.k;
+    return B12X_OK;
+  });
+}
+
+int b12x_flags_by_image(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, int op, uint8_t f, uint8_t f2,
+                        uint8_t d, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret || op < 0 || op > 2) return setErr(B12X_E_ARG, "bad argument");
+    b12x_tree_s& T = *t;
+    checkScheme(T, s);
+    const DepthClass dc = classifyDepth(T, d);
+    const Source src = makeSource(T, dIt, check);
+    runImage(T, *s, src, dc == DC_ROOT, 0, nullptr);
+    if (dc == DC_ROOT) {  // ImplicitBoxTree.hpp:2722-2748 (surrounding box hit by any valid point)
+      uint8_t h0 = 0;
+      HIPCHK(hipMemcpyAsync(&h0, T.hit, 1, hipMemcpyDeviceToHost, T.stream));
+      hipLaunchKernelGGL(fillU8Kernel, dim3(1), dim3(kBlock), 0, T.stream, T.hit, uint64_t(1), uint8_t(0));
+      HIPCHK(hipStreamSynchronize(T.stream));
+      const LeafStats ls = leafStats(T);
+      if (h0 && rootPred(op, ls, f)) {
+        applyFlags(T, nullptr, false, op, f, f2);
+        *ret = 1;
+      } else {
+        *ret = 0;
+      }
+      return B12X_OK;
+    }
+    *ret = applyFlags(T, T.hit, true, op, f, f2);
+    return B12X_OK;
+  });
+}
+
+int b12x_search_by_image(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, uint8_t d, uint32_t* out) {
+  return guarded([&]() -> int {
+    if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    checkScheme(T, s);
+    const DepthClass dc = classifyDepth(T, d);
+    const Source src = makeSource(T, dIt, check);
+    const uint64_t P = src.nBoxes * s->grid.k;
+    if (P == 0) return B12X_OK;
+    uint32_t* dout = dalloc<uint32_t>(P);
+    runImage(T, *s, src, dc == DC_ROOT, 1, dout);
+    HIPCHK(hipMemcpyAsync(out, dout, P * 4, hipMemcpyDeviceToHost, T.stream));
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(dout);
+    return B12X_OK;
+  });
+}
+
+int b12x_search_compacted(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, uint8_t d, uint32_t* out,
+                          uint64_t* nOut) {
+  return guarded([&]() -> int {
+    if (!t || !nOut) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    checkScheme(T, s);
+    const DepthClass dc = classifyDepth(T, d);
+    const Source src = makeSource(T, dIt, check);
+    runImage(T, *s, src, dc == DC_ROOT, 0, nullptr);
+    const uint32_t m = dc == DC_ROOT ? 1 : T.n;
+    IndexOp op{nullptr, T.hit, 0, T.list};
+    runScan(T, op, m, T.totals, 0);
+    syncScal(T, 1);
+    const uint32_t cnt = T.hscal[0];
+    if (out && cnt) HIPCHK(hipMemcpyAsync(out, T.list, size_t(cnt) * 4, hipMemcpyDeviceToHost, T.stream));
+    hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(m, kBlock)), dim3(kBlock), 0, T.stream, T.hit, uint64_t(m),
+                       uint8_t(0));
+    HIPCHK(hipStreamSynchronize(T.stream));
+    *nOut = cnt;
+    return B12X_OK;
+  });
+}
+
+int b12x_search_by_points(b12x_tree t, const double* pts, uint64_t n, uint8_t d, uint32_t* out) {
+  return guarded([&]() -> int {
+    if (!t || (n && (!pts || !out))) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    const DepthClass dc = classifyDepth(T, d);
+    if (n == 0) return B12X_OK;
+    double* dp = dalloc<double>(n * T.dim);
+    uint32_t* dout = dalloc<uint32_t>(n);
+    HIPCHK(hipMemcpyAsync(dp, pts, n * T.dim * 8, hipMemcpyHostToDevice, T.stream));
+    const uint32_t nb = ceilDiv(n, kBlock);
+    const int d0 = dc == DC_ROOT;
+    switch (T.dim) {
+      case 1: hipLaunchKernelGGL(pointsSearchKernel<1>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
+      case 2: hipLaunchKernelGGL(pointsSearchKernel<2>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
+      case 3: hipLaunchKernelGGL(pointsSearchKernel<3>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
+      default: hipLaunchKernelGGL(pointsSearchKernel<4>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, T.stream));
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(dp);
+    dfree(dout);
+    return B12X_OK;
+  });
+}
+
+int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, uint32_t* counts) {
+  return guarded([&]() -> int {
+    if (!t || nSteps < 0) return setErr(B12X_E_ARG, "bad argument");
+    b12x_tree_s& T = *t;
+    checkScheme(T, s);
+    for (int step = 0; step < nSteps; ++step) {
+      // subdivide(sd)
+      ensureCap(T, uint64_t(T.n) * 2);
+      zeroScal(T);
+      hipLaunchKernelGGL(countFlagsKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur],
+                         T.n, sd, T.scal + 1);
+      SubdivideOp sop{T.view(T.cur), T.view(1 - T.cur), sd};
+      runScan(T, sop, T.n, T.totals, 0);
+      syncScal(T, 2);
+      const uint32_t cSub = T.hscal[1];
+      T.n = T.hscal[0];
+      T.cur = 1 - T.cur;
+      // setFlagsByIterators(OverLeaves, HIT, -1): image -> hit bytes
+      const Source src{T.key[T.cur], T.depth[T.cur], nullptr, T.n};
+      runImage(T, *s, src, false, 0, nullptr);
+      zeroScal(T);
+      hipLaunchKernelGGL(countSelKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur],
+                         T.hit, T.n, uint8_t(F_HIT), T.scal + 1);
+      // remove(HIT) + unsetFlags(HIT, -1) in one compaction
+      KeepOp kop{T.view(T.cur), T.view(1 - T.cur), T.hit, uint8_t(F_HIT), uint8_t(F_HIT), uint8_t(F_HIT)};
+      runScan(T, kop, T.n, T.totals, 0);
+      hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.hit, uint64_t(T.n),
+                         uint8_t(0));
+      syncScal(T, 2);
+      const uint32_t kept = T.hscal[0], cSet = T.hscal[1];
+      const uint32_t cRem = T.n - kept;
+      T.n = kept;
+      T.cur = 1 - T.cur;
+      insertRootIfEmpty(T);
+      if (counts) {
+        counts[4 * step + 0] = cSub;
+        counts[4 * step + 1] = cSet;
+        counts[4 * step + 2] = cRem;
+        counts[4 * step + 3] = kept;  // every kept leaf had HIT, which unsetFlags clears
+      }
+    }
+    return B12X_OK;
+  });
+}
+
+// ------------------------------------------------------------------------------------------------
+int b12x_tree_image_timing(b12x_tree t, int enable, double* ms, uint64_t* launches, uint64_t* points) {
+  return guarded([&]() -> int {
+    if (!t) return setErr(B12X_E_ARG, "null tree");
+    HIPCHK(hipStreamSynchronize(t->stream));
+    collectTimings(*t);
+    if (ms) *ms = t->imageMs;
+    if (launches) *launches = t->imageLaunches;
+    if (points) *points = t->imagePoints;
+    if (enable) {
+      t->imageMs = 0.0;
+      t->imageLaunches = t->imagePoints = 0;
+    }
+    t->timing = enable != 0;
+    return B12X_OK;
+  });
+}
+
+int b12x_scheme_create(int dim, int gk, const uint64_t* gp, int ngp, int mk, const double* mp, int nmp,
+                       b12x_scheme* out) {
+  return guarded([&]() -> int {
+    if (!out || dim < 1 || dim > kMaxDim) return setErr(B12X_E_ARG, "bad argument");
+    if (int e = ensureDevice()) return e;
+    auto P = [&](int i, double d) { return i < nmp ? mp[i] : d; };
+    auto* S = new b12x_scheme_s();
+    S->dim = dim;
+    S->grid.kind = gk;
+    S->map.kind = mk;
+    try {
+      if (gk == GRID_INNER) {
+        const uint64_t n = ngp > 0 ? gp[0] : 1;
+        uint64_t k = 1;
+        for (int j = 0; j < dim; ++j) k *= n;
+        if (n == 0 || k > (1u << 20)) throw HipFail(B12X_E_ARG, "InnerGrid size");
+        S->grid.k = uint32_t(k);
+        // Grids.hpp:112-117 evaluated for i = 0..k-1 (the offsets depend on i mod k only)
+        std::vector<double> tab(k * dim);
+        for (uint64_t i = 0; i < k; ++i) {
+          uint64_t npd = 1;
+          for (int j = 0; j < dim; ++j) {
+            tab[i * dim + j] = (double((i / npd) % n) * 2.0 + 1.0 - double(n)) / double(n);
+            npd *= n;
+          }
+        }
+        S->dTable = dalloc<double>(tab.size());
+        HIPCHK(hipMemcpy(S->dTable, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+        S->grid.table = S->dTable;
+      } else if (gk == GRID_UNIFORM) {
+        S->grid.k = uint32_t(ngp > 0 ? gp[0] : 1);
+        S->grid.seed = ngp > 1 ? gp[1] : 0;
+        if (S->grid.k == 0) throw HipFail(B12X_E_ARG, "UniformGrid size");
+      } else {
+        throw HipFail(B12X_E_UNSUPPORTED, "grid kind");
+      }
+      MapParams& M = S->map;
+      switch (mk) {
+        case MAP_IDENTITY: break;
+        case MAP_HENON:
+          if (dim != 2) throw HipFail(B12X_E_ARG, "Henon needs dim 2");
+          M.p[0] = P(0, 1.4); M.p[1] = P(1, 0.3); M.steps = uint32_t(P(2, 1));
+          break;
+        case MAP_LORENZ: {
+          if (dim != 3) throw HipFail(B12X_E_ARG, "Lorenz needs dim 3");
+          const double h = P(3, 0.01);
+          M.p[0] = P(0, 10.0); M.p[1] = P(1, 28.0); M.p[2] = P(2, 8.0 / 3.0);
+          M.p[3] = h; M.p[4] = h / 6.0; M.p[5] = h / 2.0; M.p[6] = h / 3.0;
+          M.steps = uint32_t(P(4, 20));
+          break;
+        }
+        case MAP_HENON_HEILES: {
+          if (dim != 4) throw HipFail(B12X_E_ARG, "Henon-Heiles needs dim 4");
+          M.order = int(P(0, 2));
+          if (M.order < 1 || M.order > 4) throw HipFail(B12X_E_ARG, "symplectic order must be 1..4");
+          M.p[0] = P(1, 0.02); M.steps = uint32_t(P(2, 50));
+          M.p[2] = std::cbrt(2.0);
+          M.p[1] = 1.0 / (2.0 - M.p[2]);
+          break;
+        }
+        case MAP_DUFFING: {
+          if (dim != 2) throw HipFail(B12X_E_ARG, "Duffing needs dim 2");
+          M.p[0] = P(0, 0.25); M.p[1] = P(1, -1.0); M.p[2] = P(2, 1.0);
+          const double gamma = P(3, 0.3), omega = P(4, 1.0), t0 = P(5, 0.0), t1 = P(6, 6.283185307179586),
+                       hIn = P(7, 6.283185307179586 / 100.0);
+          if (!(hIn > 0.0)) throw HipFail(B12X_E_ARG, "Duffing step must be positive");
+          // MapTools.hpp:54-56: the state-independent time loop, evaluated once on the host
+          std::vector<ForcingStep> tab;
+          for (double tt = t0; tt < t1; tt += hIn) {
+            const double h = (t1 - tt) < hIn ? (t1 - tt) : hIn;
+            ForcingStep st;
+            st.h = h;
+            st.h6 = h / 6.0;
+            st.h2 = h / 2.0;
+            st.h3 = h / 3.0;
+            st.f0 = gamma * std::cos(omega * tt);
+            st.fh = gamma * std::cos(omega * (tt + h / 2.0));
+            st.f1 = gamma * std::cos(omega * (tt + h));
+            tab.push_back(st);
+            if (tab.size() > (1u << 24)) throw HipFail(B12X_E_ARG, "Duffing time loop too long");
+          }
+          S->dForcing = dalloc<ForcingStep>(tab.size());
+          if (!tab.empty())
+            HIPCHK(hipMemcpy(S->dForcing, tab.data(), tab.size() * sizeof(ForcingStep), hipMemcpyHostToDevice));
+          M.forcing = S->dForcing;
+          M.nForcing = uint32_t(tab.size());
+          break;
+        }
+        default: throw HipFail(B12X_E_UNSUPPORTED, "map kind");
+      }
+    } catch (...) {
+      delete S;
+      throw;
+    }
+    *out = S;
+    return B12X_OK;
+  });
+}
+
+int b12x_scheme_destroy(b12x_scheme s) {
+  delete s;
+  return B12X_OK;
+}
+
+int b12x_scheme_points_per_box(b12x_scheme s, uint64_t* out) {
+  if (!s || !out) return setErr(B12X_E_ARG, "null argument");
+  *out = s->grid.k;
+  return B12X_OK;
+}
+
+int b12x_map_points(b12x_scheme s, const double* in, double* out, uint64_t n) {
+  return guarded([&]() -> int {
+    if (!s || (n && (!in || !out))) return setErr(B12X_E_ARG, "null argument");
+    if (n == 0) return B12X_OK;
+    const int D = s->dim;
+    double* di = dalloc<double>(n * D);
+    double* dout = dalloc<double>(n * D);
+    HIPCHK(hipMemcpy(di, in, n * D * 8, hipMemcpyHostToDevice));
+    const uint32_t nb = ceilDiv(n, kBlock);
+    switch (s->map.kind) {
+      case MAP_IDENTITY:
+        if (D == 1) hipLaunchKernelGGL((mapPointsKernel<1, MAP_IDENTITY>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n);
+        else if (D == 2) hipLaunchKernelGGL((mapPointsKernel<2, MAP_IDENTITY>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n);
+        else if (D == 3) hipLaunchKernelGGL((mapPointsKernel<3, MAP_IDENTITY>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n);
+        else hipLaunchKernelGGL((mapPointsKernel<4, MAP_IDENTITY>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n);
+        break;
+      case MAP_HENON: hipLaunchKernelGGL((mapPointsKernel<2, MAP_HENON>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n); break;
+      case MAP_LORENZ: hipLaunchKernelGGL((mapPointsKernel<3, MAP_LORENZ>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n); break;
+      case MAP_HENON_HEILES: hipLaunchKernelGGL((mapPointsKernel<4, MAP_HENON_HEILES>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n); break;
+      case MAP_DUFFING: hipLaunchKernelGGL((mapPointsKernel<2, MAP_DUFFING>), dim3(nb), dim3(kBlock), 0, 0, s->map, di, dout, n); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dout, n * D * 8, hipMemcpyDeviceToHost));
+    dfree(di);
+    dfree(dout);
+    return B12X_OK;
+  });
+}
+
+// ------------------------------------------------------------------------------------------------
+// Transition matrix
+// ------------------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+template <int S>
+void launchTmRows(const TmRowArgs& a, hipStream_t st) {
+  const uint64_t groups = (a.nRows + (kWave / S) - 1) / (kWave / S);
+  const uint64_t blocks64 = (groups + (kBlock / kWave) - 1) / (kBlock / kWave);
+  const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(blocks64, 1u << 20)));
+  hipLaunchKernelGGL(tmRowsKernel<S>, dim3(blocks), dim3(kBlock), 0, st, a);
+}
+
+void tmRows(const TmRowArgs& a, hipStream_t st) {
+  if (a.k <= 1) launchTmRows<1>(a, st);
+  else if (a.k <= 2) launchTmRows<2>(a, st);
+  else if (a.k <= 4) launchTmRows<4>(a, st);
+  else if (a.k <= 8) launchTmRows<8>(a, st);
+  else if (a.k <= 16) launchTmRows<16>(a, st);
+  else if (a.k <= 32) launchTmRows<32>(a, st);
+  else launchTmRows<64>(a, st);
+  HIPCHK(hipGetLastError());
+}
+
+// Stable sort of (key, p0, p1) triples by key using ceil(bits/8) LSD passes.
+void radixSort3(hipStream_t st, uint32_t*& k, uint32_t*& p0, uint32_t*& p1, uint64_t n, int bits,
+                uint32_t* hTotal) {
+  if (n <= 1) return;
+  uint32_t* k2 = dalloc<uint32_t>(n);
+  uint32_t* a2 = dalloc<uint32_t>(n);
+  uint32_t* b2 = dalloc<uint32_t>(n);
+  const uint32_t nb = ceilDiv(n, kRadixTile);
+  const uint64_t nh = uint64_t(nb) * 256;
+  uint32_t* hist = dalloc<uint32_t>(nh);
+  uint32_t* tt = dalloc<uint32_t>(nh / kTile + 16);
+  uint32_t* dTot = dalloc<uint32_t>(1);
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(radixHistKernel, dim3(nb), dim3(kBlock), 0, st, k, n, shift, hist, nb);
+    scanArray(st, hist, nh, tt, dTot, hTotal);
+    hipLaunchKernelGGL(radixScatterKernel, dim3(nb), dim3(kBlock), 0, st, k, p0, p1, n, shift, hist, nb, k2, a2, b2);
+    HIPCHK(hipGetLastError());
+    std::swap(k, k2);
+    std::swap(p0, a2);
+    std::swap(p1, b2);
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(k2);
+  dfree(a2);
+  dfree(b2);
+  dfree(hist);
+  dfree(tt);
+  dfree(dTot);
+}
+
+int bitsFor(uint64_t maxValue) {
+  int b = 0;
+  while (b < 32 && (maxValue >> b)) ++b;
+  return std::max(8, (b + 7) / 8 * 8);
+}
+
+}  // namespace
+
+extern "C" {
+
+int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, uint8_t d, int useAbs,
+                           int rowMajor, int useInvalid, b12x_coo* out) {
+  return guarded([&]() -> int {
+    if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    checkScheme(T, s);
+    const DepthClass dc = classifyDepth(T, d);
+    const Source src = makeSource(T, dIt, check);
+    const uint32_t k = s->grid.k;
+    const uint64_t P = src.nBoxes * k;
+    const uint32_t nBoxes = dc == DC_ROOT ? 1u : T.n;  // count(depth)
+    auto* M = new b12x_coo_s();
+    try {
+      HIPCHK(hipStreamCreate(&M->stream));
+      M->nRows = M->nCols = nBoxes;
+      M->rowMajor = rowMajor != 0;
+      uint32_t* col = dalloc<uint32_t>(P);
+      runImage(T, *s, src, dc == DC_ROOT, 1, col);
+      const uint64_t nRowsSrc = src.nBoxes;
+      uint32_t* rowPtr = dalloc<uint32_t>(nRowsSrc + 1);
+      TmRowArgs a{};
+      a.col = col;
+      a.nRows = nRowsSrc;
+      a.k = k;
+      a.keepInvalid = useInvalid ? 1 : 0;
+      a.invalidLabel = nBoxes + 1;  // nColumns_ after its increment (ImplicitBoxTree.hpp:3081-3087,3127-3132)
+      a.rowCount = rowPtr;
+      a.anyInvalid = T.scal + 2;
+      a.pass = 0;
+      zeroScal(T);
+      tmRows(a, T.stream);
+      HIPCHK(hipMemsetAsync(rowPtr + nRowsSrc, 0, 4, T.stream));
+      const uint32_t nnz = scanArray(T.stream, rowPtr, nRowsSrc + 1, T.totals ? T.totals : nullptr, T.scal + 3,
+                                     T.hscal + 3);
+      syncScal(T, 3);
+      const bool anyInvalid = T.hscal[2] != 0;
+      if (useInvalid && anyInvalid) M->nCols = nBoxes + 1;
+      uint32_t* cols = dalloc<uint32_t>(nnz);
+      uint32_t* cnt = dalloc<uint32_t>(nnz);
+      a.pass = 1;
+      a.rowPtr = rowPtr;
+      a.outCol = cols;
+      a.outCnt = cnt;
+      tmRows(a, T.stream);
+      uint32_t* rows = dalloc<uint32_t>(nnz);
+      if (nRowsSrc)
+        hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nRowsSrc, kBlock)), dim3(kBlock), 0, T.stream, rowPtr,
+                           nRowsSrc, rows);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(T.stream));
+      dfree(col);
+      dfree(rowPtr);
+      if (!rowMajor) {
+        // column-major order (CooMatrix.hpp:95-137): stable LSD radix sort by column of the
+        // row-major entries leaves rows ascending within each column
+        radixSort3(T.stream, cols, rows, cnt, nnz, bitsFor(uint64_t(nBoxes) + 1), T.hscal + 4);
+      }
+      M->nnz = nnz;
+      M->rows = rows;
+      M->cols = cols;
+      M->vals = dalloc<double>(nnz);
+      if (nnz)
+        hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, T.stream, cnt, uint64_t(nnz),
+                           double(k), useAbs ? 1 : 0, M->vals);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(T.stream));
+      dfree(cnt);
+    } catch (...) {
+      delete M;
+      throw;
+    }
+    *out = M;
+    return B12X_OK;
+  });
+}
+
+int b12x_coo_destroy(b12x_coo m) {
+  delete m;
+  return B12X_OK;
+}
+
+int b12x_coo_info(b12x_coo m, uint64_t* nnz, uint32_t* nr, uint32_t* nc, int* rm) {
+  if (!m) return setErr(B12X_E_ARG, "null matrix");
+  if (nnz) *nnz = m->nnz;
+  if (nr) *nr = m->nRows;
+  if (nc) *nc = m->nCols;
+  if (rm) *rm = m->rowMajor ? 1 : 0;
+  return B12X_OK;
+}
+
+int b12x_coo_download(b12x_coo m, uint32_t* rows, uint32_t* cols, double* vals) {
+  return guarded([&]() -> int {
+    if (!m) return setErr(B12X_E_ARG, "null matrix");
+    if (rows && m->nnz) HIPCHK(hipMemcpyAsync(rows, m->rows, m->nnz * 4, hipMemcpyDeviceToHost, m->stream));
+    if (cols && m->nnz) HIPCHK(hipMemcpyAsync(cols, m->cols, m->nnz * 4, hipMemcpyDeviceToHost, m->stream));
+    if (vals && m->nnz) HIPCHK(hipMemcpyAsync(vals, m->vals, m->nnz * 8, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return B12X_OK;
+  });
+}
+
+int b12x_coo_upload(const uint32_t* rows, const uint32_t* cols, const double* vals, uint64_t nnz, uint32_t nr,
+                    uint32_t nc, b12x_coo* out) {
+  return guarded([&]() -> int {
+    if (!out || (nnz && (!rows || !cols || !vals))) return setErr(B12X_E_ARG, "null argument");
+    if (int e = ensureDevice()) return e;
+    auto* M = new b12x_coo_s();
+    try {
+      HIPCHK(hipStreamCreate(&M->stream));
+      M->nRows = nr;
+      M->nCols = nc;
+      M->nnz = nnz;
+      M->rows = dalloc<uint32_t>(nnz);
+      M->cols = dalloc<uint32_t>(nnz);
+      M->vals = dalloc<double>(nnz);
+      if (nnz) {
+        HIPCHK(hipMemcpy(M->rows, rows, nnz * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M->cols, cols, nnz * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M->vals, vals, nnz * 8, hipMemcpyHostToDevice));
+      }
+      bool rm = true;
+      for (uint64_t e = 1; e < nnz && rm; ++e)
+        rm = rows[e - 1] < rows[e] || (rows[e - 1] == rows[e] && cols[e - 1] < cols[e]);
+      M->rowMajor = rm;
+    } catch (...) {
+      delete M;
+      throw;
+    }
+    *out = M;
+    return B12X_OK;
+  });
+}
+
+int b12x_compress_index_vector(const uint32_t* idx, uint64_t n, uint32_t dim, uint32_t* out) {
+  return guarded([&]() -> int {
+    if ((n && !idx) || !out) return setErr(B12X_E_ARG, "null argument");
+    if (n > 0xFFFFFFFFull) return setErr(B12X_E_ARG, "index vector longer than the NrBoxes range");
+    if (int e = ensureDevice()) return e;
+    uint32_t* di = dalloc<uint32_t>(n);
+    uint32_t* dout = dalloc<uint32_t>(uint64_t(dim) + 1);
+    if (n) HIPCHK(hipMemcpy(di, idx, n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(dim) + 1, kBlock)), dim3(kBlock), 0, 0, di, n, dim, dout);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dout, (uint64_t(dim) + 1) * 4, hipMemcpyDeviceToHost));
+    dfree(di);
+    dfree(dout);
+    return B12X_OK;
+  });
+}
+
+// ------------------------------------------------------------------------------------------------
+// Invariant measure
+// ------------------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+struct PiState {
+  double norm;
+  double resid;
+  int iters;
+  int done;
+};
+
+__global__ void piSpmvKernel(const uint32_t* colPtr, const uint32_t* rowIdx, const double* val, const double* mu,
+                             uint32_t n, double* y, double* partial, const PiState* st) {
+  if (st->done) return;
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  double s = 0.0;
+  if (j < n) {
+    for (uint32_t e = colPtr[j]; e < colPtr[j + 1]; ++e) s += mu[rowIdx[e]] * val[e];
+    y[j] = s;
+  }
+  const double t = blockReduceSumF64(fabs(s));
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void piNormKernel(const double* partial, uint32_t m, PiState* st) {
+  if (st->done) return;
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < m; i += kBlock) s += partial[i];
+  s = blockReduceSumF64(s);
+  if (threadIdx.x == 0) st->norm = s;
+}
+
+__global__ void piNormalizeKernel(const double* y, double* mu, uint32_t n, double* partial, const PiState* st) {
+  if (st->done) return;
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  double r = 0.0;
+  if (j < n) {
+    const double v = y[j] / st->norm;
+    r = fabs(v - mu[j]);
+    mu[j] = v;
+  }
+  const double t = blockReduceSumF64(r);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void piResidKernel(const double* partial, uint32_t m, PiState* st, int maxIter, double tol) {
+  if (st->done) return;
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < m; i += kBlock) s += partial[i];
+  s = blockReduceSumF64(s);
+  if (threadIdx.x == 0) {
+    st->resid = s;
+    st->iters += 1;
+    st->done = (st->iters >= maxIter) || (tol >= 0.0 && s <= tol);
+  }
+}
+
+__global__ void sequenceKernel(uint32_t* p, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = uint32_t(i);
+}
+
+__global__ void gatherF64Kernel(const double* src, const uint32_t* idx, uint64_t n, double* dst) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ void fillF64Kernel(double* p, uint32_t n, double v) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+}  // namespace
+
+extern "C" {
+
+int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double* mu, int* iters, double* resid) {
+  return guarded([&]() -> int {
+    if (!m || !mu) return setErr(B12X_E_ARG, "null argument");
+    if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
+    const uint32_t n = m->nRows;
+    hipStream_t st = m->stream;
+    std::vector<uint32_t> hT(8);
+    // CSC in ascending row order per column: a column-major copy of the entries
+    const uint64_t nnz = m->nnz;
+    uint32_t* ck = dalloc<uint32_t>(nnz);
+    uint32_t* cr = dalloc<uint32_t>(nnz);
+    double* vals = dalloc<double>(nnz);
+    if (nnz) {
+      HIPCHK(hipMemcpyAsync(ck, m->cols, nnz * 4, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(cr, m->rows, nnz * 4, hipMemcpyDeviceToDevice, st));
+    }
+    if (!m->rowMajor) {
+      if (nnz) HIPCHK(hipMemcpyAsync(vals, m->vals, nnz * 8, hipMemcpyDeviceToDevice, st));
+    } else if (nnz) {
+      // stable sort by column carrying (row, entry index); then gather the values
+      uint32_t* idx = dalloc<uint32_t>(nnz);
+      hipLaunchKernelGGL(sequenceKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, idx, nnz);
+      radixSort3(st, ck, cr, idx, nnz, bitsFor(uint64_t(m->nCols) + 1), hT.data());
+      hipLaunchKernelGGL(gatherF64Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, m->vals, idx, nnz, vals);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+      dfree(idx);
+    }
+    uint32_t* colPtr = dalloc<uint32_t>(uint64_t(n) + 1);
+    hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, st, ck, m->nnz, n,
+                       colPtr);
+    double* dmu = dalloc<double>(n);
+    double* dy = dalloc<double>(n);
+    const uint32_t nb = ceilDiv(n, kBlock);
+    double* partial = dalloc<double>(nb);
+    PiState* pst = dalloc<PiState>(1);
+    PiState init{0.0, -1.0, 0, maxIter <= 0 ? 1 : 0};
+    HIPCHK(hipMemcpyAsync(pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    if (muInit) HIPCHK(hipMemcpyAsync(dmu, mu, size_t(n) * 8, hipMemcpyHostToDevice, st));
+    else hipLaunchKernelGGL(fillF64Kernel, dim3(nb), dim3(kBlock), 0, st, dmu, n, 1.0 / double(n));
+    PiState hs{};
+    const int batch = 16;
+    for (int it = 0; it < maxIter; it += batch) {
+      for (int b = 0; b < batch; ++b) {
+        hipLaunchKernelGGL(piSpmvKernel, dim3(nb), dim3(kBlock), 0, st, colPtr, cr, vals, dmu, n, dy, partial, pst);
+        hipLaunchKernelGGL(piNormKernel, dim3(1), dim3(kBlock), 0, st, partial, nb, pst);
+        hipLaunchKernelGGL(piNormalizeKernel, dim3(nb), dim3(kBlock), 0, st, dy, dmu, n, partial, pst);
+        hipLaunchKernelGGL(piResidKernel, dim3(1), dim3(kBlock), 0, st, partial, nb, pst, maxIter, tol);
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (hs.done) break;
+    }
+    HIPCHK(hipMemcpy(mu, dmu, size_t(n) * 8, hipMemcpyDeviceToHost));
+    if (iters) *iters = hs.iters;
+    if (resid) *resid = hs.resid;
+    dfree(ck); dfree(cr); dfree(vals); dfree(colPtr); dfree(dmu); dfree(dy); dfree(partial); dfree(pst);
+    return B12X_OK;
+  });
+}
+
+}  // extern "C"

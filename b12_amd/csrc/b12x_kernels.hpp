@@ -1,0 +1,631 @@
+// HIP kernels of the b12x hot path (gfx950, wave64).
+//
+// Tree storage in HBM: three structure-of-arrays vectors per tree, sorted by comparable key:
+//   key[n]   uint64  comparable key K (Box.hpp:65-72) — the descent compares these directly
+//   depth[n] uint8
+//   flags[n] uint8
+// plus a hit byte per leaf used by the image kernel. Set operations are scan-based: a count
+// pass, a scan of per-tile totals, and a scatter pass that recomputes the per-element counts.
+#pragma once
+
+#include "b12x_device.hpp"
+
+namespace b12x {
+
+constexpr int kBlock = 256;                   // threads per block for streaming kernels
+constexpr int kItems = 8;                     // elements per thread in scan kernels
+constexpr int kTile = kBlock * kItems;        // elements per block in scan kernels
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------------------------
+// Wave / block scans
+// ---------------------------------------------------------------------------------------------
+__device__ inline uint32_t waveInclusiveScan(uint32_t v) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, kWave);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+__device__ inline uint32_t waveReduceSum(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Exclusive scan over the block; returns the exclusive prefix of v and the block total.
+__device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t waveSums[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint32_t inc = waveInclusiveScan(v);
+  if (lane == kWave - 1) waveSums[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    const uint32_t s = waveSums[w];
+    before += (w < wave) ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return before + inc - v;
+}
+
+__device__ inline uint32_t blockReduceSum(uint32_t v) {
+  __shared__ uint32_t waveSums[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  v = waveReduceSum(v);
+  if (lane == 0) waveSums[wave] = v;
+  __syncthreads();
+  uint32_t tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) tot += waveSums[w];
+  __syncthreads();
+  return tot;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Generic three-phase scan: Op::count(i) -> number of outputs of element i (0, 1 or 2),
+// Op::emit(i, c, pos) writes them at pos. Block totals -> exclusive offsets -> scatter.
+// ---------------------------------------------------------------------------------------------
+template <class Op>
+__global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, uint32_t* blockTotals) {
+  const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int t = 0; t < kItems; ++t) {
+    const uint64_t i = base + t;
+    if (i < n) s += op.count(uint32_t(i));
+  }
+  s = blockReduceSum(s);
+  if (threadIdx.x == 0) blockTotals[blockIdx.x] = s;
+}
+
+// Single-block exclusive scan of m block totals in place; writes the grand total to *total.
+__global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t m, uint32_t* total) {
+  uint32_t carry = 0;
+  for (uint32_t off = 0; off < m; off += kTile) {
+    uint32_t loc[kItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int t = 0; t < kItems; ++t) {
+      const uint32_t i = off + threadIdx.x * kItems + t;
+      loc[t] = i < m ? v[i] : 0u;
+      s += loc[t];
+    }
+    uint32_t tot;
+    uint32_t ex = blockExclusiveScan(s, tot);
+#pragma unroll
+    for (int t = 0; t < kItems; ++t) {
+      const uint32_t i = off + threadIdx.x * kItems + t;
+      if (i < m) v[i] = carry + ex;
+      ex += loc[t];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32_t n, const uint32_t* blockOffsets) {
+  const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
+  uint32_t c[kItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int t = 0; t < kItems; ++t) {
+    const uint64_t i = base + t;
+    c[t] = i < n ? op.count(uint32_t(i)) : 0u;
+    s += c[t];
+  }
+  uint32_t tot;
+  uint32_t pos = blockExclusiveScan(s, tot) + blockOffsets[blockIdx.x];
+#pragma unroll
+  for (int t = 0; t < kItems; ++t) {
+    const uint64_t i = base + t;
+    if (i < n && c[t]) op.emit(uint32_t(i), c[t], pos);
+    pos += c[t];
+  }
+}
+
+struct TreeView {
+  uint64_t* key;
+  uint8_t* depth;
+  uint8_t* flags;
+};
+
+// subdivide (ImplicitBoxTree.hpp:2883-2930): every leaf with all `mask` flags and depth < 64 is
+// replaced in place by its two halves (IncrementDepthByFunctor, ComputeOtherHalfFunctor). The
+// halves of a leaf are adjacent to it in key order, so the output stays sorted without a sort.
+struct SubdivideOp {
+  TreeView in, out;
+  uint8_t mask;
+  __device__ uint32_t count(uint32_t i) const {
+    return (areFlagsSet(in.flags[i], mask) && in.depth[i] < kMaxDepth) ? 2u : 1u;
+  }
+  __device__ void emit(uint32_t i, uint32_t c, uint32_t pos) const {
+    const uint64_t k = in.key[i];
+    const int d = in.depth[i];
+    const uint8_t f = in.flags[i];
+    if (c == 1) {
+      out.key[pos] = k;
+      out.depth[pos] = uint8_t(d);
+      out.flags[pos] = f;
+    } else {
+      const uint64_t b = bitsOf(k, d) << 1;
+      out.key[pos] = keyOf(b, d + 1);
+      out.key[pos + 1] = keyOf(b | 1ull, d + 1);
+      out.depth[pos] = out.depth[pos + 1] = uint8_t(d + 1);
+      out.flags[pos] = out.flags[pos + 1] = f;
+    }
+  }
+};
+
+// remove (ImplicitBoxTree.hpp:2343-2371): stable compaction keeping leaves with all `mask` flags.
+// With `hit` set, a leaf counts as carrying `hitFlag` when its hit byte is set (the fused
+// setFlagsBySearch + remove of one subdivision step), and `clearMask` is unset in the output
+// (the step's trailing unsetFlags).
+struct KeepOp {
+  TreeView in, out;
+  const uint8_t* hit;
+  uint8_t mask, hitFlag, clearMask;
+  __device__ uint8_t flagsAt(uint32_t i) const {
+    uint8_t f = in.flags[i];
+    if (hit && hit[i]) f |= hitFlag;
+    return f;
+  }
+  __device__ uint32_t count(uint32_t i) const { return areFlagsSet(flagsAt(i), mask) ? 1u : 0u; }
+  __device__ void emit(uint32_t i, uint32_t, uint32_t pos) const {
+    out.key[pos] = in.key[i];
+    out.depth[pos] = in.depth[i];
+    out.flags[pos] = uint8_t(flagsAt(i) & ~clearMask);
+  }
+};
+
+// unsubdivide (ImplicitBoxTree.hpp:3413-3461): first pass marks and lifts unsubdividable leaves
+// (Functors.hpp:337-377 over the predecessor/successor stencil of ImplicitBoxTree.hpp:2023-2058);
+// the compaction keeps the first of each run of equal boxes with the AND of the run's flags.
+struct UnsubMarkKernelArgs {
+  TreeView in;
+  uint32_t n;
+  uint8_t mask;
+  uint64_t* liftedKey;
+  uint8_t* liftedDepth;
+  uint32_t* nUnsub;
+};
+
+__device__ inline bool isUnsubdividableAt(const TreeView& t, uint32_t n, uint32_t i, uint8_t mask) {
+  const int d = t.depth[i];
+  if (!(areFlagsSet(t.flags[i], mask) && d > 0)) return false;
+  const uint64_t bits = bitsOf(t.key[i], d);
+  const uint32_t j = (bits & 1ull) ? (i > 0 ? i - 1 : 0) : (i + 1 < n ? i + 1 : n - 1);
+  const int dn = t.depth[j];
+  const uint64_t bn = bitsOf(t.key[j], dn);
+  // parents: decrementDepthBy(., 1)
+  const uint64_t pb = bits >> 1;
+  const int pd = d - 1;
+  const uint64_t nb = dn > 0 ? (bn >> 1) : bn;
+  const int nd = dn > 0 ? dn - 1 : 0;
+  if (pb == nb && pd == nd) return areFlagsSet(t.flags[j], mask);
+  // !hasSamePath(parent, neighbour)
+  const int diff = pd - dn;
+  return shr(pb, diff) != shr(bn, -diff);
+}
+
+__global__ __launch_bounds__(kBlock) void unsubMarkKernel(UnsubMarkKernelArgs a) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t u = 0;
+  if (i < a.n) {
+    const bool un = isUnsubdividableAt(a.in, a.n, i, a.mask);
+    int d = a.in.depth[i];
+    uint64_t k = a.in.key[i];
+    if (un) {
+      const uint64_t b = bitsOf(k, d) >> 1;
+      d -= 1;
+      k = keyOf(b, d);
+    }
+    a.liftedKey[i] = k;
+    a.liftedDepth[i] = uint8_t(d);
+    u = un ? 1u : 0u;
+  }
+  u = blockReduceSum(u);
+  if (threadIdx.x == 0 && u) atomicAdd(a.nUnsub, u);
+}
+
+struct UnsubMergeOp {
+  const uint64_t* key;
+  const uint8_t* depth;
+  const uint8_t* flags;
+  uint32_t n;
+  TreeView out;
+  __device__ bool same(uint32_t a, uint32_t b) const { return key[a] == key[b] && depth[a] == depth[b]; }
+  __device__ uint32_t count(uint32_t i) const { return (i == 0 || !same(i, i - 1)) ? 1u : 0u; }
+  __device__ void emit(uint32_t i, uint32_t, uint32_t pos) const {
+    uint8_t f = flags[i];
+    for (uint32_t j = i + 1; j < n && same(j, i); ++j) f &= flags[j];
+    out.key[pos] = key[i];
+    out.depth[pos] = depth[i];
+    out.flags[pos] = f;
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Flag operations at leaf depth (ImplicitBoxTree.hpp:2803-2881, 3334-3411, 912-1058 and the
+// *BySearch forms :2716-2801, 3247-3332, 822-910). op 0 = set, 1 = unset, 2 = change.
+// The return value counts leaves whose state the operation changes, as the reference does.
+// ---------------------------------------------------------------------------------------------
+__device__ inline bool flagPred(int op, uint8_t f, uint8_t a) {
+  return op == 0 ? !areFlagsSet(f, a) : (op == 1 ? isAnyFlagSet(f, a) : areFlagsSet(f, a));
+}
+__device__ inline uint8_t flagApply(int op, uint8_t f, uint8_t a, uint8_t b) {
+  return op == 0 ? uint8_t(f | a)
+                 : (op == 1 ? uint8_t(f & ~a) : (areFlagsSet(f, a) ? uint8_t((f & ~a) | b) : f));
+}
+
+// sel: per-leaf selector bytes (stencil or hit bytes), nullptr = all leaves. With clearSel the
+// selector bytes are zeroed after use (hit bytes are reused by the next image step).
+__global__ __launch_bounds__(kBlock) void flagsKernel(uint8_t* flags, uint32_t n, uint8_t* sel, int clearSel, int op,
+                                                      uint8_t a, uint8_t b, uint32_t* counter) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t c = 0;
+  if (i < n) {
+    const bool on = sel ? sel[i] != 0 : true;
+    if (on) {
+      const uint8_t f = flags[i];
+      c = flagPred(op, f, a) ? 1u : 0u;
+      flags[i] = flagApply(op, f, a, b);
+      if (clearSel) sel[i] = 0;
+    }
+  }
+  c = blockReduceSum(c);
+  if (threadIdx.x == 0 && c) atomicAdd(counter, c);
+}
+
+// AND and OR of all flags (depth-0 forms of countIfFlags, ImplicitBoxTree.hpp:1105-1114) and the
+// maximum depth (ImplicitBoxTree.hpp:1264-1271).
+__global__ __launch_bounds__(kBlock) void flagsReduceKernel(const uint8_t* flags, const uint8_t* depth, uint32_t n,
+                                                            uint32_t* out /*[and, or, maxDepth]*/) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t fa = 0xFF, fo = 0, md = 0;
+  if (i < n) {
+    fa = flags[i];
+    fo = flags[i];
+    md = depth[i];
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    fa &= __shfl_xor(fa, o, kWave);
+    fo |= __shfl_xor(fo, o, kWave);
+    md = max(md, (uint32_t)__shfl_xor(md, o, kWave));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicAnd(out + 0, fa);
+    atomicOr(out + 1, fo);
+    atomicMax(out + 2, md);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The fused image kernel: one lane per test point. Point i of source box b = i / k:
+// box geometry -> grid offset -> x = c + g*r (Functors.hpp:476-511) -> map -> key -> descent.
+// mode 0: hit[target] = 1; mode 1: out[i] = target.
+// ---------------------------------------------------------------------------------------------
+struct ImageArgs {
+  Domain dom;
+  MapParams map;
+  GridParams grid;
+  const uint64_t* key;      // leaves (descent target)
+  const uint8_t* depth;
+  uint32_t n;
+  const uint64_t* srcKey;   // source boxes
+  const uint8_t* srcDepth;
+  const uint32_t* srcList;  // optional indirection into srcKey/srcDepth
+  uint64_t nPoints;
+  uint64_t pointBase;       // global index of point 0 (grid index offset)
+  int mode;
+  int depth0;               // search on depth 0 (root only)
+  uint8_t* hit;
+  uint32_t* out;
+};
+
+template <int DIM, int MAP, int GRID>
+__global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < a.nPoints; i += stride) {
+    const uint64_t gi = a.pointBase + i;
+    const uint64_t b = gi / a.grid.k;
+    const uint32_t src = a.srcList ? a.srcList[b] : uint32_t(b);
+    double c[DIM], rb[DIM], g[DIM], x[DIM], y[DIM];
+    boxCoords<DIM>(a.dom, a.srcKey[src], a.srcDepth[src], c, rb);
+    if constexpr (GRID == GRID_INNER) {
+      const uint32_t gl = uint32_t(gi % a.grid.k);
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) g[j] = a.grid.table[gl * DIM + j];
+    } else {
+      uniformGridPoint<DIM>(a.grid.seed, gi, g);
+    }
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) x[j] = c[j] + g[j] * rb[j];
+    applyMap<DIM, MAP>(a.map, x, y);
+    const uint64_t pk = pointKey<DIM>(a.dom, y);
+    uint32_t t;
+    if (a.depth0) t = (pk == kInvalidKey) ? kInvalid : 0u;
+    else t = locate(a.key, a.depth, a.n, pk);
+    if (a.mode == 0) {
+      if (t != kInvalid) a.hit[t] = 1;
+    } else {
+      a.out[i] = t;
+    }
+  }
+}
+
+// Keys of arbitrary points (searchByPoints, ImplicitBoxTree.hpp:2588-2620)
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void pointsSearchKernel(Domain dom, const double* pts, uint64_t m,
+                                                             const uint64_t* key, const uint8_t* depth, uint32_t n,
+                                                             int depth0, uint32_t* out) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= m) return;
+  double p[DIM];
+#pragma unroll
+  for (int j = 0; j < DIM; ++j) p[j] = pts[i * DIM + j];
+  const uint64_t pk = pointKey<DIM>(dom, p);
+  out[i] = depth0 ? ((pk == kInvalidKey) ? kInvalid : 0u) : locate(key, depth, n, pk);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Transition matrix rows (ImplicitBoxTree.hpp:2932-3159): the k targets of each source box are
+// sorted inside a wave segment of S = next_pow2(k) lanes by a bitonic network over shuffles and
+// run-length encoded; a row's entries come out in ascending column order (row-major COO).
+// pass 0 writes per-row entry counts, pass 1 writes (column, count) at rowPtr[row].
+// ---------------------------------------------------------------------------------------------
+struct TmRowArgs {
+  const uint32_t* col;   // P targets, box-major
+  uint64_t nRows;
+  uint32_t k;
+  int keepInvalid;
+  uint32_t invalidLabel;
+  uint32_t* rowCount;    // pass 0
+  const uint32_t* rowPtr;  // pass 1
+  uint32_t* outCol;
+  uint32_t* outCnt;
+  uint32_t* anyInvalid;
+  int pass;
+};
+
+template <int S>
+__global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
+  constexpr int SEGS = kWave / S;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int seg = lane / S, pos = lane % S;
+  const uint64_t waveId = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / kWave;
+  const uint64_t nWaves = uint64_t(gridDim.x) * kBlock / kWave;
+  for (uint64_t grp = waveId; grp * SEGS < a.nRows; grp += nWaves) {
+    const uint64_t row = grp * SEGS + seg;
+    const bool realRow = row < a.nRows;
+    uint32_t v = 0xFFFFFFFFu;
+    if (realRow && uint32_t(pos) < a.k) v = a.col[row * a.k + pos];
+    // bitonic sort within the segment
+#pragma unroll
+    for (int size = 2; size <= S; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const uint32_t o = __shfl_xor(v, stride, kWave);
+        const bool up = (pos & size) == 0;
+        const bool lower = (pos & stride) == 0;
+        v = (lower == up) ? min(v, o) : max(v, o);
+      }
+    }
+    const uint32_t prev = __shfl_up(v, 1, kWave);
+    const bool start = (pos == 0) || (v != prev);
+    const uint64_t startMask = __ballot(start);
+    const uint64_t segMask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << (seg * S));
+    // next run start in this segment (or segment end)
+    const uint64_t aboveMask = (lane == 63) ? 0ull : (startMask & segMask & ~((1ull << (lane + 1)) - 1ull));
+    const int next = aboveMask ? __builtin_ctzll(aboveMask) : (seg * S + S);
+    uint32_t cnt = uint32_t(next - lane);
+    bool entry = start && realRow;
+    if (v == 0xFFFFFFFFu) {
+      cnt -= uint32_t(S) - a.k;  // padding lanes sort last, after the invalid targets
+      entry = entry && cnt > 0 && a.keepInvalid;
+      if (start && realRow && cnt > 0 && a.pass == 0) atomicOr(a.anyInvalid, 1u);
+    }
+    const uint64_t entryMask = __ballot(entry) & segMask;
+    if (a.pass == 0) {
+      if (pos == 0 && realRow) a.rowCount[row] = uint32_t(__popcll(entryMask));
+    } else if (entry) {
+      const uint64_t below = entryMask & ((1ull << lane) - 1ull);
+      const uint32_t dst = a.rowPtr[row] + uint32_t(__popcll(below));
+      a.outCol[dst] = (v == 0xFFFFFFFFu) ? a.invalidLabel : v;
+      a.outCnt[dst] = cnt;
+    }
+  }
+}
+
+// Expand CSR row pointers into the COO row index per entry.
+__global__ __launch_bounds__(kBlock) void expandRowsKernel(const uint32_t* rowPtr, uint64_t nRows, uint32_t* rows) {
+  const uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (r >= nRows) return;
+  for (uint32_t e = rowPtr[r]; e < rowPtr[r + 1]; ++e) rows[e] = uint32_t(r);
+}
+
+// values = count / k (or count) as double (ImplicitBoxTree.hpp:3145-3158)
+__global__ __launch_bounds__(kBlock) void valuesKernel(const uint32_t* cnt, uint64_t nnz, double k, int useAbs,
+                                                       double* val) {
+  const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= nnz) return;
+  const double c = double(cnt[e]);
+  val[e] = useAbs ? c : c / k;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stable LSD radix sort of uint32 keys with two uint32 payloads (8-bit digits). Block-local
+// stable ranks come from wave ballots over the digit bits plus per-wave digit counts in LDS.
+// hist layout: [digit * nBlocks + block], so one exclusive scan yields every output offset.
+// ---------------------------------------------------------------------------------------------
+constexpr int kRadixItems = 8;
+constexpr int kRadixTile = kBlock * kRadixItems;
+
+__global__ __launch_bounds__(kBlock) void radixHistKernel(const uint32_t* keys, uint64_t n, int shift,
+                                                          uint32_t* hist, uint32_t nBlocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = uint64_t(blockIdx.x) * kRadixTile;
+  for (int t = 0; t < kRadixItems; ++t) {
+    const uint64_t i = base + uint64_t(t) * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  hist[uint64_t(threadIdx.x) * nBlocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void radixScatterKernel(const uint32_t* keysIn, const uint32_t* p0In,
+                                                             const uint32_t* p1In, uint64_t n, int shift,
+                                                             const uint32_t* offsets, uint32_t nBlocks,
+                                                             uint32_t* keysOut, uint32_t* p0Out, uint32_t* p1Out) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t waveCnt[kBlock / kWave][256];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  running[threadIdx.x] = offsets[uint64_t(threadIdx.x) * nBlocks + blockIdx.x];
+  const uint64_t base = uint64_t(blockIdx.x) * kRadixTile;
+  for (int t = 0; t < kRadixItems; ++t) {
+    for (int w = 0; w < kBlock / kWave; ++w) waveCnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t i = base + uint64_t(t) * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t key = 0, p0 = 0, p1 = 0, dg = 0;
+    if (valid) {
+      key = keysIn[i];
+      p0 = p0In[i];
+      p1 = p1In[i];
+      dg = (key >> shift) & 0xFFu;
+    }
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const uint64_t m = __ballot((dg >> bit) & 1u);
+      peers &= ((dg >> bit) & 1u) ? m : ~m;
+    }
+    const uint32_t rankInWave = uint32_t(__popcll(peers & ((1ull << lane) - 1ull)));
+    const int leader = peers ? __builtin_ctzll(peers) : -1;
+    if (valid && lane == leader) waveCnt[wave][dg] = uint32_t(__popcll(peers));
+    __syncthreads();
+    if (valid) {
+      uint32_t before = 0;
+      for (int w = 0; w < wave; ++w) before += waveCnt[w][dg];
+      const uint32_t dst = running[dg] + before + rankInWave;
+      keysOut[dst] = key;
+      p0Out[dst] = p0;
+      p1Out[dst] = p1;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) add += waveCnt[w][threadIdx.x];
+    running[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// Exclusive scan of a large uint32 array in place (three phases over kTile-element blocks).
+struct ScanArrayOp {
+  uint32_t* v;
+  __device__ uint32_t count(uint32_t i) const { return v[i]; }
+  __device__ void emit(uint32_t, uint32_t, uint32_t) const {}
+};
+
+__global__ __launch_bounds__(kBlock) void scanArrayScatterKernel(uint32_t* v, uint32_t n, const uint32_t* blockOffsets) {
+  const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
+  uint32_t c[kItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int t = 0; t < kItems; ++t) {
+    const uint64_t i = base + t;
+    c[t] = i < n ? v[i] : 0u;
+    s += c[t];
+  }
+  uint32_t tot;
+  uint32_t pos = blockExclusiveScan(s, tot) + blockOffsets[blockIdx.x];
+#pragma unroll
+  for (int t = 0; t < kItems; ++t) {
+    const uint64_t i = base + t;
+    if (i < n) v[i] = pos;
+    pos += c[t];
+  }
+}
+
+// offsets[v] = number of entries with idx < v, for sorted idx (compressIndexVector,
+// CooMatrix.hpp:14-56), v = 0..dim
+__global__ __launch_bounds__(kBlock) void compressKernel(const uint32_t* idx, uint64_t nnz, uint32_t dim,
+                                                         uint32_t* out) {
+  const uint64_t v = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (v > dim) return;
+  uint64_t lo = 0, len = nnz;
+  while (len > 0) {
+    const uint64_t half = len >> 1;
+    if (idx[lo + half] < v) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  out[v] = uint32_t(lo);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Power iteration on the CSC form (column j holds the entries P_ij in ascending row order):
+// y_j = sum_i mu_i P_ij summed in ascending row order, then mu <- y / ||y||_1. Block partial
+// sums go to fixed slots and are reduced in a fixed order, so the result is deterministic.
+// ---------------------------------------------------------------------------------------------
+__device__ inline double blockReduceSumF64(double v) {
+  __shared__ double waveSums[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  if (lane == 0) waveSums[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) t += waveSums[w];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kBlock) void spmvColsKernel(const uint32_t* colPtr, const uint32_t* rowIdx,
+                                                         const double* val, const double* mu, uint32_t n, double* y,
+                                                         double* partial) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  double s = 0.0;
+  if (j < n) {
+    for (uint32_t e = colPtr[j]; e < colPtr[j + 1]; ++e) s += mu[rowIdx[e]] * val[e];
+    y[j] = s;
+  }
+  const double t = blockReduceSumF64(fabs(s));
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void sumPartialsKernel(const double* partial, uint32_t m, double* out) {
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < m; i += kBlock) s += partial[i];
+  s = blockReduceSumF64(s);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ __launch_bounds__(kBlock) void normalizeKernel(const double* y, const double* norm, double* mu, uint32_t n,
+                                                          double* partial) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  double r = 0.0;
+  if (j < n) {
+    const double v = y[j] / *norm;
+    r = fabs(v - mu[j]);
+    mu[j] = v;
+  }
+  const double t = blockReduceSumF64(r);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+}  // namespace b12x

@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark: B12's set-oriented hot path on one MI355X.
+
+Workload (one "step" = one whole job of the Lorenz configuration): Lorenz flow map
+(sigma=10, rho=28, beta=8/3) as 20 RK4 steps of h=0.01, FP64; domain centre (0,0,27), radius
+(30,30,30); 24 subdivision steps {subdivide(SD); setFlagsByIterators(OverLeaves(InnerGrid(3)),
+HIT); remove(HIT); unsetFlags(HIT)}; then the transition matrix on the final cover and the
+invariant measure by power iteration until ||mu_{k+1} - mu_k||_1 <= 1e-12.
+
+Metric: test-point images per second over the whole job (images of the 24 subdivision levels
+plus the transition-matrix pass). Inputs are generated on the GPU; nothing crosses PCIe in the
+timed region except the per-level counts the API returns.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 vector, FMA counted as 2 flops
+PEAK_FP64_ISSUE_TFLOPS = 39.3    # the same peak when every FP64 op is a separate instruction
+LORENZ_FLOPS_PER_IMAGE = 1590    # 20 RK4 steps x 78 + grid/affine/key 30 (each + - * / once)
+
+CENTER, RADIUS, DEPTH, K = [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], 24, 27
+TOL = 1e-12
+MAX_ITER = 1_000_000
+
+
+def images_of(counts, n0=1):
+    """Test points imaged by the subdivision steps: k x (leaves after each subdivide)."""
+    imgs, n = 0, n0
+    for c in counts:
+        n_sub = n + int(c[0])   # every flagged leaf splits in two (depth < 64)
+        imgs += K * n_sub
+        n = int(c[3])           # leaves kept (all carried HIT)
+    return imgs, n
+
+
+def gpu_job(gx, flavour, timing=False):
+    t = gx.ImplicitBoxTree(CENTER, RADIUS, flavour=flavour)
+    grid, mp = gx.InnerGrid(3), gx.Lorenz()
+    if timing:
+        t.image_timing(1)
+    t.setFlags(gx.SD, 0)
+    t0 = time.perf_counter()
+    counts = t.subdivision_steps(grid, mp, DEPTH)
+    t1 = time.perf_counter()
+    m = t.transitionMatrixForLeaves(grid, mp)
+    t2 = time.perf_counter()
+    mu, iters, resid = m.power_iteration(MAX_ITER, TOL)
+    t3 = time.perf_counter()
+    imgs, n = images_of(counts)
+    imgs += K * n
+    res = {"images": imgs, "boxes": n, "nnz": m.nnz, "iters": iters, "resid": resid,
+           "subdiv_ms": (t1 - t0) * 1e3, "tm_ms": (t2 - t1) * 1e3, "measure_ms": (t3 - t2) * 1e3,
+           "level24_points": K * (int(counts[-2][3]) + int(counts[-1][0])) if DEPTH > 1 else 0}
+    if timing:
+        res["image_timing"] = t.image_timing(0)
+    return res
+
+
+def cpu_baseline(sample_depth, threads):
+    """B12's CPU path restated (oracle): same algorithm structure as the reference's host
+    Thrust path, including its second image evaluation per search, OpenMP over `threads`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import b12_amd as gx
+    from oracle_api import OracleTree
+
+    o = OracleTree(CENTER, RADIUS)
+    o.set_options(eval_twice=True)
+    grid, mp = gx.InnerGrid(3), gx.Lorenz()
+    o.setFlags(gx.SD, 0)
+    t0 = time.perf_counter()
+    counts = o.subdivision_steps(grid, mp, sample_depth)
+    m = o.transitionMatrixForLeaves(grid, mp)
+    _, iters, _ = m.power_iteration(MAX_ITER, TOL)
+    t1 = time.perf_counter()
+    imgs, n = images_of(counts)
+    imgs += K * n
+    return imgs / (t1 - t0), t1 - t0, imgs, iters
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--flavour", default="parity", choices=["parity", "fast"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-depth", type=int, default=24)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build's hot path runs on a single MI355X "
+              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import b12_amd as gx
+
+    dev = gx.device_name()
+    for _ in range(args.warmup):
+        gpu_job(gx, args.flavour)
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        runs.append(gpu_job(gx, args.flavour))
+    t1 = time.perf_counter()
+    prof = gpu_job(gx, args.flavour, timing=True)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    images = runs[0]["images"]
+    value = images * args.steps / (t1 - t0)
+
+    ms_img, launches, pts = prof["image_timing"]
+    achieved_tflops = pts * LORENZ_FLOPS_PER_IMAGE / (ms_img * 1e-3) / 1e12 if ms_img > 0 else 0.0
+    out = {
+        "metric": "test-point images/s (Lorenz RK4, depth 24)",
+        "value": value,
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seedless InnerGrid test points; the tree is generated on the GPU)",
+        "config": {"workload": "lorenz_rk4_depth24_tm_power1e-12", "depth": DEPTH, "points_per_box": K,
+                   "map": "Lorenz(10,28,8/3) RK4 20x0.01", "domain": {"center": CENTER, "radius": RADIUS},
+                   "build": args.flavour, "device": dev},
+        "roofline": {"bound": "fp64-valu", "kernel": "imageKernel<3,LORENZ,INNER>", "achieved": achieved_tflops,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+                     "frac_issue_rate": achieved_tflops / PEAK_FP64_ISSUE_TFLOPS,
+                     "flops_per_image": LORENZ_FLOPS_PER_IMAGE, "image_launches": launches,
+                     "image_points": pts, "image_ms": ms_img, "traffic": None},
+        "breakdown_ms": {k: runs[-1][k] for k in ("subdiv_ms", "tm_ms", "measure_ms")},
+        "job": {k: runs[-1][k] for k in ("images", "boxes", "nnz", "iters", "resid", "level24_points")},
+    }
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = min(threads, 16)
+        v, secs, imgs, iters = cpu_baseline(args.cpu_depth, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "images/s", "cores": threads, "kind": "port",
+                               "sample": f"same Lorenz job to depth {args.cpu_depth} + transition matrix "
+                                         f"and {iters} power iterations ({imgs} images, {secs:.1f} s), oracle "
+                                         f"with the reference's double image evaluation per search"}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

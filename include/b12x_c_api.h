@@ -1,0 +1,166 @@
+/*
+ * b12x C ABI — MI355X (gfx950) implementation of B12's set-oriented hot path.
+ *
+ * Every entry point takes plain pointers and sizes and returns an int status: 0 on success, a
+ * negative B12X_E_* code on failure (b12x_last_error() then holds a message). Reference
+ * citations are relative to B12c/src/include/ of SON-GPU-Enabler/B12.
+ *
+ * Object model (ImplicitBoxTree.h:32-640, CooMatrix.h:56-121):
+ *   b12x_tree    an implicit box tree: sorted leaves (key, depth, flags) resident in HBM
+ *   b12x_scheme  a (Grid, Map) pair — the functors a point-iterator pair is built from
+ *                (ImplicitBoxTree.h:266-430, MEX scheme{1,2}: ...mex.hpp.in:1501-1513,1619-1631)
+ *   b12x_coo     a transition matrix in COO form resident in HBM (CooMatrix.h:56-121)
+ * The point iterator of a call is (scheme, depth_it, flags_to_check): depth_it = 255 for
+ * makePointIteratorPairOverLeaves / ByFlags(-1, flags) and 0 for makePointIteratorPairOverRoot.
+ * Depth arguments follow B12: 255 (= -1) is the leaf depth, 0 the root; any other depth is
+ * accepted when every leaf lies on it, otherwise B12X_E_UNSUPPORTED.
+ *
+ * Threading: one tree or matrix handle must not be used from two threads at once (B12 is not
+ * thread-safe either, ImplicitBoxTree.h:116-122). Distinct handles are independent; each tree
+ * owns a HIP stream. Calls return after their GPU work is complete.
+ */
+#ifndef B12X_C_API_H
+#define B12X_C_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct b12x_tree_s* b12x_tree;
+typedef struct b12x_scheme_s* b12x_scheme;
+typedef struct b12x_coo_s* b12x_coo;
+
+enum {
+  B12X_OK = 0,
+  B12X_E_ARG = -1,          /* invalid argument */
+  B12X_E_HIP = -2,          /* HIP runtime error (message names the call) */
+  B12X_E_NOMEM = -3,        /* device allocation failed */
+  B12X_E_UNSUPPORTED = -4,  /* depth or map/grid combination outside this implementation */
+  B12X_E_NODEVICE = -5      /* no usable GPU */
+};
+
+/* Flags.h:11-22 */
+enum { B12X_NONE = 0x00, B12X_HIT = 0x01, B12X_INS = 0x02, B12X_EXPD = 0x04, B12X_SD = 0x08,
+       B12X_EQU = 0x10, B12X_ENTR = 0x20, B12X_EXTR = 0x40, B12X_ALL = 0xFF };
+#define B12X_LEAF_DEPTH 255 /* TypeDefinitions.hpp getLeafDepth */
+#define B12X_INVALID_NRBOXES 0xFFFFFFFFu /* TypeDefinitions.hpp getInvalidNrBoxes */
+
+/* Grid kinds (Grids.h:333-370); params are uint64:
+ *   B12X_GRID_INNER   {nPointsPerDim}       InnerGrid(n), k = n^dim      (Grids.hpp:99-134)
+ *   B12X_GRID_UNIFORM {nPoints, seed}       UniformGrid(k, seed), host (mt19937) semantics
+ *                                           (Grids.hpp:136-176)                              */
+enum { B12X_GRID_INNER = 0, B12X_GRID_UNIFORM = 1 };
+
+/* Map kinds; params are double (missing trailing params take the defaults shown):
+ *   B12X_MAP_IDENTITY       {}                              Map_Identity.h:9-21, any dim
+ *   B12X_MAP_HENON          {a=1.4, b=0.3, steps=1}         Map_Henon.h:26-40, dim 2
+ *   B12X_MAP_LORENZ         {sigma=10, rho=28, beta=8/3, h=0.01, steps=20}
+ *                                                           Map_Lorenz.h:29-44 (RK4Auto), dim 3
+ *   B12X_MAP_HENON_HEILES   {order=2, h=0.02, steps=50}     SymplecticIntegratorAuto<4>
+ *                           (MapTools.hpp:361-523) with the Hénon–Heiles force; no
+ *                           counterpart map in the reference, dim 4 (q1, q2, p1, p2)
+ *   B12X_MAP_DUFFING        {delta=0.25, alpha=-1, beta=1, gamma=0.3, omega=1, t0=0,
+ *                            tEnd=2*pi, h=2*pi/100}  RK4 (MapTools.hpp:35-108) of
+ *                           x'' + delta x' + alpha x + beta x^3 = gamma cos(omega t); no
+ *                           counterpart map in the reference, dim 2                           */
+enum { B12X_MAP_IDENTITY = 0, B12X_MAP_HENON = 1, B12X_MAP_LORENZ = 2, B12X_MAP_HENON_HEILES = 3,
+       B12X_MAP_DUFFING = 4 };
+
+const char* b12x_last_error(void);
+/* Name of the current device, for logs (ThrustSystem.hpp:52-62 printDevice). */
+int b12x_device_name(char* buf, int len);
+
+/* --- tree ------------------------------------------------------------------------------- */
+/* ImplicitBoxTree(center, radius[, sdScheme]) — ImplicitBoxTree.h:49-56, .hpp:120-145.
+ * sd_scheme may be NULL (default i % dim); otherwise n_sd entries repeated periodically. */
+int b12x_tree_create(int dim, const double* center, const double* radius, const uint8_t* sd_scheme, int n_sd,
+                     b12x_tree* out);
+int b12x_tree_destroy(b12x_tree t);
+/* size(), count(depth), depth() — ImplicitBoxTree.h:110-111,129-130,603 */
+int b12x_size(b12x_tree t, uint32_t* out);
+int b12x_count(b12x_tree t, uint8_t depth, uint32_t* out);
+int b12x_depth(b12x_tree t, uint8_t* out);
+/* Copy the leaves out / in as B12's SoA (bitPatternVector_, depthVector_, flagsVector_,
+ * ImplicitBoxTree.h:619-624). upload sorts the leaves into B12's order. */
+int b12x_tree_download(b12x_tree t, uint64_t* bits, uint8_t* depth, uint8_t* flags);
+int b12x_tree_upload(b12x_tree t, uint32_t n, const uint64_t* bits, const uint8_t* depth, const uint8_t* flags);
+
+/* subdivide(flags) — ImplicitBoxTree.h:487-489 (returns #leaves with flags set) */
+int b12x_subdivide(b12x_tree t, uint8_t flags, uint32_t* ret);
+/* unsubdivide(flags) — ImplicitBoxTree.h:543-545 */
+int b12x_unsubdivide(b12x_tree t, uint8_t flags, uint32_t* ret);
+/* remove(flags) — ImplicitBoxTree.h:432-434 */
+int b12x_remove(b12x_tree t, uint8_t flags, uint32_t* ret);
+/* setFlags / unsetFlags / changeFlags(…, depth) — ImplicitBoxTree.h:461-463, 527-529, 93-94 */
+int b12x_set_flags(b12x_tree t, uint8_t flags, uint8_t depth, uint32_t* ret);
+int b12x_unset_flags(b12x_tree t, uint8_t flags, uint8_t depth, uint32_t* ret);
+int b12x_change_flags(b12x_tree t, uint8_t flags_to_unset, uint8_t flags_to_set, uint8_t depth, uint32_t* ret);
+/* *ByStencil with a host bool array of count(depth) bytes — ImplicitBoxTree.h:482-485, 540-541,
+ * 106-108. op: 0 set, 1 unset, 2 change (flags2 used by change only). */
+int b12x_flags_by_stencil(b12x_tree t, int op, const uint8_t* stencil, uint8_t flags, uint8_t flags2, uint8_t depth,
+                          uint32_t* ret);
+/* *ByIterators over the image points of (scheme, depth_it, flags_to_check) —
+ * setFlagsByIterators ImplicitBoxTree.h:465-469 (= setFlagsBySearch(searchCompacted(...))),
+ * unsetFlagsByIterators :531-532, changeFlagsByIterators :96-98. */
+int b12x_flags_by_image(b12x_tree t, b12x_scheme s, uint8_t depth_it, uint8_t flags_to_check, int op, uint8_t flags,
+                        uint8_t flags2, uint8_t depth, uint32_t* ret);
+/* Number of points of the iterator pair (points.second - points.first). */
+int b12x_num_points(b12x_tree t, b12x_scheme s, uint8_t depth_it, uint8_t flags_to_check, uint64_t* out);
+/* searchByIterators(points, depth, out) — ImplicitBoxTree.h:439-444; out has num_points entries */
+int b12x_search_by_image(b12x_tree t, b12x_scheme s, uint8_t depth_it, uint8_t flags_to_check, uint8_t depth,
+                         uint32_t* out);
+/* searchCompacted(points, depth) — ImplicitBoxTree.h:458-459: sorted unique hit indices.
+ * out may be NULL to query n_out; otherwise it must hold count(depth) entries. */
+int b12x_search_compacted(b12x_tree t, b12x_scheme s, uint8_t depth_it, uint8_t flags_to_check, uint8_t depth,
+                          uint32_t* out, uint64_t* n_out);
+/* searchByPoints(points, depth) — ImplicitBoxTree.h:448-453; points row-major n x dim */
+int b12x_search_by_points(b12x_tree t, const double* points, uint64_t n, uint8_t depth, uint32_t* out);
+
+/* One subdivision step of subdivision.m:18-25 (subdivide(sd_flag); setFlagsByIterators(
+ * OverLeaves, HIT); remove(HIT); unsetFlags(HIT)) fused into three kernels; steps are repeated
+ * n_steps times. counts (may be NULL) receives 4 values per step: the return values of the
+ * four reference calls. */
+int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd_flag, int n_steps, uint32_t* counts);
+
+/* Measurement hook (no counterpart in the reference; used by the benchmark): enable=1 clears
+ * and enables HIP-event timing of every image-kernel launch on the tree's stream; the totals
+ * since then are returned (milliseconds, launches, test points). enable=0 disables timing. */
+int b12x_tree_image_timing(b12x_tree t, int enable, double* ms, uint64_t* launches, uint64_t* points);
+
+/* --- scheme ----------------------------------------------------------------------------- */
+int b12x_scheme_create(int dim, int grid_kind, const uint64_t* grid_params, int n_grid_params, int map_kind,
+                       const double* map_params, int n_map_params, b12x_scheme* out);
+int b12x_scheme_destroy(b12x_scheme s);
+/* Grid::getNumberOfPointsPerBox — Grids.h:76 */
+int b12x_scheme_points_per_box(b12x_scheme s, uint64_t* out);
+/* Apply the map to n host points (row-major n x dim) on the GPU — the functor protocol of
+ * doc/B12_doc.tex:109-146, exposed for integrator checks. */
+int b12x_map_points(b12x_scheme s, const double* in, double* out, uint64_t n);
+
+/* --- transition matrix -------------------------------------------------------------------- */
+/* transitionMatrixForLeaves/ForRoot<double>(grid, map, useAbsoluteValues, useRowMajor,
+ * useInvalidNrBoxes) — ImplicitBoxTree.h:491-525, .hpp:2932-3211. */
+int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t depth_it, uint8_t flags_to_check, uint8_t depth,
+                           int use_abs, int row_major, int use_invalid, b12x_coo* out);
+int b12x_coo_destroy(b12x_coo m);
+/* nnz, nRows_, nColumns_ — CooMatrix.h:118-120 */
+int b12x_coo_info(b12x_coo m, uint64_t* nnz, uint32_t* n_rows, uint32_t* n_cols, int* row_major);
+/* rowIndices_, columnIndices_, values_ to host (any pointer may be NULL) */
+int b12x_coo_download(b12x_coo m, uint32_t* rows, uint32_t* cols, double* vals);
+/* Build a matrix from host COO arrays (CooMatrix.h:66-69). */
+int b12x_coo_upload(const uint32_t* rows, const uint32_t* cols, const double* vals, uint64_t nnz, uint32_t n_rows,
+                    uint32_t n_cols, b12x_coo* out);
+/* compressIndexVector(indices, dim) — CooMatrix.h:20-23: sorted index vector -> dim+1 offsets */
+int b12x_compress_index_vector(const uint32_t* indices, uint64_t n, uint32_t dim, uint32_t* out);
+/* Invariant measure: mu <- mu P / ||mu P||_1, mu0 = 1/n (or the given mu if mu_init != 0),
+ * until max_iter iterations or ||mu_{k+1} - mu_k||_1 <= tol (tol < 0: fixed count). The
+ * reference computes this step in MATLAB (subdivisionByNim.m:30-59); there is no C++ API. */
+int b12x_power_iteration(b12x_coo m, int max_iter, double tol, int mu_init, double* mu, int* iters, double* resid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* B12X_C_API_H */

@@ -1,0 +1,958 @@
+// b12cpu — CPU restatement of B12's set-oriented hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This header is the oracle that the HIP product is
+// checked against. Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it (through liboracle_b12cpu.so); the product
+// library never includes, links or calls it.
+//
+// It restates, in plain C++17 with FP contraction off, the host (Thrust CPP/OMP)
+// semantics of the reference files cited per function below (paths relative to
+// B12c/src/include/). Every FP64 expression keeps the reference's operation
+// order so that covers, leaf keys and hit counts are bit-exact.
+//
+// Parity pins (see tests/): integrators are pinned bit-for-bit against the
+// reference's own Map headers compiled into oracle/_ref; key algebra, descent and
+// tree operations against closed forms and brute force on small trees; UniformGrid
+// against libstdc++'s std::mt19937 / uniform_real_distribution; power iteration
+// against numpy/scipy eigensolvers. Functions with no such check say
+// "parity unpinned" in their comment.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace b12cpu {
+
+using Depth = uint8_t;      // TypeDefinitions.h:19
+using Flags = uint8_t;      // Flags.h:9
+using NrBoxes = uint32_t;   // TypeDefinitions.h:45
+using NrPoints = uint64_t;  // TypeDefinitions.h:57
+
+constexpr Depth kLeafDepth = 255;        // TypeDefinitions.hpp:4-7 getLeafDepth
+constexpr Depth kUnsubDepth = 254;       // TypeDefinitions.hpp:9-12 getUnsubdividableDepth
+constexpr Depth kMaxDepth = 64;          // TypeDefinitions.hpp:14-19 getMaxDepth<64>
+constexpr NrBoxes kInvalid = 0xFFFFFFFFu; // TypeDefinitions.hpp:21-26
+
+// Flags.h:11-22
+enum : Flags { NONE = 0x00, HIT = 0x01, INS = 0x02, EXPD = 0x04, SD = 0x08,
+               EQU = 0x10, ENTR = 0x20, EXTR = 0x40, ALL = 0xFF };
+
+// Flags.hpp:9-37
+inline bool areFlagsSet(Flags f, Flags c) { return c == (f & c); }
+inline bool isAnyFlagSet(Flags f, Flags c) { return (f & c) != 0; }
+inline Flags setF(Flags f, Flags s) { return f | s; }
+inline Flags unsetF(Flags f, Flags u) { return f & Flags(~u); }
+inline Flags changeF(Flags f, Flags u, Flags s) { return areFlagsSet(f, u) ? setF(unsetF(f, u), s) : f; }
+
+struct Box {
+  uint64_t bits;
+  Depth depth;
+  Flags flags;
+};
+
+// BitPattern.hpp:4-28
+inline uint64_t shl(uint64_t b, int i) { return i <= 0 ? b : (i >= 64 ? 0ull : b << i); }
+inline uint64_t shr(uint64_t b, int i) { return i <= 0 ? b : (i >= 64 ? 0ull : b >> i); }
+
+// Box.hpp:7-19, 74-83
+inline Box surroundingBox() { return {0ull, 0, NONE}; }
+inline Box invalidBox() { return {~0ull, kLeafDepth, NONE}; }
+inline bool isInvalid(const Box& b) { return b.depth > kMaxDepth; }
+inline bool isSurrounding(const Box& b) { return b.depth == 0 && b.bits == 0; }
+
+// Box.hpp:65-72 getComparableBitPattern
+inline uint64_t comparableKey(const Box& b) {
+  int d = int(kMaxDepth) - int(b.depth);
+  return shl(b.bits, d) | (shl(1ull, d - 1) - 1ull);
+}
+
+// Box.hpp:74-83 hasSamePath
+inline bool hasSamePath(const Box& a, const Box& b) {
+  int diff = int(a.depth) - int(b.depth);
+  return (isInvalid(a) && isInvalid(b)) || shr(a.bits, diff) == shr(b.bits, -diff);
+}
+
+// Box.hpp:113-120 isSameBox
+inline bool isSameBox(const Box& a, const Box& b) {
+  return (isInvalid(a) && isInvalid(b)) || (a.bits == b.bits && a.depth == b.depth);
+}
+
+// Box.hpp:122-135 isStrictlyPreceding
+inline bool isStrictlyPreceding(const Box& a, const Box& b) {
+  uint64_t ka = comparableKey(a), kb = comparableKey(b);
+  return !isInvalid(a) && (isInvalid(b) || ka < kb || (ka == kb && b.depth < a.depth));
+}
+
+// Box.hpp:138-147 incrementDepthBy
+inline Box incrementDepthBy(const Box& b, Depth d) {
+  d = std::min<Depth>(d, Depth(kMaxDepth - b.depth));
+  return {shl(b.bits, d), Depth(b.depth + d), b.flags};
+}
+
+// Box.hpp:149-157 decrementDepthBy
+inline Box decrementDepthBy(const Box& b, Depth d) {
+  d = std::min<Depth>(d, b.depth);
+  return {shr(b.bits, d), Depth(b.depth - d), b.flags};
+}
+
+// Box.hpp:159-174 decrementDepthTo
+inline Box decrementDepthTo(const Box& b, Depth d) {
+  Box r = b;
+  if (d < b.depth) {
+    r.bits = shr(b.bits, int(b.depth) - int(d));
+    r.depth = d;
+  } else if (d == kUnsubDepth) {
+    r.bits = shr(b.bits, 1);
+    r.depth = Depth(r.depth - (b.depth > 0 ? 1 : 0));
+  }
+  return r;
+}
+
+// Box.hpp:200-204 hasAtLeastDepth
+inline bool hasAtLeastDepth(const Box& b, Depth d) {
+  return b.depth >= d || d == kUnsubDepth || d == kLeafDepth;
+}
+
+// Box.hpp:225-239
+inline bool isRightHalf(const Box& b) { return (b.bits & 1ull) != 0; }
+inline Box computeOtherHalf(const Box& b) {
+  return {b.bits ^ uint64_t(!isSurrounding(b) && !isInvalid(b)), b.depth, b.flags};
+}
+
+// ---------------------------------------------------------------------------------------------
+// Domain geometry: centre, radius, subdivision scheme (ImplicitBoxTree.hpp:27-45, 574-587,
+// 1972-1979, 3512-3552)
+// ---------------------------------------------------------------------------------------------
+template <int DIM>
+struct Geometry {
+  double center[DIM];
+  double radius[DIM];
+  std::vector<uint8_t> sdScheme;   // sdScheme[i] = coordinate bisected at level i
+  std::vector<uint8_t> sdCount;    // sdCount[d*DIM + j] = bisections of j above depth d
+  bool isDefault = true;
+
+  Geometry() {
+    for (int j = 0; j < DIM; ++j) { center[j] = 0.5; radius[j] = 0.5; }
+    sdScheme.resize(kMaxDepth);
+    for (int i = 0; i < kMaxDepth; ++i) sdScheme[i] = uint8_t(i % DIM);
+    adapt();
+  }
+
+  // ImplicitBoxTree.hpp:3512-3552 setSdScheme (periodic extension to maxDepth+1 entries)
+  bool setScheme(const uint8_t* s, int n) {
+    for (int i = 0; i < n; ++i) if (s[i] >= DIM) return false;
+    sdScheme.assign(kMaxDepth + 1, 0);
+    if (n > 0) {
+      for (int i = 0; i < kMaxDepth + 1; ++i) sdScheme[i] = s[i % n];
+    } else {
+      for (int i = 0; i < kMaxDepth + 1; ++i) sdScheme[i] = uint8_t(i % DIM);
+    }
+    adapt();
+    return true;
+  }
+
+  // ImplicitBoxTree.hpp:574-587 adaptSdCount; :1972-1979 isSdSchemeDefault
+  void adapt() {
+    sdCount.assign((kMaxDepth + 2) * DIM, 0);
+    for (size_t i = 0; i < sdScheme.size(); ++i) {
+      for (int j = 0; j < DIM; ++j) sdCount[(i + 1) * DIM + j] = sdCount[i * DIM + j];
+      ++sdCount[(i + 1) * DIM + sdScheme[i]];
+    }
+    isDefault = true;
+    for (size_t i = 0; i < sdScheme.size(); ++i)
+      if (sdScheme[i] != uint8_t(uint8_t(i) % DIM)) isDefault = false;
+  }
+
+  // coordinateFunctions.hpp:7-19 isPointInBox
+  bool inDomain(const double* p) const {
+    bool res = true;
+    for (int j = 0; j < DIM; ++j) res = res && center[j] - radius[j] <= p[j] && p[j] < center[j] + radius[j];
+    return res;
+  }
+
+  // coordinateFunctions.hpp:61-93 computeBoxFromPointFixedSdScheme
+  Box boxFromPointFixed(const double* p, Flags flags) const {
+    Box res{0ull, kMaxDepth, flags};
+    if (!inDomain(p)) return invalidBox();
+    for (int j = 0; j < DIM; ++j) {
+      int nb = kMaxDepth / DIM + (j < kMaxDepth % DIM ? 1 : 0);
+      uint64_t b = (uint64_t)std::floor(std::ldexp(((p[j] - center[j]) / radius[j] + 1.0) / 2.0, nb));
+      int initialShift = (kMaxDepth - 1 - j) % DIM;
+      for (int i = 0; i * DIM + initialShift < kMaxDepth; ++i)
+        res.bits |= shl(b, i * (DIM - 1) + initialShift) & shl(1ull, i * DIM + initialShift);
+    }
+    return res;
+  }
+
+  // coordinateFunctions.hpp:21-59 computeBoxFromPoint (general subdivision scheme)
+  Box boxFromPointGeneral(const double* p, Flags flags) const {
+    Box res{0ull, kMaxDepth, flags};
+    if (!inDomain(p)) return invalidBox();
+    const uint8_t* cnt = sdCount.data() + kMaxDepth * DIM;
+    uint64_t bits[DIM];
+    for (int j = 0; j < DIM; ++j)
+      bits[j] = (uint64_t)std::floor(std::ldexp(((p[j] - center[j]) / radius[j] + 1.0) / 2.0, cnt[j]));
+    for (int i = kMaxDepth - 1; i >= 0; --i) {
+      for (int j = 0; j < DIM; ++j) {
+        bool t = sdScheme[i] == j;
+        res.bits |= shl(uint64_t(t) & bits[j], kMaxDepth - 1 - i);
+        bits[j] = shr(bits[j], int(t));
+      }
+    }
+    return res;
+  }
+
+  Box boxFromPoint(const double* p, Flags flags) const {
+    return isDefault ? boxFromPointFixed(p, flags) : boxFromPointGeneral(p, flags);
+  }
+
+  // coordinateFunctions.hpp:128-159 computeCoordinatesOfBoxFixedSdScheme -> centre c, radius rb
+  void coordsFixed(const Box& box, double* c, double* rb) const {
+    int d = box.depth;
+    for (int j = 0; j < DIM; ++j) {
+      uint64_t bits = 0;
+      int initialShift = (DIM + d - 1 - j) % DIM;
+      for (int i = 0; i * DIM < kMaxDepth; ++i)
+        bits |= shr(box.bits, i * (DIM - 1) + initialShift) & shl(1ull, i);
+      int nb = d / DIM + (j < d % DIM ? 1 : 0);
+      rb[j] = std::ldexp(radius[j], -nb);
+      c[j] = (std::ldexp(double(bits), -nb) * 2.0 - 1.0) * radius[j] + center[j] + rb[j];
+    }
+  }
+
+  // coordinateFunctions.hpp:95-126 computeCoordinatesOfBox (general scheme)
+  void coordsGeneral(const Box& box, double* c, double* rb) const {
+    uint64_t bits[DIM] = {};
+    int d = box.depth;
+    const uint8_t* cnt = sdCount.data() + d * DIM;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < DIM; ++j) {
+        bool t = sdScheme[i] == j;
+        bits[j] = shl(bits[j], int(t)) | (uint64_t(t) & shr(box.bits, d - 1 - i));
+      }
+    for (int j = 0; j < DIM; ++j) {
+      rb[j] = std::ldexp(radius[j], -int(cnt[j]));
+      c[j] = (std::ldexp(double(bits[j]), -int(cnt[j])) * 2.0 - 1.0) * radius[j] + center[j] + rb[j];
+    }
+  }
+
+  void coords(const Box& box, double* c, double* rb) const {
+    if (isDefault) coordsFixed(box, c, rb); else coordsGeneral(box, c, rb);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Random numbers for UniformGrid: std::mt19937 + uniform_real_distribution<double>(-1,1) as in
+// libstdc++ (bits/random.tcc generate_canonical; bits/random.h:1870), restated so the result
+// does not depend on the host's library. Pinned against std:: in tests/.
+// ---------------------------------------------------------------------------------------------
+struct MT19937 {
+  uint32_t x[624];
+  int idx;
+  explicit MT19937(uint64_t seed) {
+    x[0] = uint32_t(seed);
+    for (int i = 1; i < 624; ++i) x[i] = 1812433253u * (x[i - 1] ^ (x[i - 1] >> 30)) + uint32_t(i);
+    idx = 624;
+  }
+  void twist() {
+    for (int k = 0; k < 624; ++k) {
+      uint32_t y = (x[k] & 0x80000000u) | (x[(k + 1) % 624] & 0x7fffffffu);
+      x[k] = x[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    idx = 0;
+  }
+  uint32_t operator()() {
+    if (idx >= 624) twist();
+    uint32_t y = x[idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+  }
+};
+
+// generate_canonical<double,53>(mt19937) then *(b-a)+a with a=-1, b=1
+inline double uniformMinus1To1(MT19937& g) {
+  double sum = 0.0, tmp = 1.0;
+  for (int k = 0; k < 2; ++k) {
+    sum += double(g()) * tmp;
+    tmp *= 4294967296.0;
+  }
+  double ret = sum / tmp;
+  if (ret >= 1.0) ret = std::nextafter(1.0, 0.0);
+  return (ret * 2.0) + -1.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Grids (Grids.hpp:99-176). kind 0 = InnerGrid(n), kind 1 = UniformGrid(k, seed) host semantics.
+// ---------------------------------------------------------------------------------------------
+enum GridKind : int { GRID_INNER = 0, GRID_UNIFORM = 1 };
+
+template <int DIM>
+struct Grid {
+  int kind = GRID_INNER;
+  uint64_t n = 1;      // InnerGrid points per dimension
+  uint64_t k = 1;      // points per box
+  uint64_t seed = 0;   // UniformGrid seed
+
+  uint64_t pointsPerBox() const { return k; }
+
+  // Grids.hpp:112-117 InnerGrid::operator(); Grids.hpp:152-158 UniformGrid host branch
+  void operator()(uint64_t i, double* out) const {
+    if (kind == GRID_INNER) {
+      uint64_t npd = 1;
+      for (int j = 0; j < DIM; ++j) {
+        out[j] = (double((i / npd) % n) * 2.0 + 1.0 - double(n)) / double(n);
+        npd *= n;
+      }
+    } else {
+      MT19937 g(seed + i);
+      for (int j = 0; j < DIM; ++j) out[j] = uniformMinus1To1(g);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Maps (MapTools.hpp, Map_Henon.h, Map_Lorenz.h, Map_Identity.h; the Hénon–Heiles and forced
+// Duffing right-hand sides are defined by this build: they have no counterpart in the reference)
+// ---------------------------------------------------------------------------------------------
+enum MapKind : int { MAP_IDENTITY = 0, MAP_HENON = 1, MAP_LORENZ = 2, MAP_HENON_HEILES = 3,
+                     MAP_DUFFING = 4 };
+
+// Forcing table for the non-autonomous RK4 (MapTools.hpp:54-105): the time loop
+// for (t = t0; t < tEnd; t += h_) { h = min(h_, tEnd - t); ... t, t + h/2, t + h ... }
+// is state independent, so it is evaluated once; f = gamma*cos(omega*time).
+struct ForcingStep {
+  double h, h6, h2, h3, f0, fh, f1;
+};
+
+inline std::vector<ForcingStep> buildForcingTable(double t0, double tEnd, double hIn, double gamma,
+                                                  double omega) {
+  std::vector<ForcingStep> tab;
+  for (double t = t0; t < tEnd; t += hIn) {
+    double h = (tEnd - t) < hIn ? (tEnd - t) : hIn;
+    ForcingStep s;
+    s.h = h;
+    s.h6 = h / 6.0;
+    s.h2 = h / 2.0;
+    s.h3 = h / 3.0;
+    s.f0 = gamma * std::cos(omega * t);
+    s.fh = gamma * std::cos(omega * (t + h / 2.0));
+    s.f1 = gamma * std::cos(omega * (t + h));
+    tab.push_back(s);
+    if (tab.size() > 100000000) break;
+  }
+  return tab;
+}
+
+template <int DIM>
+struct Map {
+  int kind = MAP_IDENTITY;
+  // Henon: a, b; Lorenz: sigma, rho, beta; HH: -; Duffing: delta, alpha, beta
+  double p0 = 0, p1 = 0, p2 = 0;
+  double h = 0;
+  uint32_t steps = 1;
+  int order = 2;  // symplectic order
+  double cbrt2 = 0, factor = 0;  // order-4 constants (host cbrt)
+  std::vector<ForcingStep> forcing;
+
+  // Map_Henon.h:17-22 HenonMap::operator()
+  void henon(const double* x, double* y) const {
+    y[0] = 1.0 - p0 * x[0] * x[0] + x[1];
+    y[1] = p1 * x[0];
+  }
+
+  // Map_Lorenz.h:20-25 LorenzRHS::operator()
+  void lorenzRhs(const double* x, double* f) const {
+    f[0] = p0 * (x[1] - x[0]);
+    f[1] = p1 * x[0] - x[1] - x[0] * x[2];
+    f[2] = -p2 * x[2] + x[0] * x[1];
+  }
+
+  // Hénon–Heiles V(q) = (q1^2+q2^2)/2 + q1^2 q2 - q2^3/3: F2(q) = -grad V (build-defined)
+  static void hhForce(const double* q, double* f) {
+    f[0] = -q[0] - 2.0 * q[0] * q[1];
+    f[1] = -q[1] - q[0] * q[0] + q[1] * q[1];
+  }
+
+  // Forced Duffing x'' + delta x' + alpha x + beta x^3 = gamma cos(omega t) (build-defined)
+  void duffingRhs(double f, const double* x, double* r) const {
+    r[0] = x[1];
+    r[1] = f - p0 * x[1] - p1 * x[0] - p2 * x[0] * x[0] * x[0];
+  }
+
+  void operator()(const double* in, double* out) const {
+    switch (kind) {
+      case MAP_IDENTITY:  // Map_Identity.h:11-20
+        for (int j = 0; j < DIM; ++j) out[j] = in[j];
+        break;
+      case MAP_HENON: {  // MapTools.hpp:4-32 IteratingMap
+        uint32_t i = 0;
+        if (steps & 1u) {
+          henon(in, out);
+          ++i;
+        } else {
+          for (int j = 0; j < DIM; ++j) out[j] = in[j];
+        }
+        for (; i < steps; i += 2u) {
+          double t[2];
+          henon(out, t);
+          henon(t, out);
+        }
+        break;
+      }
+      case MAP_LORENZ: {  // MapTools.hpp:116-182 RK4Auto::operator()
+        double x[3], k[3], a[3];
+        for (int j = 0; j < 3; ++j) x[j] = in[j];
+        for (uint32_t s = 0; s < steps; ++s) {
+          lorenzRhs(x, k);
+          for (int j = 0; j < 3; ++j) out[j] = x[j] + h / 6.0 * k[j];
+          for (int j = 0; j < 3; ++j) a[j] = x[j] + h / 2.0 * k[j];
+          lorenzRhs(a, k);
+          for (int j = 0; j < 3; ++j) out[j] += h / 3.0 * k[j];
+          for (int j = 0; j < 3; ++j) a[j] = x[j] + h / 2.0 * k[j];
+          lorenzRhs(a, k);
+          for (int j = 0; j < 3; ++j) out[j] += h / 3.0 * k[j];
+          for (int j = 0; j < 3; ++j) a[j] = x[j] + h * k[j];
+          lorenzRhs(a, k);
+          for (int j = 0; j < 3; ++j) out[j] += h / 6.0 * k[j];
+          for (int j = 0; j < 3; ++j) x[j] = out[j];
+        }
+        break;
+      }
+      case MAP_HENON_HEILES: {  // MapTools.hpp:369-523 SymplecticIntegratorAuto<4>
+        double* q = out;
+        double* p = out + 2;
+        double t[2];
+        for (int j = 0; j < 4; ++j) out[j] = in[j];
+        for (uint32_t s = 0; s < steps; ++s) {
+          if (order == 1) {
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] += t[j] * h;
+          } else if (order == 2) {
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] += 0.5 * t[j] * h;
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] += 0.5 * t[j] * h;
+          } else if (order == 3) {
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] -= 1.0 / 24.0 * t[j] * h;
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] -= 2.0 / 3.0 * t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] += 3.0 / 4.0 * t[j] * h;
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += 2.0 / 3.0 * t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] += 7.0 / 24.0 * t[j] * h;
+          } else {
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += 0.5 * factor * t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] += factor * t[j] * h;
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += 0.5 * (1.0 - cbrt2) * factor * t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] -= cbrt2 * factor * t[j] * h;
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += 0.5 * (1.0 - cbrt2) * factor * t[j] * h;
+            for (int j = 0; j < 2; ++j) t[j] = p[j];
+            for (int j = 0; j < 2; ++j) q[j] += factor * t[j] * h;
+            hhForce(q, t);
+            for (int j = 0; j < 2; ++j) p[j] += 0.5 * factor * t[j] * h;
+          }
+        }
+        break;
+      }
+      case MAP_DUFFING: {  // MapTools.hpp:40-108 RK4::operator() with the forcing table
+        double x[2], k[2], a[2];
+        for (int j = 0; j < 2; ++j) x[j] = in[j];
+        for (const ForcingStep& st : forcing) {
+          duffingRhs(st.f0, x, k);
+          for (int j = 0; j < 2; ++j) out[j] = x[j] + st.h6 * k[j];
+          for (int j = 0; j < 2; ++j) a[j] = x[j] + st.h2 * k[j];
+          duffingRhs(st.fh, a, k);
+          for (int j = 0; j < 2; ++j) out[j] += st.h3 * k[j];
+          for (int j = 0; j < 2; ++j) a[j] = x[j] + st.h2 * k[j];
+          duffingRhs(st.fh, a, k);
+          for (int j = 0; j < 2; ++j) out[j] += st.h3 * k[j];
+          for (int j = 0; j < 2; ++j) a[j] = x[j] + st.h * k[j];
+          duffingRhs(st.f1, a, k);
+          for (int j = 0; j < 2; ++j) out[j] += st.h6 * k[j];
+          for (int j = 0; j < 2; ++j) x[j] = out[j];
+        }
+        break;
+      }
+      default:
+        throw std::runtime_error("unknown map kind");
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// COO matrix (CooMatrix.h:56-121)
+// ---------------------------------------------------------------------------------------------
+struct Coo {
+  std::vector<NrBoxes> rows, cols;
+  std::vector<double> vals;
+  NrBoxes nRows = 0, nCols = 0;
+};
+
+// CooMatrix.hpp:14-56 compressIndexVector: sorted index vector -> offsets of length dim+1
+inline std::vector<NrBoxes> compressIndexVector(const std::vector<NrBoxes>& idx, NrBoxes dim) {
+  std::vector<NrBoxes> temp(size_t(dim) + 1, 0);
+  size_t i = 0;
+  while (i < idx.size()) {
+    size_t j = i;
+    while (j < idx.size() && idx[j] == idx[i]) ++j;
+    if (size_t(idx[i]) + 1 < temp.size()) temp[idx[i] + 1] = NrBoxes(j - i);
+    i = j;
+  }
+  std::vector<NrBoxes> out(temp.size());
+  NrBoxes s = 0;
+  for (size_t k = 0; k < temp.size(); ++k) { s += temp[k]; out[k] = s; }
+  return out;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The tree: leaf-only sorted SoA (ImplicitBoxTree.h:607-639)
+// ---------------------------------------------------------------------------------------------
+template <int DIM>
+struct PointSet {
+  const Grid<DIM>* grid;
+  const Map<DIM>* map;
+  std::vector<Box> boxes;  // source boxes (already reduced to the iterator's depth)
+  Flags flagsToSet = NONE;
+  uint64_t size() const { return uint64_t(boxes.size()) * grid->pointsPerBox(); }
+};
+
+template <int DIM>
+class Tree {
+ public:
+  Geometry<DIM> geo;
+  std::vector<uint64_t> bits;
+  std::vector<Depth> depthV;
+  std::vector<Flags> flagsV;
+  bool evalTwice = false;        // mirror the reference's second image evaluation per search
+  uint64_t chunkPoints = ~0ull;  // search buffer size (ImplicitBoxTree.hpp:2643-2680 chunking)
+
+  Tree() { insertSurroundingBoxIfEmpty(); }
+
+  NrBoxes size() const { return NrBoxes(bits.size()); }
+  Box box(size_t i) const { return {bits[i], depthV[i], flagsV[i]}; }
+  void setBox(size_t i, const Box& b) { bits[i] = b.bits; depthV[i] = b.depth; flagsV[i] = b.flags; }
+  void resize(size_t n) { bits.resize(n); depthV.resize(n); flagsV.resize(n); }
+
+  // ImplicitBoxTree.hpp:1959-1970
+  void insertSurroundingBoxIfEmpty() {
+    if (bits.empty()) { resize(1); setBox(0, surroundingBox()); }
+  }
+
+  // ImplicitBoxTree.hpp:1264-1271
+  Depth depth() const {
+    Depth m = 0;
+    for (Depth d : depthV) m = std::max(m, d);
+    return m;
+  }
+
+  // ImplicitBoxTree.hpp:589-609
+  bool areAllLeavesOnDepth(Depth d) const {
+    if (d == kLeafDepth) return true;
+    if (d == kUnsubDepth) return false;
+    for (Depth x : depthV) if (x != d) return false;
+    return true;
+  }
+
+  // Functors.hpp:337-377 IsUnsubdividableFunctor over (box, pred, succ) of
+  // ImplicitBoxTree.hpp:2023-2058 isUnsubdividable_begin
+  bool isUnsubdividable(size_t i, Flags f) const {
+    size_t n = bits.size();
+    Box b = box(i), pr = box(i > 0 ? i - 1 : 0), su = box(std::min(i + 1, n - 1));
+    bool res = areFlagsSet(b.flags, f) && b.depth > 0;
+    if (res) {
+      const Box& nb = isRightHalf(b) ? pr : su;
+      res = isSameBox(decrementDepthBy(b, 1), decrementDepthBy(nb, 1))
+                ? areFlagsSet(nb.flags, f)
+                : !hasSamePath(decrementDepthBy(b, 1), nb);
+    }
+    return res;
+  }
+
+  // Functors.hpp:278-285 IsUnequalToPredecessorInDepthFunctor over
+  // ImplicitBoxTree.hpp:1990-2021 isNewBoxInDepth_begin
+  bool isNewBoxInDepth(size_t i, Depth d) const {
+    Box b = decrementDepthTo(box(i), d);
+    Box p = decrementDepthTo(box(i > 0 ? i - 1 : 0), d);
+    return hasAtLeastDepth(b, d) && (i == 0 || !isSameBox(b, p));
+  }
+
+  // ImplicitBoxTree.hpp:1060-1095 count
+  NrBoxes count(Depth d) const {
+    if (d == 0) return 1;
+    if (d == kLeafDepth || areAllLeavesOnDepth(d)) return size();
+    NrBoxes c = 0;
+    if (d == kUnsubDepth) {
+      for (size_t i = 0; i < bits.size(); ++i) c += isNewBoxInDepth(i, d) && isUnsubdividable(i, NONE);
+      return c;
+    }
+    if (d <= depth()) {
+      for (size_t i = 0; i < bits.size(); ++i) c += isNewBoxInDepth(i, d);
+      return c;
+    }
+    return 0;
+  }
+
+  // Depth variants other than {0, leaf, all-leaves-on-depth} are not part of this restatement.
+  void requireSupportedDepth(Depth d) const {
+    if (!(d == 0 || d == kLeafDepth || areAllLeavesOnDepth(d)))
+      throw std::runtime_error("depth must be 0, -1 (leaves) or the common depth of all leaves");
+  }
+
+  // countIfFlags (ImplicitBoxTree.hpp:1097-1214) fused into the stencil flag operations
+  // setFlagsByStencil :2803-2881, unsetFlagsByStencil :3334-3411, changeFlagsByStencil :912-1058
+  template <class Pred, class Comb, class Apply>
+  NrBoxes flagsByStencil(const uint8_t* stencil, Depth d, Pred pred, Comb comb, Apply apply) {
+    requireSupportedDepth(d);
+    NrBoxes n = 0;
+    if (d == 0) {
+      Flags acc = flagsV.front();
+      for (Flags f : flagsV) acc = comb(acc, f);
+      n = (stencil[0] && pred(acc)) ? 1 : 0;
+      if (stencil[0]) for (auto& f : flagsV) f = apply(f);
+    } else {
+      for (size_t i = 0; i < flagsV.size(); ++i) n += (pred(flagsV[i]) && stencil[i]) ? 1 : 0;
+      for (size_t i = 0; i < flagsV.size(); ++i) if (stencil[i]) flagsV[i] = apply(flagsV[i]);
+    }
+    return n;
+  }
+
+  NrBoxes setFlagsByStencil(const uint8_t* st, Flags f, Depth d) {
+    return flagsByStencil(st, d, [f](Flags x) { return !areFlagsSet(x, f); },
+                          [](Flags a, Flags b) { return Flags(a & b); }, [f](Flags x) { return setF(x, f); });
+  }
+  NrBoxes unsetFlagsByStencil(const uint8_t* st, Flags f, Depth d) {
+    return flagsByStencil(st, d, [f](Flags x) { return isAnyFlagSet(x, f); },
+                          [](Flags a, Flags b) { return Flags(a | b); }, [f](Flags x) { return unsetF(x, f); });
+  }
+  NrBoxes changeFlagsByStencil(const uint8_t* st, Flags fu, Flags fs, Depth d) {
+    return flagsByStencil(st, d, [fu](Flags x) { return areFlagsSet(x, fu); },
+                          [](Flags a, Flags b) { return Flags(a & b); },
+                          [fu, fs](Flags x) { return changeF(x, fu, fs); });
+  }
+
+  std::vector<uint8_t> allTrue() const { return std::vector<uint8_t>(std::max<size_t>(bits.size(), 1), 1); }
+  NrBoxes setFlags(Flags f, Depth d) { auto s = allTrue(); return setFlagsByStencil(s.data(), f, d); }
+  NrBoxes unsetFlags(Flags f, Depth d) { auto s = allTrue(); return unsetFlagsByStencil(s.data(), f, d); }
+  NrBoxes changeFlags(Flags fu, Flags fs, Depth d) {
+    auto s = allTrue();
+    return changeFlagsByStencil(s.data(), fu, fs, d);
+  }
+
+  // setFlagsBySearch :2716-2801, unsetFlagsBySearch :3247-3332, changeFlagsBySearch :822-910
+  // searchVector is sorted and unique; a trailing invalid entry is ignored.
+  template <class Pred, class Apply, class AllPred>
+  NrBoxes flagsBySearch(const std::vector<NrBoxes>& sv, Depth d, Pred pred, Apply apply, AllPred rootPred) {
+    requireSupportedDepth(d);
+    size_t m = sv.size() - ((!sv.empty() && sv.back() == kInvalid) ? 1 : 0);
+    if (d == 0) {
+      bool hit = false;
+      for (size_t i = 0; i < m; ++i) hit = hit || sv[i] == 0;
+      if (hit && rootPred()) {
+        for (auto& f : flagsV) f = apply(f);
+        return 1;
+      }
+      return 0;
+    }
+    NrBoxes n = 0;
+    for (size_t i = 0; i < m; ++i) n += pred(flagsV[sv[i]]) ? 1 : 0;
+    for (size_t i = 0; i < m; ++i) flagsV[sv[i]] = apply(flagsV[sv[i]]);
+    return n;
+  }
+
+  NrBoxes setFlagsBySearch(const std::vector<NrBoxes>& sv, Flags f, Depth d) {
+    return flagsBySearch(sv, d, [f](Flags x) { return !areFlagsSet(x, f); }, [f](Flags x) { return setF(x, f); },
+                         [&]() { for (Flags x : flagsV) if (!areFlagsSet(x, f)) return true; return false; });
+  }
+  NrBoxes unsetFlagsBySearch(const std::vector<NrBoxes>& sv, Flags f, Depth d) {
+    return flagsBySearch(sv, d, [f](Flags x) { return isAnyFlagSet(x, f); }, [f](Flags x) { return unsetF(x, f); },
+                         [&]() { for (Flags x : flagsV) if (isAnyFlagSet(x, f)) return true; return false; });
+  }
+  NrBoxes changeFlagsBySearch(const std::vector<NrBoxes>& sv, Flags fu, Flags fs, Depth d) {
+    return flagsBySearch(sv, d, [fu](Flags x) { return areFlagsSet(x, fu); },
+                         [fu, fs](Flags x) { return changeF(x, fu, fs); },
+                         [&]() { for (Flags x : flagsV) if (!areFlagsSet(x, fu)) return false; return true; });
+  }
+
+  // ImplicitBoxTree.hpp:2343-2371 remove: keep leaves having all of flags (stable)
+  NrBoxes remove(Flags f) {
+    size_t n = bits.size(), w = 0;
+    for (size_t i = 0; i < n; ++i)
+      if (areFlagsSet(flagsV[i], f)) { setBox(w, box(i)); ++w; }
+    resize(w);
+    NrBoxes res = NrBoxes(n - w);
+    insertSurroundingBoxIfEmpty();
+    return res;
+  }
+
+  // helpFunctions.hpp:40-98 sortBoxes (ordering = isStrictlyPreceding)
+  void sortBoxes() {
+    std::vector<Box> v(bits.size());
+    for (size_t i = 0; i < v.size(); ++i) v[i] = box(i);
+    std::stable_sort(v.begin(), v.end(), isStrictlyPreceding);
+    for (size_t i = 0; i < v.size(); ++i) setBox(i, v[i]);
+  }
+
+  // ImplicitBoxTree.hpp:2883-2930 subdivide; returns #leaves with flags set
+  NrBoxes subdivide(Flags f) {
+    NrBoxes res = 0;
+    for (Flags x : flagsV) res += areFlagsSet(x, f) ? 1 : 0;
+    size_t n = bits.size();
+    std::vector<Box> add;
+    for (size_t i = 0; i < n; ++i) {
+      Box b = box(i);
+      if (areFlagsSet(b.flags, f) && b.depth < kMaxDepth) {  // Functors.hpp:326-335
+        Box l = incrementDepthBy(b, 1);
+        setBox(i, l);
+        add.push_back(computeOtherHalf(l));
+      }
+    }
+    for (const Box& b : add) {
+      resize(bits.size() + 1);
+      setBox(bits.size() - 1, b);
+    }
+    sortBoxes();
+    return res;
+  }
+
+  // ImplicitBoxTree.hpp:3413-3461 unsubdivide
+  NrBoxes unsubdivide(Flags f) {
+    size_t n = bits.size();
+    std::vector<uint8_t> un(n);
+    NrBoxes res = 0;
+    for (size_t i = 0; i < n; ++i) { un[i] = isUnsubdividable(i, f); res += un[i]; }
+    for (size_t i = 0; i < n; ++i) if (un[i]) setBox(i, decrementDepthBy(box(i), 1));
+    // reverse segmented AND of flags over equal boxes, keep first of each run
+    std::vector<Box> out;
+    size_t i = 0;
+    while (i < n) {
+      size_t j = i;
+      Flags acc = flagsV[i];
+      while (j + 1 < n && isSameBox(box(j + 1), box(i))) { ++j; acc &= flagsV[j]; }
+      Box b = box(i);
+      b.flags = acc;
+      out.push_back(b);
+      i = j + 1;
+    }
+    res -= NrBoxes(n - out.size());
+    resize(out.size());
+    for (size_t k = 0; k < out.size(); ++k) setBox(k, out[k]);
+    return res;
+  }
+
+  // Point iterators: ImplicitBoxTree.hpp:2082-2120 (root), 2148-2196 (leaves), 2275-2341 (by
+  // flags, with initializeLeafIndicesFromDepth :1475-1611 at leaf depth / depth 0)
+  PointSet<DIM> makePoints(const Grid<DIM>& grid, const Map<DIM>& map, Depth d, Flags check) const {
+    PointSet<DIM> ps{&grid, &map, {}, NONE};
+    if (d == 0) {
+      bool ok = true;
+      if (check != NONE) for (Flags x : flagsV) ok = ok && areFlagsSet(x, check);
+      if (ok) ps.boxes.push_back(surroundingBox());
+    } else if (d == kLeafDepth || areAllLeavesOnDepth(d)) {
+      for (size_t i = 0; i < bits.size(); ++i)
+        if (check == NONE || areFlagsSet(flagsV[i], check)) ps.boxes.push_back(box(i));
+    } else {
+      throw std::runtime_error("point iterator depth must be 0 or -1");
+    }
+    return ps;
+  }
+
+  // Functors.hpp:476-511 GridPointsFunctor::operator()
+  Box pointBox(const PointSet<DIM>& ps, uint64_t i) const {
+    double c[3 * DIM];
+    const Box& b = ps.boxes[i / ps.grid->pointsPerBox()];
+    geo.coords(b, c, c + DIM);
+    (*ps.grid)(i, c + 2 * DIM);
+    for (int j = 0; j < DIM; ++j) c[j] += c[j + 2 * DIM] * c[j + DIM];
+    (*ps.map)(c, c + DIM);
+    return geo.boxFromPoint(c + DIM, ps.flagsToSet);
+  }
+
+  // lower_bound with isStrictlyPreceding over the leaves (ImplicitBoxTree.hpp:2397-2401)
+  NrBoxes lowerBound(const Box& p) const {
+    size_t lo = 0, hi = bits.size();
+    while (lo < hi) {
+      size_t mid = (lo + hi) / 2;
+      if (isStrictlyPreceding(box(mid), p)) lo = mid + 1; else hi = mid;
+    }
+    return NrBoxes(lo);
+  }
+
+  // Functors.hpp:287-315 ComputeSearchIndexFunctor at leaf depth
+  NrBoxes searchIndex(NrBoxes s, const Box& p) const {
+    NrBoxes n = size();
+    Box b1 = box(s > 0 ? s - 1 : 0);
+    Box b2 = box(std::min<NrBoxes>(s, n - 1));
+    if (p.depth >= b1.depth && hasSamePath(p, b1)) return s > 0 ? s - 1 : 0;
+    if (p.depth >= b2.depth && hasSamePath(p, b2)) return s;
+    return kInvalid;
+  }
+
+  NrBoxes searchOne(const Box& p, Depth d) const {
+    if (d == 0) return isInvalid(p) ? kInvalid : 0;
+    return searchIndex(lowerBound(p), p);
+  }
+
+  // ImplicitBoxTree.hpp:2373-2432 searchByIterators (depth 0 and leaf depth)
+  std::vector<NrBoxes> search(const PointSet<DIM>& ps, Depth d) const {
+    requireSupportedDepth(d);
+    uint64_t P = ps.size();
+    std::vector<NrBoxes> out(P);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < int64_t(P); ++i) {
+      Box p = pointBox(ps, uint64_t(i));
+      if (evalTwice) {
+        NrBoxes s = (d == 0) ? 0 : lowerBound(p);
+        Box p2 = pointBox(ps, uint64_t(i));
+        out[i] = (d == 0) ? (isInvalid(p2) ? kInvalid : 0) : searchIndex(s, p2);
+      } else {
+        out[i] = searchOne(p, d);
+      }
+    }
+    return out;
+  }
+
+  std::vector<NrBoxes> searchPoints(const double* pts, uint64_t n, Depth d) const {
+    requireSupportedDepth(d);
+    std::vector<NrBoxes> out(n);
+    for (uint64_t i = 0; i < n; ++i) out[i] = searchOne(geo.boxFromPoint(pts + i * DIM, NONE), d);
+    return out;
+  }
+
+  // ImplicitBoxTree.hpp:2622-2681 searchCompacted: sorted unique hit indices, chunked
+  std::vector<NrBoxes> searchCompacted(const PointSet<DIM>& ps, Depth d) const {
+    requireSupportedDepth(d);
+    std::vector<NrBoxes> res;
+    if (d == 0) {
+      bool allInvalid = true;
+      for (uint64_t i = 0; i < ps.size(); ++i) allInvalid = allInvalid && isInvalid(pointBox(ps, i));
+      if (!allInvalid) res.push_back(0);
+      return res;
+    }
+    std::vector<NrBoxes> all = search(ps, d);
+    NrBoxes nBoxes = count(d);
+    uint64_t P = all.size(), start = 0;
+    while (start < P && res.size() < nBoxes) {
+      uint64_t m = std::min<uint64_t>(chunkPoints, P - start);
+      res.insert(res.end(), all.begin() + start, all.begin() + start + m);
+      std::sort(res.begin(), res.end());
+      res.erase(std::unique(res.begin(), res.end()), res.end());
+      if (!res.empty() && res.back() == kInvalid) res.pop_back();
+      start += m;
+    }
+    return res;
+  }
+
+  // ImplicitBoxTree.hpp:2932-3159 transitionMatrix (COO; invalid targets dropped or, with
+  // useInvalid, relabelled to column nColumns after its increment, as the reference does)
+  Coo transitionMatrix(const PointSet<DIM>& ps, Depth d, bool useAbs, bool rowMajor, bool useInvalid) const {
+    std::vector<NrBoxes> col = search(ps, d);
+    uint64_t k = ps.grid->pointsPerBox();
+    std::vector<uint64_t> key(col.size());
+    for (uint64_t i = 0; i < col.size(); ++i) key[i] = (uint64_t(NrBoxes(i / k)) << 32) | col[i];
+    std::sort(key.begin(), key.end());
+    std::vector<uint64_t> uk;
+    std::vector<uint64_t> cnt;
+    for (uint64_t i = 0; i < key.size();) {
+      uint64_t j = i;
+      while (j < key.size() && key[j] == key[i]) ++j;
+      uk.push_back(key[i]);
+      cnt.push_back(j - i);
+      i = j;
+    }
+    Coo m;
+    m.nRows = m.nCols = count(d);
+    bool anyInvalid = false;
+    for (uint64_t x : uk) anyInvalid = anyInvalid || NrBoxes(x) == kInvalid;
+    if (anyInvalid && useInvalid) m.nCols += 1;
+    std::vector<std::pair<uint64_t, uint64_t>> e;  // (sort key, count)
+    for (size_t i = 0; i < uk.size(); ++i) {
+      NrBoxes r = NrBoxes(uk[i] >> 32), c = NrBoxes(uk[i]);
+      if (c == kInvalid) {
+        if (!useInvalid) continue;
+        if (rowMajor) c = m.nCols;  // replace_if before any re-sort
+      }
+      uint64_t sk = rowMajor ? ((uint64_t(r) << 32) | c) : ((uint64_t(c) << 32) | r);
+      e.push_back({sk, cnt[i]});
+    }
+    if (!rowMajor) std::sort(e.begin(), e.end());
+    for (auto& x : e) {
+      NrBoxes a = NrBoxes(x.first >> 32), b = NrBoxes(x.first);
+      NrBoxes r = rowMajor ? a : b, c = rowMajor ? b : a;
+      if (!rowMajor && c == kInvalid) c = m.nCols;  // fill after the column-major sort
+      m.rows.push_back(r);
+      m.cols.push_back(c);
+      m.vals.push_back(useAbs ? double(x.second) : double(x.second) / double(k));
+    }
+    return m;
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Invariant measure (build-defined; the reference leaves it to MATLAB, subdivisionByNim.m:30-59):
+// mu <- mu P / ||mu P||_1 with mu0 = 1/n, P given as COO; gather form over columns in ascending
+// row order; L1 norm summed in index order. Stops after maxIter iterations or when
+// ||mu_{k+1} - mu_k||_1 <= tol (tol < 0 disables the test).
+// ---------------------------------------------------------------------------------------------
+inline int powerIteration(const Coo& P, int maxIter, double tol, std::vector<double>& mu, double* residOut) {
+  NrBoxes n = P.nRows;
+  std::vector<size_t> order(P.rows.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return P.cols[a] != P.cols[b] ? P.cols[a] < P.cols[b] : P.rows[a] < P.rows[b];
+  });
+  std::vector<NrBoxes> cptr(size_t(n) + 1, 0);
+  for (size_t e : order) if (P.cols[e] < n) ++cptr[P.cols[e] + 1];
+  for (NrBoxes j = 0; j < n; ++j) cptr[j + 1] += cptr[j];
+  std::vector<NrBoxes> ri;
+  std::vector<double> va;
+  for (size_t e : order) if (P.cols[e] < n) { ri.push_back(P.rows[e]); va.push_back(P.vals[e]); }
+  if (mu.size() != n) mu.assign(n, 1.0 / double(n));
+  std::vector<double> nx(n);
+  int it = 0;
+  double resid = -1.0;
+  for (it = 0; it < maxIter;) {
+    for (NrBoxes j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (NrBoxes e = cptr[j]; e < cptr[j + 1]; ++e) s += mu[ri[e]] * va[e];
+      nx[j] = s;
+    }
+    double nrm = 0.0;
+    for (NrBoxes j = 0; j < n; ++j) nrm += std::fabs(nx[j]);
+    double r = 0.0;
+    for (NrBoxes j = 0; j < n; ++j) {
+      double v = nx[j] / nrm;
+      r += std::fabs(v - mu[j]);
+      mu[j] = v;
+    }
+    ++it;
+    resid = r;
+    if (tol >= 0.0 && r <= tol) break;
+  }
+  if (residOut) *residOut = resid;
+  return it;
+}
+
+}  // namespace b12cpu

@@ -1,0 +1,241 @@
+"""GPU parity tests: the HIP library (through its C ABI) against the CPU oracle on the same
+seeded inputs. Integer, key and index results must match bit for bit; FP64 map images must be
+bit-identical (parity build, contraction off); the invariant measure within 1e-12 relative."""
+import numpy as np
+import pytest
+
+import b12_amd as gx
+from helpers import random_tree, same_tree
+from oracle_api import OracleTree, OCoo
+
+pytestmark = pytest.mark.gpu
+
+MEASURE_RTOL = 1e-12
+
+
+def both(center, radius, sd=None):
+    return gx.ImplicitBoxTree(center, radius, sd), OracleTree(center, radius, sd)
+
+
+def assert_same_state(g, o):
+    a, b = g.download(), o.download()
+    assert same_tree(a, b), "leaf arrays differ"
+
+
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name,dim,mp,scale,shift", [
+    ("lorenz", 3, gx.Lorenz(), 20.0, (0, 0, 27)),
+    ("lorenz_rho", 3, gx.Lorenz(rho=24.5, steps=7), 20.0, (0, 0, 27)),
+    ("henon1", 2, gx.Henon(1.4, 0.3, 1), 1.5, (0, 0)),
+    ("henon4", 2, gx.Henon(1.2, 0.3, 4), 1.0, (0, 0)),
+    ("henon3", 2, gx.Henon(1.0, 0.3, 3), 1.0, (0, 0)),
+    ("hh2", 4, gx.HenonHeiles(2, 0.02, 50), 0.4, (0, 0, 0, 0)),
+    ("hh1", 4, gx.HenonHeiles(1, 0.02, 10), 0.4, (0, 0, 0, 0)),
+    ("hh3", 4, gx.HenonHeiles(3, 0.02, 10), 0.4, (0, 0, 0, 0)),
+    ("hh4", 4, gx.HenonHeiles(4, 0.02, 10), 0.4, (0, 0, 0, 0)),
+    ("duffing", 2, gx.Duffing(), 1.5, (0, 0)),
+    ("identity3", 3, gx.Identity(3), 5.0, (0, 0, 0)),
+])
+def test_map_images_bit_exact(name, dim, mp, scale, shift):
+    print(f"[map] {name}", flush=True)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-scale, scale, (4096, dim)) + np.asarray(shift, float)
+    g = gx.Scheme(dim, gx.InnerGrid(2), mp)
+    o = OracleTree(np.zeros(dim), np.ones(dim)).scheme(gx.InnerGrid(2), mp)
+    yg, yo = g.map_points(x), o.map_points(x)
+    assert np.array_equal(yg.view(np.uint64), yo.view(np.uint64))
+
+
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,seed", [(1, 0), (2, 1), (3, 2), (4, 3), (2, 4), (3, 5)])
+def test_tree_operations_random(dim, seed):
+    print(f"[ops] dim={dim} seed={seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, dim)
+    r = rng.uniform(0.5, 2, dim)
+    g, o = both(c, r)
+    bits, depth, flags = random_tree(rng, dim, 300, max_depth=10)
+    g.upload(bits, depth, flags)
+    o.upload(bits, depth, flags)
+    assert_same_state(g, o)
+    ops = [
+        ("subdivide", (8,)), ("setFlags", (1, -1)), ("unsetFlags", (8, -1)), ("changeFlags", (1, 0x10, -1)),
+        ("subdivide", (0x10,)), ("remove", (0x10,)), ("unsubdivide", (0x10,)), ("setFlags", (8, 0)),
+        ("unsubdivide", (8,)), ("unsetFlags", (0x10, 0)), ("changeFlags", (8, 1, 0)), ("subdivide", (1,)),
+        ("remove", (0x40,)), ("subdivide", (0,)), ("unsubdivide", (0,)),
+    ]
+    for name, args in ops:
+        rg, ro = getattr(g, name)(*args), getattr(o, name)(*args)
+        assert rg == ro, f"{name}{args}: {rg} != {ro}"
+        assert_same_state(g, o)
+        assert g.count(-1) == o.count(-1) and g.depth() == o.depth()
+    n = g.size()
+    st = rng.random(n) < 0.5
+    for name, args in [("setFlagsByStencil", (st, 0x20)), ("unsetFlagsByStencil", (st, 0x20)),
+                       ("changeFlagsByStencil", (st, 0x08, 0x40))]:
+        assert getattr(g, name)(*args) == getattr(o, name)(*args)
+        assert_same_state(g, o)
+
+
+def test_search_by_points_and_invalid_quirk():
+    rng = np.random.default_rng(7)
+    g, o = both([0.5, -0.25], [1.5, 0.75])
+    bits, depth, flags = random_tree(rng, 2, 2000, max_depth=16)
+    g.upload(bits, depth, flags)
+    o.upload(bits, depth, flags)
+    pts = rng.uniform(-1.5, 2.5, (20000, 2))
+    pts[:50] = [0.5 - 1.5, -1.0]  # exactly on the lower domain edge
+    assert np.array_equal(g.searchByPoints(pts), o.searchByPoints(pts))
+    assert np.array_equal(g.searchByPoints(pts, 0), o.searchByPoints(pts, 0))
+    # single all-zero leaf: points outside the domain are reported on leaf 0 (B12 quirk)
+    g2, o2 = both([0.0, 0.0], [1.0, 1.0])
+    for t in (g2, o2):
+        t.upload(np.array([0], np.uint64), np.array([3], np.uint8), np.array([0], np.uint8))
+    p = np.array([[5.0, 5.0], [-0.9, -0.9], [0.5, 0.5]])
+    rg, ro = g2.searchByPoints(p), o2.searchByPoints(p)
+    assert np.array_equal(rg, ro) and rg[0] == 0 and rg[2] == gx.INVALID
+
+
+# ---------------------------------------------------------------------------------------------
+def subdivision_run(center, radius, grid, mp, steps):
+    g, o = both(center, radius)
+    assert g.setFlags(gx.SD, 0) == o.setFlags(gx.SD, 0)
+    for s in range(steps):
+        cg = g.subdivision_steps(grid, mp, 1)[0]
+        co = o.subdivision_steps(grid, mp, 1)[0]
+        assert np.array_equal(cg, co), f"step {s}: counts {cg} vs {co}"
+        assert_same_state(g, o)
+    return g, o
+
+
+def test_subdivision_henon_cover():
+    print("[subdivision] henon d14", flush=True)
+    g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 14)
+    assert g.count() > 500
+
+
+def test_subdivision_lorenz_cover():
+    print("[subdivision] lorenz d12", flush=True)
+    g, o = subdivision_run([0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz(), 12)
+    assert g.count() > 100
+
+
+def test_subdivision_duffing_cover():
+    print("[subdivision] duffing d10", flush=True)
+    subdivision_run([0.0, 0.0], [2.0, 2.0], gx.InnerGrid(4), gx.Duffing(), 10)
+
+
+def test_stepwise_api_matches_fused_step():
+    """The four reference calls one by one give the same cover as the fused step."""
+    g1, o = both([0.0, 0.0], [1.5, 0.5])
+    g2 = gx.ImplicitBoxTree([0.0, 0.0], [1.5, 0.5])
+    for t in (g1, g2, o):
+        t.setFlags(gx.SD, 0)
+    grid, mp = gx.InnerGrid(4), gx.Henon()
+    for _ in range(10):
+        r1 = [g1.subdivide(gx.SD)]
+        it = g1.makePointIteratorPairOverLeaves(grid, mp)
+        r1 += [g1.setFlagsByIterators(it, gx.HIT, -1), g1.remove(gx.HIT), g1.unsetFlags(gx.HIT, -1)]
+        r2 = g2.subdivision_steps(grid, mp, 1)[0]
+        r3 = o.subdivision_steps(grid, mp, 1)[0]
+        assert list(r1) == list(r2) == list(r3)
+        assert same_tree(g1.download(), o.download()) and same_tree(g2.download(), o.download())
+
+
+def test_search_iterators_lorenz():
+    g, o = subdivision_run([0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz(), 8)
+    for mp in (gx.Lorenz(), gx.Lorenz(rho=26.0, steps=10)):
+        ig = g.makePointIteratorPairOverLeaves(gx.InnerGrid(3), mp)
+        io = o.makePointIteratorPairOverLeaves(gx.InnerGrid(3), mp)
+        assert np.array_equal(g.searchByIterators(ig), o.searchByIterators(io))
+        assert np.array_equal(g.searchCompacted(ig), o.searchCompacted(io))
+        rg = g.makePointIteratorPairOverRoot(gx.InnerGrid(3), mp)
+        ro = o.makePointIteratorPairOverRoot(gx.InnerGrid(3), mp)
+        assert np.array_equal(g.searchByIterators(rg), o.searchByIterators(ro))
+        assert np.array_equal(g.searchByIterators(ig, 0), o.searchByIterators(io, 0))
+
+
+def test_uniform_grid_search_hh():
+    rng = np.random.default_rng(3)
+    g, o = both([0.0, 0.0, 0.0, 0.0], [0.6, 0.6, 0.5, 0.5])
+    bits, depth, flags = random_tree(rng, 4, 1500, max_depth=14)
+    g.upload(bits, depth, flags)
+    o.upload(bits, depth, flags)
+    for grid in (gx.UniformGrid(64, 0), gx.UniformGrid(5, 12345)):
+        ig = g.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
+        io = o.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
+        assert np.array_equal(g.searchByIterators(ig), o.searchByIterators(io))
+
+
+def test_by_flags_iterator_and_flag_ops():
+    rng = np.random.default_rng(11)
+    g, o = both([0.0, 0.0], [1.5, 0.5])
+    bits, depth, flags = random_tree(rng, 2, 800, max_depth=12)
+    g.upload(bits, depth, flags)
+    o.upload(bits, depth, flags)
+    for check in (0x08, 0x09, 0x00):
+        ig = g.makePointIteratorPairByFlags(-1, check, gx.InnerGrid(3), gx.Henon())
+        io = o.makePointIteratorPairByFlags(-1, check, gx.InnerGrid(3), gx.Henon())
+        assert g.num_points(ig) == o.num_points(io)
+        assert np.array_equal(g.searchByIterators(ig), o.searchByIterators(io))
+        assert g.setFlagsByIterators(ig, 0x20) == o.setFlagsByIterators(io, 0x20)
+        assert g.changeFlagsByIterators(ig, 0x20, 0x40) == o.changeFlagsByIterators(io, 0x20, 0x40)
+        assert g.unsetFlagsByIterators(ig, 0x40) == o.unsetFlagsByIterators(io, 0x40)
+        assert_same_state(g, o)
+        assert g.setFlagsByIterators(ig, 0x02, 0) == o.setFlagsByIterators(io, 0x02, 0)
+        assert_same_state(g, o)
+
+
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("use_abs,row_major,use_invalid", [
+    (False, False, False), (False, True, False), (True, False, False), (True, True, True), (False, False, True),
+    (False, True, True)])
+def test_transition_matrix_henon(use_abs, row_major, use_invalid):
+    g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 10)
+    # remove half the leaves so that some targets fall outside the cover (invalid)
+    st = np.arange(g.size()) % 3 != 0
+    g.setFlagsByStencil(st, 0x20)
+    o.setFlagsByStencil(st, 0x20)
+    g.remove(0x20)
+    o.remove(0x20)
+    mg = g.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), use_abs, row_major, use_invalid)
+    mo = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), use_abs, row_major, use_invalid)
+    assert (mg.nnz, mg.nRows, mg.nColumns) == (mo.nnz, mo.nRows, mo.nColumns)
+    for a, b in zip(mg.download(), mo.download()):
+        assert np.array_equal(a, b)
+
+
+def test_transition_matrix_lorenz_and_root():
+    g, o = subdivision_run([0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz(), 9)
+    mg = g.transitionMatrixForLeaves(gx.InnerGrid(3), gx.Lorenz())
+    mo = o.transitionMatrixForLeaves(gx.InnerGrid(3), gx.Lorenz())
+    for a, b in zip(mg.download(), mo.download()):
+        assert np.array_equal(a, b)
+    rg = g.transitionMatrixForRoot(gx.InnerGrid(3), gx.Lorenz(), use_abs=True)
+    ro = o.transitionMatrixForRoot(gx.InnerGrid(3), gx.Lorenz(), use_abs=True)
+    for a, b in zip(rg.download(), ro.download()):
+        assert np.array_equal(a, b)
+
+
+def test_power_iteration_matches_oracle():
+    g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 12)
+    for rm in (False, True):
+        mg = g.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), row_major=rm)
+        mo = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), row_major=rm)
+        mug, itg, resg = mg.power_iteration(200)
+        muo, ito, reso = mo.power_iteration(200)
+        assert itg == ito == 200
+        assert np.allclose(mug, muo, rtol=MEASURE_RTOL, atol=0.0)
+        assert abs(mug.sum() - 1.0) < 1e-12
+    mug, itg, resg = mg.power_iteration(100000, 1e-12)
+    muo, ito, reso = mo.power_iteration(100000, 1e-12)
+    assert resg <= 1e-12 and reso <= 1e-12 and abs(itg - ito) <= 1
+    assert np.allclose(mug, muo, rtol=1e-10, atol=1e-15)
+
+
+def test_compress_index_vector():
+    rng = np.random.default_rng(5)
+    idx = np.sort(rng.integers(0, 1000, 20000)).astype(np.uint32)
+    from oracle_api import compressIndexVector as ocv
+    assert np.array_equal(gx.compressIndexVector(idx, 1000), ocv(idx, 1000))
+    assert np.array_equal(gx.compressIndexVector(idx, 1000), np.searchsorted(idx, np.arange(1001)).astype(np.uint32))
