@@ -1110,6 +1110,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
     const DepthClass dc = classifyDepth(T, d);
     const Source src = makeSource(T, dIt, check);
     const uint32_t k = s->grid.k;
+    if (k > 64) return setErr(B12X_E_UNSUPPORTED, "transition matrix with more than 64 points per box");
     const uint64_t P = src.nBoxes * k;
     const uint32_t nBoxes = dc == DC_ROOT ? 1u : T.n;  // count(depth)
     auto* M = new b12x_coo_s();
