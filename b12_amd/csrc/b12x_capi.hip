@@ -952,7 +952,7 @@ int b12x_scheme_create(int dim, int gk, const uint64_t* gp, int ngp, int mk, con
           M.order = int(P(0, 2));
           if (M.order < 1 || M.order > 4) throw HipFail(B12X_E_ARG, "symplectic order must be 1..4");
           M.p[0] = P(1, 0.02); M.steps = uint32_t(P(2, 50));
-          M.p[2] = std::cbrt(2.0);
+          M.p[2] = 0x1.428a2f98d728bp+0;  // cbrt(2) correctly rounded (MapTools.hpp:468)
           M.p[1] = 1.0 / (2.0 - M.p[2]);
           break;
         }

@@ -66,7 +66,8 @@ SchemeBase* makeScheme(int gk, const uint64_t* gp, int ngp, int mk, const double
       break;
     case MAP_HENON_HEILES:
       m.order = int(P(0, 2)); m.h = P(1, 0.02); m.steps = uint32_t(P(2, 50));
-      m.cbrt2 = std::cbrt(2.0); m.factor = 1.0 / (2.0 - m.cbrt2);
+      // cbrt(2) correctly rounded, the value g++ folds cbrtreal(2.0) to (MapTools.hpp:468)
+      m.cbrt2 = 0x1.428a2f98d728bp+0; m.factor = 1.0 / (2.0 - m.cbrt2);
       break;
     case MAP_DUFFING: {
       m.p0 = P(0, 0.25); m.p1 = P(1, -1.0); m.p2 = P(2, 1.0);
@@ -199,12 +200,15 @@ int b12cpu_download(void* h, uint64_t* bits, uint8_t* depth, uint8_t* flags) {
   });
 }
 
+// Leaves are put into B12's order (isStrictlyPreceding, helpFunctions.hpp:40-98) as every
+// B12 tree is kept sorted.
 int b12cpu_upload(void* h, uint32_t n, const uint64_t* bits, const uint8_t* depth, const uint8_t* flags) {
   TREE_CALL(h, {
     T.resize(n);
     std::memcpy(T.bits.data(), bits, size_t(n) * 8);
     std::memcpy(T.depthV.data(), depth, n);
     std::memcpy(T.flagsV.data(), flags, n);
+    T.sortBoxes();
     T.insertSurroundingBoxIfEmpty();
     return 0;
   });

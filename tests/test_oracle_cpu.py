@@ -1,0 +1,389 @@
+"""CPU tests (no GPU): pin the oracle against the reference's own code where it builds
+(oracle/_ref), against library routines and golden vectors, and against closed forms,
+invariants and brute force; check that the HIP library loads and exports its C ABI."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import b12_amd as gx
+from helpers import random_tree, same_tree
+from oracle_api import OCoo, OracleTree, compressIndexVector, ref_lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+P = ctypes.c_void_p
+D = ctypes.c_double
+
+
+def oscheme(dim, grid, mp):
+    return OracleTree(np.zeros(dim), np.ones(dim)).scheme(grid, mp)
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
+
+
+# ------------------------------------------------------------------ integrators vs reference code
+@pytest.fixture(scope="module")
+def ref():
+    r = ref_lib()
+    if r is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    return r
+
+
+def test_lorenz_rk4auto_matches_reference_code(ref):
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-25, 25, (3000, 3)) + [0, 0, 27]
+    for sig, rho, beta, h, st in [(10, 28, 8 / 3, 0.01, 20), (10, 24.3, 8 / 3, 0.01, 20), (9.5, 31, 2.5, 0.005, 7)]:
+        y = oscheme(3, gx.InnerGrid(3), gx.Lorenz(sig, rho, beta, h, st)).map_points(x)
+        z = np.empty_like(x)
+        ref.ref_lorenz(D(sig), D(rho), D(beta), D(h), ctypes.c_uint32(st), x.ctypes.data_as(P), z.ctypes.data_as(P),
+                       ctypes.c_uint64(len(x)))
+        assert bits_equal(y, z)
+
+
+def test_henon_iterating_map_matches_reference_code(ref):
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1.5, 1.5, (3000, 2))
+    for a, b, st in [(1.4, 0.3, 1), (1.4, 0.3, 2), (1.2, 0.3, 5), (1.0, 0.3, 0)]:
+        y = oscheme(2, gx.InnerGrid(4), gx.Henon(a, b, st)).map_points(x)
+        z = np.empty_like(x)
+        ref.ref_henon(D(a), D(b), ctypes.c_uint32(st), x.ctypes.data_as(P), z.ctypes.data_as(P),
+                      ctypes.c_uint64(len(x)))
+        assert bits_equal(y, z)
+
+
+def test_duffing_rk4_matches_reference_code(ref):
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-2, 2, (2000, 2))
+    for t1, h in [(2 * np.pi, 2 * np.pi / 100), (1.0, 0.3), (2.0, 0.7)]:
+        mp = gx.Duffing(0.25, -1.0, 1.0, 0.3, 1.0, 0.0, t1, h)
+        y = oscheme(2, gx.InnerGrid(4), mp).map_points(x)
+        z = np.empty_like(x)
+        ref.ref_duffing(D(0.25), D(-1.0), D(1.0), D(0.3), D(1.0), D(0.0), D(t1), D(h), x.ctypes.data_as(P),
+                        z.ctypes.data_as(P), ctypes.c_uint64(len(x)))
+        assert bits_equal(y, z)
+
+
+def test_duffing_step_count_follows_float_time_loop():
+    """The non-autonomous RK4 loop counts steps with a float time variable (MapTools.hpp:54):
+    T = 1.0 with h = 0.1 accumulates to 0.9999999999999999 < 1 and takes an 11th tiny step."""
+    t, n = 0.0, 0
+    while t < 1.0:
+        t += 0.1
+        n += 1
+    assert n == 11
+    x = np.array([[0.3, -0.2]])
+    full = oscheme(2, gx.InnerGrid(1), gx.Duffing(0.25, -1, 1, 0.3, 1, 0.0, 1.0, 0.1)).map_points(x)
+    # the 11th step has h ~ 1e-16: result equals 10 steps up to that tiny step
+    ten = oscheme(2, gx.InnerGrid(1), gx.Duffing(0.25, -1, 1, 0.3, 1, 0.0, 0.9999999999999999, 0.1)).map_points(x)
+    assert np.allclose(full, ten, rtol=0, atol=1e-14)
+
+
+# ------------------------------------------------------------------ integrator invariants
+def test_rk4_order_four_convergence():
+    x = np.array([[1.0, 2.0, 20.0]])
+    T = 0.2
+    sols = [oscheme(3, gx.InnerGrid(1), gx.Lorenz(h=T / n, steps=n)).map_points(x) for n in (10, 20, 40, 80)]
+    e = [np.abs(sols[i] - sols[i + 1]).max() for i in range(3)]
+    ratios = [e[i] / e[i + 1] for i in range(2)]
+    assert all(12 < r < 20 for r in ratios), ratios  # 2^4 = 16
+
+
+def hh_energy(z):
+    q1, q2, p1, p2 = z.T
+    return 0.5 * (p1 ** 2 + p2 ** 2) + 0.5 * (q1 ** 2 + q2 ** 2) + q1 ** 2 * q2 - q2 ** 3 / 3
+
+
+@pytest.mark.parametrize("order,expected", [(2, 4.0), (4, 16.0)])
+def test_symplectic_energy_error_order(order, expected):
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-0.3, 0.3, (200, 4))
+    errs = []
+    for h, st in [(0.04, 25), (0.02, 50), (0.01, 100)]:
+        y = oscheme(4, gx.InnerGrid(1), gx.HenonHeiles(order, h, st)).map_points(x)
+        errs.append(np.abs(hh_energy(y) - hh_energy(x)).max())
+    r = errs[0] / errs[1], errs[1] / errs[2]
+    assert all(0.6 * expected < v < 1.6 * expected for v in r), r
+
+
+def test_stormer_verlet_time_reversible():
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-0.3, 0.3, (100, 4))
+    s = oscheme(4, gx.InnerGrid(1), gx.HenonHeiles(2, 0.02, 50))
+    y = s.map_points(x)
+    y[:, 2:] *= -1
+    z = s.map_points(y)
+    z[:, 2:] *= -1
+    assert np.abs(z - x).max() < 1e-13
+
+
+def test_henon_closed_form():
+    x = np.array([[0.5, 0.25], [-1.0, 0.1]])
+    y = oscheme(2, gx.InnerGrid(1), gx.Henon(1.4, 0.3, 1)).map_points(x)
+    assert bits_equal(y, np.array([[1.0 - 1.4 * 0.5 * 0.5 + 0.25, 0.3 * 0.5], [1.0 - 1.4 * -1.0 * -1.0 + 0.1, -0.3]]))
+
+
+# ------------------------------------------------------------------ UniformGrid vs libstdc++
+def mt19937_python(seed):
+    """Independent MT19937 (Matsumoto & Nishimura 1998) for the KAT."""
+    mt = [0] * 624
+    mt[0] = seed & 0xFFFFFFFF
+    for i in range(1, 624):
+        mt[i] = (1812433253 * (mt[i - 1] ^ (mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+    idx = 624
+    while True:
+        if idx >= 624:
+            for k in range(624):
+                y = (mt[k] & 0x80000000) | (mt[(k + 1) % 624] & 0x7FFFFFFF)
+                mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+            idx = 0
+        y = mt[idx]
+        idx += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        yield y & 0xFFFFFFFF
+
+
+def test_mt19937_known_answer():
+    g = mt19937_python(5489)
+    v = None
+    for _ in range(10000):
+        v = next(g)
+    golden = json.load(open(os.path.join(GOLDEN, "uniform_grid.json")))
+    assert v == 4123659995 == golden["mt19937_default_10000th"]
+
+
+def test_uniform_grid_matches_libstdcxx_golden():
+    golden = json.load(open(os.path.join(GOLDEN, "uniform_grid.json")))
+    for row in golden["rows"]:
+        s = oscheme(4, gx.UniformGrid(64, row["seed"]), gx.Identity(4))
+        got = s.grid_points(row["i"], 1)[0]
+        want = np.array([float.fromhex(h) for h in row["draws_hex"]])
+        assert bits_equal(got, want), row
+
+
+# ------------------------------------------------------------------ keys and descent
+def full_tree(dim, d, c, r):
+    o = OracleTree(c, r)
+    o.setFlags(gx.SD, 0)
+    for _ in range(d):
+        o.subdivide(gx.SD)
+    return o
+
+
+@pytest.mark.parametrize("dim,d", [(1, 7), (2, 8), (3, 9), (4, 8)])
+def test_descent_brute_force_geometry(dim, d):
+    """On a complete tree of depth d, the leaf found for a point is the box that contains it,
+    with the box bounds recomputed independently from the leaf's bit pattern."""
+    rng = np.random.default_rng(d)
+    c = rng.uniform(-1, 1, dim)
+    r = rng.uniform(0.5, 3, dim)
+    o = full_tree(dim, d, c, r)
+    bits, depth, _ = o.download()
+    assert o.size() == 2 ** d and np.all(depth == d)
+    pts = rng.uniform(c - r, c + r, (3000, dim))
+    idx = o.searchByPoints(pts)
+    assert np.all(idx < 2 ** d)
+    # independent geometry: level i bisects coordinate i % dim; bit (d-1-i) is the side
+    for p, i in zip(pts, idx):
+        b = int(bits[i])
+        lo, hi = c - r, c + r
+        lo, hi = lo.copy(), hi.copy()
+        for lev in range(d):
+            j = lev % dim
+            mid = 0.5 * (lo[j] + hi[j])
+            if (b >> (d - 1 - lev)) & 1:
+                lo[j] = mid
+            else:
+                hi[j] = mid
+        assert np.all(p >= lo - 1e-12) and np.all(p <= hi + 1e-12)
+    # outside the domain -> invalid
+    out = o.searchByPoints(np.array([c + 2 * r, c - 2 * r]))
+    assert np.all(out == gx.INVALID)
+
+
+def test_descent_random_trees_brute_force():
+    rng = np.random.default_rng(9)
+    for trial in range(5):
+        o = OracleTree([0.0, 0.0], [1.0, 1.0])
+        bits, depth, flags = random_tree(rng, 2, 400, max_depth=10)
+        o.upload(bits, depth, flags)
+        b, d, _ = o.download()
+        pts = rng.uniform(-1, 1, (2000, 2))
+        idx = o.searchByPoints(pts)
+        full = full_tree(2, 10, [0.0, 0.0], [1.0, 1.0])
+        fb, fd, _ = full.download()
+        cell = fb[full.searchByPoints(pts)]  # depth-10 cell of each point
+        for ci, i in zip(cell, idx):
+            owners = [k for k in range(len(b)) if (int(ci) >> (10 - int(d[k]))) == int(b[k])]
+            assert len(owners) <= 1
+            assert (i == gx.INVALID and not owners) or (owners and i == owners[0])
+
+
+def test_sorted_by_comparable_key():
+    rng = np.random.default_rng(5)
+    o = OracleTree([0.0, 0.0, 0.0], [1.0, 1.0, 1.0])
+    o.upload(*random_tree(rng, 3, 500, max_depth=12))
+    b, d, _ = o.download()
+    key = [(int(x) << (64 - int(dd))) | ((1 << (63 - int(dd))) - 1) if dd < 64 else int(x) for x, dd in zip(b, d)]
+    assert key == sorted(key) and len(set(key)) == len(key)
+
+
+# ------------------------------------------------------------------ tree invariants
+def test_subdivide_unsubdivide_roundtrip():
+    rng = np.random.default_rng(6)
+    for dim in (1, 2, 3, 4):
+        o = OracleTree(np.zeros(dim), np.ones(dim))
+        bits, depth, _ = random_tree(rng, dim, 200, max_depth=9)
+        o.upload(bits, depth, np.full(len(bits), gx.SD, np.uint8))
+        before = o.download()
+        n = o.size()
+        assert o.subdivide(gx.SD) == n
+        assert o.size() == 2 * n
+        assert o.unsubdivide(gx.SD) == n
+        assert same_tree(o.download(), before)
+
+
+def test_remove_keeps_flagged_and_reinserts_root():
+    o = OracleTree([0.0, 0.0], [1.0, 1.0])
+    o.setFlags(gx.SD, 0)
+    for _ in range(4):
+        o.subdivide(gx.SD)
+    st = np.arange(16) % 3 == 0
+    assert o.setFlagsByStencil(st, gx.HIT) == st.sum()
+    assert o.remove(gx.HIT) == 16 - st.sum()
+    assert o.size() == st.sum()
+    assert o.unsetFlags(gx.HIT) == st.sum()
+    assert o.remove(gx.HIT) == st.sum()
+    b, d, f = o.download()
+    assert list(b) == [0] and list(d) == [0] and list(f) == [0]
+
+
+def test_flag_return_values():
+    o = OracleTree([0.0], [1.0])
+    o.upload(np.array([0, 1, 2, 3], np.uint64), np.full(4, 2, np.uint8), np.array([0, 1, 3, 2], np.uint8))
+    assert o.setFlags(1) == 2          # leaves lacking HIT
+    assert o.unsetFlags(2) == 2        # leaves with any of flags
+    assert o.changeFlags(1, 4) == 4    # leaves with all of flagsToUnset
+    assert o.count(-1) == 4 and o.count(2) == 4 and o.count(0) == 1 and o.count(1) == 2 and o.count(3) == 0
+    assert o.depth() == 2
+
+
+def test_identity_map_hits_every_leaf_and_gives_identity_matrix():
+    rng = np.random.default_rng(7)
+    o = OracleTree([0.0, 0.0, 0.0], [1.0, 2.0, 3.0])
+    o.upload(*random_tree(rng, 3, 300, max_depth=9))
+    it = o.makePointIteratorPairOverLeaves(gx.InnerGrid(2), gx.Identity(3))
+    assert np.array_equal(o.searchCompacted(it), np.arange(o.size()))
+    m = o.transitionMatrixForLeaves(gx.InnerGrid(2), gx.Identity(3))
+    r, c, v = m.download()
+    assert np.array_equal(r, np.arange(o.size())) and np.array_equal(c, r) and np.all(v == 1.0)
+
+
+def test_transition_matrix_counts_and_orderings():
+    """Row sums equal the fraction of a box's points landing inside the cover; the column-major
+    and row-major matrices hold the same entries; absolute values are integer counts."""
+    o = OracleTree([0.0, 0.0], [1.5, 0.5])
+    o.setFlags(gx.SD, 0)
+    o.subdivision_steps(gx.InnerGrid(4), gx.Henon(), 8)
+    it = o.makePointIteratorPairOverLeaves(gx.InnerGrid(4), gx.Henon())
+    idx = o.searchByIterators(it).reshape(-1, 16)
+    inside = (idx != gx.INVALID).sum(axis=1)
+    mc = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon())
+    mr = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), row_major=True)
+    ma = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), use_abs=True, row_major=True)
+    rc, cc, vc = mc.download()
+    rr, cr, vr = mr.download()
+    ra, ca, va = ma.download()
+    assert np.all(np.diff(cc.astype(np.int64) * 2 ** 32 + rc) > 0)
+    assert np.all(np.diff(rr.astype(np.int64) * 2 ** 32 + cr) > 0)
+    sums = np.bincount(rr, weights=vr, minlength=o.size())
+    assert np.allclose(sums, inside / 16.0)
+    assert np.array_equal(va * 16 / 16, va) and np.all(va == np.round(va))
+    assert np.allclose(np.bincount(ra, weights=va, minlength=o.size()), inside)
+    o1 = np.lexsort((rc, cc))
+    assert set(zip(rr, cr, vr)) == set(zip(rc, cc, vc)) and len(o1) == len(rr)
+
+
+def test_search_compacted_chunk_invariance():
+    o = OracleTree([0.0, 0.0, 27.0], [30.0, 30.0, 30.0])
+    o.setFlags(gx.SD, 0)
+    o.subdivision_steps(gx.InnerGrid(3), gx.Lorenz(), 9)
+    it = o.makePointIteratorPairOverLeaves(gx.InnerGrid(3), gx.Lorenz(rho=25.0))
+    full = o.searchCompacted(it)
+    for chunk in (1, 7, 1000):
+        o.set_options(False, chunk)
+        assert np.array_equal(o.searchCompacted(it), full)
+    o.set_options(True, 0)
+    assert np.array_equal(o.searchCompacted(it), full)
+
+
+def test_compress_index_vector_closed_form():
+    rng = np.random.default_rng(8)
+    idx = np.sort(rng.integers(0, 300, 5000)).astype(np.uint32)
+    assert np.array_equal(compressIndexVector(idx, 300), np.searchsorted(idx, np.arange(301)).astype(np.uint32))
+    assert np.array_equal(compressIndexVector(np.array([], np.uint32), 3), np.zeros(4, np.uint32))
+
+
+# ------------------------------------------------------------------ invariant measure
+def test_power_iteration_closed_forms():
+    # doubly stochastic -> uniform
+    n = 5
+    P = np.full((n, n), 1.0 / n)
+    r, c = np.nonzero(P)
+    m = OCoo.from_arrays(r, c, P[r, c], n, n)
+    mu, it, res = m.power_iteration(10)
+    assert np.allclose(mu, 1.0 / n, rtol=1e-15)
+    # 2-state chain: stationary distribution (b, a)/(a+b)
+    a, b = 0.3, 0.1
+    P = np.array([[1 - a, a], [b, 1 - b]])
+    r, c = np.nonzero(P)
+    mu, it, res = OCoo.from_arrays(r, c, P[r, c], 2, 2).power_iteration(100000, 1e-15)
+    assert np.allclose(mu, [b / (a + b), a / (a + b)], rtol=1e-13)
+    # a cyclic permutation keeps the uniform start fixed
+    perm = np.roll(np.arange(7), 1)
+    mu, _, res = OCoo.from_arrays(np.arange(7), perm, np.ones(7), 7, 7).power_iteration(3)
+    assert np.allclose(mu, 1 / 7, rtol=1e-15) and res < 1e-15
+
+
+def test_power_iteration_matches_numpy_eig():
+    rng = np.random.default_rng(10)
+    n = 40
+    A = rng.random((n, n)) * (rng.random((n, n)) < 0.3) + np.eye(n) * 0.1
+    P = A / A.sum(axis=1, keepdims=True) * 0.97  # substochastic, as when mass leaves the cover
+    r, c = np.nonzero(P)
+    mu, it, res = OCoo.from_arrays(r, c, P[r, c], n, n).power_iteration(100000, 1e-14)
+    w, V = np.linalg.eig(P.T)
+    k = np.argmax(np.abs(w))
+    v = np.abs(V[:, k].real)
+    v /= v.sum()
+    assert np.allclose(mu, v, rtol=1e-9, atol=1e-13)
+
+
+# ------------------------------------------------------------------ HIP library C ABI
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "b12x_c_api.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(b12x_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.mark.parametrize("flavour", ["parity", "fast"])
+def test_library_exports_every_declared_symbol(flavour):
+    path = gx.library_path(flavour)
+    assert os.path.exists(path), f"{path} not built"
+    lib = ctypes.CDLL(path)
+    syms = header_symbols()
+    assert len(syms) >= 30
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # every declared entry point is bound by the Python mirror as well
+    from b12_amd.api import SIGNATURES
+    assert set(SIGNATURES) | {"b12x_last_error"} >= set(syms)
