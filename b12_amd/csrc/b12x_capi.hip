@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -200,17 +201,18 @@ void ensureCap(b12x_tree_s& T, uint64_t need) {
   T.cap = uint32_t(nc);
 }
 
-// Runs a three-phase scan over n elements; returns the total number of outputs (synchronises).
+// Enqueues a three-phase scan over n elements (or *nDev, with n an upper bound); the total lands
+// in T.scal[scalSlot] and the summed Op::stat values in *stat (when set). Does not synchronise.
 template <class Op>
-uint32_t runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int scalSlot) {
-  if (n == 0) return 0;
+void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int scalSlot, const uint32_t* nDev = nullptr,
+             uint32_t* stat = nullptr) {
+  if (n == 0) return;
   const uint32_t nb = ceilDiv(n, kTile);
   if (nb > kMaxScanTiles) throw HipFail(B12X_E_ARG, "scan too large");
-  hipLaunchKernelGGL(scanCountKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, totals);
-  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, T.stream, totals, nb, T.scal + scalSlot);
-  hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, totals);
+  hipLaunchKernelGGL(scanCountKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals, stat);
+  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, T.stream, totals, nb, nDev, T.scal + scalSlot);
+  hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals);
   HIPCHK(hipGetLastError());
-  return 0;
 }
 
 // Exclusive scan of a device uint32 array in place on a stream; returns the total (synchronises).
@@ -219,9 +221,12 @@ uint32_t scanArray(hipStream_t st, uint32_t* v, uint64_t n, uint32_t* scratchTot
   if (n == 0) return 0;
   if (n > 0xFFFFFFFFull) throw HipFail(B12X_E_ARG, "scan too large");
   const uint32_t nb = ceilDiv(n, kTile);
-  ScanArrayOp op{v};
-  hipLaunchKernelGGL(scanCountKernel<ScanArrayOp>, dim3(nb), dim3(kBlock), 0, st, op, uint32_t(n), scratchTotals);
-  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, st, scratchTotals, nb, dTotal);
+  ScanArrayOp op{};
+  op.v = v;
+  hipLaunchKernelGGL(scanCountKernel<ScanArrayOp>, dim3(nb), dim3(kBlock), 0, st, op, uint32_t(n),
+                     (const uint32_t*)nullptr, scratchTotals, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, st, scratchTotals, nb, (const uint32_t*)nullptr,
+                     dTotal);
   hipLaunchKernelGGL(scanArrayScatterKernel, dim3(nb), dim3(kBlock), 0, st, v, uint32_t(n), scratchTotals);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(hTotal, dTotal, 4, hipMemcpyDeviceToHost, st));
@@ -229,7 +234,7 @@ uint32_t scanArray(hipStream_t st, uint32_t* v, uint64_t n, uint32_t* scratchTot
   return *hTotal;
 }
 
-struct IndexOp {  // indices i with sel(i) — flag-filtered source lists and searchCompacted
+struct IndexOp : OpBase {  // indices i with sel(i) — flag-filtered source lists and searchCompacted
   const uint8_t* flags;
   const uint8_t* hit;
   uint8_t check;
@@ -243,21 +248,6 @@ struct IndexOp {  // indices i with sel(i) — flag-filtered source lists and se
 __global__ void fillU8Kernel(uint8_t* p, uint64_t n, uint8_t v) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
-}
-
-__global__ void countSelKernel(const uint8_t* flags, const uint8_t* hit, uint32_t n, uint8_t a, uint32_t* counter) {
-  // counts leaves with a set hit byte that lack flags a (setFlagsBySearch's return value)
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t c = (i < n && hit[i] && !areFlagsSet(flags[i], a)) ? 1u : 0u;
-  c = blockReduceSum(c);
-  if (threadIdx.x == 0 && c) atomicAdd(counter, c);
-}
-
-__global__ void countFlagsKernel(const uint8_t* flags, uint32_t n, uint8_t a, uint32_t* counter) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t c = (i < n && areFlagsSet(flags[i], a)) ? 1u : 0u;
-  c = blockReduceSum(c);
-  if (threadIdx.x == 0 && c) atomicAdd(counter, c);
 }
 
 template <int DIM, int MAP>
@@ -339,7 +329,11 @@ Source makeSource(b12x_tree_s& T, uint8_t dIt, uint8_t check) {
   }
   if (classifyDepth(T, dIt) != DC_LEAF) throw HipFail(B12X_E_UNSUPPORTED, "point iterator depth");
   if (check == 0) return {T.key[T.cur], T.depth[T.cur], nullptr, T.n};
-  IndexOp op{T.flags[T.cur], nullptr, check, T.list};
+  IndexOp op{};
+  op.flags = T.flags[T.cur];
+  op.hit = nullptr;
+  op.check = check;
+  op.out = T.list;
   runScan(T, op, T.n, T.totals, 0);
   syncScal(T, 1);
   return {T.key[T.cur], T.depth[T.cur], T.list, T.hscal[0]};
@@ -636,10 +630,11 @@ int b12x_subdivide(b12x_tree t, uint8_t f, uint32_t* ret) {
     b12x_tree_s& T = *t;
     ensureCap(T, uint64_t(T.n) * 2);
     zeroScal(T);
-    hipLaunchKernelGGL(countFlagsKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.n,
-                       f, T.scal + 1);
-    SubdivideOp op{T.view(T.cur), T.view(1 - T.cur), f};
-    runScan(T, op, T.n, T.totals, 0);
+    SubdivideOp op{};
+    op.in = T.view(T.cur);
+    op.out = T.view(1 - T.cur);
+    op.mask = f;
+    runScan(T, op, T.n, T.totals, 0, nullptr, T.scal + 1);
     syncScal(T, 2);
     T.n = T.hscal[0];
     T.cur = 1 - T.cur;
@@ -657,7 +652,11 @@ int b12x_unsubdivide(b12x_tree t, uint8_t f, uint32_t* ret) {
     UnsubMarkKernelArgs a{T.view(T.cur), T.n, f, T.key[o], T.depth[o], T.scal + 1};
     hipLaunchKernelGGL(unsubMarkKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, a);
     // lifted keys/depths sit in the other buffer; merge back into the current one
-    UnsubMergeOp op{T.key[o], T.depth[o], T.flags[T.cur], T.n, T.view(o)};
+    UnsubMergeOp op{};
+    op.key = T.key[o];
+    op.depth = T.depth[o];
+    op.flags = T.flags[T.cur];
+    op.n = T.n;
     // the merge writes keys/depths in place over the lifted arrays: positions never exceed i,
     // but other blocks may still read them, so go through the hit-sized staging instead
     uint64_t* k2 = dalloc<uint64_t>(T.n);
@@ -685,7 +684,11 @@ int b12x_remove(b12x_tree t, uint8_t f, uint32_t* ret) {
     if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
     b12x_tree_s& T = *t;
     zeroScal(T);
-    KeepOp op{T.view(T.cur), T.view(1 - T.cur), nullptr, f, 0, 0};
+    KeepOp op{};
+    op.in = T.view(T.cur);
+    op.out = T.view(1 - T.cur);
+    op.hit = nullptr;
+    op.mask = f;
     runScan(T, op, T.n, T.totals, 0);
     syncScal(T, 1);
     const uint32_t m = T.hscal[0];
@@ -794,7 +797,11 @@ int b12x_search_compacted(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check
     const Source src = makeSource(T, dIt, check);
     runImage(T, *s, src, dc == DC_ROOT, 0, nullptr);
     const uint32_t m = dc == DC_ROOT ? 1 : T.n;
-    IndexOp op{nullptr, T.hit, 0, T.list};
+    IndexOp op{};
+    op.flags = nullptr;
+    op.hit = T.hit;
+    op.check = 0;
+    op.out = T.list;
     runScan(T, op, m, T.totals, 0);
     syncScal(T, 1);
     const uint32_t cnt = T.hscal[0];
@@ -838,39 +845,64 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
     if (!t || nSteps < 0) return setErr(B12X_E_ARG, "bad argument");
     b12x_tree_s& T = *t;
     checkScheme(T, s);
+    // Device scalars: [0] leaves after subdivide, [1] #leaves with sd, [2] leaves kept,
+    // [3] #leaves newly hit. One host synchronisation per step.
     for (int step = 0; step < nSteps; ++step) {
-      // subdivide(sd)
-      ensureCap(T, uint64_t(T.n) * 2);
+      const uint32_t n = T.n;
+      ensureCap(T, uint64_t(n) * 2);
       zeroScal(T);
-      hipLaunchKernelGGL(countFlagsKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur],
-                         T.n, sd, T.scal + 1);
-      SubdivideOp sop{T.view(T.cur), T.view(1 - T.cur), sd};
-      runScan(T, sop, T.n, T.totals, 0);
-      syncScal(T, 2);
-      const uint32_t cSub = T.hscal[1];
-      T.n = T.hscal[0];
+      // subdivide(sd): ImplicitBoxTree.hpp:2883-2930
+      SubdivideOp sop{};
+      sop.in = T.view(T.cur);
+      sop.out = T.view(1 - T.cur);
+      sop.mask = sd;
+      runScan(T, sop, n, T.totals, 0, nullptr, T.scal + 1);
       T.cur = 1 - T.cur;
-      // setFlagsByIterators(OverLeaves, HIT, -1): image -> hit bytes
-      const Source src{T.key[T.cur], T.depth[T.cur], nullptr, T.n};
-      runImage(T, *s, src, false, 0, nullptr);
-      zeroScal(T);
-      hipLaunchKernelGGL(countSelKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur],
-                         T.hit, T.n, uint8_t(F_HIT), T.scal + 1);
-      // remove(HIT) + unsetFlags(HIT, -1) in one compaction
-      KeepOp kop{T.view(T.cur), T.view(1 - T.cur), T.hit, uint8_t(F_HIT), uint8_t(F_HIT), uint8_t(F_HIT)};
-      runScan(T, kop, T.n, T.totals, 0);
-      hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.hit, uint64_t(T.n),
-                         uint8_t(0));
-      syncScal(T, 2);
-      const uint32_t kept = T.hscal[0], cSet = T.hscal[1];
-      const uint32_t cRem = T.n - kept;
+      // setFlagsByIterators(OverLeaves(grid, map), HIT, -1): image of every test point
+      ImageArgs a{};
+      a.dom = T.dom;
+      a.map = s->map;
+      a.grid = s->grid;
+      a.key = T.key[T.cur];
+      a.depth = T.depth[T.cur];
+      a.srcKey = T.key[T.cur];
+      a.srcDepth = T.depth[T.cur];
+      a.nPoints = uint64_t(n) * 2 * s->grid.k;  // grid bound; the kernel reads the count
+      a.nBoxesDev = T.scal + 0;
+      a.mode = 0;
+      a.hit = T.hit;
+      {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (T.timing) {
+          HIPCHK(hipEventCreate(&e0));
+          HIPCHK(hipEventCreate(&e1));
+          HIPCHK(hipEventRecord(e0, T.stream));
+        }
+        launchImage(T.dim, a, s->grid.kind, T.stream);
+        if (T.timing) {
+          HIPCHK(hipEventRecord(e1, T.stream));
+          T.pending.push_back({e0, e1});
+          T.imageLaunches += 1;
+        }
+      }
+      // remove(HIT) + unsetFlags(HIT, -1) as one stable compaction; counts setFlags' return value
+      KeepOp kop{};
+      kop.in = T.view(T.cur);
+      kop.out = T.view(1 - T.cur);
+      kop.hit = T.hit;
+      kop.mask = kop.hitFlag = kop.clearMask = uint8_t(F_HIT);
+      kop.clearHit = 1;
+      runScan(T, kop, n * 2 > 0 ? n * 2 : 1, T.totals, 2, T.scal + 0, T.scal + 3);
+      T.cur = 1 - T.cur;
+      syncScal(T, 4);
+      const uint32_t nSub = T.hscal[0], cSub = T.hscal[1], kept = T.hscal[2], cSet = T.hscal[3];
+      if (T.timing) T.imagePoints += uint64_t(nSub) * s->grid.k;
       T.n = kept;
-      T.cur = 1 - T.cur;
       insertRootIfEmpty(T);
       if (counts) {
         counts[4 * step + 0] = cSub;
         counts[4 * step + 1] = cSet;
-        counts[4 * step + 2] = cRem;
+        counts[4 * step + 2] = nSub - kept;
         counts[4 * step + 3] = kept;  // every kept leaf had HIT, which unsetFlags clears
       }
     }
@@ -1258,59 +1290,6 @@ int b12x_compress_index_vector(const uint32_t* idx, uint64_t n, uint32_t dim, ui
 }  // extern "C"
 
 namespace {
-struct PiState {
-  double norm;
-  double resid;
-  int iters;
-  int done;
-};
-
-__global__ void piSpmvKernel(const uint32_t* colPtr, const uint32_t* rowIdx, const double* val, const double* mu,
-                             uint32_t n, double* y, double* partial, const PiState* st) {
-  if (st->done) return;
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  double s = 0.0;
-  if (j < n) {
-    for (uint32_t e = colPtr[j]; e < colPtr[j + 1]; ++e) s += mu[rowIdx[e]] * val[e];
-    y[j] = s;
-  }
-  const double t = blockReduceSumF64(fabs(s));
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
-
-__global__ void piNormKernel(const double* partial, uint32_t m, PiState* st) {
-  if (st->done) return;
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < m; i += kBlock) s += partial[i];
-  s = blockReduceSumF64(s);
-  if (threadIdx.x == 0) st->norm = s;
-}
-
-__global__ void piNormalizeKernel(const double* y, double* mu, uint32_t n, double* partial, const PiState* st) {
-  if (st->done) return;
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  double r = 0.0;
-  if (j < n) {
-    const double v = y[j] / st->norm;
-    r = fabs(v - mu[j]);
-    mu[j] = v;
-  }
-  const double t = blockReduceSumF64(r);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
-
-__global__ void piResidKernel(const double* partial, uint32_t m, PiState* st, int maxIter, double tol) {
-  if (st->done) return;
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < m; i += kBlock) s += partial[i];
-  s = blockReduceSumF64(s);
-  if (threadIdx.x == 0) {
-    st->resid = s;
-    st->iters += 1;
-    st->done = (st->iters >= maxIter) || (tol >= 0.0 && s <= tol);
-  }
-}
-
 __global__ void sequenceKernel(uint32_t* p, uint64_t n) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) p[i] = uint32_t(i);
@@ -1358,35 +1337,58 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
       dfree(idx);
     }
     uint32_t* colPtr = dalloc<uint32_t>(uint64_t(n) + 1);
-    hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, st, ck, m->nnz, n,
+    hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, st, ck, nnz, n,
                        colPtr);
-    double* dmu = dalloc<double>(n);
-    double* dy = dalloc<double>(n);
-    const uint32_t nb = ceilDiv(n, kBlock);
-    double* partial = dalloc<double>(nb);
+    double* mu0 = dalloc<double>(n);
+    double* ybuf = dalloc<double>(uint64_t(n) * 3);
+    const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
+    double* partial = dalloc<double>(uint64_t(nb) * 2);
     PiState* pst = dalloc<PiState>(1);
-    PiState init{0.0, -1.0, 0, maxIter <= 0 ? 1 : 0};
+    PiState init{};
+    init.t = 1;
+    init.resid = -1.0;
+    init.done = (maxIter <= 0 || n == 0) ? 1 : 0;
     HIPCHK(hipMemcpyAsync(pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
-    if (muInit) HIPCHK(hipMemcpyAsync(dmu, mu, size_t(n) * 8, hipMemcpyHostToDevice, st));
-    else hipLaunchKernelGGL(fillF64Kernel, dim3(nb), dim3(kBlock), 0, st, dmu, n, 1.0 / double(n));
+    if (muInit) HIPCHK(hipMemcpyAsync(mu0, mu, size_t(n) * 8, hipMemcpyHostToDevice, st));
+    else if (n) hipLaunchKernelGGL(fillF64Kernel, dim3(nb), dim3(kBlock), 0, st, mu0, n, 1.0 / double(n));
     PiState hs{};
-    const int batch = 16;
-    for (int it = 0; it < maxIter; it += batch) {
-      for (int b = 0; b < batch; ++b) {
-        hipLaunchKernelGGL(piSpmvKernel, dim3(nb), dim3(kBlock), 0, st, colPtr, cr, vals, dmu, n, dy, partial, pst);
-        hipLaunchKernelGGL(piNormKernel, dim3(1), dim3(kBlock), 0, st, partial, nb, pst);
-        hipLaunchKernelGGL(piNormalizeKernel, dim3(nb), dim3(kBlock), 0, st, dy, dmu, n, partial, pst);
-        hipLaunchKernelGGL(piResidKernel, dim3(1), dim3(kBlock), 0, st, partial, nb, pst, maxIter, tol);
-      }
+    // iterations are launched in batches from a captured HIP graph (B12X_GRAPHS=0: eager launches)
+    const int batch = 32;
+    const char* ge = std::getenv("B12X_GRAPHS");
+    const bool useGraph = !(ge && ge[0] == '0');
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto enqueueBatch = [&]() {
+      for (int b = 0; b < batch; ++b)
+        hipLaunchKernelGGL(piStepKernel, dim3(nb), dim3(kBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n, partial,
+                           partial + nb, pst, uint32_t(std::max(maxIter, 0)), tol);
+    };
+    if (useGraph && !init.done) {
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueueBatch();
+      HIPCHK(hipStreamEndCapture(st, &graph));
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    for (int64_t it = 0; !init.done && it <= int64_t(maxIter); it += batch) {
+      if (useGraph) HIPCHK(hipGraphLaunch(exec, st));
+      else enqueueBatch();
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       if (hs.done) break;
     }
-    HIPCHK(hipMemcpy(mu, dmu, size_t(n) * 8, hipMemcpyDeviceToHost));
-    if (iters) *iters = hs.iters;
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    double* dmu = dalloc<double>(n);
+    if (n) hipLaunchKernelGGL(piFinishKernel, dim3(nb), dim3(kBlock), 0, st, ybuf, mu0, n, pst, dmu);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
+    if (n) HIPCHK(hipMemcpyAsync(mu, dmu, size_t(n) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (iters) *iters = int(hs.final);
     if (resid) *resid = hs.resid;
-    dfree(ck); dfree(cr); dfree(vals); dfree(colPtr); dfree(dmu); dfree(dy); dfree(partial); dfree(pst);
+    dfree(mu0); dfree(ybuf); dfree(partial); dfree(pst); dfree(dmu); dfree(colPtr);
+    dfree(ck); dfree(cr); dfree(vals);
     return B12X_OK;
   });
 }

@@ -72,21 +72,44 @@ __device__ inline uint32_t blockReduceSum(uint32_t v) {
 // Generic three-phase scan: Op::count(i) -> number of outputs of element i (0, 1 or 2),
 // Op::emit(i, c, pos) writes them at pos. Block totals -> exclusive offsets -> scatter.
 // ---------------------------------------------------------------------------------------------
+// Element counts come from `n`, or from *nDev when nDev is set (a count produced on the device
+// by the previous kernel; the grid is then sized for an upper bound). Op::stat(i) is summed into
+// *statCounter by the count pass; Op::after(i) runs for every element in the scatter pass.
+struct OpBase {
+  __device__ uint32_t stat(uint32_t) const { return 0; }
+  __device__ void after(uint32_t) const {}
+};
+
+__device__ inline uint32_t elemCount(uint32_t n, const uint32_t* nDev) { return nDev ? *nDev : n; }
+
 template <class Op>
-__global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, uint32_t* blockTotals) {
+__global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, const uint32_t* nDev,
+                                                          uint32_t* blockTotals, uint32_t* statCounter) {
+  n = elemCount(n, nDev);
+  if (uint64_t(blockIdx.x) * kTile >= n && blockIdx.x > 0) return;
   const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
-  uint32_t s = 0;
+  uint32_t s = 0, st = 0;
 #pragma unroll
   for (int t = 0; t < kItems; ++t) {
     const uint64_t i = base + t;
-    if (i < n) s += op.count(uint32_t(i));
+    if (i < n) {
+      s += op.count(uint32_t(i));
+      if (statCounter) st += op.stat(uint32_t(i));
+    }
   }
   s = blockReduceSum(s);
   if (threadIdx.x == 0) blockTotals[blockIdx.x] = s;
+  if (statCounter) {
+    st = blockReduceSum(st);
+    if (threadIdx.x == 0 && st) atomicAdd(statCounter, st);
+  }
 }
 
-// Single-block exclusive scan of m block totals in place; writes the grand total to *total.
-__global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t m, uint32_t* total) {
+// Single-block exclusive scan of the block totals in place (m of them, or ceil(*nDev / kTile));
+// writes the grand total to *total.
+__global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t m, const uint32_t* nDev,
+                                                           uint32_t* total) {
+  if (nDev) m = uint32_t((uint64_t(*nDev) + kTile - 1) / kTile);
   uint32_t carry = 0;
   for (uint32_t off = 0; off < m; off += kTile) {
     uint32_t loc[kItems];
@@ -111,7 +134,10 @@ __global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t
 }
 
 template <class Op>
-__global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32_t n, const uint32_t* blockOffsets) {
+__global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32_t n, const uint32_t* nDev,
+                                                            const uint32_t* blockOffsets) {
+  n = elemCount(n, nDev);
+  if (uint64_t(blockIdx.x) * kTile >= n) return;
   const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
   uint32_t c[kItems];
   uint32_t s = 0;
@@ -126,7 +152,10 @@ __global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32_t n, c
 #pragma unroll
   for (int t = 0; t < kItems; ++t) {
     const uint64_t i = base + t;
-    if (i < n && c[t]) op.emit(uint32_t(i), c[t], pos);
+    if (i < n) {
+      if (c[t]) op.emit(uint32_t(i), c[t], pos);
+      op.after(uint32_t(i));
+    }
     pos += c[t];
   }
 }
@@ -140,9 +169,10 @@ struct TreeView {
 // subdivide (ImplicitBoxTree.hpp:2883-2930): every leaf with all `mask` flags and depth < 64 is
 // replaced in place by its two halves (IncrementDepthByFunctor, ComputeOtherHalfFunctor). The
 // halves of a leaf are adjacent to it in key order, so the output stays sorted without a sort.
-struct SubdivideOp {
+struct SubdivideOp : OpBase {
   TreeView in, out;
   uint8_t mask;
+  __device__ uint32_t stat(uint32_t i) const { return areFlagsSet(in.flags[i], mask) ? 1u : 0u; }
   __device__ uint32_t count(uint32_t i) const {
     return (areFlagsSet(in.flags[i], mask) && in.depth[i] < kMaxDepth) ? 2u : 1u;
   }
@@ -168,10 +198,16 @@ struct SubdivideOp {
 // With `hit` set, a leaf counts as carrying `hitFlag` when its hit byte is set (the fused
 // setFlagsBySearch + remove of one subdivision step), and `clearMask` is unset in the output
 // (the step's trailing unsetFlags).
-struct KeepOp {
+struct KeepOp : OpBase {
   TreeView in, out;
-  const uint8_t* hit;
+  uint8_t* hit;
   uint8_t mask, hitFlag, clearMask;
+  int clearHit;  // zero the hit byte after use (scatter pass)
+  // leaves newly gaining hitFlag (setFlagsBySearch's return value)
+  __device__ uint32_t stat(uint32_t i) const { return (hit && hit[i] && !areFlagsSet(in.flags[i], hitFlag)) ? 1u : 0u; }
+  __device__ void after(uint32_t i) const {
+    if (clearHit) hit[i] = 0;
+  }
   __device__ uint8_t flagsAt(uint32_t i) const {
     uint8_t f = in.flags[i];
     if (hit && hit[i]) f |= hitFlag;
@@ -235,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void unsubMarkKernel(UnsubMarkKernelArgs a)
   if (threadIdx.x == 0 && u) atomicAdd(a.nUnsub, u);
 }
 
-struct UnsubMergeOp {
+struct UnsubMergeOp : OpBase {
   const uint64_t* key;
   const uint8_t* depth;
   const uint8_t* flags;
@@ -324,6 +360,7 @@ struct ImageArgs {
   const uint8_t* srcDepth;
   const uint32_t* srcList;  // optional indirection into srcKey/srcDepth
   uint64_t nPoints;
+  const uint32_t* nBoxesDev;  // when set, nPoints = *nBoxesDev * k (grid sized for a bound)
   uint64_t pointBase;       // global index of point 0 (grid index offset)
   int mode;
   int depth0;               // search on depth 0 (root only)
@@ -334,7 +371,9 @@ struct ImageArgs {
 template <int DIM, int MAP, int GRID>
 __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
   const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < a.nPoints; i += stride) {
+  const uint64_t nPoints = a.nBoxesDev ? uint64_t(*a.nBoxesDev) * a.grid.k : a.nPoints;
+  const uint32_t n = a.nBoxesDev ? *a.nBoxesDev : a.n;
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < nPoints; i += stride) {
     const uint64_t gi = a.pointBase + i;
     const uint64_t b = gi / a.grid.k;
     const uint32_t src = a.srcList ? a.srcList[b] : uint32_t(b);
@@ -353,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
     const uint64_t pk = pointKey<DIM>(a.dom, y);
     uint32_t t;
     if (a.depth0) t = (pk == kInvalidKey) ? kInvalid : 0u;
-    else t = locate(a.key, a.depth, a.n, pk);
+    else t = locate(a.key, a.depth, n, pk);
     if (a.mode == 0) {
       if (t != kInvalid) a.hit[t] = 1;
     } else {
@@ -531,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void radixScatterKernel(const uint32_t* key
 }
 
 // Exclusive scan of a large uint32 array in place (three phases over kTile-element blocks).
-struct ScanArrayOp {
+struct ScanArrayOp : OpBase {
   uint32_t* v;
   __device__ uint32_t count(uint32_t i) const { return v[i]; }
   __device__ void emit(uint32_t, uint32_t, uint32_t) const {}
@@ -595,37 +634,91 @@ __device__ inline double blockReduceSumF64(double v) {
   return t;
 }
 
-__global__ __launch_bounds__(kBlock) void spmvColsKernel(const uint32_t* colPtr, const uint32_t* rowIdx,
-                                                         const double* val, const double* mu, uint32_t n, double* y,
-                                                         double* partial) {
+// One launch per iteration t = 1, 2, ...: y_t = mu_{t-1} P, where mu_{t-1} = y_{t-1} / ||y_{t-1}||_1
+// is recomputed on the fly from the previous iterate (the same IEEE quotient the oracle stores),
+// together with the residual ||mu_{t-1} - mu_{t-2}||_1 of the previous iteration. The last block
+// to finish (atomic ticket) sums the block partials in block order, so the norms and residuals
+// are deterministic, and decides whether iteration t-1 ends the loop. y rotates over 3 slots.
+struct PiState {
+  double norm[3];
+  double resid;
+  uint32_t t;
+  uint32_t ticket;
+  int32_t done;
+  uint32_t final;
+};
+
+__global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restrict__ colPtr,
+                                                       const uint32_t* __restrict__ rowIdx,
+                                                       const double* __restrict__ val, const double* __restrict__ mu0,
+                                                       double* y, uint32_t n, double* partAbs, double* partRes,
+                                                       PiState* st, uint32_t maxIter, double tol) {
+  __shared__ int isLast;
+  if (st->done) return;
+  const uint32_t t = st->t;
+  const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;  // slots of t, t-1, t-2
+  const double* yPrev = y + uint64_t(s1) * n;
+  const double* yPrev2 = y + uint64_t(s2) * n;
+  double* yCur = y + uint64_t(s0) * n;
+  const double nPrev = st->norm[s1], nPrev2 = st->norm[s2];
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  double s = 0.0;
+  double sAbs = 0.0, sRes = 0.0;
   if (j < n) {
-    for (uint32_t e = colPtr[j]; e < colPtr[j + 1]; ++e) s += mu[rowIdx[e]] * val[e];
-    y[j] = s;
+    double s = 0.0;
+    const uint32_t e1 = colPtr[j + 1];
+    for (uint32_t e = colPtr[j]; e < e1; ++e) {
+      const uint32_t i = rowIdx[e];
+      const double m = (t == 1) ? mu0[i] : yPrev[i] / nPrev;
+      s += m * val[e];
+    }
+    yCur[j] = s;
+    sAbs = fabs(s);
+    if (t >= 2) {
+      const double a = yPrev[j] / nPrev;
+      const double b = (t == 2) ? mu0[j] : yPrev2[j] / nPrev2;
+      sRes = fabs(a - b);
+    }
   }
-  const double t = blockReduceSumF64(fabs(s));
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  sAbs = blockReduceSumF64(sAbs);
+  sRes = blockReduceSumF64(sRes);
+  if (threadIdx.x == 0) {
+    partAbs[blockIdx.x] = sAbs;
+    partRes[blockIdx.x] = sRes;
+    __threadfence();
+    isLast = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!isLast) return;
+  __threadfence();
+  double ta = 0.0, tr = 0.0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+    ta += __builtin_nontemporal_load(partAbs + b);
+    tr += __builtin_nontemporal_load(partRes + b);
+  }
+  ta = blockReduceSumF64(ta);
+  tr = blockReduceSumF64(tr);
+  if (threadIdx.x == 0) {
+    st->norm[s0] = ta;
+    const uint32_t completed = t - 1;
+    if (completed >= 1) {
+      st->resid = tr;
+      if ((tol >= 0.0 && tr <= tol) || completed >= maxIter) {
+        st->done = 1;
+        st->final = completed;
+      }
+    }
+    st->t = t + 1;
+    st->ticket = 0;
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void sumPartialsKernel(const double* partial, uint32_t m, double* out) {
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < m; i += kBlock) s += partial[i];
-  s = blockReduceSumF64(s);
-  if (threadIdx.x == 0) *out = s;
-}
-
-__global__ __launch_bounds__(kBlock) void normalizeKernel(const double* y, const double* norm, double* mu, uint32_t n,
-                                                          double* partial) {
+// mu = y_final / ||y_final||_1 (or mu0 when no iteration ran)
+__global__ __launch_bounds__(kBlock) void piFinishKernel(const double* y, const double* mu0, uint32_t n,
+                                                         const PiState* st, double* mu) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  double r = 0.0;
-  if (j < n) {
-    const double v = y[j] / *norm;
-    r = fabs(v - mu[j]);
-    mu[j] = v;
-  }
-  const double t = blockReduceSumF64(r);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  if (j >= n) return;
+  const uint32_t f = st->final;
+  mu[j] = f == 0 ? mu0[j] : y[uint64_t(f % 3) * n + j] / st->norm[f % 3];
 }
 
 }  // namespace b12x

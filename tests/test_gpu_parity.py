@@ -239,3 +239,19 @@ def test_compress_index_vector():
     from oracle_api import compressIndexVector as ocv
     assert np.array_equal(gx.compressIndexVector(idx, 1000), ocv(idx, 1000))
     assert np.array_equal(gx.compressIndexVector(idx, 1000), np.searchsorted(idx, np.arange(1001)).astype(np.uint32))
+
+
+def test_power_iteration_graph_replays_match_eager(monkeypatch):
+    """Iterations replayed from the captured HIP graph (many replays, host syncs between them)
+    give bit-identical measures to eagerly launched iterations."""
+    g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 11)
+    m = g.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon())
+    mu_graph, it_graph, r_graph = m.power_iteration(700)
+    monkeypatch.setenv("B12X_GRAPHS", "0")
+    mu_eager, it_eager, r_eager = m.power_iteration(700)
+    monkeypatch.delenv("B12X_GRAPHS")
+    mu_again, it_again, _ = m.power_iteration(700)
+    assert it_graph == it_eager == it_again == 700
+    assert np.array_equal(mu_graph, mu_eager) and np.array_equal(mu_graph, mu_again) and r_graph == r_eager
+    muo, _, _ = OracleTree and o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()).power_iteration(700)
+    assert np.allclose(mu_graph, muo, rtol=MEASURE_RTOL, atol=0.0)
