@@ -253,5 +253,5 @@ def test_power_iteration_graph_replays_match_eager(monkeypatch):
     mu_again, it_again, _ = m.power_iteration(700)
     assert it_graph == it_eager == it_again == 700
     assert np.array_equal(mu_graph, mu_eager) and np.array_equal(mu_graph, mu_again) and r_graph == r_eager
-    muo, _, _ = OracleTree and o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()).power_iteration(700)
+    muo, _, _ = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()).power_iteration(700)
     assert np.allclose(mu_graph, muo, rtol=MEASURE_RTOL, atol=0.0)
