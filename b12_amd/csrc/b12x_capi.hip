@@ -205,12 +205,14 @@ void ensureCap(b12x_tree_s& T, uint64_t need) {
 // in T.scal[scalSlot] and the summed Op::stat values in *stat (when set). Does not synchronise.
 template <class Op>
 void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int scalSlot, const uint32_t* nDev = nullptr,
-             uint32_t* stat = nullptr) {
+             int statSlot = -1) {
   if (n == 0) return;
   const uint32_t nb = ceilDiv(n, kTile);
   if (nb > kMaxScanTiles) throw HipFail(B12X_E_ARG, "scan too large");
-  hipLaunchKernelGGL(scanCountKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals, stat);
-  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, T.stream, totals, nb, nDev, T.scal + scalSlot);
+  uint32_t* stats = statSlot >= 0 ? totals + nb : nullptr;  // totals holds 2*nb + 16 entries
+  hipLaunchKernelGGL(scanCountKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals, stats);
+  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, T.stream, totals, nb, nDev, T.scal + scalSlot,
+                     (const uint32_t*)stats, statSlot >= 0 ? T.scal + statSlot : nullptr);
   hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals);
   HIPCHK(hipGetLastError());
 }
@@ -226,7 +228,7 @@ uint32_t scanArray(hipStream_t st, uint32_t* v, uint64_t n, uint32_t* scratchTot
   hipLaunchKernelGGL(scanCountKernel<ScanArrayOp>, dim3(nb), dim3(kBlock), 0, st, op, uint32_t(n),
                      (const uint32_t*)nullptr, scratchTotals, (uint32_t*)nullptr);
   hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, st, scratchTotals, nb, (const uint32_t*)nullptr,
-                     dTotal);
+                     dTotal, (const uint32_t*)nullptr, (uint32_t*)nullptr);
   hipLaunchKernelGGL(scanArrayScatterKernel, dim3(nb), dim3(kBlock), 0, st, v, uint32_t(n), scratchTotals);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(hTotal, dTotal, 4, hipMemcpyDeviceToHost, st));
@@ -634,7 +636,7 @@ int b12x_subdivide(b12x_tree t, uint8_t f, uint32_t* ret) {
     op.in = T.view(T.cur);
     op.out = T.view(1 - T.cur);
     op.mask = f;
-    runScan(T, op, T.n, T.totals, 0, nullptr, T.scal + 1);
+    runScan(T, op, T.n, T.totals, 0, nullptr, 1);
     syncScal(T, 2);
     T.n = T.hscal[0];
     T.cur = 1 - T.cur;
@@ -850,13 +852,12 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
     for (int step = 0; step < nSteps; ++step) {
       const uint32_t n = T.n;
       ensureCap(T, uint64_t(n) * 2);
-      zeroScal(T);
       // subdivide(sd): ImplicitBoxTree.hpp:2883-2930
       SubdivideOp sop{};
       sop.in = T.view(T.cur);
       sop.out = T.view(1 - T.cur);
       sop.mask = sd;
-      runScan(T, sop, n, T.totals, 0, nullptr, T.scal + 1);
+      runScan(T, sop, n, T.totals, 0, nullptr, 1);
       T.cur = 1 - T.cur;
       // setFlagsByIterators(OverLeaves(grid, map), HIT, -1): image of every test point
       ImageArgs a{};
@@ -892,7 +893,7 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
       kop.hit = T.hit;
       kop.mask = kop.hitFlag = kop.clearMask = uint8_t(F_HIT);
       kop.clearHit = 1;
-      runScan(T, kop, n * 2 > 0 ? n * 2 : 1, T.totals, 2, T.scal + 0, T.scal + 3);
+      runScan(T, kop, n * 2 > 0 ? n * 2 : 1, T.totals, 2, T.scal + 0, 3);
       T.cur = 1 - T.cur;
       syncScal(T, 4);
       const uint32_t nSub = T.hscal[0], cSub = T.hscal[1], kept = T.hscal[2], cSet = T.hscal[3];
@@ -1342,7 +1343,8 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     double* mu0 = dalloc<double>(n);
     double* ybuf = dalloc<double>(uint64_t(n) * 3);
     const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
-    double* partial = dalloc<double>(uint64_t(nb) * 2);
+    const uint32_t nbStep = std::max<uint32_t>(1, ceilDiv(uint64_t(n) * kPiLanes, kBlock));
+    double* partial = dalloc<double>(uint64_t(nbStep) * 2);
     PiState* pst = dalloc<PiState>(1);
     PiState init{};
     init.t = 1;
@@ -1360,8 +1362,8 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     hipGraphExec_t exec = nullptr;
     auto enqueueBatch = [&]() {
       for (int b = 0; b < batch; ++b)
-        hipLaunchKernelGGL(piStepKernel, dim3(nb), dim3(kBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n, partial,
-                           partial + nb, pst, uint32_t(std::max(maxIter, 0)), tol);
+        hipLaunchKernelGGL(piStepKernel, dim3(nbStep), dim3(kBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n, partial,
+                           partial + nbStep, pst, uint32_t(std::max(maxIter, 0)), tol);
     };
     if (useGraph && !init.done) {
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

@@ -84,7 +84,7 @@ __device__ inline uint32_t elemCount(uint32_t n, const uint32_t* nDev) { return 
 
 template <class Op>
 __global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, const uint32_t* nDev,
-                                                          uint32_t* blockTotals, uint32_t* statCounter) {
+                                                          uint32_t* blockTotals, uint32_t* blockStats) {
   n = elemCount(n, nDev);
   if (uint64_t(blockIdx.x) * kTile >= n && blockIdx.x > 0) return;
   const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
@@ -94,23 +94,26 @@ __global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, con
     const uint64_t i = base + t;
     if (i < n) {
       s += op.count(uint32_t(i));
-      if (statCounter) st += op.stat(uint32_t(i));
+      if (blockStats) st += op.stat(uint32_t(i));
     }
   }
   s = blockReduceSum(s);
   if (threadIdx.x == 0) blockTotals[blockIdx.x] = s;
-  if (statCounter) {
+  if (blockStats) {
     st = blockReduceSum(st);
-    if (threadIdx.x == 0 && st) atomicAdd(statCounter, st);
+    if (threadIdx.x == 0) blockStats[blockIdx.x] = st;
   }
 }
 
 // Single-block exclusive scan of the block totals in place (m of them, or ceil(*nDev / kTile));
-// writes the grand total to *total.
+// writes the grand total to *total and, when blockStats is set, the sum of the per-block
+// statistics to *statTotal.
 __global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t m, const uint32_t* nDev,
-                                                           uint32_t* total) {
+                                                           uint32_t* total, const uint32_t* blockStats,
+                                                           uint32_t* statTotal) {
   if (nDev) m = uint32_t((uint64_t(*nDev) + kTile - 1) / kTile);
-  uint32_t carry = 0;
+  if (m == 0) m = 1;
+  uint32_t carry = 0, stat = 0;
   for (uint32_t off = 0; off < m; off += kTile) {
     uint32_t loc[kItems];
     uint32_t s = 0;
@@ -119,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t
       const uint32_t i = off + threadIdx.x * kItems + t;
       loc[t] = i < m ? v[i] : 0u;
       s += loc[t];
+      if (blockStats && i < m) stat += blockStats[i];
     }
     uint32_t tot;
     uint32_t ex = blockExclusiveScan(s, tot);
@@ -130,7 +134,11 @@ __global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t
     }
     carry += tot;
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (blockStats) stat = blockReduceSum(stat);
+  if (threadIdx.x == 0) {
+    *total = carry;
+    if (blockStats) *statTotal = stat;
+  }
 }
 
 template <class Op>
@@ -369,34 +377,49 @@ struct ImageArgs {
 };
 
 template <int DIM, int MAP, int GRID>
+__device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, uint64_t b, uint32_t gl, uint32_t n) {
+  const uint32_t src = a.srcList ? a.srcList[b] : uint32_t(b);
+  double c[DIM], rb[DIM], g[DIM], x[DIM], y[DIM];
+  boxCoords<DIM>(a.dom, a.srcKey[src], a.srcDepth[src], c, rb);
+  if constexpr (GRID == GRID_INNER) {
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) g[j] = a.grid.table[gl * DIM + j];
+  } else {
+    uniformGridPoint<DIM>(a.grid.seed, gi, g);
+  }
+#pragma unroll
+  for (int j = 0; j < DIM; ++j) x[j] = c[j] + g[j] * rb[j];
+  applyMap<DIM, MAP>(a.map, x, y);
+  const uint64_t pk = pointKey<DIM>(a.dom, y);
+  uint32_t t;
+  if (a.depth0) t = (pk == kInvalidKey) ? kInvalid : 0u;
+  else t = locate(a.key, a.depth, n, pk);
+  if (a.mode == 0) {
+    if (t != kInvalid) a.hit[t] = 1;
+  } else {
+    a.out[i] = t;
+  }
+}
+
+template <int DIM, int MAP, int GRID>
 __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
-  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
   const uint64_t nPoints = a.nBoxesDev ? uint64_t(*a.nBoxesDev) * a.grid.k : a.nPoints;
   const uint32_t n = a.nBoxesDev ? *a.nBoxesDev : a.n;
-  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < nPoints; i += stride) {
-    const uint64_t gi = a.pointBase + i;
-    const uint64_t b = gi / a.grid.k;
-    const uint32_t src = a.srcList ? a.srcList[b] : uint32_t(b);
-    double c[DIM], rb[DIM], g[DIM], x[DIM], y[DIM];
-    boxCoords<DIM>(a.dom, a.srcKey[src], a.srcDepth[src], c, rb);
-    if constexpr (GRID == GRID_INNER) {
-      const uint32_t gl = uint32_t(gi % a.grid.k);
-#pragma unroll
-      for (int j = 0; j < DIM; ++j) g[j] = a.grid.table[gl * DIM + j];
-    } else {
-      uniformGridPoint<DIM>(a.grid.seed, gi, g);
+  if (a.pointBase + nPoints <= 0xFFFFFFFFull && uint64_t(gridDim.x) * kBlock <= 0xFFFFFFFFull) {
+    // 32-bit index arithmetic (the common case)
+    const uint32_t k = a.grid.k, base = uint32_t(a.pointBase), np = uint32_t(nPoints);
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < np; i += stride) {
+      const uint32_t gi = base + i;
+      const uint32_t b = gi / k;
+      imagePoint<DIM, MAP, GRID>(a, i, gi, b, gi - b * k, n);
     }
-#pragma unroll
-    for (int j = 0; j < DIM; ++j) x[j] = c[j] + g[j] * rb[j];
-    applyMap<DIM, MAP>(a.map, x, y);
-    const uint64_t pk = pointKey<DIM>(a.dom, y);
-    uint32_t t;
-    if (a.depth0) t = (pk == kInvalidKey) ? kInvalid : 0u;
-    else t = locate(a.key, a.depth, n, pk);
-    if (a.mode == 0) {
-      if (t != kInvalid) a.hit[t] = 1;
-    } else {
-      a.out[i] = t;
+  } else {
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < nPoints; i += stride) {
+      const uint64_t gi = a.pointBase + i;
+      const uint64_t b = gi / a.grid.k;
+      imagePoint<DIM, MAP, GRID>(a, i, gi, b, uint32_t(gi - b * a.grid.k), n);
     }
   }
 }
@@ -648,6 +671,10 @@ struct PiState {
   uint32_t final;
 };
 
+// kPiLanes lanes share a column: they load and multiply its entries in parallel, and the first
+// lane of the group adds the products in ascending-row order (the oracle's order) via shuffles.
+constexpr int kPiLanes = 8;
+
 __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restrict__ colPtr,
                                                        const uint32_t* __restrict__ rowIdx,
                                                        const double* __restrict__ val, const double* __restrict__ mu0,
@@ -661,16 +688,35 @@ __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restric
   const double* yPrev2 = y + uint64_t(s2) * n;
   double* yCur = y + uint64_t(s0) * n;
   const double nPrev = st->norm[s1], nPrev2 = st->norm[s2];
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  double sAbs = 0.0, sRes = 0.0;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sub = lane % kPiLanes, lead = lane - sub;
+  const uint32_t j = (blockIdx.x * kBlock + threadIdx.x) / kPiLanes;
+  uint32_t e0 = 0, e1 = 0;
   if (j < n) {
-    double s = 0.0;
-    const uint32_t e1 = colPtr[j + 1];
-    for (uint32_t e = colPtr[j]; e < e1; ++e) {
+    e0 = colPtr[j];
+    e1 = colPtr[j + 1];
+  }
+  // trip count uniform across the wave so that every lane takes part in every shuffle
+  uint32_t trips = (e1 - e0 + kPiLanes - 1) / kPiLanes;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) trips = max(trips, (uint32_t)__shfl_xor(trips, o, kWave));
+  double s = 0.0;
+  for (uint32_t r = 0; r < trips; ++r) {
+    const uint32_t e = e0 + r * kPiLanes + sub;
+    double prod = 0.0;
+    if (e < e1) {
       const uint32_t i = rowIdx[e];
       const double m = (t == 1) ? mu0[i] : yPrev[i] / nPrev;
-      s += m * val[e];
+      prod = m * val[e];
     }
+#pragma unroll
+    for (int k = 0; k < kPiLanes; ++k) {
+      const double pk = __shfl(prod, lead + k, kWave);
+      if (e0 + r * kPiLanes + k < e1) s += pk;
+    }
+  }
+  double sAbs = 0.0, sRes = 0.0;
+  if (j < n && sub == 0) {
     yCur[j] = s;
     sAbs = fabs(s);
     if (t >= 2) {
