@@ -1362,8 +1362,12 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     hipGraphExec_t exec = nullptr;
     auto enqueueBatch = [&]() {
       for (int b = 0; b < batch; ++b)
+      {
         hipLaunchKernelGGL(piStepKernel, dim3(nbStep), dim3(kBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n, partial,
-                           partial + nbStep, pst, uint32_t(std::max(maxIter, 0)), tol);
+                           partial + nbStep, (const PiState*)pst);
+        hipLaunchKernelGGL(piReduceKernel, dim3(1), dim3(kBlock), 0, st, (const double*)partial,
+                           (const double*)(partial + nbStep), nbStep, pst, uint32_t(std::max(maxIter, 0)), tol);
+      }
     };
     if (useGraph && !init.done) {
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

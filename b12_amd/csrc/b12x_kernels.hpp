@@ -659,14 +659,13 @@ __device__ inline double blockReduceSumF64(double v) {
 
 // One launch per iteration t = 1, 2, ...: y_t = mu_{t-1} P, where mu_{t-1} = y_{t-1} / ||y_{t-1}||_1
 // is recomputed on the fly from the previous iterate (the same IEEE quotient the oracle stores),
-// together with the residual ||mu_{t-1} - mu_{t-2}||_1 of the previous iteration. The last block
-// to finish (atomic ticket) sums the block partials in block order, so the norms and residuals
-// are deterministic, and decides whether iteration t-1 ends the loop. y rotates over 3 slots.
+// together with the residual ||mu_{t-1} - mu_{t-2}||_1 of the previous iteration; block partials
+// of both sums go to fixed slots for piReduceKernel. y rotates over 3 slots.
 struct PiState {
   double norm[3];
   double resid;
   uint32_t t;
-  uint32_t ticket;
+  uint32_t pad;
   int32_t done;
   uint32_t final;
 };
@@ -679,8 +678,7 @@ __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restric
                                                        const uint32_t* __restrict__ rowIdx,
                                                        const double* __restrict__ val, const double* __restrict__ mu0,
                                                        double* y, uint32_t n, double* partAbs, double* partRes,
-                                                       PiState* st, uint32_t maxIter, double tol) {
-  __shared__ int isLast;
+                                                       const PiState* st) {
   if (st->done) return;
   const uint32_t t = st->t;
   const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;  // slots of t, t-1, t-2
@@ -730,21 +728,24 @@ __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restric
   if (threadIdx.x == 0) {
     partAbs[blockIdx.x] = sAbs;
     partRes[blockIdx.x] = sRes;
-    __threadfence();
-    isLast = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
   }
-  __syncthreads();
-  if (!isLast) return;
-  __threadfence();
+}
+
+// Sums the block partials of piStepKernel in block order (deterministic) and decides whether
+// iteration t-1 ends the loop. One block; runs after every piStepKernel.
+__global__ __launch_bounds__(kBlock) void piReduceKernel(const double* partAbs, const double* partRes, uint32_t nb,
+                                                         PiState* st, uint32_t maxIter, double tol) {
+  if (st->done) return;
+  const uint32_t t = st->t;
   double ta = 0.0, tr = 0.0;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-    ta += __builtin_nontemporal_load(partAbs + b);
-    tr += __builtin_nontemporal_load(partRes + b);
+  for (uint32_t b = threadIdx.x; b < nb; b += kBlock) {
+    ta += partAbs[b];
+    tr += partRes[b];
   }
   ta = blockReduceSumF64(ta);
   tr = blockReduceSumF64(tr);
   if (threadIdx.x == 0) {
-    st->norm[s0] = ta;
+    st->norm[t % 3] = ta;
     const uint32_t completed = t - 1;
     if (completed >= 1) {
       st->resid = tr;
@@ -754,7 +755,6 @@ __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restric
       }
     }
     st->t = t + 1;
-    st->ticket = 0;
   }
 }
 
