@@ -1360,13 +1360,29 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     const bool useGraph = !(ge && ge[0] == '0');
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // small matrices: one fused kernel per iteration; larger ones: step + single-block reduce
+    const uint32_t nbFused = std::max<uint32_t>(1, ceilDiv(uint64_t(n) * kPiLanes, kPiBlock));
+    const bool fused = nbFused <= kPiFusedMaxBlocks && !(std::getenv("B12X_PI_FUSED") &&
+                                                         std::getenv("B12X_PI_FUSED")[0] == '0');
+    double* partF = nullptr;
+    uint32_t* blockIter = nullptr;
+    if (fused) {
+      partF = dalloc<double>(uint64_t(nbFused) * 4);
+      blockIter = dalloc<uint32_t>(nbFused);
+      HIPCHK(hipMemsetAsync(blockIter, 0, nbFused * 4, st));
+      HIPCHK(hipMemsetAsync(partF, 0, uint64_t(nbFused) * 4 * 8, st));
+    }
     auto enqueueBatch = [&]() {
-      for (int b = 0; b < batch; ++b)
-      {
-        hipLaunchKernelGGL(piStepKernel, dim3(nbStep), dim3(kBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n, partial,
-                           partial + nbStep, (const PiState*)pst);
-        hipLaunchKernelGGL(piReduceKernel, dim3(1), dim3(kBlock), 0, st, (const double*)partial,
-                           (const double*)(partial + nbStep), nbStep, pst, uint32_t(std::max(maxIter, 0)), tol);
+      for (int b = 0; b < batch; ++b) {
+        if (fused) {
+          hipLaunchKernelGGL(piFusedKernel, dim3(nbFused), dim3(kPiBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n,
+                             partF, blockIter, pst, uint32_t(std::max(maxIter, 0)), tol);
+        } else {
+          hipLaunchKernelGGL(piStepKernel, dim3(nbStep), dim3(kBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n,
+                             partial, partial + nbStep, (const PiState*)pst);
+          hipLaunchKernelGGL(piReduceKernel, dim3(1), dim3(kBlock), 0, st, (const double*)partial,
+                             (const double*)(partial + nbStep), nbStep, pst, uint32_t(std::max(maxIter, 0)), tol);
+        }
       }
     };
     if (useGraph && !init.done) {
@@ -1375,7 +1391,7 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
       HIPCHK(hipStreamEndCapture(st, &graph));
       HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
-    for (int64_t it = 0; !init.done && it <= int64_t(maxIter); it += batch) {
+    for (int64_t it = 0; !init.done && it <= int64_t(maxIter) + 2; it += batch) {
       if (useGraph) HIPCHK(hipGraphLaunch(exec, st));
       else enqueueBatch();
       HIPCHK(hipGetLastError());
@@ -1393,7 +1409,7 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     HIPCHK(hipStreamSynchronize(st));
     if (iters) *iters = int(hs.final);
     if (resid) *resid = hs.resid;
-    dfree(mu0); dfree(ybuf); dfree(partial); dfree(pst); dfree(dmu); dfree(colPtr);
+    dfree(mu0); dfree(ybuf); dfree(partial); dfree(pst); dfree(dmu); dfree(colPtr); dfree(partF); dfree(blockIter);
     dfree(ck); dfree(cr); dfree(vals);
     return B12X_OK;
   });

@@ -758,6 +758,116 @@ __global__ __launch_bounds__(kBlock) void piReduceKernel(const double* partAbs, 
   }
 }
 
+// Single-kernel form for grids of at most kPiFusedMaxBlocks blocks of kPiBlock threads: launch t
+// first re-derives norm_{t-1} and the residual of iteration t-2 from the previous launch's block
+// partials (every block sums them in the same order, so all blocks agree), decides whether
+// iteration t-2 ended the loop, then computes y_t and its partials into the other parity slot.
+// Each block counts its own launches (blockIter[b]), so no block waits on or races with another.
+constexpr int kPiBlock = 1024;
+constexpr uint32_t kPiFusedMaxBlocks = 512;
+
+__device__ inline double blockSumF64Big(double v) {
+  __shared__ double ws[kPiBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  if (lane == 0) ws[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kPiBlock / kWave; ++w) t += ws[w];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t* __restrict__ colPtr,
+                                                          const uint32_t* __restrict__ rowIdx,
+                                                          const double* __restrict__ val,
+                                                          const double* __restrict__ mu0, double* y, uint32_t n,
+                                                          double* part /* [2][2][nb] */, uint32_t* blockIter,
+                                                          PiState* st, uint32_t maxIter, double tol) {
+  const uint32_t nb = gridDim.x;
+  const uint32_t t = blockIter[blockIdx.x] + 1;  // this block's launch index
+  if (st->done) return;
+  double nPrev = 0.0, rPrev = -1.0;
+  if (t >= 2) {
+    const double* pa = part + uint64_t((t - 1) & 1) * 2 * nb;
+    double a = 0.0, r = 0.0;
+    for (uint32_t b = threadIdx.x; b < nb; b += kPiBlock) {
+      a += pa[b];
+      r += pa[nb + b];
+    }
+    nPrev = blockSumF64Big(a);
+    rPrev = blockSumF64Big(r);
+  }
+  const uint32_t completed = t >= 2 ? t - 2 : 0;  // iteration whose residual is rPrev
+  if (completed >= 1 && ((tol >= 0.0 && rPrev <= tol) || completed >= maxIter)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->final = completed;
+      st->resid = rPrev;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) blockIter[blockIdx.x] = t;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && t >= 2) {
+    st->norm[(t - 1) % 3] = nPrev;
+    if (completed >= 1) st->resid = rPrev;
+  }
+  const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
+  const double* yPrev = y + uint64_t(s1) * n;
+  const double* yPrev2 = y + uint64_t(s2) * n;
+  double* yCur = y + uint64_t(s0) * n;
+  const double nPrev2 = t >= 3 ? st->norm[s2] : 0.0;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sub = lane % kPiLanes, lead = lane - sub;
+  double sAbs = 0.0, sRes = 0.0;
+  const uint64_t total = uint64_t(n) * kPiLanes;
+  const uint64_t stride = uint64_t(nb) * kPiBlock;
+  for (uint64_t g = uint64_t(blockIdx.x) * kPiBlock + threadIdx.x; g - lane < total; g += stride) {
+    const uint32_t j = uint32_t(g / kPiLanes);
+    uint32_t e0 = 0, e1 = 0;
+    if (g < total) {
+      e0 = colPtr[j];
+      e1 = colPtr[j + 1];
+    }
+    uint32_t trips = (e1 - e0 + kPiLanes - 1) / kPiLanes;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) trips = max(trips, (uint32_t)__shfl_xor(trips, o, kWave));
+    double s = 0.0;
+    for (uint32_t r = 0; r < trips; ++r) {
+      const uint32_t e = e0 + r * kPiLanes + sub;
+      double prod = 0.0;
+      if (e < e1) {
+        const uint32_t i = rowIdx[e];
+        const double m = (t == 1) ? mu0[i] : yPrev[i] / nPrev;
+        prod = m * val[e];
+      }
+#pragma unroll
+      for (int k = 0; k < kPiLanes; ++k) {
+        const double pk = __shfl(prod, lead + k, kWave);
+        if (e0 + r * kPiLanes + k < e1) s += pk;
+      }
+    }
+    if (g < total && sub == 0) {
+      yCur[j] = s;
+      sAbs += fabs(s);
+      if (t >= 2) {
+        const double a = yPrev[j] / nPrev;
+        const double b = (t == 2) ? mu0[j] : yPrev2[j] / nPrev2;
+        sRes += fabs(a - b);
+      }
+    }
+  }
+  sAbs = blockSumF64Big(sAbs);
+  sRes = blockSumF64Big(sRes);
+  if (threadIdx.x == 0) {
+    double* pc = part + uint64_t(t & 1) * 2 * nb;
+    pc[blockIdx.x] = sAbs;
+    pc[nb + blockIdx.x] = sRes;
+  }
+}
+
 // mu = y_final / ||y_final||_1 (or mu0 when no iteration ran)
 __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* y, const double* mu0, uint32_t n,
                                                          const PiState* st, double* mu) {
