@@ -1362,8 +1362,9 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     hipGraphExec_t exec = nullptr;
     // small matrices: one fused kernel per iteration; larger ones: step + single-block reduce
     const uint32_t nbFused = std::max<uint32_t>(1, ceilDiv(uint64_t(n) * kPiLanes, kPiBlock));
-    const bool fused = nbFused <= kPiFusedMaxBlocks && !(std::getenv("B12X_PI_FUSED") &&
-                                                         std::getenv("B12X_PI_FUSED")[0] == '0');
+    // the fused form is opt-in (B12X_PI_FUSED=1): on MI355X it measured slower than step+reduce
+    const char* pf = std::getenv("B12X_PI_FUSED");
+    const bool fused = nbFused <= kPiFusedMaxBlocks && pf && pf[0] == '1';
     double* partF = nullptr;
     uint32_t* blockIter = nullptr;
     if (fused) {

@@ -737,10 +737,23 @@ __global__ __launch_bounds__(kBlock) void piReduceKernel(const double* partAbs, 
                                                          PiState* st, uint32_t maxIter, double tol) {
   if (st->done) return;
   const uint32_t t = st->t;
+  // issue all of a thread's loads before summing them (one memory latency instead of one per
+  // loop trip); the summation order per thread stays fixed
+  constexpr int U = 8;
   double ta = 0.0, tr = 0.0;
-  for (uint32_t b = threadIdx.x; b < nb; b += kBlock) {
-    ta += partAbs[b];
-    tr += partRes[b];
+  for (uint32_t b0 = 0; b0 < nb; b0 += kBlock * U) {
+    double va[U], vr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t b = b0 + u * kBlock + threadIdx.x;
+      va[u] = b < nb ? partAbs[b] : 0.0;
+      vr[u] = b < nb ? partRes[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ta += va[u];
+      tr += vr[u];
+    }
   }
   ta = blockReduceSumF64(ta);
   tr = blockReduceSumF64(tr);

@@ -260,6 +260,7 @@ def test_power_iteration_graph_replays_match_eager(monkeypatch):
 def test_power_iteration_fused_and_two_kernel_paths_agree(monkeypatch):
     g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 13)
     m = g.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon())
+    monkeypatch.setenv("B12X_PI_FUSED", "1")
     mu_f, it_f, r_f = m.power_iteration(100000, 1e-12)
     monkeypatch.setenv("B12X_PI_FUSED", "0")
     mu_2, it_2, r_2 = m.power_iteration(100000, 1e-12)
@@ -268,6 +269,7 @@ def test_power_iteration_fused_and_two_kernel_paths_agree(monkeypatch):
     assert abs(it_f - ito) <= 1 and abs(it_2 - ito) <= 1
     assert np.allclose(mu_f, muo, rtol=1e-10, atol=1e-16) and np.allclose(mu_2, muo, rtol=1e-10, atol=1e-16)
     for mi in (5, 200):
+        monkeypatch.setenv("B12X_PI_FUSED", "1")
         a, ia, _ = m.power_iteration(mi)
         monkeypatch.setenv("B12X_PI_FUSED", "0")
         b, ib, _ = m.power_iteration(mi)
