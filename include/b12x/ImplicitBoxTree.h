@@ -1,0 +1,463 @@
+// C++ façade of B12's tree / matrix / functor API over the b12x C ABI (b12x_c_api.h).
+//
+// A B12 program switches its hot path to the MI355X by including this header instead of B12's
+// ImplicitBoxTree.h, Grids.h and Map_*.h and using the architecture tag b12::MI355X:
+//
+//   ImplicitBoxTree<MI355X, 2, double, 64> ibt(center, radius);      // ImplicitBoxTree.h:49-51
+//   auto itPair = ibt.makePointIteratorPairOverLeaves(grid, map);    // ImplicitBoxTree.h:308-326
+//   ibt.setFlagsByIterators(itPair, Flag::HIT, -1);                  // ImplicitBoxTree.h:468-469
+//
+// Names, argument meaning and return values follow the reference (citations relative to
+// B12c/src/include/). ThrustSystem<MI355X>::Vector<T> is a host std::vector: the tree's own
+// arrays live in HBM behind the handle, and vectors passed in or returned are host copies, as in
+// B12's MEX binding. Error convention as in B12: invalid input prints "Warning: ..." to
+// std::cout and leaves the tree unchanged (ImplicitBoxTree.hpp:3501,3548); a GPU runtime failure
+// throws std::runtime_error (B12 surfaces those as thrust::system_error).
+#pragma once
+
+#include <cstdint>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../b12x_c_api.h"
+
+namespace b12 {
+
+enum Architecture { MI355X = 16 };
+
+using Depth = uint8_t;       // TypeDefinitions.h:19
+using Dimension = uint8_t;   // TypeDefinitions.h:37
+using NrBoxes = uint32_t;    // TypeDefinitions.h:45
+using NrPoints = uint64_t;   // TypeDefinitions.h:57
+using Flags = uint8_t;       // Flags.h:9
+
+enum Flag : Flags { NONE = 0x00, HIT = 0x01, INS = 0x02, EXPD = 0x04, SD = 0x08, EQU = 0x10, ENTR = 0x20,
+                    EXTR = 0x40, ALL = 0xFF };  // Flags.h:11-22
+
+inline Depth getLeafDepth() { return Depth(-1); }          // TypeDefinitions.hpp
+inline NrBoxes getInvalidNrBoxes() { return NrBoxes(-1); }  // TypeDefinitions.hpp
+
+template <Architecture A>
+struct ThrustSystem {
+  template <typename T>
+  using Vector = std::vector<T>;
+};
+
+namespace detail {
+inline bool check(int rc, const char* what) {
+  if (rc == B12X_OK) return true;
+  if (rc == B12X_E_HIP || rc == B12X_E_NOMEM || rc == B12X_E_NODEVICE)
+    throw std::runtime_error(std::string(what) + ": " + b12x_last_error());
+  std::cout << "Warning: " << b12x_last_error() << std::endl;
+  return false;
+}
+
+struct SchemeHolder {
+  b12x_scheme h = nullptr;
+  ~SchemeHolder() {
+    if (h) b12x_scheme_destroy(h);
+  }
+};
+}  // namespace detail
+
+// ---- Grids (Grids.h:307-370): kind + parameters ------------------------------------------------
+template <Dimension DIM, typename REAL>
+struct InnerGrid {
+  NrPoints nPointsPerDim_;
+  explicit InnerGrid(NrPoints nPointsPerDim = 1) : nPointsPerDim_(nPointsPerDim) {}
+  NrPoints getNumberOfPointsPerBox() const {
+    NrPoints k = 1;
+    for (int j = 0; j < DIM; ++j) k *= nPointsPerDim_;
+    return k;
+  }
+  Dimension dim() const { return DIM; }
+  int kind() const { return B12X_GRID_INNER; }
+  std::vector<uint64_t> params() const { return {nPointsPerDim_}; }
+};
+
+template <Dimension DIM, typename REAL>
+struct UniformGrid {
+  NrPoints nPoints_;
+  unsigned long long seed_;
+  explicit UniformGrid(NrPoints nPoints = 1, unsigned long long seed = 0) : nPoints_(nPoints), seed_(seed) {}
+  NrPoints getNumberOfPointsPerBox() const { return nPoints_; }
+  Dimension dim() const { return DIM; }
+  int kind() const { return B12X_GRID_UNIFORM; }
+  std::vector<uint64_t> params() const { return {nPoints_, seed_}; }
+};
+
+// ---- Maps (Map_Henon.h:26-40, Map_Lorenz.h:29-44, Map_Identity.h:9-21) ---------------------------
+template <Dimension DIM, typename REAL>
+struct Henon {
+  REAL a_, b_;
+  uint32_t steps_;
+  explicit Henon(REAL a = 1.4, REAL b = 0.3, uint32_t steps = 1) : a_(a), b_(b), steps_(steps) {}
+  int kind() const { return B12X_MAP_HENON; }
+  std::vector<double> params() const { return {double(a_), double(b_), double(steps_)}; }
+};
+
+template <Dimension DIM, typename REAL>
+struct Lorenz {
+  REAL sigma_, rho_, beta_, h_;
+  uint32_t steps_;
+  explicit Lorenz(REAL sigma = 10.0, REAL rho = 28.0, REAL beta = 8.0 / 3.0, REAL h = 0.01, uint32_t steps = 20)
+      : sigma_(sigma), rho_(rho), beta_(beta), h_(h), steps_(steps) {}
+  int kind() const { return B12X_MAP_LORENZ; }
+  std::vector<double> params() const {
+    return {double(sigma_), double(rho_), double(beta_), double(h_), double(steps_)};
+  }
+};
+
+template <Dimension DIM, typename REAL>
+struct Identity {
+  int kind() const { return B12X_MAP_IDENTITY; }
+  std::vector<double> params() const { return {}; }
+};
+
+// Hénon–Heiles flow as SymplecticIntegratorAuto<4> (MapTools.hpp:361-523); no B12 counterpart map.
+template <Dimension DIM, typename REAL>
+struct HenonHeiles {
+  uint8_t order_;
+  REAL h_;
+  uint32_t steps_;
+  explicit HenonHeiles(uint8_t order = 2, REAL h = 0.02, uint32_t steps = 50) : order_(order), h_(h), steps_(steps) {}
+  int kind() const { return B12X_MAP_HENON_HEILES; }
+  std::vector<double> params() const { return {double(order_), double(h_), double(steps_)}; }
+};
+
+// Forced Duffing period map as RK4 (MapTools.hpp:35-108); no B12 counterpart map.
+template <Dimension DIM, typename REAL>
+struct ForcedDuffing {
+  REAL delta_, alpha_, beta_, gamma_, omega_, t0_, tEnd_, h_;
+  explicit ForcedDuffing(REAL delta = 0.25, REAL alpha = -1.0, REAL beta = 1.0, REAL gamma = 0.3, REAL omega = 1.0,
+                         REAL t0 = 0.0, REAL tEnd = 6.283185307179586, REAL h = 6.283185307179586 / 100.0)
+      : delta_(delta), alpha_(alpha), beta_(beta), gamma_(gamma), omega_(omega), t0_(t0), tEnd_(tEnd), h_(h) {}
+  int kind() const { return B12X_MAP_DUFFING; }
+  std::vector<double> params() const { return {delta_, alpha_, beta_, gamma_, omega_, t0_, tEnd_, h_}; }
+};
+
+// The lazy point range of makePointIteratorPair* (ImplicitBoxTree.h:246-430): instead of a
+// thrust iterator pair it carries the compiled (grid, map) scheme and the iterator's depth/flags.
+struct PointIteratorPair {
+  std::shared_ptr<detail::SchemeHolder> scheme;
+  Depth depth;
+  Flags flagsToCheck;
+  NrPoints nPointsPerBox;
+};
+
+// ---- CooMatrix (CooMatrix.h:56-121) -----------------------------------------------------------
+template <Architecture A, typename T>
+struct CooMatrix {
+  typename ThrustSystem<A>::template Vector<NrBoxes> rowIndices_, columnIndices_;
+  typename ThrustSystem<A>::template Vector<T> values_;
+  NrBoxes nRows_ = 0, nColumns_ = 0;
+};
+
+// compressIndexVector<A>(indices, dim) — CooMatrix.h:20-23
+template <Architecture A>
+inline void compressIndexVector(typename ThrustSystem<A>::template Vector<NrBoxes>& indices, NrBoxes dim) {
+  std::vector<NrBoxes> out(size_t(dim) + 1);
+  if (detail::check(b12x_compress_index_vector(indices.data(), indices.size(), dim, out.data()),
+                    "compressIndexVector"))
+    indices.swap(out);
+}
+
+// Invariant measure of a row-stochastic transition matrix (build-defined; B12 does this step in
+// MATLAB, subdivisionByNim.m:30-59): returns mu with mu = mu P / ||mu P||_1.
+template <Architecture A, typename T>
+inline std::vector<double> invariantMeasure(const CooMatrix<A, T>& m, int maxIter, double tol, int* iters = nullptr,
+                                            double* resid = nullptr) {
+  b12x_coo h = nullptr;
+  std::vector<double> vals(m.values_.begin(), m.values_.end());
+  detail::check(b12x_coo_upload(m.rowIndices_.data(), m.columnIndices_.data(), vals.data(), m.rowIndices_.size(),
+                                m.nRows_, m.nColumns_, &h),
+                "invariantMeasure");
+  std::vector<double> mu(m.nRows_);
+  int it = 0;
+  double r = 0.0;
+  detail::check(b12x_power_iteration(h, maxIter, tol, 0, mu.data(), &it, &r), "invariantMeasure");
+  b12x_coo_destroy(h);
+  if (iters) *iters = it;
+  if (resid) *resid = r;
+  return mu;
+}
+
+// ---- ImplicitBoxTree (ImplicitBoxTree.h:32-640) ---------------------------------------------------
+template <Architecture A, Dimension DIM, typename REAL = double, Depth N = 64>
+class ImplicitBoxTree {
+  static_assert(A == MI355X, "this header implements the MI355X architecture");
+  static_assert(N == 64, "bit patterns of 64 bits (every configuration on the path fits N = 64)");
+  static_assert(DIM >= 1 && DIM <= 4, "dimensions 1..4");
+
+ public:
+  template <typename T>
+  using Vector = typename ThrustSystem<A>::template Vector<T>;
+
+  // ImplicitBoxTree.h:44 default constructor: the unit hypercube [0,1]^DIM
+  ImplicitBoxTree() : ImplicitBoxTree(Vector<REAL>(DIM, REAL(0.5)), Vector<REAL>(DIM, REAL(0.5))) {}
+
+  // ImplicitBoxTree.h:49-51
+  template <typename REAL2>
+  ImplicitBoxTree(const Vector<REAL2>& center, const Vector<REAL2>& radius) {
+    create(center, radius, nullptr, 0);
+  }
+
+  // ImplicitBoxTree.h:53-56
+  template <typename REAL2, typename UInt>
+  ImplicitBoxTree(const Vector<REAL2>& center, const Vector<REAL2>& radius, const Vector<UInt>& sdScheme) {
+    std::vector<uint8_t> sd(sdScheme.begin(), sdScheme.end());
+    create(center, radius, sd.data(), int(sd.size()));
+  }
+
+  ImplicitBoxTree(const ImplicitBoxTree&) = delete;
+  ImplicitBoxTree& operator=(const ImplicitBoxTree&) = delete;
+  ~ImplicitBoxTree() {
+    if (h_) b12x_tree_destroy(h_);
+  }
+
+  // ---- queries
+  NrBoxes size() const { return ret([&](uint32_t* r) { return b12x_size(h_, r); }, "size"); }
+  NrBoxes count(Depth depth = getLeafDepth()) const {
+    return ret([&](uint32_t* r) { return b12x_count(h_, depth, r); }, "count");
+  }
+  Depth depth() const {
+    uint8_t d = 0;
+    detail::check(b12x_depth(h_, &d), "depth");
+    return d;
+  }
+  Dimension dim() const { return DIM; }
+
+  // bitPatternVector_, depthVector_, flagsVector_ as host copies (ImplicitBoxTree.h:619-624)
+  void getLeaves(Vector<uint64_t>& bits, Vector<Depth>& depths, Vector<Flags>& flags) const {
+    const NrBoxes n = size();
+    bits.resize(n);
+    depths.resize(n);
+    flags.resize(n);
+    detail::check(b12x_tree_download(h_, bits.data(), depths.data(), flags.data()), "getLeaves");
+  }
+  Vector<Depth> getDepthVector() const {
+    Vector<uint64_t> b;
+    Vector<Depth> d;
+    Vector<Flags> f;
+    getLeaves(b, d, f);
+    return d;
+  }
+  Vector<Flags> getFlagsVector() const {
+    Vector<uint64_t> b;
+    Vector<Depth> d;
+    Vector<Flags> f;
+    getLeaves(b, d, f);
+    return f;
+  }
+
+  // ---- structure
+  NrBoxes subdivide(Flags flags = Flag::SD) { return ret([&](uint32_t* r) { return b12x_subdivide(h_, flags, r); }, "subdivide"); }
+  NrBoxes unsubdivide(Flags flags = Flag::SD) {
+    return ret([&](uint32_t* r) { return b12x_unsubdivide(h_, flags, r); }, "unsubdivide");
+  }
+  NrBoxes remove(Flags flags) { return ret([&](uint32_t* r) { return b12x_remove(h_, flags, r); }, "remove"); }
+
+  // ---- flags (ImplicitBoxTree.h:93-108, 461-485, 527-541)
+  NrBoxes setFlags(Flags flags, Depth depth = getLeafDepth()) {
+    return ret([&](uint32_t* r) { return b12x_set_flags(h_, flags, depth, r); }, "setFlags");
+  }
+  NrBoxes unsetFlags(Flags flags, Depth depth = getLeafDepth()) {
+    return ret([&](uint32_t* r) { return b12x_unset_flags(h_, flags, depth, r); }, "unsetFlags");
+  }
+  NrBoxes changeFlags(Flags flagsToUnset, Flags flagsToSet, Depth depth = getLeafDepth()) {
+    return ret([&](uint32_t* r) { return b12x_change_flags(h_, flagsToUnset, flagsToSet, depth, r); }, "changeFlags");
+  }
+  template <typename BoolIterator>
+  NrBoxes setFlagsByStencil(BoolIterator stencil_begin, Flags flags, Depth depth = getLeafDepth()) {
+    return stencilOp(0, stencil_begin, flags, 0, depth);
+  }
+  template <typename BoolIterator>
+  NrBoxes unsetFlagsByStencil(BoolIterator stencil_begin, Flags flags, Depth depth = getLeafDepth()) {
+    return stencilOp(1, stencil_begin, flags, 0, depth);
+  }
+  template <typename BoolIterator>
+  NrBoxes changeFlagsByStencil(BoolIterator stencil_begin, Flags flagsToUnset, Flags flagsToSet,
+                               Depth depth = getLeafDepth()) {
+    return stencilOp(2, stencil_begin, flagsToUnset, flagsToSet, depth);
+  }
+  NrBoxes setFlagsByIterators(const PointIteratorPair& points, Flags flags, Depth depth = getLeafDepth()) {
+    return imageOp(points, 0, flags, 0, depth);
+  }
+  NrBoxes unsetFlagsByIterators(const PointIteratorPair& points, Flags flags, Depth depth = getLeafDepth()) {
+    return imageOp(points, 1, flags, 0, depth);
+  }
+  NrBoxes changeFlagsByIterators(const PointIteratorPair& points, Flags flagsToUnset, Flags flagsToSet,
+                                 Depth depth = getLeafDepth()) {
+    return imageOp(points, 2, flagsToUnset, flagsToSet, depth);
+  }
+  // setFlagsBySearch(sorted unique searchVector, flags, depth) — ImplicitBoxTree.h:477-480
+  NrBoxes setFlagsBySearch(const Vector<NrBoxes>& searchVector, Flags flags, Depth depth = getLeafDepth()) {
+    return stencilOp(0, searchStencil(searchVector, depth).begin(), flags, 0, depth);
+  }
+  NrBoxes unsetFlagsBySearch(const Vector<NrBoxes>& searchVector, Flags flags, Depth depth = getLeafDepth()) {
+    return stencilOp(1, searchStencil(searchVector, depth).begin(), flags, 0, depth);
+  }
+  // setFlagsByPoints(points {x0..x_{DIM-1}; ...}, flags, depth) — ImplicitBoxTree.h:474-475
+  NrBoxes setFlagsByPoints(const Vector<REAL>& points, Flags flags, Depth depth = getLeafDepth()) {
+    Vector<NrBoxes> s = searchByPoints(points, depth);
+    return stencilOp(0, searchStencil(s, depth).begin(), flags, 0, depth);
+  }
+
+  // ---- point iterators (ImplicitBoxTree.h:266-430)
+  template <typename Grid, typename Map>
+  PointIteratorPair makePointIteratorPairOverLeaves(Grid grid, Map map, Flags flagsToSet = Flag::NONE) {
+    return makePair(grid, map, getLeafDepth(), Flag::NONE);
+  }
+  template <typename Grid, typename Map>
+  PointIteratorPair makePointIteratorPairOverRoot(Grid grid, Map map, Flags flagsToSet = Flag::NONE) {
+    return makePair(grid, map, 0, Flag::NONE);
+  }
+  template <typename Grid, typename Map>
+  PointIteratorPair makePointIteratorPairByFlags(Depth depth, Flags flagsToCheck, Grid grid, Map map,
+                                                 Flags flagsToSet = Flag::NONE) {
+    return makePair(grid, map, depth, flagsToCheck);
+  }
+
+  // ---- search (ImplicitBoxTree.h:436-459)
+  Vector<NrBoxes> searchByIterators(const PointIteratorPair& points, Depth depth = getLeafDepth()) {
+    uint64_t n = 0;
+    if (!detail::check(b12x_num_points(h_, points.scheme->h, points.depth, points.flagsToCheck, &n),
+                       "searchByIterators"))
+      return {};
+    Vector<NrBoxes> out(n);
+    detail::check(b12x_search_by_image(h_, points.scheme->h, points.depth, points.flagsToCheck, depth, out.data()),
+                  "searchByIterators");
+    return out;
+  }
+  Vector<NrBoxes> searchCompacted(const PointIteratorPair& points, Depth depth = getLeafDepth()) {
+    Vector<NrBoxes> out(std::max<NrBoxes>(size(), 1));
+    uint64_t n = 0;
+    detail::check(b12x_search_compacted(h_, points.scheme->h, points.depth, points.flagsToCheck, depth, out.data(), &n),
+                  "searchCompacted");
+    out.resize(n);
+    return out;
+  }
+  Vector<NrBoxes> searchByPoints(const Vector<REAL>& points, Depth depth = getLeafDepth()) {
+    std::vector<double> p(points.begin(), points.end());
+    Vector<NrBoxes> out(p.size() / DIM);
+    detail::check(b12x_search_by_points(h_, p.data(), out.size(), depth, out.data()), "searchByPoints");
+    return out;
+  }
+
+  // ---- transition matrices (ImplicitBoxTree.h:491-525)
+  template <typename T>
+  void transitionMatrix(const PointIteratorPair& points, Depth depth, NrPoints nPointsPerBox, bool useAbsoluteValues,
+                        bool useRowMajor, bool useInvalidNrBoxes, CooMatrix<A, T>& res) {
+    (void)nPointsPerBox;  // implied by the scheme's grid
+    b12x_coo m = nullptr;
+    if (!detail::check(b12x_transition_matrix(h_, points.scheme->h, points.depth, points.flagsToCheck, depth,
+                                              useAbsoluteValues, useRowMajor, useInvalidNrBoxes, &m),
+                       "transitionMatrix"))
+      return;
+    uint64_t nnz = 0;
+    uint32_t nr = 0, nc = 0;
+    int rm = 0;
+    b12x_coo_info(m, &nnz, &nr, &nc, &rm);
+    res.rowIndices_.resize(nnz);
+    res.columnIndices_.resize(nnz);
+    std::vector<double> v(nnz);
+    detail::check(b12x_coo_download(m, res.rowIndices_.data(), res.columnIndices_.data(), v.data()),
+                  "transitionMatrix");
+    res.values_.assign(v.begin(), v.end());
+    res.nRows_ = nr;
+    res.nColumns_ = nc;
+    b12x_coo_destroy(m);
+  }
+  template <typename T, typename Grid, typename Map>
+  CooMatrix<A, T> transitionMatrixForLeaves(Grid grid, Map map, bool useAbsoluteValues = false,
+                                            bool useRowMajor = false, bool useInvalidNrBoxes = false) {
+    CooMatrix<A, T> res;
+    transitionMatrix<T>(makePointIteratorPairOverLeaves(grid, map), getLeafDepth(), grid.getNumberOfPointsPerBox(),
+                        useAbsoluteValues, useRowMajor, useInvalidNrBoxes, res);
+    return res;
+  }
+  template <typename T, typename Grid, typename Map>
+  CooMatrix<A, T> transitionMatrixForRoot(Grid grid, Map map, bool useAbsoluteValues = false,
+                                          bool useRowMajor = false, bool useInvalidNrBoxes = false) {
+    CooMatrix<A, T> res;
+    transitionMatrix<T>(makePointIteratorPairOverRoot(grid, map), 0, grid.getNumberOfPointsPerBox(),
+                        useAbsoluteValues, useRowMajor, useInvalidNrBoxes, res);
+    return res;
+  }
+
+  // ---- fused subdivision steps (subdivision.m:18-25): returns the four counts per step
+  template <typename Grid, typename Map>
+  std::vector<NrBoxes> subdivisionSteps(Grid grid, Map map, int steps, Flags sd = Flag::SD) {
+    PointIteratorPair p = makePair(grid, map, getLeafDepth(), Flag::NONE);
+    std::vector<NrBoxes> counts(size_t(4) * std::max(steps, 1));
+    detail::check(b12x_subdivision_steps(h_, p.scheme->h, sd, steps, counts.data()), "subdivisionSteps");
+    counts.resize(size_t(4) * steps);
+    return counts;
+  }
+
+  b12x_tree handle() const { return h_; }
+
+ private:
+  b12x_tree h_ = nullptr;
+
+  template <typename REAL2>
+  void create(const Vector<REAL2>& center, const Vector<REAL2>& radius, const uint8_t* sd, int nsd) {
+    if (center.size() != DIM || radius.size() != DIM)
+      throw std::invalid_argument("center and radius must have DIM entries");
+    double c[DIM], r[DIM];
+    for (int j = 0; j < DIM; ++j) {
+      c[j] = double(center[j]);
+      r[j] = double(radius[j]);
+    }
+    int rc = b12x_tree_create(DIM, c, r, sd, nsd, &h_);
+    if (rc == B12X_E_ARG && sd) {  // ImplicitBoxTree.hpp:3548: warn and keep the default scheme
+      std::cout << "Warning: Input has values >= DIM. SdScheme remains unchanged." << std::endl;
+      rc = b12x_tree_create(DIM, c, r, nullptr, 0, &h_);
+    }
+    if (rc != B12X_OK) throw std::runtime_error(std::string("ImplicitBoxTree: ") + b12x_last_error());
+  }
+
+  template <typename F>
+  NrBoxes ret(F f, const char* what) const {
+    uint32_t r = 0;
+    detail::check(f(&r), what);
+    return r;
+  }
+
+  template <typename Grid, typename Map>
+  PointIteratorPair makePair(const Grid& grid, const Map& map, Depth depth, Flags flagsToCheck) {
+    auto holder = std::make_shared<detail::SchemeHolder>();
+    std::vector<uint64_t> gp = grid.params();
+    std::vector<double> mp = map.params();
+    int rc = b12x_scheme_create(DIM, grid.kind(), gp.data(), int(gp.size()), map.kind(), mp.data(), int(mp.size()),
+                                &holder->h);
+    if (rc != B12X_OK) throw std::runtime_error(std::string("makePointIteratorPair: ") + b12x_last_error());
+    return PointIteratorPair{holder, depth, flagsToCheck, grid.getNumberOfPointsPerBox()};
+  }
+
+  template <typename BoolIterator>
+  NrBoxes stencilOp(int op, BoolIterator st, Flags a, Flags b, Depth depth) {
+    const NrBoxes n = (depth == 0) ? 1 : size();
+    std::vector<uint8_t> s(n);
+    for (NrBoxes i = 0; i < n; ++i, ++st) s[i] = *st ? 1 : 0;
+    return ret([&](uint32_t* r) { return b12x_flags_by_stencil(h_, op, s.data(), a, b, depth, r); }, "flagsByStencil");
+  }
+
+  std::vector<uint8_t> searchStencil(const Vector<NrBoxes>& sv, Depth depth) {
+    std::vector<uint8_t> s((depth == 0) ? 1 : size(), 0);
+    for (NrBoxes i : sv)
+      if (i != getInvalidNrBoxes() && i < s.size()) s[i] = 1;
+    return s;
+  }
+
+  NrBoxes imageOp(const PointIteratorPair& p, int op, Flags a, Flags b, Depth depth) {
+    return ret([&](uint32_t* r) {
+      return b12x_flags_by_image(h_, p.scheme->h, p.depth, p.flagsToCheck, op, a, b, depth, r);
+    }, "flagsByIterators");
+  }
+};
+
+}  // namespace b12

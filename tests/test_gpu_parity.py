@@ -277,3 +277,22 @@ def test_power_iteration_fused_and_two_kernel_paths_agree(monkeypatch):
         c, ic, _ = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()).power_iteration(mi)
         assert ia == ib == ic == mi
         assert np.allclose(a, c, rtol=MEASURE_RTOL, atol=0) and np.allclose(b, c, rtol=MEASURE_RTOL, atol=0)
+
+
+def test_cpp_facade_doc_example():
+    """B12's documented C++ example built against include/b12x/ImplicitBoxTree.h gives the
+    oracle's leaf counts per step."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "doc_example")
+    assert os.path.exists(exe), "tests/cpp/doc_example not built (run __graft_entry__.build())"
+    out = subprocess.run([exe, "12"], check=True, capture_output=True, text=True, timeout=120).stdout.splitlines()
+    o = OracleTree([0.0, 0.0], [3.0, 3.0])
+    o.setFlags(gx.SD, 0)
+    counts = o.subdivision_steps(gx.InnerGrid(10), gx.Henon(1.4, 0.3, 1), 12)
+    got = [int(line.split(":")[1]) for line in out[:12]]
+    assert got == [int(c[3]) for c in counts]
+    m = o.transitionMatrixForLeaves(gx.InnerGrid(10), gx.Henon(1.4, 0.3, 1))
+    assert out[12] == f"nnz {m.nnz} rows {m.nRows} cols {m.nColumns}"
+    mu, _, _ = m.power_iteration(50)
+    assert abs(float(out[13].split()[1]) - mu[0]) <= 1e-12 * abs(mu[0])

@@ -387,3 +387,8 @@ def test_library_exports_every_declared_symbol(flavour):
     # every declared entry point is bound by the Python mirror as well
     from b12_amd.api import SIGNATURES
     assert set(SIGNATURES) | {"b12x_last_error"} >= set(syms)
+
+
+def test_cpp_facade_builds():
+    exe = os.path.join(ROOT, "tests", "cpp", "doc_example")
+    assert os.path.exists(exe), "façade example not built"
