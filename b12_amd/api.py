@@ -60,6 +60,18 @@ SIGNATURES = {
     "b12x_coo_upload": [_P, _P, _P, _U64, _U32, _U32, _P],
     "b12x_compress_index_vector": [_P, _U64, _U32, _P],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
+    "b12x_forest_create": [_I, _I, _P, _I, _I, _I, _I, _I, _D, _P],
+    "b12x_forest_destroy": [_P],
+    "b12x_forest_add": [_P, _P, _P, _P, _I, _U8, _P],
+    "b12x_forest_step": [_P, _P, _P],
+    "b12x_forest_run": [_P],
+    "b12x_forest_job_info": [_P, _U32, _P, _P, _P, _P, _P],
+    "b12x_forest_job_cover": [_P, _U32, _P, _P, _P],
+    "b12x_forest_job_matrix": [_P, _U32, _P, _P, _P],
+    "b12x_forest_job_measure": [_P, _U32, _P],
+    "b12x_forest_job_release": [_P, _U32],
+    "b12x_forest_footprint": [_P, _P],
+    "b12x_serve_run": [_P, _I, _U32, _P, _P, _P, _P, _I, _P, _I, _P, _D, _D, _P, _P, _P],
 }
 
 _lib = None
@@ -410,3 +422,97 @@ class ImplicitBoxTree:
 
     def transitionMatrixForRoot(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):
         return self.transitionMatrix(self.makePointIteratorPairOverRoot(grid, mp), 0, use_abs, row_major, use_invalid)
+
+
+class Forest:
+    """Many subdivision jobs of one (grid, map kind) run in shared launches (batched sweeps and
+    continuous-batching serving). Each job has its own domain, map parameters and depth."""
+
+    def __init__(self, dim, grid, map_kind, use_abs=False, row_major=False, pi_max_iter=200, pi_tol=-1.0,
+                 flavour="parity"):
+        self.api = _Api(flavour)
+        self.dim = dim
+        gp = np.asarray(grid.params, np.uint64)
+        h = _P()
+        self.api("b12x_forest_create", dim, grid.kind, _ptr(gp), len(grid.params), map_kind, int(use_abs),
+                 int(row_major), int(pi_max_iter), float(pi_tol), ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.api.lib.b12x_forest_destroy(self.h)
+            self.h = None
+
+    def add(self, center, radius, mp, depth):
+        c = np.ascontiguousarray(center, np.float64)
+        r = np.ascontiguousarray(radius, np.float64)
+        p = np.ascontiguousarray(mp.params if mp.params else [0.0], np.float64)
+        j = _U32()
+        self.api("b12x_forest_add", self.h, _ptr(c), _ptr(r), _ptr(p), len(mp.params), int(depth), ctypes.byref(j))
+        return j.value
+
+    def step(self):
+        a, r = _U32(), _U32()
+        self.api("b12x_forest_step", self.h, ctypes.byref(a), ctypes.byref(r))
+        return a.value, r.value
+
+    def run(self):
+        self.api("b12x_forest_run", self.h)
+
+    def info(self, job):
+        st, nl, nnz, it, res = _I(), _U32(), _U64(), _I(), _D()
+        self.api("b12x_forest_job_info", self.h, job, ctypes.byref(st), ctypes.byref(nl), ctypes.byref(nnz),
+                 ctypes.byref(it), ctypes.byref(res))
+        return {"state": st.value, "n": nl.value, "nnz": nnz.value, "iters": it.value, "resid": res.value}
+
+    def cover(self, job):
+        n = self.info(job)["n"]
+        b, d, f = np.empty(n, np.uint64), np.empty(n, np.uint8), np.empty(n, np.uint8)
+        self.api("b12x_forest_job_cover", self.h, job, _ptr(b), _ptr(d), _ptr(f))
+        return b, d, f
+
+    def matrix(self, job):
+        nnz = self.info(job)["nnz"]
+        r, c, v = np.empty(nnz, np.uint32), np.empty(nnz, np.uint32), np.empty(nnz, np.float64)
+        self.api("b12x_forest_job_matrix", self.h, job, _ptr(r), _ptr(c), _ptr(v))
+        return r, c, v
+
+    def measure(self, job):
+        mu = np.empty(self.info(job)["n"], np.float64)
+        self.api("b12x_forest_job_measure", self.h, job, _ptr(mu))
+        return mu
+
+    def release(self, job):
+        self.api("b12x_forest_job_release", self.h, job)
+
+    def footprint(self):
+        b = _U64()
+        self.api("b12x_forest_footprint", self.h, ctypes.byref(b))
+        return b.value
+
+
+def serve_run(forests, arrival, forest_of, centers, radii, map_params, depths, budget_bytes=200e9,
+              reserve_per_job=64e6):
+    """Continuous-batching request loop (b12x_serve_run); returns (latency, completion, job_ids)."""
+    api = forests[0].api
+    n = len(arrival)
+    arr = (_P * len(forests))(*[f.h for f in forests])
+    arrival = np.ascontiguousarray(arrival, np.float64)
+    forest_of = np.ascontiguousarray(forest_of, np.int32)
+    stride = max(f.dim for f in forests)
+    C = np.zeros((n, stride))
+    R = np.zeros((n, stride))
+    for i in range(n):
+        C[i, :len(centers[i])] = centers[i]
+        R[i, :len(radii[i])] = radii[i]
+    nmp = max(len(p) for p in map_params) if n else 0
+    MP = np.zeros((n, max(nmp, 1)))
+    for i in range(n):
+        MP[i, :len(map_params[i])] = map_params[i]
+    depths = np.ascontiguousarray(depths, np.uint8)
+    lat = np.empty(n, np.float64)
+    comp = np.empty(n, np.float64)
+    ids = np.empty(n, np.uint32)
+    api("b12x_serve_run", arr, len(forests), n, _ptr(arrival), _ptr(forest_of), _ptr(C), _ptr(R), stride, _ptr(MP),
+        max(nmp, 1), _ptr(depths), float(budget_bytes), float(reserve_per_job), _ptr(lat), _ptr(comp), _ptr(ids))
+    return lat, comp, ids

@@ -1,0 +1,812 @@
+// Batched and served forms of the hot path (b12x_forest_* and b12x_serve_run in
+// include/b12x_c_api.h). A forest holds the trees of many jobs in one set of leaf arrays; every
+// step runs one subdivision step (subdivision.m:18-25) for all of its active jobs in shared
+// launches, and jobs that reach their depth retire through one batched transition matrix
+// (ImplicitBoxTree.hpp:2932-3159) and one block-diagonal power iteration with per-job stopping.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "../../include/b12x_c_api.h"
+#include "b12x_forest_kernels.hpp"
+#include "b12x_host.hpp"
+#include "b12x_launch.hpp"
+
+using namespace b12x;
+
+namespace {
+
+// Device results of one retirement wave; jobs keep offsets into it.
+struct ResultArena {
+  uint64_t* key = nullptr;
+  uint8_t* depth = nullptr;
+  uint8_t* flags = nullptr;
+  uint32_t* rows = nullptr;
+  uint32_t* cols = nullptr;
+  double* vals = nullptr;
+  double* mu = nullptr;
+  uint64_t bytes = 0;
+  ~ResultArena() {
+    dfree(key); dfree(depth); dfree(flags); dfree(rows); dfree(cols); dfree(vals); dfree(mu);
+  }
+};
+
+struct Job {
+  int state = 0;  // 0 waiting for upload, 1 active, 2 done, 3 released
+  uint8_t target = 0, level = 0;
+  JobDev dev{};
+  ForcingStep* dForcing = nullptr;
+  uint32_t nLeaves = 0;
+  uint64_t nnz = 0;
+  int iters = 0;
+  double resid = -1.0;
+  std::shared_ptr<ResultArena> arena;
+  uint64_t leafBase = 0, nnzBase = 0;
+  double tDone = 0.0;
+};
+
+double nowSeconds() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr int kForestScal = 16;
+
+}  // namespace
+
+struct b12x_forest_s {
+  int dim = 0, mapKind = 0;
+  GridParams grid{};
+  double* dTable = nullptr;
+  int useAbs = 0, rowMajor = 0, piMaxIter = 200;
+  double piTol = -1.0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0, cap = 0;
+  ForestView arr[2]{};
+  int cur = 0;
+  uint8_t* hit = nullptr;
+  uint32_t* totals = nullptr;
+  uint32_t* scal = nullptr;
+  uint32_t* hscal = nullptr;
+  uint32_t slotCap = 0;
+  JobDev* dJobs = nullptr;
+  uint32_t* dOff = nullptr;
+  uint32_t* dEnd = nullptr;
+  uint8_t* dRetiring = nullptr;
+  std::vector<Job> jobs;
+  std::vector<uint32_t> active;   // slots of active jobs
+  std::vector<uint32_t> waiting;  // slots added but not yet uploaded
+  double imageMs = 0.0;
+  uint64_t steps = 0;
+
+  ~b12x_forest_s() {
+    for (int b = 0; b < 2; ++b) {
+      dfree(arr[b].key); dfree(arr[b].depth); dfree(arr[b].flags); dfree(arr[b].job);
+    }
+    dfree(hit); dfree(totals); dfree(scal); dfree(dTable); dfree(dJobs); dfree(dOff); dfree(dEnd); dfree(dRetiring);
+    for (auto& j : jobs) dfree(j.dForcing);
+    if (hscal) (void)hipHostFree(hscal);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void fSync(b12x_forest_s& F, int count) {
+  HIPCHK(hipMemcpyAsync(F.hscal, F.scal, count * sizeof(uint32_t), hipMemcpyDeviceToHost, F.stream));
+  HIPCHK(hipStreamSynchronize(F.stream));
+}
+
+void ensureLeafCap(b12x_forest_s& F, uint64_t need) {
+  if (need > 0x7FFFFFFEull) throw HipFail(B12X_E_ARG, "forest exceeds 2^31 leaves");
+  if (need <= F.cap) return;
+  const uint64_t nc = std::max<uint64_t>({need, uint64_t(F.cap) * 2, 4096});
+  ForestView a[2];
+  for (int b = 0; b < 2; ++b) {
+    a[b].key = dalloc<uint64_t>(nc);
+    a[b].depth = dalloc<uint8_t>(nc);
+    a[b].flags = dalloc<uint8_t>(nc);
+    a[b].job = dalloc<uint32_t>(nc);
+  }
+  if (F.n) {
+    const ForestView& c = F.arr[F.cur];
+    HIPCHK(hipMemcpyAsync(a[0].key, c.key, size_t(F.n) * 8, hipMemcpyDeviceToDevice, F.stream));
+    HIPCHK(hipMemcpyAsync(a[0].depth, c.depth, F.n, hipMemcpyDeviceToDevice, F.stream));
+    HIPCHK(hipMemcpyAsync(a[0].flags, c.flags, F.n, hipMemcpyDeviceToDevice, F.stream));
+    HIPCHK(hipMemcpyAsync(a[0].job, c.job, size_t(F.n) * 4, hipMemcpyDeviceToDevice, F.stream));
+  }
+  uint8_t* h = dalloc<uint8_t>(nc);
+  HIPCHK(hipMemsetAsync(h, 0, nc, F.stream));
+  uint32_t* tt = dalloc<uint32_t>(nc * 2 / kTile + 16);
+  HIPCHK(hipStreamSynchronize(F.stream));
+  for (int b = 0; b < 2; ++b) {
+    dfree(F.arr[b].key); dfree(F.arr[b].depth); dfree(F.arr[b].flags); dfree(F.arr[b].job);
+    F.arr[b] = a[b];
+  }
+  dfree(F.hit);
+  dfree(F.totals);
+  F.hit = h;
+  F.totals = tt;
+  F.cur = 0;
+  F.cap = uint32_t(nc);
+}
+
+void ensureSlotCap(b12x_forest_s& F, uint32_t need) {
+  if (need <= F.slotCap) return;
+  const uint32_t nc = std::max<uint32_t>({need, F.slotCap * 2, 64});
+  JobDev* j = dalloc<JobDev>(nc);
+  uint32_t* o = dalloc<uint32_t>(nc);
+  uint32_t* e = dalloc<uint32_t>(nc);
+  uint8_t* r = dalloc<uint8_t>(nc);
+  HIPCHK(hipMemsetAsync(r, 0, nc, F.stream));
+  HIPCHK(hipMemsetAsync(o, 0, size_t(nc) * 4, F.stream));
+  HIPCHK(hipMemsetAsync(e, 0, size_t(nc) * 4, F.stream));
+  if (F.slotCap) {
+    HIPCHK(hipMemcpyAsync(j, F.dJobs, size_t(F.slotCap) * sizeof(JobDev), hipMemcpyDeviceToDevice, F.stream));
+    HIPCHK(hipMemcpyAsync(o, F.dOff, size_t(F.slotCap) * 4, hipMemcpyDeviceToDevice, F.stream));
+    HIPCHK(hipMemcpyAsync(e, F.dEnd, size_t(F.slotCap) * 4, hipMemcpyDeviceToDevice, F.stream));
+  }
+  HIPCHK(hipStreamSynchronize(F.stream));
+  dfree(F.dJobs); dfree(F.dOff); dfree(F.dEnd); dfree(F.dRetiring);
+  F.dJobs = j;
+  F.dOff = o;
+  F.dEnd = e;
+  F.dRetiring = r;
+  F.slotCap = nc;
+}
+
+template <class Op>
+void fScan(b12x_forest_s& F, const Op& op, uint32_t n, int scalSlot, const uint32_t* nDev = nullptr) {
+  if (n == 0) return;
+  const uint32_t nb = ceilDiv(n, kTile);
+  hipLaunchKernelGGL(scanCountKernel<Op>, dim3(nb), dim3(kBlock), 0, F.stream, op, n, nDev, F.totals,
+                     (uint32_t*)nullptr);
+  hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, F.stream, F.totals, nb, nDev, F.scal + scalSlot,
+                     (const uint32_t*)nullptr, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, F.stream, op, n, nDev, F.totals);
+  HIPCHK(hipGetLastError());
+}
+
+// Recompute every job's leaf range from the job ids (n leaves, or *nDev with n a bound).
+void fBoundary(b12x_forest_s& F, uint32_t n, const uint32_t* nDev = nullptr) {
+  hipLaunchKernelGGL(forestClearRangesKernel, dim3(ceilDiv(F.slotCap, kBlock)), dim3(kBlock), 0, F.stream, F.dOff,
+                     F.dEnd, F.slotCap);
+  if (n)
+    hipLaunchKernelGGL(forestBoundaryKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, F.stream,
+                       (const uint32_t*)F.arr[F.cur].job, n, nDev, F.dOff, F.dEnd);
+  HIPCHK(hipGetLastError());
+}
+
+template <int DIM, int MAP>
+void launchForestImage2(const ForestImageArgs& a, uint64_t pointsBound, hipStream_t st) {
+  const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((pointsBound + kBlock - 1) / kBlock,
+                                                                              1u << 22)));
+  if (a.grid.kind == GRID_INNER)
+    hipLaunchKernelGGL((forestImageKernel<DIM, MAP, GRID_INNER>), dim3(blocks), dim3(kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((forestImageKernel<DIM, MAP, GRID_UNIFORM>), dim3(blocks), dim3(kBlock), 0, st, a);
+}
+
+void launchForestImage(int dim, int mapKind, const ForestImageArgs& a, uint64_t pointsBound, hipStream_t st) {
+  switch (mapKind) {
+    case MAP_IDENTITY:
+      if (dim == 1) launchForestImage2<1, MAP_IDENTITY>(a, pointsBound, st);
+      else if (dim == 2) launchForestImage2<2, MAP_IDENTITY>(a, pointsBound, st);
+      else if (dim == 3) launchForestImage2<3, MAP_IDENTITY>(a, pointsBound, st);
+      else launchForestImage2<4, MAP_IDENTITY>(a, pointsBound, st);
+      break;
+    case MAP_HENON: launchForestImage2<2, MAP_HENON>(a, pointsBound, st); break;
+    case MAP_LORENZ: launchForestImage2<3, MAP_LORENZ>(a, pointsBound, st); break;
+    case MAP_HENON_HEILES: launchForestImage2<4, MAP_HENON_HEILES>(a, pointsBound, st); break;
+    case MAP_DUFFING: launchForestImage2<2, MAP_DUFFING>(a, pointsBound, st); break;
+    default: throw HipFail(B12X_E_UNSUPPORTED, "map kind");
+  }
+  HIPCHK(hipGetLastError());
+}
+
+ForestImageArgs imageArgs(b12x_forest_s& F) {
+  ForestImageArgs a{};
+  a.jobs = F.dJobs;
+  a.off = F.dOff;
+  a.end = F.dEnd;
+  a.key = F.arr[F.cur].key;
+  a.depth = F.arr[F.cur].depth;
+  a.job = F.arr[F.cur].job;
+  a.grid = F.grid;
+  return a;
+}
+
+// Upload the root leaves of newly added jobs (one root each, flags SD: setFlags(SD, 0) of a new
+// tree, subdivision.m:15) and their parameter tables.
+void admitWaiting(b12x_forest_s& F) {
+  if (F.waiting.empty()) return;
+  const uint32_t m = uint32_t(F.waiting.size());
+  ensureLeafCap(F, uint64_t(F.n) + m);
+  std::vector<uint64_t> k(m, keyOf(0, 0));
+  std::vector<uint8_t> d(m, 0), f(m, uint8_t(F_SD));
+  std::vector<uint32_t> j(F.waiting.begin(), F.waiting.end());
+  const ForestView& c = F.arr[F.cur];
+  HIPCHK(hipMemcpyAsync(c.key + F.n, k.data(), size_t(m) * 8, hipMemcpyHostToDevice, F.stream));
+  HIPCHK(hipMemcpyAsync(c.depth + F.n, d.data(), m, hipMemcpyHostToDevice, F.stream));
+  HIPCHK(hipMemcpyAsync(c.flags + F.n, f.data(), m, hipMemcpyHostToDevice, F.stream));
+  HIPCHK(hipMemcpyAsync(c.job + F.n, j.data(), size_t(m) * 4, hipMemcpyHostToDevice, F.stream));
+  for (uint32_t s : F.waiting)
+    HIPCHK(hipMemcpyAsync(F.dJobs + s, &F.jobs[s].dev, sizeof(JobDev), hipMemcpyHostToDevice, F.stream));
+  HIPCHK(hipStreamSynchronize(F.stream));
+  F.n += m;
+  for (uint32_t s : F.waiting) {
+    F.jobs[s].state = 1;
+    F.active.push_back(s);
+  }
+  F.waiting.clear();
+  fBoundary(F, F.n);
+}
+
+// Retire the jobs in R (slots): batched transition matrix, block-diagonal power iteration, copy
+// of the covers, then removal of their leaves from the forest.
+void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_t>& off,
+            const std::vector<uint32_t>& end) {
+  std::sort(R.begin(), R.end(), [&](uint32_t a, uint32_t b) { return off[a] < off[b]; });
+  const uint32_t nR = uint32_t(R.size());
+  std::vector<uint32_t> lo(nR + 1, 0), offR(nR);
+  for (uint32_t r = 0; r < nR; ++r) {
+    lo[r + 1] = lo[r] + (end[R[r]] - off[R[r]]);
+    offR[r] = off[R[r]];
+  }
+  const uint32_t nList = lo[nR];
+  const uint32_t k = F.grid.k;
+  hipStream_t st = F.stream;
+  // retiring mask
+  std::vector<uint8_t> mask(F.slotCap, 0);
+  for (uint32_t s : R) mask[s] = 1;
+  HIPCHK(hipMemcpyAsync(F.dRetiring, mask.data(), F.slotCap, hipMemcpyHostToDevice, st));
+  // source list (array order = job-major)
+  uint32_t* list = dalloc<uint32_t>(std::max<uint32_t>(F.n, nList));  // the scan writes at most F.n entries
+  ForestListOp lop{};
+  lop.job = F.arr[F.cur].job;
+  lop.retiring = F.dRetiring;
+  lop.out = list;
+  fScan(F, lop, F.n, 0);
+  fSync(F, 1);
+  if (F.hscal[0] != nList) throw HipFail(B12X_E_HIP, "forest leaf ranges disagree with the job ids");
+  // images of every test point of the retiring jobs
+  const uint64_t P = uint64_t(nList) * k;
+  uint32_t* col = dalloc<uint32_t>(P);
+  ForestImageArgs a = imageArgs(F);
+  a.srcList = list;
+  a.nSrc = nList;
+  a.mode = 1;
+  a.out = col;
+  launchForestImage(F.dim, F.mapKind, a, P, st);
+  // rows: per-box sort + run-length encoding (invalid targets dropped)
+  uint32_t* rowPtr = dalloc<uint32_t>(uint64_t(nList) + 1);
+  TmRowArgs ta{};
+  ta.col = col;
+  ta.nRows = nList;
+  ta.k = k;
+  ta.keepInvalid = 0;
+  ta.rowCount = rowPtr;
+  ta.anyInvalid = F.scal + 4;
+  ta.pass = 0;
+  tmRows(ta, st);
+  HIPCHK(hipMemsetAsync(rowPtr + nList, 0, 4, st));
+  const uint32_t nnz = scanArray(st, rowPtr, uint64_t(nList) + 1, F.totals, F.scal + 5, F.hscal + 5);
+  uint32_t* ecol = dalloc<uint32_t>(nnz);
+  uint32_t* ecnt = dalloc<uint32_t>(nnz);
+  uint32_t* erow = dalloc<uint32_t>(nnz);
+  ta.pass = 1;
+  ta.rowPtr = rowPtr;
+  ta.outCol = ecol;
+  ta.outCnt = ecnt;
+  tmRows(ta, st);
+  if (nList)
+    hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nList, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)rowPtr,
+                       uint64_t(nList), erow);
+  HIPCHK(hipGetLastError());
+  std::vector<uint32_t> hRowPtr(uint64_t(nList) + 1);
+  HIPCHK(hipMemcpyAsync(hRowPtr.data(), rowPtr, (uint64_t(nList) + 1) * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(col);
+  // column-major copy (stable radix sort by global column leaves rows ascending per column)
+  uint32_t* ccol = dalloc<uint32_t>(nnz);
+  uint32_t* crow = dalloc<uint32_t>(nnz);
+  uint32_t* ccnt = dalloc<uint32_t>(nnz);
+  if (nnz) {
+    HIPCHK(hipMemcpyAsync(ccol, ecol, uint64_t(nnz) * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(crow, erow, uint64_t(nnz) * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(ccnt, ecnt, uint64_t(nnz) * 4, hipMemcpyDeviceToDevice, st));
+  }
+  radixSort3(st, ccol, crow, ccnt, nnz, bitsFor(uint64_t(F.n) + 1), F.hscal + 6);
+  // local indices and columns as list positions
+  uint32_t* dLo = dalloc<uint32_t>(nR + 1);
+  uint32_t* dOffR = dalloc<uint32_t>(nR);
+  uint32_t* jobOfPos = dalloc<uint32_t>(nList);
+  HIPCHK(hipMemcpyAsync(dLo, lo.data(), (nR + 1) * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dOffR, offR.data(), nR * 4, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(jobOfPosKernel, dim3(nR), dim3(kBlock), 0, st, (const uint32_t*)dLo, nR, jobOfPos);
+  uint32_t* cLocRow = dalloc<uint32_t>(nnz);
+  uint32_t* cLocCol = dalloc<uint32_t>(nnz);
+  uint32_t* cColPos = dalloc<uint32_t>(nnz);
+  uint32_t* rLocRow = dalloc<uint32_t>(nnz);
+  uint32_t* rLocCol = dalloc<uint32_t>(nnz);
+  uint32_t* rColPos = dalloc<uint32_t>(nnz);
+  if (nnz) {
+    hipLaunchKernelGGL(forestLocalizeKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)crow,
+                       (const uint32_t*)ccol, uint64_t(nnz), (const uint32_t*)jobOfPos, (const uint32_t*)dLo,
+                       (const uint32_t*)dOffR, cLocRow, cLocCol, cColPos);
+    hipLaunchKernelGGL(forestLocalizeKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)erow,
+                       (const uint32_t*)ecol, uint64_t(nnz), (const uint32_t*)jobOfPos, (const uint32_t*)dLo,
+                       (const uint32_t*)dOffR, rLocRow, rLocCol, rColPos);
+  }
+  double* cval = dalloc<double>(nnz);
+  double* rval = dalloc<double>(nnz);
+  if (nnz) {
+    hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)ccnt,
+                       uint64_t(nnz), double(k), F.useAbs, cval);
+    hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)ecnt,
+                       uint64_t(nnz), double(k), F.useAbs, rval);
+  }
+  HIPCHK(hipGetLastError());
+  // CSC over list positions and the per-job entry ranges of the column-major order
+  uint32_t* colPtr = dalloc<uint32_t>(uint64_t(nList) + 1);
+  hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(nList) + 1, kBlock)), dim3(kBlock), 0, st,
+                     (const uint32_t*)cColPos, uint64_t(nnz), nList, colPtr);
+  std::vector<uint32_t> hColPtr(uint64_t(nList) + 1);
+  HIPCHK(hipMemcpyAsync(hColPtr.data(), colPtr, (uint64_t(nList) + 1) * 4, hipMemcpyDeviceToHost, st));
+  // block-diagonal power iteration (values as probabilities count/k even when useAbs is set)
+  double* pval = cval;
+  if (F.useAbs) {
+    pval = dalloc<double>(nnz);
+    if (nnz)
+      hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)ccnt,
+                         uint64_t(nnz), double(k), 0, pval);
+  }
+  double* ybuf = dalloc<double>(uint64_t(nList) * 3);
+  double* absCol = dalloc<double>(nList);
+  double* resCol = dalloc<double>(nList);
+  double* norm = dalloc<double>(uint64_t(nR) * 3);
+  double* resid = dalloc<double>(nR);
+  int* done = dalloc<int>(nR);
+  uint32_t* finalIt = dalloc<uint32_t>(nR);
+  BatchPiState* pst = dalloc<BatchPiState>(1);
+  BatchPiState init{1, 1, 0, 0};
+  if (F.piMaxIter <= 0) init.nDone = nR;
+  HIPCHK(hipMemcpyAsync(pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(done, 0, nR * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(finalIt, 0, nR * 4, st));
+  HIPCHK(hipMemsetAsync(norm, 0, uint64_t(nR) * 3 * 8, st));
+  const uint32_t nbStep = std::max<uint32_t>(1, ceilDiv(uint64_t(nList) * kPiLanes, kBlock));
+  const int batch = 16;
+  auto enqueue = [&]() {
+    for (int b = 0; b < batch; ++b) {
+      hipLaunchKernelGGL(batchPiStepKernel, dim3(nbStep), dim3(kBlock), 0, st, (const uint32_t*)colPtr,
+                         (const uint32_t*)crow, (const double*)pval, (const uint32_t*)jobOfPos, (const uint32_t*)dLo,
+                         nR, ybuf, nList, (const double*)norm, (const int*)done, absCol, resCol, pst);
+      hipLaunchKernelGGL(batchPiReduceKernel, dim3(nR), dim3(kBlock), 0, st, (const double*)absCol,
+                         (const double*)resCol, (const uint32_t*)dLo, nR, norm, resid, done, finalIt, pst,
+                         uint32_t(std::max(F.piMaxIter, 0)), F.piTol);
+    }
+  };
+  const char* ge = std::getenv("B12X_GRAPHS");
+  const bool useGraph = !(ge && ge[0] == '0');
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (useGraph && init.nDone < nR) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  BatchPiState hs{};
+  for (int64_t it = 0; init.nDone < nR && it <= int64_t(F.piMaxIter) + 2; it += batch) {
+    if (useGraph) HIPCHK(hipGraphLaunch(exec, st));
+    else enqueue();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hs.nDone >= nR) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  // results arena: covers, matrices (requested order), measures
+  auto A = std::make_shared<ResultArena>();
+  A->key = dalloc<uint64_t>(nList);
+  A->depth = dalloc<uint8_t>(nList);
+  A->flags = dalloc<uint8_t>(nList);
+  A->rows = dalloc<uint32_t>(nnz);
+  A->cols = dalloc<uint32_t>(nnz);
+  A->vals = dalloc<double>(nnz);
+  A->mu = dalloc<double>(nList);
+  A->bytes = uint64_t(nList) * (8 + 1 + 1 + 8) + uint64_t(nnz) * 16;
+  if (nList)
+    hipLaunchKernelGGL(batchPiFinishKernel, dim3(ceilDiv(nList, kBlock)), dim3(kBlock), 0, st, (const double*)ybuf,
+                       nList, (const uint32_t*)jobOfPos, (const uint32_t*)dLo, nR, (const double*)norm,
+                       (const uint32_t*)finalIt, A->mu);
+  HIPCHK(hipGetLastError());
+  if (nnz) {
+    HIPCHK(hipMemcpyAsync(A->rows, F.rowMajor ? rLocRow : cLocRow, uint64_t(nnz) * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(A->cols, F.rowMajor ? rLocCol : cLocCol, uint64_t(nnz) * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(A->vals, F.rowMajor ? rval : cval, uint64_t(nnz) * 8, hipMemcpyDeviceToDevice, st));
+  }
+  std::vector<uint32_t> hFinal(nR);
+  std::vector<double> hResid(nR);
+  HIPCHK(hipMemcpyAsync(hFinal.data(), finalIt, nR * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hResid.data(), resid, nR * 8, hipMemcpyDeviceToHost, st));
+  const ForestView& c = F.arr[F.cur];
+  for (uint32_t r = 0; r < nR; ++r) {
+    const uint32_t cnt = lo[r + 1] - lo[r];
+    HIPCHK(hipMemcpyAsync(A->key + lo[r], c.key + offR[r], uint64_t(cnt) * 8, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(A->depth + lo[r], c.depth + offR[r], cnt, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(A->flags + lo[r], c.flags + offR[r], cnt, hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  const double tNow = nowSeconds();
+  for (uint32_t r = 0; r < nR; ++r) {
+    Job& J = F.jobs[R[r]];
+    J.arena = A;
+    J.leafBase = lo[r];
+    J.nLeaves = lo[r + 1] - lo[r];
+    const uint64_t e0 = F.rowMajor ? hRowPtr[lo[r]] : hColPtr[lo[r]];
+    const uint64_t e1 = F.rowMajor ? hRowPtr[lo[r + 1]] : hColPtr[lo[r + 1]];
+    J.nnzBase = e0;
+    J.nnz = e1 - e0;
+    J.iters = int(hFinal[r]);
+    J.resid = hResid[r];
+    J.state = 2;
+    J.tDone = tNow;
+  }
+  dfree(list); dfree(rowPtr); dfree(ecol); dfree(ecnt); dfree(erow); dfree(ccol); dfree(crow); dfree(ccnt);
+  dfree(dLo); dfree(dOffR); dfree(jobOfPos); dfree(cLocRow); dfree(cLocCol); dfree(cColPos); dfree(rLocRow);
+  dfree(rLocCol); dfree(rColPos); dfree(cval); dfree(rval); dfree(colPtr); if (pval != cval) dfree(pval);
+  dfree(ybuf); dfree(absCol); dfree(resCol); dfree(norm); dfree(resid); dfree(done); dfree(finalIt); dfree(pst);
+  // drop the retired leaves
+  ForestKeepOp kop{};
+  kop.in = F.arr[F.cur];
+  kop.out = F.arr[1 - F.cur];
+  kop.hit = F.hit;
+  kop.retiring = F.dRetiring;
+  kop.mode = 1;
+  fScan(F, kop, F.n, 0);
+  F.cur = 1 - F.cur;
+  fSync(F, 1);
+  F.n = F.hscal[0];
+  std::vector<uint8_t> zero(F.slotCap, 0);
+  HIPCHK(hipMemcpyAsync(F.dRetiring, zero.data(), F.slotCap, hipMemcpyHostToDevice, st));
+  fBoundary(F, F.n);
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<uint32_t> still;
+  for (uint32_t s : F.active)
+    if (F.jobs[s].state == 1) still.push_back(s);
+  F.active.swap(still);
+}
+
+// One subdivision step for every active job, then retirement of the jobs that reached their depth.
+void forestStep(b12x_forest_s& F, uint32_t* nRetired) {
+  admitWaiting(F);
+  if (nRetired) *nRetired = 0;
+  if (F.active.empty()) return;
+  const uint32_t n = F.n;
+  ensureLeafCap(F, uint64_t(n) * 2);
+  hipStream_t st = F.stream;
+  // subdivide(SD)
+  ForestSubdivideOp sop{};
+  sop.in = F.arr[F.cur];
+  sop.out = F.arr[1 - F.cur];
+  sop.mask = F_SD;
+  fScan(F, sop, n, 0);
+  F.cur = 1 - F.cur;
+  fBoundary(F, 2 * n, F.scal + 0);
+  // setFlagsByIterators(OverLeaves, HIT): images of every test point of every job
+  ForestImageArgs a = imageArgs(F);
+  a.nSrc = 2 * n;
+  a.nSrcDev = F.scal + 0;
+  a.mode = 0;
+  a.hit = F.hit;
+  launchForestImage(F.dim, F.mapKind, a, uint64_t(2 * n) * F.grid.k, st);
+  // remove(HIT) + unsetFlags(HIT)
+  ForestKeepOp kop{};
+  kop.in = F.arr[F.cur];
+  kop.out = F.arr[1 - F.cur];
+  kop.hit = F.hit;
+  kop.retiring = F.dRetiring;
+  kop.mode = 0;
+  fScan(F, kop, 2 * n, 2, F.scal + 0);
+  F.cur = 1 - F.cur;
+  fBoundary(F, 2 * n, F.scal + 2);
+  fSync(F, 3);
+  F.n = F.hscal[2];
+  ++F.steps;
+  std::vector<uint32_t> off(F.slotCap), end(F.slotCap);
+  HIPCHK(hipMemcpyAsync(off.data(), F.dOff, size_t(F.slotCap) * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(end.data(), F.dEnd, size_t(F.slotCap) * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // a job whose cover emptied gets its root back with no flags (ImplicitBoxTree.hpp:2363)
+  std::vector<uint32_t> empty;
+  for (uint32_t s : F.active)
+    if (end[s] <= off[s]) empty.push_back(s);
+  if (!empty.empty()) {
+    ensureLeafCap(F, uint64_t(F.n) + empty.size());
+    const ForestView& c = F.arr[F.cur];
+    for (uint32_t s : empty) {
+      const uint64_t rk = keyOf(0, 0);
+      const uint8_t z = 0;
+      HIPCHK(hipMemcpyAsync(c.key + F.n, &rk, 8, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(c.depth + F.n, &z, 1, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(c.flags + F.n, &z, 1, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(c.job + F.n, &s, 4, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      off[s] = F.n;
+      end[s] = F.n + 1;
+      ++F.n;
+    }
+    fBoundary(F, F.n);
+  }
+  std::vector<uint32_t> R;
+  for (uint32_t s : F.active) {
+    Job& J = F.jobs[s];
+    J.level += 1;
+    if (J.level >= J.target) R.push_back(s);
+  }
+  if (!R.empty()) retire(F, R, off, end);
+  if (nRetired) *nRetired = uint32_t(R.size());
+}
+
+Job& jobAt(b12x_forest_s& F, uint32_t id) {
+  if (id >= F.jobs.size()) throw HipFail(B12X_E_ARG, "no such job");
+  return F.jobs[id];
+}
+
+}  // namespace
+
+extern "C" {
+
+int b12x_forest_create(int dim, int gk, const uint64_t* gp, int ngp, int mk, int useAbs, int rowMajor, int piMaxIter,
+                       double piTol, b12x_forest* out) {
+  return guarded([&]() -> int {
+    if (!out || dim < 1 || dim > kMaxDim) return setErr(B12X_E_ARG, "bad argument");
+    if (int e = ensureDevice()) return e;
+    auto* F = new b12x_forest_s();
+    try {
+      F->dim = dim;
+      F->mapKind = mk;
+      std::vector<double> tab;
+      buildGrid(dim, gk, gp, ngp, F->grid, tab);
+      if (F->grid.k > 64) throw HipFail(B12X_E_UNSUPPORTED, "forests support up to 64 points per box");
+      if (!tab.empty()) {
+        F->dTable = dalloc<double>(tab.size());
+        HIPCHK(hipMemcpy(F->dTable, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+        F->grid.table = F->dTable;
+      }
+      // validate the map kind against the dimension
+      MapParams probe;
+      std::vector<ForcingStep> ft;
+      buildMapParams(dim, mk, nullptr, 0, probe, ft);
+      F->useAbs = useAbs;
+      F->rowMajor = rowMajor;
+      F->piMaxIter = piMaxIter;
+      F->piTol = piTol;
+      HIPCHK(hipStreamCreate(&F->stream));
+      F->scal = dalloc<uint32_t>(kForestScal);
+      HIPCHK(hipHostMalloc((void**)&F->hscal, kForestScal * sizeof(uint32_t), hipHostMallocDefault));
+      ensureSlotCap(*F, 64);
+      ensureLeafCap(*F, 4096);
+    } catch (...) {
+      delete F;
+      throw;
+    }
+    *out = F;
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_destroy(b12x_forest f) {
+  delete f;
+  return B12X_OK;
+}
+
+int b12x_forest_add(b12x_forest f, const double* center, const double* radius, const double* mp, int nmp,
+                    uint8_t target, uint32_t* job) {
+  return guarded([&]() -> int {
+    if (!f || !center || !radius || !job) return setErr(B12X_E_ARG, "null argument");
+    if (target < 1 || target > kMaxDepth) return setErr(B12X_E_ARG, "target depth must be 1..64");
+    b12x_forest_s& F = *f;
+    Job J;
+    J.target = target;
+    if (!buildDomain(F.dim, center, radius, nullptr, 0, J.dev.dom)) return setErr(B12X_E_ARG, "domain");
+    std::vector<ForcingStep> tab;
+    buildMapParams(F.dim, F.mapKind, mp, nmp, J.dev.map, tab);
+    if (!tab.empty()) {
+      J.dForcing = dalloc<ForcingStep>(tab.size());
+      HIPCHK(hipMemcpy(J.dForcing, tab.data(), tab.size() * sizeof(ForcingStep), hipMemcpyHostToDevice));
+      J.dev.map.forcing = J.dForcing;
+    }
+    const uint32_t slot = uint32_t(F.jobs.size());
+    ensureSlotCap(F, slot + 1);
+    F.jobs.push_back(std::move(J));
+    F.waiting.push_back(slot);
+    *job = slot;
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_step(b12x_forest f, uint32_t* nActive, uint32_t* nRetired) {
+  return guarded([&]() -> int {
+    if (!f) return setErr(B12X_E_ARG, "null forest");
+    forestStep(*f, nRetired);
+    if (nActive) *nActive = uint32_t(f->active.size() + f->waiting.size());
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_run(b12x_forest f) {
+  return guarded([&]() -> int {
+    if (!f) return setErr(B12X_E_ARG, "null forest");
+    while (!f->active.empty() || !f->waiting.empty()) forestStep(*f, nullptr);
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_job_info(b12x_forest f, uint32_t id, int* state, uint32_t* nLeaves, uint64_t* nnz, int* iters,
+                         double* resid) {
+  return guarded([&]() -> int {
+    if (!f) return setErr(B12X_E_ARG, "null forest");
+    Job& J = jobAt(*f, id);
+    if (state) *state = J.state;
+    if (nLeaves) *nLeaves = J.nLeaves;
+    if (nnz) *nnz = J.nnz;
+    if (iters) *iters = J.iters;
+    if (resid) *resid = J.resid;
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_job_cover(b12x_forest f, uint32_t id, uint64_t* bits, uint8_t* depth, uint8_t* flags) {
+  return guarded([&]() -> int {
+    if (!f) return setErr(B12X_E_ARG, "null forest");
+    Job& J = jobAt(*f, id);
+    if (J.state != 2) return setErr(B12X_E_ARG, "job not finished");
+    std::vector<uint64_t> k(J.nLeaves);
+    std::vector<uint8_t> d(J.nLeaves);
+    HIPCHK(hipMemcpy(k.data(), J.arena->key + J.leafBase, uint64_t(J.nLeaves) * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(d.data(), J.arena->depth + J.leafBase, J.nLeaves, hipMemcpyDeviceToHost));
+    if (flags) HIPCHK(hipMemcpy(flags, J.arena->flags + J.leafBase, J.nLeaves, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < J.nLeaves; ++i) {
+      if (bits) bits[i] = bitsOf(k[i], d[i]);
+      if (depth) depth[i] = d[i];
+    }
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_job_matrix(b12x_forest f, uint32_t id, uint32_t* rows, uint32_t* cols, double* vals) {
+  return guarded([&]() -> int {
+    if (!f) return setErr(B12X_E_ARG, "null forest");
+    Job& J = jobAt(*f, id);
+    if (J.state != 2) return setErr(B12X_E_ARG, "job not finished");
+    if (J.nnz) {
+      if (rows) HIPCHK(hipMemcpy(rows, J.arena->rows + J.nnzBase, J.nnz * 4, hipMemcpyDeviceToHost));
+      if (cols) HIPCHK(hipMemcpy(cols, J.arena->cols + J.nnzBase, J.nnz * 4, hipMemcpyDeviceToHost));
+      if (vals) HIPCHK(hipMemcpy(vals, J.arena->vals + J.nnzBase, J.nnz * 8, hipMemcpyDeviceToHost));
+    }
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_job_measure(b12x_forest f, uint32_t id, double* mu) {
+  return guarded([&]() -> int {
+    if (!f || !mu) return setErr(B12X_E_ARG, "null argument");
+    Job& J = jobAt(*f, id);
+    if (J.state != 2) return setErr(B12X_E_ARG, "job not finished");
+    HIPCHK(hipMemcpy(mu, J.arena->mu + J.leafBase, uint64_t(J.nLeaves) * 8, hipMemcpyDeviceToHost));
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_job_release(b12x_forest f, uint32_t id) {
+  return guarded([&]() -> int {
+    if (!f) return setErr(B12X_E_ARG, "null forest");
+    Job& J = jobAt(*f, id);
+    if (J.state == 2) {
+      J.arena.reset();
+      J.state = 3;
+    }
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_footprint(b12x_forest f, uint64_t* bytes) {
+  return guarded([&]() -> int {
+    if (!f || !bytes) return setErr(B12X_E_ARG, "null argument");
+    const b12x_forest_s& F = *f;
+    uint64_t b = uint64_t(F.cap) * 2 * (8 + 1 + 1 + 4) + F.cap + uint64_t(F.slotCap) * (sizeof(JobDev) + 9);
+    std::vector<const ResultArena*> seen;
+    for (const Job& J : F.jobs)
+      if (J.arena && std::find(seen.begin(), seen.end(), J.arena.get()) == seen.end()) {
+        seen.push_back(J.arena.get());
+        b += J.arena->bytes;
+      }
+    *bytes = b;
+    return B12X_OK;
+  });
+}
+
+// Continuous-batching request loop. Requests (sorted by arrival time, seconds from the start)
+// go to forest forestOf[i]; a request is admitted once it has arrived and the forests'
+// footprint plus reservePerJob stays within budgetBytes. Every pass runs one merged step of
+// each forest with work; completion time = end of the step in which the job retired.
+int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const double* arrival, const int* forestOf,
+                   const double* centers, const double* radii, int stride, const double* mapParams, int nmp,
+                   const uint8_t* depths, double budgetBytes, double reservePerJob, double* latency,
+                   double* completion, uint32_t* jobIds) {
+  return guarded([&]() -> int {
+    if (!forests || nForests <= 0 ||
+        (nReq && (!arrival || !forestOf || !centers || !radii || !depths || !latency || !completion)))
+      return setErr(B12X_E_ARG, "null argument");
+    for (uint32_t i = 0; i < nReq; ++i) completion[i] = latency[i] = -1.0;
+    for (uint32_t i = 1; i < nReq; ++i)
+      if (arrival[i] < arrival[i - 1]) return setErr(B12X_E_ARG, "requests must be sorted by arrival");
+    std::vector<int> reqOfJob;
+    std::vector<std::vector<int>> reqOf(nForests);
+    std::vector<double> admitted(nReq, -1.0);
+    uint32_t next = 0, completed = 0, inFlight = 0;
+    const double t0 = nowSeconds();
+    while (completed < nReq) {
+      double now = nowSeconds() - t0;
+      // admission by arrival and HBM footprint
+      while (next < nReq && arrival[next] <= now) {
+        uint64_t used = 0;
+        for (int q = 0; q < nForests; ++q) {
+          uint64_t b = 0;
+          b12x_forest_footprint(forests[q], &b);
+          used += b;
+        }
+        if (inFlight > 0 && double(used) + reservePerJob * (inFlight + 1) > budgetBytes) break;
+        const int q = forestOf[next];
+        if (q < 0 || q >= nForests) return setErr(B12X_E_ARG, "bad forest index");
+        uint32_t id = 0;
+        int rc = b12x_forest_add(forests[q], centers + size_t(next) * stride, radii + size_t(next) * stride,
+                                 mapParams ? mapParams + size_t(next) * nmp : nullptr, mapParams ? nmp : 0,
+                                 depths[next], &id);
+        if (rc) return rc;
+        if (reqOf[q].size() <= id) reqOf[q].resize(id + 1, -1);
+        reqOf[q][id] = int(next);
+        if (jobIds) jobIds[next] = id;
+        admitted[next] = now;
+        ++next;
+        ++inFlight;
+      }
+      bool work = false;
+      for (int q = 0; q < nForests; ++q) {
+        b12x_forest_s& F = *forests[q];
+        if (F.active.empty() && F.waiting.empty()) continue;
+        work = true;
+        forestStep(F, nullptr);
+      }
+      const double tEnd = nowSeconds() - t0;
+      for (int q = 0; q < nForests; ++q) {
+        b12x_forest_s& F = *forests[q];
+        for (uint32_t id = 0; id < reqOf[q].size(); ++id) {
+          const int r = reqOf[q][id];
+          if (r < 0 || completion[r] >= 0.0 || F.jobs[id].state < 2) continue;
+          completion[r] = tEnd;
+          latency[r] = tEnd - arrival[r];
+          ++completed;
+          --inFlight;
+        }
+      }
+      if (!work && next < nReq) {
+        const double wait = arrival[next] - (nowSeconds() - t0);
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+      }
+    }
+    return B12X_OK;
+  });
+}
+
+}  // extern "C"

@@ -83,7 +83,7 @@ struct OpBase {
 __device__ inline uint32_t elemCount(uint32_t n, const uint32_t* nDev) { return nDev ? *nDev : n; }
 
 template <class Op>
-__global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, const uint32_t* nDev,
+static __global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, const uint32_t* nDev,
                                                           uint32_t* blockTotals, uint32_t* blockStats) {
   n = elemCount(n, nDev);
   if (uint64_t(blockIdx.x) * kTile >= n && blockIdx.x > 0) return;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void scanCountKernel(Op op, uint32_t n, con
 // Single-block exclusive scan of the block totals in place (m of them, or ceil(*nDev / kTile));
 // writes the grand total to *total and, when blockStats is set, the sum of the per-block
 // statistics to *statTotal.
-__global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t m, const uint32_t* nDev,
+static __global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t m, const uint32_t* nDev,
                                                            uint32_t* total, const uint32_t* blockStats,
                                                            uint32_t* statTotal) {
   if (nDev) m = uint32_t((uint64_t(*nDev) + kTile - 1) / kTile);
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void scanTotalsKernel(uint32_t* v, uint32_t
 }
 
 template <class Op>
-__global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32_t n, const uint32_t* nDev,
+static __global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32_t n, const uint32_t* nDev,
                                                             const uint32_t* blockOffsets) {
   n = elemCount(n, nDev);
   if (uint64_t(blockIdx.x) * kTile >= n) return;
@@ -259,7 +259,7 @@ __device__ inline bool isUnsubdividableAt(const TreeView& t, uint32_t n, uint32_
   return shr(pb, diff) != shr(bn, -diff);
 }
 
-__global__ __launch_bounds__(kBlock) void unsubMarkKernel(UnsubMarkKernelArgs a) {
+static __global__ __launch_bounds__(kBlock) void unsubMarkKernel(UnsubMarkKernelArgs a) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t u = 0;
   if (i < a.n) {
@@ -311,7 +311,7 @@ __device__ inline uint8_t flagApply(int op, uint8_t f, uint8_t a, uint8_t b) {
 
 // sel: per-leaf selector bytes (stencil or hit bytes), nullptr = all leaves. With clearSel the
 // selector bytes are zeroed after use (hit bytes are reused by the next image step).
-__global__ __launch_bounds__(kBlock) void flagsKernel(uint8_t* flags, uint32_t n, uint8_t* sel, int clearSel, int op,
+static __global__ __launch_bounds__(kBlock) void flagsKernel(uint8_t* flags, uint32_t n, uint8_t* sel, int clearSel, int op,
                                                       uint8_t a, uint8_t b, uint32_t* counter) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t c = 0;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void flagsKernel(uint8_t* flags, uint32_t n
 
 // AND and OR of all flags (depth-0 forms of countIfFlags, ImplicitBoxTree.hpp:1105-1114) and the
 // maximum depth (ImplicitBoxTree.hpp:1264-1271).
-__global__ __launch_bounds__(kBlock) void flagsReduceKernel(const uint8_t* flags, const uint8_t* depth, uint32_t n,
+static __global__ __launch_bounds__(kBlock) void flagsReduceKernel(const uint8_t* flags, const uint8_t* depth, uint32_t n,
                                                             uint32_t* out /*[and, or, maxDepth]*/) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t fa = 0xFF, fo = 0, md = 0;
@@ -402,7 +402,7 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
 }
 
 template <int DIM, int MAP, int GRID>
-__global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
+static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
   const uint64_t nPoints = a.nBoxesDev ? uint64_t(*a.nBoxesDev) * a.grid.k : a.nPoints;
   const uint32_t n = a.nBoxesDev ? *a.nBoxesDev : a.n;
   if (a.pointBase + nPoints <= 0xFFFFFFFFull && uint64_t(gridDim.x) * kBlock <= 0xFFFFFFFFull) {
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
 
 // Keys of arbitrary points (searchByPoints, ImplicitBoxTree.hpp:2588-2620)
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void pointsSearchKernel(Domain dom, const double* pts, uint64_t m,
+static __global__ __launch_bounds__(kBlock) void pointsSearchKernel(Domain dom, const double* pts, uint64_t m,
                                                              const uint64_t* key, const uint8_t* depth, uint32_t n,
                                                              int depth0, uint32_t* out) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -459,7 +459,7 @@ struct TmRowArgs {
 };
 
 template <int S>
-__global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
+static __global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
   constexpr int SEGS = kWave / S;
   const int lane = threadIdx.x & (kWave - 1);
   const int seg = lane / S, pos = lane % S;
@@ -490,11 +490,14 @@ __global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
     const int next = aboveMask ? __builtin_ctzll(aboveMask) : (seg * S + S);
     uint32_t cnt = uint32_t(next - lane);
     bool entry = start && realRow;
+    bool invalidHere = false;
     if (v == 0xFFFFFFFFu) {
       cnt -= uint32_t(S) - a.k;  // padding lanes sort last, after the invalid targets
-      entry = entry && cnt > 0 && a.keepInvalid;
-      if (start && realRow && cnt > 0 && a.pass == 0) atomicOr(a.anyInvalid, 1u);
+      invalidHere = entry && cnt > 0;
+      entry = invalidHere && a.keepInvalid;
     }
+    // one atomic per wave that saw an invalid target (same-address atomics serialise chip-wide)
+    if (a.pass == 0 && __ballot(invalidHere) && lane == 0) atomicOr(a.anyInvalid, 1u);
     const uint64_t entryMask = __ballot(entry) & segMask;
     if (a.pass == 0) {
       if (pos == 0 && realRow) a.rowCount[row] = uint32_t(__popcll(entryMask));
@@ -508,14 +511,14 @@ __global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
 }
 
 // Expand CSR row pointers into the COO row index per entry.
-__global__ __launch_bounds__(kBlock) void expandRowsKernel(const uint32_t* rowPtr, uint64_t nRows, uint32_t* rows) {
+static __global__ __launch_bounds__(kBlock) void expandRowsKernel(const uint32_t* rowPtr, uint64_t nRows, uint32_t* rows) {
   const uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (r >= nRows) return;
   for (uint32_t e = rowPtr[r]; e < rowPtr[r + 1]; ++e) rows[e] = uint32_t(r);
 }
 
 // values = count / k (or count) as double (ImplicitBoxTree.hpp:3145-3158)
-__global__ __launch_bounds__(kBlock) void valuesKernel(const uint32_t* cnt, uint64_t nnz, double k, int useAbs,
+static __global__ __launch_bounds__(kBlock) void valuesKernel(const uint32_t* cnt, uint64_t nnz, double k, int useAbs,
                                                        double* val) {
   const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (e >= nnz) return;
@@ -531,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void valuesKernel(const uint32_t* cnt, uint
 constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kBlock * kRadixItems;
 
-__global__ __launch_bounds__(kBlock) void radixHistKernel(const uint32_t* keys, uint64_t n, int shift,
+static __global__ __launch_bounds__(kBlock) void radixHistKernel(const uint32_t* keys, uint64_t n, int shift,
                                                           uint32_t* hist, uint32_t nBlocks) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void radixHistKernel(const uint32_t* keys, 
   hist[uint64_t(threadIdx.x) * nBlocks + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kBlock) void radixScatterKernel(const uint32_t* keysIn, const uint32_t* p0In,
+static __global__ __launch_bounds__(kBlock) void radixScatterKernel(const uint32_t* keysIn, const uint32_t* p0In,
                                                              const uint32_t* p1In, uint64_t n, int shift,
                                                              const uint32_t* offsets, uint32_t nBlocks,
                                                              uint32_t* keysOut, uint32_t* p0Out, uint32_t* p1Out) {
@@ -599,7 +602,7 @@ struct ScanArrayOp : OpBase {
   __device__ void emit(uint32_t, uint32_t, uint32_t) const {}
 };
 
-__global__ __launch_bounds__(kBlock) void scanArrayScatterKernel(uint32_t* v, uint32_t n, const uint32_t* blockOffsets) {
+static __global__ __launch_bounds__(kBlock) void scanArrayScatterKernel(uint32_t* v, uint32_t n, const uint32_t* blockOffsets) {
   const uint64_t base = uint64_t(blockIdx.x) * kTile + uint64_t(threadIdx.x) * kItems;
   uint32_t c[kItems];
   uint32_t s = 0;
@@ -621,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void scanArrayScatterKernel(uint32_t* v, ui
 
 // offsets[v] = number of entries with idx < v, for sorted idx (compressIndexVector,
 // CooMatrix.hpp:14-56), v = 0..dim
-__global__ __launch_bounds__(kBlock) void compressKernel(const uint32_t* idx, uint64_t nnz, uint32_t dim,
+static __global__ __launch_bounds__(kBlock) void compressKernel(const uint32_t* idx, uint64_t nnz, uint32_t dim,
                                                          uint32_t* out) {
   const uint64_t v = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (v > dim) return;
@@ -674,7 +677,7 @@ struct PiState {
 // lane of the group adds the products in ascending-row order (the oracle's order) via shuffles.
 constexpr int kPiLanes = 8;
 
-__global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restrict__ colPtr,
+static __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restrict__ colPtr,
                                                        const uint32_t* __restrict__ rowIdx,
                                                        const double* __restrict__ val, const double* __restrict__ mu0,
                                                        double* y, uint32_t n, double* partAbs, double* partRes,
@@ -733,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restric
 
 // Sums the block partials of piStepKernel in block order (deterministic) and decides whether
 // iteration t-1 ends the loop. One block; runs after every piStepKernel.
-__global__ __launch_bounds__(kBlock) void piReduceKernel(const double* partAbs, const double* partRes, uint32_t nb,
+static __global__ __launch_bounds__(kBlock) void piReduceKernel(const double* partAbs, const double* partRes, uint32_t nb,
                                                          PiState* st, uint32_t maxIter, double tol) {
   if (st->done) return;
   const uint32_t t = st->t;
@@ -793,7 +796,7 @@ __device__ inline double blockSumF64Big(double v) {
   return t;
 }
 
-__global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t* __restrict__ colPtr,
+static __global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t* __restrict__ colPtr,
                                                           const uint32_t* __restrict__ rowIdx,
                                                           const double* __restrict__ val,
                                                           const double* __restrict__ mu0, double* y, uint32_t n,
@@ -882,7 +885,7 @@ __global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t* __rest
 }
 
 // mu = y_final / ||y_final||_1 (or mu0 when no iteration ran)
-__global__ __launch_bounds__(kBlock) void piFinishKernel(const double* y, const double* mu0, uint32_t n,
+static __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* y, const double* mu0, uint32_t n,
                                                          const PiState* st, double* mu) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;

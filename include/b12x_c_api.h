@@ -159,6 +159,45 @@ int b12x_compress_index_vector(const uint32_t* indices, uint64_t n, uint32_t dim
  * reference computes this step in MATLAB (subdivisionByNim.m:30-59); there is no C++ API. */
 int b12x_power_iteration(b12x_coo m, int max_iter, double tol, int mu_init, double* mu, int* iters, double* resid);
 
+/* --- batched and served forms (configs 4 and 5) --------------------------------------------
+ * A forest runs the subdivision algorithm (subdivision.m:15-25) for many jobs at once: jobs of
+ * one (grid, map kind) share every launch, each job with its own domain, map parameters and
+ * target depth. A job whose depth is reached retires: its transition matrix
+ * (transitionMatrixForLeaves, ImplicitBoxTree.hpp:3178-3193; invalid targets dropped) and its
+ * invariant measure (power iteration with pi_max_iter / pi_tol as in b12x_power_iteration) are
+ * computed for all jobs retiring in the same step in shared launches. Results do not depend on
+ * which jobs share a launch. These entry points have no counterpart in the reference (B12 runs
+ * one tree per call). */
+typedef struct b12x_forest_s* b12x_forest;
+int b12x_forest_create(int dim, int grid_kind, const uint64_t* grid_params, int n_grid_params, int map_kind,
+                       int use_abs, int row_major, int pi_max_iter, double pi_tol, b12x_forest* out);
+int b12x_forest_destroy(b12x_forest f);
+/* queue a job (root box flagged SD); it joins the next step */
+int b12x_forest_add(b12x_forest f, const double* center, const double* radius, const double* map_params,
+                    int n_map_params, uint8_t target_depth, uint32_t* job);
+/* one merged subdivision step for every active job, then retirement of finished jobs */
+int b12x_forest_step(b12x_forest f, uint32_t* n_active, uint32_t* n_retired);
+int b12x_forest_run(b12x_forest f);
+/* state: 0 queued, 1 active, 2 done, 3 released */
+int b12x_forest_job_info(b12x_forest f, uint32_t job, int* state, uint32_t* n_leaves, uint64_t* nnz, int* iters,
+                         double* resid);
+int b12x_forest_job_cover(b12x_forest f, uint32_t job, uint64_t* bits, uint8_t* depth, uint8_t* flags);
+int b12x_forest_job_matrix(b12x_forest f, uint32_t job, uint32_t* rows, uint32_t* cols, double* vals);
+int b12x_forest_job_measure(b12x_forest f, uint32_t job, double* mu);
+int b12x_forest_job_release(b12x_forest f, uint32_t job);
+/* HBM bytes held by the forest's leaf arrays, tables and unreleased results */
+int b12x_forest_footprint(b12x_forest f, uint64_t* bytes);
+/* Continuous-batching request loop over n_forests forests: request i arrives arrival[i] seconds
+ * after the start (sorted), goes to forests[forest_of[i]] with centre/radius (stride doubles
+ * apart), map_params (n_map_params each, may be NULL) and depth; it is admitted once it has
+ * arrived and the forests' footprint plus reserve_per_job per job in flight fits budget_bytes.
+ * latency[i] = completion[i] - arrival[i] (seconds); job_ids (may be NULL) gets each request's
+ * job id in its forest. */
+int b12x_serve_run(b12x_forest* forests, int n_forests, uint32_t n_req, const double* arrival, const int* forest_of,
+                   const double* centers, const double* radii, int stride, const double* map_params,
+                   int n_map_params, const uint8_t* depths, double budget_bytes, double reserve_per_job,
+                   double* latency, double* completion, uint32_t* job_ids);
+
 #ifdef __cplusplus
 }
 #endif

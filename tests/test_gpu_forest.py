@@ -1,0 +1,138 @@
+"""GPU tests of the batched and served forms: every job's cover and transition matrix equal the
+single-tree results bit for bit (and the oracle's), its invariant measure agrees within 1e-12,
+and a job's results are identical alone and in any mix."""
+import numpy as np
+import pytest
+
+import b12_amd as gx
+from helpers import same_tree
+from oracle_api import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+HENON_C, HENON_R = [0.0, 0.0], [1.5, 0.5]
+
+
+def oracle_job(center, radius, grid, mp, depth, pi_iter=200, tol=-1.0, row_major=False):
+    o = OracleTree(center, radius)
+    o.setFlags(gx.SD, 0)
+    o.subdivision_steps(grid, mp, depth)
+    m = o.transitionMatrixForLeaves(grid, mp, row_major=row_major)
+    mu, it, res = m.power_iteration(pi_iter, tol)
+    return o.download(), m.download(), mu, it
+
+
+def assert_job_matches(forest, job, ref, rtol=1e-12):
+    cover, mat, mu, it = ref
+    assert same_tree(forest.cover(job), cover)
+    for a, b in zip(forest.matrix(job), mat):
+        assert np.array_equal(a, b)
+    got = forest.measure(job)
+    assert np.allclose(got, mu, rtol=rtol, atol=0.0)
+    return got
+
+
+def test_batched_henon_sweep_matches_single_jobs():
+    print("[forest] henon sweep", flush=True)
+    rng = np.random.default_rng(0)
+    grid = gx.InnerGrid(4)
+    F = gx.Forest(2, grid, 1, pi_max_iter=200)
+    jobs = []
+    for _ in range(10):
+        a = float(rng.uniform(1.0, 1.4))
+        d = int(rng.integers(6, 13))
+        jobs.append((F.add(HENON_C, HENON_R, gx.Henon(a, 0.3, 1), d), a, d))
+    F.run()
+    for j, a, d in jobs:
+        info = F.info(j)
+        assert info["state"] == 2 and info["iters"] == 200
+        assert_job_matches(F, j, oracle_job(HENON_C, HENON_R, grid, gx.Henon(a, 0.3, 1), d))
+
+
+def test_job_identical_alone_and_in_the_mix():
+    grid = gx.InnerGrid(4)
+    mp = gx.Henon(1.3, 0.3, 1)
+    alone = gx.Forest(2, grid, 1, pi_max_iter=150)
+    ja = alone.add(HENON_C, HENON_R, mp, 11)
+    alone.run()
+    mixed = gx.Forest(2, grid, 1, pi_max_iter=150)
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        mixed.add(HENON_C, HENON_R, gx.Henon(float(rng.uniform(1.0, 1.4)), 0.3, 1), int(rng.integers(4, 14)))
+    jm = mixed.add(HENON_C, HENON_R, mp, 11)
+    for _ in range(3):
+        mixed.step()
+    for _ in range(4):  # jobs joining while others are in flight
+        mixed.add(HENON_C, HENON_R, gx.Henon(float(rng.uniform(1.0, 1.4)), 0.3, 1), int(rng.integers(3, 9)))
+    mixed.run()
+    assert same_tree(alone.cover(ja), mixed.cover(jm))
+    for a, b in zip(alone.matrix(ja), mixed.matrix(jm)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(alone.measure(ja), mixed.measure(jm))
+    assert alone.info(ja)["iters"] == mixed.info(jm)["iters"] == 150
+
+
+def test_lorenz_forest_tolerance_and_row_major():
+    print("[forest] lorenz", flush=True)
+    grid = gx.InnerGrid(3)
+    c, r = [0.0, 0.0, 27.0], [30.0, 30.0, 30.0]
+    for row_major in (False, True):
+        F = gx.Forest(3, grid, 2, row_major=row_major, pi_max_iter=100000, pi_tol=1e-12)
+        jobs = [(F.add(c, r, gx.Lorenz(rho=rho), d), rho, d) for rho, d in [(28.0, 10), (24.5, 9), (31.0, 11)]]
+        F.run()
+        for j, rho, d in jobs:
+            ref = oracle_job(c, r, grid, gx.Lorenz(rho=rho), d, 100000, 1e-12, row_major)
+            cover, mat, mu, it = ref
+            assert same_tree(F.cover(j), cover)
+            for a, b in zip(F.matrix(j), mat):
+                assert np.array_equal(a, b)
+            info = F.info(j)
+            assert abs(info["iters"] - it) <= 1 and info["resid"] <= 1e-12
+            assert np.allclose(F.measure(j), mu, rtol=1e-10, atol=1e-16)
+
+
+def test_duffing_and_uniform_grid_forests():
+    grid = gx.InnerGrid(4)
+    F = gx.Forest(2, grid, 4, pi_max_iter=50)
+    j = F.add([0.0, 0.0], [2.0, 2.0], gx.Duffing(), 8)
+    F.run()
+    assert_job_matches(F, j, oracle_job([0.0, 0.0], [2.0, 2.0], grid, gx.Duffing(), 8, 50))
+    ug = gx.UniformGrid(16, 7)
+    G = gx.Forest(2, ug, 1, pi_max_iter=30)
+    jobs = [G.add(HENON_C, HENON_R, gx.Henon(a, 0.3, 1), d) for a, d in [(1.4, 7), (1.2, 8)]]
+    G.run()
+    for jj, (a, d) in zip(jobs, [(1.4, 7), (1.2, 8)]):
+        assert_job_matches(G, jj, oracle_job(HENON_C, HENON_R, ug, gx.Henon(a, 0.3, 1), d, 30))
+
+
+def test_serve_run_results_match_alone_runs():
+    print("[serve] small stream", flush=True)
+    rng = np.random.default_rng(2)
+    lor = gx.Forest(3, gx.InnerGrid(3), 2, pi_max_iter=100000, pi_tol=1e-12)
+    duf = gx.Forest(2, gx.InnerGrid(4), 4, pi_max_iter=100000, pi_tol=1e-12)
+    n = 12
+    arrival = np.cumsum(rng.exponential(1 / 200.0, n))
+    kinds = (rng.random(n) < 0.6).astype(int)  # 1 = Lorenz
+    forest_of, centers, radii, params, depths = [], [], [], [], []
+    for k in kinds:
+        if k:
+            forest_of.append(0); centers.append([0.0, 0.0, 27.0]); radii.append([30.0, 30.0, 30.0])
+            params.append(gx.Lorenz(rho=float(rng.uniform(24, 32))).params); depths.append(int(rng.integers(6, 11)))
+        else:
+            forest_of.append(1); centers.append([0.0, 0.0]); radii.append([2.0, 2.0])
+            params.append(gx.Duffing().params); depths.append(int(rng.integers(5, 9)))
+    lat, comp, ids = gx.serve_run([lor, duf], arrival, forest_of, centers, radii, params, depths)
+    assert np.all(lat > 0) and np.all(comp >= arrival)
+    for i in range(n):
+        F = lor if forest_of[i] == 0 else duf
+        dim = 3 if forest_of[i] == 0 else 2
+        kind = 2 if forest_of[i] == 0 else 4
+        grid = gx.InnerGrid(3) if dim == 3 else gx.InnerGrid(4)
+        alone = gx.Forest(dim, grid, kind, pi_max_iter=100000, pi_tol=1e-12)
+        mp = gx.Lorenz(*params[i]) if dim == 3 else gx.Duffing(*params[i])
+        ja = alone.add(centers[i], radii[i], mp, depths[i])
+        alone.run()
+        assert same_tree(alone.cover(ja), F.cover(int(ids[i])))
+        for a, b in zip(alone.matrix(ja), F.matrix(int(ids[i]))):
+            assert np.array_equal(a, b)
+        assert np.array_equal(alone.measure(ja), F.measure(int(ids[i])))
