@@ -927,8 +927,8 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
     const DepthClass dc = classifyDepth(T, d);
     const Source src = makeSource(T, dIt, check);
     const uint32_t k = s->grid.k;
-    if (k > 64) return setErr(B12X_E_UNSUPPORTED, "transition matrix with more than 64 points per box");
     const uint64_t P = src.nBoxes * k;
+    if (P > 0xFFFFFFFFull) return setErr(B12X_E_UNSUPPORTED, "more than 2^32 test points in one matrix");
     const uint32_t nBoxes = dc == DC_ROOT ? 1u : T.n;  // count(depth)
     auto* M = new b12x_coo_s();
     try {
@@ -938,39 +938,78 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
       uint32_t* col = dalloc<uint32_t>(P);
       runImage(T, *s, src, dc == DC_ROOT, 1, col);
       const uint64_t nRowsSrc = src.nBoxes;
-      uint32_t* rowPtr = dalloc<uint32_t>(nRowsSrc + 1);
-      TmRowArgs a{};
-      a.col = col;
-      a.nRows = nRowsSrc;
-      a.k = k;
-      a.keepInvalid = useInvalid ? 1 : 0;
-      a.invalidLabel = nBoxes + 1;  // nColumns_ after its increment (ImplicitBoxTree.hpp:3081-3087,3127-3132)
-      a.rowCount = rowPtr;
-      a.anyInvalid = T.scal + 2;
-      a.pass = 0;
+      uint32_t nnz = 0;
+      uint32_t* rows = nullptr;
+      uint32_t* cols = nullptr;
+      uint32_t* cnt = nullptr;
       zeroScal(T);
-      tmRows(a, T.stream);
-      HIPCHK(hipMemsetAsync(rowPtr + nRowsSrc, 0, 4, T.stream));
-      const uint32_t nnz = scanArray(T.stream, rowPtr, nRowsSrc + 1, T.totals ? T.totals : nullptr, T.scal + 3,
-                                     T.hscal + 3);
+      if (k <= 64) {
+        // k targets of a box sorted inside one wave segment, run-length encoded
+        uint32_t* rowPtr = dalloc<uint32_t>(nRowsSrc + 1);
+        TmRowArgs a{};
+        a.col = col;
+        a.nRows = nRowsSrc;
+        a.k = k;
+        a.keepInvalid = useInvalid ? 1 : 0;
+        a.invalidLabel = nBoxes + 1;  // nColumns_ after its increment (ImplicitBoxTree.hpp:3081-3087,3127-3132)
+        a.rowCount = rowPtr;
+        a.anyInvalid = T.scal + 2;
+        a.pass = 0;
+        tmRows(a, T.stream);
+        HIPCHK(hipMemsetAsync(rowPtr + nRowsSrc, 0, 4, T.stream));
+        nnz = scanArray(T.stream, rowPtr, nRowsSrc + 1, T.totals, T.scal + 3, T.hscal + 3);
+        cols = dalloc<uint32_t>(nnz);
+        cnt = dalloc<uint32_t>(nnz);
+        a.pass = 1;
+        a.rowPtr = rowPtr;
+        a.outCol = cols;
+        a.outCnt = cnt;
+        tmRows(a, T.stream);
+        rows = dalloc<uint32_t>(nnz);
+        if (nRowsSrc)
+          hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nRowsSrc, kBlock)), dim3(kBlock), 0, T.stream, rowPtr,
+                             nRowsSrc, rows);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(T.stream));
+        dfree(rowPtr);
+        dfree(col);
+      } else {
+        // any k: stable sort by column, then by row (row-major), then run-length encoding
+        uint32_t* prow = dalloc<uint32_t>(P);
+        uint32_t* dummy = dalloc<uint32_t>(P);
+        hipLaunchKernelGGL(rowOfPointKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, prow, P, k);
+        hipLaunchKernelGGL(anyInvalidKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream,
+                           (const uint32_t*)col, P, T.scal + 2);
+        radixSort3(T.stream, col, prow, dummy, P, 32, T.hscal + 4);
+        radixSort3(T.stream, prow, col, dummy, P, bitsFor(nRowsSrc), T.hscal + 4);
+        dfree(dummy);
+        if (P > 0xFFFFFFFEull) throw HipFail(B12X_E_ARG, "scan too large");
+        RunLengthOp op{};
+        op.row = prow;
+        op.col = col;
+        op.n = P;
+        op.keepInvalid = useInvalid ? 1 : 0;
+        op.invalidLabel = nBoxes + 1;
+        // count first to size the outputs (tile totals sized for P elements, not for the tree)
+        uint32_t* ptot = dalloc<uint32_t>(2 * uint64_t(ceilDiv(P, kTile)) + 16);
+        runScan(T, op, uint32_t(P), ptot, 3);
+        syncScal(T, 4);
+        nnz = T.hscal[3];
+        rows = dalloc<uint32_t>(nnz);
+        cols = dalloc<uint32_t>(nnz);
+        cnt = dalloc<uint32_t>(nnz);
+        op.outRow = rows;
+        op.outCol = cols;
+        op.outCnt = cnt;
+        runScan(T, op, uint32_t(P), ptot, 3);
+        HIPCHK(hipStreamSynchronize(T.stream));
+        dfree(ptot);
+        dfree(prow);
+        dfree(col);
+      }
       syncScal(T, 3);
       const bool anyInvalid = T.hscal[2] != 0;
       if (useInvalid && anyInvalid) M->nCols = nBoxes + 1;
-      uint32_t* cols = dalloc<uint32_t>(nnz);
-      uint32_t* cnt = dalloc<uint32_t>(nnz);
-      a.pass = 1;
-      a.rowPtr = rowPtr;
-      a.outCol = cols;
-      a.outCnt = cnt;
-      tmRows(a, T.stream);
-      uint32_t* rows = dalloc<uint32_t>(nnz);
-      if (nRowsSrc)
-        hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nRowsSrc, kBlock)), dim3(kBlock), 0, T.stream, rowPtr,
-                           nRowsSrc, rows);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(T.stream));
-      dfree(col);
-      dfree(rowPtr);
       if (!rowMajor) {
         // column-major order (CooMatrix.hpp:95-137): stable LSD radix sort by column of the
         // row-major entries leaves rows ascending within each column

@@ -510,6 +510,41 @@ static __global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
   }
 }
 
+// General transition-matrix rows for any k: (row, col) pairs sorted row-major by two stable radix
+// passes, then run-length encoded here. count(i) = 1 at the start of a run that becomes an entry.
+struct RunLengthOp : OpBase {
+  const uint32_t* row;
+  const uint32_t* col;
+  uint64_t n;
+  int keepInvalid;
+  uint32_t invalidLabel;
+  uint32_t* outRow;
+  uint32_t* outCol;
+  uint32_t* outCnt;
+  __device__ bool isStart(uint32_t i) const { return i == 0 || row[i] != row[i - 1] || col[i] != col[i - 1]; }
+  __device__ uint32_t count(uint32_t i) const {
+    return (isStart(i) && (col[i] != 0xFFFFFFFFu || keepInvalid)) ? 1u : 0u;
+  }
+  __device__ void emit(uint32_t i, uint32_t, uint32_t pos) const {
+    uint64_t j = i + 1;
+    while (j < n && row[j] == row[i] && col[j] == col[i]) ++j;
+    outRow[pos] = row[i];
+    outCol[pos] = col[i] == 0xFFFFFFFFu ? invalidLabel : col[i];
+    outCnt[pos] = uint32_t(j - i);
+  }
+};
+
+static __global__ __launch_bounds__(kBlock) void rowOfPointKernel(uint32_t* row, uint64_t n, uint32_t k) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) row[i] = uint32_t(i / k);
+}
+
+static __global__ __launch_bounds__(kBlock) void anyInvalidKernel(const uint32_t* col, uint64_t n, uint32_t* flag) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const bool inv = i < n && col[i] == 0xFFFFFFFFu;
+  if (__ballot(inv) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1u);
+}
+
 // Expand CSR row pointers into the COO row index per entry.
 static __global__ __launch_bounds__(kBlock) void expandRowsKernel(const uint32_t* rowPtr, uint64_t nRows, uint32_t* rows) {
   const uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x;

@@ -296,3 +296,14 @@ def test_cpp_facade_doc_example():
     assert out[12] == f"nnz {m.nnz} rows {m.nRows} cols {m.nColumns}"
     mu, _, _ = m.power_iteration(50)
     assert abs(float(out[13].split()[1]) - mu[0]) <= 1e-12 * abs(mu[0])
+
+
+@pytest.mark.parametrize("grid,use_invalid,row_major", [(gx.InnerGrid(10), False, False), (gx.InnerGrid(9), True, True),
+                                                       (gx.UniformGrid(100, 3), False, True)])
+def test_transition_matrix_more_than_64_points_per_box(grid, use_invalid, row_major):
+    g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 8)
+    mg = g.transitionMatrixForLeaves(grid, gx.Henon(), row_major=row_major, use_invalid=use_invalid)
+    mo = o.transitionMatrixForLeaves(grid, gx.Henon(), row_major=row_major, use_invalid=use_invalid)
+    assert (mg.nnz, mg.nRows, mg.nColumns) == (mo.nnz, mo.nRows, mo.nColumns)
+    for a, b in zip(mg.download(), mo.download()):
+        assert np.array_equal(a, b)
