@@ -149,9 +149,10 @@ void ensureCap(b12x_tree_s& T, uint64_t need) {
 
 // Enqueues a three-phase scan over n elements (or *nDev, with n an upper bound); the total lands
 // in T.scal[scalSlot] and the summed Op::stat values in *stat (when set). Does not synchronise.
+// With countOnly, only the count and total passes run (the op's outputs may be unset).
 template <class Op>
 void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int scalSlot, const uint32_t* nDev = nullptr,
-             int statSlot = -1) {
+             int statSlot = -1, bool countOnly = false) {
   if (n == 0) return;
   const uint32_t nb = ceilDiv(n, kTile);
   if (nb > kMaxScanTiles) throw HipFail(B12X_E_ARG, "scan too large");
@@ -159,7 +160,7 @@ void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int sca
   hipLaunchKernelGGL(scanCountKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals, stats);
   hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, T.stream, totals, nb, nDev, T.scal + scalSlot,
                      (const uint32_t*)stats, statSlot >= 0 ? T.scal + statSlot : nullptr);
-  hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals);
+  if (!countOnly) hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals);
   HIPCHK(hipGetLastError());
 }
 
@@ -992,7 +993,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
         op.invalidLabel = nBoxes + 1;
         // count first to size the outputs (tile totals sized for P elements, not for the tree)
         uint32_t* ptot = dalloc<uint32_t>(2 * uint64_t(ceilDiv(P, kTile)) + 16);
-        runScan(T, op, uint32_t(P), ptot, 3);
+        runScan(T, op, uint32_t(P), ptot, 3, nullptr, -1, /*countOnly=*/true);
         syncScal(T, 4);
         nnz = T.hscal[3];
         rows = dalloc<uint32_t>(nnz);
