@@ -62,6 +62,101 @@ def gpu_job(gx, flavour, timing=False):
     return res
 
 
+HENON_C, HENON_R = [0.0, 0.0], [1.5, 0.5]
+
+
+def sweep_jobs(n_jobs, seed=0):
+    """Config 4: Hénon a ~ U[1.0, 1.4], b = 0.3, depth ~ U{14..22} (numpy PCG64, seed 0, drawn
+    in the order (a, depth) per job)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [(float(rng.uniform(1.0, 1.4)), int(rng.integers(14, 23))) for _ in range(n_jobs)]
+
+
+def sweep_bench(gx, n_jobs, pi_iter=200):
+    jobs = sweep_jobs(n_jobs)
+    grid = gx.InnerGrid(4)
+    # plain form: one job per launch sequence, sequentially
+    t0 = time.perf_counter()
+    for a, d in jobs:
+        t = gx.ImplicitBoxTree(HENON_C, HENON_R)
+        t.setFlags(gx.SD, 0)
+        t.subdivision_steps(grid, gx.Henon(a, 0.3, 1), d)
+        m = t.transitionMatrixForLeaves(grid, gx.Henon(a, 0.3, 1))
+        m.power_iteration(pi_iter)
+    plain_s = time.perf_counter() - t0
+    # batched form: every job in one forest, shared launches
+    t0 = time.perf_counter()
+    F = gx.Forest(2, grid, 1, pi_max_iter=pi_iter)
+    ids = [F.add(HENON_C, HENON_R, gx.Henon(a, 0.3, 1), d) for a, d in jobs]
+    F.run()
+    batched_s = time.perf_counter() - t0
+    assert all(F.info(i)["state"] == 2 for i in ids)
+    return {"config": f"{n_jobs} Hénon jobs, a~U[1.0,1.4], b=0.3, depth~U{{14..22}}, k=16, TM + {pi_iter} power "
+                      f"iterations each", "plain_jobs_per_s": n_jobs / plain_s,
+            "batched_jobs_per_s": n_jobs / batched_s, "ratio": plain_s / batched_s,
+            "plain_s": plain_s, "batched_s": batched_s}
+
+
+def serve_requests(n_req, rate=200.0, seed=0):
+    """Config 5: Poisson arrivals at `rate`; 60% Lorenz (rho~U[24,32], depth U{18..26}, k=27,
+    RK4 h=0.01, T=0.2), 40% forced Duffing (period map, 100 RK4 steps, domain radius (2,2),
+    depth U{16..24}, k=16)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    req = []
+    for i in range(n_req):
+        if rng.random() < 0.6:
+            req.append((0, [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], [10.0, float(rng.uniform(24, 32)), 8.0 / 3.0, 0.01, 20.0],
+                        int(rng.integers(18, 27))))
+        else:
+            req.append((1, [0.0, 0.0], [2.0, 2.0], [0.25, -1.0, 1.0, 0.3, 1.0, 0.0, 2 * 3.141592653589793,
+                                                   2 * 3.141592653589793 / 100], int(rng.integers(16, 25))))
+    return arrival, req
+
+
+def serve_bench(gx, n_req, rate=200.0, tol=TOL):
+    import numpy as np
+    arrival, req = serve_requests(n_req, rate)
+    def forests():
+        return [gx.Forest(3, gx.InnerGrid(3), 2, pi_max_iter=MAX_ITER, pi_tol=tol),
+                gx.Forest(2, gx.InnerGrid(4), 4, pi_max_iter=MAX_ITER, pi_tol=tol)]
+    args = ([r[0] for r in req], [r[1] for r in req], [r[2] for r in req], [r[3] for r in req], [r[4] for r in req])
+    fs = forests()
+    lat, comp, _ = gx.serve_run(fs, arrival, *args)
+    served = {"jobs_per_s": n_req / comp.max(), "p50_s": float(np.percentile(lat, 50)),
+              "p99_s": float(np.percentile(lat, 99)), "makespan_s": float(comp.max())}
+    fs = forests()
+    lat0, comp0, _ = gx.serve_run(fs, np.zeros(n_req), *args)
+    saturated = {"jobs_per_s": n_req / comp0.max(), "p50_s": float(np.percentile(lat0, 50)),
+                 "p99_s": float(np.percentile(lat0, 99))}
+    # plain: the same jobs one at a time through the single-tree path, in arrival order
+    t0 = time.perf_counter()
+    plain_lat = []
+    clock = 0.0
+    for (q, c, r, mp, d), a in zip(req, arrival):
+        s0 = time.perf_counter()
+        grid = gx.InnerGrid(3) if q == 0 else gx.InnerGrid(4)
+        mapf = gx.Lorenz(*mp) if q == 0 else gx.Duffing(*mp)
+        t = gx.ImplicitBoxTree(c, r)
+        t.setFlags(gx.SD, 0)
+        t.subdivision_steps(grid, mapf, d)
+        m = t.transitionMatrixForLeaves(grid, mapf)
+        m.power_iteration(MAX_ITER, tol)
+        dur = time.perf_counter() - s0
+        clock = max(clock, a) + dur      # a single server queue
+        plain_lat.append(clock - a)
+    plain_s = time.perf_counter() - t0
+    return {"config": f"{n_req} requests, Poisson {rate:g} jobs/s, 60% Lorenz rho~U[24,32] depth U{{18..26}} k=27, "
+                      f"40% forced Duffing depth U{{16..24}} k=16; TM + power iteration to {tol:g}; "
+                      f"200 GB admission budget",
+            "served": served, "saturated": saturated,
+            "plain": {"jobs_per_s": n_req / plain_s, "p50_s": float(np.percentile(plain_lat, 50)),
+                      "p99_s": float(np.percentile(plain_lat, 99))},
+            "ratio_saturated_vs_plain": saturated["jobs_per_s"] / (n_req / plain_s)}
+
+
 def cpu_baseline(sample_depth, threads):
     """B12's CPU path restated (oracle): same algorithm structure as the reference's host
     Thrust path, including its second image evaluation per search, OpenMP over `threads`."""
@@ -92,6 +187,8 @@ def main():
     ap.add_argument("--flavour", default="parity", choices=["parity", "fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-depth", type=int, default=24)
+    ap.add_argument("--sweep-jobs", type=int, default=512, help="config 4 jobs (0 to skip)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="config 5 requests (0 to skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build's hot path runs on a single MI355X "
@@ -139,6 +236,10 @@ def main():
         "breakdown_ms": {k: runs[-1][k] for k in ("subdiv_ms", "tm_ms", "measure_ms")},
         "job": {k: runs[-1][k] for k in ("images", "boxes", "nnz", "iters", "resid", "level24_points")},
     }
+    if args.sweep_jobs > 0:
+        out["sweep"] = sweep_bench(gx, args.sweep_jobs)
+    if args.serve_requests > 0:
+        out["serving"] = serve_bench(gx, args.serve_requests)
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         threads = min(threads, 16)
