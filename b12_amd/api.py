@@ -71,6 +71,8 @@ SIGNATURES = {
     "b12x_forest_job_measure": [_P, _U32, _P],
     "b12x_forest_job_release": [_P, _U32],
     "b12x_forest_footprint": [_P, _P],
+    "b12x_forest_stats": [_P, _P, ctypes.c_int],
+    "b12x_forest_set_pool_iterations": [_P, ctypes.c_int],
     "b12x_serve_run": [_P, _I, _U32, _P, _P, _P, _P, _I, _P, _I, _P, _D, _D, _P, _P, _P],
 }
 
@@ -489,6 +491,17 @@ class Forest:
         b = _U64()
         self.api("b12x_forest_footprint", self.h, ctypes.byref(b))
         return b.value
+
+    STAT_KEYS = ("steps", "subdiv_s", "retire_s", "pool_s", "pool_iters", "pool_max_jobs", "pool_max_cols",
+                 "pool_max_nnz")
+
+    def stats(self):
+        v = np.zeros(len(self.STAT_KEYS), np.float64)
+        self.api("b12x_forest_stats", self.h, _ptr(v), len(v))
+        return dict(zip(self.STAT_KEYS, v.tolist()))
+
+    def set_pool_iterations(self, iters):
+        self.api("b12x_forest_set_pool_iterations", self.h, int(iters))
 
 
 def serve_run(forests, arrival, forest_of, centers, radii, map_params, depths, budget_bytes=200e9,

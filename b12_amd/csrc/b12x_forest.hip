@@ -38,7 +38,7 @@ struct ResultArena {
 };
 
 struct Job {
-  int state = 0;  // 0 waiting for upload, 1 active, 2 done, 3 released
+  int state = 0;  // 0 waiting for upload, 1 active, 4 measure iterating, 2 done, 3 released
   uint8_t target = 0, level = 0;
   JobDev dev{};
   ForcingStep* dForcing = nullptr;
@@ -49,6 +49,49 @@ struct Job {
   std::shared_ptr<ResultArena> arena;
   uint64_t leafBase = 0, nnzBase = 0;
   double tDone = 0.0;
+};
+
+// Host mirror of one job in the power-iteration pool.
+struct PoolJob {
+  uint32_t slot;      // forest job slot
+  uint32_t lo, hi;    // pool positions
+  uint64_t elo, ehi;  // pool entries
+  double* muOut;      // where the measure goes (the job's place in its result arena)
+};
+
+// Block-diagonal transition matrices of finished jobs still iterating towards their measure.
+struct Pool {
+  uint32_t n = 0, capN = 0, capJ = 0, nB = 0, capB = 0;
+  uint64_t nnz = 0, capE = 0;
+  uint32_t* colPtr = nullptr;    // capN + 1
+  uint32_t* jobOfPos = nullptr;  // capN
+  double* y = nullptr;           // 3 slots of stride capN
+  uint32_t* rowIdx = nullptr;    // capE
+  double* vals = nullptr;
+  uint32_t* jobOfBlock = nullptr;  // capB: pool job of each step-kernel block
+  double* partAbs = nullptr;       // capB
+  double* partRes = nullptr;
+  PoolJobDev* dJobs = nullptr;  // capJ
+  std::vector<PoolJob> jobs;
+  // captured batch of iterations for the current pool layout (recaptured after joins and exits)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graphIters = 0;
+  void dropGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graphIters = 0;
+  }
+  void freeAll() {
+    dropGraph();
+    dfree(colPtr); dfree(jobOfPos); dfree(y); dfree(rowIdx); dfree(vals); dfree(jobOfBlock); dfree(partAbs);
+    dfree(partRes); dfree(dJobs);
+    colPtr = jobOfPos = rowIdx = jobOfBlock = nullptr;
+    y = vals = partAbs = partRes = nullptr;
+    dJobs = nullptr;
+  }
 };
 
 double nowSeconds() {
@@ -84,8 +127,14 @@ struct b12x_forest_s {
   std::vector<uint32_t> waiting;  // slots added but not yet uploaded
   double imageMs = 0.0;
   uint64_t steps = 0;
+  // telemetry: wall seconds per phase, pool iterations, pool high-water marks
+  double tSubdiv = 0.0, tRetire = 0.0, tPool = 0.0;
+  uint64_t poolIters = 0, poolMaxJobs = 0, poolMaxN = 0, poolMaxNnz = 0;
+  Pool pool;
+  int poolItersPerStep = 64;
 
   ~b12x_forest_s() {
+    pool.freeAll();
     for (int b = 0; b < 2; ++b) {
       dfree(arr[b].key); dfree(arr[b].depth); dfree(arr[b].flags); dfree(arr[b].job);
     }
@@ -248,8 +297,244 @@ void admitWaiting(b12x_forest_s& F) {
   fBoundary(F, F.n);
 }
 
-// Retire the jobs in R (slots): batched transition matrix, block-diagonal power iteration, copy
-// of the covers, then removal of their leaves from the forest.
+static __global__ __launch_bounds__(kBlock) void addCopyKernel(const uint32_t* in, uint64_t n, uint32_t add,
+                                                               uint32_t* out) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) out[i] = in[i] + add;
+}
+
+template <class T>
+void growCopy(T*& p, uint64_t used, uint64_t newCap, hipStream_t st) {
+  T* q = dalloc<T>(newCap);
+  if (used) HIPCHK(hipMemcpyAsync(q, p, used * sizeof(T), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(p);
+  p = q;
+}
+
+// Append a wave of finished jobs: wave positions 0..m-1 (job-major), CSC colPtr[m+1] over them,
+// rows as wave positions, values, and each job's wave position and entry range.
+void poolJoin(b12x_forest_s& F, uint32_t m, uint64_t nnz, const uint32_t* colPtr, const uint32_t* rows,
+              const double* vals, const std::vector<PoolJob>& joiners) {
+  Pool& P = F.pool;
+  hipStream_t st = F.stream;
+  P.dropGraph();
+  const uint32_t n0 = P.n;
+  const uint64_t e0 = P.nnz;
+  if (uint64_t(n0) + m + 1 > 0xFFFFFFFEull || e0 + nnz > 0xFFFFFFFEull)
+    throw HipFail(B12X_E_UNSUPPORTED, "power-iteration pool exceeds 2^32 entries");
+  if (n0 + m + 1 > P.capN) {
+    const uint32_t c = std::max<uint32_t>({n0 + m + 1, P.capN * 2, 4096});
+    growCopy(P.colPtr, n0 + (n0 ? 1 : 0), uint64_t(c) + 1, st);
+    growCopy(P.jobOfPos, n0, c, st);
+    double* y = dalloc<double>(uint64_t(c) * 3);
+    for (int sl = 0; sl < 3 && n0; ++sl)
+      HIPCHK(hipMemcpyAsync(y + uint64_t(sl) * c, P.y + uint64_t(sl) * P.capN, uint64_t(n0) * 8,
+                            hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(P.y);
+    P.y = y;
+    P.capN = c;
+  }
+  if (e0 + nnz > P.capE) {
+    const uint64_t c = std::max<uint64_t>({e0 + nnz, P.capE * 2, 65536});
+    growCopy(P.rowIdx, e0, c, st);
+    growCopy(P.vals, e0, c, st);
+    P.capE = c;
+  }
+  const uint32_t nJ0 = uint32_t(P.jobs.size());
+  const uint32_t nJ = nJ0 + uint32_t(joiners.size());
+  uint32_t nB = P.nB;
+  for (const PoolJob& pj : joiners) nB += poolBlocksOf(pj.hi - pj.lo);
+  if (nB > P.capB) {
+    const uint32_t c = std::max<uint32_t>({nB, P.capB * 2, 1024});
+    growCopy(P.jobOfBlock, P.nB, c, st);
+    dfree(P.partAbs);
+    dfree(P.partRes);
+    P.partAbs = dalloc<double>(c);
+    P.partRes = dalloc<double>(c);
+    P.capB = c;
+  }
+  if (nJ > P.capJ) {
+    const uint32_t c = std::max<uint32_t>({nJ, P.capJ * 2, 64});
+    growCopy(P.dJobs, nJ0, c, st);
+    P.capJ = c;
+  }
+  // colPtr (m+1 values, rebased) ; rows rebased to pool positions ; values ; job of position
+  hipLaunchKernelGGL(addCopyKernel, dim3(ceilDiv(uint64_t(m) + 1, kBlock)), dim3(kBlock), 0, st, colPtr,
+                     uint64_t(m) + 1, uint32_t(e0), P.colPtr + n0);
+  if (nnz) {
+    hipLaunchKernelGGL(addCopyKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, rows, nnz, n0,
+                       P.rowIdx + e0);
+    HIPCHK(hipMemcpyAsync(P.vals + e0, vals, nnz * 8, hipMemcpyDeviceToDevice, st));
+  }
+  std::vector<uint32_t> jop(m);
+  std::vector<PoolJobDev> jd(joiners.size());
+  uint32_t b = P.nB;
+  for (size_t r = 0; r < joiners.size(); ++r) {
+    PoolJob pj = joiners[r];
+    for (uint32_t q = pj.lo; q < pj.hi; ++q) jop[q] = nJ0 + uint32_t(r);
+    pj.lo += n0;
+    pj.hi += n0;
+    pj.elo += e0;
+    pj.ehi += e0;
+    jd[r] = PoolJobDev{pj.lo, pj.hi, 1u, 0, 0u, b, {0.0, 0.0, 0.0}, -1.0};
+    b += poolBlocksOf(pj.hi - pj.lo);
+    P.jobs.push_back(pj);
+  }
+  if (m) HIPCHK(hipMemcpyAsync(P.jobOfPos + n0, jop.data(), uint64_t(m) * 4, hipMemcpyHostToDevice, st));
+  if (!jd.empty()) {
+    HIPCHK(hipMemcpyAsync(P.dJobs + nJ0, jd.data(), jd.size() * sizeof(PoolJobDev), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(poolJobOfBlockKernel, dim3(uint32_t(jd.size())), dim3(kBlock), 0, st,
+                       (const PoolJobDev*)P.dJobs, nJ0, nJ, P.jobOfBlock);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  P.n = n0 + m;
+  P.nnz = e0 + nnz;
+  P.nB = nB;
+}
+
+// Advance every pool job by `iters` iterations, then hand finished jobs their measures and drop
+// them from the pool.
+void poolIterateOnce(b12x_forest_s& F, int iters);
+
+void poolIterate(b12x_forest_s& F, int iters) {
+  if (F.pool.jobs.empty()) return;
+  const double t0 = nowSeconds();
+  F.poolIters += uint64_t(iters);
+  F.poolMaxJobs = std::max<uint64_t>(F.poolMaxJobs, F.pool.jobs.size());
+  F.poolMaxN = std::max<uint64_t>(F.poolMaxN, F.pool.n);
+  F.poolMaxNnz = std::max<uint64_t>(F.poolMaxNnz, F.pool.nnz);
+  poolIterateOnce(F, iters);
+  F.tPool += nowSeconds() - t0;
+}
+
+void poolIterateOnce(b12x_forest_s& F, int iters) {
+  Pool& P = F.pool;
+  hipStream_t st = F.stream;
+  const uint32_t nJ = uint32_t(P.jobs.size());
+  auto enqueue = [&]() {
+    for (int b = 0; b < iters; ++b) {
+      hipLaunchKernelGGL(poolStepKernel, dim3(P.nB), dim3(kBlock), 0, st, (const uint32_t*)P.colPtr,
+                         (const uint32_t*)P.rowIdx, (const double*)P.vals, (const uint32_t*)P.jobOfBlock,
+                         (const PoolJobDev*)P.dJobs, P.y, P.capN, P.partAbs, P.partRes);
+      hipLaunchKernelGGL(poolReduceKernel, dim3(nJ), dim3(kBlock), 0, st, (const double*)P.partAbs,
+                         (const double*)P.partRes, P.dJobs, nJ, uint32_t(std::max(F.piMaxIter, 0)), F.piTol);
+    }
+  };
+  // the batch runs from a captured HIP graph (B12X_GRAPHS=0: eager launches)
+  const char* ge = std::getenv("B12X_GRAPHS");
+  if (ge && ge[0] == '0') {
+    enqueue();
+  } else {
+    if (!P.exec || P.graphIters != iters) {
+      P.dropGraph();
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      HIPCHK(hipStreamEndCapture(st, &P.graph));
+      HIPCHK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+      P.graphIters = iters;
+    }
+    HIPCHK(hipGraphLaunch(P.exec, st));
+  }
+  HIPCHK(hipGetLastError());
+  std::vector<PoolJobDev> hj(nJ);
+  HIPCHK(hipMemcpyAsync(hj.data(), P.dJobs, nJ * sizeof(PoolJobDev), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  bool anyDone = false;
+  for (uint32_t r = 0; r < nJ; ++r) anyDone = anyDone || hj[r].done;
+  if (!anyDone) return;
+  P.dropGraph();
+  const double tNow = nowSeconds();
+  std::vector<int> keep(nJ, 1);
+  std::vector<uint32_t> posShift(nJ, 0), nnzShift(nJ, 0), newIndex(nJ, 0);
+  std::vector<PoolJob> kept;
+  std::vector<PoolJobDev> keptDev;
+  uint32_t nNew = 0, bNew = 0;
+  uint64_t eNew = 0;
+  for (uint32_t r = 0; r < nJ; ++r) {
+    PoolJob& pj = P.jobs[r];
+    if (hj[r].done) {
+      keep[r] = 0;
+      hipLaunchKernelGGL(poolFinishKernel, dim3(std::max<uint32_t>(1, ceilDiv(pj.hi - pj.lo, kBlock))), dim3(kBlock), 0, st,
+                         (const double*)P.y, P.capN, (const PoolJobDev*)P.dJobs, r, pj.muOut);
+      Job& J = F.jobs[pj.slot];
+      J.iters = int(hj[r].final);
+      J.resid = hj[r].resid;
+      J.state = 2;
+      J.tDone = tNow;
+      continue;
+    }
+    posShift[r] = pj.lo - nNew;
+    nnzShift[r] = uint32_t(pj.elo - eNew);
+    newIndex[r] = uint32_t(kept.size());
+    PoolJobDev d = hj[r];
+    const uint32_t len = pj.hi - pj.lo;
+    const uint64_t elen = pj.ehi - pj.elo;
+    d.lo = nNew;
+    d.hi = nNew + len;
+    d.b0 = bNew;
+    bNew += poolBlocksOf(len);
+    PoolJob q = pj;
+    q.lo = nNew;
+    q.hi = nNew + len;
+    q.elo = eNew;
+    q.ehi = eNew + elen;
+    kept.push_back(q);
+    keptDev.push_back(d);
+    nNew += len;
+    eNew += elen;
+  }
+  HIPCHK(hipGetLastError());
+  // compact into fresh buffers
+  uint32_t* dShiftP = dalloc<uint32_t>(nJ);
+  uint32_t* dShiftE = dalloc<uint32_t>(nJ);
+  uint32_t* dNewIdx = dalloc<uint32_t>(nJ);
+  int* dKeep = dalloc<int>(nJ);
+  HIPCHK(hipMemcpyAsync(dShiftP, posShift.data(), nJ * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dShiftE, nnzShift.data(), nJ * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dNewIdx, newIndex.data(), nJ * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dKeep, keep.data(), nJ * sizeof(int), hipMemcpyHostToDevice, st));
+  uint32_t* colPtr = dalloc<uint32_t>(uint64_t(P.capN) + 1);
+  uint32_t* jobOfPos = dalloc<uint32_t>(P.capN);
+  double* y = dalloc<double>(uint64_t(P.capN) * 3);
+  uint32_t* rowIdx = dalloc<uint32_t>(P.capE);
+  double* vals = dalloc<double>(P.capE);
+  if (P.n)
+    hipLaunchKernelGGL(poolMoveColumnsKernel, dim3(ceilDiv(P.n, kBlock)), dim3(kBlock), 0, st,
+                       (const uint32_t*)P.colPtr, (const uint32_t*)P.jobOfPos, (const double*)P.y, P.n, P.capN,
+                       (const uint32_t*)dShiftP, (const uint32_t*)dShiftE, (const int*)dKeep,
+                       (const uint32_t*)dNewIdx, colPtr, jobOfPos, y, P.capN);
+  if (P.nnz)
+    hipLaunchKernelGGL(poolMoveEntriesKernel, dim3(ceilDiv(P.nnz, kBlock)), dim3(kBlock), 0, st,
+                       (const uint32_t*)P.rowIdx, (const double*)P.vals, P.nnz, (const uint32_t*)P.jobOfPos,
+                       (const uint32_t*)dShiftP, (const uint32_t*)dShiftE, (const int*)dKeep, rowIdx, vals);
+  HIPCHK(hipGetLastError());
+  const uint32_t eEnd = uint32_t(eNew);
+  HIPCHK(hipMemcpyAsync(colPtr + nNew, &eEnd, 4, hipMemcpyHostToDevice, st));
+  if (!keptDev.empty()) {
+    HIPCHK(hipMemcpyAsync(P.dJobs, keptDev.data(), keptDev.size() * sizeof(PoolJobDev), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(poolJobOfBlockKernel, dim3(uint32_t(keptDev.size())), dim3(kBlock), 0, st,
+                       (const PoolJobDev*)P.dJobs, 0u, uint32_t(keptDev.size()), P.jobOfBlock);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(P.colPtr); dfree(P.jobOfPos); dfree(P.y); dfree(P.rowIdx); dfree(P.vals);
+  P.colPtr = colPtr;
+  P.jobOfPos = jobOfPos;
+  P.y = y;
+  P.rowIdx = rowIdx;
+  P.vals = vals;
+  P.jobs.swap(kept);
+  P.n = nNew;
+  P.nnz = eNew;
+  P.nB = bNew;
+  dfree(dShiftP); dfree(dShiftE); dfree(dNewIdx); dfree(dKeep);
+}
+
+// Retire the jobs in R (slots): batched transition matrix, copy of the covers, entry into the
+// power-iteration pool, then removal of their leaves from the forest.
 void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_t>& off,
             const std::vector<uint32_t>& end) {
   std::sort(R.begin(), R.end(), [&](uint32_t a, uint32_t b) { return off[a] < off[b]; });
@@ -359,7 +644,7 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
                      (const uint32_t*)cColPos, uint64_t(nnz), nList, colPtr);
   std::vector<uint32_t> hColPtr(uint64_t(nList) + 1);
   HIPCHK(hipMemcpyAsync(hColPtr.data(), colPtr, (uint64_t(nList) + 1) * 4, hipMemcpyDeviceToHost, st));
-  // block-diagonal power iteration (values as probabilities count/k even when useAbs is set)
+  // the measure is iterated in the pool (values as probabilities count/k even when useAbs is set)
   double* pval = cval;
   if (F.useAbs) {
     pval = dalloc<double>(nnz);
@@ -367,53 +652,6 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
       hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)ccnt,
                          uint64_t(nnz), double(k), 0, pval);
   }
-  double* ybuf = dalloc<double>(uint64_t(nList) * 3);
-  double* absCol = dalloc<double>(nList);
-  double* resCol = dalloc<double>(nList);
-  double* norm = dalloc<double>(uint64_t(nR) * 3);
-  double* resid = dalloc<double>(nR);
-  int* done = dalloc<int>(nR);
-  uint32_t* finalIt = dalloc<uint32_t>(nR);
-  BatchPiState* pst = dalloc<BatchPiState>(1);
-  BatchPiState init{1, 1, 0, 0};
-  if (F.piMaxIter <= 0) init.nDone = nR;
-  HIPCHK(hipMemcpyAsync(pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(done, 0, nR * sizeof(int), st));
-  HIPCHK(hipMemsetAsync(finalIt, 0, nR * 4, st));
-  HIPCHK(hipMemsetAsync(norm, 0, uint64_t(nR) * 3 * 8, st));
-  const uint32_t nbStep = std::max<uint32_t>(1, ceilDiv(uint64_t(nList) * kPiLanes, kBlock));
-  const int batch = 16;
-  auto enqueue = [&]() {
-    for (int b = 0; b < batch; ++b) {
-      hipLaunchKernelGGL(batchPiStepKernel, dim3(nbStep), dim3(kBlock), 0, st, (const uint32_t*)colPtr,
-                         (const uint32_t*)crow, (const double*)pval, (const uint32_t*)jobOfPos, (const uint32_t*)dLo,
-                         nR, ybuf, nList, (const double*)norm, (const int*)done, absCol, resCol, pst);
-      hipLaunchKernelGGL(batchPiReduceKernel, dim3(nR), dim3(kBlock), 0, st, (const double*)absCol,
-                         (const double*)resCol, (const uint32_t*)dLo, nR, norm, resid, done, finalIt, pst,
-                         uint32_t(std::max(F.piMaxIter, 0)), F.piTol);
-    }
-  };
-  const char* ge = std::getenv("B12X_GRAPHS");
-  const bool useGraph = !(ge && ge[0] == '0');
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  if (useGraph && init.nDone < nR) {
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    enqueue();
-    HIPCHK(hipStreamEndCapture(st, &graph));
-    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
-  BatchPiState hs{};
-  for (int64_t it = 0; init.nDone < nR && it <= int64_t(F.piMaxIter) + 2; it += batch) {
-    if (useGraph) HIPCHK(hipGraphLaunch(exec, st));
-    else enqueue();
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (hs.nDone >= nR) break;
-  }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
   // results arena: covers, matrices (requested order), measures
   auto A = std::make_shared<ResultArena>();
   A->key = dalloc<uint64_t>(nList);
@@ -424,20 +662,11 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
   A->vals = dalloc<double>(nnz);
   A->mu = dalloc<double>(nList);
   A->bytes = uint64_t(nList) * (8 + 1 + 1 + 8) + uint64_t(nnz) * 16;
-  if (nList)
-    hipLaunchKernelGGL(batchPiFinishKernel, dim3(ceilDiv(nList, kBlock)), dim3(kBlock), 0, st, (const double*)ybuf,
-                       nList, (const uint32_t*)jobOfPos, (const uint32_t*)dLo, nR, (const double*)norm,
-                       (const uint32_t*)finalIt, A->mu);
-  HIPCHK(hipGetLastError());
   if (nnz) {
     HIPCHK(hipMemcpyAsync(A->rows, F.rowMajor ? rLocRow : cLocRow, uint64_t(nnz) * 4, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(A->cols, F.rowMajor ? rLocCol : cLocCol, uint64_t(nnz) * 4, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(A->vals, F.rowMajor ? rval : cval, uint64_t(nnz) * 8, hipMemcpyDeviceToDevice, st));
   }
-  std::vector<uint32_t> hFinal(nR);
-  std::vector<double> hResid(nR);
-  HIPCHK(hipMemcpyAsync(hFinal.data(), finalIt, nR * 4, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(hResid.data(), resid, nR * 8, hipMemcpyDeviceToHost, st));
   const ForestView& c = F.arr[F.cur];
   for (uint32_t r = 0; r < nR; ++r) {
     const uint32_t cnt = lo[r + 1] - lo[r];
@@ -446,7 +675,7 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
     HIPCHK(hipMemcpyAsync(A->flags + lo[r], c.flags + offR[r], cnt, hipMemcpyDeviceToDevice, st));
   }
   HIPCHK(hipStreamSynchronize(st));
-  const double tNow = nowSeconds();
+  std::vector<PoolJob> joiners(nR);
   for (uint32_t r = 0; r < nR; ++r) {
     Job& J = F.jobs[R[r]];
     J.arena = A;
@@ -456,15 +685,13 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
     const uint64_t e1 = F.rowMajor ? hRowPtr[lo[r + 1]] : hColPtr[lo[r + 1]];
     J.nnzBase = e0;
     J.nnz = e1 - e0;
-    J.iters = int(hFinal[r]);
-    J.resid = hResid[r];
-    J.state = 2;
-    J.tDone = tNow;
+    J.state = 4;  // measuring
+    joiners[r] = PoolJob{R[r], lo[r], lo[r + 1], hColPtr[lo[r]], hColPtr[lo[r + 1]], A->mu + lo[r]};
   }
+  poolJoin(F, nList, nnz, colPtr, crow, pval, joiners);
   dfree(list); dfree(rowPtr); dfree(ecol); dfree(ecnt); dfree(erow); dfree(ccol); dfree(crow); dfree(ccnt);
   dfree(dLo); dfree(dOffR); dfree(jobOfPos); dfree(cLocRow); dfree(cLocCol); dfree(cColPos); dfree(rLocRow);
   dfree(rLocCol); dfree(rColPos); dfree(cval); dfree(rval); dfree(colPtr); if (pval != cval) dfree(pval);
-  dfree(ybuf); dfree(absCol); dfree(resCol); dfree(norm); dfree(resid); dfree(done); dfree(finalIt); dfree(pst);
   // drop the retired leaves
   ForestKeepOp kop{};
   kop.in = F.arr[F.cur];
@@ -490,7 +717,11 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
 void forestStep(b12x_forest_s& F, uint32_t* nRetired) {
   admitWaiting(F);
   if (nRetired) *nRetired = 0;
-  if (F.active.empty()) return;
+  if (F.active.empty()) {
+    poolIterate(F, F.poolItersPerStep);
+    return;
+  }
+  const double t0 = nowSeconds();
   const uint32_t n = F.n;
   ensureLeafCap(F, uint64_t(n) * 2);
   hipStream_t st = F.stream;
@@ -553,9 +784,15 @@ void forestStep(b12x_forest_s& F, uint32_t* nRetired) {
     J.level += 1;
     if (J.level >= J.target) R.push_back(s);
   }
+  const double t1 = nowSeconds();
+  F.tSubdiv += t1 - t0;
   if (!R.empty()) retire(F, R, off, end);
+  F.tRetire += nowSeconds() - t1;
   if (nRetired) *nRetired = uint32_t(R.size());
+  poolIterate(F, F.poolItersPerStep);
 }
+
+bool forestBusy(const b12x_forest_s& F) { return !F.active.empty() || !F.waiting.empty() || !F.pool.jobs.empty(); }
 
 Job& jobAt(b12x_forest_s& F, uint32_t id) {
   if (id >= F.jobs.size()) throw HipFail(B12X_E_ARG, "no such job");
@@ -639,7 +876,7 @@ int b12x_forest_step(b12x_forest f, uint32_t* nActive, uint32_t* nRetired) {
   return guarded([&]() -> int {
     if (!f) return setErr(B12X_E_ARG, "null forest");
     forestStep(*f, nRetired);
-    if (nActive) *nActive = uint32_t(f->active.size() + f->waiting.size());
+    if (nActive) *nActive = uint32_t(f->active.size() + f->waiting.size() + f->pool.jobs.size());
     return B12X_OK;
   });
 }
@@ -647,7 +884,7 @@ int b12x_forest_step(b12x_forest f, uint32_t* nActive, uint32_t* nRetired) {
 int b12x_forest_run(b12x_forest f) {
   return guarded([&]() -> int {
     if (!f) return setErr(B12X_E_ARG, "null forest");
-    while (!f->active.empty() || !f->waiting.empty()) forestStep(*f, nullptr);
+    while (forestBusy(*f)) forestStep(*f, nullptr);
     return B12X_OK;
   });
 }
@@ -725,6 +962,8 @@ int b12x_forest_footprint(b12x_forest f, uint64_t* bytes) {
     if (!f || !bytes) return setErr(B12X_E_ARG, "null argument");
     const b12x_forest_s& F = *f;
     uint64_t b = uint64_t(F.cap) * 2 * (8 + 1 + 1 + 4) + F.cap + uint64_t(F.slotCap) * (sizeof(JobDev) + 9);
+    b += (uint64_t(F.pool.capN) + 1) * 4 + uint64_t(F.pool.capN) * (4 + 3 * 8) + F.pool.capE * 12 +
+         uint64_t(F.pool.capB) * (4 + 2 * 8) + uint64_t(F.pool.capJ) * sizeof(PoolJobDev);
     std::vector<const ResultArena*> seen;
     for (const Job& J : F.jobs)
       if (J.arena && std::find(seen.begin(), seen.end(), J.arena.get()) == seen.end()) {
@@ -736,10 +975,29 @@ int b12x_forest_footprint(b12x_forest f, uint64_t* bytes) {
   });
 }
 
+int b12x_forest_stats(b12x_forest f, double* out, int n) {
+  return guarded([&]() -> int {
+    if (!f || !out) return setErr(B12X_E_ARG, "null argument");
+    const b12x_forest_s& F = *f;
+    const double v[8] = {double(F.steps), F.tSubdiv, F.tRetire, F.tPool, double(F.poolIters),
+                         double(F.poolMaxJobs), double(F.poolMaxN), double(F.poolMaxNnz)};
+    for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+    return B12X_OK;
+  });
+}
+
+int b12x_forest_set_pool_iterations(b12x_forest f, int iters) {
+  return guarded([&]() -> int {
+    if (!f || iters < 1) return setErr(B12X_E_ARG, "bad argument");
+    f->poolItersPerStep = iters;
+    return B12X_OK;
+  });
+}
+
 // Continuous-batching request loop. Requests (sorted by arrival time, seconds from the start)
 // go to forest forestOf[i]; a request is admitted once it has arrived and the forests'
 // footprint plus reservePerJob stays within budgetBytes. Every pass runs one merged step of
-// each forest with work; completion time = end of the step in which the job retired.
+// each forest with work; completion time = end of the step in which the job's measure converged.
 int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const double* arrival, const int* forestOf,
                    const double* centers, const double* radii, int stride, const double* mapParams, int nmp,
                    const uint8_t* depths, double budgetBytes, double reservePerJob, double* latency,
@@ -784,7 +1042,7 @@ int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const doub
       bool work = false;
       for (int q = 0; q < nForests; ++q) {
         b12x_forest_s& F = *forests[q];
-        if (F.active.empty() && F.waiting.empty()) continue;
+        if (!forestBusy(F)) continue;
         work = true;
         forestStep(F, nullptr);
       }
@@ -793,7 +1051,7 @@ int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const doub
         b12x_forest_s& F = *forests[q];
         for (uint32_t id = 0; id < reqOf[q].size(); ++id) {
           const int r = reqOf[q][id];
-          if (r < 0 || completion[r] >= 0.0 || F.jobs[id].state < 2) continue;
+          if (r < 0 || completion[r] >= 0.0 || (F.jobs[id].state != 2 && F.jobs[id].state != 3)) continue;
           completion[r] = tEnd;
           latency[r] = tEnd - arrival[r];
           ++completed;

@@ -171,119 +171,181 @@ static __global__ __launch_bounds__(kBlock) void forestLocalizeKernel(const uint
 }
 
 // ---------------------------------------------------------------------------------------------
-// Block-diagonal power iteration over the retiring jobs (columns and rows are list positions).
-// Each job has its own norm, residual and stopping decision; a job's sums are taken over its own
-// column range in a fixed order, so its measure does not depend on the other jobs.
+// Power-iteration pool: the transition matrices of finished jobs, block-diagonal over pool
+// positions (a job's rows and columns are its own contiguous position range [lo, hi)). Jobs join
+// and leave at any time and each carries its own iteration counter t, norm ring, residual and
+// stopping decision. A job's sums run over its own range in a fixed order, so its measure does
+// not depend on the other jobs in the pool.
 // ---------------------------------------------------------------------------------------------
-struct BatchPiState {
-  uint32_t t;      // iteration of the next step launch
-  uint32_t tR;     // iteration being reduced
-  uint32_t nDone;
-  uint32_t pad;
+// Blocks of poolStepKernel cover kPoolCols consecutive columns of one job (job-relative chunks),
+// so each block's partial sums depend only on that job's data.
+constexpr uint32_t kPoolCols = kBlock;
+
+struct PoolJobDev {
+  uint32_t lo, hi;  // pool positions
+  uint32_t t;       // next iteration of this job
+  int32_t done;
+  uint32_t final;
+  uint32_t b0;      // first step-kernel block of this job
+  double norm[3];
+  double resid;
 };
 
-static __global__ __launch_bounds__(kBlock) void batchPiStepKernel(const uint32_t* __restrict__ colPtr,
-                                                            const uint32_t* __restrict__ rowIdx,
-                                                            const double* __restrict__ val,
-                                                            const uint32_t* __restrict__ jobOfPos,
-                                                            const uint32_t* __restrict__ lo, uint32_t nR,
-                                                            double* y, uint32_t n, const double* norm /*[3][nR]*/,
-                                                            const int* done, double* absCol, double* resCol,
-                                                            BatchPiState* st) {
-  if (st->nDone >= nR) return;
-  const uint32_t t = st->t;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->tR = t;
+__host__ __device__ inline uint32_t poolBlocksOf(uint32_t len) { return (len + kPoolCols - 1) / kPoolCols; }
+
+// jobOfBlock[b] = r for the blocks of pool jobs r in [r0, r1)
+static __global__ __launch_bounds__(kBlock) void poolJobOfBlockKernel(const PoolJobDev* jobs, uint32_t r0, uint32_t r1,
+                                                                     uint32_t* jobOfBlock) {
+  const uint32_t r = r0 + blockIdx.x;
+  if (r >= r1) return;
+  const uint32_t b0 = jobs[r].b0, nb = poolBlocksOf(jobs[r].hi - jobs[r].lo);
+  for (uint32_t q = threadIdx.x; q < nb; q += kBlock) jobOfBlock[b0 + q] = r;
+}
+
+// y_t for kPoolCols columns of one job, one lane per column: the lane adds its column's products
+// in ascending-row order (the oracle's order) with kPoolUnroll entries' loads in flight at a time.
+// Also the block's partial sums of |y_t| and of the residual of iteration t-1
+// (sum |mu_{t-1} - mu_{t-2}|), mu_s = y_s / norm_s, mu_0 uniform.
+constexpr int kPoolUnroll = 8;
+
+static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* __restrict__ colPtr,
+                                                               const uint32_t* __restrict__ rowIdx,
+                                                               const double* __restrict__ val,
+                                                               const uint32_t* __restrict__ jobOfBlock,
+                                                               const PoolJobDev* __restrict__ jobs, double* y,
+                                                               uint32_t ys, double* partAbs, double* partRes) {
+  const PoolJobDev& J = jobs[jobOfBlock[blockIdx.x]];
+  if (J.done) return;  // uniform over the block
+  const uint32_t t = J.t;
   const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
-  const double* yPrev = y + uint64_t(s1) * n;
-  const double* yPrev2 = y + uint64_t(s2) * n;
-  double* yCur = y + uint64_t(s0) * n;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int sub = lane % kPiLanes, lead = lane - sub;
-  const uint32_t p = (blockIdx.x * kBlock + threadIdx.x) / kPiLanes;
-  uint32_t e0 = 0, e1 = 0, r = 0;
-  bool active = false;
-  if (p < n) {
-    r = jobOfPos[p];
-    active = !done[r];
-    if (active) {
-      e0 = colPtr[p];
-      e1 = colPtr[p + 1];
-    }
-  }
-  const double nPrev = active && t >= 2 ? norm[uint64_t(s1) * nR + r] : 1.0;
-  const double nPrev2 = active && t >= 3 ? norm[uint64_t(s2) * nR + r] : 1.0;
-  const double mu0 = active ? 1.0 / double(lo[r + 1] - lo[r]) : 0.0;
-  uint32_t trips = (e1 - e0 + kPiLanes - 1) / kPiLanes;
+  const double nPrev = t >= 2 ? J.norm[s1] : 1.0;
+  const double nPrev2 = t >= 3 ? J.norm[s2] : 1.0;
+  const double mu0 = 1.0 / double(J.hi - J.lo);
+  const uint32_t p = J.lo + (blockIdx.x - J.b0) * kPoolCols + threadIdx.x;
+  const double* yPrev = y + uint64_t(s1) * ys;
+  const double* yPrev2 = y + uint64_t(s2) * ys;
+  double sAbs = 0.0, sRes = 0.0;
+  if (p < J.hi) {
+    const uint32_t e1 = colPtr[p + 1];
+    double s = 0.0;
+    for (uint32_t e = colPtr[p]; e < e1; e += kPoolUnroll) {
+      uint32_t i[kPoolUnroll];
+      double v[kPoolUnroll], m[kPoolUnroll];
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) trips = max(trips, (uint32_t)__shfl_xor(trips, o, kWave));
-  double s = 0.0;
-  for (uint32_t q = 0; q < trips; ++q) {
-    const uint32_t e = e0 + q * kPiLanes + sub;
-    double prod = 0.0;
-    if (e < e1) {
-      const uint32_t i = rowIdx[e];
-      const double m = (t == 1) ? mu0 : yPrev[i] / nPrev;
-      prod = m * val[e];
-    }
+      for (int u = 0; u < kPoolUnroll; ++u)
+        if (e + u < e1) {
+          i[u] = rowIdx[e + u];
+          v[u] = val[e + u];
+        }
 #pragma unroll
-    for (int k = 0; k < kPiLanes; ++k) {
-      const double pk = __shfl(prod, lead + k, kWave);
-      if (e0 + q * kPiLanes + k < e1) s += pk;
+      for (int u = 0; u < kPoolUnroll; ++u)
+        if (e + u < e1) m[u] = (t == 1) ? mu0 : yPrev[i[u]] / nPrev;
+#pragma unroll
+      for (int u = 0; u < kPoolUnroll; ++u)
+        if (e + u < e1) {
+          const double prod = m[u] * v[u];
+          s += prod;
+        }
     }
-  }
-  if (active && sub == 0) {
-    yCur[p] = s;
-    absCol[p] = fabs(s);
-    double res = 0.0;
+    y[uint64_t(s0) * ys + p] = s;
+    sAbs = fabs(s);
     if (t >= 2) {
       const double a = yPrev[p] / nPrev;
       const double b = (t == 2) ? mu0 : yPrev2[p] / nPrev2;
-      res = fabs(a - b);
+      sRes = fabs(a - b);
     }
-    resCol[p] = res;
+  }
+  sAbs = blockReduceSumF64(sAbs);
+  sRes = blockReduceSumF64(sRes);
+  if (threadIdx.x == 0) {
+    partAbs[blockIdx.x] = sAbs;
+    partRes[blockIdx.x] = sRes;
   }
 }
 
-// One block per job: norm of y_t and residual of iteration t-1 over the job's columns.
-static __global__ __launch_bounds__(kBlock) void batchPiReduceKernel(const double* absCol, const double* resCol,
-                                                              const uint32_t* lo, uint32_t nR, double* norm,
-                                                              double* resid, int* done, uint32_t* finalIt,
-                                                              BatchPiState* st, uint32_t maxIter, double tol) {
+// One block per pool job: sums its block partials (fixed job-relative order), norm of y_t,
+// residual of iteration t-1, stopping decision, t += 1.
+static __global__ __launch_bounds__(kBlock) void poolReduceKernel(const double* partAbs, const double* partRes,
+                                                                 PoolJobDev* jobs, uint32_t nJobs, uint32_t maxIter,
+                                                                 double tol) {
   const uint32_t r = blockIdx.x;
-  if (st->nDone >= nR) return;
-  const uint32_t t = st->tR;
-  if (r == 0 && threadIdx.x == 0) st->t = t + 1;
-  if (r >= nR || done[r]) return;
-  const uint32_t a = lo[r], b = lo[r + 1];
+  if (r >= nJobs) return;
+  PoolJobDev& J = jobs[r];
+  if (J.done) return;
+  const uint32_t t = J.t, b0 = J.b0, nb = poolBlocksOf(J.hi - J.lo);
+  constexpr int U = 8;  // all of a thread's loads in flight before it sums them
   double ta = 0.0, tr = 0.0;
-  for (uint32_t p = a + threadIdx.x; p < b; p += kBlock) {
-    ta += absCol[p];
-    tr += resCol[p];
+  for (uint32_t q0 = 0; q0 < nb; q0 += kBlock * U) {
+    double va[U], vr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t q = q0 + u * kBlock + threadIdx.x;
+      va[u] = q < nb ? partAbs[b0 + q] : 0.0;
+      vr[u] = q < nb ? partRes[b0 + q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ta += va[u];
+      tr += vr[u];
+    }
   }
   ta = blockReduceSumF64(ta);
   tr = blockReduceSumF64(tr);
   if (threadIdx.x == 0) {
-    norm[uint64_t(t % 3) * nR + r] = ta;
+    J.norm[t % 3] = ta;
     const uint32_t completed = t - 1;
     if (completed >= 1) {
-      resid[r] = tr;
+      J.resid = tr;
       if ((tol >= 0.0 && tr <= tol) || completed >= maxIter) {
-        done[r] = 1;
-        finalIt[r] = completed;
-        atomicAdd(&st->nDone, 1u);
+        J.done = 1;
+        J.final = completed;
       }
     }
+    J.t = t + 1;
   }
 }
 
-static __global__ __launch_bounds__(kBlock) void batchPiFinishKernel(const double* y, uint32_t n, const uint32_t* jobOfPos,
-                                                              const uint32_t* lo, uint32_t nR, const double* norm,
-                                                              const uint32_t* finalIt, double* mu) {
+// mu of a finished pool job into out[0 .. hi-lo)
+static __global__ __launch_bounds__(kBlock) void poolFinishKernel(const double* y, uint32_t ys, const PoolJobDev* jobs,
+                                                                 uint32_t r, double* out) {
+  const PoolJobDev& J = jobs[r];
+  const uint32_t len = J.hi - J.lo;
+  const uint32_t f = J.final;
+  for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < len; q += gridDim.x * kBlock) {
+    const uint32_t p = J.lo + q;
+    out[q] = f == 0 ? 1.0 / double(len) : y[uint64_t(f % 3) * ys + p] / J.norm[f % 3];
+  }
+}
+
+// Pool compaction: position p of a kept job moves to p - shiftOfJob[job]; entries likewise.
+static __global__ __launch_bounds__(kBlock) void poolMoveColumnsKernel(const uint32_t* colPtrIn, const uint32_t* jobOfPosIn,
+                                                                      const double* yIn, uint32_t nIn, uint32_t ysIn,
+                                                                      const uint32_t* posShift, const uint32_t* nnzShift,
+                                                                      const int* keep, const uint32_t* newIndex,
+                                                                      uint32_t* colPtrOut, uint32_t* jobOfPosOut,
+                                                                      double* yOut, uint32_t ysOut) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t r = jobOfPos[p];
-  const uint32_t f = finalIt[r];
-  mu[p] = f == 0 ? 1.0 / double(lo[r + 1] - lo[r]) : y[uint64_t(f % 3) * n + p] / norm[uint64_t(f % 3) * nR + r];
+  if (p >= nIn) return;
+  const uint32_t r = jobOfPosIn[p];
+  if (!keep[r]) return;
+  const uint32_t q = p - posShift[r];
+  colPtrOut[q] = colPtrIn[p] - nnzShift[r];
+  jobOfPosOut[q] = newIndex[r];
+  for (int sl = 0; sl < 3; ++sl) yOut[uint64_t(sl) * ysOut + q] = yIn[uint64_t(sl) * ysIn + p];
+}
+
+static __global__ __launch_bounds__(kBlock) void poolMoveEntriesKernel(const uint32_t* rowIn, const double* valIn,
+                                                                      uint64_t nnzIn, const uint32_t* jobOfPosIn,
+                                                                      const uint32_t* posShift, const uint32_t* nnzShift,
+                                                                      const int* keep, uint32_t* rowOut,
+                                                                      double* valOut) {
+  const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= nnzIn) return;
+  const uint32_t row = rowIn[e];
+  const uint32_t r = jobOfPosIn[row];  // rows and columns of an entry belong to the same job
+  if (!keep[r]) return;
+  rowOut[e - nnzShift[r]] = row - posShift[r];
+  valOut[e - nnzShift[r]] = valIn[e];
 }
 
 static __global__ __launch_bounds__(kBlock) void jobOfPosKernel(const uint32_t* lo, uint32_t nR, uint32_t* jobOfPos) {

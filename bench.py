@@ -192,7 +192,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build's hot path runs on a single MI355X "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "(one GPU per process; sharding across GPUs is not part of it); run with --gpus 1", file=sys.stderr)
         return 2
 
     import b12_amd as gx

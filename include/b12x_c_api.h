@@ -178,15 +178,20 @@ int b12x_forest_add(b12x_forest f, const double* center, const double* radius, c
 /* one merged subdivision step for every active job, then retirement of finished jobs */
 int b12x_forest_step(b12x_forest f, uint32_t* n_active, uint32_t* n_retired);
 int b12x_forest_run(b12x_forest f);
-/* state: 0 queued, 1 active, 2 done, 3 released */
+/* state: 0 queued, 1 subdividing, 4 iterating its invariant measure, 2 done, 3 released */
 int b12x_forest_job_info(b12x_forest f, uint32_t job, int* state, uint32_t* n_leaves, uint64_t* nnz, int* iters,
                          double* resid);
 int b12x_forest_job_cover(b12x_forest f, uint32_t job, uint64_t* bits, uint8_t* depth, uint8_t* flags);
 int b12x_forest_job_matrix(b12x_forest f, uint32_t job, uint32_t* rows, uint32_t* cols, double* vals);
 int b12x_forest_job_measure(b12x_forest f, uint32_t job, double* mu);
 int b12x_forest_job_release(b12x_forest f, uint32_t job);
-/* HBM bytes held by the forest's leaf arrays, tables and unreleased results */
+/* HBM bytes held by the forest's leaf arrays, tables, measure pool and unreleased results */
 int b12x_forest_footprint(b12x_forest f, uint64_t* bytes);
+/* Telemetry, first n of: steps, seconds subdividing, seconds retiring (transition matrices),
+ * seconds iterating measures, measure iterations, and the pool's peak jobs, columns, entries. */
+int b12x_forest_stats(b12x_forest f, double* out, int n);
+/* Measure iterations the pool advances per forest step (default 64). */
+int b12x_forest_set_pool_iterations(b12x_forest f, int iters);
 /* Continuous-batching request loop over n_forests forests: request i arrives arrival[i] seconds
  * after the start (sorted), goes to forests[forest_of[i]] with centre/radius (stride doubles
  * apart), map_params (n_map_params each, may be NULL) and depth; it is admitted once it has

@@ -136,3 +136,26 @@ def test_serve_run_results_match_alone_runs():
         for a, b in zip(alone.matrix(ja), F.matrix(int(ids[i]))):
             assert np.array_equal(a, b)
         assert np.array_equal(alone.measure(ja), F.measure(int(ids[i])))
+
+
+def test_pool_graph_replays_match_eager_launches(monkeypatch):
+    """The pool's captured batches (replayed many times, recaptured as jobs join and leave) give
+    the same measures and iteration counts as eager launches."""
+    print("[forest] pool graph vs eager", flush=True)
+    grid = gx.InnerGrid(3)
+    c, r = [0.0, 0.0, 27.0], [30.0, 30.0, 30.0]
+    cases = [(28.0, 10), (24.5, 9), (31.0, 11), (26.0, 8), (29.5, 12)]
+    out = []
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("B12X_GRAPHS", graphs)
+        F = gx.Forest(3, grid, 2, pi_max_iter=100000, pi_tol=1e-12)
+        F.set_pool_iterations(16)
+        jobs = [F.add(c, r, gx.Lorenz(rho=rho), d) for rho, d in cases[:3]]
+        F.step()
+        jobs += [F.add(c, r, gx.Lorenz(rho=rho), d) for rho, d in cases[3:]]
+        F.run()
+        out.append([(F.measure(j), F.info(j)["iters"]) for j in jobs])
+        assert F.stats()["pool_iters"] > 16
+    for (ma, ia), (mb, ib) in zip(*out):
+        assert ia == ib
+        assert np.array_equal(ma, mb)
