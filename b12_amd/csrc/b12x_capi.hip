@@ -1064,6 +1064,8 @@ int b12x_coo_upload(const uint32_t* rows, const uint32_t* cols, const double* va
                     uint32_t nc, b12x_coo* out) {
   return guarded([&]() -> int {
     if (!out || (nnz && (!rows || !cols || !vals))) return setErr(B12X_E_ARG, "null argument");
+    for (uint64_t e = 0; e < nnz; ++e)
+      if (rows[e] >= nr || cols[e] >= nc) return setErr(B12X_E_ARG, "entry index outside the matrix");
     if (int e = ensureDevice()) return e;
     auto* M = new b12x_coo_s();
     try {
@@ -1128,6 +1130,31 @@ __global__ void gatherF64Kernel(const double* src, const uint32_t* idx, uint64_t
 __global__ void fillF64Kernel(double* p, uint32_t n, double v) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+__global__ void fillU32Kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void fillU64Kernel(uint64_t* p, uint64_t n, uint64_t v) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Blocks for the persistent power iteration over n columns: one lane per column up to
+// kCoopMaxBlocks blocks, never more than the device holds resident at once, so that on an idle
+// device every block runs from the start (0: use the batched form).
+uint32_t coopGrid(uint32_t n) {
+  int dev = 0, cus = 0, occ = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&piCoopKernel), kBlock, 0) !=
+          hipSuccess ||
+      occ <= 0)
+    return 0;
+  const uint64_t want = std::max<uint64_t>(1, ceilDiv(uint64_t(n), kBlock));
+  return uint32_t(std::min<uint64_t>({want, uint64_t(kCoopMaxBlocks), uint64_t(occ) * uint64_t(cus)}));
 }
 }  // namespace
 
@@ -1210,6 +1237,31 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
         }
       }
     };
+    // default: one persistent launch for the whole loop (B12X_PI_PERSISTENT=0: batches; =2: the
+    // persistent form or an error)
+    const char* pp = std::getenv("B12X_PI_PERSISTENT");
+    uint32_t coopBlocks = 0;
+    if (!fused && !init.done && !(pp && pp[0] == '0')) coopBlocks = coopGrid(n);
+    if (coopBlocks) {
+      const uint64_t nPart = uint64_t(coopBlocks) * 6;
+      uint64_t* part = dalloc<uint64_t>(nPart);
+      int* err = dalloc<int>(1);
+      hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nPart, kBlock)), dim3(kBlock), 0, st, part, nPart, kCoopEmpty);
+      hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)err, uint64_t(1), 0u);
+      HIPCHK(hipGetLastError());
+      PiCoopArgs ca{colPtr, cr, vals, mu0, ybuf, part, pst, err, n, uint32_t(std::max(maxIter, 0)), tol, 1u << 21};
+      hipLaunchKernelGGL(piCoopKernel, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
+      HIPCHK(hipGetLastError());
+      int herr = 0;
+      HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      dfree(part); dfree(err);
+      // a wait past its limit (other work holding the CUs): every block has left; recompute in
+      // batches (B12X_PI_PERSISTENT=2: report it instead)
+      if (herr && pp && pp[0] == '2') throw HipFail(B12X_E_HIP, "persistent power iteration: a wait passed its limit");
+      if (!herr) init.done = 1;
+      else HIPCHK(hipMemcpyAsync(pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    }
     if (useGraph && !init.done) {
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       enqueueBatch();

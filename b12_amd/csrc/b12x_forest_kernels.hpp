@@ -228,24 +228,26 @@ static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* 
   if (p < J.hi) {
     const uint32_t e1 = colPtr[p + 1];
     double s = 0.0;
+    // branch-free body (entry indices clamped to the column, values and sums selected) so that
+    // all of a lane's gathers are in flight at once; y is always allocated, so the unused read of
+    // iteration 1 stays in bounds
     for (uint32_t e = colPtr[p]; e < e1; e += kPoolUnroll) {
       uint32_t i[kPoolUnroll];
       double v[kPoolUnroll], m[kPoolUnroll];
 #pragma unroll
-      for (int u = 0; u < kPoolUnroll; ++u)
-        if (e + u < e1) {
-          i[u] = rowIdx[e + u];
-          v[u] = val[e + u];
-        }
+      for (int u = 0; u < kPoolUnroll; ++u) {
+        const uint32_t eu = min(e + u, e1 - 1);
+        i[u] = rowIdx[eu];
+        v[u] = val[eu];
+      }
 #pragma unroll
-      for (int u = 0; u < kPoolUnroll; ++u)
-        if (e + u < e1) m[u] = (t == 1) ? mu0 : yPrev[i[u]] / nPrev;
+      for (int u = 0; u < kPoolUnroll; ++u) m[u] = yPrev[i[u]];
 #pragma unroll
-      for (int u = 0; u < kPoolUnroll; ++u)
-        if (e + u < e1) {
-          const double prod = m[u] * v[u];
-          s += prod;
-        }
+      for (int u = 0; u < kPoolUnroll; ++u) {
+        const double mu = (t == 1) ? mu0 : m[u] / nPrev;
+        const double prod = mu * v[u];
+        s = (e + u < e1) ? s + prod : s;
+      }
     }
     y[uint64_t(s0) * ys + p] = s;
     sAbs = fabs(s);

@@ -920,6 +920,176 @@ static __global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t*
 }
 
 // mu = y_final / ||y_final||_1 (or mu0 when no iteration ran)
+// Persistent form of the loop: one launch whose grid fits the device at once (every block is
+// resident for the whole run on an otherwise idle device).
+// Per iteration each block computes y_t for its columns (one lane per column, products added in
+// ascending-row order as the oracle does) and publishes its partial sums of |y_t| and of the
+// previous residual in its slot of buffer t % 3. A slot holds kCoopEmpty until written; every
+// block polls all slots of the buffer until none is empty, then sums them in block order, so all
+// blocks derive the same norm and the same stopping decision. A block clears its slot of buffer
+// (t+1) % 3 at the start of iteration t: that buffer was last read in iteration t-2, which every
+// block has finished once the wait of iteration t-1 is over. y rotates over 3 slots likewise.
+// Every wait is bounded: past spinLimit polls a block raises *err and leaves, and the others
+// leave at their next wait (the host then recomputes with the batched form).
+// Coherence across XCDs: everything written during the run (y, partials) is written and read
+// with agent-scope relaxed atomics (coherent without L2 write-back or invalidation), and each wave
+// waits for its stores to complete before the block publishes. The matrix is read-only and stays
+// cached in L2 across iterations.
+constexpr uint32_t kCoopMaxBlocks = 512;
+constexpr int kCoopUnroll = 8;
+constexpr uint64_t kCoopEmpty = 0xFFF4B12B12B12B12ull;  // a NaN pattern arithmetic never produces
+
+struct PiCoopArgs {
+  const uint32_t* colPtr;
+  const uint32_t* rowIdx;
+  const double* val;
+  const double* mu0;
+  double* y;          // 3 * n
+  uint64_t* part;     // 3 buffers * gridDim.x slots * 2, all kCoopEmpty at launch
+  PiState* st;
+  int* err;           // zero at launch
+  uint32_t n;
+  uint32_t maxIter;
+  double tol;
+  uint32_t spinLimit;
+};
+
+template <class T>
+__device__ inline T ldAgent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void stAgent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
+  constexpr uint32_t kSlots = kCoopMaxBlocks / kWave;  // slots per polling lane
+  const uint32_t G = gridDim.x, b = blockIdx.x, tid = threadIdx.x, n = a.n;
+  __shared__ double sums[2];
+  __shared__ int leave;
+  if (a.st->done) return;
+  uint32_t t = a.st->t;
+  double nPrev = t >= 2 ? a.st->norm[(t + 2) % 3] : 1.0;  // norm of y_{t-1}
+  double nPrev2 = t >= 3 ? a.st->norm[(t + 1) % 3] : 1.0;  // norm of y_{t-2}
+  for (;;) {
+    if (tid == 0) {
+      uint64_t* nextSlot = a.part + (uint64_t((t + 1) % 3) * G + b) * 2;
+      stAgent(nextSlot, kCoopEmpty);
+      stAgent(nextSlot + 1, kCoopEmpty);
+    }
+    const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
+    const double* yPrev = a.y + uint64_t(s1) * n;
+    const double* yPrev2 = a.y + uint64_t(s2) * n;
+    double* yCur = a.y + uint64_t(s0) * n;
+    // iteration 1 multiplies by mu_0 itself: mu_0[i] / 1.0 is mu_0[i] exactly
+    const double* src = (t == 1) ? a.mu0 : yPrev;
+    const double div = (t == 1) ? 1.0 : nPrev;
+    double sAbs = 0.0, sRes = 0.0;
+    for (uint32_t p = b * kBlock + tid; p < n; p += G * kBlock) {
+      const uint32_t e1 = a.colPtr[p + 1];
+      double s = 0.0;
+      // branch-free body (entry indices clamped to the column, sums selected) so that all of a
+      // lane's gathers are in flight at once
+      for (uint32_t e = a.colPtr[p]; e < e1; e += kCoopUnroll) {
+        uint32_t i[kCoopUnroll];
+        double v[kCoopUnroll], m[kCoopUnroll];
+#pragma unroll
+        for (int u = 0; u < kCoopUnroll; ++u) {
+          const uint32_t eu = min(e + u, e1 - 1);
+          i[u] = a.rowIdx[eu];
+          v[u] = a.val[eu];
+        }
+#pragma unroll
+        for (int u = 0; u < kCoopUnroll; ++u) m[u] = ldAgent(src + i[u]);
+#pragma unroll
+        for (int u = 0; u < kCoopUnroll; ++u) {
+          const double prod = (m[u] / div) * v[u];
+          s = (e + u < e1) ? s + prod : s;
+        }
+      }
+      stAgent(yCur + p, s);
+      sAbs += fabs(s);
+      if (t >= 2) {
+        const double yp = ldAgent(yPrev + p), yp2 = ldAgent(((t == 2) ? a.mu0 : yPrev2) + p);
+        const double x = yp / nPrev;
+        const double z = yp2 / ((t == 2) ? 1.0 : nPrev2);
+        sRes += fabs(x - z);
+      }
+    }
+    sAbs = blockReduceSumF64(sAbs);
+    sRes = blockReduceSumF64(sRes);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's y stores have completed
+    __syncthreads();
+    uint64_t* buf = a.part + uint64_t(t % 3) * G * 2;
+    if (tid == 0) {
+      stAgent(buf + 2 * b, uint64_t(__double_as_longlong(sAbs)));
+      stAgent(buf + 2 * b + 1, uint64_t(__double_as_longlong(sRes)));
+    }
+    if (tid < kWave) {
+      // lane l holds slots l, l + 64, ... ; sums in that order, then a fixed shuffle tree
+      uint64_t va[kSlots], vr[kSlots];
+#pragma unroll
+      for (uint32_t k = 0; k < kSlots; ++k) va[k] = vr[k] = 0;
+      bool ok = false;
+      for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
+        bool mine = true;
+#pragma unroll
+        for (uint32_t k = 0; k < kSlots; ++k) {
+          const uint32_t q = tid + k * kWave;
+          if (q < G && (va[k] == kCoopEmpty || vr[k] == kCoopEmpty || spin == 0)) {
+            va[k] = ldAgent(buf + 2 * q);
+            vr[k] = ldAgent(buf + 2 * q + 1);
+          }
+          mine = mine && (q >= G || (va[k] != kCoopEmpty && vr[k] != kCoopEmpty));
+        }
+        ok = __all(mine);
+        if (ok || ldAgent(a.err)) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      double ta = 0.0, tr = 0.0;
+#pragma unroll
+      for (uint32_t k = 0; k < kSlots; ++k)
+        if (tid + k * kWave < G) {
+          ta += __longlong_as_double((long long)va[k]);
+          tr += __longlong_as_double((long long)vr[k]);
+        }
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        ta += __shfl_xor(ta, o, kWave);
+        tr += __shfl_xor(tr, o, kWave);
+      }
+      if (tid == 0) {
+        if (!ok) stAgent(a.err, 1);
+        leave = ok ? 0 : 1;
+        sums[0] = ta;
+        sums[1] = tr;
+      }
+    }
+    __syncthreads();
+    if (leave) return;
+    const double ta = sums[0], tr = sums[1];
+    const uint32_t completed = t - 1;
+    const bool done = completed >= 1 && ((a.tol >= 0.0 && tr <= a.tol) || completed >= a.maxIter);
+    if (done) {
+      if (b == 0 && tid == 0) {
+        a.st->norm[t % 3] = ta;
+        a.st->norm[(t + 2) % 3] = nPrev;
+        a.st->norm[(t + 1) % 3] = nPrev2;
+        a.st->resid = tr;
+        a.st->final = completed;
+        a.st->t = t + 1;
+        a.st->done = 1;
+      }
+      return;
+    }
+    nPrev2 = nPrev;
+    nPrev = ta;
+    ++t;
+    __syncthreads();  // sums[] is rewritten by the next wait
+  }
+}
+
 static __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* y, const double* mu0, uint32_t n,
                                                          const PiState* st, double* mu) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;

@@ -246,6 +246,7 @@ def test_power_iteration_graph_replays_match_eager(monkeypatch):
     give bit-identical measures to eagerly launched iterations."""
     g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 11)
     m = g.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon())
+    monkeypatch.setenv("B12X_PI_PERSISTENT", "0")
     mu_graph, it_graph, r_graph = m.power_iteration(700)
     monkeypatch.setenv("B12X_GRAPHS", "0")
     mu_eager, it_eager, r_eager = m.power_iteration(700)
@@ -277,6 +278,46 @@ def test_power_iteration_fused_and_two_kernel_paths_agree(monkeypatch):
         c, ic, _ = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()).power_iteration(mi)
         assert ia == ib == ic == mi
         assert np.allclose(a, c, rtol=MEASURE_RTOL, atol=0) and np.allclose(b, c, rtol=MEASURE_RTOL, atol=0)
+
+
+def test_persistent_power_iteration_agrees_with_batched_form(monkeypatch):
+    """The single-launch (cooperative) power iteration against the batched step+reduce form and
+    the oracle: a subdivision matrix and an uploaded 300k-column stochastic matrix (more columns
+    than the persistent grid has lanes), iteration limits 1, 2, 3, 50 and a 1e-12 tolerance."""
+    print("[pi] persistent vs batched", flush=True)
+    g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 13)
+    cases = [(g.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()),
+              o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()))]
+    rng = np.random.default_rng(11)
+    n, per = 300_000, 4
+    rows = np.repeat(np.arange(n, dtype=np.uint32), per)
+    cols = rng.integers(0, n, n * per).astype(np.uint32)
+    w = rng.random(n * per)
+    vals = w / np.repeat(np.add.reduceat(w, np.arange(0, n * per, per)), per)
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    keep = np.ones(len(rows), bool)
+    keep[1:] = (rows[1:] != rows[:-1]) | (cols[1:] != cols[:-1])
+    rows, cols, vals = rows[keep], cols[keep], vals[keep]
+    from oracle_api import OCoo
+    cases.append((gx.CooMatrix.from_arrays(rows, cols, vals, n, n), OCoo.from_arrays(rows, cols, vals, n, n)))
+    for mg, mo in cases:
+        for mi, tol in ((1, -1.0), (2, -1.0), (3, -1.0), (50, -1.0), (100000, 1e-12)):
+            monkeypatch.setenv("B12X_PI_PERSISTENT", "2")  # persistent form or an error, no fallback
+            a, ia, ra = mg.power_iteration(mi, tol)
+            a2, ia2, _ = mg.power_iteration(mi, tol)
+            monkeypatch.setenv("B12X_PI_PERSISTENT", "0")
+            b, ib, rb = mg.power_iteration(mi, tol)
+            monkeypatch.delenv("B12X_PI_PERSISTENT")
+            assert ia == ia2 and np.array_equal(a, a2)
+            if tol < 0:
+                c, ic, _ = mo.power_iteration(mi, tol)
+                assert ia == ib == ic == mi
+                assert np.allclose(a, c, rtol=MEASURE_RTOL, atol=0.0)
+                assert np.allclose(a, b, rtol=MEASURE_RTOL, atol=0.0)
+            else:
+                assert abs(ia - ib) <= 1 and ra <= tol and rb <= tol
+                assert np.allclose(a, b, rtol=1e-10, atol=1e-16)
 
 
 def test_cpp_facade_doc_example():
