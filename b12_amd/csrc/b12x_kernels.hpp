@@ -937,6 +937,7 @@ static __global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t*
 // cached in L2 across iterations.
 constexpr uint32_t kCoopMaxBlocks = 512;
 constexpr int kCoopUnroll = 8;
+constexpr int kCoopCache = 3 * kCoopUnroll;  // entries of a lane's first column held in registers
 constexpr uint64_t kCoopEmpty = 0xFFF4B12B12B12B12ull;  // a NaN pattern arithmetic never produces
 
 struct PiCoopArgs {
@@ -972,6 +973,29 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
   uint32_t t = a.st->t;
   double nPrev = t >= 2 ? a.st->norm[(t + 2) % 3] : 1.0;  // norm of y_{t-1}
   double nPrev2 = t >= 3 ? a.st->norm[(t + 1) % 3] : 1.0;  // norm of y_{t-2}
+  // the lane's first column keeps its first kCoopCache entries in registers for the whole run
+  // (indices clamped into the column; an empty column holds row 0 and adds nothing)
+  const uint32_t p0 = b * kBlock + tid;
+  uint32_t c0 = 0, c1 = 0;
+  uint32_t ci[kCoopCache];
+  double cv[kCoopCache];
+#pragma unroll
+  for (int u = 0; u < kCoopCache; ++u) {
+    ci[u] = 0;
+    cv[u] = 0.0;
+  }
+  if (p0 < n) {
+    c0 = a.colPtr[p0];
+    c1 = a.colPtr[p0 + 1];
+    if (c1 > c0) {
+#pragma unroll
+      for (int u = 0; u < kCoopCache; ++u) {
+        const uint32_t eu = min(c0 + u, c1 - 1);
+        ci[u] = a.rowIdx[eu];
+        cv[u] = a.val[eu];
+      }
+    }
+  }
   for (;;) {
     if (tid == 0) {
       uint64_t* nextSlot = a.part + (uint64_t((t + 1) % 3) * G + b) * 2;
@@ -986,12 +1010,40 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
     const double* src = (t == 1) ? a.mu0 : yPrev;
     const double div = (t == 1) ? 1.0 : nPrev;
     double sAbs = 0.0, sRes = 0.0;
-    for (uint32_t p = b * kBlock + tid; p < n; p += G * kBlock) {
-      const uint32_t e1 = a.colPtr[p + 1];
+    for (uint32_t p = p0; p < n; p += G * kBlock) {
+      // residual inputs first, unconditionally (y is always allocated; iteration 1 ignores them),
+      // so that their loads overlap the gathers
+      const double yp = ldAgent(yPrev + p);
+      const double yp2 = ldAgent(((t == 2) ? a.mu0 : yPrev2) + p);
       double s = 0.0;
+      uint32_t e, e1;
+      if (p == p0) {
+        // cached entries: groups of kCoopUnroll gathers, each issued where some column of the
+        // wave reaches it (clamped indices keep the others in bounds), all in flight together
+        double m[kCoopCache];
+#pragma unroll
+        for (int u = 0; u < kCoopCache; ++u) m[u] = 0.0;
+#pragma unroll
+        for (int g = 0; g < kCoopCache; g += kCoopUnroll) {
+          if (__any(c0 + g < c1)) {  // wave-uniform: no wait at the join
+#pragma unroll
+            for (int u = g; u < g + kCoopUnroll; ++u) m[u] = ldAgent(src + ci[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kCoopCache; ++u) {
+          const double prod = (m[u] / div) * cv[u];
+          s = (c0 + u < c1) ? s + prod : s;
+        }
+        e = c0 + kCoopCache;
+        e1 = c1;
+      } else {
+        e = a.colPtr[p];
+        e1 = a.colPtr[p + 1];
+      }
       // branch-free body (entry indices clamped to the column, sums selected) so that all of a
       // lane's gathers are in flight at once
-      for (uint32_t e = a.colPtr[p]; e < e1; e += kCoopUnroll) {
+      for (; e < e1; e += kCoopUnroll) {
         uint32_t i[kCoopUnroll];
         double v[kCoopUnroll], m[kCoopUnroll];
 #pragma unroll
@@ -1011,7 +1063,6 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
       stAgent(yCur + p, s);
       sAbs += fabs(s);
       if (t >= 2) {
-        const double yp = ldAgent(yPrev + p), yp2 = ldAgent(((t == 2) ? a.mu0 : yPrev2) + p);
         const double x = yp / nPrev;
         const double z = yp2 / ((t == 2) ? 1.0 : nPrev2);
         sRes += fabs(x - z);
