@@ -49,6 +49,7 @@ SIGNATURES = {
     "b12x_search_by_points": [_P, _P, _U64, _U8, _P],
     "b12x_subdivision_steps": [_P, _P, _U8, _I, _P],
     "b12x_tree_image_timing": [_P, _I, _P, _P, _P],
+    "b12x_bench_copy": [ctypes.c_uint64, _I, _P],
     "b12x_scheme_create": [_I, _I, _P, _I, _I, _P, _I, _P],
     "b12x_scheme_destroy": [_P],
     "b12x_scheme_points_per_box": [_P, _P],
@@ -120,6 +121,13 @@ class _Api:
         rc = getattr(self.lib, name)(*args)
         if rc != 0:
             raise B12xError(f"{name}: {self.lib.b12x_last_error().decode()} (status {rc})")
+
+
+def bench_copy(nbytes, reps=20, flavour="parity"):
+    """Average milliseconds of a plain device copy of nbytes (b12x_bench_copy)."""
+    ms = _D()
+    _Api(flavour)("b12x_bench_copy", int(nbytes), int(reps), ctypes.byref(ms))
+    return ms.value
 
 
 def device_name(flavour="parity"):

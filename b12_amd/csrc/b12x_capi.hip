@@ -50,6 +50,7 @@ struct b12x_tree_s {
   uint32_t* scal = nullptr;    // device scalars
   uint32_t* hscal = nullptr;   // pinned host mirror of scal
   uint64_t* rootKey = nullptr; // the surrounding box as a one-element source
+  uint64_t* samples = nullptr; // descent samples of the leaf keys (kDescentSamples)
   bool timing = false;          // HIP-event timing of image launches
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   std::vector<hipEvent_t> eventPool;
@@ -70,6 +71,7 @@ struct b12x_tree_s {
     dfree(scal);
     dfree(rootKey);
     dfree(rootDepth);
+    dfree(samples);
     if (hscal) (void)hipHostFree(hscal);
     for (auto& p : pending) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
     for (auto e : eventPool) (void)hipEventDestroy(e);
@@ -269,6 +271,20 @@ Source makeSource(b12x_tree_s& T, uint8_t dIt, uint8_t check) {
   return {T.key[T.cur], T.depth[T.cur], T.list, T.hscal[0]};
 }
 
+// Writes the descent samples of the tree's current leaves (count from nDev when given) and
+// returns them for ImageArgs::samples. Opt-in (B12X_DESCENT_LDS=1): on MI355X the upper levels
+// of the all-global search stay in L1/L2 anyway, and the LDS fill per block plus the sampling
+// launch per level measured slightly slower on the Lorenz subdivision (null: global descent).
+const uint64_t* stageSamples(b12x_tree_s& T, const uint32_t* nDev) {
+  const char* e = std::getenv("B12X_DESCENT_LDS");
+  if (!(e && e[0] == '1')) return nullptr;
+  if (!T.samples) T.samples = dalloc<uint64_t>(kDescentSamples);
+  hipLaunchKernelGGL(sampleKeysKernel, dim3(kDescentSamples / kBlock), dim3(kBlock), 0, T.stream,
+                     (const uint64_t*)T.key[T.cur], T.n, nDev, T.samples);
+  HIPCHK(hipGetLastError());
+  return T.samples;
+}
+
 template <int DIM, int MAP>
 void launchImage2(const ImageArgs& a, int gridKind, hipStream_t st) {
   const uint64_t blocks64 = (a.nPoints + kBlock - 1) / kBlock;
@@ -317,6 +333,7 @@ void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, 
   a.hit = T.hit;
   a.out = out;
   if (a.nPoints == 0) return;
+  if (!depth0 && T.n > 0) a.samples = stageSamples(T, nullptr);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (T.timing) {
     for (hipEvent_t* e : {&e0, &e1}) {
@@ -401,6 +418,34 @@ uint32_t stencilOp(b12x_tree_s& T, int op, const uint8_t* hostStencil, uint8_t a
 extern "C" {
 
 const char* b12x_last_error(void) { return lastError().c_str(); }
+
+int b12x_bench_copy(uint64_t bytes, int reps, double* ms) {
+  return guarded([&]() -> int {
+    if (!ms || reps < 1 || bytes < 16 || bytes % 16) return setErr(B12X_E_ARG, "bytes: a positive multiple of 16");
+    if (int e = ensureDevice()) return e;
+    const uint64_t n = bytes / 16;
+    uint4* a = dalloc<uint4>(n);
+    uint4* b = dalloc<uint4>(n);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const uint32_t grid = uint32_t(std::min<uint64_t>(ceilDiv(n, kBlock), 65536));
+    hipLaunchKernelGGL(copyKernel, dim3(grid), dim3(kBlock), 0, nullptr, (const uint4*)a, b, n);  // first touch
+    HIPCHK(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(copyKernel, dim3(grid), dim3(kBlock), 0, nullptr, (const uint4*)a, b, n);
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = double(t) / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    return B12X_OK;
+  });
+}
 
 int b12x_device_name(char* buf, int len) {
   return guarded([&]() -> int {
@@ -777,6 +822,7 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
       a.nBoxesDev = T.scal + 0;
       a.mode = 0;
       a.hit = T.hit;
+      a.samples = stageSamples(T, T.scal + 0);
       {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (T.timing) {

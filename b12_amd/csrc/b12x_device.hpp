@@ -419,4 +419,52 @@ __device__ inline uint32_t locate(const uint64_t* __restrict__ keys, const uint8
   return kInvalid;
 }
 
+// Descent with the upper levels staged in LDS. sm[0..S) holds the leaf keys at positions
+// pos(j) = floor(j * n / S), S = min(kDescentSamples, n) (sampleKeysKernel writes them). The
+// count j of samples below the point key brackets the lower bound: it is 0 when j = 0, else it
+// lies in (pos(j-1), pos(j)] (pos(S) = n), where the global search finishes. S = n makes the LDS
+// search exact. Same result as locate().
+constexpr uint32_t kDescentSamples = 2048;
+
+__device__ inline uint32_t samplePos(uint32_t j, uint32_t n, uint32_t S) {
+  return uint32_t((uint64_t(j) * n) / S);
+}
+
+__device__ inline uint32_t locateStaged(const uint64_t* __restrict__ keys, const uint8_t* __restrict__ depth,
+                                        uint32_t n, uint64_t pk, const uint64_t* sm, uint32_t S) {
+  if (pk == kInvalidKey) {
+    const uint32_t l = n - 1;
+    return bitsOf(keys[l], depth[l]) == 0ull ? l : kInvalid;
+  }
+  uint32_t lo = 0, len = S;
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (sm[lo + half] < pk) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  uint32_t s = 0;
+  if (lo > 0) {
+    s = samplePos(lo - 1, n, S) + 1;
+    len = (lo < S ? samplePos(lo, n, S) : n) - s;
+    while (len > 0) {
+      const uint32_t half = len >> 1;
+      if (keys[s + half] < pk) {
+        s += half + 1;
+        len -= half + 1;
+      } else {
+        len = half;
+      }
+    }
+  }
+  const uint32_t i1 = s > 0 ? s - 1 : 0;
+  const uint32_t i2 = s < n - 1 ? s : n - 1;
+  if (leafContains(pk, keys[i1], depth[i1])) return i1;
+  if (leafContains(pk, keys[i2], depth[i2])) return s;
+  return kInvalid;
+}
+
 }  // namespace b12x

@@ -374,10 +374,21 @@ struct ImageArgs {
   int depth0;               // search on depth 0 (root only)
   uint8_t* hit;
   uint32_t* out;
+  const uint64_t* samples;  // descent samples of key[0..n) (sampleKeysKernel), or null
 };
 
+// samples[j] = keys[pos(j)], j < min(kDescentSamples, n); n from nDev when given
+static __global__ __launch_bounds__(kBlock) void sampleKeysKernel(const uint64_t* keys, uint32_t n0,
+                                                                 const uint32_t* nDev, uint64_t* samples) {
+  const uint32_t n = nDev ? *nDev : n0;
+  const uint32_t S = min(kDescentSamples, n);
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j < S) samples[j] = keys[samplePos(j, n, S)];
+}
+
 template <int DIM, int MAP, int GRID>
-__device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, uint64_t b, uint32_t gl, uint32_t n) {
+__device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, uint64_t b, uint32_t gl, uint32_t n,
+                                  const uint64_t* sm, uint32_t S) {
   const uint32_t src = a.srcList ? a.srcList[b] : uint32_t(b);
   double c[DIM], rb[DIM], g[DIM], x[DIM], y[DIM];
   boxCoords<DIM>(a.dom, a.srcKey[src], a.srcDepth[src], c, rb);
@@ -393,6 +404,7 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
   const uint64_t pk = pointKey<DIM>(a.dom, y);
   uint32_t t;
   if (a.depth0) t = (pk == kInvalidKey) ? kInvalid : 0u;
+  else if (S) t = locateStaged(a.key, a.depth, n, pk, sm, S);
   else t = locate(a.key, a.depth, n, pk);
   if (a.mode == 0) {
     if (t != kInvalid) a.hit[t] = 1;
@@ -403,8 +415,17 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
 
 template <int DIM, int MAP, int GRID>
 static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
+  __shared__ uint64_t sm[kDescentSamples];
   const uint64_t nPoints = a.nBoxesDev ? uint64_t(*a.nBoxesDev) * a.grid.k : a.nPoints;
   const uint32_t n = a.nBoxesDev ? *a.nBoxesDev : a.n;
+  if (uint64_t(blockIdx.x) * kBlock >= nPoints) return;  // grid sized for a bound: no points here
+  // upper levels of the descent: the samples of the leaf keys, staged in LDS
+  uint32_t S = 0;
+  if (a.samples && !a.depth0 && n > 0) {
+    S = min(kDescentSamples, n);
+    for (uint32_t j = threadIdx.x; j < S; j += kBlock) sm[j] = a.samples[j];
+    __syncthreads();
+  }
   if (a.pointBase + nPoints <= 0xFFFFFFFFull && uint64_t(gridDim.x) * kBlock <= 0xFFFFFFFFull) {
     // 32-bit index arithmetic (the common case)
     const uint32_t k = a.grid.k, base = uint32_t(a.pointBase), np = uint32_t(nPoints);
@@ -412,14 +433,14 @@ static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < np; i += stride) {
       const uint32_t gi = base + i;
       const uint32_t b = gi / k;
-      imagePoint<DIM, MAP, GRID>(a, i, gi, b, gi - b * k, n);
+      imagePoint<DIM, MAP, GRID>(a, i, gi, b, gi - b * k, n, sm, S);
     }
   } else {
     const uint64_t stride = uint64_t(gridDim.x) * kBlock;
     for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < nPoints; i += stride) {
       const uint64_t gi = a.pointBase + i;
       const uint64_t b = gi / a.grid.k;
-      imagePoint<DIM, MAP, GRID>(a, i, gi, b, uint32_t(gi - b * a.grid.k), n);
+      imagePoint<DIM, MAP, GRID>(a, i, gi, b, uint32_t(gi - b * a.grid.k), n, sm, S);
     }
   }
 }
@@ -1139,6 +1160,12 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
     ++t;
     __syncthreads();  // sums[] is rewritten by the next wait
   }
+}
+
+// Plain copy (traffic-counter calibration): every byte read once and written once.
+static __global__ __launch_bounds__(kBlock) void copyKernel(const uint4* in, uint4* out, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock)
+    out[i] = in[i];
 }
 
 static __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* y, const double* mu0, uint32_t n,

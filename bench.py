@@ -189,6 +189,8 @@ def main():
     ap.add_argument("--cpu-depth", type=int, default=24)
     ap.add_argument("--sweep-jobs", type=int, default=512, help="config 4 jobs (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="config 5 requests (0 to skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only time a plain device copy of BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build's hot path runs on a single MI355X "
@@ -197,6 +199,11 @@ def main():
 
     import b12_amd as gx
 
+    if args.copy_calibration:
+        ms = gx.bench_copy(args.copy_calibration, 20)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "launches": 21, "ms_per_copy": ms,
+                          "GB_per_s": 2 * args.copy_calibration / (ms * 1e-3) / 1e9}), flush=True)
+        return 0
     dev = gx.device_name()
     for _ in range(args.warmup):
         gpu_job(gx, args.flavour)

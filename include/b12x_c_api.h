@@ -129,6 +129,11 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd_flag, int n_st
  * since then are returned (milliseconds, launches, test points). enable=0 disables timing. */
 int b12x_tree_image_timing(b12x_tree t, int enable, double* ms, uint64_t* launches, uint64_t* points);
 
+/* Measurement hook (no counterpart in the reference): `reps` launches of a plain device copy of
+ * `bytes` bytes (16-byte vector loads and stores) between two fresh buffers; the average launch
+ * time in milliseconds by HIP events. Used to calibrate HBM traffic counters. */
+int b12x_bench_copy(uint64_t bytes, int reps, double* ms);
+
 /* --- scheme ----------------------------------------------------------------------------- */
 int b12x_scheme_create(int dim, int grid_kind, const uint64_t* grid_params, int n_grid_params, int map_kind,
                        const double* map_params, int n_map_params, b12x_scheme* out);

@@ -96,6 +96,32 @@ def test_search_by_points_and_invalid_quirk():
     assert np.array_equal(rg, ro) and rg[0] == 0 and rg[2] == gx.INVALID
 
 
+@pytest.mark.parametrize("n", [1, 3, 2047, 2048, 2049, 6000, 30000])
+def test_descent_staged_in_lds_matches_global_descent(monkeypatch, n):
+    """The descent with its upper levels staged in LDS (samples of the leaf keys) gives the same
+    leaf indices as the all-global binary search and as the oracle, on trees with holes of sizes
+    around the sample count, with points outside the domain and on the domain edge."""
+    print(f"[descent] n={n}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    bits, depth, flags = random_tree(rng, 2, int(n * 1.3) + 2, max_depth=22)
+    pick = rng.permutation(len(bits))[:n]
+    bits, depth, flags = bits[pick], depth[pick], flags[pick]
+    g, o = both([0.0, 0.0], [1.5, 1.5])
+    g.upload(bits, depth, flags)
+    o.upload(bits, depth, flags)
+    assert g.size() == n
+    mp = gx.Henon(1.4, 0.3, 1)
+    ig = g.makePointIteratorPairOverLeaves(gx.InnerGrid(4), mp)
+    io = o.makePointIteratorPairOverLeaves(gx.InnerGrid(4), mp)
+    ref = o.searchByIterators(io)
+    plain = g.searchByIterators(ig)
+    monkeypatch.setenv("B12X_DESCENT_LDS", "1")
+    staged = g.searchByIterators(ig)
+    assert np.array_equal(g.searchCompacted(ig), o.searchCompacted(io))
+    monkeypatch.delenv("B12X_DESCENT_LDS")
+    assert np.array_equal(staged, ref) and np.array_equal(plain, ref)
+
+
 # ---------------------------------------------------------------------------------------------
 def subdivision_run(center, radius, grid, mp, steps):
     g, o = both(center, radius)
