@@ -67,12 +67,20 @@ struct Pool {
   uint32_t* jobOfPos = nullptr;  // capN
   double* y = nullptr;           // 3 slots of stride capN
   uint32_t* rowIdx = nullptr;    // capE
-  double* vals = nullptr;
+  uint8_t* cnt = nullptr;        // capE: hit count of each entry (value = table[count])
+  double* table = nullptr;       // kPoolTable values count / k
   uint32_t* jobOfBlock = nullptr;  // capB: pool job of each step-kernel block
   double* partAbs = nullptr;       // capB
   double* partRes = nullptr;
+  uint64_t* coopPart = nullptr;    // 6 * capB: partial slots of the persistent form
+  int* coopErr = nullptr;
+  bool coopOff = false;            // a persistent wait passed its limit: batched form from then on
+  bool coopInFlight = false;
   PoolJobDev* dJobs = nullptr;  // capJ
   std::vector<PoolJob> jobs;
+  hipStream_t stream = nullptr;  // the pool's own stream: its batches overlap the subdivision steps
+  bool inFlight = false;         // a batch has been launched and not yet harvested
+  double tLaunch = 0.0;
   // captured batch of iterations for the current pool layout (recaptured after joins and exits)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -85,12 +93,19 @@ struct Pool {
     graphIters = 0;
   }
   void freeAll() {
+    if (stream) (void)hipStreamSynchronize(stream);
     dropGraph();
-    dfree(colPtr); dfree(jobOfPos); dfree(y); dfree(rowIdx); dfree(vals); dfree(jobOfBlock); dfree(partAbs);
-    dfree(partRes); dfree(dJobs);
+    dfree(colPtr); dfree(jobOfPos); dfree(y); dfree(rowIdx); dfree(cnt); dfree(table); dfree(jobOfBlock);
+    dfree(partAbs); dfree(partRes); dfree(dJobs); dfree(coopPart); dfree(coopErr);
+    cnt = nullptr;
+    table = nullptr;
+    coopPart = nullptr;
+    coopErr = nullptr;
     colPtr = jobOfPos = rowIdx = jobOfBlock = nullptr;
-    y = vals = partAbs = partRes = nullptr;
+    y = partAbs = partRes = nullptr;
     dJobs = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
   }
 };
 
@@ -297,6 +312,31 @@ void admitWaiting(b12x_forest_s& F) {
   fBoundary(F, F.n);
 }
 
+static __global__ __launch_bounds__(kBlock) void poolFillU64Kernel(uint64_t* p, uint64_t n, uint64_t v) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Largest pool (in step blocks) the persistent form takes: every block must be resident at once.
+uint32_t poolCoopBlocks() {
+  static uint32_t cached = 0xFFFFFFFFu;
+  if (cached != 0xFFFFFFFFu) return cached;
+  int dev = 0, cus = 0, occ = 0;
+  cached = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&poolCoopKernel), kBlock, 0) ==
+          hipSuccess &&
+      occ > 0)
+    cached = uint32_t(std::min<uint64_t>(kPoolCoopMaxBlocks, uint64_t(occ) * uint64_t(cus)));
+  return cached;
+}
+
+static __global__ __launch_bounds__(kBlock) void narrowCountKernel(const uint32_t* in, uint64_t n, uint8_t* out) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) out[i] = uint8_t(in[i]);
+}
+
 static __global__ __launch_bounds__(kBlock) void addCopyKernel(const uint32_t* in, uint64_t n, uint32_t add,
                                                                uint32_t* out) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -315,9 +355,9 @@ void growCopy(T*& p, uint64_t used, uint64_t newCap, hipStream_t st) {
 // Append a wave of finished jobs: wave positions 0..m-1 (job-major), CSC colPtr[m+1] over them,
 // rows as wave positions, values, and each job's wave position and entry range.
 void poolJoin(b12x_forest_s& F, uint32_t m, uint64_t nnz, const uint32_t* colPtr, const uint32_t* rows,
-              const double* vals, const std::vector<PoolJob>& joiners) {
+              const uint32_t* counts, const std::vector<PoolJob>& joiners) {
   Pool& P = F.pool;
-  hipStream_t st = F.stream;
+  hipStream_t st = P.stream;  // idle here: the last batch has been harvested
   P.dropGraph();
   const uint32_t n0 = P.n;
   const uint64_t e0 = P.nnz;
@@ -339,7 +379,7 @@ void poolJoin(b12x_forest_s& F, uint32_t m, uint64_t nnz, const uint32_t* colPtr
   if (e0 + nnz > P.capE) {
     const uint64_t c = std::max<uint64_t>({e0 + nnz, P.capE * 2, 65536});
     growCopy(P.rowIdx, e0, c, st);
-    growCopy(P.vals, e0, c, st);
+    growCopy(P.cnt, e0, c, st);
     P.capE = c;
   }
   const uint32_t nJ0 = uint32_t(P.jobs.size());
@@ -351,8 +391,10 @@ void poolJoin(b12x_forest_s& F, uint32_t m, uint64_t nnz, const uint32_t* colPtr
     growCopy(P.jobOfBlock, P.nB, c, st);
     dfree(P.partAbs);
     dfree(P.partRes);
+    dfree(P.coopPart);
     P.partAbs = dalloc<double>(c);
     P.partRes = dalloc<double>(c);
+    P.coopPart = dalloc<uint64_t>(uint64_t(c) * 6);
     P.capB = c;
   }
   if (nJ > P.capJ) {
@@ -366,7 +408,7 @@ void poolJoin(b12x_forest_s& F, uint32_t m, uint64_t nnz, const uint32_t* colPtr
   if (nnz) {
     hipLaunchKernelGGL(addCopyKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, rows, nnz, n0,
                        P.rowIdx + e0);
-    HIPCHK(hipMemcpyAsync(P.vals + e0, vals, nnz * 8, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(narrowCountKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, counts, nnz, P.cnt + e0);
   }
   std::vector<uint32_t> jop(m);
   std::vector<PoolJobDev> jd(joiners.size());
@@ -395,31 +437,44 @@ void poolJoin(b12x_forest_s& F, uint32_t m, uint64_t nnz, const uint32_t* colPtr
   P.nB = nB;
 }
 
-// Advance every pool job by `iters` iterations, then hand finished jobs their measures and drop
-// them from the pool.
-void poolIterateOnce(b12x_forest_s& F, int iters);
-
-void poolIterate(b12x_forest_s& F, int iters) {
-  if (F.pool.jobs.empty()) return;
-  const double t0 = nowSeconds();
-  F.poolIters += uint64_t(iters);
-  F.poolMaxJobs = std::max<uint64_t>(F.poolMaxJobs, F.pool.jobs.size());
-  F.poolMaxN = std::max<uint64_t>(F.poolMaxN, F.pool.n);
-  F.poolMaxNnz = std::max<uint64_t>(F.poolMaxNnz, F.pool.nnz);
-  poolIterateOnce(F, iters);
-  F.tPool += nowSeconds() - t0;
-}
-
-void poolIterateOnce(b12x_forest_s& F, int iters) {
+// Launch a batch of `iters` iterations of every pool job on the pool's stream (asynchronous; the
+// forest's subdivision work runs meanwhile on its own stream).
+void poolLaunch(b12x_forest_s& F, int iters) {
   Pool& P = F.pool;
-  hipStream_t st = F.stream;
+  if (P.jobs.empty() || P.inFlight) return;
+  P.tLaunch = nowSeconds();
+  F.poolIters += uint64_t(iters);
+  F.poolMaxJobs = std::max<uint64_t>(F.poolMaxJobs, P.jobs.size());
+  F.poolMaxN = std::max<uint64_t>(F.poolMaxN, P.n);
+  F.poolMaxNnz = std::max<uint64_t>(F.poolMaxNnz, P.nnz);
+  hipStream_t st = P.stream;
   const uint32_t nJ = uint32_t(P.jobs.size());
+  // small pools: the persistent form (every block resident; B12X_POOL_PERSISTENT=0: batches).
+  // With no subdivision running, a launch may run a longer budget.
+  const char* pe = std::getenv("B12X_POOL_PERSISTENT");
+  if (!P.coopOff && !(pe && pe[0] == '0') && P.nB <= poolCoopBlocks()) {
+    if (!P.coopErr) P.coopErr = dalloc<int>(1);
+    const uint64_t nPart = uint64_t(P.nB) * 6;
+    hipLaunchKernelGGL(poolFillU64Kernel, dim3(ceilDiv(nPart, kBlock)), dim3(kBlock), 0, st, P.coopPart, nPart,
+                       kCoopEmpty);
+    HIPCHK(hipMemsetAsync(P.coopErr, 0, sizeof(int), st));
+    const uint32_t budget = uint32_t(iters) * (F.active.empty() ? 16u : 1u);
+    PoolCoopArgs ca{P.colPtr, P.rowIdx, P.cnt, P.table, P.jobOfBlock, P.dJobs, P.y, P.capN, P.coopPart, P.nB, P.coopErr,
+                    uint32_t(std::max(F.piMaxIter, 0)), F.piTol, budget, 1u << 21};
+    hipLaunchKernelGGL(poolCoopKernel, dim3(P.nB), dim3(kBlock), 0, st, ca);
+    HIPCHK(hipGetLastError());
+    F.poolIters += budget - uint64_t(iters);
+    P.coopInFlight = true;
+    P.inFlight = true;
+    return;
+  }
   auto enqueue = [&]() {
     for (int b = 0; b < iters; ++b) {
       hipLaunchKernelGGL(poolStepKernel, dim3(P.nB), dim3(kBlock), 0, st, (const uint32_t*)P.colPtr,
-                         (const uint32_t*)P.rowIdx, (const double*)P.vals, (const uint32_t*)P.jobOfBlock,
+                         (const uint32_t*)P.rowIdx, (const uint8_t*)P.cnt, (const double*)P.table,
+                         (const uint32_t*)P.jobOfBlock,
                          (const PoolJobDev*)P.dJobs, P.y, P.capN, P.partAbs, P.partRes);
-      hipLaunchKernelGGL(poolReduceKernel, dim3(nJ), dim3(kBlock), 0, st, (const double*)P.partAbs,
+      hipLaunchKernelGGL(poolReduceKernel, dim3(nJ), dim3(kWave), 0, st, (const double*)P.partAbs,
                          (const double*)P.partRes, P.dJobs, nJ, uint32_t(std::max(F.piMaxIter, 0)), F.piTol);
     }
   };
@@ -439,12 +494,29 @@ void poolIterateOnce(b12x_forest_s& F, int iters) {
     HIPCHK(hipGraphLaunch(P.exec, st));
   }
   HIPCHK(hipGetLastError());
+  P.inFlight = true;
+}
+
+// Wait for the launched batch, then hand finished jobs their measures and drop them from the pool.
+void poolHarvest(b12x_forest_s& F) {
+  Pool& P = F.pool;
+  if (!P.inFlight) return;
+  P.inFlight = false;
+  hipStream_t st = P.stream;
+  const uint32_t nJ = uint32_t(P.jobs.size());
   std::vector<PoolJobDev> hj(nJ);
   HIPCHK(hipMemcpyAsync(hj.data(), P.dJobs, nJ * sizeof(PoolJobDev), hipMemcpyDeviceToHost, st));
+  int coopErr = 0;
+  if (P.coopInFlight) HIPCHK(hipMemcpyAsync(&coopErr, P.coopErr, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (P.coopInFlight && coopErr) P.coopOff = true;  // every block has left; jobs hold consistent states
+  P.coopInFlight = false;
   bool anyDone = false;
   for (uint32_t r = 0; r < nJ; ++r) anyDone = anyDone || hj[r].done;
-  if (!anyDone) return;
+  if (!anyDone) {
+    F.tPool += nowSeconds() - P.tLaunch;
+    return;
+  }
   P.dropGraph();
   const double tNow = nowSeconds();
   std::vector<int> keep(nJ, 1);
@@ -500,7 +572,7 @@ void poolIterateOnce(b12x_forest_s& F, int iters) {
   uint32_t* jobOfPos = dalloc<uint32_t>(P.capN);
   double* y = dalloc<double>(uint64_t(P.capN) * 3);
   uint32_t* rowIdx = dalloc<uint32_t>(P.capE);
-  double* vals = dalloc<double>(P.capE);
+  uint8_t* cnts = dalloc<uint8_t>(P.capE);
   if (P.n)
     hipLaunchKernelGGL(poolMoveColumnsKernel, dim3(ceilDiv(P.n, kBlock)), dim3(kBlock), 0, st,
                        (const uint32_t*)P.colPtr, (const uint32_t*)P.jobOfPos, (const double*)P.y, P.n, P.capN,
@@ -508,8 +580,8 @@ void poolIterateOnce(b12x_forest_s& F, int iters) {
                        (const uint32_t*)dNewIdx, colPtr, jobOfPos, y, P.capN);
   if (P.nnz)
     hipLaunchKernelGGL(poolMoveEntriesKernel, dim3(ceilDiv(P.nnz, kBlock)), dim3(kBlock), 0, st,
-                       (const uint32_t*)P.rowIdx, (const double*)P.vals, P.nnz, (const uint32_t*)P.jobOfPos,
-                       (const uint32_t*)dShiftP, (const uint32_t*)dShiftE, (const int*)dKeep, rowIdx, vals);
+                       (const uint32_t*)P.rowIdx, (const uint8_t*)P.cnt, P.nnz, (const uint32_t*)P.jobOfPos,
+                       (const uint32_t*)dShiftP, (const uint32_t*)dShiftE, (const int*)dKeep, rowIdx, cnts);
   HIPCHK(hipGetLastError());
   const uint32_t eEnd = uint32_t(eNew);
   HIPCHK(hipMemcpyAsync(colPtr + nNew, &eEnd, 4, hipMemcpyHostToDevice, st));
@@ -520,17 +592,18 @@ void poolIterateOnce(b12x_forest_s& F, int iters) {
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(st));
-  dfree(P.colPtr); dfree(P.jobOfPos); dfree(P.y); dfree(P.rowIdx); dfree(P.vals);
+  dfree(P.colPtr); dfree(P.jobOfPos); dfree(P.y); dfree(P.rowIdx); dfree(P.cnt);
   P.colPtr = colPtr;
   P.jobOfPos = jobOfPos;
   P.y = y;
   P.rowIdx = rowIdx;
-  P.vals = vals;
+  P.cnt = cnts;
   P.jobs.swap(kept);
   P.n = nNew;
   P.nnz = eNew;
   P.nB = bNew;
   dfree(dShiftP); dfree(dShiftE); dfree(dNewIdx); dfree(dKeep);
+  F.tPool += nowSeconds() - P.tLaunch;
 }
 
 // Retire the jobs in R (slots): batched transition matrix, copy of the covers, entry into the
@@ -644,14 +717,6 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
                      (const uint32_t*)cColPos, uint64_t(nnz), nList, colPtr);
   std::vector<uint32_t> hColPtr(uint64_t(nList) + 1);
   HIPCHK(hipMemcpyAsync(hColPtr.data(), colPtr, (uint64_t(nList) + 1) * 4, hipMemcpyDeviceToHost, st));
-  // the measure is iterated in the pool (values as probabilities count/k even when useAbs is set)
-  double* pval = cval;
-  if (F.useAbs) {
-    pval = dalloc<double>(nnz);
-    if (nnz)
-      hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)ccnt,
-                         uint64_t(nnz), double(k), 0, pval);
-  }
   // results arena: covers, matrices (requested order), measures
   auto A = std::make_shared<ResultArena>();
   A->key = dalloc<uint64_t>(nList);
@@ -688,10 +753,11 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
     J.state = 4;  // measuring
     joiners[r] = PoolJob{R[r], lo[r], lo[r + 1], hColPtr[lo[r]], hColPtr[lo[r + 1]], A->mu + lo[r]};
   }
-  poolJoin(F, nList, nnz, colPtr, crow, pval, joiners);
+  // the measure is iterated in the pool, on probabilities count / k even when useAbs is set
+  poolJoin(F, nList, nnz, colPtr, crow, ccnt, joiners);
   dfree(list); dfree(rowPtr); dfree(ecol); dfree(ecnt); dfree(erow); dfree(ccol); dfree(crow); dfree(ccnt);
   dfree(dLo); dfree(dOffR); dfree(jobOfPos); dfree(cLocRow); dfree(cLocCol); dfree(cColPos); dfree(rLocRow);
-  dfree(rLocCol); dfree(rColPos); dfree(cval); dfree(rval); dfree(colPtr); if (pval != cval) dfree(pval);
+  dfree(rLocCol); dfree(rColPos); dfree(cval); dfree(rval); dfree(colPtr);
   // drop the retired leaves
   ForestKeepOp kop{};
   kop.in = F.arr[F.cur];
@@ -717,8 +783,9 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
 void forestStep(b12x_forest_s& F, uint32_t* nRetired) {
   admitWaiting(F);
   if (nRetired) *nRetired = 0;
+  poolLaunch(F, F.poolItersPerStep);
   if (F.active.empty()) {
-    poolIterate(F, F.poolItersPerStep);
+    poolHarvest(F);
     return;
   }
   const double t0 = nowSeconds();
@@ -786,10 +853,11 @@ void forestStep(b12x_forest_s& F, uint32_t* nRetired) {
   }
   const double t1 = nowSeconds();
   F.tSubdiv += t1 - t0;
+  poolHarvest(F);  // the pool is idle before retiring jobs join it
+  const double t2 = nowSeconds();
   if (!R.empty()) retire(F, R, off, end);
-  F.tRetire += nowSeconds() - t1;
+  F.tRetire += nowSeconds() - t2;
   if (nRetired) *nRetired = uint32_t(R.size());
-  poolIterate(F, F.poolItersPerStep);
 }
 
 bool forestBusy(const b12x_forest_s& F) { return !F.active.empty() || !F.waiting.empty() || !F.pool.jobs.empty(); }
@@ -829,6 +897,11 @@ int b12x_forest_create(int dim, int gk, const uint64_t* gp, int ngp, int mk, int
       F->piMaxIter = piMaxIter;
       F->piTol = piTol;
       HIPCHK(hipStreamCreate(&F->stream));
+      HIPCHK(hipStreamCreate(&F->pool.stream));
+      std::vector<double> vt(kPoolTable);
+      for (uint32_t c = 0; c < kPoolTable; ++c) vt[c] = double(c) / double(F->grid.k);
+      F->pool.table = dalloc<double>(kPoolTable);
+      HIPCHK(hipMemcpy(F->pool.table, vt.data(), kPoolTable * sizeof(double), hipMemcpyHostToDevice));
       F->scal = dalloc<uint32_t>(kForestScal);
       HIPCHK(hipHostMalloc((void**)&F->hscal, kForestScal * sizeof(uint32_t), hipHostMallocDefault));
       ensureSlotCap(*F, 64);
@@ -962,8 +1035,8 @@ int b12x_forest_footprint(b12x_forest f, uint64_t* bytes) {
     if (!f || !bytes) return setErr(B12X_E_ARG, "null argument");
     const b12x_forest_s& F = *f;
     uint64_t b = uint64_t(F.cap) * 2 * (8 + 1 + 1 + 4) + F.cap + uint64_t(F.slotCap) * (sizeof(JobDev) + 9);
-    b += (uint64_t(F.pool.capN) + 1) * 4 + uint64_t(F.pool.capN) * (4 + 3 * 8) + F.pool.capE * 12 +
-         uint64_t(F.pool.capB) * (4 + 2 * 8) + uint64_t(F.pool.capJ) * sizeof(PoolJobDev);
+    b += (uint64_t(F.pool.capN) + 1) * 4 + uint64_t(F.pool.capN) * (4 + 3 * 8) + F.pool.capE * 5 +
+         uint64_t(F.pool.capB) * (4 + 2 * 8 + 6 * 8) + uint64_t(F.pool.capJ) * sizeof(PoolJobDev);
     std::vector<const ResultArena*> seen;
     for (const Job& J : F.jobs)
       if (J.arena && std::find(seen.begin(), seen.end(), J.arena.get()) == seen.end()) {

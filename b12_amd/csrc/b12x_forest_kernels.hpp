@@ -180,6 +180,9 @@ static __global__ __launch_bounds__(kBlock) void forestLocalizeKernel(const uint
 // Blocks of poolStepKernel cover kPoolCols consecutive columns of one job (job-relative chunks),
 // so each block's partial sums depend only on that job's data.
 constexpr uint32_t kPoolCols = kBlock;
+// Pool entries store the hit count (1 byte); the value count / k comes from a table of the
+// IEEE quotients double(c) / double(k), c = 0..64 (forests take k <= 64).
+constexpr uint32_t kPoolTable = 65;
 
 struct PoolJobDev {
   uint32_t lo, hi;  // pool positions
@@ -210,12 +213,16 @@ constexpr int kPoolUnroll = 8;
 
 static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* __restrict__ colPtr,
                                                                const uint32_t* __restrict__ rowIdx,
-                                                               const double* __restrict__ val,
+                                                               const uint8_t* __restrict__ cnt,
+                                                               const double* __restrict__ table,
                                                                const uint32_t* __restrict__ jobOfBlock,
                                                                const PoolJobDev* __restrict__ jobs, double* y,
                                                                uint32_t ys, double* partAbs, double* partRes) {
+  __shared__ double val[kPoolTable];
   const PoolJobDev& J = jobs[jobOfBlock[blockIdx.x]];
   if (J.done) return;  // uniform over the block
+  if (threadIdx.x < kPoolTable) val[threadIdx.x] = table[threadIdx.x];
+  __syncthreads();
   const uint32_t t = J.t;
   const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
   const double nPrev = t >= 2 ? J.norm[s1] : 1.0;
@@ -238,7 +245,7 @@ static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* 
       for (int u = 0; u < kPoolUnroll; ++u) {
         const uint32_t eu = min(e + u, e1 - 1);
         i[u] = rowIdx[eu];
-        v[u] = val[eu];
+        v[u] = val[cnt[eu]];
       }
 #pragma unroll
       for (int u = 0; u < kPoolUnroll; ++u) m[u] = yPrev[i[u]];
@@ -265,35 +272,45 @@ static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* 
   }
 }
 
-// One block per pool job: sums its block partials (fixed job-relative order), norm of y_t,
-// residual of iteration t-1, stopping decision, t += 1.
-static __global__ __launch_bounds__(kBlock) void poolReduceKernel(const double* partAbs, const double* partRes,
-                                                                 PoolJobDev* jobs, uint32_t nJobs, uint32_t maxIter,
-                                                                 double tol) {
+// Canonical sum of a job's block partials q = 0..nb-1 (shared by the batched and the persistent
+// forms, so a job's iterates do not depend on which form ran them): lane l of one wave adds
+// q = l, l + 64, ... in ascending order, then the lanes combine by a fixed shuffle tree.
+__device__ inline double poolLaneTreeSum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// One wave per pool job: sums its block partials, norm of y_t, residual of iteration t-1,
+// stopping decision, t += 1.
+static __global__ __launch_bounds__(kWave) void poolReduceKernel(const double* partAbs, const double* partRes,
+                                                                PoolJobDev* jobs, uint32_t nJobs, uint32_t maxIter,
+                                                                double tol) {
   const uint32_t r = blockIdx.x;
   if (r >= nJobs) return;
   PoolJobDev& J = jobs[r];
   if (J.done) return;
-  const uint32_t t = J.t, b0 = J.b0, nb = poolBlocksOf(J.hi - J.lo);
-  constexpr int U = 8;  // all of a thread's loads in flight before it sums them
+  const uint32_t t = J.t, b0 = J.b0, nb = poolBlocksOf(J.hi - J.lo), lane = threadIdx.x;
+  constexpr int U = 8;  // loads in flight per lane before they are added (order unchanged)
   double ta = 0.0, tr = 0.0;
-  for (uint32_t q0 = 0; q0 < nb; q0 += kBlock * U) {
+  for (uint32_t q0 = 0; q0 < nb; q0 += kWave * U) {
     double va[U], vr[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t q = q0 + u * kBlock + threadIdx.x;
+      const uint32_t q = q0 + u * kWave + lane;
       va[u] = q < nb ? partAbs[b0 + q] : 0.0;
       vr[u] = q < nb ? partRes[b0 + q] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      ta += va[u];
-      tr += vr[u];
-    }
+    for (int u = 0; u < U; ++u)
+      if (q0 + u * kWave + lane < nb) {
+        ta += va[u];
+        tr += vr[u];
+      }
   }
-  ta = blockReduceSumF64(ta);
-  tr = blockReduceSumF64(tr);
-  if (threadIdx.x == 0) {
+  ta = poolLaneTreeSum(ta);
+  tr = poolLaneTreeSum(tr);
+  if (lane == 0) {
     J.norm[t % 3] = ta;
     const uint32_t completed = t - 1;
     if (completed >= 1) {
@@ -304,6 +321,204 @@ static __global__ __launch_bounds__(kBlock) void poolReduceKernel(const double* 
       }
     }
     J.t = t + 1;
+  }
+}
+
+// Persistent form of the pool for pools of at most kPoolCoopMaxBlocks step blocks (every block
+// resident): each job's blocks iterate together, synchronised only with one another through
+// the job's partial slots (buffer t % 3, kCoopEmpty until written, cleared one iteration ahead,
+// as in piCoopKernel), so jobs advance independently. Per launch a job runs until it stops or
+// `budget` iterations are done; its first block then stores the job state. A wait past
+// spinLimit polls raises *err; the job's blocks all leave at that iteration, and its first block
+// stores the state as of the last completed iteration (y_{t-1}, y_{t-2} are intact).
+// Same arithmetic and summation orders as poolStepKernel + poolReduceKernel.
+constexpr uint32_t kPoolCoopMaxBlocks = 512;
+constexpr int kPoolCache = 24;  // entries of a lane's column held in registers
+
+struct PoolCoopArgs {
+  const uint32_t* colPtr;
+  const uint32_t* rowIdx;
+  const uint8_t* cnt;
+  const double* table;  // kPoolTable values
+  const uint32_t* jobOfBlock;
+  PoolJobDev* jobs;
+  double* y;
+  uint32_t ys;
+  uint64_t* part;  // 3 buffers x nBlocks slots x 2, kCoopEmpty at launch
+  uint32_t nBlocks;
+  int* err;        // zero at launch
+  uint32_t maxIter;
+  double tol;
+  uint32_t budget;
+  uint32_t spinLimit;
+};
+
+static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) {
+  constexpr uint32_t kSlots = kPoolCoopMaxBlocks / kWave;
+  const uint32_t b = blockIdx.x, tid = threadIdx.x, G = a.nBlocks;
+  __shared__ double sums[2];
+  __shared__ int leave;
+  const uint32_t r = a.jobOfBlock[b];
+  PoolJobDev* J = a.jobs + r;
+  if (J->done) return;
+  const uint32_t lo = J->lo, hi = J->hi, b0 = J->b0, nb = poolBlocksOf(hi - lo);
+  uint32_t t = J->t;
+  double nPrev = t >= 2 ? J->norm[(t + 2) % 3] : 1.0;
+  double nPrev2 = t >= 3 ? J->norm[(t + 1) % 3] : 1.0;
+  const double mu0 = 1.0 / double(hi - lo);
+  const uint32_t p = lo + (b - b0) * kPoolCols + tid;
+  const bool valid = p < hi;
+  // the lane's column: up to kPoolCache entries in registers, the rest read per iteration
+  uint32_t c0 = 0, c1 = 0;
+  uint32_t ci[kPoolCache];
+  double cv[kPoolCache];
+#pragma unroll
+  for (int u = 0; u < kPoolCache; ++u) {
+    ci[u] = 0;
+    cv[u] = 0.0;
+  }
+  if (valid) {
+    c0 = a.colPtr[p];
+    c1 = a.colPtr[p + 1];
+    if (c1 > c0) {
+#pragma unroll
+      for (int u = 0; u < kPoolCache; ++u) {
+        const uint32_t eu = min(c0 + u, c1 - 1);
+        ci[u] = a.rowIdx[eu];
+        cv[u] = a.table[a.cnt[eu]];
+      }
+    }
+  }
+  for (uint32_t m = 0;; ++m) {
+    if (tid == 0) {
+      uint64_t* nextSlot = a.part + (uint64_t((t + 1) % 3) * G + b) * 2;
+      stAgent(nextSlot, kCoopEmpty);
+      stAgent(nextSlot + 1, kCoopEmpty);
+    }
+    const double* yPrev = a.y + uint64_t((t + 2) % 3) * a.ys;
+    const double* yPrev2 = a.y + uint64_t((t + 1) % 3) * a.ys;
+    double* yCur = a.y + uint64_t(t % 3) * a.ys;
+    double sAbs = 0.0, sRes = 0.0;
+    if (valid) {
+      const double yp = ldAgent(yPrev + p), yp2 = ldAgent(yPrev2 + p);
+      double mv[kPoolCache];
+#pragma unroll
+      for (int u = 0; u < kPoolCache; ++u) mv[u] = 0.0;
+#pragma unroll
+      for (int g = 0; g < kPoolCache; g += 8) {
+        if (__any(c0 + g < c1)) {
+#pragma unroll
+          for (int u = g; u < g + 8; ++u) mv[u] = ldAgent(yPrev + ci[u]);
+        }
+      }
+      double s = 0.0;
+#pragma unroll
+      for (int u = 0; u < kPoolCache; ++u) {
+        const double mu = (t == 1) ? mu0 : mv[u] / nPrev;
+        const double prod = mu * cv[u];
+        s = (c0 + u < c1) ? s + prod : s;
+      }
+      for (uint32_t e = c0 + kPoolCache; e < c1; e += 8) {
+        uint32_t i[8];
+        double v[8], mm[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const uint32_t eu = min(e + u, c1 - 1);
+          i[u] = a.rowIdx[eu];
+          v[u] = a.table[a.cnt[eu]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mm[u] = ldAgent(yPrev + i[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const double mu = (t == 1) ? mu0 : mm[u] / nPrev;
+          const double prod = mu * v[u];
+          s = (e + u < c1) ? s + prod : s;
+        }
+      }
+      stAgent(yCur + p, s);
+      sAbs = fabs(s);
+      if (t >= 2) {
+        const double x = yp / nPrev;
+        const double z = (t == 2) ? mu0 : yp2 / nPrev2;
+        sRes = fabs(x - z);
+      }
+    }
+    sAbs = blockReduceSumF64(sAbs);
+    sRes = blockReduceSumF64(sRes);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's y stores have completed
+    __syncthreads();
+    uint64_t* buf = a.part + uint64_t(t % 3) * G * 2;
+    if (tid == 0) {
+      stAgent(buf + 2 * b, uint64_t(__double_as_longlong(sAbs)));
+      stAgent(buf + 2 * b + 1, uint64_t(__double_as_longlong(sRes)));
+    }
+    if (tid < kWave) {
+      uint64_t va[kSlots], vr[kSlots];
+#pragma unroll
+      for (uint32_t k = 0; k < kSlots; ++k) va[k] = vr[k] = 0;
+      bool ok = false;
+      for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
+        bool mine = true;
+#pragma unroll
+        for (uint32_t k = 0; k < kSlots; ++k) {
+          const uint32_t q = tid + k * kWave;
+          if (q < nb && (va[k] == kCoopEmpty || vr[k] == kCoopEmpty || spin == 0)) {
+            va[k] = ldAgent(buf + 2 * (b0 + q));
+            vr[k] = ldAgent(buf + 2 * (b0 + q) + 1);
+          }
+          mine = mine && (q >= nb || (va[k] != kCoopEmpty && vr[k] != kCoopEmpty));
+        }
+        ok = __all(mine);
+        if (ok || ldAgent(a.err)) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      double ta = 0.0, tr = 0.0;
+#pragma unroll
+      for (uint32_t k = 0; k < kSlots; ++k)
+        if (tid + k * kWave < nb) {
+          ta += __longlong_as_double((long long)va[k]);
+          tr += __longlong_as_double((long long)vr[k]);
+        }
+      ta = poolLaneTreeSum(ta);
+      tr = poolLaneTreeSum(tr);
+      if (tid == 0) {
+        if (!ok) stAgent(a.err, 1);
+        leave = ok ? 0 : 1;
+        sums[0] = ta;
+        sums[1] = tr;
+      }
+    }
+    __syncthreads();
+    if (leave) {
+      if (b == b0 && tid == 0) {  // state as of the last completed iteration
+        J->t = t;
+        J->norm[(t + 2) % 3] = nPrev;
+        J->norm[(t + 1) % 3] = nPrev2;
+      }
+      return;
+    }
+    const double ta = sums[0], tr = sums[1];
+    const uint32_t completed = t - 1;
+    const bool done = completed >= 1 && ((a.tol >= 0.0 && tr <= a.tol) || completed >= a.maxIter);
+    if (done || m + 1 >= a.budget) {
+      if (b == b0 && tid == 0) {
+        J->norm[t % 3] = ta;
+        J->norm[(t + 2) % 3] = nPrev;
+        J->norm[(t + 1) % 3] = nPrev2;
+        if (completed >= 1) J->resid = tr;
+        if (done) {
+          J->done = 1;
+          J->final = completed;
+        }
+        J->t = t + 1;
+      }
+      return;
+    }
+    nPrev2 = nPrev;
+    nPrev = ta;
+    ++t;
+    __syncthreads();  // sums[] is rewritten by the next wait
   }
 }
 
@@ -336,11 +551,11 @@ static __global__ __launch_bounds__(kBlock) void poolMoveColumnsKernel(const uin
   for (int sl = 0; sl < 3; ++sl) yOut[uint64_t(sl) * ysOut + q] = yIn[uint64_t(sl) * ysIn + p];
 }
 
-static __global__ __launch_bounds__(kBlock) void poolMoveEntriesKernel(const uint32_t* rowIn, const double* valIn,
+static __global__ __launch_bounds__(kBlock) void poolMoveEntriesKernel(const uint32_t* rowIn, const uint8_t* valIn,
                                                                       uint64_t nnzIn, const uint32_t* jobOfPosIn,
                                                                       const uint32_t* posShift, const uint32_t* nnzShift,
                                                                       const int* keep, uint32_t* rowOut,
-                                                                      double* valOut) {
+                                                                      uint8_t* valOut) {
   const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (e >= nnzIn) return;
   const uint32_t row = rowIn[e];

@@ -24,6 +24,13 @@ PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 vector, FMA counted as 2 flops
 PEAK_FP64_ISSUE_TFLOPS = 39.3    # the same peak when every FP64 op is a separate instruction
 LORENZ_FLOPS_PER_IMAGE = 1590    # 20 RK4 steps x 78 + grid/affine/key 30 (each + - * / once)
 
+PEAK_HBM_BYTES = 8.0e12          # MI355X HBM3E
+# algorithmic bytes per leaf and subdivision step: subdivide (10 B read, 20 B written per parent,
+# i.e. 15 B per child), image + flag (10 B of the box, 1 B hit store per test point), remove +
+# unset (11 B read, 10 B written)
+BYTES_PER_BOX_STEP = 15 + 10 + 27 + 21
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "lorenz_d24_job_traffic.md")
+
 CENTER, RADIUS, DEPTH, K = [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], 24, 27
 TOL = 1e-12
 MAX_ITER = 1_000_000
@@ -37,6 +44,18 @@ def images_of(counts, n0=1):
         imgs += K * n_sub
         n = int(c[3])           # leaves kept (all carried HIT)
     return imgs, n
+
+
+def profiled_traffic():
+    """HBM bytes per image-kernel launch from the committed FETCH_SIZE/WRITE_SIZE profile
+    (scaled by the copy-kernel calibration), or None."""
+    import re
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\(imageKernel<3,LORENZ,INNER>\):\s*(\d+)", text)
+    return int(m.group(1)) if m else None
 
 
 def gpu_job(gx, flavour, timing=False):
@@ -55,6 +74,7 @@ def gpu_job(gx, flavour, timing=False):
     imgs, n = images_of(counts)
     imgs += K * n
     res = {"images": imgs, "boxes": n, "nnz": m.nnz, "iters": iters, "resid": resid,
+           "boxes_stepped": (imgs - K * n) // K,
            "subdiv_ms": (t1 - t0) * 1e3, "tm_ms": (t2 - t1) * 1e3, "measure_ms": (t3 - t2) * 1e3,
            "level24_points": K * (int(counts[-2][3]) + int(counts[-1][0])) if DEPTH > 1 else 0}
     if timing:
@@ -239,7 +259,13 @@ def main():
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
                      "frac_issue_rate": achieved_tflops / PEAK_FP64_ISSUE_TFLOPS,
                      "flops_per_image": LORENZ_FLOPS_PER_IMAGE, "image_launches": launches,
-                     "image_points": pts, "image_ms": ms_img, "traffic": None},
+                     "image_points": pts, "image_ms": ms_img, "traffic": profiled_traffic(),
+                     "traffic_source": "profiles/lorenz_d24_job_traffic.md (FETCH_SIZE x2.000 + WRITE_SIZE x1.000)"},
+        "boxes": {"boxes_per_s": runs[-1]["boxes_stepped"] / (runs[-1]["subdiv_ms"] * 1e-3),
+                  "bytes_per_box_step": BYTES_PER_BOX_STEP,
+                  "hbm_frac": runs[-1]["boxes_stepped"] * BYTES_PER_BOX_STEP / (runs[-1]["subdiv_ms"] * 1e-3)
+                  / PEAK_HBM_BYTES,
+                  "note": "leaves processed by the 24 subdivision steps / subdivision wall time"},
         "breakdown_ms": {k: runs[-1][k] for k in ("subdiv_ms", "tm_ms", "measure_ms")},
         "job": {k: runs[-1][k] for k in ("images", "boxes", "nnz", "iters", "resid", "level24_points")},
     }

@@ -159,3 +159,41 @@ def test_pool_graph_replays_match_eager_launches(monkeypatch):
     for (ma, ia), (mb, ib) in zip(*out):
         assert ia == ib
         assert np.array_equal(ma, mb)
+
+
+def test_persistent_and_batched_pool_forms_agree(monkeypatch):
+    """A pool larger than the persistent form takes (batched launches) that shrinks below it as
+    jobs leave (persistent launches): every job's measure and iteration count equal the same
+    job run alone (persistent throughout) and the all-batched run, bit for bit."""
+    print("[forest] persistent vs batched pool", flush=True)
+    grid = gx.InnerGrid(4)
+    rng = np.random.default_rng(9)
+    cases = [(float(rng.uniform(1.2, 1.4)), int(rng.integers(20, 22))) for _ in range(14)]
+
+    def run(env):
+        if env is None:
+            monkeypatch.delenv("B12X_POOL_PERSISTENT", raising=False)
+        else:
+            monkeypatch.setenv("B12X_POOL_PERSISTENT", env)
+        F = gx.Forest(2, grid, 1, pi_max_iter=100000, pi_tol=1e-12)
+        F.set_pool_iterations(8)
+        ids = [F.add(HENON_C, HENON_R, gx.Henon(a, 0.3, 1), d) for a, d in cases]
+        F.run()
+        st = F.stats()
+        return [(F.measure(j), F.info(j)["iters"]) for j in ids], st
+
+    mixed, st = run(None)
+    assert st["pool_max_cols"] > 512 * 256  # the pool started above the persistent form's size
+    batched, _ = run("0")
+    for (ma, ia), (mb, ib) in zip(mixed, batched):
+        assert ia == ib and np.array_equal(ma, mb)
+    monkeypatch.delenv("B12X_POOL_PERSISTENT", raising=False)
+    a, d = cases[0]
+    A = gx.Forest(2, grid, 1, pi_max_iter=100000, pi_tol=1e-12)
+    A.set_pool_iterations(8)
+    j = A.add(HENON_C, HENON_R, gx.Henon(a, 0.3, 1), d)
+    A.run()
+    assert A.info(j)["iters"] == mixed[0][1] and np.array_equal(A.measure(j), mixed[0][0])
+    ref = oracle_job(HENON_C, HENON_R, grid, gx.Henon(a, 0.3, 1), d, 100000, 1e-12)
+    assert abs(A.info(j)["iters"] - ref[3]) <= 1
+    assert np.allclose(A.measure(j), ref[2], rtol=1e-10, atol=1e-16)
