@@ -42,6 +42,7 @@ SIGNATURES = {
     "b12x_unset_flags": [_P, _U8, _U8, _P],
     "b12x_change_flags": [_P, _U8, _U8, _U8, _P],
     "b12x_flags_by_stencil": [_P, _I, _P, _U8, _U8, _U8, _P],
+    "b12x_flags_by_energy_band": [_P, _D, _D, _I, _U8, _U8, _U8, _P],
     "b12x_flags_by_image": [_P, _P, _U8, _U8, _I, _U8, _U8, _U8, _P],
     "b12x_num_points": [_P, _P, _U8, _U8, _P],
     "b12x_search_by_image": [_P, _P, _U8, _U8, _U8, _P],
@@ -61,6 +62,7 @@ SIGNATURES = {
     "b12x_coo_upload": [_P, _P, _P, _U64, _U32, _U32, _P],
     "b12x_compress_index_vector": [_P, _U64, _U32, _P],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
+    "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
     "b12x_forest_create": [_I, _I, _P, _I, _I, _I, _I, _I, _D, _P],
     "b12x_forest_destroy": [_P],
     "b12x_forest_add": [_P, _P, _P, _P, _I, _U8, _P],
@@ -251,6 +253,14 @@ class CooMatrix:
                  ctypes.byref(it), ctypes.byref(res))
         return mu_arr, it.value, res.value
 
+    def subspace_iteration(self, m=4, iters=100, want_basis=False):
+        """(Ritz matrix H (m x m), basis X (m x n) or None) of b12x_subspace_iteration; the
+        eigenvalues of H approximate the m leading eigenvalues of the matrix."""
+        H = np.zeros((m, m), np.float64)
+        X = np.zeros((m, self.nRows), np.float64) if want_basis else None
+        self.api("b12x_subspace_iteration", self.h, int(m), int(iters), _ptr(H), _ptr(X) if want_basis else None)
+        return H, X
+
     def __del__(self):
         if getattr(self, "h", None):
             self.api.lib.b12x_coo_destroy(self.h)
@@ -355,6 +365,11 @@ class ImplicitBoxTree:
 
     def setFlagsByStencil(self, stencil, flags, depth=-1):
         return self._stencil(0, stencil, flags, 0, depth)
+
+    def setFlagsByEnergyBand(self, emin, emax, flags, depth=-1):
+        """Hénon-Heiles energy-band predicate (4-dimensional trees): flags on the leaves whose
+        box meets [emin, emax] (b12x_flags_by_energy_band)."""
+        return self._ret("b12x_flags_by_energy_band", float(emin), float(emax), 0, flags, 0, _depth(depth))
 
     def unsetFlagsByStencil(self, stencil, flags, depth=-1):
         return self._stencil(1, stencil, flags, 0, depth)

@@ -684,6 +684,21 @@ int b12x_flags_by_stencil(b12x_tree t, int op, const uint8_t* st, uint8_t f, uin
   });
 }
 
+int b12x_flags_by_energy_band(b12x_tree t, double emin, double emax, int op, uint8_t f, uint8_t f2, uint8_t d,
+                              uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret || op < 0 || op > 2) return setErr(B12X_E_ARG, "bad argument");
+    b12x_tree_s& T = *t;
+    if (T.dim != 4) return setErr(B12X_E_ARG, "energy band: 4-dimensional trees only");
+    if (classifyDepth(T, d) != DC_LEAF) throw HipFail(B12X_E_UNSUPPORTED, "energy band: leaf depth only");
+    hipLaunchKernelGGL(energyBandKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.dom,
+                       (const uint64_t*)T.key[T.cur], (const uint8_t*)T.depth[T.cur], T.n, emin, emax, T.hit);
+    HIPCHK(hipGetLastError());
+    *ret = applyFlags(T, T.hit, true, op, f, f2);  // consumes and clears the selection
+    return B12X_OK;
+  });
+}
+
 int b12x_num_points(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, uint64_t* out) {
   return guarded([&]() -> int {
     if (!t || !out) return setErr(B12X_E_ARG, "null argument");
@@ -1188,6 +1203,94 @@ __global__ void fillU64Kernel(uint64_t* p, uint64_t n, uint64_t v) {
   if (i < n) p[i] = v;
 }
 
+// CSC over the first n columns, rows ascending within each column (a column-major copy of the
+// entries); entries in column n (the useInvalid label) fall past colPtr[n].
+struct Csc {
+  uint32_t* keys = nullptr;  // column of each entry (sorted)
+  uint32_t* rows = nullptr;
+  double* vals = nullptr;
+  uint32_t* colPtr = nullptr;  // n + 1
+};
+
+Csc buildCsc(b12x_coo_s& M, uint32_t n) {
+  hipStream_t st = M.stream;
+  std::vector<uint32_t> hT(8);
+  const uint64_t nnz = M.nnz;
+  Csc C;
+  C.keys = dalloc<uint32_t>(nnz);
+  C.rows = dalloc<uint32_t>(nnz);
+  C.vals = dalloc<double>(nnz);
+  if (nnz) {
+    HIPCHK(hipMemcpyAsync(C.keys, M.cols, nnz * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(C.rows, M.rows, nnz * 4, hipMemcpyDeviceToDevice, st));
+  }
+  if (!M.rowMajor) {
+    if (nnz) HIPCHK(hipMemcpyAsync(C.vals, M.vals, nnz * 8, hipMemcpyDeviceToDevice, st));
+  } else if (nnz) {
+    // stable sort by column carrying (row, entry index); then gather the values
+    uint32_t* idx = dalloc<uint32_t>(nnz);
+    hipLaunchKernelGGL(sequenceKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, idx, nnz);
+    radixSort3(st, C.keys, C.rows, idx, nnz, bitsFor(uint64_t(M.nCols) + 1), hT.data());
+    hipLaunchKernelGGL(gatherF64Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, M.vals, idx, nnz, C.vals);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(idx);
+  }
+  C.colPtr = dalloc<uint32_t>(uint64_t(n) + 1);
+  hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, st, C.keys, nnz, n,
+                     C.colPtr);
+  HIPCHK(hipGetLastError());
+  return C;
+}
+
+template <int M>
+void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* basis) {
+  hipStream_t st = mat.stream;
+  Csc C = buildCsc(mat, n);
+  const uint32_t nb = ceilDiv(n, kBlock);
+  const uint32_t nd = std::min<uint32_t>(nb, kDotBlocks);
+  double* X = dalloc<double>(uint64_t(M) * n);
+  double* Y = dalloc<double>(uint64_t(M) * n);
+  double* part = dalloc<double>(uint64_t(M) * M * nd);
+  double* R = dalloc<double>(uint64_t(M) * M);
+  double* H = dalloc<double>(uint64_t(M) * M);
+  int* bad = dalloc<int>(1);
+  HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), st));
+  auto cholQR = [&](double* Z) {  // in place
+    hipLaunchKernelGGL((dotPartialKernel<M, 1>), dim3(nd), dim3(kBlock), 0, st, (const double*)Z, (const double*)Z, n,
+                       part);
+    hipLaunchKernelGGL(cholKernel<M>, dim3(1), dim3(kBlock), 0, st, (const double*)part, nd, R, bad);
+    hipLaunchKernelGGL(applyRinvKernel<M>, dim3(nb), dim3(kBlock), 0, st, Z, n, (const double*)R);
+  };
+  auto spmm = [&](const double* in, double* out) {
+    hipLaunchKernelGGL(spmmKernel<M>, dim3(nb), dim3(kBlock), 0, st, (const uint32_t*)C.colPtr,
+                       (const uint32_t*)C.rows, (const double*)C.vals, in, n, out);
+  };
+  hipLaunchKernelGGL(subspaceStartKernel, dim3(nb), dim3(kBlock), 0, st, X, n, M);
+  cholQR(X);
+  cholQR(X);
+  for (int it = 0; it < iters; ++it) {
+    spmm(X, Y);
+    cholQR(Y);
+    cholQR(Y);
+    std::swap(X, Y);
+  }
+  spmm(X, Y);
+  hipLaunchKernelGGL((dotPartialKernel<M, 0>), dim3(nd), dim3(kBlock), 0, st, (const double*)X, (const double*)Y, n,
+                     part);
+  hipLaunchKernelGGL(ritzKernel, dim3(1), dim3(kBlock), 0, st, (const double*)part, nd, M, H);
+  if (basis) hipLaunchKernelGGL(transposeBasisKernel, dim3(nb), dim3(kBlock), 0, st, (const double*)X, n, M, Y);
+  HIPCHK(hipGetLastError());
+  int hbad = 0;
+  HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(ritz, H, uint64_t(M) * M * 8, hipMemcpyDeviceToHost, st));
+  if (basis) HIPCHK(hipMemcpyAsync(basis, Y, uint64_t(M) * n * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(X); dfree(Y); dfree(part); dfree(R); dfree(H); dfree(bad);
+  dfree(C.keys); dfree(C.rows); dfree(C.vals); dfree(C.colPtr);
+  if (hbad) throw HipFail(B12X_E_ARG, "subspace iteration: Gram matrix not positive definite");
+}
+
 // Blocks for the persistent power iteration over n columns: one lane per column up to
 // kCoopMaxBlocks blocks, never more than the device holds resident at once, so that on an idle
 // device every block runs from the start (0: use the batched form).
@@ -1212,31 +1315,9 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
     const uint32_t n = m->nRows;
     hipStream_t st = m->stream;
-    std::vector<uint32_t> hT(8);
-    // CSC in ascending row order per column: a column-major copy of the entries
-    const uint64_t nnz = m->nnz;
-    uint32_t* ck = dalloc<uint32_t>(nnz);
-    uint32_t* cr = dalloc<uint32_t>(nnz);
-    double* vals = dalloc<double>(nnz);
-    if (nnz) {
-      HIPCHK(hipMemcpyAsync(ck, m->cols, nnz * 4, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(cr, m->rows, nnz * 4, hipMemcpyDeviceToDevice, st));
-    }
-    if (!m->rowMajor) {
-      if (nnz) HIPCHK(hipMemcpyAsync(vals, m->vals, nnz * 8, hipMemcpyDeviceToDevice, st));
-    } else if (nnz) {
-      // stable sort by column carrying (row, entry index); then gather the values
-      uint32_t* idx = dalloc<uint32_t>(nnz);
-      hipLaunchKernelGGL(sequenceKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, idx, nnz);
-      radixSort3(st, ck, cr, idx, nnz, bitsFor(uint64_t(m->nCols) + 1), hT.data());
-      hipLaunchKernelGGL(gatherF64Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, m->vals, idx, nnz, vals);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(st));
-      dfree(idx);
-    }
-    uint32_t* colPtr = dalloc<uint32_t>(uint64_t(n) + 1);
-    hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, st, ck, nnz, n,
-                       colPtr);
+    Csc C = buildCsc(*m, n);
+    uint32_t *ck = C.keys, *cr = C.rows, *colPtr = C.colPtr;
+    double* vals = C.vals;
     double* mu0 = dalloc<double>(n);
     double* ybuf = dalloc<double>(uint64_t(n) * 3);
     const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
@@ -1334,6 +1415,26 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     if (resid) *resid = hs.resid;
     dfree(mu0); dfree(ybuf); dfree(partial); dfree(pst); dfree(dmu); dfree(colPtr); dfree(partF); dfree(blockIter);
     dfree(ck); dfree(cr); dfree(vals);
+    return B12X_OK;
+  });
+}
+
+int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, double* basis) {
+  return guarded([&]() -> int {
+    if (!m || !ritz || nvec < 1 || nvec > kSubspaceMax || iters < 0) return setErr(B12X_E_ARG, "bad argument");
+    if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
+    const uint32_t n = m->nRows;
+    if (n < uint32_t(nvec)) return setErr(B12X_E_ARG, "fewer boxes than vectors");
+    switch (nvec) {
+      case 1: subspaceRun<1>(*m, n, iters, ritz, basis); break;
+      case 2: subspaceRun<2>(*m, n, iters, ritz, basis); break;
+      case 3: subspaceRun<3>(*m, n, iters, ritz, basis); break;
+      case 4: subspaceRun<4>(*m, n, iters, ritz, basis); break;
+      case 5: subspaceRun<5>(*m, n, iters, ritz, basis); break;
+      case 6: subspaceRun<6>(*m, n, iters, ritz, basis); break;
+      case 7: subspaceRun<7>(*m, n, iters, ritz, basis); break;
+      default: subspaceRun<8>(*m, n, iters, ritz, basis); break;
+    }
     return B12X_OK;
   });
 }

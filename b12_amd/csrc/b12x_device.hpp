@@ -419,6 +419,43 @@ __device__ inline uint32_t locate(const uint64_t* __restrict__ keys, const uint8
   return kInvalid;
 }
 
+// Energy of the Hénon-Heiles Hamiltonian over a box (build-defined band predicate; no reference
+// counterpart): H(q, p) = (p1^2 + p2^2)/2 + (q1^2 + q2^2)/2 + q1^2 q2 - q2^3/3 on
+// [c - r, c + r] in (q1, q2, p1, p2), as an interval in round-to-nearest FP64: squares by sign
+// case, q1^2 q2 as the hull of the four corner products, q2^3 by monotonicity, summed in a
+// fixed order.
+__device__ inline void intervalSquare(double a, double b, double& lo, double& hi) {
+  if (a >= 0.0) {
+    lo = a * a;
+    hi = b * b;
+  } else if (b <= 0.0) {
+    lo = b * b;
+    hi = a * a;
+  } else {
+    const double x = a * a, y = b * b;
+    lo = 0.0;
+    hi = x > y ? x : y;
+  }
+}
+
+__device__ inline bool henonHeilesBand(const double* c, const double* r, double emin, double emax) {
+  const double q1a = c[0] - r[0], q1b = c[0] + r[0], q2a = c[1] - r[1], q2b = c[1] + r[1];
+  const double p1a = c[2] - r[2], p1b = c[2] + r[2], p2a = c[3] - r[3], p2b = c[3] + r[3];
+  double Q1lo, Q1hi, Q2lo, Q2hi, P1lo, P1hi, P2lo, P2hi;
+  intervalSquare(q1a, q1b, Q1lo, Q1hi);
+  intervalSquare(q2a, q2b, Q2lo, Q2hi);
+  intervalSquare(p1a, p1b, P1lo, P1hi);
+  intervalSquare(p2a, p2b, P2lo, P2hi);
+  const double m1 = Q1lo * q2a, m2 = Q1lo * q2b, m3 = Q1hi * q2a, m4 = Q1hi * q2b;
+  const double lo12 = m1 < m2 ? m1 : m2, lo34 = m3 < m4 ? m3 : m4;
+  const double hi12 = m1 > m2 ? m1 : m2, hi34 = m3 > m4 ? m3 : m4;
+  const double Mlo = lo12 < lo34 ? lo12 : lo34, Mhi = hi12 > hi34 ? hi12 : hi34;
+  const double c3lo = (q2a * q2a) * q2a, c3hi = (q2b * q2b) * q2b;
+  const double lo = ((0.5 * (P1lo + P2lo) + 0.5 * (Q1lo + Q2lo)) + Mlo) - c3hi / 3.0;
+  const double hi = ((0.5 * (P1hi + P2hi) + 0.5 * (Q1hi + Q2hi)) + Mhi) - c3lo / 3.0;
+  return hi >= emin && lo <= emax;
+}
+
 // Descent with the upper levels staged in LDS. sm[0..S) holds the leaf keys at positions
 // pos(j) = floor(j * n / S), S = min(kDescentSamples, n) (sampleKeysKernel writes them). The
 // count j of samples below the point key brackets the lower bound: it is 0 when j = 0, else it

@@ -1162,6 +1162,192 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
   }
 }
 
+// sel[i] = 1 where leaf i meets the Hénon-Heiles energy band (4-dimensional trees)
+static __global__ __launch_bounds__(kBlock) void energyBandKernel(Domain dom, const uint64_t* key, const uint8_t* depth,
+                                                                 uint32_t n, double emin, double emax, uint8_t* sel) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double c[4], r[4];
+  boxCoords<4>(dom, key[i], depth[i], c, r);
+  sel[i] = henonHeilesBand(c, r, emin, emax) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Subspace iteration for the m <= kSubspaceMax leading eigenvalues (build-defined; the reference
+// leaves eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59). The basis is stored box by box
+// (X[i * m + c]), so one matrix entry gathers m contiguous values. Y = X P with
+// y_j[c] = sum_i x_i[c] P_ij in ascending row order; the basis is re-orthonormalised by Cholesky
+// QR applied twice. Dot products: each thread sums its indices i = t, t + T, ... (T = all
+// threads of the grid) in order, the block sums its threads by a fixed tree, and one block sums
+// the block partials in a fixed order.
+// ---------------------------------------------------------------------------------------------
+constexpr int kSubspaceMax = 8;
+constexpr uint32_t kDotBlocks = 1024;
+
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// X_0: column 0 = 1/n, column c > 0 = (splitmix64(8 i + c) >> 11) 2^-53 * 2 - 1
+static __global__ __launch_bounds__(kBlock) void subspaceStartKernel(double* X, uint32_t n, int m) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  X[uint64_t(i) * m] = 1.0 / double(n);
+  for (int c = 1; c < m; ++c)
+    X[uint64_t(i) * m + c] = double(splitmix64(8 * uint64_t(i) + uint64_t(c)) >> 11) * 0x1p-53 * 2.0 - 1.0;
+}
+
+// one lane per column j; kSpmmUnroll entries' gathers in flight at a time (indices clamped into
+// the column, sums selected), products added in ascending row order
+constexpr int kSpmmUnroll = 8;
+
+template <int M>
+static __global__ __launch_bounds__(kBlock) void spmmKernel(const uint32_t* __restrict__ colPtr,
+                                                           const uint32_t* __restrict__ rows,
+                                                           const double* __restrict__ vals,
+                                                           const double* __restrict__ X, uint32_t n, double* Y) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  double s[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) s[c] = 0.0;
+  const uint32_t e1 = colPtr[j + 1];
+  for (uint32_t e = colPtr[j]; e < e1; e += kSpmmUnroll) {
+    uint32_t i[kSpmmUnroll];
+    double v[kSpmmUnroll];
+#pragma unroll
+    for (int u = 0; u < kSpmmUnroll; ++u) {
+      const uint32_t eu = min(e + u, e1 - 1);
+      i[u] = rows[eu];
+      v[u] = vals[eu];
+    }
+    double x[kSpmmUnroll][M];
+#pragma unroll
+    for (int u = 0; u < kSpmmUnroll; ++u)
+#pragma unroll
+      for (int c = 0; c < M; ++c) x[u][c] = X[uint64_t(i[u]) * M + c];
+#pragma unroll
+    for (int u = 0; u < kSpmmUnroll; ++u) {
+      const bool on = e + u < e1;
+#pragma unroll
+      for (int c = 0; c < M; ++c) {
+        const double prod = x[u][c] * v[u];
+        s[c] = on ? s[c] + prod : s[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < M; ++c) Y[uint64_t(j) * M + c] = s[c];
+}
+
+// Block partials of the dot products a . b of columns of A and B: all pairs (SYM = 0) or a <= b
+// (SYM = 1, A = B); part[pair * gridDim.x + block]
+template <int M, int SYM>
+static __global__ __launch_bounds__(kBlock) void dotPartialKernel(const double* A, const double* B, uint32_t n,
+                                                                 double* part) {
+  constexpr int P = SYM ? M * (M + 1) / 2 : M * M;
+  double acc[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) acc[q] = 0.0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double a[M], b[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      a[c] = A[uint64_t(i) * M + c];
+      b[c] = SYM ? a[c] : B[uint64_t(i) * M + c];
+    }
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < M; ++x)
+#pragma unroll
+      for (int y = SYM ? x : 0; y < M; ++y, ++q) {
+        const double prod = a[x] * b[y];
+        acc[q] += prod;
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const double v = blockReduceSumF64(acc[q]);
+    if (threadIdx.x == 0) part[uint64_t(q) * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// Sums the partials of one pair in block order (threads stride the blocks, then a fixed tree)
+__device__ inline double sumPartials(const double* p, uint32_t nb) {
+  double t = 0.0;
+  for (uint32_t q = threadIdx.x; q < nb; q += kBlock) t += p[q];
+  return blockReduceSumF64(t);
+}
+
+// Gram matrix of Y (upper triangle from dotPartialKernel, SYM = 1) and its Cholesky factor
+// G = R^T R (R upper, row-major m x m); *bad is set (and stays set) if G is not positive definite.
+template <int M>
+static __global__ __launch_bounds__(kBlock) void cholKernel(const double* part, uint32_t nb, double* R, int* bad) {
+  __shared__ double G[M * M];
+  int pair = 0;
+  for (int a = 0; a < M; ++a)
+    for (int b = a; b < M; ++b, ++pair) {
+      const double g = sumPartials(part + uint64_t(pair) * nb, nb);
+      if (threadIdx.x == 0) G[a * M + b] = g;
+    }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double r[M * M];
+#pragma unroll
+  for (int q = 0; q < M * M; ++q) r[q] = 0.0;
+  for (int j = 0; j < M; ++j) {
+    double d = G[j * M + j];
+    for (int k = 0; k < j; ++k) d -= r[k * M + j] * r[k * M + j];
+    if (!(d > 0.0)) {
+      *bad = 1;
+      d = 1.0;
+    }
+    r[j * M + j] = sqrt(d);
+    for (int l = j + 1; l < M; ++l) {
+      double v = G[j * M + l];
+      for (int k = 0; k < j; ++k) v -= r[k * M + j] * r[k * M + l];
+      r[j * M + l] = v / r[j * M + j];
+    }
+  }
+  for (int q = 0; q < M * M; ++q) R[q] = r[q];
+}
+
+// In place: x_c = (y_c - sum_{d<c} R_dc x_d) / R_cc, d ascending
+template <int M>
+static __global__ __launch_bounds__(kBlock) void applyRinvKernel(double* Y, uint32_t n, const double* R) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double x[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    double v = Y[uint64_t(i) * M + c];
+#pragma unroll
+    for (int d = 0; d < c; ++d) v -= R[d * M + c] * x[d];
+    x[c] = v / R[c * M + c];
+  }
+#pragma unroll
+  for (int c = 0; c < M; ++c) Y[uint64_t(i) * M + c] = x[c];
+}
+
+// H_ab = x_a . z_b from dotPartialKernel (SYM = 0): one block
+static __global__ __launch_bounds__(kBlock) void ritzKernel(const double* part, uint32_t nb, int m, double* H) {
+  for (int q = 0; q < m * m; ++q) {
+    const double h = sumPartials(part + uint64_t(q) * nb, nb);
+    if (threadIdx.x == 0) H[q] = h;
+  }
+}
+
+// box-by-box basis to column-by-column (the C ABI's layout)
+static __global__ __launch_bounds__(kBlock) void transposeBasisKernel(const double* X, uint32_t n, int m,
+                                                                     double* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < m; ++c) out[uint64_t(c) * n + i] = X[uint64_t(i) * m + c];
+}
+
 // Plain copy (traffic-counter calibration): every byte read once and written once.
 static __global__ __launch_bounds__(kBlock) void copyKernel(const uint4* in, uint4* out, uint64_t n) {
   for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock)

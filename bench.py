@@ -177,6 +177,70 @@ def serve_bench(gx, n_req, rate=200.0, tol=TOL):
             "ratio_saturated_vs_plain": saturated["jobs_per_s"] / (n_req / plain_s)}
 
 
+HH_C, HH_R = [0.0, 0.0, 0.0, 0.0], [0.6, 0.6, 0.5, 0.5]
+HH_BAND = (0.11, 0.12)
+HH_FLAG = 0x80                   # build-defined flag for the band predicate
+HH_FLOPS_PER_IMAGE = 1194        # 50 Verlet steps x 23 + grid/affine/key 44
+
+
+def hh_loop(t, grid, mp, depth):
+    """Config 3 step: subdivide(SD); keep the boxes meeting the energy band; keep the boxes hit
+    by an image. Returns the test points imaged."""
+    imaged = 0
+    t.setFlags(0x08, 0)
+    for _ in range(depth):
+        t.subdivide(0x08)
+        t.setFlagsByEnergyBand(HH_BAND[0], HH_BAND[1], HH_FLAG)
+        t.remove(HH_FLAG)
+        t.unsetFlags(HH_FLAG, -1)
+        imaged += t.size() * grid.points_per_box(4)
+        t.setFlagsByIterators(t.makePointIteratorPairOverLeaves(grid, mp), 0x01, -1)
+        t.remove(0x01)
+        t.unsetFlags(0x01, -1)
+    return imaged
+
+
+def hh_bench(gx, depth, iters, cpu_depth, threads):
+    """Config 3: Hénon-Heiles, 50 Störmer-Verlet steps of h = 0.02, energy band [0.11, 0.12],
+    UniformGrid(64, seed 0), the band-restricted subdivision to `depth`, the transition matrix and
+    `iters` subspace iterations for the 4 leading eigenvalues."""
+    import numpy as np
+    grid, mp = gx.UniformGrid(64, 0), gx.HenonHeiles(2, 0.02, 50)
+    t = gx.ImplicitBoxTree(HH_C, HH_R)
+    t.image_timing(1)
+    t0 = time.perf_counter()
+    imaged = hh_loop(t, grid, mp, depth)
+    t1 = time.perf_counter()
+    m = t.transitionMatrixForLeaves(grid, mp)
+    t2 = time.perf_counter()
+    H, _ = m.subspace_iteration(4, iters)
+    t3 = time.perf_counter()
+    ms_img, launches, pts = t.image_timing(0)
+    imaged += t.size() * grid.points_per_box(4)
+    ev = np.linalg.eigvals(H)
+    ev = ev[np.argsort(-np.abs(ev))]
+    out = {"config": f"Hénon-Heiles, 50 Verlet steps h=0.02, band E in [0.11, 0.12], UniformGrid(64, 0), depth {depth}, "
+                     f"TM + {iters} subspace iterations (4 vectors)",
+           "boxes": t.size(), "nnz": m.nnz, "images": imaged,
+           "subdiv_s": t1 - t0, "tm_s": t2 - t1, "subspace_s": t3 - t2,
+           "images_per_s": imaged / (t3 - t0),
+           "image_kernel_tflops": pts * HH_FLOPS_PER_IMAGE / (ms_img * 1e-3) / 1e12 if ms_img > 0 else None,
+           "ritz_values": [[float(x.real), float(x.imag)] for x in ev]}
+    if cpu_depth > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        from oracle_api import OracleTree
+        o = OracleTree(HH_C, HH_R)
+        o.set_options(eval_twice=True)
+        c0 = time.perf_counter()
+        ci = hh_loop(o, grid, mp, cpu_depth)
+        c1 = time.perf_counter()
+        out["cpu_baseline"] = {"value": ci / (c1 - c0), "unit": "images/s", "cores": threads, "kind": "port",
+                               "sample": f"the same band-restricted subdivision to depth {cpu_depth} ({ci} images, "
+                                         f"{c1 - c0:.1f} s), oracle with the reference's double image evaluation"}
+    return out
+
+
 def cpu_baseline(sample_depth, threads):
     """B12's CPU path restated (oracle): same algorithm structure as the reference's host
     Thrust path, including its second image evaluation per search, OpenMP over `threads`."""
@@ -209,6 +273,9 @@ def main():
     ap.add_argument("--cpu-depth", type=int, default=24)
     ap.add_argument("--sweep-jobs", type=int, default=512, help="config 4 jobs (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="config 5 requests (0 to skip)")
+    ap.add_argument("--hh-depth", type=int, default=28, help="config 3 depth (0 to skip)")
+    ap.add_argument("--hh-iters", type=int, default=100, help="config 3 subspace iterations")
+    ap.add_argument("--hh-cpu-depth", type=int, default=18, help="config 3 CPU-baseline sample depth (0 to skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -273,9 +340,12 @@ def main():
         out["sweep"] = sweep_bench(gx, args.sweep_jobs)
     if args.serve_requests > 0:
         out["serving"] = serve_bench(gx, args.serve_requests)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = min(threads, 16)
+    if args.hh_depth > 0:
+        out["hh"] = hh_bench(gx, args.hh_depth, args.hh_iters, 0 if args.no_cpu_baseline else args.hh_cpu_depth,
+                             threads)
     if not args.no_cpu_baseline:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = min(threads, 16)
         v, secs, imgs, iters = cpu_baseline(args.cpu_depth, threads)
         out["cpu_baseline"] = {"value": v, "unit": "images/s", "cores": threads, "kind": "port",
                                "sample": f"same Lorenz job to depth {args.cpu_depth} + transition matrix "

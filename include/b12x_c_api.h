@@ -101,6 +101,13 @@ int b12x_change_flags(b12x_tree t, uint8_t flags_to_unset, uint8_t flags_to_set,
  * 106-108. op: 0 set, 1 unset, 2 change (flags2 used by change only). */
 int b12x_flags_by_stencil(b12x_tree t, int op, const uint8_t* stencil, uint8_t flags, uint8_t flags2, uint8_t depth,
                           uint32_t* ret);
+
+/* Energy-band predicate (no counterpart in the reference: the Hénon-Heiles configuration's
+ * box predicate). op and flags as b12x_flags_by_stencil, with the stencil computed on the GPU:
+ * leaf i is selected iff the interval bound of H(q,p) = (p1^2+p2^2)/2 + (q1^2+q2^2)/2 + q1^2 q2
+ * - q2^3/3 over its box meets [emin, emax]. 4-dimensional trees, leaf depth only. */
+int b12x_flags_by_energy_band(b12x_tree t, double emin, double emax, int op, uint8_t flags, uint8_t flags2,
+                              uint8_t depth, uint32_t* ret);
 /* *ByIterators over the image points of (scheme, depth_it, flags_to_check) —
  * setFlagsByIterators ImplicitBoxTree.h:465-469 (= setFlagsBySearch(searchCompacted(...))),
  * unsetFlagsByIterators :531-532, changeFlagsByIterators :96-98. */
@@ -163,6 +170,12 @@ int b12x_compress_index_vector(const uint32_t* indices, uint64_t n, uint32_t dim
  * until max_iter iterations or ||mu_{k+1} - mu_k||_1 <= tol (tol < 0: fixed count). The
  * reference computes this step in MATLAB (subdivisionByNim.m:30-59); there is no C++ API. */
 int b12x_power_iteration(b12x_coo m, int max_iter, double tol, int mu_init, double* mu, int* iters, double* resid);
+/* Subspace iteration for the nvec (<= 8) leading eigenvalues (no counterpart in the reference,
+ * which leaves eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59): X_0 = [1/n, 3 fixed
+ * splitmix64 columns], then `iters` times X <- cholQR2(X P) (row-vector convention, as the power
+ * iteration); ritz (nvec x nvec, row-major) = X^T (X P), whose eigenvalues approximate the
+ * leading eigenvalues of P. basis (nvec x n, column by column) may be NULL. */
+int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, double* basis);
 
 /* --- batched and served forms (configs 4 and 5) --------------------------------------------
  * A forest runs the subdivision algorithm (subdivision.m:15-25) for many jobs at once: jobs of

@@ -955,4 +955,146 @@ inline int powerIteration(const Coo& P, int maxIter, double tol, std::vector<dou
   return it;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Energy band of the Hénon-Heiles Hamiltonian (build-defined; not a reference feature)
+// H(q, p) = (p1^2 + p2^2)/2 + (q1^2 + q2^2)/2 + q1^2 q2 - q2^3/3 over the box
+// [c - r, c + r] in (q1, q2, p1, p2), as an interval in round-to-nearest FP64 in this order:
+// squares by sign case, q1^2 q2 as the hull of the four corner products, q2^3 by monotonicity.
+// A box meets the band [emin, emax] iff hi >= emin and lo <= emax.
+// ---------------------------------------------------------------------------------------------
+struct Interval {
+  double lo, hi;
+};
+
+inline Interval intervalSquare(double a, double b) {
+  if (a >= 0.0) return {a * a, b * b};
+  if (b <= 0.0) return {b * b, a * a};
+  const double x = a * a, y = b * b;
+  return {0.0, x > y ? x : y};
+}
+
+inline Interval intervalProduct(Interval u, double c, double d) {
+  const double p1 = u.lo * c, p2 = u.lo * d, p3 = u.hi * c, p4 = u.hi * d;
+  const double lo12 = p1 < p2 ? p1 : p2, lo34 = p3 < p4 ? p3 : p4;
+  const double hi12 = p1 > p2 ? p1 : p2, hi34 = p3 > p4 ? p3 : p4;
+  return {lo12 < lo34 ? lo12 : lo34, hi12 > hi34 ? hi12 : hi34};
+}
+
+inline Interval henonHeilesEnergy(const double* c, const double* r) {
+  const double q1a = c[0] - r[0], q1b = c[0] + r[0], q2a = c[1] - r[1], q2b = c[1] + r[1];
+  const double p1a = c[2] - r[2], p1b = c[2] + r[2], p2a = c[3] - r[3], p2b = c[3] + r[3];
+  const Interval Q1 = intervalSquare(q1a, q1b), Q2 = intervalSquare(q2a, q2b);
+  const Interval P1 = intervalSquare(p1a, p1b), P2 = intervalSquare(p2a, p2b);
+  const Interval M = intervalProduct(Q1, q2a, q2b);
+  const double c3lo = (q2a * q2a) * q2a, c3hi = (q2b * q2b) * q2b;
+  const double lo = ((0.5 * (P1.lo + P2.lo) + 0.5 * (Q1.lo + Q2.lo)) + M.lo) - c3hi / 3.0;
+  const double hi = ((0.5 * (P1.hi + P2.hi) + 0.5 * (Q1.hi + Q2.hi)) + M.hi) - c3lo / 3.0;
+  return {lo, hi};
+}
+
+inline std::vector<uint8_t> energyBandStencil(const Tree<4>& T, double emin, double emax) {
+  std::vector<uint8_t> st(T.size());
+  for (size_t i = 0; i < st.size(); ++i) {
+    double c[4], r[4];
+    T.geo.coords(T.box(i), c, r);
+    const Interval e = henonHeilesEnergy(c, r);
+    st[i] = (e.hi >= emin && e.lo <= emax) ? 1 : 0;
+  }
+  return st;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Subspace iteration for the m leading eigenvalues (build-defined; the reference leaves
+// eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59). Row-vector convention as the power
+// iteration: Y = X P with y_j[c] = sum_i x_i[c] P_ij over column j in ascending row order.
+//   X_0: column 0 = 1/n, column c > 0 = u(i, c) = (splitmix64(8 i + c) >> 11) 2^-53 * 2 - 1
+//   orthonormalise by Cholesky QR applied twice (Gram G = Y^T Y, G = R^T R, X = Y R^-1
+//   column by column: x_c = (y_c - sum_{d<c} R_dc x_d) / R_cc, d ascending)
+//   `iters` times: X <- cholQR2(X P); then the Ritz matrix H_ab = x_a . (X P)_b.
+// Dot products here are summed in index order (the GPU sums fixed blocks; results agree within
+// rounding). X is stored column by column (X[c * n + i]).
+// ---------------------------------------------------------------------------------------------
+inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+inline double subspaceStart(uint64_t i, int c) {
+  return double(splitmix64(8 * i + uint64_t(c)) >> 11) * 0x1p-53 * 2.0 - 1.0;
+}
+
+// false if the Gram matrix is not positive definite
+inline bool cholQR(std::vector<double>& X, NrBoxes n, int m) {
+  std::vector<double> G(size_t(m) * m), R(size_t(m) * m, 0.0);
+  for (int a = 0; a < m; ++a)
+    for (int b = a; b < m; ++b) {
+      double s = 0.0;
+      for (NrBoxes i = 0; i < n; ++i) s += X[size_t(a) * n + i] * X[size_t(b) * n + i];
+      G[a * m + b] = s;
+    }
+  for (int j = 0; j < m; ++j) {
+    double d = G[j * m + j];
+    for (int k = 0; k < j; ++k) d -= R[k * m + j] * R[k * m + j];
+    if (!(d > 0.0)) return false;
+    R[j * m + j] = std::sqrt(d);
+    for (int l = j + 1; l < m; ++l) {
+      double v = G[j * m + l];
+      for (int k = 0; k < j; ++k) v -= R[k * m + j] * R[k * m + l];
+      R[j * m + l] = v / R[j * m + j];
+    }
+  }
+  for (NrBoxes i = 0; i < n; ++i)
+    for (int c = 0; c < m; ++c) {
+      double v = X[size_t(c) * n + i];
+      for (int d = 0; d < c; ++d) v -= R[d * m + c] * X[size_t(d) * n + i];
+      X[size_t(c) * n + i] = v / R[c * m + c];
+    }
+  return true;
+}
+
+inline bool subspaceIteration(const Coo& P, int m, int iters, std::vector<double>& H, std::vector<double>& X) {
+  const NrBoxes n = P.nRows;
+  std::vector<size_t> order(P.rows.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return P.cols[a] != P.cols[b] ? P.cols[a] < P.cols[b] : P.rows[a] < P.rows[b];
+  });
+  std::vector<NrBoxes> cptr(size_t(n) + 1, 0);
+  for (size_t e : order) if (P.cols[e] < n) ++cptr[P.cols[e] + 1];
+  for (NrBoxes j = 0; j < n; ++j) cptr[j + 1] += cptr[j];
+  std::vector<NrBoxes> ri;
+  std::vector<double> va;
+  for (size_t e : order) if (P.cols[e] < n) { ri.push_back(P.rows[e]); va.push_back(P.vals[e]); }
+  auto spmm = [&](const std::vector<double>& in, std::vector<double>& out) {
+    out.assign(size_t(m) * n, 0.0);
+    for (NrBoxes j = 0; j < n; ++j)
+      for (int c = 0; c < m; ++c) {
+        double s = 0.0;
+        for (NrBoxes e = cptr[j]; e < cptr[j + 1]; ++e) s += in[size_t(c) * n + ri[e]] * va[e];
+        out[size_t(c) * n + j] = s;
+      }
+  };
+  X.assign(size_t(m) * n, 0.0);
+  for (NrBoxes i = 0; i < n; ++i)
+    for (int c = 0; c < m; ++c) X[size_t(c) * n + i] = c == 0 ? 1.0 / double(n) : subspaceStart(i, c);
+  if (!cholQR(X, n, m) || !cholQR(X, n, m)) return false;
+  std::vector<double> Y;
+  for (int it = 0; it < iters; ++it) {
+    spmm(X, Y);
+    if (!cholQR(Y, n, m) || !cholQR(Y, n, m)) return false;
+    X.swap(Y);
+  }
+  spmm(X, Y);
+  H.assign(size_t(m) * m, 0.0);
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      double s = 0.0;
+      for (NrBoxes i = 0; i < n; ++i) s += X[size_t(a) * n + i] * Y[size_t(b) * n + i];
+      H[a * m + b] = s;
+    }
+  return true;
+}
+
 }  // namespace b12cpu

@@ -363,4 +363,24 @@ int b12cpu_power_iteration(void* c, int maxIter, double tol, double* mu, int* it
   return 0;
 }
 
+// stencil of the leaves meeting the Hénon-Heiles energy band (4-dimensional trees only)
+int b12cpu_energy_band_stencil(void* h, double emin, double emax, uint8_t* out) {
+  auto* tb = static_cast<TreeBase*>(h);
+  if (tb->dim != 4) return fail("energy band: 4-dimensional trees only");
+  auto st = energyBandStencil(static_cast<TreeH<4>*>(tb)->t, emin, emax);
+  std::memcpy(out, st.data(), st.size());
+  return 0;
+}
+
+// H: m x m Ritz matrix (row-major); X: m x nRows basis (column by column), may be null
+int b12cpu_subspace_iteration(void* c, int m, int iters, double* H, double* X) {
+  const Coo& P = static_cast<CooH*>(c)->m;
+  if (m < 1 || m > 8 || iters < 0) return fail("subspace iteration: 1 <= m <= 8, iters >= 0");
+  std::vector<double> h, x;
+  if (!subspaceIteration(P, m, iters, h, x)) return fail("subspace iteration: Gram matrix not positive definite");
+  std::memcpy(H, h.data(), h.size() * 8);
+  if (X) std::memcpy(X, x.data(), x.size() * 8);
+  return 0;
+}
+
 }  // extern "C"

@@ -29,6 +29,8 @@ def lib():
         _lib.b12cpu_scheme_points_per_box.restype = ctypes.c_uint64
         _lib.b12cpu_power_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_double, _P, _P, _P]
         _lib.b12cpu_set_options.argtypes = [_P, ctypes.c_int, ctypes.c_uint64]
+        _lib.b12cpu_energy_band_stencil.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P]
+        _lib.b12cpu_subspace_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
     return _lib
 
 
@@ -110,6 +112,13 @@ class OCoo:
         _chk(lib().b12cpu_power_iteration(self.h, int(max_iter), float(tol), _ptr(m), ctypes.byref(it),
                                           ctypes.byref(res)))
         return m, it.value, res.value
+
+    def subspace_iteration(self, m=4, iters=100, want_basis=False):
+        """(Ritz matrix H (m x m), basis X (m x n) or None) of the build-defined subspace iteration."""
+        H = np.zeros((m, m), np.float64)
+        X = np.zeros((m, self.nRows), np.float64) if want_basis else None
+        _chk(lib().b12cpu_subspace_iteration(self.h, int(m), int(iters), _ptr(H), _ptr(X) if want_basis else None))
+        return H, X
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -203,6 +212,14 @@ class OracleTree:
 
     def changeFlagsByStencil(self, st, fu, fs, depth=-1):
         return self._stencil(2, st, fu, fs, depth)
+
+    def energy_band_stencil(self, emin, emax):
+        st = np.zeros(self.size(), np.uint8)
+        _chk(lib().b12cpu_energy_band_stencil(self.h, float(emin), float(emax), _ptr(st)))
+        return st
+
+    def setFlagsByEnergyBand(self, emin, emax, flags, depth=-1):
+        return self.setFlagsByStencil(self.energy_band_stencil(emin, emax), flags, depth)
 
     def scheme(self, grid, mp):
         key = (grid.kind, tuple(grid.params), mp.kind, tuple(mp.params))

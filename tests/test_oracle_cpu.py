@@ -368,6 +368,94 @@ def test_power_iteration_matches_numpy_eig():
     assert np.allclose(mu, v, rtol=1e-9, atol=1e-13)
 
 
+def hh_energy(x):
+    q1, q2, p1, p2 = x[..., 0], x[..., 1], x[..., 2], x[..., 3]
+    return 0.5 * (p1 ** 2 + p2 ** 2) + 0.5 * (q1 ** 2 + q2 ** 2) + q1 ** 2 * q2 - q2 ** 3 / 3.0
+
+
+def leaf_boxes(o, c, r):
+    """Centre and radius of every leaf, from the bit pattern (independent of the oracle)."""
+    bits, depth, _ = o.download()
+    out = []
+    for b, d in zip(bits.tolist(), depth.tolist()):
+        lo, hi = np.array(c) - np.array(r), np.array(c) + np.array(r)
+        for lvl in range(d):  # default scheme: coordinate lvl % 4, bit d-1-lvl of the pattern
+            j = lvl % 4
+            mid = 0.5 * (lo[j] + hi[j])
+            if (b >> (d - 1 - lvl)) & 1:
+                lo[j] = mid
+            else:
+                hi[j] = mid
+        out.append((lo, hi))
+    return out
+
+
+def test_energy_band_predicate_contains_the_energy_range():
+    """The band predicate's interval bound contains H at every sampled point of every box
+    (soundness), and on small boxes its width stays within a few times the sampled range."""
+    rng = np.random.default_rng(21)
+    c, r = [0.0, 0.0, 0.0, 0.0], [0.6, 0.6, 0.5, 0.5]
+    o = OracleTree(c, r)
+    bits, depth, flags = random_tree(rng, 4, 600, max_depth=16)
+    o.upload(bits, depth, flags)
+    boxes = leaf_boxes(o, c, r)
+    for emin, emax in [(0.11, 0.12), (0.0, 0.01), (-0.05, 0.0), (0.2, 0.3)]:
+        st = o.energy_band_stencil(emin, emax)
+        for (lo, hi), s in zip(boxes, st):
+            x = lo + (hi - lo) * rng.random((400, 4))
+            corners = np.array([[lo[j] if (m >> j) & 1 else hi[j] for j in range(4)] for m in range(16)])
+            h = hh_energy(np.vstack([x, corners]))
+            if np.any((h >= emin) & (h <= emax)):
+                assert s == 1, "a box containing band points was excluded"
+    # tightness on small boxes: the predicate keeps few boxes whose sampled range misses the band
+    o2 = OracleTree(c, r)
+    deep = random_tree(rng, 4, 3000, max_depth=24)
+    o2.upload(*deep)
+    boxes = leaf_boxes(o2, c, r)
+    st = o2.energy_band_stencil(0.11, 0.12)
+    far = 0
+    for (lo, hi), s in zip(boxes, st):
+        x = lo + (hi - lo) * rng.random((200, 4))
+        h = hh_energy(x)
+        width = h.max() - h.min()
+        if s and (h.min() > 0.12 + 4 * width + 1e-12 or h.max() < 0.11 - 4 * width - 1e-12):
+            far += 1
+    assert far == 0
+
+
+def test_subspace_iteration_cycle_walk_closed_form():
+    """Lazy random walk on a 20-cycle: eigenvalues 0.4 + 0.6 cos(2 pi k / 20); the four leading
+    are 1, lambda_1 (double) and lambda_2."""
+    n, p = 20, 0.3
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, v in ((i, 1 - 2 * p), ((i + 1) % n, p), ((i - 1) % n, p)):
+            rows.append(i); cols.append(j); vals.append(v)
+    o = np.lexsort((cols, rows))
+    m = OCoo.from_arrays(np.array(rows)[o], np.array(cols)[o], np.array(vals)[o], n, n)
+    H, _ = m.subspace_iteration(4, 400)
+    ev = np.sort(np.linalg.eigvals(H).real)[::-1]
+    lam = 0.4 + 0.6 * np.cos(2 * np.pi * np.arange(3) / n)
+    assert np.allclose(ev, [lam[0], lam[1], lam[1], lam[2]], rtol=0, atol=1e-10)
+
+
+def test_subspace_iteration_matches_numpy_eig():
+    rng = np.random.default_rng(22)
+    n = 60
+    A = rng.random((n, n)) * (rng.random((n, n)) < 0.2)
+    # four weakly coupled blocks give four real leading eigenvalues close to 1 (almost-invariant sets)
+    for b in range(4):
+        A[b * 15:(b + 1) * 15, b * 15:(b + 1) * 15] += 2.0 * rng.random((15, 15))
+    P = A / A.sum(axis=1, keepdims=True)
+    r, c = np.nonzero(P)
+    H, X = OCoo.from_arrays(r, c, P[r, c], n, n).subspace_iteration(4, 300, want_basis=True)
+    ev = np.sort(np.linalg.eigvals(H).real)[::-1]
+    w = np.linalg.eigvals(P.T)
+    ref = np.sort(w[np.argsort(-np.abs(w))[:4]].real)[::-1]
+    assert np.allclose(ev, ref, rtol=0, atol=1e-10)
+    assert np.allclose(X @ X.T, np.eye(4), atol=1e-12)
+
+
 # ------------------------------------------------------------------ HIP library C ABI
 def header_symbols():
     text = open(os.path.join(ROOT, "include", "b12x_c_api.h")).read()
