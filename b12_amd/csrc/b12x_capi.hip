@@ -8,7 +8,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/b12x_c_api.h"
@@ -285,15 +287,92 @@ const uint64_t* stageSamples(b12x_tree_s& T, const uint32_t* nDev) {
   return T.samples;
 }
 
-template <int DIM, int MAP>
-void launchImage2(const ImageArgs& a, int gridKind, hipStream_t st) {
-  const uint64_t blocks64 = (a.nPoints + kBlock - 1) / kBlock;
-  const uint32_t blocks = uint32_t(std::min<uint64_t>(blocks64, 1u << 22));
+// Image-kernel variants: test points per lane x block size.
+struct ImageVariant {
+  int ppl, bs;
+};
+constexpr ImageVariant kImageVariants[] = {{1, 256}, {1, 128}, {1, 64}, {2, 256}, {2, 128},
+                                           {2, 64},  {4, 256}, {4, 128}, {4, 64}};
+constexpr int kNumImageVariants = int(sizeof(kImageVariants) / sizeof(kImageVariants[0]));
+constexpr uint64_t kTuneSamplePoints = 1ull << 21;
+
+template <int DIM, int MAP, int PPL>
+void launchImage3(const ImageArgs& a, int gridKind, int bs, hipStream_t st) {
+  const uint64_t per = uint64_t(bs) * PPL;
+  const uint32_t blocks = uint32_t(std::min<uint64_t>((a.nPoints + per - 1) / per, 1u << 22));
   if (blocks == 0) return;
   if (gridKind == GRID_INNER)
-    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_INNER>), dim3(blocks), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_INNER, PPL>), dim3(blocks), dim3(bs), 0, st, a);
   else
-    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_UNIFORM>), dim3(blocks), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_UNIFORM, PPL>), dim3(blocks), dim3(bs), 0, st, a);
+}
+
+template <int DIM, int MAP>
+void launchVariant(const ImageArgs& a, int gridKind, ImageVariant v, hipStream_t st) {
+  switch (v.ppl) {
+    case 4: launchImage3<DIM, MAP, 4>(a, gridKind, v.bs, st); break;
+    case 2: launchImage3<DIM, MAP, 2>(a, gridKind, v.bs, st); break;
+    default: launchImage3<DIM, MAP, 1>(a, gridKind, v.bs, st); break;
+  }
+}
+
+// Autotuner: for each (dimension, map kind, grid kind, log2 of the launch's point count) the
+// first launch runs every variant once untimed (code loading) and once timed with HIP events on
+// a prefix of at most kTuneSamplePoints of its points, and later launches use the fastest.
+// Every variant computes the same images and a repeated launch stores the same hit bytes or
+// targets, so tuning never changes a result.
+// B12X_IMAGE_TUNE=0: one point per lane, 256 threads; B12X_IMAGE_PPL / B12X_IMAGE_BLOCK fix a
+// variant. The staged descent always takes one point per lane.
+std::mutex g_tuneMutex;
+std::unordered_map<uint64_t, int> g_tuned;
+
+int fixedImageVariant(const ImageArgs& a) {
+  if (a.samples) return 0;
+  const char* t = std::getenv("B12X_IMAGE_TUNE");
+  const char* p = std::getenv("B12X_IMAGE_PPL");
+  const char* b = std::getenv("B12X_IMAGE_BLOCK");
+  if (!p && !b && !(t && t[0] == '0')) return -1;
+  const int ppl = p ? std::atoi(p) : 1, bs = b ? std::atoi(b) : 256;
+  for (int v = 0; v < kNumImageVariants; ++v)
+    if (kImageVariants[v].ppl == ppl && kImageVariants[v].bs == bs) return v;
+  return 0;
+}
+
+template <int DIM, int MAP>
+void launchImage2(const ImageArgs& a, int gridKind, hipStream_t st) {
+  if (a.nPoints == 0) return;
+  const int fixed = fixedImageVariant(a);
+  if (fixed >= 0) return launchVariant<DIM, MAP>(a, gridKind, kImageVariants[fixed], st);
+  int bucket = 0;
+  while ((a.nPoints >> (bucket + 1)) != 0) ++bucket;
+  const uint64_t key = (uint64_t(DIM) << 48) | (uint64_t(MAP) << 40) | (uint64_t(gridKind) << 32) | uint64_t(bucket);
+  std::lock_guard<std::mutex> lock(g_tuneMutex);
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) return launchVariant<DIM, MAP>(a, gridKind, kImageVariants[it->second], st);
+  ImageArgs sample = a;
+  sample.nPoints = std::min<uint64_t>(a.nPoints, kTuneSamplePoints);
+  for (int v = 0; v < kNumImageVariants; ++v) launchVariant<DIM, MAP>(sample, gridKind, kImageVariants[v], st);
+  hipEvent_t ev[kNumImageVariants + 1];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], st));
+  for (int v = 0; v < kNumImageVariants; ++v) {
+    launchVariant<DIM, MAP>(sample, gridKind, kImageVariants[v], st);
+    HIPCHK(hipEventRecord(ev[v + 1], st));
+  }
+  HIPCHK(hipEventSynchronize(ev[kNumImageVariants]));
+  int best = 0;
+  float bestMs = 0.0f;
+  for (int v = 0; v < kNumImageVariants; ++v) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev[v], ev[v + 1]));
+    if (v == 0 || ms < bestMs) {
+      best = v;
+      bestMs = ms;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  g_tuned[key] = best;
+  launchVariant<DIM, MAP>(a, gridKind, kImageVariants[best], st);  // the whole launch
 }
 
 void launchImage(int dim, const ImageArgs& a, int gridKind, hipStream_t st) {

@@ -333,6 +333,92 @@ __device__ inline void applyMap(const MapParams& M, const double* in, double* ou
   }
 }
 
+// The same maps for P points at once: every integrator step runs over the P points before the
+// next, so a lane carries P independent FP64 chains. Each point's operations and their order are
+// those of applyMap.
+template <int DIM, int MAP, int P>
+__device__ inline void applyMapN(const MapParams& M, double (&x)[P][DIM], double (&y)[P][DIM]) {
+  if constexpr (MAP == MAP_LORENZ) {  // MapTools.hpp:116-182 RK4Auto
+    const double s = M.p[0], r = M.p[1], b = M.p[2], h = M.p[3], h6 = M.p[4], h2 = M.p[5], h3 = M.p[6];
+    double z[P][3], rr[P][3], a[P][3], k[P][3];
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) z[q][j] = x[q][j];
+    for (uint32_t st = 0; st < M.steps; ++st) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) lorenzRhs(s, r, b, z[q], k[q]);
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          rr[q][j] = z[q][j] + h6 * k[q][j];
+          a[q][j] = z[q][j] + h2 * k[q][j];
+        }
+#pragma unroll
+      for (int q = 0; q < P; ++q) lorenzRhs(s, r, b, a[q], k[q]);
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          rr[q][j] += h3 * k[q][j];
+          a[q][j] = z[q][j] + h2 * k[q][j];
+        }
+#pragma unroll
+      for (int q = 0; q < P; ++q) lorenzRhs(s, r, b, a[q], k[q]);
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          rr[q][j] += h3 * k[q][j];
+          a[q][j] = z[q][j] + h * k[q][j];
+        }
+#pragma unroll
+      for (int q = 0; q < P; ++q) lorenzRhs(s, r, b, a[q], k[q]);
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) z[q][j] = rr[q][j] + h6 * k[q][j];
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) y[q][j] = z[q][j];
+  } else if constexpr (MAP == MAP_HENON_HEILES) {
+    if (M.order != 2) {  // orders 1, 3, 4 point by point
+#pragma unroll
+      for (int q = 0; q < P; ++q) applyMap<DIM, MAP>(M, x[q], y[q]);
+      return;
+    }
+    const double h = M.p[0];  // MapTools.hpp:399-420 Störmer-Verlet
+    double qq[P][2], pp[P][2], t[P][2];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      qq[q][0] = x[q][0]; qq[q][1] = x[q][1]; pp[q][0] = x[q][2]; pp[q][1] = x[q][3];
+    }
+    for (uint32_t st = 0; st < M.steps; ++st) {
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        qq[q][0] += 0.5 * pp[q][0] * h; qq[q][1] += 0.5 * pp[q][1] * h;
+      }
+#pragma unroll
+      for (int q = 0; q < P; ++q) hhForce(qq[q], t[q]);
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        pp[q][0] += t[q][0] * h; pp[q][1] += t[q][1] * h;
+        qq[q][0] += 0.5 * pp[q][0] * h; qq[q][1] += 0.5 * pp[q][1] * h;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      y[q][0] = qq[q][0]; y[q][1] = qq[q][1]; y[q][2] = pp[q][0]; y[q][3] = pp[q][1];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < P; ++q) applyMap<DIM, MAP>(M, x[q], y[q]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Grids (Grids.hpp:99-176). InnerGrid offsets depend only on i mod k, so they come from a table
 // the host fills with the reference's expression. UniformGrid follows the host branch:
@@ -417,6 +503,44 @@ __device__ inline uint32_t locate(const uint64_t* __restrict__ keys, const uint8
   if (leafContains(pk, keys[i1], depth[i1])) return i1;
   if (leafContains(pk, keys[i2], depth[i2])) return s;
   return kInvalid;
+}
+
+// locate() for P keys at once: a lower bound whose trip count depends on n only (so the P
+// searches run in lockstep, each probe of every key in flight together), then locate()'s leaf
+// checks and its invalid-point case. n >= 1.
+template <int P>
+__device__ inline void locateN(const uint64_t* __restrict__ keys, const uint8_t* __restrict__ depth, uint32_t n,
+                               const uint64_t (&pk)[P], uint32_t (&out)[P]) {
+  uint32_t base[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) base[q] = 0;
+  uint32_t len = n;
+  while (len > 1) {
+    const uint32_t half = len >> 1;
+    uint64_t kv[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) kv[q] = keys[base[q] + half];
+#pragma unroll
+    for (int q = 0; q < P; ++q) base[q] = kv[q] < pk[q] ? base[q] + half : base[q];
+    len -= half;
+  }
+  uint64_t kb[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) kb[q] = keys[base[q]];
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const uint32_t s = base[q] + (kb[q] < pk[q] ? 1u : 0u);  // lower bound
+    if (pk[q] == kInvalidKey) {
+      const uint32_t l = n - 1;
+      out[q] = bitsOf(keys[l], depth[l]) == 0ull ? l : kInvalid;
+      continue;
+    }
+    const uint32_t i1 = s > 0 ? s - 1 : 0;
+    const uint32_t i2 = s < n - 1 ? s : n - 1;
+    if (leafContains(pk[q], keys[i1], depth[i1])) out[q] = i1;
+    else if (leafContains(pk[q], keys[i2], depth[i2])) out[q] = s;
+    else out[q] = kInvalid;
+  }
 }
 
 // Energy of the Hénon-Heiles Hamiltonian over a box (build-defined band predicate; no reference

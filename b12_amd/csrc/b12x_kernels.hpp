@@ -413,35 +413,102 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
   }
 }
 
-template <int DIM, int MAP, int GRID>
+// PPL points of one lane at once (local indices idx[q], idx[0] < np): lanes past the end repeat
+// point idx[0] and store nothing. Same arithmetic per point as imagePoint.
+template <int DIM, int MAP, int GRID, int PPL>
+__device__ inline void imagePointsN(const ImageArgs& a, const uint32_t (&idx)[PPL], uint32_t np, uint32_t n) {
+  const uint32_t k = a.grid.k, base = uint32_t(a.pointBase);
+  double x[PPL][DIM], y[PPL][DIM];
+  bool valid[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    valid[q] = idx[q] < np;
+    const uint32_t gi = base + (valid[q] ? idx[q] : idx[0]);
+    const uint32_t b = gi / k;
+    const uint32_t src = a.srcList ? a.srcList[b] : b;
+    double c[DIM], rb[DIM], g[DIM];
+    boxCoords<DIM>(a.dom, a.srcKey[src], a.srcDepth[src], c, rb);
+    if constexpr (GRID == GRID_INNER) {
+      const uint32_t gl = gi - b * k;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) g[j] = a.grid.table[gl * DIM + j];
+    } else {
+      uniformGridPoint<DIM>(a.grid.seed, gi, g);
+    }
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) x[q][j] = c[j] + g[j] * rb[j];
+  }
+  applyMapN<DIM, MAP, PPL>(a.map, x, y);
+  uint64_t pk[PPL];
+  uint32_t t[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) pk[q] = pointKey<DIM>(a.dom, y[q]);
+  if (a.depth0) {
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) t[q] = (pk[q] == kInvalidKey) ? kInvalid : 0u;
+  } else {
+    locateN<PPL>(a.key, a.depth, n, pk, t);
+  }
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    if (!valid[q]) continue;
+    if (a.mode == 0) {
+      if (t[q] != kInvalid) a.hit[t[q]] = 1;
+    } else {
+      a.out[idx[q]] = t[q];
+    }
+  }
+}
+
+// Any block size up to kBlock (the launch picks it); PPL points per lane.
+template <int DIM, int MAP, int GRID, int PPL>
 static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
-  __shared__ uint64_t sm[kDescentSamples];
-  const uint64_t nPoints = a.nBoxesDev ? uint64_t(*a.nBoxesDev) * a.grid.k : a.nPoints;
+  const uint32_t BS = blockDim.x;
+  __shared__ uint64_t sm[PPL == 1 ? kDescentSamples : 1];
+  // a device-side count is capped by the host's bound (a prefix when the launch is a sample)
+  const uint64_t nPoints = a.nBoxesDev ? min(uint64_t(*a.nBoxesDev) * a.grid.k, a.nPoints) : a.nPoints;
   const uint32_t n = a.nBoxesDev ? *a.nBoxesDev : a.n;
-  if (uint64_t(blockIdx.x) * kBlock >= nPoints) return;  // grid sized for a bound: no points here
+  if (uint64_t(blockIdx.x) * BS * PPL >= nPoints) return;  // grid sized for a bound: no points here
+  if constexpr (PPL > 1) {
+    if (a.pointBase + nPoints + uint64_t(PPL) * BS <= 0xFFFFFFFFull &&
+        uint64_t(gridDim.x) * BS * PPL <= 0xFFFFFFFFull) {
+      const uint32_t np = uint32_t(nPoints), stride = gridDim.x * BS * PPL;
+      for (uint32_t i0 = blockIdx.x * BS * PPL + threadIdx.x; i0 < np; i0 += stride) {
+        uint32_t idx[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) idx[q] = i0 + q * BS;
+        imagePointsN<DIM, MAP, GRID, PPL>(a, idx, np, n);
+      }
+      return;
+    }
+  }
+  // one point per lane (and every launch too large for 32-bit indices)
   // upper levels of the descent: the samples of the leaf keys, staged in LDS
   uint32_t S = 0;
-  if (a.samples && !a.depth0 && n > 0) {
+  if (PPL == 1 && a.samples && !a.depth0 && n > 0) {
     S = min(kDescentSamples, n);
-    for (uint32_t j = threadIdx.x; j < S; j += kBlock) sm[j] = a.samples[j];
+    for (uint32_t j = threadIdx.x; j < S; j += BS) sm[j] = a.samples[j];
     __syncthreads();
   }
-  if (a.pointBase + nPoints <= 0xFFFFFFFFull && uint64_t(gridDim.x) * kBlock <= 0xFFFFFFFFull) {
+  if (PPL == 1 && a.pointBase + nPoints <= 0xFFFFFFFFull && uint64_t(gridDim.x) * BS <= 0xFFFFFFFFull) {
     // 32-bit index arithmetic (the common case)
     const uint32_t k = a.grid.k, base = uint32_t(a.pointBase), np = uint32_t(nPoints);
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < np; i += stride) {
+    const uint32_t stride = gridDim.x * BS;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < np; i += stride) {
       const uint32_t gi = base + i;
       const uint32_t b = gi / k;
       imagePoint<DIM, MAP, GRID>(a, i, gi, b, gi - b * k, n, sm, S);
     }
   } else {
-    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < nPoints; i += stride) {
-      const uint64_t gi = a.pointBase + i;
-      const uint64_t b = gi / a.grid.k;
-      imagePoint<DIM, MAP, GRID>(a, i, gi, b, uint32_t(gi - b * a.grid.k), n, sm, S);
-    }
+    const uint64_t stride = uint64_t(gridDim.x) * BS * PPL;
+    for (uint64_t i0 = uint64_t(blockIdx.x) * BS * PPL + threadIdx.x; i0 < nPoints; i0 += stride)
+      for (int q = 0; q < PPL; ++q) {
+        const uint64_t i = i0 + uint64_t(q) * BS;
+        if (i >= nPoints) break;
+        const uint64_t gi = a.pointBase + i;
+        const uint64_t b = gi / a.grid.k;
+        imagePoint<DIM, MAP, GRID>(a, i, gi, b, uint32_t(gi - b * a.grid.k), n, sm, S);
+      }
   }
 }
 

@@ -374,3 +374,26 @@ def test_transition_matrix_more_than_64_points_per_box(grid, use_invalid, row_ma
     assert (mg.nnz, mg.nRows, mg.nColumns) == (mo.nnz, mo.nRows, mo.nColumns)
     for a, b in zip(mg.download(), mo.download()):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("ppl,block", [(1, 64), (2, 128), (4, 256), (4, 64), (2, 256)])
+def test_image_kernel_variants_are_bit_exact(monkeypatch, ppl, block):
+    """Every (points per lane, block size) variant the autotuner chooses between gives the
+    oracle's covers, matrices and search results: Lorenz with InnerGrid (subdivision + matrix),
+    Hénon-Heiles with UniformGrid on a random tree."""
+    print(f"[variant] ppl={ppl} block={block}", flush=True)
+    monkeypatch.setenv("B12X_IMAGE_PPL", str(ppl))
+    monkeypatch.setenv("B12X_IMAGE_BLOCK", str(block))
+    g, o = subdivision_run([0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz(), 11)
+    for a, b in zip(g.transitionMatrixForLeaves(gx.InnerGrid(3), gx.Lorenz()).download(),
+                    o.transitionMatrixForLeaves(gx.InnerGrid(3), gx.Lorenz()).download()):
+        assert np.array_equal(a, b)
+    rng = np.random.default_rng(41)
+    g2, o2 = both([0.0, 0.0, 0.0, 0.0], [0.6, 0.6, 0.5, 0.5])
+    bits, depth, flags = random_tree(rng, 4, 900, max_depth=14)
+    g2.upload(bits, depth, flags)
+    o2.upload(bits, depth, flags)
+    grid = gx.UniformGrid(7, 3)
+    ig = g2.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
+    io = o2.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
+    assert np.array_equal(g2.searchByIterators(ig), o2.searchByIterators(io))
