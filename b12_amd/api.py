@@ -390,6 +390,9 @@ class ImplicitBoxTree:
     def makePointIteratorPairOverRoot(self, grid, mp):
         return PointIterator(self.scheme(grid, mp), 0)
 
+    def makePointIteratorPairOverDepth(self, depth, grid, mp):
+        return self.makePointIteratorPairByFlags(depth, 0, grid, mp)
+
     def makePointIteratorPairByFlags(self, depth, flags_to_check, grid, mp):
         return PointIterator(self.scheme(grid, mp), depth, flags_to_check)
 
@@ -443,6 +446,10 @@ class ImplicitBoxTree:
 
     def transitionMatrixForLeaves(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):
         return self.transitionMatrix(self.makePointIteratorPairOverLeaves(grid, mp), LEAF, use_abs, row_major,
+                                     use_invalid)
+
+    def transitionMatrixForDepth(self, depth, grid, mp, use_abs=False, row_major=False, use_invalid=False):
+        return self.transitionMatrix(self.makePointIteratorPairOverDepth(depth, grid, mp), depth, use_abs, row_major,
                                      use_invalid)
 
     def transitionMatrixForRoot(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):

@@ -615,18 +615,72 @@ class Tree {
     return 0;
   }
 
-  // Depth variants other than {0, leaf, all-leaves-on-depth} are not part of this restatement.
-  void requireSupportedDepth(Depth d) const {
-    if (!(d == 0 || d == kLeafDepth || areAllLeavesOnDepth(d)))
-      throw std::runtime_error("depth must be 0, -1 (leaves) or the common depth of all leaves");
+  // Depths other than the root, the leaves and a depth every leaf has: a depth d between them
+  // (0 < d <= depth()) or the unsubdividable depth. The boxes at such a depth are the leaves'
+  // ancestors at d (parents that could be merged, at the unsubdividable depth).
+  bool isGeneralDepth(Depth d) const { return d != 0 && d != kLeafDepth && !areAllLeavesOnDepth(d); }
+  bool depthExists(Depth d) const { return d == kUnsubDepth || d <= depth(); }
+  // leaf i belongs to a box at depth d: hasAtLeastDepth, or unsubdividable at the unsubdividable depth
+  bool inDepth(size_t i, Depth d) const {
+    return d == kUnsubDepth ? isUnsubdividable(i, NONE) : hasAtLeastDepth(box(i), d);
+  }
+  // leaf i is the first leaf of a box at depth d (ImplicitBoxTree.hpp:1990-2021, and with
+  // isUnsubdividable at the unsubdividable depth)
+  bool startsBoxInDepth(size_t i, Depth d) const {
+    return isNewBoxInDepth(i, d) && (d != kUnsubDepth || isUnsubdividable(i, NONE));
+  }
+
+  // ImplicitBoxTree.hpp:1383-1473 initializeIndicesInDepth: index of the depth-d box of every
+  // leaf (inclusive count of box starts, minus one where positive)
+  std::vector<NrBoxes> indicesInDepth(Depth d) const {
+    std::vector<NrBoxes> idx(bits.size(), 0);
+    if (d == 0) return idx;
+    if (d == kLeafDepth || areAllLeavesOnDepth(d)) {
+      std::iota(idx.begin(), idx.end(), NrBoxes(0));
+      return idx;
+    }
+    NrBoxes acc = 0;
+    for (size_t i = 0; i < idx.size(); ++i) {
+      acc += startsBoxInDepth(i, d) ? 1 : 0;
+      idx[i] = acc > 0 ? acc - 1 : 0;
+    }
+    return idx;
+  }
+
+  // ImplicitBoxTree.hpp:1097-1214 countIfFlags at a general depth: the combination of the flags
+  // of every leaf of a box (reverse scan over runs of equal (index, inDepth)) tested at the box's
+  // first leaf
+  template <class Pred, class Comb>
+  NrBoxes countIfFlagsInDepth(const uint8_t* stencil, Depth d, Pred pred, Comb comb,
+                              const std::vector<NrBoxes>& idx) const {
+    if (!depthExists(d)) return 0;
+    const size_t n = bits.size();
+    std::vector<Flags> acc(n);
+    std::vector<uint8_t> in(n);
+    for (size_t i = 0; i < n; ++i) in[i] = inDepth(i, d);
+    for (size_t i = n; i-- > 0;) {
+      const bool same = i + 1 < n && idx[i + 1] == idx[i] && in[i + 1] == in[i];
+      acc[i] = same ? comb(acc[i + 1], flagsV[i]) : flagsV[i];
+    }
+    NrBoxes c = 0;
+    for (size_t i = 0; i < n; ++i) c += (startsBoxInDepth(i, d) && pred(acc[i]) && stencil[idx[i]]) ? 1 : 0;
+    return c;
   }
 
   // countIfFlags (ImplicitBoxTree.hpp:1097-1214) fused into the stencil flag operations
   // setFlagsByStencil :2803-2881, unsetFlagsByStencil :3334-3411, changeFlagsByStencil :912-1058
   template <class Pred, class Comb, class Apply>
   NrBoxes flagsByStencil(const uint8_t* stencil, Depth d, Pred pred, Comb comb, Apply apply) {
-    requireSupportedDepth(d);
     NrBoxes n = 0;
+    if (isGeneralDepth(d)) {  // set on every leaf of a selected box (:2848-2878)
+      if (!depthExists(d)) return 0;
+      const std::vector<NrBoxes> idx = indicesInDepth(d);
+      n = countIfFlagsInDepth(stencil, d, pred, comb, idx);
+      std::vector<uint8_t> sel(bits.size());
+      for (size_t i = 0; i < sel.size(); ++i) sel[i] = inDepth(i, d) && stencil[idx[i]];
+      for (size_t i = 0; i < sel.size(); ++i) if (sel[i]) flagsV[i] = apply(flagsV[i]);
+      return n;
+    }
     if (d == 0) {
       Flags acc = flagsV.front();
       for (Flags f : flagsV) acc = comb(acc, f);
@@ -665,7 +719,6 @@ class Tree {
   // searchVector is sorted and unique; a trailing invalid entry is ignored.
   template <class Pred, class Apply, class AllPred>
   NrBoxes flagsBySearch(const std::vector<NrBoxes>& sv, Depth d, Pred pred, Apply apply, AllPred rootPred) {
-    requireSupportedDepth(d);
     size_t m = sv.size() - ((!sv.empty() && sv.back() == kInvalid) ? 1 : 0);
     if (d == 0) {
       bool hit = false;
@@ -683,17 +736,38 @@ class Tree {
   }
 
   NrBoxes setFlagsBySearch(const std::vector<NrBoxes>& sv, Flags f, Depth d) {
+    if (isGeneralDepth(d)) {  // :2780-2795 a stencil over the boxes at depth d
+      if (!depthExists(d)) return 0;
+      auto st = searchStencil(sv, d);
+      return setFlagsByStencil(st.data(), f, d);
+    }
     return flagsBySearch(sv, d, [f](Flags x) { return !areFlagsSet(x, f); }, [f](Flags x) { return setF(x, f); },
                          [&]() { for (Flags x : flagsV) if (!areFlagsSet(x, f)) return true; return false; });
   }
   NrBoxes unsetFlagsBySearch(const std::vector<NrBoxes>& sv, Flags f, Depth d) {
+    if (isGeneralDepth(d)) {  // :2780-2795 a stencil over the boxes at depth d
+      if (!depthExists(d)) return 0;
+      auto st = searchStencil(sv, d);
+      return unsetFlagsByStencil(st.data(), f, d);
+    }
     return flagsBySearch(sv, d, [f](Flags x) { return isAnyFlagSet(x, f); }, [f](Flags x) { return unsetF(x, f); },
                          [&]() { for (Flags x : flagsV) if (isAnyFlagSet(x, f)) return true; return false; });
   }
   NrBoxes changeFlagsBySearch(const std::vector<NrBoxes>& sv, Flags fu, Flags fs, Depth d) {
+    if (isGeneralDepth(d)) {
+      if (!depthExists(d)) return 0;
+      auto st = searchStencil(sv, d);
+      return changeFlagsByStencil(st.data(), fu, fs, d);
+    }
     return flagsBySearch(sv, d, [fu](Flags x) { return areFlagsSet(x, fu); },
                          [fu, fs](Flags x) { return changeF(x, fu, fs); },
                          [&]() { for (Flags x : flagsV) if (!areFlagsSet(x, fu)) return false; return true; });
+  }
+
+  std::vector<uint8_t> searchStencil(const std::vector<NrBoxes>& sv, Depth d) const {
+    std::vector<uint8_t> st(std::max<NrBoxes>(count(d), 1), 0);
+    for (NrBoxes r : sv) if (r != kInvalid) st[r] = 1;
+    return st;
   }
 
   // ImplicitBoxTree.hpp:2343-2371 remove: keep leaves having all of flags (stable)
@@ -773,8 +847,16 @@ class Tree {
     } else if (d == kLeafDepth || areAllLeavesOnDepth(d)) {
       for (size_t i = 0; i < bits.size(); ++i)
         if (check == NONE || areFlagsSet(flagsV[i], check)) ps.boxes.push_back(box(i));
-    } else {
-      throw std::runtime_error("point iterator depth must be 0 or -1");
+    } else if (depthExists(d)) {  // :1528-1601 the first leaf of every box at depth d
+      const size_t n = bits.size();
+      std::vector<uint8_t> ok(n, 1);
+      if (check != NONE)  // AND of the check over runs of leaves with the same box at depth d
+        for (size_t i = n; i-- > 0;) {
+          const bool same = i + 1 < n && isSameBox(decrementDepthTo(box(i + 1), d), decrementDepthTo(box(i), d));
+          ok[i] = areFlagsSet(flagsV[i], check) && (same ? ok[i + 1] : 1);
+        }
+      for (size_t i = 0; i < n; ++i)
+        if (startsBoxInDepth(i, d) && ok[i]) ps.boxes.push_back(decrementDepthTo(box(i), d));
     }
     return ps;
   }
@@ -810,40 +892,57 @@ class Tree {
     return kInvalid;
   }
 
-  NrBoxes searchOne(const Box& p, Depth d) const {
+  // ComputeSearchIndexFunctor with the two neighbours decremented to depth d and their inDepth
+  // flags, then the leaf's box index at d (ImplicitBoxTree.hpp:2433-2566)
+  NrBoxes searchIndexInDepth(NrBoxes s, const Box& p, Depth d, const std::vector<NrBoxes>& idx) const {
+    NrBoxes n = size();
+    const NrBoxes i1 = s > 0 ? s - 1 : 0, i2 = std::min<NrBoxes>(s, n - 1);
+    const Box b1 = decrementDepthTo(box(i1), d), b2 = decrementDepthTo(box(i2), d);
+    NrBoxes r = kInvalid;
+    if (inDepth(i1, d) && p.depth >= b1.depth && hasSamePath(p, b1)) r = i1;
+    else if (inDepth(i2, d) && p.depth >= b2.depth && hasSamePath(p, b2)) r = s;
+    return r == kInvalid ? kInvalid : idx[r];
+  }
+
+  NrBoxes searchOne(const Box& p, Depth d, const std::vector<NrBoxes>* idx = nullptr) const {
     if (d == 0) return isInvalid(p) ? kInvalid : 0;
+    if (isGeneralDepth(d)) return depthExists(d) ? searchIndexInDepth(lowerBound(p), p, d, *idx) : kInvalid;
     return searchIndex(lowerBound(p), p);
   }
 
   // ImplicitBoxTree.hpp:2373-2432 searchByIterators (depth 0 and leaf depth)
   std::vector<NrBoxes> search(const PointSet<DIM>& ps, Depth d) const {
-    requireSupportedDepth(d);
     uint64_t P = ps.size();
     std::vector<NrBoxes> out(P);
+    const bool general = isGeneralDepth(d);
+    std::vector<NrBoxes> idx;
+    if (general) idx = indicesInDepth(d);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < int64_t(P); ++i) {
       Box p = pointBox(ps, uint64_t(i));
       if (evalTwice) {
         NrBoxes s = (d == 0) ? 0 : lowerBound(p);
         Box p2 = pointBox(ps, uint64_t(i));
-        out[i] = (d == 0) ? (isInvalid(p2) ? kInvalid : 0) : searchIndex(s, p2);
+        if (d == 0) out[i] = isInvalid(p2) ? kInvalid : 0;
+        else if (general) out[i] = depthExists(d) ? searchIndexInDepth(s, p2, d, idx) : kInvalid;
+        else out[i] = searchIndex(s, p2);
       } else {
-        out[i] = searchOne(p, d);
+        out[i] = searchOne(p, d, &idx);
       }
     }
     return out;
   }
 
   std::vector<NrBoxes> searchPoints(const double* pts, uint64_t n, Depth d) const {
-    requireSupportedDepth(d);
     std::vector<NrBoxes> out(n);
-    for (uint64_t i = 0; i < n; ++i) out[i] = searchOne(geo.boxFromPoint(pts + i * DIM, NONE), d);
+    std::vector<NrBoxes> idx;
+    if (isGeneralDepth(d)) idx = indicesInDepth(d);
+    for (uint64_t i = 0; i < n; ++i) out[i] = searchOne(geo.boxFromPoint(pts + i * DIM, NONE), d, &idx);
     return out;
   }
 
   // ImplicitBoxTree.hpp:2622-2681 searchCompacted: sorted unique hit indices, chunked
   std::vector<NrBoxes> searchCompacted(const PointSet<DIM>& ps, Depth d) const {
-    requireSupportedDepth(d);
     std::vector<NrBoxes> res;
     if (d == 0) {
       bool allInvalid = true;

@@ -233,6 +233,9 @@ class OracleTree:
     def makePointIteratorPairOverRoot(self, grid, mp):
         return OIter(self.scheme(grid, mp), 0)
 
+    def makePointIteratorPairOverDepth(self, depth, grid, mp):
+        return self.makePointIteratorPairByFlags(depth, 0, grid, mp)
+
     def makePointIteratorPairByFlags(self, depth, check, grid, mp):
         return OIter(self.scheme(grid, mp), depth, check)
 
@@ -296,6 +299,10 @@ class OracleTree:
 
     def transitionMatrixForLeaves(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):
         return self.transitionMatrix(self.makePointIteratorPairOverLeaves(grid, mp), -1, use_abs, row_major,
+                                     use_invalid)
+
+    def transitionMatrixForDepth(self, depth, grid, mp, use_abs=False, row_major=False, use_invalid=False):
+        return self.transitionMatrix(self.makePointIteratorPairOverDepth(depth, grid, mp), depth, use_abs, row_major,
                                      use_invalid)
 
     def transitionMatrixForRoot(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):

@@ -368,6 +368,100 @@ def test_power_iteration_matches_numpy_eig():
     assert np.allclose(mu, v, rtol=1e-9, atol=1e-13)
 
 
+# ------------------------------------------------------------------ depths between root and leaves
+def depth_groups(bits, depth, d):
+    """Boxes at depth d, brute force: the ancestors at d of the leaves of depth >= d, in leaf
+    (key) order, and for every leaf its group index (-1 for leaves shallower than d)."""
+    groups, gid = [], []
+    for b, dd in zip(bits.tolist(), depth.tolist()):
+        if dd < d:
+            gid.append(-1)
+            continue
+        a = b >> (dd - d)
+        if not groups or groups[-1] != a:
+            groups.append(a)
+        gid.append(len(groups) - 1)
+    return groups, np.array(gid)
+
+
+def general_tree(rng, dim=2, n=700, max_depth=14):
+    o = OracleTree([0.0] * dim, [1.0] * dim)
+    o.upload(*random_tree(rng, dim, n, max_depth=max_depth))
+    return o
+
+
+def test_count_and_flags_at_depths_between_root_and_leaves():
+    """count(d), and set/unset/changeFlags(ByStencil) at depth d against a brute-force grouping
+    of the leaves by their depth-d ancestor (flags go to every leaf of a selected box; the
+    return value counts the selected boxes whose leaves fail the predicate jointly)."""
+    rng = np.random.default_rng(51)
+    for trial in range(4):
+        o = general_tree(rng)
+        bits, depth, flags = o.download()
+        for d in range(1, int(depth.max()) + 1):
+            if np.all(depth == d):
+                continue
+            groups, gid = depth_groups(bits, depth, d)
+            assert o.count(d) == len(groups)
+            st = rng.random(len(groups)) < 0.5
+            f = int(rng.choice([0x01, 0x08, 0x10]))
+            before = o.download()[2].copy()
+            ret = o.setFlagsByStencil(st.astype(np.uint8), f, d)
+            after = o.download()[2]
+            sel = np.array([g >= 0 and st[g] for g in gid])
+            assert np.array_equal(after, np.where(sel, before | f, before))
+            exp = sum(1 for g in range(len(groups)) if st[g] and not np.all((before[gid == g] & f) == f))
+            assert ret == exp
+            ret = o.unsetFlagsByStencil(st.astype(np.uint8), f, d)
+            exp = sum(1 for g in range(len(groups)) if st[g] and np.any(after[gid == g] & f))
+            assert ret == exp
+            assert np.array_equal(o.download()[2], np.where(sel, after & np.uint8(0xFF ^ f), after))
+            o.upload(bits, depth, flags)
+
+
+def test_search_at_depth_maps_points_to_their_depth_box():
+    """A point is found in the box at depth d containing it iff that box holds a leaf of depth
+    >= d (a hole inside the box still finds it); otherwise invalid."""
+    rng = np.random.default_rng(52)
+    o = general_tree(rng, 2, 900, 12)
+    bits, depth, _ = o.download()
+    pts = rng.uniform(-1.2, 1.2, (3000, 2))
+    for d in (1, 3, 5, 8):
+        groups, _ = depth_groups(bits, depth, d)
+        pos = {a: i for i, a in enumerate(groups)}
+        got = o.searchByPoints(pts, d)
+        for p, g in zip(pts, got):
+            if np.any(p < -1.0) or np.any(p >= 1.0):
+                assert g == gx.INVALID
+                continue
+            a = 0  # depth-d ancestor of the point: bisect x then y, alternately
+            lo, hi = np.array([-1.0, -1.0]), np.array([1.0, 1.0])
+            for lvl in range(d):
+                j = lvl % 2
+                mid = 0.5 * (lo[j] + hi[j])
+                bit = 1 if p[j] >= mid else 0
+                a = (a << 1) | bit
+                if bit:
+                    lo[j] = mid
+                else:
+                    hi[j] = mid
+            assert g == pos.get(a, gx.INVALID)
+
+
+def test_identity_matrix_at_depth_and_unsubdividable_count():
+    rng = np.random.default_rng(53)
+    o = general_tree(rng, 2, 500, 10)
+    bits, depth, flags = o.download()
+    for d in (2, 4, 6):
+        m = o.transitionMatrixForDepth(d, gx.InnerGrid(3), gx.Identity(2))
+        r, c, v = m.download()
+        n = o.count(d)
+        assert m.nRows == m.nColumns == n
+        assert np.array_equal(r, np.arange(n)) and np.array_equal(c, np.arange(n)) and np.all(v == 1.0)
+    n_unsub = o.count(254)
+    assert n_unsub == o.unsubdivide(0)
+
+
 def hh_energy(x):
     q1, q2, p1, p2 = x[..., 0], x[..., 1], x[..., 2], x[..., 3]
     return 0.5 * (p1 ** 2 + p2 ** 2) + 0.5 * (q1 ** 2 + q2 ** 2) + q1 ** 2 * q2 - q2 ** 3 / 3.0
