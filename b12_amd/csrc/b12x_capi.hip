@@ -53,6 +53,9 @@ struct b12x_tree_s {
   uint32_t* hscal = nullptr;   // pinned host mirror of scal
   uint64_t* rootKey = nullptr; // the surrounding box as a one-element source
   uint64_t* samples = nullptr; // descent samples of the leaf keys (kDescentSamples)
+  uint64_t* srcKey = nullptr;  // sources of a point iterator at a general depth (truncated leaves)
+  uint8_t* srcDepth = nullptr;
+  uint32_t srcCap = 0;
   bool timing = false;          // HIP-event timing of image launches
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   std::vector<hipEvent_t> eventPool;
@@ -74,6 +77,8 @@ struct b12x_tree_s {
     dfree(rootKey);
     dfree(rootDepth);
     dfree(samples);
+    dfree(srcKey);
+    dfree(srcDepth);
     if (hscal) (void)hipHostFree(hscal);
     for (auto& p : pending) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
     for (auto e : eventPool) (void)hipEventDestroy(e);
@@ -184,6 +189,11 @@ __global__ void fillU8Kernel(uint8_t* p, uint64_t n, uint8_t v) {
   if (i < n) p[i] = v;
 }
 
+__global__ void fillU32Kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 template <int DIM, int MAP>
 __global__ void mapPointsKernel(MapParams m, const double* in, double* out, uint64_t n) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -222,17 +232,75 @@ LeafStats leafStats(b12x_tree_s& T) {
 }
 
 // Depth classes of the B12 API: 0 = root, 255 = leaves; a depth on which all leaves lie
-// behaves as the leaf depth (ImplicitBoxTree.hpp areAllLeavesOnDepth branches).
-enum DepthClass { DC_ROOT, DC_LEAF };
+// behaves as the leaf depth (ImplicitBoxTree.hpp areAllLeavesOnDepth branches). Other depths up
+// to the deepest leaf, and the unsubdividable depth, address the boxes at that depth
+// (DC_GENERAL); deeper ones address none (DC_BEYOND: counts 0, searches invalid).
+enum DepthClass { DC_ROOT, DC_LEAF, DC_GENERAL, DC_BEYOND };
 
 DepthClass classifyDepth(b12x_tree_s& T, uint8_t d) {
   if (d == 0) return DC_ROOT;
   if (d == kLeafDepth) return DC_LEAF;
-  if (d != kUnsubDepth) {
-    LeafStats s = leafStats(T);
-    if (s.minDepth == d && s.maxDepth == d) return DC_LEAF;
+  if (d == kUnsubDepth) return DC_GENERAL;
+  LeafStats s = leafStats(T);
+  if (s.minDepth == d && s.maxDepth == d) return DC_LEAF;
+  return d > s.maxDepth ? DC_BEYOND : DC_GENERAL;
+}
+
+// Boxes at a general depth (ImplicitBoxTree.hpp:1383-1473): box index, membership and box start
+// of every leaf. Owns its device arrays; callers synchronise the tree's stream before it ends.
+constexpr int kScalDepthCount = 16, kScalDepthFlags = 17;
+
+struct DepthMap {
+  uint32_t* box = nullptr;
+  uint8_t* in = nullptr;
+  uint8_t* start = nullptr;
+  uint32_t count = 0;
+  uint8_t d = 0;
+  DepthMap() = default;
+  DepthMap(const DepthMap&) = delete;
+  DepthMap& operator=(const DepthMap&) = delete;
+  ~DepthMap() {
+    dfree(box);
+    dfree(in);
+    dfree(start);
   }
-  throw HipFail(B12X_E_UNSUPPORTED, "depth must be 0, 255 (leaves) or the common depth of all leaves");
+};
+
+void buildDepthMap(b12x_tree_s& T, uint8_t d, DepthMap& M) {
+  const uint32_t n = T.n;
+  M.d = d;
+  M.box = dalloc<uint32_t>(n);
+  M.in = dalloc<uint8_t>(n);
+  M.start = dalloc<uint8_t>(n);
+  hipLaunchKernelGGL(depthMapKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.view(T.cur), n, d, M.box,
+                     M.start, M.in);
+  HIPCHK(hipGetLastError());
+  M.count = scanArray(T.stream, M.box, n, T.totals, T.scal + kScalDepthCount, T.hscal + kScalDepthCount);
+  hipLaunchKernelGGL(depthBoxKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, M.box,
+                     (const uint8_t*)M.start, n);
+  HIPCHK(hipGetLastError());
+}
+
+// Flag operation op (0 set, 1 unset, 2 change) on the leaves of the boxes at depth M.d selected
+// by the device stencil sel (M.count bytes); returns countIfFlags (ImplicitBoxTree.hpp:2803-2881,
+// 3334-3411, 912-1058, 1097-1214)
+uint32_t depthFlagsOp(b12x_tree_s& T, const DepthMap& M, const uint8_t* sel, int op, uint8_t a, uint8_t b) {
+  const uint32_t n = T.n;
+  uint32_t* acc = dalloc<uint32_t>(std::max<uint32_t>(M.count, 1));
+  hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(std::max<uint32_t>(M.count, 1), kBlock)), dim3(kBlock), 0, T.stream,
+                     acc, uint64_t(std::max<uint32_t>(M.count, 1)), op == 1 ? 0u : 0xFFu);
+  hipLaunchKernelGGL(depthCombineKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream,
+                     (const uint8_t*)T.flags[T.cur], n, (const uint32_t*)M.box, (const uint8_t*)M.in, op == 1 ? 1 : 0,
+                     acc);
+  hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, T.stream, T.scal + kScalDepthFlags, uint64_t(1), 0u);
+  hipLaunchKernelGGL(depthFlagsKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur], n,
+                     (const uint32_t*)M.box, (const uint8_t*)M.in, (const uint8_t*)M.start, sel, (const uint32_t*)acc,
+                     op, a, b, T.scal + kScalDepthFlags);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(T.hscal + kScalDepthFlags, T.scal + kScalDepthFlags, 4, hipMemcpyDeviceToHost, T.stream));
+  HIPCHK(hipStreamSynchronize(T.stream));
+  dfree(acc);
+  return T.hscal[kScalDepthFlags];
 }
 
 void insertRootIfEmpty(b12x_tree_s& T) {  // ImplicitBoxTree.hpp:1959-1970
@@ -261,7 +329,41 @@ Source makeSource(b12x_tree_s& T, uint8_t dIt, uint8_t check) {
     if (check != 0) nb = areFlagsSet(leafStats(T).flagsAnd, check) ? 1 : 0;
     return {T.rootKey, T.rootDepth, nullptr, nb};
   }
-  if (classifyDepth(T, dIt) != DC_LEAF) throw HipFail(B12X_E_UNSUPPORTED, "point iterator depth");
+  const DepthClass dc = classifyDepth(T, dIt);
+  if (dc == DC_BEYOND) return {T.key[T.cur], T.depth[T.cur], nullptr, 0};
+  if (dc == DC_GENERAL) {  // :1528-1601 the first leaf of every box at dIt, truncated to dIt
+    DepthMap M;
+    buildDepthMap(T, dIt, M);
+    const uint32_t n = T.n;
+    uint32_t* ok = nullptr;
+    if (check != 0) {
+      ok = dalloc<uint32_t>(std::max<uint32_t>(M.count, 1));
+      hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(std::max<uint32_t>(M.count, 1), kBlock)), dim3(kBlock), 0,
+                         T.stream, ok, uint64_t(std::max<uint32_t>(M.count, 1)), 1u);
+      hipLaunchKernelGGL(depthCheckKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream,
+                         (const uint8_t*)T.flags[T.cur], n, (const uint32_t*)M.box, (const uint8_t*)M.in, check, ok);
+    }
+    uint32_t* pos = dalloc<uint32_t>(n);
+    hipLaunchKernelGGL(depthSourceCountKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream,
+                       (const uint8_t*)M.start, (const uint32_t*)M.box, (const uint32_t*)ok, n, pos);
+    HIPCHK(hipGetLastError());
+    const uint32_t m = scanArray(T.stream, pos, n, T.totals, T.scal + kScalDepthFlags, T.hscal + kScalDepthFlags);
+    if (m > T.srcCap) {
+      dfree(T.srcKey);
+      dfree(T.srcDepth);
+      T.srcKey = dalloc<uint64_t>(m);
+      T.srcDepth = dalloc<uint8_t>(m);
+      T.srcCap = m;
+    }
+    hipLaunchKernelGGL(depthSourceWriteKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.view(T.cur),
+                       n, dIt, (const uint8_t*)M.start, (const uint32_t*)M.box, (const uint32_t*)ok,
+                       (const uint32_t*)pos, T.srcKey, T.srcDepth);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(ok);
+    dfree(pos);
+    return {T.srcKey, T.srcDepth, nullptr, m};
+  }
   if (check == 0) return {T.key[T.cur], T.depth[T.cur], nullptr, T.n};
   IndexOp op{};
   op.flags = T.flags[T.cur];
@@ -322,12 +424,13 @@ void launchVariant(const ImageArgs& a, int gridKind, ImageVariant v, hipStream_t
 // Every variant computes the same images and a repeated launch stores the same hit bytes or
 // targets, so tuning never changes a result.
 // B12X_IMAGE_TUNE=0: one point per lane, 256 threads; B12X_IMAGE_PPL / B12X_IMAGE_BLOCK fix a
-// variant. The staged descent always takes one point per lane.
+// variant. The staged descent and the search among boxes at a general depth always take one
+// point per lane.
 std::mutex g_tuneMutex;
 std::unordered_map<uint64_t, int> g_tuned;
 
 int fixedImageVariant(const ImageArgs& a) {
-  if (a.samples) return 0;
+  if (a.samples || a.depthBox) return 0;
   const char* t = std::getenv("B12X_IMAGE_TUNE");
   const char* p = std::getenv("B12X_IMAGE_PPL");
   const char* b = std::getenv("B12X_IMAGE_BLOCK");
@@ -394,8 +497,14 @@ void launchImage(int dim, const ImageArgs& a, int gridKind, hipStream_t st) {
 
 // Image of every point of `src` located on the leaves (or on the root when depth0).
 // mode 0 sets hit bytes, mode 1 writes targets to out.
-void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, int mode, uint32_t* out) {
+void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, int mode, uint32_t* out,
+              const DepthMap* dm = nullptr) {
   ImageArgs a{};
+  if (dm) {  // search among the boxes at depth dm->d
+    a.depthBox = dm->box;
+    a.depthIn = dm->in;
+    a.searchDepth = dm->d;
+  }
   a.dom = T.dom;
   a.map = S.map;
   a.grid = S.grid;
@@ -412,7 +521,7 @@ void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, 
   a.hit = T.hit;
   a.out = out;
   if (a.nPoints == 0) return;
-  if (!depth0 && T.n > 0) a.samples = stageSamples(T, nullptr);
+  if (!depth0 && !dm && T.n > 0) a.samples = stageSamples(T, nullptr);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (T.timing) {
     for (hipEvent_t* e : {&e0, &e1}) {
@@ -468,6 +577,18 @@ bool rootPred(int op, const LeafStats& s, uint8_t a) {  // countIfFlags at depth
 
 uint32_t stencilOp(b12x_tree_s& T, int op, const uint8_t* hostStencil, uint8_t a, uint8_t b, uint8_t d) {
   const DepthClass dc = classifyDepth(T, d);
+  if (dc == DC_BEYOND) return 0;
+  if (dc == DC_GENERAL) {  // the stencil has one entry per box at depth d
+    DepthMap M;
+    buildDepthMap(T, d, M);
+    const uint32_t m = std::max<uint32_t>(M.count, 1);
+    uint8_t* sel = dalloc<uint8_t>(m);
+    if (hostStencil && M.count) HIPCHK(hipMemcpyAsync(sel, hostStencil, M.count, hipMemcpyHostToDevice, T.stream));
+    else hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(m, kBlock)), dim3(kBlock), 0, T.stream, sel, uint64_t(m), uint8_t(1));
+    const uint32_t r = depthFlagsOp(T, M, sel, op, a, b);
+    dfree(sel);
+    return r;
+  }
   if (dc == DC_ROOT) {
     const bool on = hostStencil ? hostStencil[0] != 0 : true;
     const LeafStats s = leafStats(T);
@@ -585,13 +706,14 @@ int b12x_count(b12x_tree t, uint8_t d, uint32_t* out) {
   return guarded([&]() -> int {
     if (!t || !out) return setErr(B12X_E_ARG, "null argument");
     // ImplicitBoxTree.hpp:1060-1095 for the root, the leaves and a depth shared by all leaves
-    if (d == 0) *out = 1;
-    else if (d == kLeafDepth) *out = t->n;
+    const DepthClass dc = classifyDepth(*t, d);
+    if (dc == DC_ROOT) *out = 1;
+    else if (dc == DC_LEAF) *out = t->n;
+    else if (dc == DC_BEYOND) *out = 0;
     else {
-      LeafStats s = leafStats(*t);
-      if (d != kUnsubDepth && s.minDepth == d && s.maxDepth == d) *out = t->n;
-      else if (d != kUnsubDepth && d > s.maxDepth) *out = 0;
-      else return setErr(B12X_E_UNSUPPORTED, "count at a depth above the leaves is not implemented");
+      DepthMap M;
+      buildDepthMap(*t, d, M);
+      *out = M.count;
     }
     return B12X_OK;
   });
@@ -795,6 +917,20 @@ int b12x_flags_by_image(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, 
     checkScheme(T, s);
     const DepthClass dc = classifyDepth(T, d);
     const Source src = makeSource(T, dIt, check);
+    if (dc == DC_BEYOND) {  // no box to hit (:2797)
+      *ret = 0;
+      return B12X_OK;
+    }
+    if (dc == DC_GENERAL) {  // :2780-2795 hits become a stencil over the boxes at d
+      DepthMap M;
+      buildDepthMap(T, d, M);
+      runImage(T, *s, src, false, 0, nullptr, &M);
+      *ret = depthFlagsOp(T, M, T.hit, op, f, f2);
+      hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(std::max<uint32_t>(M.count, 1), kBlock)), dim3(kBlock), 0,
+                         T.stream, T.hit, uint64_t(M.count), uint8_t(0));
+      HIPCHK(hipStreamSynchronize(T.stream));
+      return B12X_OK;
+    }
     runImage(T, *s, src, dc == DC_ROOT, 0, nullptr);
     if (dc == DC_ROOT) {  // ImplicitBoxTree.hpp:2722-2748 (surrounding box hit by any valid point)
       uint8_t h0 = 0;
@@ -825,7 +961,12 @@ int b12x_search_by_image(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check,
     const uint64_t P = src.nBoxes * s->grid.k;
     if (P == 0) return B12X_OK;
     uint32_t* dout = dalloc<uint32_t>(P);
-    runImage(T, *s, src, dc == DC_ROOT, 1, dout);
+    DepthMap M;
+    if (dc == DC_GENERAL) buildDepthMap(T, d, M);
+    if (dc == DC_BEYOND)
+      hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, dout, P, kInvalid);
+    else
+      runImage(T, *s, src, dc == DC_ROOT, 1, dout, dc == DC_GENERAL ? &M : nullptr);
     HIPCHK(hipMemcpyAsync(out, dout, P * 4, hipMemcpyDeviceToHost, T.stream));
     HIPCHK(hipStreamSynchronize(T.stream));
     dfree(dout);
@@ -841,8 +982,18 @@ int b12x_search_compacted(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check
     checkScheme(T, s);
     const DepthClass dc = classifyDepth(T, d);
     const Source src = makeSource(T, dIt, check);
-    runImage(T, *s, src, dc == DC_ROOT, 0, nullptr);
-    const uint32_t m = dc == DC_ROOT ? 1 : T.n;
+    if (dc == DC_BEYOND) {
+      *nOut = 0;
+      return B12X_OK;
+    }
+    DepthMap M;
+    if (dc == DC_GENERAL) buildDepthMap(T, d, M);
+    runImage(T, *s, src, dc == DC_ROOT, 0, nullptr, dc == DC_GENERAL ? &M : nullptr);
+    const uint32_t m = dc == DC_ROOT ? 1 : (dc == DC_GENERAL ? M.count : T.n);
+    if (m == 0) {  // no box at d: nothing can be hit
+      *nOut = 0;
+      return B12X_OK;
+    }
     IndexOp op{};
     op.flags = nullptr;
     op.hit = T.hit;
@@ -871,11 +1022,19 @@ int b12x_search_by_points(b12x_tree t, const double* pts, uint64_t n, uint8_t d,
     HIPCHK(hipMemcpyAsync(dp, pts, n * T.dim * 8, hipMemcpyHostToDevice, T.stream));
     const uint32_t nb = ceilDiv(n, kBlock);
     const int d0 = dc == DC_ROOT;
-    switch (T.dim) {
-      case 1: hipLaunchKernelGGL(pointsSearchKernel<1>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
-      case 2: hipLaunchKernelGGL(pointsSearchKernel<2>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
-      case 3: hipLaunchKernelGGL(pointsSearchKernel<3>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
-      default: hipLaunchKernelGGL(pointsSearchKernel<4>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout); break;
+    DepthMap M;
+    if (dc == DC_GENERAL) buildDepthMap(T, d, M);
+    const uint32_t* mb = dc == DC_GENERAL ? M.box : nullptr;
+    const uint8_t* mi = M.in;
+    if (dc == DC_BEYOND) {
+      hipLaunchKernelGGL(fillU32Kernel, dim3(nb), dim3(kBlock), 0, T.stream, dout, n, kInvalid);
+    } else {
+      switch (T.dim) {
+        case 1: hipLaunchKernelGGL(pointsSearchKernel<1>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout, mb, mi, d); break;
+        case 2: hipLaunchKernelGGL(pointsSearchKernel<2>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout, mb, mi, d); break;
+        case 3: hipLaunchKernelGGL(pointsSearchKernel<3>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout, mb, mi, d); break;
+        default: hipLaunchKernelGGL(pointsSearchKernel<4>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, n, T.key[T.cur], T.depth[T.cur], T.n, d0, dout, mb, mi, d); break;
+      }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, T.stream));
@@ -1070,14 +1229,19 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
     const uint32_t k = s->grid.k;
     const uint64_t P = src.nBoxes * k;
     if (P > 0xFFFFFFFFull) return setErr(B12X_E_UNSUPPORTED, "more than 2^32 test points in one matrix");
-    const uint32_t nBoxes = dc == DC_ROOT ? 1u : T.n;  // count(depth)
+    DepthMap DM;
+    if (dc == DC_GENERAL) buildDepthMap(T, d, DM);
+    const uint32_t nBoxes =  // count(depth)
+        dc == DC_ROOT ? 1u : (dc == DC_GENERAL ? DM.count : (dc == DC_BEYOND ? 0u : T.n));
     auto* M = new b12x_coo_s();
     try {
       HIPCHK(hipStreamCreate(&M->stream));
       M->nRows = M->nCols = nBoxes;
       M->rowMajor = rowMajor != 0;
       uint32_t* col = dalloc<uint32_t>(P);
-      runImage(T, *s, src, dc == DC_ROOT, 1, col);
+      if (dc != DC_BEYOND) runImage(T, *s, src, dc == DC_ROOT, 1, col, dc == DC_GENERAL ? &DM : nullptr);
+      else if (P)
+        hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, col, P, kInvalid);
       const uint64_t nRowsSrc = src.nBoxes;
       uint32_t nnz = 0;
       uint32_t* rows = nullptr;
@@ -1269,11 +1433,6 @@ __global__ void gatherF64Kernel(const double* src, const uint32_t* idx, uint64_t
 
 __global__ void fillF64Kernel(double* p, uint32_t n, double v) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-__global__ void fillU32Kernel(uint32_t* p, uint64_t n, uint32_t v) {
-  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
 }
 

@@ -505,6 +505,60 @@ __device__ inline uint32_t locate(const uint64_t* __restrict__ keys, const uint8
   return kInvalid;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Boxes at a depth d between the root and the leaves, or at the unsubdividable depth
+// (ImplicitBoxTree.hpp:1383-1473, 1990-2021, Box.hpp:159-174, 200-204): a leaf truncated to d
+// (decrementDepthTo; at the unsubdividable depth its parent) and the search of a point among the
+// boxes at d (ImplicitBoxTree.hpp:2433-2566: ComputeSearchIndexFunctor on the two neighbours of
+// the lower bound, truncated to d and admitted by their `in` flag, then the leaf's box index).
+// ---------------------------------------------------------------------------------------------
+__device__ inline void truncateTo(uint64_t key, int depth, uint8_t d, uint64_t& bits, int& td) {
+  bits = bitsOf(key, depth);
+  td = depth;
+  if (d < depth) {
+    bits >>= (depth - d);
+    td = d;
+  } else if (d == kUnsubDepth) {
+    bits >>= 1;
+    td = depth > 0 ? depth - 1 : 0;
+  }
+}
+
+// hasSamePath(point, box) for a point key (an invalid point matches an all-zero pattern only)
+__device__ inline bool pointInBox(uint64_t pk, uint64_t bits, int td) {
+  if (pk == kInvalidKey) return bits == 0ull;
+  return td == 0 || ((pk ^ keyOf(bits, td)) >> (kMaxDepth - td)) == 0ull;
+}
+
+__device__ inline uint32_t locateInDepth(const uint64_t* __restrict__ keys, const uint8_t* __restrict__ depth,
+                                         uint32_t n, uint64_t pk, uint8_t d, const uint8_t* __restrict__ in,
+                                         const uint32_t* __restrict__ box) {
+  uint32_t lo = 0, len = n;
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (keys[lo + half] < pk) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  const uint32_t s = lo;
+  const uint32_t i1 = s > 0 ? s - 1 : 0;
+  const uint32_t i2 = s < n - 1 ? s : n - 1;
+  uint64_t b;
+  int td;
+  if (in[i1]) {
+    truncateTo(keys[i1], depth[i1], d, b, td);
+    if (pointInBox(pk, b, td)) return box[i1];
+  }
+  if (in[i2]) {
+    truncateTo(keys[i2], depth[i2], d, b, td);
+    if (pointInBox(pk, b, td)) return box[s < n - 1 ? s : n - 1];
+  }
+  return kInvalid;
+}
+
 // locate() for P keys at once: a lower bound whose trip count depends on n only (so the P
 // searches run in lockstep, each probe of every key in flight together), then locate()'s leaf
 // checks and its invalid-point case. n >= 1.

@@ -375,6 +375,10 @@ struct ImageArgs {
   uint8_t* hit;
   uint32_t* out;
   const uint64_t* samples;  // descent samples of key[0..n) (sampleKeysKernel), or null
+  // search among the boxes at a depth between the root and the leaves (null: leaf search)
+  const uint32_t* depthBox;  // box index of every leaf
+  const uint8_t* depthIn;    // leaf belongs to a box at that depth
+  uint8_t searchDepth;
 };
 
 // samples[j] = keys[pos(j)], j < min(kDescentSamples, n); n from nDev when given
@@ -404,6 +408,7 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
   const uint64_t pk = pointKey<DIM>(a.dom, y);
   uint32_t t;
   if (a.depth0) t = (pk == kInvalidKey) ? kInvalid : 0u;
+  else if (a.depthBox) t = locateInDepth(a.key, a.depth, n, pk, a.searchDepth, a.depthIn, a.depthBox);
   else if (S) t = locateStaged(a.key, a.depth, n, pk, sm, S);
   else t = locate(a.key, a.depth, n, pk);
   if (a.mode == 0) {
@@ -516,14 +521,104 @@ static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
 template <int DIM>
 static __global__ __launch_bounds__(kBlock) void pointsSearchKernel(Domain dom, const double* pts, uint64_t m,
                                                              const uint64_t* key, const uint8_t* depth, uint32_t n,
-                                                             int depth0, uint32_t* out) {
+                                                             int depth0, uint32_t* out, const uint32_t* depthBox,
+                                                             const uint8_t* depthIn, uint8_t d) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= m) return;
   double p[DIM];
 #pragma unroll
   for (int j = 0; j < DIM; ++j) p[j] = pts[i * DIM + j];
   const uint64_t pk = pointKey<DIM>(dom, p);
-  out[i] = depth0 ? ((pk == kInvalidKey) ? kInvalid : 0u) : locate(key, depth, n, pk);
+  if (depth0) out[i] = (pk == kInvalidKey) ? kInvalid : 0u;
+  else if (depthBox) out[i] = locateInDepth(key, depth, n, pk, d, depthIn, depthBox);
+  else out[i] = locate(key, depth, n, pk);
+}
+
+// Boxes at depth d: in[i] (leaf i belongs to one: depth >= d, or unsubdividable at the
+// unsubdividable depth), startFlag[i] and start[i] (leaf i is the first leaf of its box; start
+// is scanned into the box indices by depthBoxKernel)
+static __global__ __launch_bounds__(kBlock) void depthMapKernel(TreeView v, uint32_t n, uint8_t d, uint32_t* start,
+                                                               uint8_t* startFlag, uint8_t* in) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint64_t b, bp;
+  int td, tdp;
+  truncateTo(v.key[i], v.depth[i], d, b, td);
+  const uint32_t ip = i > 0 ? i - 1 : 0;
+  truncateTo(v.key[ip], v.depth[ip], d, bp, tdp);
+  const bool deep = d == kUnsubDepth || td >= int(d);  // hasAtLeastDepth of the truncated box
+  const bool fresh = deep && (i == 0 || !(b == bp && td == tdp));
+  const bool un = d == kUnsubDepth ? isUnsubdividableAt(v, n, i, 0) : true;
+  const bool inBox = d == kUnsubDepth ? un : int(v.depth[i]) >= int(d);
+  const bool st = fresh && un;
+  in[i] = inBox ? 1 : 0;
+  startFlag[i] = st ? 1 : 0;
+  start[i] = st ? 1u : 0u;
+}
+
+// exclusive scan of the starts -> box index (inclusive count minus one, 0 before the first box)
+static __global__ __launch_bounds__(kBlock) void depthBoxKernel(uint32_t* box, const uint8_t* startFlag, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inc = box[i] + startFlag[i];
+  box[i] = inc > 0 ? inc - 1 : 0;
+}
+
+// Flags combined over the leaves of every box (AND for set and change, OR for unset)
+static __global__ __launch_bounds__(kBlock) void depthCombineKernel(const uint8_t* flags, uint32_t n,
+                                                                   const uint32_t* box, const uint8_t* in, int useOr,
+                                                                   uint32_t* acc) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || !in[i]) return;
+  if (useOr) atomicOr(acc + box[i], uint32_t(flags[i]));
+  else atomicAnd(acc + box[i], uint32_t(flags[i]));
+}
+
+// Flag operation on every leaf of a selected box at depth d; counts (into *count) the selected
+// boxes whose combined flags pass the operation's predicate (ImplicitBoxTree.hpp:1161-1209)
+static __global__ __launch_bounds__(kBlock) void depthFlagsKernel(uint8_t* flags, uint32_t n, const uint32_t* box,
+                                                                 const uint8_t* in, const uint8_t* startFlag,
+                                                                 const uint8_t* sel, const uint32_t* acc, int op,
+                                                                 uint8_t a, uint8_t b2, uint32_t* count) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t c = 0;
+  if (i < n) {
+    const uint32_t bx = box[i];
+    if (startFlag[i] && sel[bx]) {
+      c = flagPred(op, uint8_t(acc[bx]), a) ? 1u : 0u;
+    }
+    if (in[i] && sel[bx]) flags[i] = flagApply(op, flags[i], a, b2);
+  }
+  c = blockReduceSum(c);
+  if (threadIdx.x == 0 && c) atomicAdd(count, c);
+}
+
+// Sources at depth d: the first leaf of every box whose leaves all carry `check`, truncated to d
+static __global__ __launch_bounds__(kBlock) void depthCheckKernel(const uint8_t* flags, uint32_t n, const uint32_t* box,
+                                                                 const uint8_t* in, uint8_t check, uint32_t* ok) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || !in[i]) return;
+  if ((flags[i] & check) != check) atomicAnd(ok + box[i], 0u);
+}
+
+static __global__ __launch_bounds__(kBlock) void depthSourceCountKernel(const uint8_t* startFlag, const uint32_t* box,
+                                                                       const uint32_t* ok, uint32_t n, uint32_t* pick) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  pick[i] = (startFlag[i] && (!ok || ok[box[i]])) ? 1u : 0u;
+}
+
+static __global__ __launch_bounds__(kBlock) void depthSourceWriteKernel(TreeView v, uint32_t n, uint8_t d,
+                                                                       const uint8_t* startFlag, const uint32_t* box,
+                                                                       const uint32_t* ok, const uint32_t* pos,
+                                                                       uint64_t* srcKey, uint8_t* srcDepth) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || !(startFlag[i] && (!ok || ok[box[i]]))) return;
+  uint64_t b;
+  int td;
+  truncateTo(v.key[i], v.depth[i], d, b, td);
+  srcKey[pos[i]] = keyOf(b, td);
+  srcDepth[pos[i]] = uint8_t(td);
 }
 
 // ---------------------------------------------------------------------------------------------
