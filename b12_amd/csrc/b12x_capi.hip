@@ -1536,7 +1536,7 @@ uint32_t coopGrid(uint32_t n) {
   int dev = 0, cus = 0, occ = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&piCoopKernel), kBlock, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&piCoopKernel<kCoopMaxBlocks / kWave>), kBlock, 0) !=
           hipSuccess ||
       occ <= 0)
     return 0;
@@ -1608,14 +1608,19 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     uint32_t coopBlocks = 0;
     if (!fused && !init.done && !(pp && pp[0] == '0')) coopBlocks = coopGrid(n);
     if (coopBlocks) {
-      const uint64_t nPart = uint64_t(coopBlocks) * 6;
+      const uint64_t nPart = uint64_t(coopBlocks) * 6 + kPiResWords;  // partial slots, then the result slots
       uint64_t* part = dalloc<uint64_t>(nPart);
       int* err = dalloc<int>(1);
       hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nPart, kBlock)), dim3(kBlock), 0, st, part, nPart, kCoopEmpty);
+      if (n)  // every y slot starts empty
+        hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(uint64_t(n) * 3, kBlock)), dim3(kBlock), 0, st,
+                           reinterpret_cast<uint64_t*>(ybuf), uint64_t(n) * 3, kCoopEmpty);
       hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)err, uint64_t(1), 0u);
       HIPCHK(hipGetLastError());
-      PiCoopArgs ca{colPtr, cr, vals, mu0, ybuf, part, pst, err, n, uint32_t(std::max(maxIter, 0)), tol, 1u << 21};
-      hipLaunchKernelGGL(piCoopKernel, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
+      PiCoopArgs ca{colPtr, cr, vals, mu0, ybuf, part, part + uint64_t(coopBlocks) * 6, pst, err, n,
+                    uint32_t(std::max(maxIter, 0)), tol, 1u << 21};
+      if (coopBlocks <= 4 * kWave) hipLaunchKernelGGL(piCoopKernel<4>, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
+      else hipLaunchKernelGGL(piCoopKernel<kCoopMaxBlocks / kWave>, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
       HIPCHK(hipGetLastError());
       int herr = 0;
       HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -1107,29 +1107,44 @@ static __global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t*
 // resident for the whole run on an otherwise idle device).
 // Per iteration each block computes y_t for its columns (one lane per column, products added in
 // ascending-row order as the oracle does) and publishes its partial sums of |y_t| and of the
-// previous residual in its slot of buffer t % 3. A slot holds kCoopEmpty until written; every
-// block polls all slots of the buffer until none is empty, then sums them in block order, so all
-// blocks derive the same norm and the same stopping decision. A block clears its slot of buffer
-// (t+1) % 3 at the start of iteration t: that buffer was last read in iteration t-2, which every
-// block has finished once the wait of iteration t-1 is over. y rotates over 3 slots likewise.
+// previous residual in its slot of buffer t % 3. A slot holds kCoopEmpty until written. Block 0
+// polls every slot of the buffer until none is empty, sums them in block order and publishes the
+// totals in result slot t % 3 (kPiResCopies copies); the other blocks poll their copy. So all
+// blocks derive the same norm and the same stopping decision. A block clears its partial slot of
+// buffer (t+1) % 3 at the start of iteration t, and block 0 the result slot (t+1) % 3: both were
+// last read in iteration t-2, which every block has finished once the wait of iteration t-1 is
+// over.
+// y rotates over 3 slots (y_t in slot t % 3) and a lane clears its columns of slot (t+1) % 3
+// (y_{t-2}, read for the last time in iteration t-1) to kCoopEmpty in iteration t, before its
+// block publishes. So once a block has seen the totals of iteration t-1, slot t % 3 holds
+// kCoopEmpty or y_t, never an older iterate. That lets each lane gather the y_t entries of its
+// first column right after its block publishes y_t, while the totals are still on their way:
+// the gathers overlap the wait, and a lane re-reads the entries that came back empty (their
+// owners had not stored them yet) until none is left. Everything else is gathered after the
+// wait, when every y_t entry is in place.
 // Every wait is bounded: past spinLimit polls a block raises *err and leaves, and the others
 // leave at their next wait (the host then recomputes with the batched form).
-// Coherence across XCDs: everything written during the run (y, partials) is written and read
-// with agent-scope relaxed atomics (coherent without L2 write-back or invalidation), and each wave
-// waits for its stores to complete before the block publishes. The matrix is read-only and stays
-// cached in L2 across iterations.
+// Coherence across XCDs: everything written during the run (y, partials, totals) is written and
+// read with agent-scope relaxed atomics (coherent without L2 write-back or invalidation), and
+// each wave waits for its stores to complete before its block publishes. The matrix is read-only
+// and stays cached in L2 across iterations.
 constexpr uint32_t kCoopMaxBlocks = 512;
 constexpr int kCoopUnroll = 8;
 constexpr int kCoopCache = 3 * kCoopUnroll;  // entries of a lane's first column held in registers
 constexpr uint64_t kCoopEmpty = 0xFFF4B12B12B12B12ull;  // a NaN pattern arithmetic never produces
+// block 0 writes its totals to kPiResCopies copies, one 128-byte line each, and block b polls
+// copy b % kPiResCopies, so that no single line takes every block's polls
+constexpr uint32_t kPiResCopies = 16, kPiResStride = 16;
+constexpr uint32_t kPiResWords = 3 * kPiResCopies * kPiResStride;
 
 struct PiCoopArgs {
   const uint32_t* colPtr;
   const uint32_t* rowIdx;
   const double* val;
   const double* mu0;
-  double* y;          // 3 * n
+  double* y;          // 3 * n, all kCoopEmpty at launch
   uint64_t* part;     // 3 buffers * gridDim.x slots * 2, all kCoopEmpty at launch
+  uint64_t* res;      // 3 slots * kPiResCopies copies * kPiResStride (block 0's totals), all kCoopEmpty
   PiState* st;
   int* err;           // zero at launch
   uint32_t n;
@@ -1146,18 +1161,62 @@ template <class T>
 __device__ inline void stAgent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// y entries as bit patterns (the empty pattern is compared bitwise)
+__device__ inline uint64_t ldBits(const double* p) { return ldAgent(reinterpret_cast<const uint64_t*>(p)); }
+__device__ inline void stBits(double* p, uint64_t v) { stAgent(reinterpret_cast<uint64_t*>(p), v); }
 
-static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
-  constexpr uint32_t kSlots = kCoopMaxBlocks / kWave;  // slots per polling lane
+// y entry at a byte offset below 2^32 from a uniform base (one 32-bit offset register per gather)
+__device__ inline uint64_t ldAt(const double* base, uint32_t off) {
+  return ldAgent(reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(base) + off));
+}
+
+// the cached entries of a lane's first column (byte offsets ci), gathered from src: groups of
+// kCoopUnroll gathers, each issued where some column of the wave reaches it (clamped indices
+// keep the others in bounds), all in flight together
+__device__ inline void piGatherCached(const double* src, const uint32_t (&ci)[kCoopCache], uint32_t c0, uint32_t c1,
+                                      uint64_t (&m)[kCoopCache]) {
+#pragma unroll
+  for (int g = 0; g < kCoopCache; g += kCoopUnroll) {
+    if (__any(c0 + g < c1)) {  // wave-uniform: no wait at the join
+#pragma unroll
+      for (int u = g; u < g + kCoopUnroll; ++u) m[u] = ldAt(src, ci[u]);
+    }
+  }
+}
+
+// true where a cached entry the column uses came back empty
+__device__ inline bool piMissing(const uint64_t (&m)[kCoopCache], uint32_t c0, uint32_t c1) {
+  bool miss = false;
+#pragma unroll
+  for (int u = 0; u < kCoopCache; ++u) miss = miss || (c0 + u < c1 && m[u] == kCoopEmpty);
+  return miss;
+}
+
+// re-read the empty entries (all of a lane's re-reads in flight together)
+__device__ inline void piRegather(const double* src, const uint32_t (&ci)[kCoopCache], uint32_t c0, uint32_t c1,
+                                  uint64_t (&m)[kCoopCache]) {
+#pragma unroll
+  for (int u = 0; u < kCoopCache; ++u)
+    if (c0 + u < c1 && m[u] == kCoopEmpty) m[u] = ldAt(src, ci[u]);
+}
+
+// SLOTS: partial slots per polling lane of block 0 (grids up to SLOTS * 64 blocks)
+template <uint32_t SLOTS>
+static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void piCoopKernel(PiCoopArgs a) {
+  constexpr uint32_t kSlots = SLOTS;
+  constexpr int kWaves = kBlock / kWave;
   const uint32_t G = gridDim.x, b = blockIdx.x, tid = threadIdx.x, n = a.n;
+  const uint32_t lane = tid & (kWave - 1), wv = tid / kWave;
+  __shared__ double wsum[2][kWaves];
   __shared__ double sums[2];
-  __shared__ int leave;
+  __shared__ int leave, failed;
   if (a.st->done) return;
   uint32_t t = a.st->t;
   double nPrev = t >= 2 ? a.st->norm[(t + 2) % 3] : 1.0;  // norm of y_{t-1}
   double nPrev2 = t >= 3 ? a.st->norm[(t + 1) % 3] : 1.0;  // norm of y_{t-2}
-  // the lane's first column keeps its first kCoopCache entries in registers for the whole run
-  // (indices clamped into the column; an empty column holds row 0 and adds nothing)
+  // the lane's first column keeps its first kCoopCache entries (row byte offsets and values) in
+  // registers for the whole run (indices clamped into the column; an empty column holds row 0
+  // and adds nothing)
   const uint32_t p0 = b * kBlock + tid;
   uint32_t c0 = 0, c1 = 0;
   uint32_t ci[kCoopCache];
@@ -1167,6 +1226,8 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
     ci[u] = 0;
     cv[u] = 0.0;
   }
+  // the first column's own y_{t-1} and y_{t-2} (mu_0 stands for y_0, with norm 1)
+  double own1 = 0.0, own2 = 0.0;
   if (p0 < n) {
     c0 = a.colPtr[p0];
     c1 = a.colPtr[p0 + 1];
@@ -1174,61 +1235,65 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
 #pragma unroll
       for (int u = 0; u < kCoopCache; ++u) {
         const uint32_t eu = min(c0 + u, c1 - 1);
-        ci[u] = a.rowIdx[eu];
+        ci[u] = a.rowIdx[eu] * uint32_t(sizeof(double));  // n < 2^29
         cv[u] = a.val[eu];
       }
     }
+    own1 = t >= 2 ? __longlong_as_double((long long)ldBits(a.y + uint64_t((t + 2) % 3) * n + p0)) : a.mu0[p0];
+    own2 = t >= 3 ? __longlong_as_double((long long)ldBits(a.y + uint64_t((t + 1) % 3) * n + p0))
+                  : (t == 2 ? a.mu0[p0] : 0.0);
   }
+  // the first gathers: y_{t-1} (mu_0 in iteration 1) is complete at launch
+  uint64_t m[kCoopCache];
+#pragma unroll
+  for (int u = 0; u < kCoopCache; ++u) m[u] = 0;
+  piGatherCached((t == 1) ? a.mu0 : a.y + uint64_t((t + 2) % 3) * n, ci, c0, c1, m);
   for (;;) {
     if (tid == 0) {
       uint64_t* nextSlot = a.part + (uint64_t((t + 1) % 3) * G + b) * 2;
       stAgent(nextSlot, kCoopEmpty);
       stAgent(nextSlot + 1, kCoopEmpty);
+      failed = 0;
+    }
+    if (b == 0 && tid < kPiResCopies) {
+      uint64_t* rn = a.res + (((t + 1) % 3) * kPiResCopies + tid) * kPiResStride;
+      stAgent(rn, kCoopEmpty);
+      stAgent(rn + 1, kCoopEmpty);
     }
     const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
     const double* yPrev = a.y + uint64_t(s1) * n;
-    const double* yPrev2 = a.y + uint64_t(s2) * n;
+    double* yOld = a.y + uint64_t(s2) * n;  // y_{t-2}: cleared below
     double* yCur = a.y + uint64_t(s0) * n;
     // iteration 1 multiplies by mu_0 itself: mu_0[i] / 1.0 is mu_0[i] exactly
     const double* src = (t == 1) ? a.mu0 : yPrev;
     const double div = (t == 1) ? 1.0 : nPrev;
+    const double div2 = (t == 2) ? 1.0 : nPrev2;
     double sAbs = 0.0, sRes = 0.0;
     for (uint32_t p = p0; p < n; p += G * kBlock) {
-      // residual inputs first, unconditionally (y is always allocated; iteration 1 ignores them),
-      // so that their loads overlap the gathers
-      const double yp = ldAgent(yPrev + p);
-      const double yp2 = ldAgent(((t == 2) ? a.mu0 : yPrev2) + p);
-      double s = 0.0;
+      double s = 0.0, yp, yp2;
       uint32_t e, e1;
       if (p == p0) {
-        // cached entries: groups of kCoopUnroll gathers, each issued where some column of the
-        // wave reaches it (clamped indices keep the others in bounds), all in flight together
-        double m[kCoopCache];
-#pragma unroll
-        for (int u = 0; u < kCoopCache; ++u) m[u] = 0.0;
-#pragma unroll
-        for (int g = 0; g < kCoopCache; g += kCoopUnroll) {
-          if (__any(c0 + g < c1)) {  // wave-uniform: no wait at the join
-#pragma unroll
-            for (int u = g; u < g + kCoopUnroll; ++u) m[u] = ldAgent(src + ci[u]);
-          }
-        }
+        // mu_{t-1}[i] = y_{t-1}[i] / ||y_{t-1}||_1 as the oracle rounds it, then the product
 #pragma unroll
         for (int u = 0; u < kCoopCache; ++u) {
-          const double prod = (m[u] / div) * cv[u];
+          const double prod = (__longlong_as_double((long long)m[u]) / div) * cv[u];
           s = (c0 + u < c1) ? s + prod : s;
         }
         e = c0 + kCoopCache;
         e1 = c1;
+        yp = own1;
+        yp2 = own2;
       } else {
+        yp = __longlong_as_double((long long)ldBits(yPrev + p));
+        yp2 = __longlong_as_double((long long)ldBits(((t == 2) ? a.mu0 : yOld) + p));
         e = a.colPtr[p];
         e1 = a.colPtr[p + 1];
       }
       // branch-free body (entry indices clamped to the column, sums selected) so that all of a
-      // lane's gathers are in flight at once
+      // lane's gathers are in flight at once; y_{t-1} is complete here
       for (; e < e1; e += kCoopUnroll) {
         uint32_t i[kCoopUnroll];
-        double v[kCoopUnroll], m[kCoopUnroll];
+        double v[kCoopUnroll], mm[kCoopUnroll];
 #pragma unroll
         for (int u = 0; u < kCoopUnroll; ++u) {
           const uint32_t eu = min(e + u, e1 - 1);
@@ -1236,62 +1301,123 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
           v[u] = a.val[eu];
         }
 #pragma unroll
-        for (int u = 0; u < kCoopUnroll; ++u) m[u] = ldAgent(src + i[u]);
+        for (int u = 0; u < kCoopUnroll; ++u) mm[u] = ldAgent(src + i[u]);
 #pragma unroll
         for (int u = 0; u < kCoopUnroll; ++u) {
-          const double prod = (m[u] / div) * v[u];
+          const double prod = (mm[u] / div) * v[u];
           s = (e + u < e1) ? s + prod : s;
         }
       }
-      stAgent(yCur + p, s);
+      stBits(yCur + p, uint64_t(__double_as_longlong(s)));
       sAbs += fabs(s);
-      if (t >= 2) {
-        const double x = yp / nPrev;
-        const double z = yp2 / ((t == 2) ? 1.0 : nPrev2);
-        sRes += fabs(x - z);
+      if (t >= 2) sRes += fabs(yp / nPrev - yp2 / div2);
+      if (p == p0) {
+        own2 = own1;
+        own1 = s;
       }
+      stBits(yOld + p, kCoopEmpty);  // after yp2 was read; the next value stored here is y_{t+1}
     }
-    sAbs = blockReduceSumF64(sAbs);
-    sRes = blockReduceSumF64(sRes);
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's y stores have completed
+    // block sums: each wave by a fixed shuffle tree, then the waves in order
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      sAbs += __shfl_xor(sAbs, o, kWave);
+      sRes += __shfl_xor(sRes, o, kWave);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's y stores and clears have completed
+    if (lane == 0) {
+      wsum[0][wv] = sAbs;
+      wsum[1][wv] = sRes;
+    }
     __syncthreads();
     uint64_t* buf = a.part + uint64_t(t % 3) * G * 2;
     if (tid == 0) {
-      stAgent(buf + 2 * b, uint64_t(__double_as_longlong(sAbs)));
-      stAgent(buf + 2 * b + 1, uint64_t(__double_as_longlong(sRes)));
-    }
-    if (tid < kWave) {
-      // lane l holds slots l, l + 64, ... ; sums in that order, then a fixed shuffle tree
-      uint64_t va[kSlots], vr[kSlots];
+      double A = 0.0, R = 0.0;
 #pragma unroll
-      for (uint32_t k = 0; k < kSlots; ++k) va[k] = vr[k] = 0;
-      bool ok = false;
-      for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
-        bool mine = true;
-#pragma unroll
-        for (uint32_t k = 0; k < kSlots; ++k) {
-          const uint32_t q = tid + k * kWave;
-          if (q < G && (va[k] == kCoopEmpty || vr[k] == kCoopEmpty || spin == 0)) {
-            va[k] = ldAgent(buf + 2 * q);
-            vr[k] = ldAgent(buf + 2 * q + 1);
-          }
-          mine = mine && (q >= G || (va[k] != kCoopEmpty && vr[k] != kCoopEmpty));
-        }
-        ok = __all(mine);
-        if (ok || ldAgent(a.err)) break;
-        __builtin_amdgcn_s_sleep(1);
+      for (int w = 0; w < kWaves; ++w) {
+        A += wsum[0][w];
+        R += wsum[1][w];
       }
+      stAgent(buf + 2 * b, uint64_t(__double_as_longlong(A)));
+      stAgent(buf + 2 * b + 1, uint64_t(__double_as_longlong(R)));
+    }
+    // the next iteration's first-column gathers of y_t, overlapping the wait
+    piGatherCached(yCur, ci, c0, c1, m);
+    if (tid < kWave) {
+      // Block 0 polls every partial slot, sums them in block order and publishes the totals in
+      // result slot t % 3; the other blocks poll that one slot. A poll round issues all of its
+      // loads (slots, the error flag and the wave's empty gathers) before testing any, so it
+      // costs one memory round trip.
+      uint64_t* rs = a.res + ((t % 3) * kPiResCopies + b % kPiResCopies) * kPiResStride;
       double ta = 0.0, tr = 0.0;
+      bool ok = false;
+      if (b == 0) {
+        // lane l holds slots l, l + 64, ... ; sums in that order, then a fixed shuffle tree.
+        // Groups past the grid are skipped by a uniform test; lanes past the last slot re-read
+        // slot G - 1.
+        uint64_t va[kSlots], vr[kSlots];
 #pragma unroll
-      for (uint32_t k = 0; k < kSlots; ++k)
-        if (tid + k * kWave < G) {
-          ta += __longlong_as_double((long long)va[k]);
-          tr += __longlong_as_double((long long)vr[k]);
+        for (uint32_t k = 0; k < kSlots; ++k) va[k] = vr[k] = 0;
+        uint32_t lt = tid;  // opaque: slot offsets are recomputed here, not held across iterations
+        asm volatile("" : "+v"(lt));
+        for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
+          const int e = ldAgent(a.err);
+#pragma unroll
+          for (uint32_t k = 0; k < kSlots; ++k) {
+            if (k * kWave < G) {
+              const uint32_t off = min(lt + k * kWave, G - 1) * 16u;  // byte offset of the slot
+              const char* base = reinterpret_cast<const char*>(buf);
+              va[k] = ldAgent(reinterpret_cast<const uint64_t*>(base + off));
+              vr[k] = ldAgent(reinterpret_cast<const uint64_t*>(base + off + 8));
+            }
+          }
+          bool mine = true;
+#pragma unroll
+          for (uint32_t k = 0; k < kSlots; ++k)
+            if (k * kWave < G) mine = mine && va[k] != kCoopEmpty && vr[k] != kCoopEmpty;
+          ok = __all(mine);
+          if (ok || e) break;
+          __builtin_amdgcn_s_sleep(1);
         }
 #pragma unroll
-      for (int o = kWave / 2; o > 0; o >>= 1) {
-        ta += __shfl_xor(ta, o, kWave);
-        tr += __shfl_xor(tr, o, kWave);
+        for (uint32_t k = 0; k < kSlots; ++k)
+          if (tid + k * kWave < G) {
+            ta += __longlong_as_double((long long)va[k]);
+            tr += __longlong_as_double((long long)vr[k]);
+          }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+          ta += __shfl_xor(ta, o, kWave);
+          tr += __shfl_xor(tr, o, kWave);
+        }
+        // the clears of result slot (t+1) % 3 (start of this iteration) have completed: the
+        // wave waited for its stores before the block published
+        if (tid < kPiResCopies && ok) {
+          uint64_t* rc = a.res + ((t % 3) * kPiResCopies + tid) * kPiResStride;
+          stAgent(rc, uint64_t(__double_as_longlong(ta)));
+          stAgent(rc + 1, uint64_t(__double_as_longlong(tr)));
+        }
+      }
+      {
+        uint64_t ra = 0, rr = 0;
+        for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
+          const int e = ldAgent(a.err);
+          if (b != 0) {
+            ra = ldAgent(rs);
+            rr = ldAgent(rs + 1);
+          }
+          if (spin > 0 && __any(piMissing(m, c0, c1))) piRegather(yCur, ci, c0, c1, m);
+          const bool have = b == 0 ? ok : (ra != kCoopEmpty && rr != kCoopEmpty);
+          const bool done = have && !__any(piMissing(m, c0, c1));
+          if (done || e) {
+            ok = done;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (b != 0) {
+          ta = __longlong_as_double((long long)ra);
+          tr = __longlong_as_double((long long)rr);
+        }
       }
       if (tid == 0) {
         if (!ok) stAgent(a.err, 1);
@@ -1299,9 +1425,27 @@ static __global__ __launch_bounds__(kBlock) void piCoopKernel(PiCoopArgs a) {
         sums[0] = ta;
         sums[1] = tr;
       }
+    } else {
+      // the other waves complete their gathers of y_t (their owners store them before they
+      // publish, so the entries fill in while the totals arrive)
+      bool ok = false;
+      for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
+        if (!__any(piMissing(m, c0, c1))) {
+          ok = true;
+          break;
+        }
+        const int e = ldAgent(a.err);
+        piRegather(yCur, ci, c0, c1, m);
+        if (e) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (!ok && lane == 0) {
+        stAgent(a.err, 1);
+        failed = 1;
+      }
     }
     __syncthreads();
-    if (leave) return;
+    if (leave || failed) return;
     const double ta = sums[0], tr = sums[1];
     const uint32_t completed = t - 1;
     const bool done = completed >= 1 && ((a.tol >= 0.0 && tr <= a.tol) || completed >= a.maxIter);
