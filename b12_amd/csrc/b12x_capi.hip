@@ -65,6 +65,7 @@ struct b12x_tree_s {
 
   TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
   ~b12x_tree_s() {
+    streamRetire(stream);
     for (int b = 0; b < 2; ++b) {
       dfree(key[b]);
       dfree(depth[b]);
@@ -95,6 +96,7 @@ struct b12x_coo_s {
   double* vals = nullptr;
   hipStream_t stream = nullptr;
   ~b12x_coo_s() {
+    streamRetire(stream);
     dfree(rows);
     dfree(cols);
     dfree(vals);
@@ -672,7 +674,7 @@ int b12x_tree_create(int dim, const double* center, const double* radius, const 
       return setErr(B12X_E_ARG, "sdScheme has values >= dim");
     }
     try {
-      HIPCHK(hipStreamCreate(&T->stream));
+      streamCreate(&T->stream);
       T->scal = dalloc<uint32_t>(kScal);
       HIPCHK(hipHostMalloc((void**)&T->hscal, kScal * sizeof(uint32_t), hipHostMallocDefault));
       T->rootKey = dalloc<uint64_t>(1);
@@ -1235,7 +1237,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
         dc == DC_ROOT ? 1u : (dc == DC_GENERAL ? DM.count : (dc == DC_BEYOND ? 0u : T.n));
     auto* M = new b12x_coo_s();
     try {
-      HIPCHK(hipStreamCreate(&M->stream));
+      streamCreate(&M->stream);
       M->nRows = M->nCols = nBoxes;
       M->rowMajor = rowMajor != 0;
       uint32_t* col = dalloc<uint32_t>(P);
@@ -1373,7 +1375,7 @@ int b12x_coo_upload(const uint32_t* rows, const uint32_t* cols, const double* va
     if (int e = ensureDevice()) return e;
     auto* M = new b12x_coo_s();
     try {
-      HIPCHK(hipStreamCreate(&M->stream));
+      streamCreate(&M->stream);
       M->nRows = nr;
       M->nCols = nc;
       M->nnz = nnz;

@@ -93,7 +93,7 @@ struct Pool {
     graphIters = 0;
   }
   void freeAll() {
-    if (stream) (void)hipStreamSynchronize(stream);
+    streamRetire(stream);
     dropGraph();
     dfree(colPtr); dfree(jobOfPos); dfree(y); dfree(rowIdx); dfree(cnt); dfree(table); dfree(jobOfBlock);
     dfree(partAbs); dfree(partRes); dfree(dJobs); dfree(coopPart); dfree(coopErr);
@@ -149,6 +149,7 @@ struct b12x_forest_s {
   int poolItersPerStep = 64;
 
   ~b12x_forest_s() {
+    streamRetire(stream);
     pool.freeAll();
     for (int b = 0; b < 2; ++b) {
       dfree(arr[b].key); dfree(arr[b].depth); dfree(arr[b].flags); dfree(arr[b].job);
@@ -896,8 +897,8 @@ int b12x_forest_create(int dim, int gk, const uint64_t* gp, int ngp, int mk, int
       F->rowMajor = rowMajor;
       F->piMaxIter = piMaxIter;
       F->piTol = piTol;
-      HIPCHK(hipStreamCreate(&F->stream));
-      HIPCHK(hipStreamCreate(&F->pool.stream));
+      streamCreate(&F->stream);
+      streamCreate(&F->pool.stream);
       std::vector<double> vt(kPoolTable);
       for (uint32_t c = 0; c < kPoolTable; ++c) vt[c] = double(c) / double(F->grid.k);
       F->pool.table = dalloc<double>(kPoolTable);

@@ -4,7 +4,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
+#include <vector>
 
 #include "../../include/b12x_c_api.h"
 
@@ -52,16 +58,192 @@ inline int ensureDevice() {
   return B12X_OK;
 }
 
+// Device memory cache. Fresh hipMalloc of a large block costs hundreds of microseconds (the
+// pages are mapped on allocation), so freed blocks are kept and handed out again. A freed block
+// may still be read or written by work queued on any of the library's streams, so it is reused
+// only after a fence: one event recorded on every registered stream after the free, all of them
+// complete. Fences are recorded when an allocation finds no idle block (one fence covers every
+// block freed since the last one) and are never recorded on a stream that is being captured into
+// a graph. A stream is registered by the object that creates it and retired (synchronised, then
+// no longer fenced) before that object frees its memory and destroys the stream. When hipMalloc
+// runs out of memory the cache waits for its fences, returns every idle block and tries again.
+// One cache per device, never destroyed (the HIP runtime may be gone at process exit).
+class DeviceCache {
+ public:
+  static DeviceCache& get() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+    static DeviceCache* caches[kMaxDevices] = {};
+    static std::mutex m;
+    std::lock_guard<std::mutex> lock(m);
+    if (!caches[dev]) caches[dev] = new DeviceCache();
+    return *caches[dev];
+  }
+
+  void* alloc(size_t bytes) {
+    const size_t want = roundUp(bytes);
+    std::lock_guard<std::mutex> lock(mu_);
+    harvest(false);
+    void* p = takeReady(want);
+    if (!p && !pending_.empty()) {
+      fence();
+      harvest(false);
+      p = takeReady(want);
+    }
+    if (!p) {
+      hipError_t e = hipMalloc(&p, want);
+      if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        if (!pending_.empty()) fence();
+        harvest(true);
+        trim(0);
+        p = takeReady(want);
+        if (!p) e = hipMalloc(&p, want);
+        else e = hipSuccess;
+      }
+      if (e != hipSuccess)
+        throw HipFail(e == hipErrorOutOfMemory ? B12X_E_NOMEM : B12X_E_HIP,
+                      std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    live_[p] = want;
+    return p;
+  }
+
+  void release(void* p) {
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = live_.find(p);
+    if (it == live_.end()) {  // not from this cache (or another device): plain free
+      (void)hipFree(p);
+      return;
+    }
+    pending_.push_back({p, it->second});
+    cached_ += it->second;
+    live_.erase(it);
+  }
+
+  void addStream(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    streams_.push_back(s);
+  }
+
+  // synchronises s and stops fencing it (its work is complete, so the blocks it used are idle)
+  void retireStream(hipStream_t s) {
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lock(mu_);
+    streams_.erase(std::remove(streams_.begin(), streams_.end(), s), streams_.end());
+  }
+
+ private:
+  static constexpr int kMaxDevices = 16;
+  static constexpr size_t kMaxCached = size_t(32) << 30;  // idle bytes kept before trimming
+
+  static size_t roundUp(size_t b) {
+    if (b == 0) b = 1;
+    const size_t g = b >= (size_t(1) << 20) ? (size_t(1) << 16) : 256;
+    return (b + g - 1) / g * g;
+  }
+
+  // the smallest idle block that holds `want` bytes without wasting more than half of it
+  void* takeReady(size_t want) {
+    auto it = ready_.lower_bound(want);
+    if (it == ready_.end() || it->first > 2 * want + (size_t(1) << 16)) return nullptr;
+    void* p = it->second;
+    cached_ -= it->first;
+    ready_.erase(it);
+    return p;
+  }
+
+  void fence() {
+    Fence f;
+    for (hipStream_t s : streams_) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    }
+    for (hipStream_t s : streams_) {
+      hipEvent_t e = nullptr;
+      if (!pool_.empty()) {
+        e = pool_.back();
+        pool_.pop_back();
+      } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        for (hipEvent_t x : f.ev) pool_.push_back(x);
+        return;
+      }
+      if (hipEventRecord(e, s) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_.push_back(e);
+        for (hipEvent_t x : f.ev) pool_.push_back(x);
+        return;
+      }
+      f.ev.push_back(e);
+    }
+    f.blocks.swap(pending_);
+    fences_.push_back(std::move(f));
+  }
+
+  // fences whose events have all completed release their blocks (wait: block on each fence)
+  void harvest(bool wait) {
+    while (!fences_.empty()) {
+      Fence& f = fences_.front();
+      bool done = true;
+      for (hipEvent_t e : f.ev) {
+        if (wait) {
+          if (hipEventSynchronize(e) != hipSuccess) (void)hipGetLastError();
+        } else if (hipEventQuery(e) != hipSuccess) {
+          (void)hipGetLastError();
+          done = false;
+          break;
+        }
+      }
+      if (!done) break;
+      for (hipEvent_t e : f.ev) pool_.push_back(e);
+      for (auto& b : f.blocks) ready_.insert({b.second, b.first});
+      fences_.pop_front();
+    }
+    if (cached_ > kMaxCached) trim(kMaxCached / 2);
+  }
+
+  // frees idle blocks, largest first, until at most `keep` cached bytes remain
+  void trim(size_t keep) {
+    while (cached_ > keep && !ready_.empty()) {
+      auto it = std::prev(ready_.end());
+      (void)hipFree(it->second);
+      cached_ -= it->first;
+      ready_.erase(it);
+    }
+  }
+
+  struct Fence {
+    std::vector<hipEvent_t> ev;
+    std::vector<std::pair<void*, size_t>> blocks;
+  };
+  std::mutex mu_;
+  std::multimap<size_t, void*> ready_;
+  std::vector<std::pair<void*, size_t>> pending_;
+  std::deque<Fence> fences_;
+  std::vector<hipStream_t> streams_;
+  std::unordered_map<void*, size_t> live_;
+  std::vector<hipEvent_t> pool_;
+  size_t cached_ = 0;  // bytes in pending, fenced and idle blocks
+};
+
 template <class T>
 T* dalloc(size_t count) {
-  void* p = nullptr;
-  if (count == 0) count = 1;
-  HIPCHK(hipMalloc(&p, count * sizeof(T)));
-  return static_cast<T*>(p);
+  return static_cast<T*>(DeviceCache::get().alloc((count ? count : 1) * sizeof(T)));
 }
 
 inline void dfree(void* p) {
-  if (p) (void)hipFree(p);
+  if (p) DeviceCache::get().release(p);
+}
+
+// a stream of one of the library's objects: registered for the cache's fences on creation,
+// synchronised and retired before the object frees its memory
+inline void streamCreate(hipStream_t* s) {
+  HIPCHK(hipStreamCreate(s));
+  DeviceCache::get().addStream(*s);
+}
+inline void streamRetire(hipStream_t s) {
+  if (s) DeviceCache::get().retireStream(s);
 }
 
 inline uint32_t ceilDiv(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
