@@ -33,6 +33,8 @@ SIGNATURES = {
     "b12x_size": [_P, _P],
     "b12x_count": [_P, _U8, _P],
     "b12x_depth": [_P, _P],
+    "b12x_boxes": [_P, _U8, _P, _P],
+    "b12x_enumerate": [_P, _P, _P],
     "b12x_tree_download": [_P, _P, _P, _P],
     "b12x_tree_upload": [_P, _U32, _P, _P, _P],
     "b12x_subdivide": [_P, _U8, _P],
@@ -317,6 +319,24 @@ class ImplicitBoxTree:
         r = _U8()
         self.api("b12x_depth", self.h, ctypes.byref(r))
         return r.value
+
+    def boxes(self, depth=-1):
+        """boxes(depth) (ImplicitBoxTree.h:86-91): array of shape (n, 2*dim + 1) holding the
+        centre, radius and flags of every box at depth, in leaf order."""
+        n = ctypes.c_uint64()
+        self.api("b12x_boxes", self.h, _depth(depth), None, ctypes.byref(n))
+        out = np.empty((n.value, 2 * self.dim + 1), np.float64)
+        if n.value:
+            self.api("b12x_boxes", self.h, _depth(depth), out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n))
+        return out
+
+    def enumerate(self):
+        """enumerate() (ImplicitBoxTree.h:132-133): {count(0), ..., count(depth())}."""
+        n = ctypes.c_int()
+        self.api("b12x_enumerate", self.h, None, ctypes.byref(n))
+        out = np.empty(n.value, np.uint32)
+        self.api("b12x_enumerate", self.h, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n))
+        return out
 
     def download(self):
         n = self.size()

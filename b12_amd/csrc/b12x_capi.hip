@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -25,6 +26,7 @@ using namespace b12x;
 // Handles
 // ------------------------------------------------------------------------------------------------
 struct b12x_scheme_s {
+  uint64_t id = 0;  // unique per scheme (a captured graph records the scheme it was built for)
   int dim = 0;
   GridParams grid{};
   MapParams map{};
@@ -63,9 +65,28 @@ struct b12x_tree_s {
   uint64_t imageLaunches = 0, imagePoints = 0;
   uint8_t* rootDepth = nullptr;
 
+  // the subdivision step (subdivide, images, keep) captured as a graph, in a small form (one-block
+  // scans) and a large one, for levels whose subdivided count fits the form's bound; rebuilt when
+  // the buffers move or the scheme or flag changes
+  struct StepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t scheme = 0, gen = 0;
+    uint8_t sd = 0;
+    int cur = -1;
+    void drop() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    }
+  } step[2];
+  uint64_t bufGen = 0;  // incremented whenever the leaf buffers are reallocated
+
   TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
   ~b12x_tree_s() {
     streamRetire(stream);
+    for (auto& g : step) g.drop();
     for (int b = 0; b < 2; ++b) {
       dfree(key[b]);
       dfree(depth[b]);
@@ -135,6 +156,7 @@ void ensureCap(b12x_tree_s& T, uint64_t need) {
     HIPCHK(hipMemcpyAsync(d[0], T.depth[T.cur], T.n, hipMemcpyDeviceToDevice, T.stream));
     HIPCHK(hipMemcpyAsync(f[0], T.flags[T.cur], T.n, hipMemcpyDeviceToDevice, T.stream));
   }
+  ++T.bufGen;
   uint8_t* h = dalloc<uint8_t>(nc);
   HIPCHK(hipMemsetAsync(h, 0, nc, T.stream));
   uint32_t* l = dalloc<uint32_t>(nc);
@@ -172,6 +194,15 @@ void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int sca
   hipLaunchKernelGGL(scanTotalsKernel, dim3(1), dim3(kBlock), 0, T.stream, totals, nb, nDev, T.scal + scalSlot,
                      (const uint32_t*)stats, statSlot >= 0 ? T.scal + statSlot : nullptr);
   if (!countOnly) hipLaunchKernelGGL(scanScatterKernel<Op>, dim3(nb), dim3(kBlock), 0, T.stream, op, n, nDev, totals);
+  HIPCHK(hipGetLastError());
+}
+
+// runScan in one block (short arrays); hostOut: where the block copies scal[0..3] at the end
+template <class Op>
+void runScanSmall(b12x_tree_s& T, const Op& op, uint32_t n, int scalSlot, const uint32_t* nDev, int statSlot,
+                  uint32_t* hostOut) {
+  hipLaunchKernelGGL(scanSmallKernel<Op>, dim3(1), dim3(kBlock), 0, T.stream, op, n, nDev, T.scal + scalSlot,
+                     statSlot >= 0 ? T.scal + statSlot : nullptr, (const uint32_t*)T.scal, hostOut);
   HIPCHK(hipGetLastError());
 }
 
@@ -401,9 +432,9 @@ constexpr int kNumImageVariants = int(sizeof(kImageVariants) / sizeof(kImageVari
 constexpr uint64_t kTuneSamplePoints = 1ull << 21;
 
 template <int DIM, int MAP, int PPL>
-void launchImage3(const ImageArgs& a, int gridKind, int bs, hipStream_t st) {
+void launchImage3(const ImageArgs& a, int gridKind, int bs, hipStream_t st, uint32_t maxBlocks = 1u << 22) {
   const uint64_t per = uint64_t(bs) * PPL;
-  const uint32_t blocks = uint32_t(std::min<uint64_t>((a.nPoints + per - 1) / per, 1u << 22));
+  const uint32_t blocks = uint32_t(std::min<uint64_t>((a.nPoints + per - 1) / per, maxBlocks));
   if (blocks == 0) return;
   if (gridKind == GRID_INNER)
     hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_INNER, PPL>), dim3(blocks), dim3(bs), 0, st, a);
@@ -480,6 +511,32 @@ void launchImage2(const ImageArgs& a, int gridKind, hipStream_t st) {
   launchVariant<DIM, MAP>(a, gridKind, kImageVariants[best], st);  // the whole launch
 }
 
+// one point per lane, 256 threads, at most maxBlocks blocks (the kernel strides over the rest):
+// the form a captured graph holds, where the grid is sized for a bound and the count is read on
+// the device
+template <int DIM, int MAP>
+void launchImageFixed2(const ImageArgs& a, int gridKind, hipStream_t st, uint32_t maxBlocks) {
+  if (a.nPoints == 0) return;
+  launchImage3<DIM, MAP, 1>(a, gridKind, kImageVariants[0].bs, st, maxBlocks);
+}
+
+void launchImageFixed(int dim, const ImageArgs& a, int gridKind, hipStream_t st, uint32_t maxBlocks) {
+  switch (a.map.kind) {
+    case MAP_IDENTITY:
+      if (dim == 1) launchImageFixed2<1, MAP_IDENTITY>(a, gridKind, st, maxBlocks);
+      else if (dim == 2) launchImageFixed2<2, MAP_IDENTITY>(a, gridKind, st, maxBlocks);
+      else if (dim == 3) launchImageFixed2<3, MAP_IDENTITY>(a, gridKind, st, maxBlocks);
+      else launchImageFixed2<4, MAP_IDENTITY>(a, gridKind, st, maxBlocks);
+      break;
+    case MAP_HENON: launchImageFixed2<2, MAP_HENON>(a, gridKind, st, maxBlocks); break;
+    case MAP_LORENZ: launchImageFixed2<3, MAP_LORENZ>(a, gridKind, st, maxBlocks); break;
+    case MAP_HENON_HEILES: launchImageFixed2<4, MAP_HENON_HEILES>(a, gridKind, st, maxBlocks); break;
+    case MAP_DUFFING: launchImageFixed2<2, MAP_DUFFING>(a, gridKind, st, maxBlocks); break;
+    default: throw HipFail(B12X_E_UNSUPPORTED, "map kind");
+  }
+  HIPCHK(hipGetLastError());
+}
+
 void launchImage(int dim, const ImageArgs& a, int gridKind, hipStream_t st) {
   switch (a.map.kind) {
     case MAP_IDENTITY:
@@ -495,6 +552,82 @@ void launchImage(int dim, const ImageArgs& a, int gridKind, hipStream_t st) {
     default: throw HipFail(B12X_E_UNSUPPORTED, "map kind");
   }
   HIPCHK(hipGetLastError());
+}
+
+// The subdivision step of the subdivision algorithm: subdivide(sd) (ImplicitBoxTree.hpp:2883-2930),
+// setFlagsByIterators(OverLeaves(grid, map), HIT, -1) (the images of every test point), then
+// remove(HIT) and unsetFlags(HIT, -1) as one stable compaction. boundN bounds the leaves before
+// the step; countDev holds their count on the device (nullptr: boundN is the count).
+// The captured forms hold a fixed image variant with a capped grid (a tree that times its images
+// launches eagerly); the small form scans in one block and its last kernel copies the step's
+// counts to the pinned scalars, the large form copies them with a memcpy node.
+enum StepForm { STEP_EAGER, STEP_LARGE, STEP_SMALL };
+constexpr uint32_t kStepGraphLeaves = 1u << 17;  // bound of the large form's subdivided count
+constexpr uint32_t kStepSmallLeaves = 1u << 14;  // bound of the small form's subdivided count
+constexpr uint32_t kStepGraphBlocks = 8192;      // image blocks of a captured step (they stride)
+constexpr int kScalStepIn = 20;                  // pinned slot staging the leaf count
+
+bool stepGraphsOn(const b12x_tree_s& T) {
+  if (T.timing) return false;  // image timing brackets each eager image launch with events
+  const char* ge = std::getenv("B12X_GRAPHS");
+  if (ge && ge[0] == '0') return false;
+  const char* ds = std::getenv("B12X_DESCENT_LDS");
+  return !(ds && ds[0] == '1');
+}
+
+void enqueueStep(b12x_tree_s& T, b12x_scheme_s& S, uint8_t sd, uint32_t boundN, const uint32_t* countDev,
+                 StepForm form) {
+  SubdivideOp sop{};
+  sop.in = T.view(T.cur);
+  sop.out = T.view(1 - T.cur);
+  sop.mask = sd;
+  if (form == STEP_SMALL) runScanSmall(T, sop, boundN, 0, countDev, 1, nullptr);
+  else runScan(T, sop, boundN, T.totals, 0, countDev, 1);
+  T.cur = 1 - T.cur;
+  ImageArgs a{};
+  a.dom = T.dom;
+  a.map = S.map;
+  a.grid = S.grid;
+  a.key = T.key[T.cur];
+  a.depth = T.depth[T.cur];
+  a.srcKey = T.key[T.cur];
+  a.srcDepth = T.depth[T.cur];
+  a.nPoints = uint64_t(boundN) * 2 * S.grid.k;  // grid bound; the kernel reads the count
+  a.nBoxesDev = T.scal + 0;
+  a.mode = 0;
+  a.hit = T.hit;
+  if (form != STEP_EAGER) {
+    launchImageFixed(T.dim, a, S.grid.kind, T.stream, kStepGraphBlocks);
+  } else {
+    a.samples = stageSamples(T, T.scal + 0);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (T.timing) {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipEventRecord(e0, T.stream));
+    }
+    launchImage(T.dim, a, S.grid.kind, T.stream);
+    if (T.timing) {
+      HIPCHK(hipEventRecord(e1, T.stream));
+      T.pending.push_back({e0, e1});
+      T.imageLaunches += 1;
+    }
+  }
+  KeepOp kop{};
+  kop.in = T.view(T.cur);
+  kop.out = T.view(1 - T.cur);
+  kop.hit = T.hit;
+  kop.mask = kop.hitFlag = kop.clearMask = uint8_t(F_HIT);
+  kop.clearHit = 1;
+  if (form == STEP_SMALL) runScanSmall(T, kop, boundN * 2, 2, T.scal + 0, 3, T.hscal);
+  else runScan(T, kop, boundN * 2 > 0 ? boundN * 2 : 1, T.totals, 2, T.scal + 0, 3);
+  T.cur = 1 - T.cur;
+}
+
+// the leaf count where a captured step reads it (the keep of every step leaves it there too)
+void stageLeafCount(b12x_tree_s& T) {
+  T.hscal[kScalStepIn] = T.n;
+  HIPCHK(hipMemcpyAsync(T.scal + 2, T.hscal + kScalStepIn, 4, hipMemcpyHostToDevice, T.stream));
 }
 
 // Image of every point of `src` located on the leaves (or on the root when depth0).
@@ -855,6 +988,85 @@ int b12x_remove(b12x_tree t, uint8_t f, uint32_t* ret) {
   });
 }
 
+// ImplicitBoxTree.hpp:611-784 boxes(depth) and :1273-1283 enumerate()
+int b12x_boxes(b12x_tree t, uint8_t d, double* out, uint64_t* nOut) {
+  return guarded([&]() -> int {
+    if (!t || !nOut) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    const int D = T.dim;
+    const uint64_t W = uint64_t(2 * D + 1);
+    const DepthClass dc = classifyDepth(T, d);
+    if (dc == DC_BEYOND) {
+      *nOut = 0;
+      return B12X_OK;
+    }
+    if (dc == DC_ROOT) {  // the surrounding box, flags ANDed over all leaves
+      *nOut = 1;
+      if (out) {
+        const LeafStats st = leafStats(T);
+        for (int j = 0; j < D; ++j) {
+          out[j] = T.dom.center[j];
+          out[D + j] = T.dom.radius[j];
+        }
+        out[2 * D] = double(st.flagsAnd);
+      }
+      return B12X_OK;
+    }
+    DepthMap M;
+    uint32_t* acc = nullptr;
+    uint64_t nb = T.n;
+    if (dc == DC_GENERAL) {
+      buildDepthMap(T, d, M);
+      nb = M.count;
+    }
+    *nOut = nb;
+    if (!out || nb == 0) return B12X_OK;
+    if (dc == DC_GENERAL) {
+      acc = dalloc<uint32_t>(nb);
+      hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(nb, kBlock)), dim3(kBlock), 0, T.stream, acc, nb, 0xFFu);
+      hipLaunchKernelGGL(depthCombineKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream,
+                         (const uint8_t*)T.flags[T.cur], T.n, (const uint32_t*)M.box, (const uint8_t*)M.in, 0, acc);
+    }
+    double* dout = dalloc<double>(nb * W);
+    const uint8_t* st = dc == DC_GENERAL ? M.start : nullptr;
+    const dim3 g(ceilDiv(T.n, kBlock)), bl(kBlock);
+    switch (D) {
+      case 1: hipLaunchKernelGGL(boxesKernel<1>, g, bl, 0, T.stream, T.dom, T.view(T.cur), T.n, d, st, (const uint32_t*)M.box, (const uint32_t*)acc, dout); break;
+      case 2: hipLaunchKernelGGL(boxesKernel<2>, g, bl, 0, T.stream, T.dom, T.view(T.cur), T.n, d, st, (const uint32_t*)M.box, (const uint32_t*)acc, dout); break;
+      case 3: hipLaunchKernelGGL(boxesKernel<3>, g, bl, 0, T.stream, T.dom, T.view(T.cur), T.n, d, st, (const uint32_t*)M.box, (const uint32_t*)acc, dout); break;
+      default: hipLaunchKernelGGL(boxesKernel<4>, g, bl, 0, T.stream, T.dom, T.view(T.cur), T.n, d, st, (const uint32_t*)M.box, (const uint32_t*)acc, dout); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout, nb * W * sizeof(double), hipMemcpyDeviceToHost, T.stream));
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(dout);
+    dfree(acc);
+    return B12X_OK;
+  });
+}
+
+int b12x_enumerate(b12x_tree t, uint32_t* out, int* nOut) {
+  return guarded([&]() -> int {
+    if (!t || !nOut) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    const int dmax = leafStats(T).maxDepth;
+    *nOut = dmax + 1;
+    if (!out) return B12X_OK;
+    for (int d = 0; d <= dmax; ++d) {
+      const DepthClass dc = classifyDepth(T, uint8_t(d));
+      if (dc == DC_ROOT) out[d] = 1;
+      else if (dc == DC_LEAF) out[d] = T.n;
+      else if (dc == DC_BEYOND) out[d] = 0;
+      else {
+        DepthMap M;
+        buildDepthMap(T, uint8_t(d), M);
+        out[d] = M.count;
+      }
+    }
+    return B12X_OK;
+  });
+}
+
 int b12x_set_flags(b12x_tree t, uint8_t f, uint8_t d, uint32_t* ret) {
   return guarded([&]() -> int {
     if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
@@ -1052,60 +1264,55 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
     if (!t || nSteps < 0) return setErr(B12X_E_ARG, "bad argument");
     b12x_tree_s& T = *t;
     checkScheme(T, s);
+    const bool graphs = stepGraphsOn(T);
     // Device scalars: [0] leaves after subdivide, [1] #leaves with sd, [2] leaves kept,
     // [3] #leaves newly hit. One host synchronisation per step.
+    if (graphs && nSteps > 0) stageLeafCount(T);
     for (int step = 0; step < nSteps; ++step) {
       const uint32_t n = T.n;
-      ensureCap(T, uint64_t(n) * 2);
-      // subdivide(sd): ImplicitBoxTree.hpp:2883-2930
-      SubdivideOp sop{};
-      sop.in = T.view(T.cur);
-      sop.out = T.view(1 - T.cur);
-      sop.mask = sd;
-      runScan(T, sop, n, T.totals, 0, nullptr, 1);
-      T.cur = 1 - T.cur;
-      // setFlagsByIterators(OverLeaves(grid, map), HIT, -1): image of every test point
-      ImageArgs a{};
-      a.dom = T.dom;
-      a.map = s->map;
-      a.grid = s->grid;
-      a.key = T.key[T.cur];
-      a.depth = T.depth[T.cur];
-      a.srcKey = T.key[T.cur];
-      a.srcDepth = T.depth[T.cur];
-      a.nPoints = uint64_t(n) * 2 * s->grid.k;  // grid bound; the kernel reads the count
-      a.nBoxesDev = T.scal + 0;
-      a.mode = 0;
-      a.hit = T.hit;
-      a.samples = stageSamples(T, T.scal + 0);
-      {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (T.timing) {
-          HIPCHK(hipEventCreate(&e0));
-          HIPCHK(hipEventCreate(&e1));
-          HIPCHK(hipEventRecord(e0, T.stream));
+      const uint64_t sub = uint64_t(n) * 2;
+      const StepForm form = !graphs ? STEP_EAGER
+                            : sub <= kStepSmallLeaves ? STEP_SMALL
+                            : sub <= kStepGraphLeaves ? STEP_LARGE
+                                                      : STEP_EAGER;
+      if (form != STEP_EAGER) {
+        // replay the captured step; the leaf count chains through scal[2] on the device and the
+        // buffers end in the same half after a step
+        ensureCap(T, kStepGraphLeaves);
+        b12x_tree_s::StepGraph& G = T.step[form == STEP_SMALL ? 1 : 0];
+        if (!G.exec || G.scheme != s->id || G.sd != sd || G.gen != T.bufGen || G.cur != T.cur) {
+          G.drop();
+          const uint32_t bound = (form == STEP_SMALL ? kStepSmallLeaves : kStepGraphLeaves) / 2;
+          HIPCHK(hipStreamBeginCapture(T.stream, hipStreamCaptureModeThreadLocal));
+          try {
+            enqueueStep(T, *s, sd, bound, T.scal + 2, form);
+            if (form == STEP_LARGE)
+              HIPCHK(hipMemcpyAsync(T.hscal, T.scal, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, T.stream));
+          } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(T.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          HIPCHK(hipStreamEndCapture(T.stream, &G.graph));
+          HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+          G.scheme = s->id;
+          G.sd = sd;
+          G.gen = T.bufGen;
+          G.cur = T.cur;
         }
-        launchImage(T.dim, a, s->grid.kind, T.stream);
-        if (T.timing) {
-          HIPCHK(hipEventRecord(e1, T.stream));
-          T.pending.push_back({e0, e1});
-          T.imageLaunches += 1;
-        }
+        HIPCHK(hipGraphLaunch(G.exec, T.stream));
+        HIPCHK(hipStreamSynchronize(T.stream));
+      } else {
+        ensureCap(T, sub);
+        enqueueStep(T, *s, sd, n, nullptr, STEP_EAGER);
+        syncScal(T, 4);
       }
-      // remove(HIT) + unsetFlags(HIT, -1) as one stable compaction; counts setFlags' return value
-      KeepOp kop{};
-      kop.in = T.view(T.cur);
-      kop.out = T.view(1 - T.cur);
-      kop.hit = T.hit;
-      kop.mask = kop.hitFlag = kop.clearMask = uint8_t(F_HIT);
-      kop.clearHit = 1;
-      runScan(T, kop, n * 2 > 0 ? n * 2 : 1, T.totals, 2, T.scal + 0, 3);
-      T.cur = 1 - T.cur;
-      syncScal(T, 4);
       const uint32_t nSub = T.hscal[0], cSub = T.hscal[1], kept = T.hscal[2], cSet = T.hscal[3];
       if (T.timing) T.imagePoints += uint64_t(nSub) * s->grid.k;
       T.n = kept;
       insertRootIfEmpty(T);
+      if (graphs && T.n != kept) stageLeafCount(T);
       if (counts) {
         counts[4 * step + 0] = cSub;
         counts[4 * step + 1] = cSet;
@@ -1140,7 +1347,9 @@ int b12x_scheme_create(int dim, int gk, const uint64_t* gp, int ngp, int mk, con
   return guarded([&]() -> int {
     if (!out || dim < 1 || dim > kMaxDim) return setErr(B12X_E_ARG, "bad argument");
     if (int e = ensureDevice()) return e;
+    static std::atomic<uint64_t> ids{0};
     auto* S = new b12x_scheme_s();
+    S->id = ++ids;
     S->dim = dim;
     S->grid.kind = gk;
     S->map.kind = mk;

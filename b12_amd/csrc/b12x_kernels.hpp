@@ -168,6 +168,52 @@ static __global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32
   }
 }
 
+// The three passes above in one block, for short arrays (a launch instead of three): tiles of
+// kTile elements in order, each scanned in the block and emitted after the carry of the tiles
+// before it. Writes the total to *total and the statistic to *statTotal; with hostOut set, thread
+// 0 then copies scal[0..3] there (a step's counts, read by the host after the stream
+// synchronises).
+template <class Op>
+static __global__ __launch_bounds__(kBlock) void scanSmallKernel(Op op, uint32_t n, const uint32_t* nDev,
+                                                          uint32_t* total, uint32_t* statTotal,
+                                                          const uint32_t* scal, uint32_t* hostOut) {
+  n = elemCount(n, nDev);
+  uint32_t carry = 0, stat = 0;
+  for (uint32_t off = 0; off < n; off += kTile) {
+    const uint64_t base = uint64_t(off) + uint64_t(threadIdx.x) * kItems;
+    uint32_t c[kItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int t = 0; t < kItems; ++t) {
+      const uint64_t i = base + t;
+      c[t] = i < n ? op.count(uint32_t(i)) : 0u;
+      s += c[t];
+      if (statTotal && i < n) stat += op.stat(uint32_t(i));
+    }
+    uint32_t tot;
+    uint32_t pos = blockExclusiveScan(s, tot) + carry;
+#pragma unroll
+    for (int t = 0; t < kItems; ++t) {
+      const uint64_t i = base + t;
+      if (i < n) {
+        if (c[t]) op.emit(uint32_t(i), c[t], pos);
+        op.after(uint32_t(i));
+      }
+      pos += c[t];
+    }
+    carry += tot;
+  }
+  if (statTotal) stat = blockReduceSum(stat);
+  if (threadIdx.x == 0) {
+    *total = carry;
+    if (statTotal) *statTotal = stat;
+    if (hostOut) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) hostOut[q] = scal[q];
+    }
+  }
+}
+
 struct TreeView {
   uint64_t* key;
   uint8_t* depth;
@@ -599,6 +645,43 @@ static __global__ __launch_bounds__(kBlock) void depthCheckKernel(const uint8_t*
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n || !in[i]) return;
   if ((flags[i] & check) != check) atomicAnd(ok + box[i], 0u);
+}
+
+// boxes(depth) rows {centre[DIM], radius[DIM], flags} (ImplicitBoxTree.hpp:611-775,
+// coordinateFunctions.hpp:95-159): one per leaf (start == nullptr), or one per box at depth d
+// from the first leaf of the box, truncated to d, with the flags combined over the box (acc)
+template <int DIM>
+static __global__ __launch_bounds__(kBlock) void boxesKernel(Domain dom, TreeView v, uint32_t n, uint8_t d,
+                                                            const uint8_t* start, const uint32_t* box,
+                                                            const uint32_t* acc, double* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint64_t key;
+  int dep;
+  uint8_t f;
+  uint64_t row;
+  if (start) {
+    if (!start[i]) return;
+    uint64_t b;
+    truncateTo(v.key[i], v.depth[i], d, b, dep);
+    key = keyOf(b, dep);
+    f = uint8_t(acc[box[i]]);
+    row = box[i];
+  } else {
+    key = v.key[i];
+    dep = v.depth[i];
+    f = v.flags[i];
+    row = i;
+  }
+  double c[DIM], r[DIM];
+  boxCoords<DIM>(dom, key, dep, c, r);
+  double* o = out + row * (2 * DIM + 1);
+#pragma unroll
+  for (int j = 0; j < DIM; ++j) {
+    o[j] = c[j];
+    o[DIM + j] = r[j];
+  }
+  o[2 * DIM] = double(f);
 }
 
 static __global__ __launch_bounds__(kBlock) void depthSourceCountKernel(const uint8_t* startFlag, const uint32_t* box,

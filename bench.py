@@ -303,6 +303,9 @@ def main():
     for _ in range(args.steps):
         runs.append(gpu_job(gx, args.flavour))
     t1 = time.perf_counter()
+    # image-kernel timing: eager launches bracketed by events on the tree's stream; the first
+    # such job settles the autotuner's choices, the second is the one reported
+    gpu_job(gx, args.flavour, timing=True)
     prof = gpu_job(gx, args.flavour, timing=True)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     images = runs[0]["images"]

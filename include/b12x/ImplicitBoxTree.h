@@ -229,6 +229,27 @@ class ImplicitBoxTree {
     return d;
   }
   Dimension dim() const { return DIM; }
+  // boxes(depth) — ImplicitBoxTree.h:86-91: {centre, radius, flags} of every box at depth
+  typename ThrustSystem<A>::template Vector<REAL> boxes(Depth depth) const {
+    uint64_t n = 0;
+    detail::check(b12x_boxes(h_, depth, nullptr, &n), "boxes");
+    typename ThrustSystem<A>::template Vector<REAL> res(n * (2 * DIM + 1));
+    if (n) detail::check(b12x_boxes(h_, depth, res.data(), &n), "boxes");
+    return res;
+  }
+  void boxes(Depth depth, REAL* res_begin) const {
+    uint64_t n = 0;
+    detail::check(b12x_boxes(h_, depth, res_begin, &n), "boxes");
+  }
+
+  // enumerate() — ImplicitBoxTree.h:132-133: {count(0), ..., count(depth())}
+  typename ThrustSystem<A>::template Vector<NrBoxes> enumerate() const {
+    int n = 0;
+    detail::check(b12x_enumerate(h_, nullptr, &n), "enumerate");
+    typename ThrustSystem<A>::template Vector<NrBoxes> res(n);
+    detail::check(b12x_enumerate(h_, res.data(), &n), "enumerate");
+    return res;
+  }
 
   // bitPatternVector_, depthVector_, flagsVector_ as host copies (ImplicitBoxTree.h:619-624)
   void getLeaves(Vector<uint64_t>& bits, Vector<Depth>& depths, Vector<Flags>& flags) const {

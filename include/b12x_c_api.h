@@ -82,6 +82,14 @@ int b12x_tree_destroy(b12x_tree t);
 int b12x_size(b12x_tree t, uint32_t* out);
 int b12x_count(b12x_tree t, uint8_t depth, uint32_t* out);
 int b12x_depth(b12x_tree t, uint8_t* out);
+/* ImplicitBoxTree::boxes(Depth) (ImplicitBoxTree.h:86-91, ImplicitBoxTree.hpp:611-784): *n_out
+   boxes of depth `depth`, each 2*dim+1 doubles {centre, radius, flags}, in leaf order; the root
+   carries the AND of all leaf flags, a box between root and leaves the AND over its leaves.
+   out may be NULL to ask for *n_out only. */
+int b12x_boxes(b12x_tree t, uint8_t depth, double* out, uint64_t* n_out);
+/* ImplicitBoxTree::enumerate() (ImplicitBoxTree.h:132-133, ImplicitBoxTree.hpp:1273-1283):
+   *n_out = depth() + 1 entries count(0), ..., count(depth()); out may be NULL. */
+int b12x_enumerate(b12x_tree t, uint32_t* out, int* n_out);
 /* Copy the leaves out / in as B12's SoA (bitPatternVector_, depthVector_, flagsVector_,
  * ImplicitBoxTree.h:619-624). upload sorts the leaves into B12's order. */
 int b12x_tree_download(b12x_tree t, uint64_t* bits, uint8_t* depth, uint8_t* flags);

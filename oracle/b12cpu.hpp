@@ -615,6 +615,57 @@ class Tree {
     return 0;
   }
 
+  // ImplicitBoxTree.hpp:1273-1283 enumerate: {count(0), ..., count(depth())}
+  std::vector<NrBoxes> enumerate() const {
+    std::vector<NrBoxes> r;
+    for (int d = 0; d <= int(depth()); ++d) r.push_back(count(Depth(d)));
+    return r;
+  }
+
+  // ImplicitBoxTree.hpp:611-775 boxes(depth): {centre, radius, flags} of every box at depth d.
+  // The root carries the AND of all leaf flags; at the leaves (or a depth all leaves share) each
+  // leaf with its flags; at a depth between root and leaves or the unsubdividable depth the runs
+  // of leaves that truncate to the same box, in leaf order, flags ANDed over the run
+  // (reduce_by_key / inclusive_scan_by_key + unique), leaves shallower than d (or not
+  // unsubdividable) left out; beyond the deepest leaf nothing.
+  std::vector<double> boxes(Depth d) const {
+    std::vector<double> out;
+    auto put = [&](const Box& b, Flags f) {
+      double c[DIM], r[DIM];
+      geo.coords(b, c, r);
+      for (int j = 0; j < DIM; ++j) out.push_back(c[j]);
+      for (int j = 0; j < DIM; ++j) out.push_back(r[j]);
+      out.push_back(double(f));
+    };
+    if (d == 0) {
+      Flags f = ALL;
+      for (Flags x : flagsV) f = Flags(f & x);
+      for (int j = 0; j < DIM; ++j) out.push_back(geo.center[j]);
+      for (int j = 0; j < DIM; ++j) out.push_back(geo.radius[j]);
+      out.push_back(double(f));
+      return out;
+    }
+    if (d == kLeafDepth || areAllLeavesOnDepth(d)) {
+      for (size_t i = 0; i < bits.size(); ++i) put(box(i), flagsV[i]);
+      return out;
+    }
+    if (!depthExists(d)) return out;
+    const size_t n = bits.size();
+    for (size_t i = 0; i < n;) {
+      if (!inDepth(i, d)) {
+        ++i;
+        continue;
+      }
+      const Box b = decrementDepthTo(box(i), d);
+      Flags f = flagsV[i];
+      size_t k = i + 1;
+      while (k < n && inDepth(k, d) && isSameBox(decrementDepthTo(box(k), d), b)) f = Flags(f & flagsV[k++]);
+      put(b, f);
+      i = k;
+    }
+    return out;
+  }
+
   // Depths other than the root, the leaves and a depth every leaf has: a depth d between them
   // (0 < d <= depth()) or the unsubdividable depth. The boxes at such a depth are the leaves'
   // ancestors at d (parents that could be merged, at the unsubdividable depth).

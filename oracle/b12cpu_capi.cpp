@@ -191,6 +191,26 @@ int b12cpu_size(void* h, uint32_t* out) { TREE_CALL(h, { *out = T.size(); return
 int b12cpu_depth(void* h, uint8_t* out) { TREE_CALL(h, { *out = T.depth(); return 0; }); }
 int b12cpu_count(void* h, uint8_t d, uint32_t* out) { TREE_CALL(h, { *out = T.count(d); return 0; }); }
 
+// boxes(depth): *n boxes of 2*dim+1 doubles each; out may be null to ask for *n only
+int b12cpu_boxes(void* h, uint8_t d, double* out, uint64_t* n) {
+  TREE_CALL(h, {
+    const std::vector<double> b = T.boxes(d);
+    *n = b.size() / (2 * D + 1);
+    if (out) std::copy(b.begin(), b.end(), out);
+    return 0;
+  });
+}
+
+// enumerate(): *n = depth() + 1 counts; out may be null
+int b12cpu_enumerate(void* h, uint32_t* out, int* n) {
+  TREE_CALL(h, {
+    const std::vector<NrBoxes> e = T.enumerate();
+    *n = int(e.size());
+    if (out) std::copy(e.begin(), e.end(), out);
+    return 0;
+  });
+}
+
 int b12cpu_download(void* h, uint64_t* bits, uint8_t* depth, uint8_t* flags) {
   TREE_CALL(h, {
     std::memcpy(bits, T.bits.data(), T.bits.size() * 8);

@@ -176,6 +176,22 @@ class OracleTree:
         _chk(lib().b12cpu_download(self.h, _ptr(b), _ptr(d), _ptr(f)))
         return b, d, f
 
+    def boxes(self, depth=-1):
+        """boxes(depth): array of shape (n, 2*dim + 1) = centre, radius, flags per box."""
+        n = ctypes.c_uint64()
+        _chk(lib().b12cpu_boxes(self.h, _u8(depth), None, ctypes.byref(n)))
+        out = np.empty((n.value, 2 * self.dim + 1), np.float64)
+        if n.value:
+            _chk(lib().b12cpu_boxes(self.h, _u8(depth), _ptr(out), ctypes.byref(n)))
+        return out
+
+    def enumerate(self):
+        n = ctypes.c_int()
+        _chk(lib().b12cpu_enumerate(self.h, None, ctypes.byref(n)))
+        out = np.empty(n.value, np.uint32)
+        _chk(lib().b12cpu_enumerate(self.h, _ptr(out), ctypes.byref(n)))
+        return out
+
     def upload(self, bits, depth, flags):
         b = np.ascontiguousarray(bits, np.uint64)
         d = np.ascontiguousarray(depth, np.uint8)

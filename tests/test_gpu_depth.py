@@ -124,3 +124,22 @@ def test_unsubdividable_depth_count_matches_unsubdivide():
     assert n == o.count(UNSUB)
     assert g.unsubdivide(0) == o.unsubdivide(0) == n
     assert same_tree(g.download(), o.download())
+
+
+@pytest.mark.parametrize("dim,sd,seed", [(1, None, 7), (2, None, 8), (3, None, 9), (4, None, 10), (2, [1, 0, 0, 1, 1], 11)])
+def test_boxes_and_enumerate_match_oracle(dim, sd, seed):
+    """boxes(depth) bit for bit (centres, radii, combined flags) at the root, every depth between
+    root and leaves, one past the deepest leaf, the unsubdividable depth and the leaves; the
+    general subdivision scheme included; enumerate() equal."""
+    print(f"[depth] boxes dim={dim} sd={sd} seed={seed}", flush=True)
+    rng = np.random.default_rng(600 + seed)
+    c, r = rng.uniform(-1, 1, dim), rng.uniform(0.5, 2, dim)
+    g, o = gx.ImplicitBoxTree(c, r, sd), OracleTree(c, r, sd)
+    bits, depth, flags = random_tree(rng, dim, 700, max_depth=12)
+    g.upload(bits, depth, flags)
+    o.upload(bits, depth, flags)
+    assert np.array_equal(g.enumerate(), o.enumerate())
+    for d in list(range(0, o.depth() + 2)) + [UNSUB, -1]:
+        a, b = g.boxes(d), o.boxes(d)
+        assert a.shape == b.shape, d
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), d

@@ -151,6 +151,27 @@ def test_subdivision_duffing_cover():
     subdivision_run([0.0, 0.0], [2.0, 2.0], gx.InnerGrid(4), gx.Duffing(), 10)
 
 
+@pytest.mark.parametrize("name,center,radius,grid,mp,steps", [
+    ("henon", [0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 18),
+    ("lorenz", [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz(), 26),
+])
+def test_subdivision_step_graph_matches_eager_launches(monkeypatch, name, center, radius, grid, mp, steps):
+    """The captured subdivision step (levels whose subdivided count fits the graph bound) and
+    the eager launches give the same per-step counts and covers; the Lorenz run crosses the
+    bound (more than 65536 leaves from depth 25 on), so both forms alternate within it."""
+    print(f"[subdivision graph] {name} d{steps}", flush=True)
+    res = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("B12X_GRAPHS", graphs)
+        g = gx.ImplicitBoxTree(center, radius)
+        g.setFlags(gx.SD, 0)
+        res[graphs] = (g.subdivision_steps(grid, mp, steps), g.download())
+    assert np.array_equal(res["1"][0], res["0"][0])
+    assert same_tree(res["1"][1], res["0"][1])
+    if name == "lorenz":
+        assert res["1"][0][-1][3] > 65536
+
+
 def test_stepwise_api_matches_fused_step():
     """The four reference calls one by one give the same cover as the fused step."""
     g1, o = both([0.0, 0.0], [1.5, 0.5])

@@ -462,6 +462,71 @@ def test_identity_matrix_at_depth_and_unsubdividable_count():
     assert n_unsub == o.unsubdivide(0)
 
 
+def bisect_box(b, d, c, r):
+    """Bounds of the box with pattern b at depth d (default scheme: coordinate lvl % dim),
+    by bisecting the domain level by level (independent of the oracle)."""
+    dim = len(c)
+    lo, hi = np.array(c, float) - np.array(r, float), np.array(c, float) + np.array(r, float)
+    for lvl in range(d):
+        j = lvl % dim
+        mid = 0.5 * (lo[j] + hi[j])
+        if (b >> (d - 1 - lvl)) & 1:
+            lo[j] = mid
+        else:
+            hi[j] = mid
+    return lo, hi
+
+
+def test_boxes_and_enumerate_against_bisection():
+    """boxes(depth): centre and radius from bisecting the domain along the pattern, flags ANDed
+    over the leaves of each box (the root: over all leaves), boxes in leaf order, nothing beyond
+    the deepest leaf; enumerate() = the number of distinct depth-d ancestors per depth."""
+    rng = np.random.default_rng(61)
+    for dim in (2, 3):
+        c, r = rng.uniform(-1, 1, dim), rng.uniform(0.5, 2, dim)
+        o = OracleTree(c, r)
+        o.upload(*random_tree(rng, dim, 600, max_depth=12))
+        bits, depth, flags = o.download()
+        dmax = int(depth.max())
+        e = o.enumerate()
+        assert len(e) == dmax + 1 and e[0] == 1
+        for d in range(0, dmax + 2):
+            got = o.boxes(d)
+            if d == 0:
+                exp_flags = np.bitwise_and.reduce(flags)
+                assert got.shape == (1, 2 * dim + 1)
+                assert np.array_equal(got[0, :dim], c) and np.array_equal(got[0, dim:2 * dim], r)
+                assert got[0, -1] == exp_flags
+                continue
+            groups, gid = depth_groups(bits, depth, d)
+            if d <= dmax:
+                assert e[d] == len(groups)
+            assert got.shape == (len(groups), 2 * dim + 1), d
+            for g, a in enumerate(groups):
+                lo, hi = bisect_box(a, d, c, r)
+                # the bisection rounds at every midpoint; a wrong bisection count is off by 2x
+                assert np.allclose(got[g, :dim], 0.5 * (lo + hi), rtol=0, atol=1e-12 * np.abs(r).max())
+                assert np.allclose(got[g, dim:2 * dim], 0.5 * (hi - lo), rtol=1e-12, atol=0)
+                assert got[g, -1] == np.bitwise_and.reduce(flags[gid == g])
+        # the unsubdividable depth: the parents unsubdivide would merge, as many as it leaves
+        # behind when every leaf is flagged and their geometry that of the merged boxes
+        u = o.boxes(254)
+        o.setFlags(0x08, -1)
+        o.unsubdivide(0x08)
+        b2, d2, _ = o.download()
+        merged = {(int(b) >> 0, int(dd)) for b, dd in zip(b2, d2)}
+        assert len(u) <= len(b2)
+        for row in u:
+            hit = False
+            for b, dd in merged:
+                lo, hi = bisect_box(b, dd, c, r)
+                if np.allclose(row[:dim], 0.5 * (lo + hi), atol=1e-12 * np.abs(r).max(), rtol=0) and \
+                        np.allclose(row[dim:2 * dim], 0.5 * (hi - lo), rtol=1e-12, atol=0):
+                    hit = True
+                    break
+            assert hit
+
+
 def hh_energy(x):
     q1, q2, p1, p2 = x[..., 0], x[..., 1], x[..., 2], x[..., 3]
     return 0.5 * (p1 ** 2 + p2 ** 2) + 0.5 * (q1 ** 2 + q2 ** 2) + q1 ** 2 * q2 - q2 ** 3 / 3.0
