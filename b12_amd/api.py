@@ -34,6 +34,12 @@ SIGNATURES = {
     "b12x_count": [_P, _U8, _P],
     "b12x_depth": [_P, _P],
     "b12x_boxes": [_P, _U8, _P, _P],
+    "b12x_tree_geometry": [_P, _P, _P, _P, _P, _P],
+    "b12x_tree_save_json": [_P, ctypes.c_char_p],
+    "b12x_tree_save_gaio": [_P, ctypes.c_char_p],
+    "b12x_tree_load": [ctypes.c_char_p, _I, _P],
+    "b12x_coo_save_json": [_P, ctypes.c_char_p],
+    "b12x_coo_load_json": [ctypes.c_char_p, _P],
     "b12x_enumerate": [_P, _P, _P],
     "b12x_tree_download": [_P, _P, _P, _P],
     "b12x_tree_upload": [_P, _U32, _P, _P, _P],
@@ -241,6 +247,18 @@ class CooMatrix:
         api("b12x_coo_upload", _ptr(rows), _ptr(cols), _ptr(vals), len(rows), n_rows, n_cols, ctypes.byref(h))
         return cls(h, api)
 
+    @classmethod
+    def load(cls, file_name, flavour="parity"):
+        """CooMatrix(fileName) (CooMatrix.h:75-76): a matrix saved by save()."""
+        api = _Api(flavour)
+        h = _P()
+        api("b12x_coo_load_json", os.fsencode(file_name), ctypes.byref(h))
+        return cls(h, api)
+
+    def save(self, file_name):
+        """save(fileName) (CooMatrix.h:110-112): B12's JSON file (".json" appended if missing)."""
+        self.api("b12x_coo_save_json", self.h, os.fsencode(file_name))
+
     def download(self):
         r = np.empty(self.nnz, np.uint32)
         c = np.empty(self.nnz, np.uint32)
@@ -297,6 +315,36 @@ class ImplicitBoxTree:
                  ctypes.byref(h))
         self.h = h
         self._schemes = {}
+
+    @classmethod
+    def load(cls, file_name, dim, flavour="parity"):
+        """ImplicitBoxTree(fileName) (ImplicitBoxTree.h:58-59): a tree saved by save(), or a
+        legacy binary file of B12a/B12b (a name without ".json")."""
+        obj = cls.__new__(cls)
+        obj.api = _Api(flavour)
+        obj.flavour = flavour
+        obj.dim = int(dim)
+        obj._schemes = {}
+        h = _P()
+        obj.api("b12x_tree_load", os.fsencode(file_name), obj.dim, ctypes.byref(h))
+        obj.h = h
+        return obj
+
+    def save(self, file_name):
+        """save(fileName) (ImplicitBoxTree.h:70-72): B12's JSON file (".json" appended if missing)."""
+        self.api("b12x_tree_save_json", self.h, os.fsencode(file_name))
+
+    def saveForGaio(self, file_name):
+        """saveForGaio(fileName) (ImplicitBoxTree.h:74-76): GAIO's binary box list."""
+        self.api("b12x_tree_save_gaio", self.h, os.fsencode(file_name))
+
+    def geometry(self):
+        """(centre, radius, sdScheme) of the tree."""
+        d = _I()
+        c, r = np.empty(4, np.float64), np.empty(4, np.float64)
+        sd, n = np.empty(65, np.uint8), _I()
+        self.api("b12x_tree_geometry", self.h, ctypes.byref(d), _ptr(c), _ptr(r), _ptr(sd), ctypes.byref(n))
+        return c[:d.value].copy(), r[:d.value].copy(), sd[:n.value].copy()
 
     def __del__(self):
         if getattr(self, "h", None):

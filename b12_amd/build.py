@@ -24,7 +24,8 @@ FLAVOURS = {
     "libb12x.so": ["-ffp-contract=off"],
     "libb12x_fast.so": ["-ffp-contract=fast", "-DB12X_FAST_BUILD=1"],
 }
-SOURCES = [os.path.join(HERE, "csrc", "b12x_capi.hip"), os.path.join(HERE, "csrc", "b12x_forest.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "b12x_capi.hip"), os.path.join(HERE, "csrc", "b12x_forest.hip"),
+           os.path.join(HERE, "csrc", "b12x_io.cpp")]
 
 
 def build(flavours=None, verbose=False):

@@ -40,6 +40,7 @@ struct b12x_scheme_s {
 
 struct b12x_tree_s {
   int dim = 0;
+  int sdEntries = 0;  // entries of the reference's sdScheme_ (64 default, 65 when given)
   Domain dom{};
   hipStream_t stream = nullptr;
   uint32_t n = 0;      // number of leaves
@@ -802,6 +803,7 @@ int b12x_tree_create(int dim, const double* center, const double* radius, const 
     if (int e = ensureDevice()) return e;
     auto* T = new b12x_tree_s();
     T->dim = dim;
+    T->sdEntries = sd ? kMaxDepth + 1 : kMaxDepth;
     if (!buildDomain(dim, center, radius, sd, nsd, T->dom)) {
       delete T;
       return setErr(B12X_E_ARG, "sdScheme has values >= dim");
@@ -822,6 +824,21 @@ int b12x_tree_create(int dim, const double* center, const double* radius, const 
       throw;
     }
     *out = T;
+    return B12X_OK;
+  });
+}
+
+int b12x_tree_geometry(b12x_tree t, int* dim, double* center, double* radius, uint8_t* sd, int* nsd) {
+  return guarded([&]() -> int {
+    if (!t) return setErr(B12X_E_ARG, "null tree");
+    if (dim) *dim = t->dim;
+    for (int j = 0; j < t->dim; ++j) {
+      if (center) center[j] = t->dom.center[j];
+      if (radius) radius[j] = t->dom.radius[j];
+    }
+    if (sd)
+      for (int i = 0; i < t->sdEntries; ++i) sd[i] = t->dom.sdScheme[i];
+    if (nsd) *nsd = t->sdEntries;
     return B12X_OK;
   });
 }

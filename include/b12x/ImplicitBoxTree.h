@@ -154,6 +154,38 @@ struct CooMatrix {
   typename ThrustSystem<A>::template Vector<NrBoxes> rowIndices_, columnIndices_;
   typename ThrustSystem<A>::template Vector<T> values_;
   NrBoxes nRows_ = 0, nColumns_ = 0;
+
+  CooMatrix() = default;
+
+  // CooMatrix.h:75-76: the matrix saved in fileName (or fileName + ".json")
+  explicit CooMatrix(const std::string& fileName) {
+    b12x_coo h = nullptr;
+    if (!detail::check(b12x_coo_load_json(fileName.c_str(), &h), "CooMatrix")) return;
+    uint64_t nnz = 0;
+    uint32_t nr = 0, nc = 0;
+    int rm = 0;
+    b12x_coo_info(h, &nnz, &nr, &nc, &rm);
+    rowIndices_.resize(nnz);
+    columnIndices_.resize(nnz);
+    std::vector<double> v(nnz);
+    if (nnz) b12x_coo_download(h, rowIndices_.data(), columnIndices_.data(), v.data());
+    values_.assign(v.begin(), v.end());
+    nRows_ = nr;
+    nColumns_ = nc;
+    b12x_coo_destroy(h);
+  }
+
+  // CooMatrix.h:110-112: B12's JSON file (".json" appended when missing)
+  void save(const std::string& fileName) const {
+    b12x_coo h = nullptr;
+    std::vector<double> v(values_.begin(), values_.end());
+    if (!detail::check(b12x_coo_upload(rowIndices_.data(), columnIndices_.data(), v.data(), rowIndices_.size(),
+                                       nRows_, nColumns_, &h),
+                       "save"))
+      return;
+    detail::check(b12x_coo_save_json(h, fileName.c_str()), "save");
+    b12x_coo_destroy(h);
+  }
 };
 
 // compressIndexVector<A>(indices, dim) — CooMatrix.h:20-23
@@ -210,6 +242,21 @@ class ImplicitBoxTree {
   ImplicitBoxTree(const Vector<REAL2>& center, const Vector<REAL2>& radius, const Vector<UInt>& sdScheme) {
     std::vector<uint8_t> sd(sdScheme.begin(), sdScheme.end());
     create(center, radius, sd.data(), int(sd.size()));
+  }
+
+  // ImplicitBoxTree.h:58-59: the tree saved in fileName (a JSON file, fileName + ".json", or a
+  // legacy binary file); on failure a warning and the default tree, as the reference does
+  explicit ImplicitBoxTree(const std::string& fileName) {
+    if (!detail::check(b12x_tree_load(fileName.c_str(), DIM, &h_), "ImplicitBoxTree")) {
+      h_ = nullptr;
+      create(Vector<REAL>(DIM, REAL(0.5)), Vector<REAL>(DIM, REAL(0.5)), nullptr, 0);
+    }
+  }
+
+  // ImplicitBoxTree.h:70-76
+  void save(const std::string& fileName) const { detail::check(b12x_tree_save_json(h_, fileName.c_str()), "save"); }
+  void saveForGaio(const std::string& fileName) const {
+    detail::check(b12x_tree_save_gaio(h_, fileName.c_str()), "saveForGaio");
   }
 
   ImplicitBoxTree(const ImplicitBoxTree&) = delete;

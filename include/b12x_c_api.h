@@ -38,7 +38,8 @@ enum {
   B12X_E_HIP = -2,          /* HIP runtime error (message names the call) */
   B12X_E_NOMEM = -3,        /* device allocation failed */
   B12X_E_UNSUPPORTED = -4,  /* depth or map/grid combination outside this implementation */
-  B12X_E_NODEVICE = -5      /* no usable GPU */
+  B12X_E_NODEVICE = -5,     /* no usable GPU */
+  B12X_E_IO = -6            /* a file could not be opened, read or written (message says which) */
 };
 
 /* Flags.h:11-22 */
@@ -78,6 +79,11 @@ int b12x_device_name(char* buf, int len);
 int b12x_tree_create(int dim, const double* center, const double* radius, const uint8_t* sd_scheme, int n_sd,
                      b12x_tree* out);
 int b12x_tree_destroy(b12x_tree t);
+/* The tree's dimension, centre and radius (dim doubles each) and subdivision scheme (sdScheme_:
+ * *n_sd_out entries, 64 for the default scheme and 65 for a given one, as ImplicitBoxTree.hpp:
+ * 27-33,3514-3545 size it; sd_scheme holds at least 65): the getters behind
+ * ImplicitBoxTree::save (ImplicitBoxTree.hpp:460-486). Any output may be NULL. */
+int b12x_tree_geometry(b12x_tree t, int* dim, double* center, double* radius, uint8_t* sd_scheme, int* n_sd_out);
 /* size(), count(depth), depth() — ImplicitBoxTree.h:110-111,129-130,603 */
 int b12x_size(b12x_tree t, uint32_t* out);
 int b12x_count(b12x_tree t, uint8_t depth, uint32_t* out);
@@ -158,6 +164,26 @@ int b12x_scheme_points_per_box(b12x_scheme s, uint64_t* out);
 /* Apply the map to n host points (row-major n x dim) on the GPU — the functor protocol of
  * doc/B12_doc.tex:109-146, exposed for integrator checks. */
 int b12x_map_points(b12x_scheme s, const double* in, double* out, uint64_t n);
+
+/* --- files ---------------------------------------------------------------------------------- */
+/* ImplicitBoxTree::save(fileName) (ImplicitBoxTree.h:70-72, .hpp:435-524): B12's JSON layout
+ * (version 3.1, centre, radius, sdScheme, nLeaves, bitPatternVector, depthVector, flagsVector,
+ * nHostBufferBytes); ".json" is appended when the name lacks it. */
+int b12x_tree_save_json(b12x_tree t, const char* file_name);
+/* ImplicitBoxTree::saveForGaio(fileName) (ImplicitBoxTree.h:74-76, .hpp:526-569): binary file
+ * for GAIO's readForGaio.m (dim, sizeof(double), centre, radius, 500*dim+1 scheme entries as
+ * int, then per leaf its centre, depth and flags). */
+int b12x_tree_save_gaio(b12x_tree t, const char* file_name);
+/* ImplicitBoxTree(fileName) (ImplicitBoxTree.h:58-59, .hpp:142-339): a JSON file (or the name
+ * with ".json" appended when the name itself cannot be opened), or a legacy binary file of B12a
+ * or B12b when the name does not end in ".json". Equal consecutive boxes are merged with their
+ * flags ANDed, as the reference's loader does. dim is the tree's dimension (a template argument
+ * in the reference). */
+int b12x_tree_load(const char* file_name, int dim, b12x_tree* out);
+/* CooMatrix::save(fileName) and CooMatrix(fileName) (CooMatrix.h:75-76,110-112,
+ * CooMatrix.hpp:235-284,473-534): nRows, nColumns, nnz, rowIndices, columnIndices, values. */
+int b12x_coo_save_json(b12x_coo m, const char* file_name);
+int b12x_coo_load_json(const char* file_name, b12x_coo* out);
 
 /* --- transition matrix -------------------------------------------------------------------- */
 /* transitionMatrixForLeaves/ForRoot<double>(grid, map, useAbsoluteValues, useRowMajor,

@@ -1,7 +1,10 @@
 // B12's documented C++ example (doc/B12_doc.tex:237-294: 20 subdivision steps of the Hénon map)
 // compiled against the b12x façade: only the include and the architecture tag differ. Prints the
-// leaf count per step, then the transition matrix size and the leading measure entries.
+// leaf count per step, then the transition matrix size and the leading measure entries. With a
+// directory as second argument it also saves the tree and the matrix there and reads them back
+// (an addition to the documented example).
 #include <cstdio>
+#include <string>
 #include <iostream>
 
 #include "b12x/ImplicitBoxTree.h"
@@ -34,5 +37,15 @@ int main(int argc, char** argv) {
   double resid = 0;
   std::vector<double> mu = invariantMeasure(P, 50, -1.0, &iters, &resid);
   std::printf("mu0 %.17g iters %d\n", mu.empty() ? 0.0 : mu[0], iters);
+  if (argc > 2) {  // files: ImplicitBoxTree::save and ImplicitBoxTree(fileName), the same for CooMatrix
+    const std::string dir = argv[2];
+    ibt.save(dir + "/tree");
+    ImplicitBoxTree<A, DIM, double, 64> loaded(dir + "/tree");
+    P.save(dir + "/matrix");
+    CooMatrix<A, double> Q(dir + "/matrix");
+    std::cout << "reloaded " << loaded.count(-1) << " " << Q.values_.size() << " "
+              << (Q.rowIndices_ == P.rowIndices_ && Q.columnIndices_ == P.columnIndices_ && Q.values_ == P.values_)
+              << std::endl;
+  }
   return 0;
 }

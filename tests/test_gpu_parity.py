@@ -367,14 +367,15 @@ def test_persistent_power_iteration_agrees_with_batched_form(monkeypatch):
                 assert np.allclose(a, b, rtol=1e-10, atol=1e-16)
 
 
-def test_cpp_facade_doc_example():
+def test_cpp_facade_doc_example(tmp_path):
     """B12's documented C++ example built against include/b12x/ImplicitBoxTree.h gives the
-    oracle's leaf counts per step."""
+    oracle's leaf counts per step; the tree and the matrix go through their files and back."""
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "doc_example")
     assert os.path.exists(exe), "tests/cpp/doc_example not built (run __graft_entry__.build())"
-    out = subprocess.run([exe, "12"], check=True, capture_output=True, text=True, timeout=120).stdout.splitlines()
+    out = subprocess.run([exe, "12", str(tmp_path)], check=True, capture_output=True, text=True,
+                         timeout=120).stdout.splitlines()
     o = OracleTree([0.0, 0.0], [3.0, 3.0])
     o.setFlags(gx.SD, 0)
     counts = o.subdivision_steps(gx.InnerGrid(10), gx.Henon(1.4, 0.3, 1), 12)
@@ -384,6 +385,8 @@ def test_cpp_facade_doc_example():
     assert out[12] == f"nnz {m.nnz} rows {m.nRows} cols {m.nColumns}"
     mu, _, _ = m.power_iteration(50)
     assert abs(float(out[13].split()[1]) - mu[0]) <= 1e-12 * abs(mu[0])
+    assert out[14] == f"reloaded {int(counts[-1][3])} {m.nnz} 1"
+    assert os.path.exists(tmp_path / "tree.json") and os.path.exists(tmp_path / "matrix.json")
 
 
 @pytest.mark.parametrize("grid,use_invalid,row_major", [(gx.InnerGrid(10), False, False), (gx.InnerGrid(9), True, True),
