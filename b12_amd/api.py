@@ -35,6 +35,9 @@ SIGNATURES = {
     "b12x_depth": [_P, _P],
     "b12x_boxes": [_P, _U8, _P, _P],
     "b12x_tree_geometry": [_P, _P, _P, _P, _P, _P],
+    "b12x_delete_depth": [_P, _U8, _P],
+    "b12x_insert_by_image": [_P, _P, _U8, _U8, _U8, _U8, _U8, _P],
+    "b12x_insert_by_points": [_P, _P, _U64, _U8, _U8, _U8, _P],
     "b12x_tree_save_json": [_P, ctypes.c_char_p],
     "b12x_tree_save_gaio": [_P, ctypes.c_char_p],
     "b12x_tree_load": [ctypes.c_char_p, _I, _P],
@@ -493,6 +496,26 @@ class ImplicitBoxTree:
                  ctypes.byref(n))
         return out[: n.value].copy()
 
+    def deleteDepth(self, depth):
+        """deleteDepth(depth) (ImplicitBoxTree.h:124-126)."""
+        return self._ret("b12x_delete_depth", _depth(depth))
+
+    def insertByIterators(self, it, depth, flags_for_inserted=INS, flags_for_hit=NONE):
+        """insertByIterators(points, depth, flagsForInserted, flagsForHit) (ImplicitBoxTree.h:170-187):
+        isToInsert per point."""
+        res = np.zeros(self.num_points(it), np.uint8)
+        self.api("b12x_insert_by_image", self.h, it.scheme.h, _depth(it.depth), it.flags_to_check, _depth(depth),
+                 flags_for_inserted, flags_for_hit, _ptr(res))
+        return res
+
+    def insertByPoints(self, points, depth, flags_for_inserted=INS, flags_for_hit=NONE):
+        """insertByPoints(points, depth, flagsForInserted, flagsForHit) (ImplicitBoxTree.h:189-202)."""
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, self.dim)
+        res = np.zeros(p.shape[0], np.uint8)
+        self.api("b12x_insert_by_points", self.h, _ptr(p), p.shape[0], _depth(depth), flags_for_inserted,
+                 flags_for_hit, _ptr(res))
+        return res
+
     def searchByPoints(self, points, depth=-1):
         p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, self.dim)
         out = np.empty(p.shape[0], np.uint32)
@@ -522,6 +545,35 @@ class ImplicitBoxTree:
 
     def transitionMatrixForRoot(self, grid, mp, use_abs=False, row_major=False, use_invalid=False):
         return self.transitionMatrix(self.makePointIteratorPairOverRoot(grid, mp), 0, use_abs, row_major, use_invalid)
+
+
+def continuation(ibt, grid, mp, depths, points, on_step=None):
+    """The continuation algorithm for unstable manifolds (B12c/src/matlab/continuation.m:1-63).
+
+    For each depth d: reset the tree to the surrounding box (deleteDepth(0)), insert the boxes of
+    depth d holding `points` with INS, then, while count(d) grows, turn INS into EXPD on depth d,
+    insert the boxes of depth d hit by the images of the EXPD boxes with INS, and unset EXPD on
+    the leaves. `points` is an (n, dim) array. Returns, per depth, the list of count(d) values:
+    the first after the initial insertion, then one per expansion step (the last equals the one
+    before it). `on_step(d, step, count)` is called after every expansion step. Any tree object
+    with the same method names works, so the oracle's tree runs the same driver."""
+    out = []
+    for d in depths:
+        ibt.deleteDepth(0)
+        ibt.insertByPoints(points, d, INS)
+        nb0, nb1 = 0, ibt.count(d)
+        it = ibt.makePointIteratorPairByFlags(d, EXPD, grid, mp)
+        counts = [nb1]
+        while nb1 > nb0:
+            ibt.changeFlags(INS, EXPD, d)
+            ibt.insertByIterators(it, d, INS, NONE)
+            ibt.unsetFlags(EXPD, LEAF)
+            nb0, nb1 = nb1, ibt.count(d)
+            counts.append(nb1)
+            if on_step is not None:
+                on_step(d, len(counts) - 1, nb1)
+        out.append(counts)
+    return out
 
 
 class Forest:

@@ -464,7 +464,7 @@ std::mutex g_tuneMutex;
 std::unordered_map<uint64_t, int> g_tuned;
 
 int fixedImageVariant(const ImageArgs& a) {
-  if (a.samples || a.depthBox) return 0;
+  if (a.samples || a.depthBox || a.outKey) return 0;
   const char* t = std::getenv("B12X_IMAGE_TUNE");
   const char* p = std::getenv("B12X_IMAGE_PPL");
   const char* b = std::getenv("B12X_IMAGE_BLOCK");
@@ -634,8 +634,9 @@ void stageLeafCount(b12x_tree_s& T) {
 // Image of every point of `src` located on the leaves (or on the root when depth0).
 // mode 0 sets hit bytes, mode 1 writes targets to out.
 void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, int mode, uint32_t* out,
-              const DepthMap* dm = nullptr) {
+              const DepthMap* dm = nullptr, uint64_t* outKey = nullptr) {
   ImageArgs a{};
+  a.outKey = outKey;
   if (dm) {  // search among the boxes at depth dm->d
     a.depthBox = dm->box;
     a.depthIn = dm->in;
@@ -1001,6 +1002,263 @@ int b12x_remove(b12x_tree t, uint8_t f, uint32_t* ret) {
     T.n = m;
     T.cur = 1 - T.cur;
     insertRootIfEmpty(T);
+    return B12X_OK;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+__global__ void iotaKernel(uint32_t* p, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = uint32_t(i);
+}
+
+// Insertion (ImplicitBoxTree.hpp:1613-1878) of the P point keys pk (maximal depth, or invalid)
+// whose search results at depth d are sv, after the depth map M of d (general depths) was built:
+// marks the points to insert and the hit boxes, sets flagsForHit, then merges the new boxes into
+// the leaves (sort, runs on one path reduced, neighbours on one path collapsed). resHost (may be
+// null) receives isToInsert per point.
+void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* sv, uint8_t d, DepthClass dc,
+                const DepthMap* M, uint8_t fIns, uint8_t fHit, uint8_t* resHost) {
+  const int form = d == kLeafDepth ? 0 : (dc == DC_BEYOND ? 2 : 1);
+  const uint32_t n = T.n;
+  uint8_t* ins = dalloc<uint8_t>(P);
+  uint32_t* insU32 = dalloc<uint32_t>(P + 1);
+  hipLaunchKernelGGL(insertMarkKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, pk, sv, P, form,
+                     (const uint8_t*)T.depth[T.cur], ins, insU32, T.hit, (uint32_t*)nullptr);
+  HIPCHK(hipGetLastError());
+  // flagsForHit on the hit boxes; the hit bytes end zero
+  if (form != 2) {
+    if (form == 0 || dc == DC_LEAF) {
+      zeroScal(T);
+      hipLaunchKernelGGL(flagsKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur], n, T.hit, 1,
+                         0, fHit, uint8_t(0), T.scal + 0);
+    } else {
+      (void)depthFlagsOp(T, *M, T.hit, 0, fHit, 0);
+      hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(std::max<uint32_t>(M->count, 1), kBlock)), dim3(kBlock), 0,
+                         T.stream, T.hit, uint64_t(M->count), uint8_t(0));
+    }
+    HIPCHK(hipGetLastError());
+  }
+  // the points may outnumber the leaves: the scan over them has scratch of its own
+  uint32_t* ptTotals = dalloc<uint32_t>(uint64_t(ceilDiv(P, kTile)) + 16);
+  const uint32_t m = scanArray(T.stream, insU32, P, ptTotals, T.scal + 4, T.hscal + 4);
+  dfree(ptTotals);
+  if (resHost) HIPCHK(hipMemcpyAsync(resHost, ins, P, hipMemcpyDeviceToHost, T.stream));
+  if (m == 0) {
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(ins);
+    dfree(insU32);
+    return;
+  }
+  // the leaves and the new boxes, in one array
+  const uint64_t N = uint64_t(n) + m;
+  ensureCap(T, N);
+  uint64_t* allK = dalloc<uint64_t>(N);
+  uint8_t* allD = dalloc<uint8_t>(N);
+  uint8_t* allF = dalloc<uint8_t>(N);
+  HIPCHK(hipMemcpyAsync(allK, T.key[T.cur], size_t(n) * 8, hipMemcpyDeviceToDevice, T.stream));
+  HIPCHK(hipMemcpyAsync(allD, T.depth[T.cur], n, hipMemcpyDeviceToDevice, T.stream));
+  HIPCHK(hipMemcpyAsync(allF, T.flags[T.cur], n, hipMemcpyDeviceToDevice, T.stream));
+  hipLaunchKernelGGL(insertNewKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, pk,
+                     (const uint8_t*)ins, (const uint32_t*)insU32, P, form == 0 ? 1 : 0, d,
+                     (const uint64_t*)T.key[T.cur], (const uint8_t*)T.depth[T.cur], n, fIns, allK + n, allD + n,
+                     allF + n);
+  HIPCHK(hipGetLastError());
+  // order: comparable key ascending, deeper first on equal keys (three stable LSD sorts)
+  uint32_t* idx = dalloc<uint32_t>(N);
+  uint32_t* k32 = dalloc<uint32_t>(N);
+  uint32_t* dummy = dalloc<uint32_t>(N);
+  hipLaunchKernelGGL(iotaKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, idx, N);
+  const int partBits[3] = {8, 32, 32};
+  for (int part = 0; part < 3; ++part) {
+    hipLaunchKernelGGL(boxSortKeyKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)allK,
+                       (const uint8_t*)allD, (const uint32_t*)idx, N, part, k32);
+    HIPCHK(hipGetLastError());
+    radixSort3(T.stream, k32, idx, dummy, N, partBits[part], T.hscal + 5);
+  }
+  uint64_t* sK = dalloc<uint64_t>(N);
+  uint8_t* sD = dalloc<uint8_t>(N);
+  uint8_t* sF = dalloc<uint8_t>(N);
+  hipLaunchKernelGGL(gatherBoxesKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)allK,
+                     (const uint8_t*)allD, (const uint8_t*)allF, (const uint32_t*)idx, N, sK, sD, sF);
+  // runs on one path, reduced
+  uint32_t* run = dalloc<uint32_t>(N);
+  uint8_t* runStart = dalloc<uint8_t>(N);
+  hipLaunchKernelGGL(pathRunKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)sK,
+                     (const uint8_t*)sD, uint32_t(N), run, runStart);
+  HIPCHK(hipGetLastError());
+  const uint32_t R = scanArray(T.stream, run, N, T.totals, T.scal + 6, T.hscal + 6);
+  hipLaunchKernelGGL(depthBoxKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, run,
+                     (const uint8_t*)runStart, uint32_t(N));
+  unsigned long long* rb = dalloc<unsigned long long>(R);
+  uint32_t* rd = dalloc<uint32_t>(R);
+  uint32_t* rf = dalloc<uint32_t>(R);
+  HIPCHK(hipMemsetAsync(rb, 0, size_t(R) * 8, T.stream));
+  HIPCHK(hipMemsetAsync(rd, 0, size_t(R) * 4, T.stream));
+  HIPCHK(hipMemsetAsync(rf, 0, size_t(R) * 4, T.stream));
+  hipLaunchKernelGGL(pathReduceKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)sK,
+                     (const uint8_t*)sD, (const uint8_t*)sF, (const uint32_t*)run, uint32_t(N), rb, rd, rf);
+  uint32_t* keep = dalloc<uint32_t>(R);
+  hipLaunchKernelGGL(pathKeepKernel, dim3(ceilDiv(R, kBlock)), dim3(kBlock), 0, T.stream,
+                     (const unsigned long long*)rb, (const uint32_t*)rd, R, keep);
+  HIPCHK(hipGetLastError());
+  const uint32_t K = scanArray(T.stream, keep, R, T.totals, T.scal + 7, T.hscal + 7);
+  hipLaunchKernelGGL(pathWriteKernel, dim3(ceilDiv(R, kBlock)), dim3(kBlock), 0, T.stream,
+                     (const unsigned long long*)rb, (const uint32_t*)rd, (const uint32_t*)rf, (const uint32_t*)keep,
+                     R, T.view(1 - T.cur));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(T.stream));
+  T.cur = 1 - T.cur;
+  T.n = K;
+  for (void* q : {(void*)ins, (void*)insU32, (void*)allK, (void*)allD, (void*)allF, (void*)idx, (void*)k32,
+                  (void*)dummy, (void*)sK, (void*)sD, (void*)sF, (void*)run, (void*)runStart, (void*)rb, (void*)rd,
+                  (void*)rf, (void*)keep})
+    dfree(q);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ImplicitBoxTree.hpp:1215-1262 deleteDepth
+int b12x_delete_depth(b12x_tree t, uint8_t d, uint32_t* ret) {
+  return guarded([&]() -> int {
+    if (!t || !ret) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    if (d == 0) {
+      T.n = 0;
+      insertRootIfEmpty(T);
+      *ret = 1;
+      return B12X_OK;
+    }
+    const DepthClass dc = classifyDepth(T, d);
+    if (d == kLeafDepth || d == kUnsubDepth || dc == DC_BEYOND) {
+      *ret = 0;
+      return B12X_OK;
+    }
+    uint32_t nb = T.n;  // count(d)
+    if (dc == DC_GENERAL) {
+      DepthMap M;
+      buildDepthMap(T, d, M);
+      nb = M.count;
+    }
+    const uint32_t n = T.n;
+    const uint8_t d1 = uint8_t(d - 1);
+    uint32_t* run = dalloc<uint32_t>(n);
+    uint8_t* runStart = dalloc<uint8_t>(n);
+    hipLaunchKernelGGL(truncRunKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.view(T.cur), n, d1, run,
+                       runStart);
+    HIPCHK(hipGetLastError());
+    const uint32_t m = scanArray(T.stream, run, n, T.totals, T.scal + 4, T.hscal + 4);
+    hipLaunchKernelGGL(depthBoxKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, run,
+                       (const uint8_t*)runStart, n);
+    uint32_t* acc = dalloc<uint32_t>(std::max<uint32_t>(m, 1));
+    hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(std::max<uint32_t>(m, 1), kBlock)), dim3(kBlock), 0, T.stream, acc,
+                       uint64_t(std::max<uint32_t>(m, 1)), 0xFFu);
+    hipLaunchKernelGGL(depthCombineKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream,
+                       (const uint8_t*)T.flags[T.cur], n, (const uint32_t*)run, (const uint8_t*)nullptr, 0, acc);
+    hipLaunchKernelGGL(truncWriteKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.view(T.cur),
+                       T.view(1 - T.cur), n, d1, (const uint8_t*)runStart, (const uint32_t*)run,
+                       (const uint32_t*)acc);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(run);
+    dfree(runStart);
+    dfree(acc);
+    T.cur = 1 - T.cur;
+    T.n = m;
+    *ret = nb;
+    return B12X_OK;
+  });
+}
+
+// ImplicitBoxTree.hpp:1613-1892 insertByIterators(points of (dIt, check, grid, map), depth,
+// flagsForInserted, flagsForHit)
+int b12x_insert_by_image(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t check, uint8_t d, uint8_t fIns,
+                         uint8_t fHit, uint8_t* res) {
+  return guarded([&]() -> int {
+    if (!t) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    checkScheme(T, s);
+    const Source src = makeSource(T, dIt, check);
+    const uint64_t P = src.nBoxes * s->grid.k;
+    if (P == 0) return B12X_OK;
+    if (P > 0xFFFFFFFFull) return setErr(B12X_E_UNSUPPORTED, "more than 2^32 points in one insertion");
+    if (d != 0 && d != kLeafDepth && d > kMaxDepth) {  // :1872-1876
+      if (res) std::memset(res, 0, P);
+      return B12X_OK;
+    }
+    const DepthClass dc = d == 0 ? DC_ROOT : classifyDepth(T, d);
+    DepthMap M;
+    if (dc == DC_GENERAL) buildDepthMap(T, d, M);
+    uint64_t* pk = dalloc<uint64_t>(P);
+    uint32_t* sv = dalloc<uint32_t>(P);
+    // the images: point keys and their search at d (leaf search when d is deeper than the tree)
+    runImage(T, *s, src, false, 1, sv, dc == DC_GENERAL ? &M : nullptr, pk);
+    if (d == 0) {  // :1621-1636 nothing inserted; flagsForHit on every leaf if a point is valid
+      std::vector<uint64_t> h(P);
+      HIPCHK(hipMemcpyAsync(h.data(), pk, P * 8, hipMemcpyDeviceToHost, T.stream));
+      HIPCHK(hipStreamSynchronize(T.stream));
+      bool any = false;
+      for (uint64_t v : h) any = any || v != kInvalidKey;
+      if (any && fHit) stencilOp(T, 0, nullptr, fHit, 0, kLeafDepth);
+      if (res) std::memset(res, 0, P);
+    } else {
+      insertCore(T, P, pk, sv, d, dc, dc == DC_GENERAL ? &M : nullptr, fIns, fHit, res);
+    }
+    dfree(pk);
+    dfree(sv);
+    return B12X_OK;
+  });
+}
+
+// ImplicitBoxTree.hpp:1894-1930 insertByPoints (the point boxes carry flagsForInserted)
+int b12x_insert_by_points(b12x_tree t, const double* pts, uint64_t P, uint8_t d, uint8_t fIns, uint8_t fHit,
+                          uint8_t* res) {
+  return guarded([&]() -> int {
+    if (!t || (P && !pts)) return setErr(B12X_E_ARG, "null argument");
+    b12x_tree_s& T = *t;
+    if (P == 0) return B12X_OK;
+    if (P > 0xFFFFFFFFull) return setErr(B12X_E_UNSUPPORTED, "more than 2^32 points in one insertion");
+    if (d != 0 && d != kLeafDepth && d > kMaxDepth) {
+      if (res) std::memset(res, 0, P);
+      return B12X_OK;
+    }
+    const DepthClass dc = d == 0 ? DC_ROOT : classifyDepth(T, d);
+    DepthMap M;
+    if (dc == DC_GENERAL) buildDepthMap(T, d, M);
+    double* dp = dalloc<double>(P * T.dim);
+    uint64_t* pk = dalloc<uint64_t>(P);
+    uint32_t* sv = dalloc<uint32_t>(P);
+    HIPCHK(hipMemcpyAsync(dp, pts, P * T.dim * sizeof(double), hipMemcpyHostToDevice, T.stream));
+    const uint32_t nb = ceilDiv(P, kBlock);
+    const uint32_t* mb = dc == DC_GENERAL ? M.box : nullptr;
+    const uint8_t* mi = dc == DC_GENERAL ? M.in : nullptr;
+    const uint8_t sd = dc == DC_GENERAL ? d : 0;
+    switch (T.dim) {
+      case 1: hipLaunchKernelGGL(pointsSearchKernel<1>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, P, T.key[T.cur], T.depth[T.cur], T.n, 0, sv, mb, mi, sd, pk); break;
+      case 2: hipLaunchKernelGGL(pointsSearchKernel<2>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, P, T.key[T.cur], T.depth[T.cur], T.n, 0, sv, mb, mi, sd, pk); break;
+      case 3: hipLaunchKernelGGL(pointsSearchKernel<3>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, P, T.key[T.cur], T.depth[T.cur], T.n, 0, sv, mb, mi, sd, pk); break;
+      default: hipLaunchKernelGGL(pointsSearchKernel<4>, dim3(nb), dim3(kBlock), 0, T.stream, T.dom, dp, P, T.key[T.cur], T.depth[T.cur], T.n, 0, sv, mb, mi, sd, pk); break;
+    }
+    HIPCHK(hipGetLastError());
+    if (d == 0) {
+      std::vector<uint64_t> h(P);
+      HIPCHK(hipMemcpyAsync(h.data(), pk, P * 8, hipMemcpyDeviceToHost, T.stream));
+      HIPCHK(hipStreamSynchronize(T.stream));
+      bool any = false;
+      for (uint64_t v : h) any = any || v != kInvalidKey;
+      if (any && fHit) stencilOp(T, 0, nullptr, fHit, 0, kLeafDepth);
+      if (res) std::memset(res, 0, P);
+    } else {
+      insertCore(T, P, pk, sv, d, dc, dc == DC_GENERAL ? &M : nullptr, fIns, fHit, res);
+    }
+    dfree(dp);
+    dfree(pk);
+    dfree(sv);
     return B12X_OK;
   });
 }

@@ -425,6 +425,7 @@ struct ImageArgs {
   const uint32_t* depthBox;  // box index of every leaf
   const uint8_t* depthIn;    // leaf belongs to a box at that depth
   uint8_t searchDepth;
+  uint64_t* outKey;          // when set, the key of every image point (insertion)
 };
 
 // samples[j] = keys[pos(j)], j < min(kDescentSamples, n); n from nDev when given
@@ -457,6 +458,7 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
   else if (a.depthBox) t = locateInDepth(a.key, a.depth, n, pk, a.searchDepth, a.depthIn, a.depthBox);
   else if (S) t = locateStaged(a.key, a.depth, n, pk, sm, S);
   else t = locate(a.key, a.depth, n, pk);
+  if (a.outKey) a.outKey[i] = pk;
   if (a.mode == 0) {
     if (t != kInvalid) a.hit[t] = 1;
   } else {
@@ -568,13 +570,15 @@ template <int DIM>
 static __global__ __launch_bounds__(kBlock) void pointsSearchKernel(Domain dom, const double* pts, uint64_t m,
                                                              const uint64_t* key, const uint8_t* depth, uint32_t n,
                                                              int depth0, uint32_t* out, const uint32_t* depthBox,
-                                                             const uint8_t* depthIn, uint8_t d) {
+                                                             const uint8_t* depthIn, uint8_t d,
+                                                             uint64_t* outKey = nullptr) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= m) return;
   double p[DIM];
 #pragma unroll
   for (int j = 0; j < DIM; ++j) p[j] = pts[i * DIM + j];
   const uint64_t pk = pointKey<DIM>(dom, p);
+  if (outKey) outKey[i] = pk;
   if (depth0) out[i] = (pk == kInvalidKey) ? kInvalid : 0u;
   else if (depthBox) out[i] = locateInDepth(key, depth, n, pk, d, depthIn, depthBox);
   else out[i] = locate(key, depth, n, pk);
@@ -615,7 +619,7 @@ static __global__ __launch_bounds__(kBlock) void depthCombineKernel(const uint8_
                                                                    const uint32_t* box, const uint8_t* in, int useOr,
                                                                    uint32_t* acc) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n || !in[i]) return;
+  if (i >= n || (in && !in[i])) return;
   if (useOr) atomicOr(acc + box[i], uint32_t(flags[i]));
   else atomicAnd(acc + box[i], uint32_t(flags[i]));
 }
@@ -645,6 +649,185 @@ static __global__ __launch_bounds__(kBlock) void depthCheckKernel(const uint8_t*
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n || !in[i]) return;
   if ((flags[i] & check) != check) atomicAnd(ok + box[i], 0u);
+}
+
+// ---------------------------------------------------------------------------------------------
+// deleteDepth (ImplicitBoxTree.hpp:1215-1262) and insertion (ImplicitBoxTree.hpp:1613-1878)
+// ---------------------------------------------------------------------------------------------
+// box (bits, depth) of a comparable key
+__device__ inline uint64_t bitsAt(uint64_t key, int depth) { return bitsOf(key, depth); }
+
+// hasSamePath (Box.hpp:74-83) of two valid boxes: one is an ancestor of (or equal to) the other
+__device__ inline bool samePath(uint64_t b1, int d1, uint64_t b2, int d2) {
+  const int diff = d1 - d2;
+  const uint64_t x = diff > 0 ? (diff >= 64 ? 0ull : b1 >> diff) : b1;
+  const uint64_t y = diff < 0 ? (-diff >= 64 ? 0ull : b2 >> -diff) : b2;
+  return x == y;
+}
+
+// computeDepthOfMutualAncestor (Box.hpp:206-220)
+__device__ inline int mutualAncestorDepth(uint64_t b1, int d1, uint64_t b2, int d2) {
+  const uint64_t x1 = (64 - d1) >= 64 ? 0ull : b1 << (64 - d1);
+  const uint64_t x2 = (64 - d2) >= 64 ? 0ull : b2 << (64 - d2);
+  const uint64_t x = x1 ^ x2;
+  const int lz = x ? __clzll((long long)x) : 64;
+  return min(lz, min(d1, d2));
+}
+
+// deleteDepth: leaf i truncated to d1 = depth - 1 starts a run when it differs from leaf i - 1
+static __global__ __launch_bounds__(kBlock) void truncRunKernel(TreeView v, uint32_t n, uint8_t d1, uint32_t* start,
+                                                               uint8_t* startFlag) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint64_t b, bp;
+  int td, tdp;
+  truncateTo(v.key[i], v.depth[i], d1, b, td);
+  const uint32_t ip = i > 0 ? i - 1 : 0;
+  truncateTo(v.key[ip], v.depth[ip], d1, bp, tdp);
+  const bool st = i == 0 || !(b == bp && td == tdp);
+  start[i] = st ? 1u : 0u;
+  startFlag[i] = st ? 1 : 0;
+}
+
+// the first leaf of each run writes the truncated box with the AND of the run's flags
+static __global__ __launch_bounds__(kBlock) void truncWriteKernel(TreeView in, TreeView out, uint32_t n, uint8_t d1,
+                                                                 const uint8_t* startFlag, const uint32_t* run,
+                                                                 const uint32_t* acc) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || !startFlag[i]) return;
+  uint64_t b;
+  int td;
+  truncateTo(in.key[i], in.depth[i], d1, b, td);
+  const uint32_t r = run[i];
+  out.key[r] = keyOf(b, td);
+  out.depth[r] = uint8_t(td);
+  out.flags[r] = uint8_t(acc[r]);
+}
+
+// insertion: per point, whether it is inserted (ins) and which existing box it hits (hit[t] = 1)
+// form 0: leaf depth (a point held by a leaf shallower than 64 is inserted too), 1: a depth with
+// boxes (inserted if no box holds it), 2: deeper than every leaf (every valid point)
+static __global__ __launch_bounds__(kBlock) void insertMarkKernel(const uint64_t* pk, const uint32_t* sv, uint64_t P,
+                                                                 int form, const uint8_t* leafDepth, uint8_t* ins,
+                                                                 uint32_t* insU32, uint8_t* hit, uint32_t* anyValid) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= P) return;
+  const bool valid = pk[i] != kInvalidKey;
+  const uint32_t t = form == 2 ? kInvalid : sv[i];
+  bool in;
+  if (form == 0) in = t == kInvalid ? valid : leafDepth[t] < kMaxDepth;
+  else in = t == kInvalid && valid;
+  ins[i] = in ? 1 : 0;
+  insU32[i] = in ? 1u : 0u;
+  if (t != kInvalid && !(form == 0 && in)) hit[t] = 1;
+  if (valid && anyValid) atomicOr(anyValid, 1u);
+}
+
+// the new boxes, compacted: at leaf depth one level below the deepest mutual ancestor with the
+// leaves on either side of the lower bound (AdaptDepthOfFirstBoxFunctor, Functors.hpp:379-389),
+// otherwise truncated to d
+static __global__ __launch_bounds__(kBlock) void insertNewKernel(const uint64_t* pk, const uint8_t* ins,
+                                                                const uint32_t* pos, uint64_t P, int leafForm,
+                                                                uint8_t d, const uint64_t* key, const uint8_t* depth,
+                                                                uint32_t n, uint8_t flags, uint64_t* outKey,
+                                                                uint8_t* outDepth, uint8_t* outFlags) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= P || !ins[i]) return;
+  const uint64_t p = pk[i];  // the point's box at depth 64: its comparable key is its pattern
+  int nd;
+  if (leafForm) {
+    uint32_t lo = 0, len = n;
+    while (len > 0) {
+      const uint32_t half = len >> 1;
+      if (key[lo + half] < p) {
+        lo += half + 1;
+        len -= half + 1;
+      } else {
+        len = half;
+      }
+    }
+    const uint32_t i1 = lo > 0 ? lo - 1 : 0, i2 = lo < n - 1 ? lo : n - 1;
+    const int m1 = mutualAncestorDepth(p, kMaxDepth, bitsAt(key[i1], depth[i1]), depth[i1]);
+    const int m2 = mutualAncestorDepth(p, kMaxDepth, bitsAt(key[i2], depth[i2]), depth[i2]);
+    nd = min(max(m1, m2) + 1, int(kMaxDepth));
+  } else {
+    nd = d;
+  }
+  const uint64_t b = nd >= kMaxDepth ? p : (p >> (kMaxDepth - nd));
+  const uint32_t o = pos[i];
+  outKey[o] = keyOf(b, nd);
+  outDepth[o] = uint8_t(nd);
+  outFlags[o] = flags;
+}
+
+// sort keys of the boxes (comparable key ascending, depth descending) by part: 0 = 255 - depth,
+// 1 = low 32 bits of the key, 2 = high 32 bits, each gathered through idx
+static __global__ __launch_bounds__(kBlock) void boxSortKeyKernel(const uint64_t* key, const uint8_t* depth,
+                                                                 const uint32_t* idx, uint64_t n, int part,
+                                                                 uint32_t* out) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = idx[i];
+  out[i] = part == 0 ? uint32_t(255 - depth[j]) : (part == 1 ? uint32_t(key[j]) : uint32_t(key[j] >> 32));
+}
+
+static __global__ __launch_bounds__(kBlock) void gatherBoxesKernel(const uint64_t* key, const uint8_t* depth,
+                                                                  const uint8_t* flags, const uint32_t* idx,
+                                                                  uint64_t n, uint64_t* ok, uint8_t* od,
+                                                                  uint8_t* of) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = idx[i];
+  ok[i] = key[j];
+  od[i] = depth[j];
+  of[i] = flags[j];
+}
+
+// runs of consecutive boxes on one path (hasSamePath)
+static __global__ __launch_bounds__(kBlock) void pathRunKernel(const uint64_t* key, const uint8_t* depth, uint32_t n,
+                                                              uint32_t* start, uint8_t* startFlag) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  bool st = i == 0;
+  if (!st) st = !samePath(bitsAt(key[i - 1], depth[i - 1]), depth[i - 1], bitsAt(key[i], depth[i]), depth[i]);
+  start[i] = st ? 1u : 0u;
+  startFlag[i] = st ? 1 : 0;
+}
+
+// each run's box: the largest pattern, the largest depth, the OR of the flags
+// (ComputeMaxBitPatternMaxDepthOrFlagsFunctor, Functors.hpp:391-401)
+static __global__ __launch_bounds__(kBlock) void pathReduceKernel(const uint64_t* key, const uint8_t* depth,
+                                                                 const uint8_t* flags, const uint32_t* run,
+                                                                 uint32_t n, unsigned long long* rb, uint32_t* rd,
+                                                                 uint32_t* rf) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = run[i];
+  atomicMax(rb + r, (unsigned long long)bitsAt(key[i], depth[i]));
+  atomicMax(rd + r, uint32_t(depth[i]));
+  atomicOr(rf + r, uint32_t(flags[i]));
+}
+
+// a run's box is kept unless it is on one path with the previous run's box (unique, comparing
+// neighbours)
+static __global__ __launch_bounds__(kBlock) void pathKeepKernel(const unsigned long long* rb, const uint32_t* rd,
+                                                               uint32_t m, uint32_t* keep) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  keep[t] = (t == 0 || !samePath(rb[t - 1], int(rd[t - 1]), rb[t], int(rd[t]))) ? 1u : 0u;
+}
+
+static __global__ __launch_bounds__(kBlock) void pathWriteKernel(const unsigned long long* rb, const uint32_t* rd,
+                                                                const uint32_t* rf, const uint32_t* pos, uint32_t m,
+                                                                TreeView out) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= m) return;
+  const bool kp = t == 0 || !samePath(rb[t - 1], int(rd[t - 1]), rb[t], int(rd[t]));
+  if (!kp) return;
+  const uint32_t o = pos[t];
+  out.key[o] = keyOf(rb[t], int(rd[t]));
+  out.depth[o] = uint8_t(rd[t]);
+  out.flags[o] = uint8_t(rf[t]);
 }
 
 // boxes(depth) rows {centre[DIM], radius[DIM], flags} (ImplicitBoxTree.hpp:611-775,

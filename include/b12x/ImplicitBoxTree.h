@@ -327,6 +327,33 @@ class ImplicitBoxTree {
     return ret([&](uint32_t* r) { return b12x_unsubdivide(h_, flags, r); }, "unsubdivide");
   }
   NrBoxes remove(Flags flags) { return ret([&](uint32_t* r) { return b12x_remove(h_, flags, r); }, "remove"); }
+  // deleteDepth(depth) — ImplicitBoxTree.h:124-127
+  NrBoxes deleteDepth(Depth depth) {
+    return ret([&](uint32_t* r) { return b12x_delete_depth(h_, depth, r); }, "deleteDepth");
+  }
+
+  // ---- insertion (ImplicitBoxTree.h:170-202); the result holds, per point, whether a box was
+  // created for it
+  Vector<bool> insertByIterators(const PointIteratorPair& points, Depth depth, Flags flagsForInserted = Flag::INS,
+                                 Flags flagsForHit = Flag::NONE) {
+    uint64_t n = 0;
+    if (!detail::check(b12x_num_points(h_, points.scheme->h, points.depth, points.flagsToCheck, &n),
+                       "insertByIterators"))
+      return {};
+    std::vector<uint8_t> r(n);
+    detail::check(b12x_insert_by_image(h_, points.scheme->h, points.depth, points.flagsToCheck, depth,
+                                       flagsForInserted, flagsForHit, r.data()),
+                  "insertByIterators");
+    return Vector<bool>(r.begin(), r.end());
+  }
+  Vector<bool> insertByPoints(const Vector<REAL>& points, Depth depth, Flags flagsForInserted = Flag::INS,
+                              Flags flagsForHit = Flag::NONE) {
+    std::vector<double> p(points.begin(), points.end());
+    std::vector<uint8_t> r(p.size() / DIM);
+    detail::check(b12x_insert_by_points(h_, p.data(), r.size(), depth, flagsForInserted, flagsForHit, r.data()),
+                  "insertByPoints");
+    return Vector<bool>(r.begin(), r.end());
+  }
 
   // ---- flags (ImplicitBoxTree.h:93-108, 461-485, 527-541)
   NrBoxes setFlags(Flags flags, Depth depth = getLeafDepth()) {

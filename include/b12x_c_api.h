@@ -165,6 +165,21 @@ int b12x_scheme_points_per_box(b12x_scheme s, uint64_t* out);
  * doc/B12_doc.tex:109-146, exposed for integrator checks. */
 int b12x_map_points(b12x_scheme s, const double* in, double* out, uint64_t n);
 
+/* --- insertion --------------------------------------------------------------------------- */
+/* ImplicitBoxTree::deleteDepth(depth) (ImplicitBoxTree.h:124-126, .hpp:1215-1262): every leaf
+ * truncated to depth-1 with equal boxes merged (flags ANDed); depth 0 leaves the surrounding box
+ * alone. *ret = count(depth) before (1 for depth 0; 0 for the leaf or unsubdividable depth or a
+ * depth below every leaf). */
+int b12x_delete_depth(b12x_tree t, uint8_t depth, uint32_t* ret);
+/* ImplicitBoxTree::insertByIterators(points, depth, flagsForInserted, flagsForHit[, res])
+ * (ImplicitBoxTree.h:170-187, .hpp:1613-1892) for the iterator of (depth_it, flags_to_check,
+ * scheme), and insertByPoints (ImplicitBoxTree.h:189-202, .hpp:1894-1930) for n host points of
+ * dim doubles each. res (may be NULL) receives isToInsert per point. */
+int b12x_insert_by_image(b12x_tree t, b12x_scheme s, uint8_t depth_it, uint8_t flags_to_check, uint8_t depth,
+                         uint8_t flags_for_inserted, uint8_t flags_for_hit, uint8_t* res);
+int b12x_insert_by_points(b12x_tree t, const double* points, uint64_t n, uint8_t depth, uint8_t flags_for_inserted,
+                          uint8_t flags_for_hit, uint8_t* res);
+
 /* --- files ---------------------------------------------------------------------------------- */
 /* ImplicitBoxTree::save(fileName) (ImplicitBoxTree.h:70-72, .hpp:435-524): B12's JSON layout
  * (version 3.1, centre, radius, sdScheme, nLeaves, bitPatternVector, depthVector, flagsVector,

@@ -89,6 +89,31 @@ inline bool isStrictlyPreceding(const Box& a, const Box& b) {
   return !isInvalid(a) && (isInvalid(b) || ka < kb || (ka == kb && b.depth < a.depth));
 }
 
+// Box.hpp:206-220 computeDepthOfMutualAncestor: leading equal bits of the two boxes' patterns
+// at maximal depth, at most the smaller depth
+inline Depth mutualAncestorDepth(const Box& a, const Box& b) {
+  const uint64_t x = shl(a.bits, int(kMaxDepth) - int(a.depth)) ^ shl(b.bits, int(kMaxDepth) - int(b.depth));
+  const int lz = x ? __builtin_clzll(x) : 64;
+  return Depth(std::min<int>(lz, std::min(a.depth, b.depth)));
+}
+
+// Box.hpp:178-198 changeDepthTo (leaf depth: unchanged; deeper: padded with zero bits)
+inline Box changeDepthTo(const Box& b, Depth d) {
+  Box r = b;
+  if (d == kUnsubDepth) {
+    r.bits = shr(b.bits, 1);
+    r.depth = Depth(b.depth - (b.depth > 0 ? 1 : 0));
+  } else if (d < b.depth) {
+    r.bits = shr(b.bits, int(b.depth) - int(d));
+    r.depth = d;
+  } else if (d > b.depth && d != kLeafDepth) {
+    d = std::min<Depth>(d, kMaxDepth);
+    r.bits = shl(b.bits, int(d) - int(b.depth));
+    r.depth = d;
+  }
+  return r;
+}
+
 // Box.hpp:138-147 incrementDepthBy
 inline Box incrementDepthBy(const Box& b, Depth d) {
   d = std::min<Depth>(d, Depth(kMaxDepth - b.depth));
@@ -830,6 +855,142 @@ class Tree {
     NrBoxes res = NrBoxes(n - w);
     insertSurroundingBoxIfEmpty();
     return res;
+  }
+
+  // ImplicitBoxTree.hpp:1215-1262 deleteDepth: every leaf truncated to depth d-1, equal boxes
+  // merged into their first with the AND of their flags (d = 0: the surrounding box alone);
+  // returns count(d) before (1 for d = 0, 0 for the leaf, unsubdividable or too deep a depth)
+  NrBoxes deleteDepth(Depth d) {
+    if (d == 0) {
+      resize(0);
+      insertSurroundingBoxIfEmpty();
+      return 1;
+    }
+    if (d == kLeafDepth || d == kUnsubDepth || d > depth()) return 0;
+    const NrBoxes nb = count(d);
+    std::vector<Box> out;
+    for (size_t i = 0; i < bits.size(); ++i) {
+      const Box b = decrementDepthTo(box(i), Depth(d - 1));
+      if (!out.empty() && isSameBox(out.back(), b)) out.back().flags = Flags(out.back().flags & b.flags);
+      else out.push_back(b);
+    }
+    resize(out.size());
+    for (size_t i = 0; i < out.size(); ++i) setBox(i, out[i]);
+    return nb;
+  }
+
+  // ImplicitBoxTree.hpp:1613-1878 insertByIterators on the point boxes pts (maximal depth, or
+  // invalid): returns isToInsert per point.
+  //  - depth 0: nothing inserted; flagsForHit set on every leaf if some point is valid.
+  //  - leaf depth: a point is inserted if no leaf holds it (and it is valid) or the leaf that
+  //    holds it is shallower than the maximal depth; every other hit leaf gets flagsForHit. An
+  //    inserted box takes the depth one below its deepest mutual ancestor with the leaves on
+  //    either side of its lower bound (AdaptDepthOfFirstBoxFunctor, Functors.hpp:379-389).
+  //  - a depth d <= depth(): a valid point is inserted if no box at d holds it; hit boxes at d get
+  //    flagsForHit; inserted boxes are truncated to d.
+  //  - a depth below depth(): every valid point is inserted at d.
+  // The new boxes (flagsForInserted set, duplicates dropped) join the leaves; after sorting, each
+  // maximal run of consecutive boxes on one path (hasSamePath) becomes one box with the largest
+  // pattern and depth of the run and the OR of its flags (inclusive_scan_by_key forwards and
+  // backwards), and consecutive boxes on one path then collapse into the first (unique,
+  // comparing neighbours).
+  std::vector<uint8_t> insertBoxes(const std::vector<Box>& pts, Depth d, Flags fIns, Flags fHit) {
+    const size_t P = pts.size();
+    std::vector<uint8_t> res(P, 0);
+    if (d == 0) {
+      bool anyValid = false;
+      for (const Box& p : pts) anyValid = anyValid || !isInvalid(p);
+      if (fHit != NONE && anyValid)
+        for (auto& f : flagsV) f = setF(f, fHit);
+      return res;
+    }
+    if (!(d == kLeafDepth || d <= kMaxDepth)) return res;
+    const bool doSearch = d == kLeafDepth || d <= depth();
+    std::vector<NrBoxes> sv(P, kInvalid);
+    if (doSearch) {
+      std::vector<NrBoxes> idx;
+      if (isGeneralDepth(d)) idx = indicesInDepth(d);
+      for (size_t i = 0; i < P; ++i) sv[i] = searchOne(pts[i], d, &idx);
+    }
+    if (d == kLeafDepth) {
+      std::vector<NrBoxes> hits;
+      for (size_t i = 0; i < P; ++i) {
+        res[i] = sv[i] == kInvalid ? !isInvalid(pts[i]) : (depthV[sv[i]] < kMaxDepth);
+        hits.push_back(res[i] ? kInvalid : sv[i]);
+      }
+      std::sort(hits.begin(), hits.end());
+      hits.erase(std::unique(hits.begin(), hits.end()), hits.end());
+      setFlagsBySearch(hits, fHit, kLeafDepth);
+    } else if (d <= depth()) {
+      for (size_t i = 0; i < P; ++i) res[i] = sv[i] == kInvalid && !isInvalid(pts[i]);
+      std::vector<NrBoxes> hits(sv);
+      std::sort(hits.begin(), hits.end());
+      hits.erase(std::unique(hits.begin(), hits.end()), hits.end());
+      setFlagsBySearch(hits, fHit, d);
+    } else {
+      for (size_t i = 0; i < P; ++i) res[i] = !isInvalid(pts[i]);
+    }
+    const size_t n = bits.size();
+    std::vector<Box> add;
+    for (size_t i = 0; i < P; ++i) {
+      if (!res[i]) continue;
+      Box b = pts[i];
+      if (d == kLeafDepth) {
+        const size_t lb = lowerBound(b);
+        const Box prev = box(lb > 0 ? lb - 1 : 0), next = box(std::min(lb, n - 1));
+        b = changeDepthTo(b, Depth(std::max(mutualAncestorDepth(b, prev), mutualAncestorDepth(b, next)) + 1));
+      } else {
+        b = decrementDepthTo(b, d);
+      }
+      add.push_back(b);
+    }
+    std::stable_sort(add.begin(), add.end(), isStrictlyPreceding);
+    add.erase(std::unique(add.begin(), add.end(), isSameBox), add.end());
+    for (Box& b : add) b.flags = setF(b.flags, fIns);
+    for (const Box& b : add) {
+      bits.push_back(b.bits);
+      depthV.push_back(b.depth);
+      flagsV.push_back(b.flags);
+    }
+    sortBoxes();
+    mergeSamePath();
+    return res;
+  }
+
+  void mergeSamePath() {
+    const size_t n = bits.size();
+    if (n == 0) return;
+    std::vector<Box> rep;
+    std::vector<size_t> seg(n);
+    for (size_t i = 0; i < n; ++i) {
+      const Box b = box(i);
+      if (i == 0 || !hasSamePath(box(i - 1), b)) {
+        rep.push_back(b);
+      } else {
+        Box& r = rep.back();
+        r.bits = std::max(r.bits, b.bits);
+        r.depth = std::max(r.depth, b.depth);
+        r.flags = Flags(r.flags | b.flags);
+      }
+      seg[i] = rep.size() - 1;
+    }
+    std::vector<Box> out;
+    for (size_t t = 0; t < rep.size(); ++t)
+      if (t == 0 || !hasSamePath(rep[t - 1], rep[t])) out.push_back(rep[t]);
+    resize(out.size());
+    for (size_t i = 0; i < out.size(); ++i) setBox(i, out[i]);
+  }
+
+  // ImplicitBoxTree.hpp:1880-1892 / 1894-1918 insertByIterators over a point set, insertByPoints
+  std::vector<uint8_t> insertByIterators(const PointSet<DIM>& ps, Depth d, Flags fIns, Flags fHit) {
+    std::vector<Box> pts(ps.size());
+    for (uint64_t i = 0; i < ps.size(); ++i) pts[i] = pointBox(ps, i);
+    return insertBoxes(pts, d, fIns, fHit);
+  }
+  std::vector<uint8_t> insertByPoints(const double* p, uint64_t n, Depth d, Flags fIns, Flags fHit) {
+    std::vector<Box> pts(n);
+    for (uint64_t i = 0; i < n; ++i) pts[i] = geo.boxFromPoint(p + i * DIM, fIns);
+    return insertBoxes(pts, d, fIns, fHit);
   }
 
   // helpFunctions.hpp:40-98 sortBoxes (ordering = isStrictlyPreceding)

@@ -318,6 +318,32 @@ int b12cpu_point_boxes(void* h, void* sh, uint8_t dIt, uint8_t check, uint64_t* 
   });
 }
 
+// ImplicitBoxTree.hpp:1215-1262 deleteDepth
+int b12cpu_delete_depth(void* h, uint8_t d, uint32_t* ret) { TREE_CALL(h, { *ret = T.deleteDepth(d); return 0; }); }
+
+// ImplicitBoxTree.hpp:1613-1892 insertByIterators(points, depth, flagsForInserted, flagsForHit):
+// res (may be null) receives isToInsert per point
+int b12cpu_insert_by_image(void* h, void* sh, uint8_t dIt, uint8_t check, uint8_t d, uint8_t fIns, uint8_t fHit,
+                           uint8_t* res) {
+  TREE_CALL(h, {
+    auto* s = static_cast<Scheme<D>*>(sh);
+    auto ps = T.makePoints(s->grid, s->map, dIt, check);
+    auto v = T.insertByIterators(ps, d, fIns, fHit);
+    if (res && !v.empty()) std::memcpy(res, v.data(), v.size());
+    return 0;
+  });
+}
+
+// ImplicitBoxTree.hpp:1894-1930 insertByPoints
+int b12cpu_insert_by_points(void* h, const double* pts, uint64_t n, uint8_t d, uint8_t fIns, uint8_t fHit,
+                            uint8_t* res) {
+  TREE_CALL(h, {
+    auto v = T.insertByPoints(pts, n, d, fIns, fHit);
+    if (res && !v.empty()) std::memcpy(res, v.data(), v.size());
+    return 0;
+  });
+}
+
 int b12cpu_search_by_points(void* h, const double* pts, uint64_t n, uint8_t d, uint32_t* out) {
   TREE_CALL(h, {
     auto v = T.searchPoints(pts, n, d);

@@ -284,6 +284,22 @@ class OracleTree:
                                            _ptr(out), ctypes.byref(n)))
         return out[: n.value].copy()
 
+    def deleteDepth(self, depth):
+        return self._ret("b12cpu_delete_depth", _u8(depth))
+
+    def insertByIterators(self, it, depth, flags_for_inserted=2, flags_for_hit=0):
+        res = np.zeros(self.num_points(it), np.uint8)
+        _chk(lib().b12cpu_insert_by_image(self.h, it.scheme.h, _u8(it.depth), _u8(it.flags_to_check), _u8(depth),
+                                          _u8(flags_for_inserted), _u8(flags_for_hit), _ptr(res)))
+        return res
+
+    def insertByPoints(self, points, depth, flags_for_inserted=2, flags_for_hit=0):
+        p = np.ascontiguousarray(points, np.float64).reshape(-1, self.dim)
+        res = np.zeros(p.shape[0], np.uint8)
+        _chk(lib().b12cpu_insert_by_points(self.h, _ptr(p), ctypes.c_uint64(p.shape[0]), _u8(depth),
+                                           _u8(flags_for_inserted), _u8(flags_for_hit), _ptr(res)))
+        return res
+
     def searchByPoints(self, points, depth=-1):
         p = np.ascontiguousarray(points, np.float64).reshape(-1, self.dim)
         out = np.empty(p.shape[0], np.uint32)

@@ -637,5 +637,179 @@ def test_library_exports_every_declared_symbol(flavour):
 
 
 def test_cpp_facade_builds():
-    exe = os.path.join(ROOT, "tests", "cpp", "doc_example")
-    assert os.path.exists(exe), "façade example not built"
+    for name in ("doc_example", "continuation"):
+        exe = os.path.join(ROOT, "tests", "cpp", name)
+        assert os.path.exists(exe), f"façade example {name} not built"
+
+
+# ------------------------------------------------------------------ insertion and deleteDepth
+def point_pattern(p, d, c, r):
+    """Pattern of the depth-d box holding p (default scheme), by bisection (independent of the
+    oracle); None outside the domain."""
+    dim = len(c)
+    lo, hi = np.array(c, float) - np.array(r, float), np.array(c, float) + np.array(r, float)
+    if np.any(p < lo) or np.any(p >= hi):
+        return None
+    a = 0
+    for lvl in range(d):
+        j = lvl % dim
+        mid = 0.5 * (lo[j] + hi[j])
+        bit = 1 if p[j] >= mid else 0
+        a = (a << 1) | bit
+        if bit:
+            lo[j] = mid
+        else:
+            hi[j] = mid
+    return a
+
+
+def test_delete_depth_against_truncation():
+    """deleteDepth(d): every leaf truncated to d-1, equal boxes merged with the AND of their
+    flags, count(d) returned; deleteDepth(0) leaves the surrounding box alone."""
+    rng = np.random.default_rng(81)
+    for trial in range(3):
+        o = general_tree(rng, 2, 500, 10)
+        bits, depth, flags = o.download()
+        for d in (3, 6, 9):
+            o.upload(bits, depth, flags)
+            exp = {}
+            order = []
+            for b, dd, f in zip(bits.tolist(), depth.tolist(), flags.tolist()):
+                key = (b >> (dd - (d - 1)), d - 1) if dd > d - 1 else (b, dd)
+                if key not in exp:
+                    exp[key] = 0xFF
+                    order.append(key)
+                exp[key] &= f
+            n_d = o.count(d)
+            assert o.deleteDepth(d) == n_d
+            b2, d2, f2 = o.download()
+            assert list(zip(b2.tolist(), d2.tolist())) == order
+            assert f2.tolist() == [exp[k] for k in order]
+    o.upload(bits, depth, flags)
+    assert o.deleteDepth(0) == 1
+    assert [a.tolist() for a in o.download()] == [[0], [0], [0]]
+    assert o.deleteDepth(254) == 0 and o.deleteDepth(255) == 0
+
+
+def test_insert_at_a_depth_of_equal_boxes():
+    """Leaves all at depth d: inserting points at d adds the boxes of the points no leaf holds
+    (isToInsert), with flagsForInserted; the leaves the points hit get flagsForHit."""
+    rng = np.random.default_rng(82)
+    c, r, d = [0.0, 0.0], [1.0, 1.0], 8
+    o = OracleTree(c, r)
+    pts0 = rng.uniform(-1, 1, (300, 2))
+    o.insertByPoints(pts0, d, 0x02)  # a first cover at depth d (the root's run takes one box)
+    bits, depth, flags = o.download()
+    assert np.all(depth == d)
+    have = set(bits.tolist())
+    pts = rng.uniform(-1.2, 1.2, (400, 2))
+    res = o.insertByPoints(pts, d, 0x20, 0x40)
+    b2, d2, f2 = o.download()
+    exp_new, hit = set(), set()
+    for i, p in enumerate(pts):
+        a = point_pattern(p, d, c, r)
+        if a is None:
+            assert res[i] == 0
+        elif a in have:
+            assert res[i] == 0
+            hit.add(a)
+        else:
+            assert res[i] == 1
+            exp_new.add(a)
+    assert set(b2.tolist()) == have | exp_new and np.all(d2 == d)
+    for b, f in zip(b2.tolist(), f2.tolist()):
+        old = dict(zip(bits.tolist(), flags.tolist())).get(b)
+        if b in exp_new:
+            assert f == 0x20
+        elif b in hit:
+            assert f == old | 0x40
+        else:
+            assert f == old
+
+
+def test_insert_into_the_surrounding_box_merges_its_run():
+    """The surrounding box sorts between the halves (its comparable key ends its lower half), so
+    after inserting boxes of both halves it joins the last lower box and the first upper box in
+    one run on a path, which becomes the box of largest pattern (ImplicitBoxTree.hpp:1860-1878):
+    the reference's result, kept as it is. Points of one half only keep every box."""
+    c, r, d = [0.0, 0.0], [1.0, 1.0], 4
+    o = OracleTree(c, r)
+    o.deleteDepth(0)
+    res = o.insertByPoints([[-0.5, 0.1], [0.3, 0.3], [0.31, 0.29]], d, 0x02)
+    assert res.tolist() == [1, 1, 1]
+    b, dd, f = o.download()
+    up = point_pattern(np.array([0.3, 0.3]), d, c, r)
+    assert b.tolist() == [up] and dd.tolist() == [d] and f.tolist() == [0x02]
+    o.deleteDepth(0)
+    low = [[-0.5, 0.1], [-0.9, -0.2], [-0.1, 0.7]]
+    o.insertByPoints(low, d, 0x02)
+    assert sorted(o.download()[0].tolist()) == sorted({point_pattern(np.array(p), d, c, r) for p in low})
+
+
+def test_insert_at_leaf_depth_into_holes():
+    """At leaf depth a point outside every leaf becomes a box one level below its deepest common
+    ancestor with the leaves on either side: it holds the point, overlaps no leaf, and its parent
+    overlaps a neighbour."""
+    rng = np.random.default_rng(83)
+    c, r = [0.0, 0.0], [1.0, 1.0]
+    o = OracleTree(c, r)
+    o.upload(*random_tree(rng, 2, 200, max_depth=9, flag_prob=0.0))
+    for _ in range(20):
+        bits, depth, _ = o.download()
+        p = rng.uniform(-1, 1, 2)
+        if o.searchByPoints(p[None, :])[0] != 0xFFFFFFFF:
+            continue
+        res = o.insertByPoints(p[None, :], 255, 0x02)
+        assert res[0] == 1
+        b2, d2, f2 = o.download()
+        new = [(b, dd) for b, dd in zip(b2.tolist(), d2.tolist()) if (b, dd) not in set(zip(bits.tolist(), depth.tolist()))]
+        assert len(new) == 1 and len(b2) == len(bits) + 1
+        nb, nd = new[0]
+        assert point_pattern(p, nd, c, r) == nb
+        def overlap(b1, d1, b2_, d2_):
+            k = min(d1, d2_)
+            return (b1 >> (d1 - k)) == (b2_ >> (d2_ - k))
+        assert not any(overlap(nb, nd, b, dd) for b, dd in zip(bits.tolist(), depth.tolist()))
+        assert any(overlap(nb >> 1, nd - 1, b, dd) for b, dd in zip(bits.tolist(), depth.tolist()))
+
+
+def henon_py(x, a=1.4, b=0.3):
+    return np.array([1.0 - a * x[0] * x[0] + x[1], b * x[0]])
+
+
+def sampled_closure(start, d, c, r, n_grid, f):
+    """Depth-d boxes reachable from the boxes `start` when every box sends the points of an
+    n_grid^dim inner grid (Grids.hpp:105-120) through f: breadth-first search over patterns,
+    independent of the oracle."""
+    dim = len(c)
+    g = np.array([[((i // n_grid ** j) % n_grid * 2.0 + 1.0 - n_grid) / n_grid for j in range(dim)]
+                  for i in range(n_grid ** dim)])
+    seen, todo = set(start), list(start)
+    while todo:
+        b = todo.pop()
+        lo, hi = bisect_box(b, d, c, r)
+        cen, rad = 0.5 * (lo + hi), 0.5 * (hi - lo)
+        for x in cen + rad * g:
+            a = point_pattern(f(x), d, c, r)
+            if a is not None and a not in seen:
+                seen.add(a)
+                todo.append(a)
+    return seen
+
+
+def test_continuation_is_the_sampled_closure_of_the_start_box():
+    """The continuation (continuation.m) from the Hénon map's fixed point ends with exactly the
+    depth-d boxes reachable from the fixed point's box under the sampled map, with no flags left;
+    the box counts grow until the last step repeats the one before it."""
+    a, b = 1.4, 0.3
+    x = (-(1 - b) + np.sqrt((1 - b) ** 2 + 4 * a)) / (2 * a)
+    p = np.array([[x, b * x]])
+    c, r = [0.0, 0.0], [3.0, 3.0]
+    o = OracleTree(c, r)
+    for d in (6, 10, 13):
+        counts = gx.continuation(o, gx.InnerGrid(4), gx.Henon(a, b, 1), [d], p)[0]
+        assert counts[0] == 1 and counts[-1] == counts[-2] and all(u < v for u, v in zip(counts[:-2], counts[1:-1]))
+        bits, depth, flags = o.download()
+        assert np.all(depth == d) and np.all(flags == 0)  # the last step turned INS into EXPD, then cleared it
+        exp = sampled_closure([point_pattern(p[0], d, c, r)], d, c, r, 4, henon_py)
+        assert set(bits.tolist()) == exp and len(bits) == len(exp) == counts[-1]
