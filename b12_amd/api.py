@@ -74,6 +74,8 @@ SIGNATURES = {
     "b12x_compress_index_vector": [_P, _U64, _U32, _P],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
+    "b12x_coo_scc": [_P, _P, _P, _P],
+    "b12x_chain_recurrent_steps": [_P, _P, _U8, _I, _P],
     "b12x_forest_create": [_I, _I, _P, _I, _I, _I, _I, _I, _D, _P],
     "b12x_forest_destroy": [_P],
     "b12x_forest_add": [_P, _P, _P, _P, _I, _U8, _P],
@@ -283,6 +285,14 @@ class CooMatrix:
         X = np.zeros((m, self.nRows), np.float64) if want_basis else None
         self.api("b12x_subspace_iteration", self.h, int(m), int(iters), _ptr(H), _ptr(X) if want_basis else None)
         return H, X
+
+    def scc(self):
+        """Strong components of the matrix's graph (b12x_coo_scc): per vertex the largest vertex
+        index of its component, the number of components and of those with more than one vertex."""
+        lab = np.empty(self.nRows, np.uint32)
+        nc, nb = ctypes.c_uint32(), ctypes.c_uint32()
+        self.api("b12x_coo_scc", self.h, _ptr(lab), ctypes.byref(nc), ctypes.byref(nb))
+        return lab, nc.value, nb.value
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -528,6 +538,15 @@ class ImplicitBoxTree:
         self.api("b12x_subdivision_steps", self.h, self.scheme(grid, mp).h, sd_flag, int(steps), _ptr(counts))
         return counts[: 4 * steps].reshape(steps, 4)
 
+    def chain_recurrent_steps(self, grid, mp, steps, sd_flag=SD):
+        """steps x {subdivide(SD); the leaves' matrix; strong components; HIT on the leaves of
+        components with more than one leaf; remove(HIT); unsetFlags(HIT)} on the device
+        (chainRecurrentSet.m:18-44). Per step: leaves after subdividing, components, components
+        of more than one leaf, leaves kept."""
+        counts = np.zeros(4 * max(steps, 1), np.uint32)
+        self.api("b12x_chain_recurrent_steps", self.h, self.scheme(grid, mp).h, sd_flag, int(steps), _ptr(counts))
+        return counts[: 4 * steps].reshape(steps, 4)
+
     # -- transition matrix
     def transitionMatrix(self, it, depth=-1, use_abs=False, row_major=False, use_invalid=False):
         h = _P()
@@ -573,6 +592,31 @@ def continuation(ibt, grid, mp, depths, points, on_step=None):
             if on_step is not None:
                 on_step(d, len(counts) - 1, nb1)
         out.append(counts)
+    return out
+
+
+def chainRecurrentSet(ibt, grid, mp, steps, on_step=None):
+    """The chain recurrent set by subdivision (B12c/src/matlab/chainRecurrentSet.m:1-64): SD on
+    every leaf, then per step subdivide(SD), the leaves' transition matrix, its strong
+    components, HIT on the leaves whose component holds more than one leaf, remove(HIT),
+    unsetFlags(HIT). Returns per step (leaves after subdividing, components, components of
+    more than one leaf, leaves kept). Any tree object with the same method names works, so the
+    oracle's tree runs the same driver; the GPU tree's chain_recurrent_steps runs the same steps
+    without leaving the device."""
+    out = []
+    ibt.setFlags(SD, LEAF)
+    for i in range(steps):
+        ibt.subdivide(SD)
+        m = ibt.transitionMatrixForLeaves(grid, mp)
+        lab, n_comp, n_big = m.scc()
+        size = np.bincount(lab, minlength=len(lab))
+        n = len(lab)
+        ibt.setFlagsByStencil(size[lab] > 1, HIT)
+        ibt.remove(HIT)
+        ibt.unsetFlags(HIT, LEAF)
+        out.append((n, n_comp, n_big, ibt.count(LEAF)))
+        if on_step is not None:
+            on_step(i, out[-1])
     return out
 
 

@@ -18,6 +18,7 @@
 #include "b12x_host.hpp"
 #include "b12x_kernels.hpp"
 #include "b12x_launch.hpp"
+#include "b12x_scc.hpp"
 
 using namespace b12x;
 
@@ -2163,6 +2164,66 @@ int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, doubl
       case 6: subspaceRun<6>(*m, n, iters, ritz, basis); break;
       case 7: subspaceRun<7>(*m, n, iters, ritz, basis); break;
       default: subspaceRun<8>(*m, n, iters, ritz, basis); break;
+    }
+    return B12X_OK;
+  });
+}
+
+// strong components of the matrix's graph (chainRecurrentSet.m:26-39, graphconncomp)
+int b12x_coo_scc(b12x_coo m, uint32_t* labels, uint32_t* nComp, uint32_t* nBig) {
+  return guarded([&]() -> int {
+    if (!m) return setErr(B12X_E_ARG, "null matrix");
+    const uint32_t n = m->nRows;
+    uint32_t* lab = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
+    const SccStats st = sccLabels(m->stream, n, m->nnz, m->rows, m->cols, lab, nullptr);
+    if (labels && n) {
+      HIPCHK(hipMemcpyAsync(labels, lab, size_t(n) * 4, hipMemcpyDeviceToHost, m->stream));
+      HIPCHK(hipStreamSynchronize(m->stream));
+    }
+    dfree(lab);
+    if (nComp) *nComp = st.components;
+    if (nBig) *nBig = st.nontrivial;
+    return B12X_OK;
+  });
+}
+
+// chainRecurrentSet.m:18-44: subdivide, the leaves' matrix, strong components, keep the leaves of
+// components with more than one leaf
+int b12x_chain_recurrent_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, uint32_t* counts) {
+  return guarded([&]() -> int {
+    if (!t || !s || nSteps < 0) return setErr(B12X_E_ARG, "bad argument");
+    b12x_tree_s& T = *t;
+    constexpr uint8_t kHitFlag = 0x01;
+    for (int i = 0; i < nSteps; ++i) {
+      uint32_t r = 0;
+      if (int e = b12x_subdivide(t, sd, &r)) return e;
+      b12x_coo M = nullptr;
+      if (int e = b12x_transition_matrix(t, s, kLeafDepth, 0, kLeafDepth, 0, 0, 0, &M)) return e;
+      const uint32_t n = T.n;
+      SccStats st;
+      uint32_t* lab = nullptr;
+      try {
+        lab = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
+        // the marks of the leaves to keep go to the hit bytes, which the flag kernel clears
+        st = sccLabels(T.stream, n, M->nnz, M->rows, M->cols, lab, T.hit);
+      } catch (...) {  // the hit bytes stay zero between operations
+        if (n) hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.hit, uint64_t(n),
+                                  uint8_t(0));
+        dfree(lab);
+        delete M;
+        throw;
+      }
+      dfree(lab);
+      delete M;
+      (void)applyFlags(T, T.hit, true, 0, kHitFlag, 0);
+      if (int e = b12x_remove(t, kHitFlag, &r)) return e;
+      if (int e = b12x_unset_flags(t, kHitFlag, kLeafDepth, &r)) return e;
+      if (counts) {
+        counts[4 * i + 0] = n;
+        counts[4 * i + 1] = st.components;
+        counts[4 * i + 2] = st.nontrivial;
+        counts[4 * i + 3] = T.n;
+      }
     }
     return B12X_OK;
   });

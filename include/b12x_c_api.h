@@ -225,6 +225,18 @@ int b12x_power_iteration(b12x_coo m, int max_iter, double tol, int mu_init, doub
  * iteration); ritz (nvec x nvec, row-major) = X^T (X P), whose eigenvalues approximate the
  * leading eigenvalues of P. basis (nvec x n, column by column) may be NULL. */
 int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, double* basis);
+/* Strong components of the matrix's graph (an edge row -> column per entry; entries naming a
+ * vertex >= n_rows, such as the column of points outside the cover, are left out), as
+ * chainRecurrentSet.m:26-39 obtains them from MATLAB's graphconncomp. labels (n_rows entries,
+ * may be NULL): the largest vertex index of the vertex's component. n_components and
+ * n_nontrivial (components of more than one vertex) may be NULL. */
+int b12x_coo_scc(b12x_coo m, uint32_t* labels, uint32_t* n_components, uint32_t* n_nontrivial);
+/* The loop body of chainRecurrentSet.m:18-44, n_steps times, on the device: subdivide(sd_flag);
+ * the leaves' transition matrix (transitionMatrix({grid, map, -1})); strong components; HIT on
+ * the leaves whose component has more than one leaf; remove(HIT); unsetFlags(HIT). counts (may
+ * be NULL) receives 4 values per step: the leaves after subdividing, the components, the
+ * components of more than one leaf, the leaves kept. */
+int b12x_chain_recurrent_steps(b12x_tree t, b12x_scheme s, uint8_t sd_flag, int n_steps, uint32_t* counts);
 
 /* --- batched and served forms (configs 4 and 5) --------------------------------------------
  * A forest runs the subdivision algorithm (subdivision.m:15-25) for many jobs at once: jobs of

@@ -1408,4 +1408,70 @@ inline bool subspaceIteration(const Coo& P, int m, int iters, std::vector<double
   return true;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Strongly connected components of the graph of a matrix (an edge row -> column per entry;
+// entries naming a vertex >= nRows, such as the column of points outside the cover, are left
+// out): chainRecurrentSet.m:26-39 labels them with MATLAB's graphconncomp. Tarjan's algorithm,
+// iterative. Each vertex gets the largest vertex index of its component as label (a canonical
+// labelling: the partition alone is what graphconncomp's labels carry).
+// ---------------------------------------------------------------------------------------------
+inline std::vector<NrBoxes> strongComponents(const Coo& P) {
+  const NrBoxes n = P.nRows;
+  std::vector<size_t> off(size_t(n) + 1, 0);
+  for (size_t e = 0; e < P.rows.size(); ++e)
+    if (P.rows[e] < n && P.cols[e] < n) ++off[P.rows[e] + 1];
+  for (NrBoxes v = 0; v < n; ++v) off[v + 1] += off[v];
+  std::vector<NrBoxes> adj(off[n]);
+  std::vector<size_t> fill(off.begin(), off.end() - 1);
+  for (size_t e = 0; e < P.rows.size(); ++e)
+    if (P.rows[e] < n && P.cols[e] < n) adj[fill[P.rows[e]]++] = P.cols[e];
+  const NrBoxes none = 0xFFFFFFFFu;
+  std::vector<NrBoxes> index(n, none), low(n, 0), label(n, none), stack;
+  std::vector<char> onStack(n, 0);
+  std::vector<std::pair<NrBoxes, size_t>> call;  // (vertex, next edge offset)
+  NrBoxes counter = 0;
+  for (NrBoxes r = 0; r < n; ++r) {
+    if (index[r] != none) continue;
+    call.push_back({r, off[r]});
+    index[r] = low[r] = counter++;
+    stack.push_back(r);
+    onStack[r] = 1;
+    while (!call.empty()) {
+      const NrBoxes v = call.back().first;
+      size_t& e = call.back().second;
+      if (e < off[v + 1]) {
+        const NrBoxes w = adj[e++];
+        if (index[w] == none) {
+          index[w] = low[w] = counter++;
+          stack.push_back(w);
+          onStack[w] = 1;
+          call.push_back({w, off[w]});
+        } else if (onStack[w]) {
+          low[v] = std::min(low[v], index[w]);
+        }
+        continue;
+      }
+      if (low[v] == index[v]) {  // v is the root of a component: pop it
+        size_t k = stack.size();
+        NrBoxes mx = 0;
+        do {
+          --k;
+          mx = std::max(mx, stack[k]);
+        } while (stack[k] != v);
+        for (size_t j = k; j < stack.size(); ++j) {
+          label[stack[j]] = mx;
+          onStack[stack[j]] = 0;
+        }
+        stack.resize(k);
+      }
+      call.pop_back();
+      if (!call.empty()) {
+        const NrBoxes u = call.back().first;
+        low[u] = std::min(low[u], low[v]);
+      }
+    }
+  }
+  return label;
+}
+
 }  // namespace b12cpu

@@ -429,4 +429,23 @@ int b12cpu_subspace_iteration(void* c, int m, int iters, double* H, double* X) {
   return 0;
 }
 
+// labels (nRows entries): the largest vertex index of the vertex's strong component; counts of
+// components and of components with more than one vertex
+int b12cpu_coo_scc(void* c, uint32_t* labels, uint32_t* nComp, uint32_t* nBig) {
+  const Coo& P = static_cast<CooH*>(c)->m;
+  const std::vector<NrBoxes> lab = strongComponents(P);
+  std::vector<uint32_t> size(P.nRows, 0);
+  for (NrBoxes l : lab) ++size[l];
+  uint32_t comps = 0, big = 0;
+  for (NrBoxes v = 0; v < P.nRows; ++v)
+    if (lab[v] == v) {
+      ++comps;
+      big += size[v] > 1 ? 1 : 0;
+    }
+  if (labels) std::memcpy(labels, lab.data(), lab.size() * 4);
+  if (nComp) *nComp = comps;
+  if (nBig) *nBig = big;
+  return 0;
+}
+
 }  // extern "C"

@@ -106,6 +106,14 @@ class OCoo:
         lib().b12cpu_coo_get(self.h, _ptr(r), _ptr(c), _ptr(v), ctypes.byref(nr), ctypes.byref(nc))
         return r, c, v
 
+    def scc(self):
+        """Strong components of the matrix's graph: per vertex the largest vertex index of its
+        component; also the number of components and of those with more than one vertex."""
+        lab = np.empty(self.nRows, np.uint32)
+        nc, nb = ctypes.c_uint32(), ctypes.c_uint32()
+        _chk(lib().b12cpu_coo_scc(self.h, _ptr(lab), ctypes.byref(nc), ctypes.byref(nb)))
+        return lab, nc.value, nb.value
+
     def power_iteration(self, max_iter=200, tol=-1.0, mu=None):
         m = np.zeros(self.nRows, np.float64) if mu is None else np.ascontiguousarray(mu, np.float64).copy()
         it, res = ctypes.c_int(), ctypes.c_double()

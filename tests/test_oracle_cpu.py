@@ -813,3 +813,71 @@ def test_continuation_is_the_sampled_closure_of_the_start_box():
         assert np.all(depth == d) and np.all(flags == 0)  # the last step turned INS into EXPD, then cleared it
         exp = sampled_closure([point_pattern(p[0], d, c, r)], d, c, r, 4, henon_py)
         assert set(bits.tolist()) == exp and len(bits) == len(exp) == counts[-1]
+
+
+# ------------------------------------------------------------------ strong components
+def canonical_labels(labels):
+    """Relabel a partition with the largest vertex index of each part."""
+    labels = np.asarray(labels)
+    mx = {}
+    for v, l in enumerate(labels.tolist()):
+        mx[l] = max(mx.get(l, -1), v)
+    return np.array([mx[l] for l in labels.tolist()], np.uint32)
+
+
+def scipy_scc(rows, cols, n):
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    keep = (rows < n) & (cols < n)
+    g = coo_matrix((np.ones(int(keep.sum())), (rows[keep], cols[keep])), shape=(n, n)).tocsr()
+    return canonical_labels(connected_components(g, directed=True, connection="strong")[1])
+
+
+def test_strong_components_closed_forms():
+    from oracle_api import OCoo
+    n = 7
+    cyc = OCoo.from_arrays(np.arange(n), (np.arange(n) + 1) % n, np.ones(n), n, n)
+    lab, nc, nb = cyc.scc()
+    assert lab.tolist() == [n - 1] * n and (nc, nb) == (1, 1)
+    path = OCoo.from_arrays(np.arange(n - 1), np.arange(1, n), np.ones(n - 1), n, n)  # a DAG
+    lab, nc, nb = path.scc()
+    assert lab.tolist() == list(range(n)) and (nc, nb) == (n, 0)
+    # two 2-cycles joined by one edge, a self loop, an isolated vertex and an entry outside the
+    # vertex range (the column of points outside the cover)
+    r = np.array([0, 1, 2, 3, 1, 4, 5])
+    c = np.array([1, 0, 3, 2, 2, 4, 6])
+    lab, nc, nb = OCoo.from_arrays(r, c, np.ones(7), 6, 7).scc()
+    assert lab.tolist() == [1, 1, 3, 3, 4, 5] and (nc, nb) == (4, 2)
+
+
+def test_strong_components_brute_force_and_scipy():
+    """Against reachability by repeated squaring on small graphs and scipy's strong components
+    (csgraph.connected_components) on larger ones, including a transition matrix's graph."""
+    from oracle_api import OCoo
+    rng = np.random.default_rng(1201)
+    for trial in range(30):
+        n = int(rng.integers(1, 25))
+        m = int(rng.integers(0, 3 * n))
+        r, c = rng.integers(0, n, m), rng.integers(0, n, m)
+        pairs = sorted(set(zip(r.tolist(), c.tolist())))
+        r = np.array([p[0] for p in pairs], np.uint32)
+        c = np.array([p[1] for p in pairs], np.uint32)
+        lab, _, _ = OCoo.from_arrays(r, c, np.ones(len(r)), n, n).scc()
+        R = np.eye(n, dtype=bool)
+        R[r, c] = True
+        for _ in range(6):
+            R = R | ((R.astype(np.int64) @ R.astype(np.int64)) > 0)
+        same = R & R.T
+        exp = np.array([max(np.nonzero(same[v])[0]) for v in range(n)], np.uint32)
+        assert np.array_equal(lab, exp)
+    for n, m in ((2000, 3000), (5000, 20000), (3000, 2500)):
+        r, c = rng.integers(0, n, m).astype(np.uint32), rng.integers(0, n, m).astype(np.uint32)
+        lab, _, _ = OCoo.from_arrays(r, c, np.ones(m), n, n).scc()
+        assert np.array_equal(lab, scipy_scc(r, c, n))
+    o = OracleTree([0.0, 0.0], [1.5, 0.5])
+    o.setFlags(gx.SD, 0)
+    o.subdivision_steps(gx.InnerGrid(4), gx.Henon(), 12)
+    m = o.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon())
+    r, c, _ = m.download()
+    lab, nc, nb = m.scc()
+    assert np.array_equal(lab, scipy_scc(r, c, m.nRows)) and nc == len(set(lab.tolist()))
