@@ -75,6 +75,7 @@ SIGNATURES = {
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
     "b12x_coo_scc": [_P, _P, _P, _P],
+    "b12x_scc_last_stats": [_P, _P, _P],
     "b12x_chain_recurrent_steps": [_P, _P, _U8, _I, _P],
     "b12x_forest_create": [_I, _I, _P, _I, _I, _I, _I, _I, _D, _P],
     "b12x_forest_destroy": [_P],
@@ -143,6 +144,13 @@ def bench_copy(nbytes, reps=20, flavour="parity"):
     ms = _D()
     _Api(flavour)("b12x_bench_copy", int(nbytes), int(reps), ctypes.byref(ms))
     return ms.value
+
+
+def scc_last_stats(flavour="parity"):
+    """(rounds, passes, host checks) of this thread's last strong-components run."""
+    v = [_U32(), _U32(), _U32()]
+    _Api(flavour)("b12x_scc_last_stats", *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def device_name(flavour="parity"):

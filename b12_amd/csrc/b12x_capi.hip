@@ -113,7 +113,8 @@ struct b12x_tree_s {
 struct b12x_coo_s {
   uint32_t nRows = 0, nCols = 0;
   uint64_t nnz = 0;
-  bool rowMajor = false;
+  bool rowMajor = false;  // entries sorted by (row, column)
+  bool colMajor = false;  // entries sorted by (column, row)
   uint32_t* rows = nullptr;
   uint32_t* cols = nullptr;
   double* vals = nullptr;
@@ -1725,6 +1726,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
       streamCreate(&M->stream);
       M->nRows = M->nCols = nBoxes;
       M->rowMajor = rowMajor != 0;
+      M->colMajor = rowMajor == 0;
       uint32_t* col = dalloc<uint32_t>(P);
       if (dc != DC_BEYOND) runImage(T, *s, src, dc == DC_ROOT, 1, col, dc == DC_GENERAL ? &DM : nullptr);
       else if (P)
@@ -1872,10 +1874,13 @@ int b12x_coo_upload(const uint32_t* rows, const uint32_t* cols, const double* va
         HIPCHK(hipMemcpy(M->cols, cols, nnz * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(M->vals, vals, nnz * 8, hipMemcpyHostToDevice));
       }
-      bool rm = true;
+      bool rm = true, cm = true;
       for (uint64_t e = 1; e < nnz && rm; ++e)
         rm = rows[e - 1] < rows[e] || (rows[e - 1] == rows[e] && cols[e - 1] < cols[e]);
+      for (uint64_t e = 1; e < nnz && cm; ++e)
+        cm = cols[e - 1] < cols[e] || (cols[e - 1] == cols[e] && rows[e - 1] < rows[e]);
       M->rowMajor = rm;
+      M->colMajor = cm;
     } catch (...) {
       delete M;
       throw;
@@ -1949,12 +1954,14 @@ Csc buildCsc(b12x_coo_s& M, uint32_t n) {
     HIPCHK(hipMemcpyAsync(C.keys, M.cols, nnz * 4, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(C.rows, M.rows, nnz * 4, hipMemcpyDeviceToDevice, st));
   }
-  if (!M.rowMajor) {
+  if (M.colMajor) {
     if (nnz) HIPCHK(hipMemcpyAsync(C.vals, M.vals, nnz * 8, hipMemcpyDeviceToDevice, st));
   } else if (nnz) {
-    // stable sort by column carrying (row, entry index); then gather the values
+    // stable sort by column carrying (row, entry index), after a stable sort by row when the
+    // entries are in no order; then gather the values
     uint32_t* idx = dalloc<uint32_t>(nnz);
     hipLaunchKernelGGL(sequenceKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, idx, nnz);
+    if (!M.rowMajor) radixSort3(st, C.rows, C.keys, idx, nnz, bitsFor(uint64_t(M.nRows) + 1), hT.data());
     radixSort3(st, C.keys, C.rows, idx, nnz, bitsFor(uint64_t(M.nCols) + 1), hT.data());
     hipLaunchKernelGGL(gatherF64Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, M.vals, idx, nnz, C.vals);
     HIPCHK(hipGetLastError());
@@ -2169,13 +2176,68 @@ int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, doubl
   });
 }
 
+namespace {
+thread_local SccStats lastSccStats;  // the calling thread's last strong-components run
+
+// In-lists of a matrix's graph for the strong components: the sources (rows) of the entries into
+// each column v < n, taken from the column-major order of the entries (sorted into that order
+// when the matrix holds another); entries of columns >= n fall past inPtr[n].
+struct InLists {
+  const uint32_t* src = nullptr;
+  uint32_t* inPtr = nullptr;
+  uint32_t* ownRows = nullptr;
+  uint32_t* ownKeys = nullptr;
+  InLists() = default;
+  InLists(const InLists&) = delete;
+  InLists& operator=(const InLists&) = delete;
+  ~InLists() {
+    dfree(inPtr);
+    dfree(ownRows);
+    dfree(ownKeys);
+  }
+};
+
+void buildInLists(const b12x_coo_s& M, uint32_t n, hipStream_t st, InLists& L) {
+  const uint64_t nnz = M.nnz;
+  const uint32_t* keys = M.cols;
+  L.src = M.rows;
+  if (!M.colMajor && nnz > 1) {
+    std::vector<uint32_t> hT(8);
+    L.ownKeys = dalloc<uint32_t>(nnz);
+    L.ownRows = dalloc<uint32_t>(nnz);
+    uint32_t* dummy = dalloc<uint32_t>(nnz);
+    HIPCHK(hipMemcpyAsync(L.ownKeys, M.cols, nnz * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(L.ownRows, M.rows, nnz * 4, hipMemcpyDeviceToDevice, st));
+    if (!M.rowMajor) radixSort3(st, L.ownRows, L.ownKeys, dummy, nnz, bitsFor(uint64_t(M.nRows) + 1), hT.data());
+    radixSort3(st, L.ownKeys, L.ownRows, dummy, nnz, bitsFor(uint64_t(M.nCols) + 1), hT.data());
+    dfree(dummy);
+    keys = L.ownKeys;
+    L.src = L.ownRows;
+  }
+  L.inPtr = dalloc<uint32_t>(uint64_t(n) + 1);
+  hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, st, keys, nnz, n,
+                     L.inPtr);
+  HIPCHK(hipGetLastError());
+}
+}  // namespace
+
+int b12x_scc_last_stats(uint32_t* rounds, uint32_t* passes, uint32_t* checks) {
+  if (rounds) *rounds = lastSccStats.rounds;
+  if (passes) *passes = lastSccStats.passes;
+  if (checks) *checks = lastSccStats.checks;
+  return B12X_OK;
+}
+
 // strong components of the matrix's graph (chainRecurrentSet.m:26-39, graphconncomp)
 int b12x_coo_scc(b12x_coo m, uint32_t* labels, uint32_t* nComp, uint32_t* nBig) {
   return guarded([&]() -> int {
     if (!m) return setErr(B12X_E_ARG, "null matrix");
     const uint32_t n = m->nRows;
     uint32_t* lab = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
-    const SccStats st = sccLabels(m->stream, n, m->nnz, m->rows, m->cols, lab, nullptr);
+    InLists L;
+    buildInLists(*m, n, m->stream, L);
+    const SccStats st = sccLabels(m->stream, n, L.inPtr, L.src, lab, nullptr);
+    lastSccStats = st;
     if (labels && n) {
       HIPCHK(hipMemcpyAsync(labels, lab, size_t(n) * 4, hipMemcpyDeviceToHost, m->stream));
       HIPCHK(hipStreamSynchronize(m->stream));
@@ -2205,7 +2267,10 @@ int b12x_chain_recurrent_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nStep
       try {
         lab = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
         // the marks of the leaves to keep go to the hit bytes, which the flag kernel clears
-        st = sccLabels(T.stream, n, M->nnz, M->rows, M->cols, lab, T.hit);
+        InLists L;  // the leaves' matrix is in column-major order
+        buildInLists(*M, n, T.stream, L);
+        st = sccLabels(T.stream, n, L.inPtr, L.src, lab, T.hit);
+        lastSccStats = st;
       } catch (...) {  // the hit bytes stay zero between operations
         if (n) hipLaunchKernelGGL(fillU8Kernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.hit, uint64_t(n),
                                   uint8_t(0));

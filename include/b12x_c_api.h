@@ -231,6 +231,10 @@ int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, doubl
  * may be NULL): the largest vertex index of the vertex's component. n_components and
  * n_nontrivial (components of more than one vertex) may be NULL. */
 int b12x_coo_scc(b12x_coo m, uint32_t* labels, uint32_t* n_components, uint32_t* n_nontrivial);
+/* Measurement (no counterpart in the reference): of the calling thread's last strong-components
+ * run (b12x_coo_scc, or the last step of b12x_chain_recurrent_steps), the colouring rounds, the
+ * edge and vertex passes launched and the host reads of the change counters. */
+int b12x_scc_last_stats(uint32_t* rounds, uint32_t* passes, uint32_t* checks);
 /* The loop body of chainRecurrentSet.m:18-44, n_steps times, on the device: subdivide(sd_flag);
  * the leaves' transition matrix (transitionMatrix({grid, map, -1})); strong components; HIT on
  * the leaves whose component has more than one leaf; remove(HIT); unsetFlags(HIT). counts (may

@@ -92,3 +92,34 @@ def test_chain_recurrent_steps_match_oracle(name, grid, mp, c, r, steps):
     h = gx.ImplicitBoxTree(c, r)
     assert gx.chainRecurrentSet(h, grid, mp, steps) == exp
     assert same_tree(h.download(), o.download())
+
+
+def test_scc_and_power_iteration_in_any_entry_order():
+    """Entries uploaded in column-major order, in no order, or with an entry repeated give the
+    same components; the power iteration of a shuffled upload equals that of the sorted matrix
+    bit for bit (its sums run over each column in ascending row order either way) and the
+    oracle's within 1e-12 relative (L1)."""
+    rng = np.random.default_rng(1401)
+    n, m = 20000, 60000
+    r, c = unique_edges(rng.integers(0, n, m), rng.integers(0, n, m))
+    lo, nco, nbo = OCoo.from_arrays(r, c, np.ones(len(r)), n, n).scc()
+    cm = np.lexsort((r, c))
+    sh = rng.permutation(len(r))
+    rep = np.concatenate([sh, sh[:500]])
+    for order in (cm, sh, rep):
+        print(f"[scc] order of {len(order)} entries", flush=True)
+        g = gx.CooMatrix.from_arrays(r[order], c[order], np.ones(len(order)), n, n)
+        lg, ncg, nbg = g.scc()
+        assert np.array_equal(lg, lo) and (ncg, nbg) == (nco, nbo)
+    t = gx.ImplicitBoxTree([0.0, 0.0], [1.5, 0.5])
+    t.setFlags(gx.SD, 0)
+    t.subdivision_steps(gx.InnerGrid(4), gx.Henon(), 14)
+    a = t.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), row_major=True)
+    rows, cols, vals = a.download()
+    ref = a.power_iteration(30)[0]
+    sh = rng.permutation(len(rows))
+    b = gx.CooMatrix.from_arrays(rows[sh], cols[sh], vals[sh], a.nRows, a.nColumns)
+    assert np.array_equal(b.power_iteration(30)[0], ref)
+    o = OCoo.from_arrays(rows[sh], cols[sh], vals[sh], a.nRows, a.nColumns)
+    mu_o = o.power_iteration(30)[0]
+    assert np.abs(mu_o - ref).sum() <= 1e-12 * np.abs(ref).sum()  # the measure's bar: 1e-12 relative
