@@ -245,6 +245,84 @@ def hh_bench(gx, depth, iters, cpu_depth, threads):
     return out
 
 
+MANIFOLD_C, MANIFOLD_R = [0.0, 0.0, 27.0], [30.0, 30.0, 40.0]
+MANIFOLD_START = [[0.01, 0.01, 0.0], [-0.01, -0.01, 0.0]]  # beside the origin, on either side
+
+
+def continuation_bench(gx, depth, cpu_depth, threads):
+    """Continuation (continuation.m): the Lorenz system's unstable manifold of the origin at one
+    depth, InnerGrid(3), started from two points beside the origin. Every box of the final cover
+    is mapped exactly once (in the step after its insertion), so the run images final boxes x 27
+    test points. Timed through the Python driver over the C ABI, as a caller runs it."""
+    import numpy as np
+    grid, mp = gx.InnerGrid(3), gx.Lorenz()
+    p = np.array(MANIFOLD_START)
+    g = gx.ImplicitBoxTree(MANIFOLD_C, MANIFOLD_R)
+    gx.continuation(g, grid, mp, [depth], p)  # warm-up (allocations, tuning)
+    t0 = time.perf_counter()
+    c = gx.continuation(g, grid, mp, [depth], p)[0]
+    t1 = time.perf_counter()
+    out = {"config": f"Lorenz RK4 20x0.01, unstable manifold of the origin, depth {depth}, InnerGrid(3), "
+                     f"domain centre {MANIFOLD_C} radius {MANIFOLD_R}",
+           "steps": len(c) - 1, "boxes": c[-1], "ms": (t1 - t0) * 1e3,
+           "boxes_per_s": c[-1] / (t1 - t0), "images_per_s": c[-1] * K / (t1 - t0)}
+    if cpu_depth > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        from oracle_api import OracleTree
+        o = OracleTree(MANIFOLD_C, MANIFOLD_R)
+        c0 = time.perf_counter()
+        co = gx.continuation(o, grid, mp, [cpu_depth], p)[0]
+        c1 = time.perf_counter()
+        out["cpu_baseline"] = {"value": co[-1] * K / (c1 - c0), "unit": "images/s", "cores": threads, "kind": "port",
+                               "sample": f"the same continuation at depth {cpu_depth} ({co[-1]} boxes, {len(co) - 1} "
+                                         f"steps, {c1 - c0:.1f} s), oracle"}
+    return out
+
+
+def chain_recurrent_bench(gx, steps, cpu, threads):
+    """Chain recurrent set (chainRecurrentSet.m) of the Lorenz system: `steps` x {subdivide; the
+    leaves' transition matrix (InnerGrid(3)); strong components; keep the leaves of components
+    with more than one leaf}, fused on the device. Images: 27 per leaf after each subdivision.
+    The strong components of the last cover's matrix are timed on their own as well."""
+    grid, mp = gx.InnerGrid(3), gx.Lorenz()
+
+    def run():
+        g = gx.ImplicitBoxTree(MANIFOLD_C, MANIFOLD_R)
+        g.setFlags(gx.SD, gx.LEAF)
+        t0 = time.perf_counter()
+        cnt = g.chain_recurrent_steps(grid, mp, steps)
+        return g, cnt, time.perf_counter() - t0
+
+    run()  # warm-up
+    g, cnt, secs = run()
+    leaves = int(cnt[:, 0].sum())
+    m = g.transitionMatrixForLeaves(grid, mp)
+    m.scc()
+    s0 = time.perf_counter()
+    _, n_comp, n_big = m.scc()
+    s1 = time.perf_counter()
+    rounds, passes, checks = gx.api.scc_last_stats()
+    out = {"config": f"Lorenz RK4 20x0.01, {steps} steps, InnerGrid(3), domain centre {MANIFOLD_C} radius {MANIFOLD_R}",
+           "boxes": int(cnt[-1, 3]), "leaves_stepped": leaves, "ms": secs * 1e3,
+           "images_per_s": leaves * K / secs, "boxes_per_s": leaves / secs,
+           "scc": {"vertices": m.nRows, "edges": m.nnz, "components": n_comp, "nontrivial": n_big,
+                   "ms": (s1 - s0) * 1e3, "rounds": rounds, "passes": passes, "host_checks": checks}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        from oracle_api import OracleTree
+        o = OracleTree(MANIFOLD_C, MANIFOLD_R)
+        c0 = time.perf_counter()
+        r = gx.chainRecurrentSet(o, grid, mp, steps)
+        c1 = time.perf_counter()
+        lo = sum(x[0] for x in r)
+        out["cpu_baseline"] = {"value": lo * K / (c1 - c0), "unit": "images/s", "cores": threads, "kind": "port",
+                               "sample": f"the same {steps} steps ({lo} leaves stepped, {c1 - c0:.1f} s), oracle "
+                                         f"with Tarjan's strong components"}
+    return out
+
+
 def cpu_baseline(sample_depth, threads):
     """B12's CPU path restated (oracle): same algorithm structure as the reference's host
     Thrust path, including its second image evaluation per search, OpenMP over `threads`."""
@@ -280,6 +358,9 @@ def main():
     ap.add_argument("--hh-depth", type=int, default=28, help="config 3 depth (0 to skip)")
     ap.add_argument("--hh-iters", type=int, default=100, help="config 3 subspace iterations")
     ap.add_argument("--hh-cpu-depth", type=int, default=18, help="config 3 CPU-baseline sample depth (0 to skip)")
+    ap.add_argument("--cont-depth", type=int, default=27, help="continuation depth (0 to skip)")
+    ap.add_argument("--cont-cpu-depth", type=int, default=24, help="continuation CPU-baseline depth (0 to skip)")
+    ap.add_argument("--chain-steps", type=int, default=30, help="chain recurrent set steps (0 to skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (traffic-counter calibration) and exit")
     args = ap.parse_args()
@@ -352,6 +433,11 @@ def main():
     if args.hh_depth > 0:
         out["hh"] = hh_bench(gx, args.hh_depth, args.hh_iters, 0 if args.no_cpu_baseline else args.hh_cpu_depth,
                              threads)
+    if args.cont_depth > 0:
+        out["continuation"] = continuation_bench(gx, args.cont_depth,
+                                                 0 if args.no_cpu_baseline else args.cont_cpu_depth, threads)
+    if args.chain_steps > 0:
+        out["chain_recurrent"] = chain_recurrent_bench(gx, args.chain_steps, not args.no_cpu_baseline, threads)
     if not args.no_cpu_baseline:
         v, secs, imgs, iters = cpu_baseline(args.cpu_depth, threads)
         out["cpu_baseline"] = {"value": v, "unit": "images/s", "cores": threads, "kind": "port",
