@@ -84,6 +84,10 @@ struct b12x_tree_s {
     }
   } step[2];
   uint64_t bufGen = 0;  // incremented whenever the leaf buffers are reallocated
+  // incremented whenever the leaves change (not their flags); the cached depth range belongs to
+  // rangeGen
+  uint64_t leafGen = 1, rangeGen = 0;
+  uint8_t minDepthC = 0, maxDepthC = 0;
 
   TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
   ~b12x_tree_s() {
@@ -247,21 +251,62 @@ struct LeafStats {
   uint8_t flagsAnd, flagsOr, maxDepth, minDepth;
 };
 
-__global__ void minDepthKernel(const uint8_t* depth, uint32_t n, uint32_t* out) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t m = i < n ? depth[i] : 255u;
+__global__ void statsInitKernel(uint32_t* s) {  // flags AND, flags OR, max depth, min depth
+  s[0] = 0xFFu;
+  s[1] = 0u;
+  s[2] = 0u;
+  s[3] = 255u;
+}
+
+// AND and OR of all flags, maximum and minimum depth (depth-0 forms of countIfFlags,
+// ImplicitBoxTree.hpp:1105-1114, and depth(), :1264-1271): a grid-stride pass, reduced per wave
+// by shuffles and per block through LDS, then one atomic per value and block
+constexpr uint32_t kStatsBlocks = 512;
+__global__ __launch_bounds__(kBlock) void leafStatsKernel(const uint8_t* flags, const uint8_t* depth, uint32_t n,
+                                                          uint32_t* out) {
+  uint32_t fa = 0xFF, fo = 0, mx = 0, mn = 255;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t f = flags[i], d = depth[i];
+    fa &= f;
+    fo |= f;
+    mx = max(mx, d);
+    mn = min(mn, d);
+  }
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor(m, o, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMin(out, m);
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    fa &= __shfl_xor(fa, o, kWave);
+    fo |= __shfl_xor(fo, o, kWave);
+    mx = max(mx, (uint32_t)__shfl_xor(mx, o, kWave));
+    mn = min(mn, (uint32_t)__shfl_xor(mn, o, kWave));
+  }
+  __shared__ uint32_t w[4][kBlock / kWave];
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    w[0][wave] = fa;
+    w[1][wave] = fo;
+    w[2][wave] = mx;
+    w[3][wave] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / kWave; ++k) {
+      fa &= w[0][k];
+      fo |= w[1][k];
+      mx = max(mx, w[2][k]);
+      mn = min(mn, w[3][k]);
+    }
+    atomicAnd(out + 0, fa);
+    atomicOr(out + 1, fo);
+    atomicMax(out + 2, mx);
+    atomicMin(out + 3, mn);
+  }
 }
 
 LeafStats leafStats(b12x_tree_s& T) {
-  const uint32_t init[4] = {0xFFu, 0u, 0u, 255u};
-  HIPCHK(hipMemcpyAsync(T.scal + 8, init, sizeof(init), hipMemcpyHostToDevice, T.stream));
-  const uint32_t nb = ceilDiv(T.n, kBlock);
-  hipLaunchKernelGGL(flagsReduceKernel, dim3(nb), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.depth[T.cur], T.n,
+  hipLaunchKernelGGL(statsInitKernel, dim3(1), dim3(1), 0, T.stream, T.scal + 8);
+  const uint32_t nb = std::max<uint32_t>(1, std::min<uint32_t>(ceilDiv(T.n, kBlock), kStatsBlocks));
+  hipLaunchKernelGGL(leafStatsKernel, dim3(nb), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.depth[T.cur], T.n,
                      T.scal + 8);
-  hipLaunchKernelGGL(minDepthKernel, dim3(nb), dim3(kBlock), 0, T.stream, T.depth[T.cur], T.n, T.scal + 11);
   HIPCHK(hipGetLastError());
   syncScal(T, 12);
   return {uint8_t(T.hscal[8]), uint8_t(T.hscal[9]), uint8_t(T.hscal[10]), uint8_t(T.hscal[11])};
@@ -277,9 +322,14 @@ DepthClass classifyDepth(b12x_tree_s& T, uint8_t d) {
   if (d == 0) return DC_ROOT;
   if (d == kLeafDepth) return DC_LEAF;
   if (d == kUnsubDepth) return DC_GENERAL;
-  LeafStats s = leafStats(T);
-  if (s.minDepth == d && s.maxDepth == d) return DC_LEAF;
-  return d > s.maxDepth ? DC_BEYOND : DC_GENERAL;
+  if (T.rangeGen != T.leafGen) {  // the depth range only changes with the leaves
+    const LeafStats s = leafStats(T);
+    T.minDepthC = s.minDepth;
+    T.maxDepthC = s.maxDepth;
+    T.rangeGen = T.leafGen;
+  }
+  if (T.minDepthC == d && T.maxDepthC == d) return DC_LEAF;
+  return d > T.maxDepthC ? DC_BEYOND : DC_GENERAL;
 }
 
 // Boxes at a general depth (ImplicitBoxTree.hpp:1383-1473): box index, membership and box start
@@ -349,6 +399,7 @@ void insertRootIfEmpty(b12x_tree_s& T) {  // ImplicitBoxTree.hpp:1959-1970
   HIPCHK(hipMemcpyAsync(T.flags[T.cur], &z, 1, hipMemcpyHostToDevice, T.stream));
   HIPCHK(hipStreamSynchronize(T.stream));
   T.n = 1;
+  ++T.leafGen;
 }
 
 // -- point iterators --------------------------------------------------------------------------
@@ -927,6 +978,7 @@ int b12x_tree_upload(b12x_tree t, uint32_t n, const uint64_t* bits, const uint8_
     HIPCHK(hipMemcpyAsync(t->flags[t->cur], f.data(), n, hipMemcpyHostToDevice, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
     t->n = n;
+    ++t->leafGen;
     insertRootIfEmpty(*t);
     return B12X_OK;
   });
@@ -946,6 +998,7 @@ int b12x_subdivide(b12x_tree t, uint8_t f, uint32_t* ret) {
     syncScal(T, 2);
     T.n = T.hscal[0];
     T.cur = 1 - T.cur;
+    ++T.leafGen;
     *ret = T.hscal[1];
     return B12X_OK;
   });
@@ -983,6 +1036,7 @@ int b12x_unsubdivide(b12x_tree t, uint8_t f, uint32_t* ret) {
     dfree(f2);
     *ret = T.hscal[1] - (T.n - m);
     T.n = m;
+    ++T.leafGen;
     return B12X_OK;
   });
 }
@@ -1003,6 +1057,7 @@ int b12x_remove(b12x_tree t, uint8_t f, uint32_t* ret) {
     *ret = T.n - m;
     T.n = m;
     T.cur = 1 - T.cur;
+    ++T.leafGen;
     insertRootIfEmpty(T);
     return B12X_OK;
   });
@@ -1029,7 +1084,8 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
   uint8_t* ins = dalloc<uint8_t>(P);
   uint32_t* insU32 = dalloc<uint32_t>(P + 1);
   hipLaunchKernelGGL(insertMarkKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, pk, sv, P, form,
-                     (const uint8_t*)T.depth[T.cur], ins, insU32, T.hit, (uint32_t*)nullptr);
+                     form == 0 ? 0 : int(kMaxDepth) - int(d), (const uint8_t*)T.depth[T.cur], ins, insU32, T.hit,
+                     (uint32_t*)nullptr);
   HIPCHK(hipGetLastError());
   // flagsForHit on the hit boxes; the hit bytes end zero
   if (form != 2) {
@@ -1055,37 +1111,55 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
     dfree(insU32);
     return;
   }
-  // the leaves and the new boxes, in one array
+  // the new boxes, sorted (comparable key ascending, deeper first on equal keys, stable), then
+  // merged into the leaves, which are sorted already
   const uint64_t N = uint64_t(n) + m;
   ensureCap(T, N);
-  uint64_t* allK = dalloc<uint64_t>(N);
-  uint8_t* allD = dalloc<uint8_t>(N);
-  uint8_t* allF = dalloc<uint8_t>(N);
-  HIPCHK(hipMemcpyAsync(allK, T.key[T.cur], size_t(n) * 8, hipMemcpyDeviceToDevice, T.stream));
-  HIPCHK(hipMemcpyAsync(allD, T.depth[T.cur], n, hipMemcpyDeviceToDevice, T.stream));
-  HIPCHK(hipMemcpyAsync(allF, T.flags[T.cur], n, hipMemcpyDeviceToDevice, T.stream));
+  uint64_t* newK = dalloc<uint64_t>(m);
+  uint8_t* newD = dalloc<uint8_t>(m);
+  uint8_t* newF = dalloc<uint8_t>(m);
   hipLaunchKernelGGL(insertNewKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, pk,
-                     (const uint8_t*)ins, (const uint32_t*)insU32, P, form == 0 ? 1 : 0, d,
-                     (const uint64_t*)T.key[T.cur], (const uint8_t*)T.depth[T.cur], n, fIns, allK + n, allD + n,
-                     allF + n);
+                     (const uint32_t*)insU32, m, P, form == 0 ? 1 : 0, d,
+                     (const uint64_t*)T.key[T.cur], (const uint8_t*)T.depth[T.cur], n, fIns, newK, newD, newF);
   HIPCHK(hipGetLastError());
-  // order: comparable key ascending, deeper first on equal keys (three stable LSD sorts)
-  uint32_t* idx = dalloc<uint32_t>(N);
-  uint32_t* k32 = dalloc<uint32_t>(N);
-  uint32_t* dummy = dalloc<uint32_t>(N);
-  hipLaunchKernelGGL(iotaKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, idx, N);
-  const int partBits[3] = {8, 32, 32};
-  for (int part = 0; part < 3; ++part) {
-    hipLaunchKernelGGL(boxSortKeyKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)allK,
-                       (const uint8_t*)allD, (const uint32_t*)idx, N, part, k32);
+  uint64_t* nsK = dalloc<uint64_t>(m);
+  uint8_t* nsD = dalloc<uint8_t>(m);
+  uint8_t* nsF = dalloc<uint8_t>(m);
+  if (m <= kSmallSortMax) {
+    hipLaunchKernelGGL(smallBoxSortKernel, dim3(1), dim3(kSmallSortThreads), 0, T.stream, (const uint64_t*)newK,
+                       (const uint8_t*)newD, (const uint8_t*)newF, m, nsK, nsD, nsF);
+  } else {
+    // two stable LSD radix sorts by the key's low and high words (equal keys are equal boxes)
+    uint32_t* idx = dalloc<uint32_t>(m);
+    uint32_t* k32 = dalloc<uint32_t>(m);
+    uint32_t* dummy = dalloc<uint32_t>(m);
+    hipLaunchKernelGGL(iotaKernel, dim3(ceilDiv(m, kBlock)), dim3(kBlock), 0, T.stream, idx, uint64_t(m));
+    for (int part = 1; part < 3; ++part) {
+      hipLaunchKernelGGL(boxSortKeyKernel, dim3(ceilDiv(m, kBlock)), dim3(kBlock), 0, T.stream,
+                         (const uint64_t*)newK, (const uint8_t*)newD, (const uint32_t*)idx, uint64_t(m), part, k32);
+      HIPCHK(hipGetLastError());
+      radixSort3(T.stream, k32, idx, dummy, m, 32, T.hscal + 5);
+    }
+    hipLaunchKernelGGL(gatherBoxesKernel, dim3(ceilDiv(m, kBlock)), dim3(kBlock), 0, T.stream,
+                       (const uint64_t*)newK, (const uint8_t*)newD, (const uint8_t*)newF, (const uint32_t*)idx,
+                       uint64_t(m), nsK, nsD, nsF);
     HIPCHK(hipGetLastError());
-    radixSort3(T.stream, k32, idx, dummy, N, partBits[part], T.hscal + 5);
+    HIPCHK(hipStreamSynchronize(T.stream));
+    dfree(idx);
+    dfree(k32);
+    dfree(dummy);
   }
+  HIPCHK(hipGetLastError());
   uint64_t* sK = dalloc<uint64_t>(N);
   uint8_t* sD = dalloc<uint8_t>(N);
   uint8_t* sF = dalloc<uint8_t>(N);
-  hipLaunchKernelGGL(gatherBoxesKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)allK,
-                     (const uint8_t*)allD, (const uint8_t*)allF, (const uint32_t*)idx, N, sK, sD, sF);
+  hipLaunchKernelGGL(mergeNewKernel, dim3(ceilDiv(m, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)nsK,
+                     (const uint8_t*)nsD, (const uint8_t*)nsF, m, (const uint64_t*)T.key[T.cur],
+                     (const uint8_t*)T.depth[T.cur], n, sK, sD, sF);
+  hipLaunchKernelGGL(mergeLeafKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream,
+                     (const uint64_t*)T.key[T.cur], (const uint8_t*)T.depth[T.cur], (const uint8_t*)T.flags[T.cur], n,
+                     (const uint64_t*)nsK, (const uint8_t*)nsD, m, sK, sD, sF);
+  HIPCHK(hipGetLastError());
   // runs on one path, reduced
   uint32_t* run = dalloc<uint32_t>(N);
   uint8_t* runStart = dalloc<uint8_t>(N);
@@ -1115,8 +1189,9 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
   HIPCHK(hipStreamSynchronize(T.stream));
   T.cur = 1 - T.cur;
   T.n = K;
-  for (void* q : {(void*)ins, (void*)insU32, (void*)allK, (void*)allD, (void*)allF, (void*)idx, (void*)k32,
-                  (void*)dummy, (void*)sK, (void*)sD, (void*)sF, (void*)run, (void*)runStart, (void*)rb, (void*)rd,
+  ++T.leafGen;
+  for (void* q : {(void*)ins, (void*)insU32, (void*)newK, (void*)newD, (void*)newF, (void*)nsK, (void*)nsD,
+                  (void*)nsF, (void*)sK, (void*)sD, (void*)sF, (void*)run, (void*)runStart, (void*)rb, (void*)rd,
                   (void*)rf, (void*)keep})
     dfree(q);
 }
@@ -1172,6 +1247,7 @@ int b12x_delete_depth(b12x_tree t, uint8_t d, uint32_t* ret) {
     dfree(acc);
     T.cur = 1 - T.cur;
     T.n = m;
+    ++T.leafGen;
     *ret = nb;
     return B12X_OK;
   });
@@ -1588,6 +1664,7 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
       const uint32_t nSub = T.hscal[0], cSub = T.hscal[1], kept = T.hscal[2], cSet = T.hscal[3];
       if (T.timing) T.imagePoints += uint64_t(nSub) * s->grid.k;
       T.n = kept;
+      ++T.leafGen;
       insertRootIfEmpty(T);
       if (graphs && T.n != kept) stageLeafCount(T);
       if (counts) {

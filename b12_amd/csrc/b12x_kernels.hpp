@@ -374,30 +374,6 @@ static __global__ __launch_bounds__(kBlock) void flagsKernel(uint8_t* flags, uin
   if (threadIdx.x == 0 && c) atomicAdd(counter, c);
 }
 
-// AND and OR of all flags (depth-0 forms of countIfFlags, ImplicitBoxTree.hpp:1105-1114) and the
-// maximum depth (ImplicitBoxTree.hpp:1264-1271).
-static __global__ __launch_bounds__(kBlock) void flagsReduceKernel(const uint8_t* flags, const uint8_t* depth, uint32_t n,
-                                                            uint32_t* out /*[and, or, maxDepth]*/) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t fa = 0xFF, fo = 0, md = 0;
-  if (i < n) {
-    fa = flags[i];
-    fo = flags[i];
-    md = depth[i];
-  }
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    fa &= __shfl_xor(fa, o, kWave);
-    fo |= __shfl_xor(fo, o, kWave);
-    md = max(md, (uint32_t)__shfl_xor(md, o, kWave));
-  }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    atomicAnd(out + 0, fa);
-    atomicOr(out + 1, fo);
-    atomicMax(out + 2, md);
-  }
-}
-
 // ---------------------------------------------------------------------------------------------
 // The fused image kernel: one lane per test point. Point i of source box b = i / k:
 // box geometry -> grid offset -> x = c + g*r (Functors.hpp:476-511) -> map -> key -> descent.
@@ -707,32 +683,92 @@ static __global__ __launch_bounds__(kBlock) void truncWriteKernel(TreeView in, T
 // insertion: per point, whether it is inserted (ins) and which existing box it hits (hit[t] = 1)
 // form 0: leaf depth (a point held by a leaf shallower than 64 is inserted too), 1: a depth with
 // boxes (inserted if no box holds it), 2: deeper than every leaf (every valid point)
+// Outside leaf depth a new box is the point's box truncated to depth d (shift = 64 - d), so a
+// point whose box equals the previous point's, with both inserted, adds nothing to the merge
+// (runs on one path reduce with max and OR): insU32, the compaction flag, drops it, while ins
+// (isToInsert) still reports it.
 static __global__ __launch_bounds__(kBlock) void insertMarkKernel(const uint64_t* pk, const uint32_t* sv, uint64_t P,
-                                                                 int form, const uint8_t* leafDepth, uint8_t* ins,
-                                                                 uint32_t* insU32, uint8_t* hit, uint32_t* anyValid) {
+                                                                 int form, int shift, const uint8_t* leafDepth,
+                                                                 uint8_t* ins, uint32_t* insU32, uint8_t* hit,
+                                                                 uint32_t* anyValid) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= P) return;
-  const bool valid = pk[i] != kInvalidKey;
+  const uint64_t p = pk[i];
+  const bool valid = p != kInvalidKey;
   const uint32_t t = form == 2 ? kInvalid : sv[i];
   bool in;
   if (form == 0) in = t == kInvalid ? valid : leafDepth[t] < kMaxDepth;
   else in = t == kInvalid && valid;
+  bool keep = in;
+  if (in && form != 0 && i > 0) {
+    const uint64_t q = pk[i - 1];
+    const bool prevIn = q != kInvalidKey && (form == 2 || sv[i - 1] == kInvalid);
+    if (prevIn && (q >> shift) == (p >> shift)) keep = false;
+  }
   ins[i] = in ? 1 : 0;
-  insU32[i] = in ? 1u : 0u;
+  insU32[i] = keep ? 1u : 0u;
   if (t != kInvalid && !(form == 0 && in)) hit[t] = 1;
   if (valid && anyValid) atomicOr(anyValid, 1u);
+}
+
+// One-workgroup sort of up to kSmallSortMax new boxes by (comparable key, index): a bitonic
+// network in LDS. Equal comparable keys mean equal boxes, so this is the stable order by
+// (key ascending, deeper first). Writes the boxes in that order.
+constexpr uint32_t kSmallSortMax = 4096, kSmallSortThreads = 1024;
+static __global__ __launch_bounds__(kSmallSortThreads) void smallBoxSortKernel(const uint64_t* k, const uint8_t* d,
+                                                                              const uint8_t* f, uint32_t m,
+                                                                              uint64_t* ok, uint8_t* od,
+                                                                              uint8_t* of) {
+  __shared__ uint64_t sk[kSmallSortMax];
+  __shared__ uint32_t si[kSmallSortMax];
+  uint32_t n2 = 1;
+  while (n2 < m) n2 <<= 1;
+  for (uint32_t i = threadIdx.x; i < n2; i += kSmallSortThreads) {
+    sk[i] = i < m ? k[i] : ~0ull;
+    si[i] = i;  // padding (index >= m) sorts last
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= n2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = threadIdx.x; t < n2 / 2; t += kSmallSortThreads) {
+        const uint32_t a = 2 * t - (t & (stride - 1));  // first of the pair
+        const uint32_t b = a + stride;
+        const bool up = (a & size) == 0;
+        const uint64_t ka = sk[a], kb = sk[b];
+        const uint32_t ia = si[a], ib = si[b];
+        const bool gt = ka > kb || (ka == kb && ia > ib);
+        if (gt == up) {
+          sk[a] = kb;
+          sk[b] = ka;
+          si[a] = ib;
+          si[b] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < m; i += kSmallSortThreads) {
+    const uint32_t j = si[i];
+    ok[i] = sk[i];
+    od[i] = d[j];
+    of[i] = f[j];
+  }
 }
 
 // the new boxes, compacted: at leaf depth one level below the deepest mutual ancestor with the
 // leaves on either side of the lower bound (AdaptDepthOfFirstBoxFunctor, Functors.hpp:379-389),
 // otherwise truncated to d
-static __global__ __launch_bounds__(kBlock) void insertNewKernel(const uint64_t* pk, const uint8_t* ins,
-                                                                const uint32_t* pos, uint64_t P, int leafForm,
-                                                                uint8_t d, const uint64_t* key, const uint8_t* depth,
-                                                                uint32_t n, uint8_t flags, uint64_t* outKey,
-                                                                uint8_t* outDepth, uint8_t* outFlags) {
+static __global__ __launch_bounds__(kBlock) void insertNewKernel(const uint64_t* pk, const uint32_t* pos,
+                                                                uint32_t total, uint64_t P, int leafForm, uint8_t d,
+                                                                const uint64_t* key, const uint8_t* depth, uint32_t n,
+                                                                uint8_t flags, uint64_t* outKey, uint8_t* outDepth,
+                                                                uint8_t* outFlags) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= P || !ins[i]) return;
+  if (i >= P) return;
+  // pos is the exclusive scan of the compaction flags: point i has a slot iff the next offset
+  // (the total after the last point) is larger
+  const uint32_t next = i + 1 < P ? pos[i + 1] : total;
+  if (next == pos[i]) return;
   const uint64_t p = pk[i];  // the point's box at depth 64: its comparable key is its pattern
   int nd;
   if (leafForm) {
@@ -781,6 +817,62 @@ static __global__ __launch_bounds__(kBlock) void gatherBoxesKernel(const uint64_
   ok[i] = key[j];
   od[i] = depth[j];
   of[i] = flags[j];
+}
+
+// the order of the insertion's sort: comparable key ascending, deeper first on equal keys
+__device__ inline bool boxLess(uint64_t ka, uint8_t da, uint64_t kb, uint8_t db) {
+  return ka < kb || (ka == kb && da > db);
+}
+
+// Merge of the m sorted new boxes into the n sorted leaves, in the order a stable sort of the
+// leaves followed by the new boxes gives: new box j goes after every leaf not greater than it
+// (mergeNewKernel), leaf i after every new box less than it (mergeLeafKernel).
+static __global__ __launch_bounds__(kBlock) void mergeNewKernel(const uint64_t* nk, const uint8_t* nd,
+                                                               const uint8_t* nf, uint32_t m, const uint64_t* lk,
+                                                               const uint8_t* ld, uint32_t n, uint64_t* ok,
+                                                               uint8_t* od, uint8_t* of) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= m) return;
+  const uint64_t k = nk[j];
+  const uint8_t d = nd[j];
+  uint32_t lo = 0, len = n;
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (!boxLess(k, d, lk[lo + half], ld[lo + half])) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  const uint64_t o = uint64_t(j) + lo;
+  ok[o] = k;
+  od[o] = d;
+  of[o] = nf[j];
+}
+
+static __global__ __launch_bounds__(kBlock) void mergeLeafKernel(const uint64_t* lk, const uint8_t* ld,
+                                                                const uint8_t* lf, uint32_t n, const uint64_t* nk,
+                                                                const uint8_t* nd, uint32_t m, uint64_t* ok,
+                                                                uint8_t* od, uint8_t* of) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = lk[i];
+  const uint8_t d = ld[i];
+  uint32_t lo = 0, len = m;
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (boxLess(nk[lo + half], nd[lo + half], k, d)) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  const uint64_t o = uint64_t(i) + lo;
+  ok[o] = k;
+  od[o] = d;
+  of[o] = lf[i];
 }
 
 // runs of consecutive boxes on one path (hasSamePath)
