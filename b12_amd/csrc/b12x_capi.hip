@@ -261,11 +261,27 @@ __global__ void statsInitKernel(uint32_t* s) {  // flags AND, flags OR, max dept
 // AND and OR of all flags, maximum and minimum depth (depth-0 forms of countIfFlags,
 // ImplicitBoxTree.hpp:1105-1114, and depth(), :1264-1271): a grid-stride pass, reduced per wave
 // by shuffles and per block through LDS, then one atomic per value and block
-constexpr uint32_t kStatsBlocks = 512;
+// (64 blocks: the per-block atomics on the four words serialise, so few blocks with a longer
+// loop of 4-byte loads are faster)
+constexpr uint32_t kStatsBlocks = 64;
 __global__ __launch_bounds__(kBlock) void leafStatsKernel(const uint8_t* flags, const uint8_t* depth, uint32_t n,
                                                           uint32_t* out) {
   uint32_t fa = 0xFF, fo = 0, mx = 0, mn = 255;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  const uint32_t n4 = n / 4;  // the leaf arrays are 256-byte aligned
+  const uint32_t* f4 = reinterpret_cast<const uint32_t*>(flags);
+  const uint32_t* d4 = reinterpret_cast<const uint32_t*>(depth);
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n4; i += gridDim.x * kBlock) {
+    const uint32_t f = f4[i], d = d4[i];
+#pragma unroll
+    for (int b = 0; b < 32; b += 8) {
+      const uint32_t fb = (f >> b) & 0xFFu, db = (d >> b) & 0xFFu;
+      fa &= fb;
+      fo |= fb;
+      mx = max(mx, db);
+      mn = min(mn, db);
+    }
+  }
+  for (uint32_t i = 4 * n4 + blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const uint32_t f = flags[i], d = depth[i];
     fa &= f;
     fo |= f;
@@ -304,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void leafStatsKernel(const uint8_t* flags, 
 
 LeafStats leafStats(b12x_tree_s& T) {
   hipLaunchKernelGGL(statsInitKernel, dim3(1), dim3(1), 0, T.stream, T.scal + 8);
-  const uint32_t nb = std::max<uint32_t>(1, std::min<uint32_t>(ceilDiv(T.n, kBlock), kStatsBlocks));
+  const uint32_t nb = std::max<uint32_t>(1, std::min<uint32_t>(ceilDiv(T.n, 4 * kBlock), kStatsBlocks));
   hipLaunchKernelGGL(leafStatsKernel, dim3(nb), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.depth[T.cur], T.n,
                      T.scal + 8);
   HIPCHK(hipGetLastError());

@@ -12,8 +12,8 @@
  *   b12x_coo     a transition matrix in COO form resident in HBM (CooMatrix.h:56-121)
  * The point iterator of a call is (scheme, depth_it, flags_to_check): depth_it = 255 for
  * makePointIteratorPairOverLeaves / ByFlags(-1, flags) and 0 for makePointIteratorPairOverRoot.
- * Depth arguments follow B12: 255 (= -1) is the leaf depth, 0 the root; any other depth is
- * accepted when every leaf lies on it, otherwise B12X_E_UNSUPPORTED.
+ * Depth arguments follow B12: 255 (= -1) is the leaf depth, 0 the root, 254 the unsubdividable
+ * depth; any other depth addresses the boxes of that depth (none beyond the deepest leaf).
  *
  * Threading: one tree or matrix handle must not be used from two threads at once (B12 is not
  * thread-safe either, ImplicitBoxTree.h:116-122). Distinct handles are independent; each tree
