@@ -35,6 +35,11 @@ SIGNATURES = {
     "b12x_depth": [_P, _P],
     "b12x_boxes": [_P, _U8, _P, _P],
     "b12x_tree_geometry": [_P, _P, _P, _P, _P, _P],
+    "b12x_tree_set_domain": [_P, _P, _P, _P, _I, _I],
+    "b12x_all_leaves_on_depth": [_P, _U8, _P],
+    "b12x_has_same_boxes": [_P, _P, _P],
+    "b12x_get_host_buffer_bytes": [_P, _P],
+    "b12x_set_host_buffer_bytes": [_P, _U64],
     "b12x_delete_depth": [_P, _U8, _P],
     "b12x_insert_by_image": [_P, _P, _U8, _U8, _U8, _U8, _U8, _P],
     "b12x_insert_by_points": [_P, _P, _U64, _U8, _U8, _U8, _P],
@@ -465,6 +470,124 @@ class ImplicitBoxTree:
 
     def changeFlagsByStencil(self, stencil, flags_to_unset, flags_to_set, depth=-1):
         return self._stencil(2, stencil, flags_to_unset, flags_to_set, depth)
+
+    # ---- by points and by search vector (ImplicitBoxTree.h:97-108, 472-480, 532-539): the boxes of
+    # depth `depth` holding the points, or named by a sorted search vector (searchCompacted's
+    # output; invalid entries are skipped), select the boxes as a stencil would
+    def _search_stencil(self, search, depth):
+        st = np.zeros(self.count(depth), np.uint8)
+        search = np.asarray(search, np.uint32)
+        ok = search[search != INVALID]
+        st[ok[ok < len(st)]] = 1
+        return st
+
+    def setFlagsByPoints(self, points, flags, depth=-1):
+        return self.setFlagsByStencil(self._search_stencil(self.searchByPoints(points, depth), depth), flags, depth)
+
+    def unsetFlagsByPoints(self, points, flags, depth=-1):
+        return self.unsetFlagsByStencil(self._search_stencil(self.searchByPoints(points, depth), depth), flags, depth)
+
+    def changeFlagsByPoints(self, points, flags_to_unset, flags_to_set, depth=-1):
+        return self.changeFlagsByStencil(self._search_stencil(self.searchByPoints(points, depth), depth),
+                                         flags_to_unset, flags_to_set, depth)
+
+    def setFlagsBySearch(self, search, flags, depth=-1):
+        return self.setFlagsByStencil(self._search_stencil(search, depth), flags, depth)
+
+    def unsetFlagsBySearch(self, search, flags, depth=-1):
+        return self.unsetFlagsByStencil(self._search_stencil(search, depth), flags, depth)
+
+    def changeFlagsBySearch(self, search, flags_to_unset, flags_to_set, depth=-1):
+        return self.changeFlagsByStencil(self._search_stencil(search, depth), flags_to_unset, flags_to_set, depth)
+
+    # ---- domain and bookkeeping (ImplicitBoxTree.h:82-85, 141-143, 244, 550-569)
+    def getCenter(self):
+        return self.geometry()[0]
+
+    def getRadius(self):
+        return self.geometry()[1]
+
+    def getSdScheme(self):
+        return self.geometry()[2]
+
+    def setCenter(self, center):
+        """setCenter: cut to dim entries, or extended with 0.5 (ImplicitBoxTree.hpp:3470-3478)."""
+        c = np.full(self.dim, 0.5)
+        v = np.asarray(center, np.float64).ravel()[: self.dim]
+        c[: len(v)] = v
+        self.api("b12x_tree_set_domain", self.h, _ptr(c), None, None, 0, 0)
+        return self.getCenter()
+
+    def setRadius(self, radius):
+        """setRadius: cut or extended with 0.5; negative entries leave the radius unchanged with a
+        warning (ImplicitBoxTree.hpp:3487-3505)."""
+        r = np.full(self.dim, 0.5)
+        v = np.asarray(radius, np.float64).ravel()[: self.dim]
+        r[: len(v)] = v
+        if np.any(v < 0):
+            print("Warning: Input has negative values. Radius vector remains unchanged.")
+            return self.getRadius()
+        self.api("b12x_tree_set_domain", self.h, None, _ptr(r), None, 0, 0)
+        return self.getRadius()
+
+    def setSdScheme(self, sd_scheme):
+        """setSdScheme: repeated periodically, the default scheme when empty; values >= dim leave it
+        unchanged with a warning (ImplicitBoxTree.hpp:3513-3552)."""
+        sd = np.ascontiguousarray(sd_scheme, np.int64).ravel()
+        if np.any(sd < 0) or np.any(sd >= self.dim):
+            print("Warning: Input has values >= DIM. SdScheme remains unchanged.")
+            return self.getSdScheme()
+        sd8 = sd.astype(np.uint8)
+        self.api("b12x_tree_set_domain", self.h, None, None, _ptr(sd8) if len(sd8) else None, len(sd8), 1)
+        return self.getSdScheme()
+
+    def isSdSchemeDefault(self):
+        sd = self.getSdScheme()
+        return bool(np.all(sd == np.arange(len(sd)) % self.dim))
+
+    def getSdCount(self):
+        """sdCount_ (adaptSdCount, ImplicitBoxTree.hpp:574-587): row i + 1 counts, per coordinate,
+        the bisections of the first i + 1 scheme entries; (64 + 2) x dim entries."""
+        sd = self.getSdScheme()
+        c = np.zeros(((64 + 2), self.dim), np.uint8)
+        for i, j in enumerate(sd):
+            c[i + 1] = c[i]
+            c[i + 1, j] += 1
+        return c.ravel()
+
+    def areAllLeavesOnDepth(self, depth):
+        r = _I()
+        self.api("b12x_all_leaves_on_depth", self.h, _depth(depth), ctypes.byref(r))
+        return bool(r.value)
+
+    def hasSameBoxes(self, other):
+        r = _I()
+        self.api("b12x_has_same_boxes", self.h, other.h, ctypes.byref(r))
+        return bool(r.value)
+
+    def getNHostBufferBytes(self):
+        r = _U64()
+        self.api("b12x_get_host_buffer_bytes", self.h, ctypes.byref(r))
+        return r.value
+
+    def setNHostBufferBytes(self, n):
+        self.api("b12x_set_host_buffer_bytes", self.h, int(n))
+        return self.getNHostBufferBytes()
+
+    def getDepthVector(self):
+        return self.download()[1]
+
+    def getFlagsVector(self):
+        return self.download()[2]
+
+    def copy(self):
+        """A new tree with this tree's domain, scheme, leaves and host buffer size (the
+        reference's copy constructor, ImplicitBoxTree.hpp:47-84)."""
+        c, r, sd = self.geometry()
+        t = ImplicitBoxTree(c, r, None if self.isSdSchemeDefault() and len(sd) == 64 else sd, flavour=self.flavour)
+        t.upload(*self.download())
+        t.setNHostBufferBytes(self.getNHostBufferBytes())
+        return t
 
     # -- point iterators
     def scheme(self, grid, mp):

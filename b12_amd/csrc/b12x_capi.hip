@@ -88,6 +88,7 @@ struct b12x_tree_s {
   // rangeGen
   uint64_t leafGen = 1, rangeGen = 0;
   uint8_t minDepthC = 0, maxDepthC = 0;
+  uint64_t hostBufferBytes = uint64_t(4) << 30;  // nHostBufferBytes_ (ImplicitBoxTree.hpp:38)
 
   TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
   ~b12x_tree_s() {
@@ -316,6 +317,12 @@ __global__ __launch_bounds__(kBlock) void leafStatsKernel(const uint8_t* flags, 
     atomicMax(out + 2, mx);
     atomicMin(out + 3, mn);
   }
+}
+
+// any difference between the first n leaves of two trees sets *diff
+__global__ void sameLeavesKernel(TreeView a, TreeView b, uint32_t n, uint32_t* diff) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && (a.key[i] != b.key[i] || a.depth[i] != b.depth[i] || a.flags[i] != b.flags[i])) atomicOr(diff, 1u);
 }
 
 LeafStats leafStats(b12x_tree_s& T) {
@@ -915,6 +922,95 @@ int b12x_tree_geometry(b12x_tree t, int* dim, double* center, double* radius, ui
 
 int b12x_tree_destroy(b12x_tree t) {
   delete t;
+  return B12X_OK;
+}
+
+// setCenter / setRadius / setSdScheme (ImplicitBoxTree.hpp:3464-3552): the leaves stay, the
+// domain they are read in changes; captured subdivision steps hold the old domain and are rebuilt
+int b12x_tree_set_domain(b12x_tree t, const double* center, const double* radius, const uint8_t* sd, int nsd,
+                         int setSd) {
+  return guarded([&]() -> int {
+    if (!t) return setErr(B12X_E_ARG, "null tree");
+    b12x_tree_s& T = *t;
+    std::vector<double> c(T.dom.center, T.dom.center + T.dim), r(T.dom.radius, T.dom.radius + T.dim);
+    if (center) c.assign(center, center + T.dim);
+    if (radius) {
+      for (int j = 0; j < T.dim; ++j)
+        if (!(radius[j] >= 0.0)) return setErr(B12X_E_ARG, "Input has negative values. Radius vector remains unchanged.");
+      r.assign(radius, radius + T.dim);
+    }
+    std::vector<uint8_t> keep;
+    const uint8_t* sdUse = nullptr;
+    int nUse = 0, entries = T.sdEntries;
+    if (setSd) {
+      for (int i = 0; i < nsd; ++i)
+        if (sd[i] >= T.dim) return setErr(B12X_E_ARG, "Input has values >= DIM. SdScheme remains unchanged.");
+      if (nsd > 0) {
+        sdUse = sd;
+        nUse = nsd;
+      }
+      entries = nsd > 0 ? kMaxDepth + 1 : kMaxDepth;  // a given scheme: 65 entries, the default 64
+    } else if (T.sdEntries != kMaxDepth) {
+      keep.assign(T.dom.sdScheme, T.dom.sdScheme + T.sdEntries);
+      sdUse = keep.data();
+      nUse = int(keep.size());
+    }
+    Domain D{};
+    if (!buildDomain(T.dim, c.data(), r.data(), sdUse, nUse, D)) return setErr(B12X_E_ARG, "sdScheme has values >= dim");
+    T.dom = D;
+    T.sdEntries = entries;
+    ++T.bufGen;  // the captured steps carry the domain as a kernel argument
+    return B12X_OK;
+  });
+}
+
+// areAllLeavesOnDepth (ImplicitBoxTree.hpp:589-609)
+int b12x_all_leaves_on_depth(b12x_tree t, uint8_t d, int* out) {
+  return guarded([&]() -> int {
+    if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+    if (d == kLeafDepth) *out = 1;
+    else if (d == kUnsubDepth) *out = 0;
+    else *out = classifyDepth(*t, d) == DC_LEAF || (d == 0 && t->n == 1 && leafStats(*t).maxDepth == 0) ? 1 : 0;
+    return B12X_OK;
+  });
+}
+
+// hasSameBoxes (ImplicitBoxTree.hpp:1299-1381): dimension, centre, radius, the leaves and the
+// subdivision scheme up to depth(); the reference compares this tree's leaves with the first
+// leaves of the other, so a longer other tree with the same first leaves counts as the same
+int b12x_has_same_boxes(b12x_tree a, b12x_tree b, int* out) {
+  return guarded([&]() -> int {
+    if (!a || !b || !out) return setErr(B12X_E_ARG, "null argument");
+    *out = 0;
+    if (a->dim != b->dim) return B12X_OK;
+    for (int j = 0; j < a->dim; ++j)
+      if (!(a->dom.center[j] == b->dom.center[j]) || !(a->dom.radius[j] == b->dom.radius[j])) return B12X_OK;
+    if (b->n < a->n) return B12X_OK;  // the reference would read past the other tree's leaves
+    const uint8_t dA = leafStats(*a).maxDepth;
+    for (int i = 0; i < int(dA) && i <= kMaxDepth; ++i)
+      if (a->dom.sdScheme[i] != b->dom.sdScheme[i]) return B12X_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemsetAsync(a->scal + 20, 0, 4, a->stream));
+    hipLaunchKernelGGL(sameLeavesKernel, dim3(ceilDiv(a->n, kBlock)), dim3(kBlock), 0, a->stream, a->view(a->cur),
+                       b->view(b->cur), a->n, a->scal + 20);
+    HIPCHK(hipGetLastError());
+    syncScal(*a, 21);
+    *out = a->hscal[20] == 0 ? 1 : 0;
+    return B12X_OK;
+  });
+}
+
+// getNHostBufferBytes / setNHostBufferBytes (ImplicitBoxTree.hpp:3572-3583): kept and written to
+// files; every tree here lives in HBM whole, so the value does not limit anything
+int b12x_get_host_buffer_bytes(b12x_tree t, uint64_t* out) {
+  if (!t || !out) return setErr(B12X_E_ARG, "null argument");
+  *out = t->hostBufferBytes;
+  return B12X_OK;
+}
+
+int b12x_set_host_buffer_bytes(b12x_tree t, uint64_t v) {
+  if (!t) return setErr(B12X_E_ARG, "null tree");
+  t->hostBufferBytes = v;
   return B12X_OK;
 }
 

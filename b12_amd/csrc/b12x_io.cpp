@@ -202,7 +202,9 @@ int b12x_tree_save_json(b12x_tree t, const char* fileName) {
     ofs << ",";
     putVector(ofs, "flagsVector", flags, asInt);
     ofs << ",";
-    putNumber(ofs, "nHostBufferBytes", kHostBufferBytes);
+    uint64_t hostBytes = kHostBufferBytes;
+    check(b12x_get_host_buffer_bytes(t, &hostBytes));
+    putNumber(ofs, "nHostBufferBytes", hostBytes);
     ofs << "\n}";
     if (!ofs.good()) return b12x::setErr(B12X_E_IO, "an error occurred during writing: " + name);
     return B12X_OK;
@@ -270,9 +272,10 @@ int b12x_tree_load(const char* fileName, int dim, b12x_tree* out) {
     std::vector<double> center, radius;
     std::vector<uint8_t> sd;
     Leaves L;
+    uint64_t hostBytes = kHostBufferBytes;
     if (legacy) {  // ImplicitBoxTree.hpp:174-257
       uint32_t vmaj = 0, vmin = 0;
-      uint64_t fdim = 0, n = 0, hostBytes = 0;
+      uint64_t fdim = 0, n = 0;
       bool ok = readRaw(ifs, &vmaj, 1) && readRaw(ifs, &vmin, 1) && readRaw(ifs, &fdim, 1);
       if (ok && fdim != uint64_t(dim)) return b12x::setErr(B12X_E_IO, "file holds a tree of another dimension");
       center.resize(dim);
@@ -317,6 +320,8 @@ int b12x_tree_load(const char* fileName, int dim, b12x_tree* out) {
       for (uint64_t v : u64Array(str, "flagsVector")) L.flags.push_back(uint8_t(v));
       if (L.bits.size() != n || L.depth.size() != n || L.flags.size() != n)
         return b12x::setErr(B12X_E_IO, "nLeaves does not match the leaf arrays");
+      if (valueStart(str, "nHostBufferBytes") != std::string::npos)
+        hostBytes = parseU64(scalarToken(str, "nHostBufferBytes"), "nHostBufferBytes");
       for (uint64_t i = 0; i < n; ++i)
         if (L.depth[i] > kMaxDepthN) {
           L.bits[i] >>= (L.depth[i] - kMaxDepthN);
@@ -326,7 +331,8 @@ int b12x_tree_load(const char* fileName, int dim, b12x_tree* out) {
     mergeEqual(L);
     b12x_tree t = nullptr;
     check(b12x_tree_create(dim, center.data(), radius.data(), sd.empty() ? nullptr : sd.data(), int(sd.size()), &t));
-    const int rc = b12x_tree_upload(t, uint32_t(L.bits.size()), L.bits.data(), L.depth.data(), L.flags.data());
+    int rc = b12x_tree_upload(t, uint32_t(L.bits.size()), L.bits.data(), L.depth.data(), L.flags.data());
+    if (rc == B12X_OK) rc = b12x_set_host_buffer_bytes(t, hostBytes);
     if (rc != B12X_OK) {
       const std::string msg = b12x::lastError();
       b12x_tree_destroy(t);

@@ -259,10 +259,99 @@ class ImplicitBoxTree {
     detail::check(b12x_tree_save_gaio(h_, fileName.c_str()), "saveForGaio");
   }
 
-  ImplicitBoxTree(const ImplicitBoxTree&) = delete;
-  ImplicitBoxTree& operator=(const ImplicitBoxTree&) = delete;
+  // ImplicitBoxTree.h:46-47, 64-67: copies of the domain, scheme, leaves and host buffer size
+  ImplicitBoxTree(const ImplicitBoxTree& o) { copyFrom(o); }
+  ImplicitBoxTree& operator=(const ImplicitBoxTree& o) {
+    if (this != &o) {
+      if (h_) b12x_tree_destroy(h_);
+      h_ = nullptr;
+      copyFrom(o);
+    }
+    return *this;
+  }
   ~ImplicitBoxTree() {
     if (h_) b12x_tree_destroy(h_);
+  }
+
+  // ---- domain and bookkeeping (ImplicitBoxTree.h:82-85, 141-143, 244, 550-569)
+  Vector<REAL> getCenter() const { return geometry(0); }
+  Vector<REAL> getRadius() const { return geometry(1); }
+  Vector<Dimension> getSdScheme() const {
+    int dim = 0, nsd = 0;
+    uint8_t sd[65];
+    detail::check(b12x_tree_geometry(h_, &dim, nullptr, nullptr, sd, &nsd), "getSdScheme");
+    return Vector<Dimension>(sd, sd + nsd);
+  }
+  // setCenter / setRadius: cut to DIM entries or extended with 0.5 (ImplicitBoxTree.hpp:3470-3505)
+  template <typename REAL2>
+  Vector<REAL> setCenter(const Vector<REAL2>& center) {
+    double c[DIM];
+    for (int j = 0; j < DIM; ++j) c[j] = j < int(center.size()) ? double(center[j]) : 0.5;
+    detail::check(b12x_tree_set_domain(h_, c, nullptr, nullptr, 0, 0), "setCenter");
+    return getCenter();
+  }
+  template <typename REAL2>
+  Vector<REAL> setRadius(const Vector<REAL2>& radius) {
+    double r[DIM];
+    for (int j = 0; j < DIM; ++j) r[j] = j < int(radius.size()) ? double(radius[j]) : 0.5;
+    for (const auto& v : radius)
+      if (v < REAL2(0)) {
+        std::cout << "Warning: Input has negative values. Radius vector remains unchanged." << std::endl;
+        return getRadius();
+      }
+    detail::check(b12x_tree_set_domain(h_, nullptr, r, nullptr, 0, 0), "setRadius");
+    return getRadius();
+  }
+  // setSdScheme: repeated periodically, the default scheme when empty (ImplicitBoxTree.hpp:3513-3552)
+  template <typename UInt>
+  Vector<Dimension> setSdScheme(const Vector<UInt>& sdScheme) {
+    std::vector<uint8_t> sd;
+    for (const auto& v : sdScheme) {
+      if (!(v >= 0 && v < UInt(DIM))) {
+        std::cout << "Warning: Input has values >= DIM. SdScheme remains unchanged." << std::endl;
+        return getSdScheme();
+      }
+      sd.push_back(uint8_t(v));
+    }
+    detail::check(b12x_tree_set_domain(h_, nullptr, nullptr, sd.empty() ? nullptr : sd.data(), int(sd.size()), 1),
+                  "setSdScheme");
+    return getSdScheme();
+  }
+  bool isSdSchemeDefault() const {
+    const Vector<Dimension> sd = getSdScheme();
+    for (size_t i = 0; i < sd.size(); ++i)
+      if (sd[i] != Dimension(i % DIM)) return false;
+    return true;
+  }
+  // sdCount_ (adaptSdCount, ImplicitBoxTree.hpp:574-587): (64 + 2) x DIM bisection counts
+  Vector<Depth> getSdCount() const {
+    const Vector<Dimension> sd = getSdScheme();
+    Vector<Depth> c(size_t(64 + 2) * DIM, 0);
+    for (size_t i = 0; i < sd.size(); ++i) {
+      for (int j = 0; j < DIM; ++j) c[(i + 1) * DIM + j] = c[i * DIM + j];
+      ++c[(i + 1) * DIM + sd[i]];
+    }
+    return c;
+  }
+  bool areAllLeavesOnDepth(Depth depth) const {
+    int r = 0;
+    detail::check(b12x_all_leaves_on_depth(h_, depth, &r), "areAllLeavesOnDepth");
+    return r != 0;
+  }
+  template <Architecture A2, Dimension DIM2, typename REAL2, Depth N2>
+  bool hasSameBoxes(const ImplicitBoxTree<A2, DIM2, REAL2, N2>& other) const {
+    int r = 0;
+    detail::check(b12x_has_same_boxes(h_, other.handle(), &r), "hasSameBoxes");
+    return r != 0;
+  }
+  uint64_t getNHostBufferBytes() const {
+    uint64_t v = 0;
+    detail::check(b12x_get_host_buffer_bytes(h_, &v), "getNHostBufferBytes");
+    return v;
+  }
+  uint64_t setNHostBufferBytes(uint64_t n) {
+    detail::check(b12x_set_host_buffer_bytes(h_, n), "setNHostBufferBytes");
+    return getNHostBufferBytes();
   }
 
   // ---- queries
@@ -395,10 +484,24 @@ class ImplicitBoxTree {
   NrBoxes unsetFlagsBySearch(const Vector<NrBoxes>& searchVector, Flags flags, Depth depth = getLeafDepth()) {
     return stencilOp(1, searchStencil(searchVector, depth).begin(), flags, 0, depth);
   }
-  // setFlagsByPoints(points {x0..x_{DIM-1}; ...}, flags, depth) — ImplicitBoxTree.h:474-475
+  NrBoxes changeFlagsBySearch(const Vector<NrBoxes>& searchVector, Flags flagsToUnset, Flags flagsToSet,
+                              Depth depth = getLeafDepth()) {
+    return stencilOp(2, searchStencil(searchVector, depth).begin(), flagsToUnset, flagsToSet, depth);
+  }
+  // setFlagsByPoints(points {x0..x_{DIM-1}; ...}, flags, depth) — ImplicitBoxTree.h:474-475, and the
+  // unset / change forms (ImplicitBoxTree.h:100-101, 534-535)
   NrBoxes setFlagsByPoints(const Vector<REAL>& points, Flags flags, Depth depth = getLeafDepth()) {
     Vector<NrBoxes> s = searchByPoints(points, depth);
     return stencilOp(0, searchStencil(s, depth).begin(), flags, 0, depth);
+  }
+  NrBoxes unsetFlagsByPoints(const Vector<REAL>& points, Flags flags, Depth depth = getLeafDepth()) {
+    Vector<NrBoxes> s = searchByPoints(points, depth);
+    return stencilOp(1, searchStencil(s, depth).begin(), flags, 0, depth);
+  }
+  NrBoxes changeFlagsByPoints(const Vector<REAL>& points, Flags flagsToUnset, Flags flagsToSet,
+                              Depth depth = getLeafDepth()) {
+    Vector<NrBoxes> s = searchByPoints(points, depth);
+    return stencilOp(2, searchStencil(s, depth).begin(), flagsToUnset, flagsToSet, depth);
   }
 
   // ---- point iterators (ImplicitBoxTree.h:266-430)
@@ -409,6 +512,10 @@ class ImplicitBoxTree {
   template <typename Grid, typename Map>
   PointIteratorPair makePointIteratorPairOverRoot(Grid grid, Map map, Flags flagsToSet = Flag::NONE) {
     return makePair(grid, map, 0, Flag::NONE);
+  }
+  template <typename Grid, typename Map>
+  PointIteratorPair makePointIteratorPairOverDepth(Depth depth, Grid grid, Map map, Flags flagsToSet = Flag::NONE) {
+    return makePair(grid, map, depth, Flag::NONE);
   }
   template <typename Grid, typename Map>
   PointIteratorPair makePointIteratorPairByFlags(Depth depth, Flags flagsToCheck, Grid grid, Map map,
@@ -475,6 +582,14 @@ class ImplicitBoxTree {
     return res;
   }
   template <typename T, typename Grid, typename Map>
+  CooMatrix<A, T> transitionMatrixForDepth(Depth depth, Grid grid, Map map, bool useAbsoluteValues = false,
+                                           bool useRowMajor = false, bool useInvalidNrBoxes = false) {
+    CooMatrix<A, T> res;
+    transitionMatrix<T>(makePointIteratorPairOverDepth(depth, grid, map), depth, grid.getNumberOfPointsPerBox(),
+                        useAbsoluteValues, useRowMajor, useInvalidNrBoxes, res);
+    return res;
+  }
+  template <typename T, typename Grid, typename Map>
   CooMatrix<A, T> transitionMatrixForRoot(Grid grid, Map map, bool useAbsoluteValues = false,
                                           bool useRowMajor = false, bool useInvalidNrBoxes = false) {
     CooMatrix<A, T> res;
@@ -533,16 +648,38 @@ class ImplicitBoxTree {
     return PointIteratorPair{holder, depth, flagsToCheck, grid.getNumberOfPointsPerBox()};
   }
 
+  Vector<REAL> geometry(int which) const {
+    int dim = 0;
+    double c[4], r[4];
+    detail::check(b12x_tree_geometry(h_, &dim, c, r, nullptr, nullptr), "geometry");
+    const double* v = which == 0 ? c : r;
+    return Vector<REAL>(v, v + DIM);
+  }
+
+  void copyFrom(const ImplicitBoxTree& o) {
+    const Vector<Dimension> sd = o.getSdScheme();
+    std::vector<uint8_t> s(sd.begin(), sd.end());
+    const bool dflt = s.size() == 64;  // the default scheme holds 64 entries, a given one 65
+    create(o.getCenter(), o.getRadius(), dflt ? nullptr : s.data(), dflt ? 0 : int(s.size()));
+    std::vector<uint64_t> bits;
+    std::vector<Depth> depths;
+    std::vector<Flags> flags;
+    o.getLeaves(bits, depths, flags);
+    detail::check(b12x_tree_upload(h_, uint32_t(bits.size()), bits.data(), depths.data(), flags.data()), "copy");
+    setNHostBufferBytes(o.getNHostBufferBytes());
+  }
+
+  // a stencil holds one entry per box of depth `depth` (count(depth))
   template <typename BoolIterator>
   NrBoxes stencilOp(int op, BoolIterator st, Flags a, Flags b, Depth depth) {
-    const NrBoxes n = (depth == 0) ? 1 : size();
+    const NrBoxes n = count(depth);
     std::vector<uint8_t> s(n);
     for (NrBoxes i = 0; i < n; ++i, ++st) s[i] = *st ? 1 : 0;
     return ret([&](uint32_t* r) { return b12x_flags_by_stencil(h_, op, s.data(), a, b, depth, r); }, "flagsByStencil");
   }
 
   std::vector<uint8_t> searchStencil(const Vector<NrBoxes>& sv, Depth depth) {
-    std::vector<uint8_t> s((depth == 0) ? 1 : size(), 0);
+    std::vector<uint8_t> s(count(depth), 0);
     for (NrBoxes i : sv)
       if (i != getInvalidNrBoxes() && i < s.size()) s[i] = 1;
     return s;

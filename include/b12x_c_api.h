@@ -84,6 +84,22 @@ int b12x_tree_destroy(b12x_tree t);
  * 27-33,3514-3545 size it; sd_scheme holds at least 65): the getters behind
  * ImplicitBoxTree::save (ImplicitBoxTree.hpp:460-486). Any output may be NULL. */
 int b12x_tree_geometry(b12x_tree t, int* dim, double* center, double* radius, uint8_t* sd_scheme, int* n_sd_out);
+/* setCenter / setRadius / setSdScheme — ImplicitBoxTree.h:550-560, .hpp:3464-3552. center and
+ * radius (dim entries each) may be NULL to keep them; a negative radius is rejected (the tree is
+ * unchanged). With set_sd != 0 the scheme becomes sd_scheme (n_sd entries repeated periodically,
+ * or the default scheme when n_sd is 0); values >= dim are rejected. The leaves stay. */
+int b12x_tree_set_domain(b12x_tree t, const double* center, const double* radius, const uint8_t* sd_scheme, int n_sd,
+                         int set_sd);
+/* areAllLeavesOnDepth(depth) — ImplicitBoxTree.h:84-85, .hpp:589-609 */
+int b12x_all_leaves_on_depth(b12x_tree t, uint8_t depth, int* out);
+/* a.hasSameBoxes(b) — ImplicitBoxTree.h:141-143, .hpp:1299-1381: same dimension, centre, radius,
+ * leaves (a's leaves against b's first leaves, as the reference compares them) and scheme up to
+ * a's depth(). */
+int b12x_has_same_boxes(b12x_tree a, b12x_tree b, int* out);
+/* getNHostBufferBytes / setNHostBufferBytes — ImplicitBoxTree.h:568-569: kept and written to the
+ * tree's files (default 4 GiB); trees live in HBM whole, so it limits nothing here. */
+int b12x_get_host_buffer_bytes(b12x_tree t, uint64_t* out);
+int b12x_set_host_buffer_bytes(b12x_tree t, uint64_t bytes);
 /* size(), count(depth), depth() — ImplicitBoxTree.h:110-111,129-130,603 */
 int b12x_size(b12x_tree t, uint32_t* out);
 int b12x_count(b12x_tree t, uint8_t depth, uint32_t* out);

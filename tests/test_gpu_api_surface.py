@@ -1,0 +1,121 @@
+"""The drop-in surface beyond the hot path: domain getters and setters, the subdivision scheme's
+bookkeeping, areAllLeavesOnDepth, hasSameBoxes, copies, the host buffer size kept in files, and
+the by-points / by-search flag forms (ImplicitBoxTree.h:46-108, 141-143, 244, 472-480, 532-569).
+The C++ façade's program (tests/cpp/api_surface.cpp) must print what the Python mirror computes,
+and the flag forms must leave the oracle's tree."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import b12_amd as gx
+from helpers import same_tree
+from oracle_api import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+PTS = np.array([[0.1, 0.05], [-0.7, 0.2], [0.9, -0.3], [5.0, 5.0], [0.12, 0.051]])
+
+
+def stencil(n, search):
+    st = np.zeros(n, np.uint8)
+    s = np.asarray(search)
+    s = s[(s != gx.INVALID) & (s < n)]
+    st[s] = 1
+    return st
+
+
+def fmt(name, v):
+    return name + "".join(f" {float(x):.17g}" for x in v)
+
+
+def test_cpp_facade_api_surface_matches_python_and_oracle(tmp_path):
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "api_surface")
+    assert os.path.exists(exe), "tests/cpp/api_surface not built (run __graft_entry__.build())"
+    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout.splitlines()
+    out = [l for l in out if not l.startswith("Warning:")]
+    grid, mp = gx.InnerGrid(4), gx.Henon()
+    t = gx.ImplicitBoxTree([0.0, 0.0], [1.5, 0.5])
+    o = OracleTree([0.0, 0.0], [1.5, 0.5])
+    for x in (t, o):
+        x.setFlags(gx.SD, 0)
+        x.subdivision_steps(grid, mp, 10)
+    exp = [fmt("center", t.getCenter()), fmt("radius", t.getRadius()), f"default {int(t.isSdSchemeDefault())}",
+           f"alldepth {int(t.areAllLeavesOnDepth(gx.LEAF))} {int(t.areAllLeavesOnDepth(10))} "
+           f"{int(t.areAllLeavesOnDepth(254))}"]
+    u = t.copy()
+    exp.append(f"same {int(t.hasSameBoxes(u))}")
+    r = t.setFlagsByPoints(PTS, 0x10)
+    assert r == o.setFlagsByStencil(stencil(o.count(-1), o.searchByPoints(PTS, -1)), 0x10, -1)
+    exp.append(f"setByPoints {r}")
+    r = t.changeFlagsByPoints(PTS, 0x10, 0x20, 6)
+    assert r == o.changeFlagsByStencil(stencil(o.count(6), o.searchByPoints(PTS, 6)), 0x10, 0x20, 6)
+    exp.append(f"changeByPoints {r}")
+    r = t.unsetFlagsByPoints(PTS, 0x20, 6)
+    assert r == o.unsetFlagsByStencil(stencil(o.count(6), o.searchByPoints(PTS, 6)), 0x20, 6)
+    exp.append(f"unsetByPoints {r}")
+    exp.append(f"same {int(t.hasSameBoxes(u))}")
+    s = t.searchCompacted(t.makePointIteratorPairOverDepth(7, grid, mp), 7)
+    assert np.array_equal(s, o.searchCompacted(o.makePointIteratorPairOverDepth(7, grid, mp), 7))
+    r = t.changeFlagsBySearch(s, 0x00, 0x40, 7)
+    assert r == o.changeFlagsByStencil(stencil(o.count(7), s), 0x00, 0x40, 7)
+    exp.append(f"changeBySearch {r}")
+    r = t.unsetFlagsBySearch(s, 0x40, 7)
+    assert r == o.unsetFlagsByStencil(stencil(o.count(7), s), 0x40, 7)
+    exp.append(f"unsetBySearch {r}")
+    assert same_tree(t.download(), o.download())
+    m = t.transitionMatrixForDepth(7, grid, mp)
+    exp.append(f"tm7 {m.nnz} {m.nRows} {m.nColumns}")
+    exp.append(f"hostbytes {t.getNHostBufferBytes()} {t.setNHostBufferBytes(1 << 20)}")
+    exp.append(fmt("sd", t.setSdScheme([1, 0, 0])))
+    exp.append(fmt("sdcount", t.getSdCount()))
+    exp.append(f"default {int(t.isSdSchemeDefault())}")
+    exp.append(fmt("radius", t.setRadius([-1.0, 2.0])))
+    exp.append(fmt("center", t.setCenter([0.25])))
+    exp.append(f"same {int(t.hasSameBoxes(u))}")
+    w = t.copy()
+    exp.append(f"same {int(w.hasSameBoxes(t))} {w.getNHostBufferBytes()}")
+    assert out == exp
+    # the values the reference fixes
+    assert exp[2] == "default 1" and exp[3] == "alldepth 1 1 0" and exp[4] == "same 1"
+    sd = t.getSdScheme()
+    assert len(sd) == 65 and list(sd[:6]) == [1, 0, 0, 1, 0, 0]
+    assert list(t.getCenter()) == [0.25, 0.5] and list(t.getRadius()) == [1.5, 0.5]
+    # the stored host buffer size goes to the file and comes back
+    path = str(tmp_path / "tree")
+    t.save(path)
+    assert '"nHostBufferBytes": 1048576' in open(path + ".json").read()
+    assert gx.ImplicitBoxTree.load(path, 2).getNHostBufferBytes() == 1 << 20
+
+
+def test_has_same_boxes_and_domain_setters():
+    """hasSameBoxes against a changed flag, leaf, centre or scheme; a tree with more leaves whose
+    first leaves match counts as the same (the reference compares this tree's leaves only); a
+    new scheme changes where points fall while the leaves stay."""
+    rng = np.random.default_rng(1501)
+    a = gx.ImplicitBoxTree([0.0, 0.0], [1.0, 1.0])
+    a.setFlags(gx.SD, 0)
+    for _ in range(6):
+        a.subdivide(gx.SD)
+    b = a.copy()
+    assert a.hasSameBoxes(b) and b.hasSameBoxes(a)
+    b.setFlagsByStencil(np.eye(1, b.size(), 5, dtype=np.uint8)[0], 0x40)
+    assert not a.hasSameBoxes(b)
+    c = a.copy()
+    c.setCenter([0.0, 1e-300])
+    assert not a.hasSameBoxes(c)
+    bits, depth, flags = a.download()
+    d = gx.ImplicitBoxTree([0.0, 0.0], [1.0, 1.0])
+    d.upload(bits[:-1], depth[:-1], flags[:-1])
+    assert d.hasSameBoxes(a) and not a.hasSameBoxes(d)
+    e = a.copy()
+    e.setSdScheme([1, 0])
+    assert not a.hasSameBoxes(e)  # schemes differ within depth() = 6
+    pts = rng.uniform(-1, 1, (200, 2))
+    o = OracleTree([0.0, 0.0], [1.0, 1.0], [1, 0])
+    o.upload(bits, depth, flags)
+    assert np.array_equal(e.searchByPoints(pts), o.searchByPoints(pts))
+    assert e.getSdScheme().tolist() == [[1, 0][i % 2] for i in range(65)]
+    e.setSdScheme([])
+    assert e.isSdSchemeDefault() and a.hasSameBoxes(e)

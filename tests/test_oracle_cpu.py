@@ -637,7 +637,7 @@ def test_library_exports_every_declared_symbol(flavour):
 
 
 def test_cpp_facade_builds():
-    for name in ("doc_example", "continuation"):
+    for name in ("doc_example", "continuation", "api_surface"):
         exe = os.path.join(ROOT, "tests", "cpp", name)
         assert os.path.exists(exe), f"façade example {name} not built"
 
