@@ -136,6 +136,11 @@ struct b12x_coo_s {
 namespace {
 
 constexpr int kScal = 32;
+// pinned host words: a mirror of the kScal device scalars, then the log of a batch of captured
+// subdivision steps (4 counts per step)
+constexpr int kStepBatch = 16;
+constexpr int kHostScal = kScal + 4 * kStepBatch;
+constexpr int kScalStepLog = 25;  // device slot: steps logged in the current batch
 constexpr int kMaxScanTiles = 1 << 24;
 
 void syncScal(b12x_tree_s& T, int count = kScal) {
@@ -701,6 +706,21 @@ void enqueueStep(b12x_tree_s& T, b12x_scheme_s& S, uint8_t sd, uint32_t boundN, 
   T.cur = 1 - T.cur;
 }
 
+// The end of a captured step: appends the step's counts (scal[0..3]) to the pinned log and, when
+// the step kept no leaf, writes the surrounding box as the only leaf (ImplicitBoxTree.hpp:
+// 1959-1970) and makes the next step read a count of 1.
+__global__ void stepLogKernel(uint32_t* scal, uint32_t* hostLog, TreeView v, uint64_t rootKey) {
+  const uint32_t i = scal[kScalStepLog];
+  scal[kScalStepLog] = i + 1;
+  for (int j = 0; j < 4; ++j) hostLog[4 * i + j] = scal[j];
+  if (scal[2] == 0) {
+    v.key[0] = rootKey;
+    v.depth[0] = 0;
+    v.flags[0] = 0;
+    scal[2] = 1;
+  }
+}
+
 // the leaf count where a captured step reads it (the keep of every step leaves it there too)
 void stageLeafCount(b12x_tree_s& T) {
   T.hscal[kScalStepIn] = T.n;
@@ -888,7 +908,7 @@ int b12x_tree_create(int dim, const double* center, const double* radius, const 
     try {
       streamCreate(&T->stream);
       T->scal = dalloc<uint32_t>(kScal);
-      HIPCHK(hipHostMalloc((void**)&T->hscal, kScal * sizeof(uint32_t), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void**)&T->hscal, kHostScal * sizeof(uint32_t), hipHostMallocDefault));
       T->rootKey = dalloc<uint64_t>(1);
       T->rootDepth = dalloc<uint8_t>(1);
       const uint64_t rk = keyOf(0, 0);
@@ -990,12 +1010,12 @@ int b12x_has_same_boxes(b12x_tree a, b12x_tree b, int* out) {
     for (int i = 0; i < int(dA) && i <= kMaxDepth; ++i)
       if (a->dom.sdScheme[i] != b->dom.sdScheme[i]) return B12X_OK;
     HIPCHK(hipStreamSynchronize(b->stream));
-    HIPCHK(hipMemsetAsync(a->scal + 20, 0, 4, a->stream));
+    HIPCHK(hipMemsetAsync(a->scal + 24, 0, 4, a->stream));
     hipLaunchKernelGGL(sameLeavesKernel, dim3(ceilDiv(a->n, kBlock)), dim3(kBlock), 0, a->stream, a->view(a->cur),
-                       b->view(b->cur), a->n, a->scal + 20);
+                       b->view(b->cur), a->n, a->scal + 24);
     HIPCHK(hipGetLastError());
-    syncScal(*a, 21);
-    *out = a->hscal[20] == 0 ? 1 : 0;
+    syncScal(*a, 25);
+    *out = a->hscal[24] == 0 ? 1 : 0;
     return B12X_OK;
   });
 }
@@ -1733,7 +1753,7 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
     // Device scalars: [0] leaves after subdivide, [1] #leaves with sd, [2] leaves kept,
     // [3] #leaves newly hit. One host synchronisation per step.
     if (graphs && nSteps > 0) stageLeafCount(T);
-    for (int step = 0; step < nSteps; ++step) {
+    for (int step = 0; step < nSteps;) {
       const uint32_t n = T.n;
       const uint64_t sub = uint64_t(n) * 2;
       const StepForm form = !graphs ? STEP_EAGER
@@ -1741,18 +1761,24 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
                             : sub <= kStepGraphLeaves ? STEP_LARGE
                                                       : STEP_EAGER;
       if (form != STEP_EAGER) {
-        // replay the captured step; the leaf count chains through scal[2] on the device and the
-        // buffers end in the same half after a step
+        // replay the captured step K times back to back: the leaf count chains through scal[2]
+        // on the device and the buffers end in the same half after a step, so the steps need no
+        // host input; the leaves can at most double per step, so K keeps every step's
+        // subdivided count within the form's bound
+        const uint64_t bound = form == STEP_SMALL ? kStepSmallLeaves : kStepGraphLeaves;
+        int K = 1;
+        while (K < kStepBatch && step + K < nSteps && (uint64_t(n) << (K + 1)) <= bound) ++K;
         ensureCap(T, kStepGraphLeaves);
         b12x_tree_s::StepGraph& G = T.step[form == STEP_SMALL ? 1 : 0];
         if (!G.exec || G.scheme != s->id || G.sd != sd || G.gen != T.bufGen || G.cur != T.cur) {
           G.drop();
-          const uint32_t bound = (form == STEP_SMALL ? kStepSmallLeaves : kStepGraphLeaves) / 2;
+          const uint32_t gb = uint32_t(bound / 2);
           HIPCHK(hipStreamBeginCapture(T.stream, hipStreamCaptureModeThreadLocal));
           try {
-            enqueueStep(T, *s, sd, bound, T.scal + 2, form);
-            if (form == STEP_LARGE)
-              HIPCHK(hipMemcpyAsync(T.hscal, T.scal, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, T.stream));
+            enqueueStep(T, *s, sd, gb, T.scal + 2, form);
+            hipLaunchKernelGGL(stepLogKernel, dim3(1), dim3(1), 0, T.stream, T.scal, T.hscal + kScal,
+                               T.view(T.cur), keyOf(0, 0));
+            HIPCHK(hipGetLastError());
           } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(T.stream, &g);
@@ -1766,24 +1792,42 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
           G.gen = T.bufGen;
           G.cur = T.cur;
         }
-        HIPCHK(hipGraphLaunch(G.exec, T.stream));
+        hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, T.stream, T.scal + kScalStepLog, uint64_t(1), 0u);
+        HIPCHK(hipGetLastError());
+        for (int k = 0; k < K; ++k) HIPCHK(hipGraphLaunch(G.exec, T.stream));
         HIPCHK(hipStreamSynchronize(T.stream));
+        uint32_t kept = 0;
+        for (int k = 0; k < K; ++k) {
+          const uint32_t* L = T.hscal + kScal + 4 * k;
+          const uint32_t nSub = L[0], cSub = L[1], cSet = L[3];
+          kept = L[2];
+          if (counts) {
+            counts[4 * (step + k) + 0] = cSub;
+            counts[4 * (step + k) + 1] = cSet;
+            counts[4 * (step + k) + 2] = nSub - kept;
+            counts[4 * (step + k) + 3] = kept;  // every kept leaf had HIT, which unsetFlags clears
+          }
+        }
+        T.n = kept ? kept : 1;  // the log kernel put the surrounding box in an empty tree
+        ++T.leafGen;
+        step += K;
       } else {
         ensureCap(T, sub);
         enqueueStep(T, *s, sd, n, nullptr, STEP_EAGER);
         syncScal(T, 4);
-      }
-      const uint32_t nSub = T.hscal[0], cSub = T.hscal[1], kept = T.hscal[2], cSet = T.hscal[3];
-      if (T.timing) T.imagePoints += uint64_t(nSub) * s->grid.k;
-      T.n = kept;
-      ++T.leafGen;
-      insertRootIfEmpty(T);
-      if (graphs && T.n != kept) stageLeafCount(T);
-      if (counts) {
-        counts[4 * step + 0] = cSub;
-        counts[4 * step + 1] = cSet;
-        counts[4 * step + 2] = nSub - kept;
-        counts[4 * step + 3] = kept;  // every kept leaf had HIT, which unsetFlags clears
+        const uint32_t nSub = T.hscal[0], cSub = T.hscal[1], kept = T.hscal[2], cSet = T.hscal[3];
+        if (T.timing) T.imagePoints += uint64_t(nSub) * s->grid.k;
+        T.n = kept;
+        ++T.leafGen;
+        insertRootIfEmpty(T);
+        if (graphs && T.n != kept) stageLeafCount(T);
+        if (counts) {
+          counts[4 * step + 0] = cSub;
+          counts[4 * step + 1] = cSet;
+          counts[4 * step + 2] = nSub - kept;
+          counts[4 * step + 3] = kept;  // every kept leaf had HIT, which unsetFlags clears
+        }
+        ++step;
       }
     }
     return B12X_OK;
