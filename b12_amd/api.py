@@ -78,6 +78,7 @@ SIGNATURES = {
     "b12x_coo_upload": [_P, _P, _P, _U64, _U32, _U32, _P],
     "b12x_compress_index_vector": [_P, _U64, _U32, _P],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
+    "b12x_power_iteration_kernel_ms": [_P],
     "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
     "b12x_coo_scc": [_P, _P, _P, _P],
     "b12x_scc_last_stats": [_P, _P, _P],
@@ -149,6 +150,13 @@ def bench_copy(nbytes, reps=20, flavour="parity"):
     ms = _D()
     _Api(flavour)("b12x_bench_copy", int(nbytes), int(reps), ctypes.byref(ms))
     return ms.value
+
+
+def power_iteration_kernel_ms(flavour="parity"):
+    """Duration (ms) of this thread's last persistent power-iteration launch (0: batched form)."""
+    v = _D()
+    _Api(flavour)("b12x_power_iteration_kernel_ms", ctypes.byref(v))
+    return v.value
 
 
 def scc_last_stats(flavour="parity"):

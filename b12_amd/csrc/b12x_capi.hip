@@ -2259,6 +2259,8 @@ void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* b
 // Blocks for the persistent power iteration over n columns: one lane per column up to
 // kCoopMaxBlocks blocks, never more than the device holds resident at once, so that on an idle
 // device every block runs from the start (0: use the batched form).
+thread_local double lastPiKernelMs = 0.0;  // duration of the calling thread's last persistent launch
+
 uint32_t coopGrid(uint32_t n) {
   int dev = 0, cus = 0, occ = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -2274,8 +2276,15 @@ uint32_t coopGrid(uint32_t n) {
 
 extern "C" {
 
+int b12x_power_iteration_kernel_ms(double* ms) {
+  if (!ms) return setErr(B12X_E_ARG, "null argument");
+  *ms = lastPiKernelMs;
+  return B12X_OK;
+}
+
 int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double* mu, int* iters, double* resid) {
   return guarded([&]() -> int {
+    lastPiKernelMs = 0.0;
     if (!m || !mu) return setErr(B12X_E_ARG, "null argument");
     if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
     const uint32_t n = m->nRows;
@@ -2346,12 +2355,22 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
       HIPCHK(hipGetLastError());
       PiCoopArgs ca{colPtr, cr, vals, mu0, ybuf, part, part + uint64_t(coopBlocks) * 6, pst, err, n,
                     uint32_t(std::max(maxIter, 0)), tol, 1u << 21};
+      hipEvent_t e0 = nullptr, e1 = nullptr;  // the launch's duration, for measurement
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipEventRecord(e0, st));
       if (coopBlocks <= 4 * kWave) hipLaunchKernelGGL(piCoopKernel<4>, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
       else hipLaunchKernelGGL(piCoopKernel<kCoopMaxBlocks / kWave>, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
       HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(e1, st));
       int herr = 0;
       HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
+      float kms = 0.f;
+      HIPCHK(hipEventElapsedTime(&kms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      lastPiKernelMs = kms;
       dfree(part); dfree(err);
       // a wait past its limit (other work holding the CUs): every block has left; recompute in
       // batches (B12X_PI_PERSISTENT=2: report it instead)

@@ -46,15 +46,15 @@ def images_of(counts, n0=1):
     return imgs, n
 
 
-def profiled_traffic():
-    """HBM bytes per image-kernel launch from the committed FETCH_SIZE/WRITE_SIZE profile
-    (scaled by the copy-kernel calibration), or None."""
+def profiled_traffic(kernel="imageKernel<3,LORENZ,INNER>"):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE profile (scaled
+    by the copy-kernel calibration), or None."""
     import re
     try:
         text = open(TRAFFIC_PROFILE).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch\(imageKernel<3,LORENZ,INNER>\):\s*(\d+)", text)
+    m = re.search(r"traffic_bytes_per_launch\(" + re.escape(kernel) + r"\):\s*(\d+)", text)
     return int(m.group(1)) if m else None
 
 
@@ -71,15 +71,27 @@ def gpu_job(gx, flavour, timing=False):
     t2 = time.perf_counter()
     mu, iters, resid = m.power_iteration(MAX_ITER, TOL)
     t3 = time.perf_counter()
+    pi_ms = gx.api.power_iteration_kernel_ms()
     imgs, n = images_of(counts)
     imgs += K * n
-    res = {"images": imgs, "boxes": n, "nnz": m.nnz, "iters": iters, "resid": resid,
+    res = {"images": imgs, "boxes": n, "nnz": m.nnz, "iters": iters, "resid": resid, "pi_kernel_ms": pi_ms,
            "boxes_stepped": (imgs - K * n) // K,
            "subdiv_ms": (t1 - t0) * 1e3, "tm_ms": (t2 - t1) * 1e3, "measure_ms": (t3 - t2) * 1e3,
            "level24_points": K * (int(counts[-2][3]) + int(counts[-1][0])) if DEPTH > 1 else 0}
     if timing:
         res["image_timing"] = t.image_timing(0)
     return res
+
+
+def measure_roofline(job):
+    n, nnz, iters, ms = job["boxes"], job["nnz"], job["iters"], job["pi_kernel_ms"]
+    alg = iters * (12 * nnz + 16 * n)
+    achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "latency (one global reduction per iteration); reported against HBM",
+            "kernel": "piCoopKernel", "achieved": achieved, "peak": PEAK_HBM_BYTES / 1e9, "unit": "GB/s",
+            "frac": achieved * 1e9 / PEAK_HBM_BYTES, "traffic": profiled_traffic("piCoopKernel"),
+            "launch_ms": ms, "iterations": iters, "us_per_iteration": ms * 1e3 / iters if iters else None,
+            "algorithmic_bytes": alg}
 
 
 HENON_C, HENON_R = [0.0, 0.0], [1.5, 0.5]
@@ -416,6 +428,11 @@ def main():
                      "flops_per_image": LORENZ_FLOPS_PER_IMAGE, "image_launches": launches,
                      "image_points": pts, "image_ms": ms_img, "traffic": profiled_traffic(),
                      "traffic_source": "profiles/lorenz_d24_job_traffic.md (FETCH_SIZE x2.000 + WRITE_SIZE x1.000)"},
+        # the job's longest kernel: the persistent power iteration, one launch per solve; its
+        # algorithmic bytes per iteration are 12 per nonzero (row index, value) and 16 per column
+        # (column offset, y read, y written, the norm's share), its bound is the per-iteration
+        # global reduction (latency), so the fraction of HBM bandwidth stays low by design
+        "roofline_measure": measure_roofline(runs[-1]),
         "boxes": {"boxes_per_s": runs[-1]["boxes_stepped"] / (runs[-1]["subdiv_ms"] * 1e-3),
                   "bytes_per_box_step": BYTES_PER_BOX_STEP,
                   "hbm_frac": runs[-1]["boxes_stepped"] * BYTES_PER_BOX_STEP / (runs[-1]["subdiv_ms"] * 1e-3)

@@ -235,6 +235,10 @@ int b12x_compress_index_vector(const uint32_t* indices, uint64_t n, uint32_t dim
  * until max_iter iterations or ||mu_{k+1} - mu_k||_1 <= tol (tol < 0: fixed count). The
  * reference computes this step in MATLAB (subdivisionByNim.m:30-59); there is no C++ API. */
 int b12x_power_iteration(b12x_coo m, int max_iter, double tol, int mu_init, double* mu, int* iters, double* resid);
+/* Measurement (no counterpart in the reference): the duration of the calling thread's last
+ * persistent power-iteration launch, from HIP events on the matrix's stream (0 when the last
+ * call ran the batched form). */
+int b12x_power_iteration_kernel_ms(double* ms);
 /* Subspace iteration for the nvec (<= 8) leading eigenvalues (no counterpart in the reference,
  * which leaves eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59): X_0 = [1/n, 3 fixed
  * splitmix64 columns], then `iters` times X <- cholQR2(X P) (row-vector convention, as the power
