@@ -145,3 +145,41 @@ def test_cpp_facade_continuation():
     o = OracleTree([0.0, 0.0], [3.0, 3.0])
     exp = gx.continuation(o, gx.InnerGrid(4), gx.Henon(), depths, henon_fixed_point())
     assert out == [f"{d}: " + " ".join(str(c) for c in cs) for d, cs in zip(depths, exp)]
+
+
+@pytest.mark.parametrize("dim,sd,seed", [(2, [1, 0, 0], 41), (3, [2, 0, 1, 1], 42)])
+def test_insertion_and_delete_depth_with_a_given_scheme(dim, sd, seed):
+    """Trees with a given subdivision scheme: the boxes of inserted points, truncation by
+    deleteDepth and the merge follow the scheme's bisection order."""
+    rng = np.random.default_rng(960 + seed)
+    c, r = rng.uniform(-1, 1, dim), rng.uniform(0.5, 2, dim)
+    tree = random_tree(rng, dim, 1500, max_depth=15)
+    g, o = gx.ImplicitBoxTree(c, r, sd), OracleTree(c, r, sd)
+    g.upload(*tree)
+    o.upload(*tree)
+    dmax = int(tree[1].max())
+    for d in (255, 5, dmax, dmax + 4):
+        print(f"[insert] scheme {sd} byPoints d={d}", flush=True)
+        pts = rng.uniform(c - 1.1 * r, c + 1.1 * r, (3000, dim))
+        assert np.array_equal(g.insertByPoints(pts, d, 0x02, 0x40), o.insertByPoints(pts, d, 0x02, 0x40))
+        check(g, o, (sd, d))
+    for d in (9, 3):
+        assert g.deleteDepth(d) == o.deleteDepth(d)
+        check(g, o, (sd, "delete", d))
+
+
+def test_continuation_and_chain_recurrent_set_with_a_given_scheme():
+    grid, mp = gx.InnerGrid(4), gx.Henon()
+    sd = [1, 0, 0]
+    g, o = gx.ImplicitBoxTree([0.0, 0.0], [3.0, 3.0], sd), OracleTree([0.0, 0.0], [3.0, 3.0], sd)
+    for d in (12, 20):
+        print(f"[insert] continuation scheme {sd} d={d}", flush=True)
+        assert gx.continuation(g, grid, mp, [d], henon_fixed_point()) == \
+            gx.continuation(o, grid, mp, [d], henon_fixed_point())
+        check(g, o, d)
+    g, o = gx.ImplicitBoxTree([0.0, 0.0], [3.0, 3.0], sd), OracleTree([0.0, 0.0], [3.0, 3.0], sd)
+    exp = gx.chainRecurrentSet(o, grid, mp, 20)
+    g.setFlags(gx.SD, gx.LEAF)
+    got = g.chain_recurrent_steps(grid, mp, 20)
+    assert [tuple(int(v) for v in row) for row in got] == [tuple(e) for e in exp]
+    check(g, o, "chain")
