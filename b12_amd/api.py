@@ -77,6 +77,9 @@ SIGNATURES = {
     "b12x_coo_download": [_P, _P, _P, _P],
     "b12x_coo_upload": [_P, _P, _P, _U64, _U32, _U32, _P],
     "b12x_compress_index_vector": [_P, _U64, _U32, _P],
+    "b12x_coo_is_sorted": [_P, _I, _P],
+    "b12x_coo_sort": [_P, _I],
+    "b12x_coo_copy": [_P, _P],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_power_iteration_kernel_ms": [_P],
     "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
@@ -306,6 +309,64 @@ class CooMatrix:
         X = np.zeros((m, self.nRows), np.float64) if want_basis else None
         self.api("b12x_subspace_iteration", self.h, int(m), int(iters), _ptr(H), _ptr(X) if want_basis else None)
         return H, X
+
+    # ---- ordering and comparison (CooMatrix.h:61-91, CooMatrix.hpp:304-447)
+    def isSortedRowMajor(self):
+        r = _I()
+        self.api("b12x_coo_is_sorted", self.h, 1, ctypes.byref(r))
+        return bool(r.value)
+
+    def isSortedColumnMajor(self):
+        r = _I()
+        self.api("b12x_coo_is_sorted", self.h, 0, ctypes.byref(r))
+        return bool(r.value)
+
+    def sortRowMajor(self):
+        self.api("b12x_coo_sort", self.h, 1)
+        return self
+
+    def sortColumnMajor(self):
+        self.api("b12x_coo_sort", self.h, 0)
+        return self
+
+    def copy(self):
+        h = _P()
+        self.api("b12x_coo_copy", self.h, ctypes.byref(h))
+        return CooMatrix(h, self.api)
+
+    def _same_entries(self, other):
+        # the reference compares this matrix's entries with the first entries of the other
+        if other.nnz < self.nnz:
+            return False
+        a, b = self.download(), other.download()
+        return all(np.array_equal(x, y[: self.nnz]) for x, y in zip(a, b))
+
+    def __eq__(self, other):
+        """operator== (CooMatrix.hpp:304-352): the same sizes and entries, independent of the
+        orderings (an unsorted or differently sorted matrix is compared through a sorted copy)."""
+        if not isinstance(other, CooMatrix):
+            return NotImplemented
+        if (self.nRows, self.nColumns) != (other.nRows, other.nColumns):
+            return False
+        frm = self.isSortedRowMajor()
+        fcm = not frm and self.isSortedColumnMajor()
+        srm = other.isSortedRowMajor()
+        scm = not srm and other.isSortedColumnMajor()
+        if frm and not srm:
+            return self == other.copy().sortRowMajor()
+        if fcm and not scm:
+            return self == other.copy().sortColumnMajor()
+        if srm and not frm:
+            return self.copy().sortRowMajor() == other
+        if scm and not fcm:
+            return self.copy().sortColumnMajor() == other
+        if self._same_entries(other):
+            return True
+        if not ((frm and srm) or (fcm and scm)):
+            return self.copy().sortRowMajor() == other.copy().sortRowMajor()
+        return False
+
+    __hash__ = None
 
     def scc(self):
         """Strong components of the matrix's graph (b12x_coo_scc): per vertex the largest vertex
@@ -732,6 +793,27 @@ def continuation(ibt, grid, mp, depths, points, on_step=None):
                 on_step(d, len(counts) - 1, nb1)
         out.append(counts)
     return out
+
+
+def fixedBoxes(ibt, grid, mp, steps=64):
+    """Boxes that hit themselves (B12c/src/matlab/fixedBoxes.m:1-23): reset to the surrounding box,
+    SD on every leaf, then `steps` times (the tree's maximal depth in the reference): subdivide,
+    search the images of every leaf's test points, HIT on the leaves one of whose points lands in
+    the leaf itself, remove(HIT), unsetFlags(HIT). Returns the boxes' centres and radii, one row
+    per leaf (ibt.boxes without the flags). Any tree object with these method names works."""
+    ibt.deleteDepth(0)
+    ibt.setFlags(SD, LEAF)
+    for _ in range(steps):
+        ibt.subdivide(SD)
+        it = ibt.makePointIteratorPairOverLeaves(grid, mp)
+        s = ibt.searchByIterators(it, LEAF)
+        n = ibt.size()
+        own = s.reshape(n, -1) == np.arange(n, dtype=np.uint32)[:, None]
+        ibt.setFlagsByStencil(own.any(axis=1).astype(np.uint8), HIT)
+        ibt.remove(HIT)
+        ibt.unsetFlags(HIT, LEAF)
+    b = ibt.boxes(LEAF)
+    return b[:, : b.shape[1] - 1]
 
 
 def chainRecurrentSet(ibt, grid, mp, steps, on_step=None):

@@ -2140,9 +2140,6 @@ int b12x_compress_index_vector(const uint32_t* idx, uint64_t n, uint32_t dim, ui
   });
 }
 
-// ------------------------------------------------------------------------------------------------
-// Invariant measure
-// ------------------------------------------------------------------------------------------------
 }  // extern "C"
 
 namespace {
@@ -2155,6 +2152,155 @@ __global__ void gatherF64Kernel(const double* src, const uint32_t* idx, uint64_t
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) dst[i] = src[idx[i]];
 }
+
+__global__ void gatherU32Kernel(const uint32_t* src, const uint32_t* idx, uint64_t n, uint32_t* dst) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// entries out of order for (major, minor) ascending (equal neighbours are in order)
+__global__ void cooOrderKernel(const uint32_t* major, const uint32_t* minor, uint64_t n, uint32_t* bad) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i == 0 || i >= n) return;
+  if (major[i] < major[i - 1] || (major[i] == major[i - 1] && minor[i] < minor[i - 1])) atomicOr(bad, 1u);
+}
+
+// strictly ascending (no repeated entry): the order the matrix flags record
+__global__ void cooStrictKernel(const uint32_t* major, const uint32_t* minor, uint64_t n, uint32_t* bad) {
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i == 0 || i >= n) return;
+  if (!(major[i] > major[i - 1] || (major[i] == major[i - 1] && minor[i] > minor[i - 1]))) atomicOr(bad, 1u);
+}
+
+bool cooStrict(const b12x_coo_s& M, bool rowMajor) {
+  if (M.nnz < 2) return true;
+  uint32_t* bad = dalloc<uint32_t>(1);
+  HIPCHK(hipMemsetAsync(bad, 0, 4, M.stream));
+  hipLaunchKernelGGL(cooStrictKernel, dim3(ceilDiv(M.nnz, kBlock)), dim3(kBlock), 0, M.stream,
+                     rowMajor ? M.rows : M.cols, rowMajor ? M.cols : M.rows, M.nnz, bad);
+  HIPCHK(hipGetLastError());
+  uint32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, bad, 4, hipMemcpyDeviceToHost, M.stream));
+  HIPCHK(hipStreamSynchronize(M.stream));
+  dfree(bad);
+  return h == 0;
+}
+
+bool cooSorted(const b12x_coo_s& M, bool rowMajor) {
+  if (M.nnz < 2) return true;
+  uint32_t* bad = dalloc<uint32_t>(1);
+  HIPCHK(hipMemsetAsync(bad, 0, 4, M.stream));
+  hipLaunchKernelGGL(cooOrderKernel, dim3(ceilDiv(M.nnz, kBlock)), dim3(kBlock), 0, M.stream,
+                     rowMajor ? M.rows : M.cols, rowMajor ? M.cols : M.rows, M.nnz, bad);
+  HIPCHK(hipGetLastError());
+  uint32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, bad, 4, hipMemcpyDeviceToHost, M.stream));
+  HIPCHK(hipStreamSynchronize(M.stream));
+  dfree(bad);
+  return h == 0;
+}
+}  // namespace
+
+extern "C" {
+
+// CooMatrix::isSortedRowMajor / isSortedColumnMajor (CooMatrix.hpp:354-374)
+int b12x_coo_is_sorted(b12x_coo m, int rowMajor, int* out) {
+  return guarded([&]() -> int {
+    if (!m || !out) return setErr(B12X_E_ARG, "null argument");
+    *out = cooSorted(*m, rowMajor != 0) ? 1 : 0;
+    return B12X_OK;
+  });
+}
+
+// CooMatrix::sortRowMajor / sortColumnMajor (CooMatrix.hpp:95-181, 428-447): stable LSD radix
+// sorts by the minor index, then by the major index, carrying the entries' positions
+int b12x_coo_sort(b12x_coo m, int rowMajor) {
+  return guarded([&]() -> int {
+    if (!m) return setErr(B12X_E_ARG, "null matrix");
+    b12x_coo_s& M = *m;
+    const bool rm = rowMajor != 0;
+    const uint64_t nnz = M.nnz;
+    if (nnz > 1 && !cooSorted(M, rm)) {
+      hipStream_t st = M.stream;
+      std::vector<uint32_t> hT(8);
+      const uint32_t* major = rm ? M.rows : M.cols;
+      const uint32_t* minor = rm ? M.cols : M.rows;
+      const uint32_t nMajor = rm ? M.nRows : M.nCols, nMinor = rm ? M.nCols : M.nRows;
+      uint32_t* idx = dalloc<uint32_t>(nnz);
+      uint32_t* key = dalloc<uint32_t>(nnz);
+      uint32_t* dummy = dalloc<uint32_t>(nnz);
+      hipLaunchKernelGGL(sequenceKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, idx, nnz);
+      HIPCHK(hipMemcpyAsync(key, minor, nnz * 4, hipMemcpyDeviceToDevice, st));
+      radixSort3(st, key, idx, dummy, nnz, bitsFor(uint64_t(nMinor) + 1), hT.data());
+      hipLaunchKernelGGL(gatherU32Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, major,
+                         (const uint32_t*)idx, nnz, key);
+      radixSort3(st, key, idx, dummy, nnz, bitsFor(uint64_t(nMajor) + 1), hT.data());
+      uint32_t* r2 = dalloc<uint32_t>(nnz);
+      uint32_t* c2 = dalloc<uint32_t>(nnz);
+      double* v2 = dalloc<double>(nnz);
+      hipLaunchKernelGGL(gatherU32Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, M.rows,
+                         (const uint32_t*)idx, nnz, r2);
+      hipLaunchKernelGGL(gatherU32Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, M.cols,
+                         (const uint32_t*)idx, nnz, c2);
+      hipLaunchKernelGGL(gatherF64Kernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, M.vals,
+                         (const uint32_t*)idx, nnz, v2);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+      dfree(M.rows);
+      dfree(M.cols);
+      dfree(M.vals);
+      M.rows = r2;
+      M.cols = c2;
+      M.vals = v2;
+      dfree(idx);
+      dfree(key);
+      dfree(dummy);
+    }
+    if (nnz > 1) {  // the flags record strict orders (no repeated entry)
+      M.rowMajor = rm && cooStrict(M, true);
+      M.colMajor = !rm && cooStrict(M, false);
+    }
+    return B12X_OK;
+  });
+}
+
+// CooMatrix copy constructor (CooMatrix.hpp:58-76): a device copy with its own stream
+int b12x_coo_copy(b12x_coo m, b12x_coo* out) {
+  return guarded([&]() -> int {
+    if (!m || !out) return setErr(B12X_E_ARG, "null argument");
+    auto* C = new b12x_coo_s();
+    try {
+      streamCreate(&C->stream);
+      C->nRows = m->nRows;
+      C->nCols = m->nCols;
+      C->nnz = m->nnz;
+      C->rowMajor = m->rowMajor;
+      C->colMajor = m->colMajor;
+      C->rows = dalloc<uint32_t>(m->nnz);
+      C->cols = dalloc<uint32_t>(m->nnz);
+      C->vals = dalloc<double>(m->nnz);
+      HIPCHK(hipStreamSynchronize(m->stream));
+      if (m->nnz) {
+        HIPCHK(hipMemcpyAsync(C->rows, m->rows, m->nnz * 4, hipMemcpyDeviceToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->cols, m->cols, m->nnz * 4, hipMemcpyDeviceToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->vals, m->vals, m->nnz * 8, hipMemcpyDeviceToDevice, C->stream));
+      }
+      HIPCHK(hipStreamSynchronize(C->stream));
+    } catch (...) {
+      delete C;
+      throw;
+    }
+    *out = C;
+    return B12X_OK;
+  });
+}
+
+// ------------------------------------------------------------------------------------------------
+// Invariant measure
+// ------------------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
 
 __global__ void fillF64Kernel(double* p, uint32_t n, double v) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;

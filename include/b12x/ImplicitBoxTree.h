@@ -175,6 +175,37 @@ struct CooMatrix {
     b12x_coo_destroy(h);
   }
 
+  // CooMatrix.h:61-91, .hpp:304-447: ordering and comparison, on the device
+  bool isSortedRowMajor() const { return sorted(1); }
+  bool isSortedColumnMajor() const { return sorted(0); }
+  CooMatrix& sortRowMajor() {
+    sortBy(1);
+    return *this;
+  }
+  CooMatrix& sortColumnMajor() {
+    sortBy(0);
+    return *this;
+  }
+  // the same sizes and entries independent of the orderings (CooMatrix.hpp:304-352); this
+  // matrix's entries are compared with the first entries of the other, as the reference does
+  template <Architecture A2, typename T2>
+  bool operator==(const CooMatrix<A2, T2>& o) const {
+    if (nRows_ != o.nRows_ || nColumns_ != o.nColumns_) return false;
+    const bool frm = isSortedRowMajor(), fcm = !frm && isSortedColumnMajor();
+    const bool srm = o.isSortedRowMajor(), scm = !srm && o.isSortedColumnMajor();
+    if (frm && !srm) return *this == CooMatrix<A2, T2>(o).sortRowMajor();
+    if (fcm && !scm) return *this == CooMatrix<A2, T2>(o).sortColumnMajor();
+    if (srm && !frm) return CooMatrix(*this).sortRowMajor() == o;
+    if (scm && !fcm) return CooMatrix(*this).sortColumnMajor() == o;
+    bool res = o.values_.size() >= values_.size();
+    for (size_t e = 0; res && e < values_.size(); ++e)
+      res = rowIndices_[e] == o.rowIndices_[e] && columnIndices_[e] == o.columnIndices_[e] &&
+            double(values_[e]) == double(o.values_[e]);
+    if (!res && !((frm && srm) || (fcm && scm)))
+      res = CooMatrix(*this).sortRowMajor() == CooMatrix<A2, T2>(o).sortRowMajor();
+    return res;
+  }
+
   // CooMatrix.h:110-112: B12's JSON file (".json" appended when missing)
   void save(const std::string& fileName) const {
     b12x_coo h = nullptr;
@@ -184,6 +215,31 @@ struct CooMatrix {
                        "save"))
       return;
     detail::check(b12x_coo_save_json(h, fileName.c_str()), "save");
+    b12x_coo_destroy(h);
+  }
+  // the matrix on the device for one operation (uploads a copy of the host vectors)
+  b12x_coo toDevice() const {
+    b12x_coo h = nullptr;
+    std::vector<double> v(values_.begin(), values_.end());
+    detail::check(b12x_coo_upload(rowIndices_.data(), columnIndices_.data(), v.data(), rowIndices_.size(), nRows_,
+                                  nColumns_, &h),
+                  "CooMatrix");
+    return h;
+  }
+  bool sorted(int rowMajor) const {
+    b12x_coo h = toDevice();
+    int r = 0;
+    if (h) detail::check(b12x_coo_is_sorted(h, rowMajor, &r), "isSorted");
+    b12x_coo_destroy(h);
+    return r != 0;
+  }
+  void sortBy(int rowMajor) {
+    b12x_coo h = toDevice();
+    if (!h) return;
+    std::vector<double> v(values_.size());
+    if (detail::check(b12x_coo_sort(h, rowMajor), "sort") && !v.empty() &&
+        detail::check(b12x_coo_download(h, rowIndices_.data(), columnIndices_.data(), v.data()), "sort"))
+      values_.assign(v.begin(), v.end());
     b12x_coo_destroy(h);
   }
 };

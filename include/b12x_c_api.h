@@ -229,6 +229,13 @@ int b12x_coo_download(b12x_coo m, uint32_t* rows, uint32_t* cols, double* vals);
 /* Build a matrix from host COO arrays (CooMatrix.h:66-69). */
 int b12x_coo_upload(const uint32_t* rows, const uint32_t* cols, const double* vals, uint64_t nnz, uint32_t n_rows,
                     uint32_t n_cols, b12x_coo* out);
+/* CooMatrix::isSortedRowMajor / isSortedColumnMajor — CooMatrix.h:80-83, .hpp:354-374 (equal
+ * neighbouring entries count as sorted) */
+int b12x_coo_is_sorted(b12x_coo m, int row_major, int* out);
+/* CooMatrix::sortRowMajor / sortColumnMajor — CooMatrix.h:88-91, .hpp:95-181: in place, stable */
+int b12x_coo_sort(b12x_coo m, int row_major);
+/* CooMatrix copy constructor — CooMatrix.h:61-62: a device copy of the matrix */
+int b12x_coo_copy(b12x_coo m, b12x_coo* out);
 /* compressIndexVector(indices, dim) — CooMatrix.h:20-23: sorted index vector -> dim+1 offsets */
 int b12x_compress_index_vector(const uint32_t* indices, uint64_t n, uint32_t dim, uint32_t* out);
 /* Invariant measure: mu <- mu P / ||mu P||_1, mu0 = 1/n (or the given mu if mu_init != 0),

@@ -55,5 +55,16 @@ int main() {
   ImplicitBoxTree<A, 2, double, 64> w;
   w = t;  // assignment
   std::printf("same %d %llu\n", int(w.hasSameBoxes(t)), (unsigned long long)w.getNHostBufferBytes());
+  // CooMatrix ordering and comparison
+  CooMatrix<A, double> P = u.transitionMatrixForLeaves<double>(grid, map);            // column-major
+  CooMatrix<A, double> Q = u.transitionMatrixForLeaves<double>(grid, map, false, true);  // row-major
+  std::printf("coo %d %d %d %d %d\n", int(P.isSortedColumnMajor()), int(P.isSortedRowMajor()),
+              int(Q.isSortedRowMajor()), int(P == Q), int(Q == P));
+  CooMatrix<A, double> R(P);
+  R.sortRowMajor();
+  std::printf("sorted %d %d %d\n", int(R.isSortedRowMajor()), int(R.rowIndices_ == Q.rowIndices_ &&
+              R.columnIndices_ == Q.columnIndices_ && R.values_ == Q.values_), int(R == P));
+  R.values_[0] += 1.0;
+  std::printf("differ %d\n", int(R == P));
   return 0;
 }

@@ -76,7 +76,19 @@ def test_cpp_facade_api_surface_matches_python_and_oracle(tmp_path):
     exp.append(f"same {int(t.hasSameBoxes(u))}")
     w = t.copy()
     exp.append(f"same {int(w.hasSameBoxes(t))} {w.getNHostBufferBytes()}")
+    P = u.transitionMatrixForLeaves(grid, mp)
+    Q = u.transitionMatrixForLeaves(grid, mp, row_major=True)
+    exp.append(f"coo {int(P.isSortedColumnMajor())} {int(P.isSortedRowMajor())} {int(Q.isSortedRowMajor())} "
+               f"{int(P == Q)} {int(Q == P)}")
+    R = P.copy().sortRowMajor()
+    exp.append(f"sorted {int(R.isSortedRowMajor())} "
+               f"{int(all(np.array_equal(x, y) for x, y in zip(R.download(), Q.download())))} {int(R == P)}")
+    rr, cc, vv = R.download()
+    vv[0] += 1.0
+    R2 = gx.CooMatrix.from_arrays(rr, cc, vv, R.nRows, R.nColumns)
+    exp.append(f"differ {int(R2 == P)}")
     assert out == exp
+    assert exp[-3:] == ["coo 1 0 1 1 1", "sorted 1 1 1", "differ 0"]
     # the values the reference fixes
     assert exp[2] == "default 1" and exp[3] == "alldepth 1 1 0" and exp[4] == "same 1"
     sd = t.getSdScheme()
@@ -119,3 +131,40 @@ def test_has_same_boxes_and_domain_setters():
     assert e.getSdScheme().tolist() == [[1, 0][i % 2] for i in range(65)]
     e.setSdScheme([])
     assert e.isSdSchemeDefault() and a.hasSameBoxes(e)
+
+
+def test_coo_sorts_are_stable_and_equality_ignores_order():
+    """sortRowMajor / sortColumnMajor reorder the entries stably (repeated entries keep their
+    order, as numpy's stable lexsort does); isSorted* accept equal neighbours; == holds between
+    orderings of one matrix and fails after one value changes."""
+    rng = np.random.default_rng(1601)
+    n, m = 3000, 20000
+    r = rng.integers(0, n, m).astype(np.uint32)
+    c = rng.integers(0, n, m).astype(np.uint32)
+    r[:100], c[:100] = r[100:200], c[100:200]  # repeated entries
+    v = rng.random(m)
+    a = gx.CooMatrix.from_arrays(r, c, v, n, n)
+    assert not a.isSortedRowMajor() and not a.isSortedColumnMajor()
+    rm = np.lexsort((c, r))
+    cm = np.lexsort((r, c))
+    b = a.copy().sortRowMajor()
+    assert b.isSortedRowMajor()
+    for x, y in zip(b.download(), (r[rm], c[rm], v[rm])):
+        assert np.array_equal(x, y)
+    d = a.copy().sortColumnMajor()
+    assert d.isSortedColumnMajor()
+    for x, y in zip(d.download(), (r[cm], c[cm], v[cm])):
+        assert np.array_equal(x, y)
+    assert a == b and b == d and d == a
+    v2 = v.copy()
+    v2[7] += 0.5
+    assert not (gx.CooMatrix.from_arrays(r, c, v2, n, n) == a)
+    assert not (gx.CooMatrix.from_arrays(r, c, v, n, n + 1) == a)
+    # the power iteration of a matrix sorted in place equals that of its row-major source
+    t = gx.ImplicitBoxTree([0.0, 0.0], [1.5, 0.5])
+    t.setFlags(gx.SD, 0)
+    t.subdivision_steps(gx.InnerGrid(4), gx.Henon(), 12)
+    P = t.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon(), row_major=True)
+    Q = P.copy().sortColumnMajor()
+    assert Q.isSortedColumnMajor() and P == Q
+    assert np.array_equal(P.power_iteration(40)[0], Q.power_iteration(40)[0])

@@ -183,3 +183,18 @@ def test_continuation_and_chain_recurrent_set_with_a_given_scheme():
     got = g.chain_recurrent_steps(grid, mp, 20)
     assert [tuple(int(v) for v in row) for row in got] == [tuple(e) for e in exp]
     check(g, o, "chain")
+
+
+def test_fixed_boxes_matches_oracle():
+    """fixedBoxes.m on the GPU tree and on the oracle: the same boxes after every number of steps
+    up to the maximal depth."""
+    for mp, grid, c, r in ((gx.Henon(), gx.InnerGrid(4), [0.0, 0.0], [3.0, 3.0]),
+                           (gx.Henon(1.4, 0.3, 2), gx.InnerGrid(3), [0.0, 0.0], [3.0, 3.0]),
+                           (gx.Lorenz(), gx.InnerGrid(2), [0.0, 0.0, 27.0], [30.0, 30.0, 40.0])):
+        for steps in (12, 30, 64):
+            print(f"[insert] fixedBoxes map={mp.kind} steps={steps}", flush=True)
+            g, o = pair(c, r)
+            bg = gx.fixedBoxes(g, grid, mp, steps)
+            bo = gx.fixedBoxes(o, grid, mp, steps)
+            assert np.array_equal(bg, bo)
+            check(g, o, steps)

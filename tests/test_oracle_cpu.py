@@ -881,3 +881,34 @@ def test_strong_components_brute_force_and_scipy():
     r, c, _ = m.download()
     lab, nc, nb = m.scc()
     assert np.array_equal(lab, scipy_scc(r, c, m.nRows)) and nc == len(set(lab.tolist()))
+
+
+def test_fixed_boxes_against_a_restatement_and_the_fixed_points():
+    """fixedBoxes.m: per step every box splits along the scheme and a child is kept when one of
+    its inner-grid points is mapped into the child itself. Restated here on bit patterns with
+    bisection (independent of the oracle); every kept box lies within two box radii of one of the
+    Hénon map's fixed points (sampling can lose a fixed point's boxes: a box is kept only when one
+    of its 16 points lands in it)."""
+    a, b = 1.4, 0.3
+    c, r = [0.0, 0.0], [3.0, 3.0]
+    g = np.array([[((i // 4 ** j) % 4 * 2.0 + 1.0 - 4) / 4 for j in range(2)] for i in range(16)])
+    steps = 24
+    boxes = [0]
+    for d in range(1, steps + 1):
+        kept = []
+        for p in boxes:
+            for child in (2 * p, 2 * p + 1):
+                lo, hi = bisect_box(child, d, c, r)
+                cen, rad = 0.5 * (lo + hi), 0.5 * (hi - lo)
+                if any(point_pattern(henon_py(x, a, b), d, c, r) == child for x in cen + rad * g):
+                    kept.append(child)
+        boxes = kept if kept else [0]
+        if not kept:
+            break
+    o = OracleTree(c, r)
+    B = gx.fixedBoxes(o, gx.InnerGrid(4), gx.Henon(a, b, 1), steps)
+    bits, depth, _ = o.download()
+    assert sorted(bits.tolist()) == sorted(boxes) and np.all(depth == steps)
+    fps = [np.array([x, b * x]) for x in ((-(1 - b) + s * np.sqrt((1 - b) ** 2 + 4 * a)) / (2 * a) for s in (1, -1))]
+    dist = np.array([[np.max(np.abs(fp - row[:2]) / row[2:4]) for fp in fps] for row in B])
+    assert len(B) >= 1 and np.all(dist.min(axis=1) <= 2.0)
