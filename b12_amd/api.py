@@ -80,6 +80,7 @@ SIGNATURES = {
     "b12x_coo_is_sorted": [_P, _I, _P],
     "b12x_coo_sort": [_P, _I],
     "b12x_coo_copy": [_P, _P],
+    "b12x_coo_make_symmetric": [_P, _I],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_power_iteration_kernel_ms": [_P],
     "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
@@ -333,6 +334,13 @@ class CooMatrix:
         h = _P()
         self.api("b12x_coo_copy", self.h, ctypes.byref(h))
         return CooMatrix(h, self.api)
+
+    def makeSymmetric(self, makeAntisymmetric=False):
+        """(P + P^T)/2, or (P - P^T)/2 (CooMatrix.hpp:376-426), in place."""
+        self.api("b12x_coo_make_symmetric", self.h, int(bool(makeAntisymmetric)))
+        nnz = _U64()
+        self.api("b12x_coo_info", self.h, ctypes.byref(nnz), None, None, None)
+        self.nnz = nnz.value
 
     def _same_entries(self, other):
         # the reference compares this matrix's entries with the first entries of the other

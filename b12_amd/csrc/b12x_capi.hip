@@ -2264,6 +2264,112 @@ int b12x_coo_sort(b12x_coo m, int rowMajor) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+// the transposed copy of the entries behind them (values negated for the antisymmetric part)
+__global__ void cooMirrorKernel(uint32_t* rows, uint32_t* cols, double* vals, uint64_t nnz, int negate) {
+  const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= nnz) return;
+  rows[nnz + e] = cols[e];
+  cols[nnz + e] = rows[e];
+  vals[nnz + e] = negate ? -vals[e] : vals[e];
+}
+
+// runs of equal (row, column): a run's first entry holds the run's sum, added from its last entry
+// backwards (the reverse inclusive scan of CooMatrix.hpp:405-412), then halved; flag[e] marks run
+// starts
+__global__ void cooRunSumKernel(const uint32_t* rows, const uint32_t* cols, const double* vals, uint64_t n,
+                                double* sum, uint32_t* flag) {
+  const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= n) return;
+  const bool start = e == 0 || rows[e] != rows[e - 1] || cols[e] != cols[e - 1];
+  flag[e] = start ? 1u : 0u;
+  if (!start) return;
+  uint64_t last = e;
+  while (last + 1 < n && rows[last + 1] == rows[e] && cols[last + 1] == cols[e]) ++last;
+  double acc = vals[last];
+  for (uint64_t k = last; k > e; --k) acc = acc + vals[k - 1];
+  sum[e] = acc / 2.0;
+}
+
+__global__ void cooRunWriteKernel(const uint32_t* rows, const uint32_t* cols, const double* sum, const uint32_t* flag,
+                                  const uint32_t* pos, uint64_t n, uint32_t* r2, uint32_t* c2, double* v2) {
+  const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= n) return;
+  const bool start = e == 0 || rows[e] != rows[e - 1] || cols[e] != cols[e - 1];
+  if (!start) return;
+  const uint32_t o = pos[e];
+  r2[o] = rows[e];
+  c2[o] = cols[e];
+  v2[o] = sum[e];
+  (void)flag;
+}
+}  // namespace
+
+extern "C" {
+
+// CooMatrix::makeSymmetric(makeAntisymmetric) (CooMatrix.hpp:376-426): (P + P^T) / 2 or
+// (P - P^T) / 2, in row-major order when P was sorted row-major, column-major otherwise
+int b12x_coo_make_symmetric(b12x_coo m, int antisymmetric) {
+  return guarded([&]() -> int {
+    if (!m) return setErr(B12X_E_ARG, "null matrix");
+    b12x_coo_s& M = *m;
+    if (M.nRows != M.nCols) return setErr(B12X_E_ARG, "matrix must be square");
+    const uint64_t nnz = M.nnz;
+    if (nnz == 0) return B12X_OK;
+    if (2 * nnz > 0xFFFFFFFFull) return setErr(B12X_E_UNSUPPORTED, "more than 2^31 entries");
+    hipStream_t st = M.stream;
+    const bool wasRowMajor = cooSorted(M, true);
+    const uint64_t n2 = 2 * nnz;
+    uint32_t* r = dalloc<uint32_t>(n2);
+    uint32_t* c = dalloc<uint32_t>(n2);
+    double* v = dalloc<double>(n2);
+    HIPCHK(hipMemcpyAsync(r, M.rows, nnz * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c, M.cols, nnz * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(v, M.vals, nnz * 8, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(cooMirrorKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, st, r, c, v, nnz,
+                       antisymmetric ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(M.rows);
+    dfree(M.cols);
+    dfree(M.vals);
+    M.rows = r;
+    M.cols = c;
+    M.vals = v;
+    M.nnz = n2;
+    M.rowMajor = M.colMajor = false;
+    if (int e = b12x_coo_sort(m, wasRowMajor ? 1 : 0)) return e;
+    double* sum = dalloc<double>(n2);
+    uint32_t* pos = dalloc<uint32_t>(n2);
+    uint32_t* tt = dalloc<uint32_t>(uint64_t(ceilDiv(n2, kTile)) + 16);
+    uint32_t* dTot = dalloc<uint32_t>(1);
+    uint32_t hTot = 0;
+    hipLaunchKernelGGL(cooRunSumKernel, dim3(ceilDiv(n2, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)M.rows,
+                       (const uint32_t*)M.cols, (const double*)M.vals, n2, sum, pos);
+    HIPCHK(hipGetLastError());
+    const uint32_t k = scanArray(st, pos, n2, tt, dTot, &hTot);
+    uint32_t* r2 = dalloc<uint32_t>(k);
+    uint32_t* c2 = dalloc<uint32_t>(k);
+    double* v2 = dalloc<double>(k);
+    hipLaunchKernelGGL(cooRunWriteKernel, dim3(ceilDiv(n2, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)M.rows,
+                       (const uint32_t*)M.cols, (const double*)sum, (const uint32_t*)nullptr, (const uint32_t*)pos,
+                       n2, r2, c2, v2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    for (void* q : {(void*)M.rows, (void*)M.cols, (void*)M.vals, (void*)sum, (void*)pos, (void*)tt, (void*)dTot})
+      dfree(q);
+    M.rows = r2;
+    M.cols = c2;
+    M.vals = v2;
+    M.nnz = k;
+    M.rowMajor = wasRowMajor;  // runs merged: strictly ordered now
+    M.colMajor = !wasRowMajor;
+    return B12X_OK;
+  });
+}
+
 // CooMatrix copy constructor (CooMatrix.hpp:58-76): a device copy with its own stream
 int b12x_coo_copy(b12x_coo m, b12x_coo* out) {
   return guarded([&]() -> int {

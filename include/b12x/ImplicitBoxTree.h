@@ -186,6 +186,21 @@ struct CooMatrix {
     sortBy(0);
     return *this;
   }
+  // CooMatrix.hpp:376-426: the symmetric (or antisymmetric) part, in place
+  void makeSymmetric(bool makeAntisymmetric = false) {
+    b12x_coo h = toDevice();
+    if (!h) return;
+    uint64_t nnz = 0;
+    if (detail::check(b12x_coo_make_symmetric(h, makeAntisymmetric ? 1 : 0), "makeSymmetric") &&
+        detail::check(b12x_coo_info(h, &nnz, nullptr, nullptr, nullptr), "makeSymmetric")) {
+      std::vector<double> v(nnz);
+      rowIndices_.resize(nnz);
+      columnIndices_.resize(nnz);
+      if (nnz) detail::check(b12x_coo_download(h, rowIndices_.data(), columnIndices_.data(), v.data()), "makeSymmetric");
+      values_.assign(v.begin(), v.end());
+    }
+    b12x_coo_destroy(h);
+  }
   // the same sizes and entries independent of the orderings (CooMatrix.hpp:304-352); this
   // matrix's entries are compared with the first entries of the other, as the reference does
   template <Architecture A2, typename T2>

@@ -234,6 +234,11 @@ int b12x_coo_upload(const uint32_t* rows, const uint32_t* cols, const double* va
 int b12x_coo_is_sorted(b12x_coo m, int row_major, int* out);
 /* CooMatrix::sortRowMajor / sortColumnMajor — CooMatrix.h:88-91, .hpp:95-181: in place, stable */
 int b12x_coo_sort(b12x_coo m, int row_major);
+/* CooMatrix::makeSymmetric(makeAntisymmetric) — CooMatrix.h:84-86, .hpp:376-426: the symmetric
+ * part (P + P^T)/2, or the antisymmetric part (P - P^T)/2, of a square matrix; equal entries are
+ * summed from the last backwards; row-major order when P was sorted row-major, else
+ * column-major */
+int b12x_coo_make_symmetric(b12x_coo m, int antisymmetric);
 /* CooMatrix copy constructor — CooMatrix.h:61-62: a device copy of the matrix */
 int b12x_coo_copy(b12x_coo m, b12x_coo* out);
 /* compressIndexVector(indices, dim) — CooMatrix.h:20-23: sorted index vector -> dim+1 offsets */

@@ -66,5 +66,8 @@ int main() {
               R.columnIndices_ == Q.columnIndices_ && R.values_ == Q.values_), int(R == P));
   R.values_[0] += 1.0;
   std::printf("differ %d\n", int(R == P));
+  CooMatrix<A, double> S(Q);
+  S.makeSymmetric();
+  std::printf("symmetric %zu %d\n", S.values_.size(), int(S.isSortedRowMajor()));
   return 0;
 }

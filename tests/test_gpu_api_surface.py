@@ -87,8 +87,11 @@ def test_cpp_facade_api_surface_matches_python_and_oracle(tmp_path):
     vv[0] += 1.0
     R2 = gx.CooMatrix.from_arrays(rr, cc, vv, R.nRows, R.nColumns)
     exp.append(f"differ {int(R2 == P)}")
+    S = Q.copy()
+    S.makeSymmetric()
+    exp.append(f"symmetric {S.nnz} {int(S.isSortedRowMajor())}")
     assert out == exp
-    assert exp[-3:] == ["coo 1 0 1 1 1", "sorted 1 1 1", "differ 0"]
+    assert exp[-4:-1] == ["coo 1 0 1 1 1", "sorted 1 1 1", "differ 0"]
     # the values the reference fixes
     assert exp[2] == "default 1" and exp[3] == "alldepth 1 1 0" and exp[4] == "same 1"
     sd = t.getSdScheme()
@@ -168,3 +171,63 @@ def test_coo_sorts_are_stable_and_equality_ignores_order():
     Q = P.copy().sortColumnMajor()
     assert Q.isSortedColumnMajor() and P == Q
     assert np.array_equal(P.power_iteration(40)[0], Q.power_iteration(40)[0])
+
+
+def ref_make_symmetric(r, c, v, anti, row_major):
+    """CooMatrix.hpp:376-426 restated in numpy: the transposed entries appended, a stable sort,
+    every run of equal (row, column) summed from its last entry backwards, halved."""
+    R = np.concatenate([r, c])
+    C = np.concatenate([c, r])
+    V = np.concatenate([v, -v if anti else v])
+    o = np.lexsort((C, R)) if row_major else np.lexsort((R, C))
+    R, C, V = R[o], C[o], V[o]
+    out = []
+    i = 0
+    while i < len(R):
+        j = i
+        while j + 1 < len(R) and R[j + 1] == R[i] and C[j + 1] == C[i]:
+            j += 1
+        acc = V[j]
+        for k in range(j - 1, i - 1, -1):
+            acc = acc + V[k]
+        out.append((R[i], C[i], acc / 2.0))
+        i = j + 1
+    return (np.array([x[0] for x in out], np.uint32), np.array([x[1] for x in out], np.uint32),
+            np.array([x[2] for x in out]))
+
+
+@pytest.mark.parametrize("anti", [False, True])
+def test_make_symmetric_matches_restatement_and_dense_part(anti):
+    rng = np.random.default_rng(1700 + int(anti))
+    n, m = 400, 3000
+    r = rng.integers(0, n, m).astype(np.uint32)
+    c = rng.integers(0, n, m).astype(np.uint32)
+    v = rng.random(m)
+    for repeated in (False, True):
+        rr, cc, vv = r.copy(), c.copy(), v.copy()
+        if not repeated:  # unique entries, in no order
+            keys = np.unique(rr.astype(np.uint64) * n + cc, return_index=True)[1]
+            rr, cc, vv = rr[keys], cc[keys], vv[keys]
+            p = rng.permutation(len(rr))
+            rr, cc, vv = rr[p], cc[p], vv[p]
+        for row_major in (False, True):
+            print(f"[api] makeSymmetric anti={anti} repeated={repeated} row_major={row_major}", flush=True)
+            a = gx.CooMatrix.from_arrays(rr, cc, vv, n, n)
+            if row_major:
+                a.sortRowMajor()
+                o = np.lexsort((cc, rr))
+                src = (rr[o], cc[o], vv[o])
+            else:
+                src = (rr, cc, vv)
+            a.makeSymmetric(anti)
+            exp = ref_make_symmetric(*src, anti, row_major)
+            for x, y in zip(a.download(), exp):
+                assert np.array_equal(x, y)
+            if not repeated:
+                D = np.zeros((n, n))
+                D[rr, cc] = vv
+                S = (D - D.T) / 2.0 if anti else (D + D.T) / 2.0
+                got = np.zeros((n, n))
+                x, y, z = a.download()
+                got[x, y] = z
+                assert np.array_equal(got, S)
