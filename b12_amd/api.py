@@ -824,6 +824,29 @@ def fixedBoxes(ibt, grid, mp, steps=64):
     return b[:, : b.shape[1] - 1]
 
 
+def subdivisionByNim(ibt, grid, mp, steps, max_iter=1_000_000, tol=1e-12, threshold=1e-9):
+    """The subdivision algorithm with selection by the natural invariant measure
+    (B12c/src/matlab/subdivisionByNim.m:1-87): SD on every leaf, then per step subdivide(SD), the
+    leaves' transition matrix, its invariant measure mu (the reference takes MATLAB's eigs or
+    power method; here the power iteration mu <- mu P / ||mu P||_1 to ||mu_{k+1} - mu_k||_1 <= tol,
+    the same measure as the measure step), then HIT on the leaves with mu >= threshold / n,
+    remove(HIT), unsetFlags(HIT). Returns per step (leaves after subdividing, power iterations,
+    leaves kept). Any tree object with these method names works."""
+    out = []
+    ibt.setFlags(SD, LEAF)
+    for _ in range(steps):
+        ibt.subdivide(SD)
+        m = ibt.transitionMatrixForLeaves(grid, mp)
+        mu, iters, _ = m.power_iteration(max_iter, tol)
+        n = len(mu)
+        v = np.abs(mu) / np.abs(mu).sum()
+        ibt.setFlagsByStencil((v >= threshold / n).astype(np.uint8), HIT)
+        ibt.remove(HIT)
+        ibt.unsetFlags(HIT, LEAF)
+        out.append((n, iters, ibt.count(LEAF)))
+    return out
+
+
 def chainRecurrentSet(ibt, grid, mp, steps, on_step=None):
     """The chain recurrent set by subdivision (B12c/src/matlab/chainRecurrentSet.m:1-64): SD on
     every leaf, then per step subdivide(SD), the leaves' transition matrix, its strong

@@ -198,3 +198,18 @@ def test_fixed_boxes_matches_oracle():
             bo = gx.fixedBoxes(o, grid, mp, steps)
             assert np.array_equal(bg, bo)
             check(g, o, steps)
+
+
+def test_subdivision_by_nim_matches_oracle():
+    """subdivisionByNim.m with the power iteration as the eigenvector step: per step the same
+    leaf counts (power iterations equal, or within one: the measure agrees within 1e-12), and the
+    same trees."""
+    for mp, grid, c, r, steps in ((gx.Henon(), gx.InnerGrid(4), [0.0, 0.0], [1.5, 0.5], 18),
+                                  (gx.Lorenz(), gx.InnerGrid(3), [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], 16)):
+        print(f"[insert] subdivisionByNim map={mp.kind} steps={steps}", flush=True)
+        g, o = pair(c, r)
+        rg = gx.subdivisionByNim(g, grid, mp, steps)
+        ro = gx.subdivisionByNim(o, grid, mp, steps)
+        for a, b in zip(rg, ro):
+            assert a[0] == b[0] and a[2] == b[2] and abs(a[1] - b[1]) <= 1
+        check(g, o, steps)

@@ -912,3 +912,20 @@ def test_fixed_boxes_against_a_restatement_and_the_fixed_points():
     fps = [np.array([x, b * x]) for x in ((-(1 - b) + s * np.sqrt((1 - b) ** 2 + 4 * a)) / (2 * a) for s in (1, -1))]
     dist = np.array([[np.max(np.abs(fp - row[:2]) / row[2:4]) for fp in fps] for row in B])
     assert len(B) >= 1 and np.all(dist.min(axis=1) <= 2.0)
+
+
+def test_subdivision_by_nim_cover_inside_the_plain_subdivision():
+    """subdivisionByNim.m keeps the leaves whose invariant measure is at least 1e-9 / n. A box of
+    positive measure is hit by an image of a kept box, so by induction over the depths the cover
+    lies inside the plain subdivision's cover (subdivision.m) at every depth; the measure of the
+    kept boxes carries almost all of the mass."""
+    grid, mp = gx.InnerGrid(4), gx.Henon()
+    o, p = OracleTree([0.0, 0.0], [1.5, 0.5]), OracleTree([0.0, 0.0], [1.5, 0.5])
+    p.setFlags(gx.SD, gx.LEAF)
+    o.setFlags(gx.SD, gx.LEAF)
+    for d in range(1, 17):
+        got = gx.subdivisionByNim(o, grid, mp, 1)[0]
+        p.subdivision_steps(grid, mp, 1)
+        kept = set(zip(*[x.tolist() for x in o.download()[:2]]))
+        plain = set(zip(*[x.tolist() for x in p.download()[:2]]))
+        assert kept and kept <= plain and got[2] == len(kept), d
