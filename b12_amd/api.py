@@ -43,6 +43,7 @@ SIGNATURES = {
     "b12x_delete_depth": [_P, _U8, _P],
     "b12x_insert_by_image": [_P, _P, _U8, _U8, _U8, _U8, _U8, _P],
     "b12x_insert_by_points": [_P, _P, _U64, _U8, _U8, _U8, _P],
+    "b12x_continuation": [_P, _P, _U8, _P, _U64, _P, _I, _P],
     "b12x_tree_save_json": [_P, ctypes.c_char_p],
     "b12x_tree_save_gaio": [_P, ctypes.c_char_p],
     "b12x_tree_load": [ctypes.c_char_p, _I, _P],
@@ -734,6 +735,20 @@ class ImplicitBoxTree:
                  flags_for_hit, _ptr(res))
         return res
 
+    def continuation_run(self, grid, mp, depth, points):
+        """continuation.m at one depth on the device (b12x_continuation): the list of count(depth)
+        after the first insertion and after every expansion step."""
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, self.dim)
+        cap = 4096
+        while True:
+            counts = np.zeros(cap, np.uint32)
+            k = _I()
+            self.api("b12x_continuation", self.h, self.scheme(grid, mp).h, _depth(depth), _ptr(p), p.shape[0],
+                     _ptr(counts), cap, ctypes.byref(k))
+            if k.value <= cap:
+                return [int(c) for c in counts[: k.value]]
+            cap = k.value  # a longer run than the buffer: run again with room for every count
+
     def searchByPoints(self, points, depth=-1):
         p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, self.dim)
         out = np.empty(p.shape[0], np.uint32)
@@ -785,6 +800,8 @@ def continuation(ibt, grid, mp, depths, points, on_step=None):
     before it). `on_step(d, step, count)` is called after every expansion step. Any tree object
     with the same method names works, so the oracle's tree runs the same driver."""
     out = []
+    if hasattr(ibt, "continuation_run") and on_step is None:  # the GPU tree: the loop on the device
+        return [ibt.continuation_run(grid, mp, d, points) for d in depths]
     for d in depths:
         ibt.deleteDepth(0)
         ibt.insertByPoints(points, d, INS)

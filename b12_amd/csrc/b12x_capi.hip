@@ -1473,6 +1473,54 @@ int b12x_insert_by_points(b12x_tree t, const double* pts, uint64_t P, uint8_t d,
   });
 }
 
+// continuation.m:13-44 for one depth: deleteDepth(0); insert the points' boxes at depth with INS;
+// then while count(depth) grows: INS -> EXPD at depth, insert the boxes hit by the images of the
+// EXPD boxes with INS, unset EXPD on the leaves. The per-point results of the insertions are not
+// kept, and where every leaf lies on `depth` the flag operations do not read their counts back.
+int b12x_continuation(b12x_tree t, b12x_scheme s, uint8_t d, const double* pts, uint64_t P, uint32_t* counts,
+                      int maxCounts, int* nCounts) {
+  return guarded([&]() -> int {
+    if (!t || !s || (P && !pts) || !nCounts || (maxCounts > 0 && !counts)) return setErr(B12X_E_ARG, "bad argument");
+    b12x_tree_s& T = *t;
+    constexpr uint8_t kIns = 0x02, kExpd = 0x04;
+    auto flagsAt = [&](int op, uint8_t a, uint8_t b, uint8_t depth) {
+      if (classifyDepth(T, depth) == DC_LEAF) {  // every leaf on `depth`: no count read back
+        hipLaunchKernelGGL(flagsKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.n,
+                           (uint8_t*)nullptr, 0, op, a, b, T.scal + 26);  // a count nobody reads
+        HIPCHK(hipGetLastError());
+      } else {
+        (void)stencilOp(T, op, nullptr, a, b, depth);
+      }
+    };
+    auto countAt = [&]() -> uint32_t {
+      uint32_t c = 0;
+      if (int e = b12x_count(t, d, &c)) throw HipFail(e, lastError());
+      return c;
+    };
+    int k = 0;
+    auto record = [&](uint32_t c) {
+      if (k < maxCounts) counts[k] = c;
+      ++k;
+    };
+    uint32_t r = 0;
+    if (int e = b12x_delete_depth(t, 0, &r)) return e;
+    if (int e = b12x_insert_by_points(t, pts, P, d, kIns, 0, nullptr)) return e;
+    uint32_t nb0 = 0, nb1 = countAt();
+    record(nb1);
+    while (nb1 > nb0) {
+      flagsAt(2, kIns, kExpd, d);
+      if (int e = b12x_insert_by_image(t, s, d, kExpd, d, kIns, 0, nullptr)) return e;
+      flagsAt(1, kExpd, 0, kLeafDepth);
+      nb0 = nb1;
+      nb1 = countAt();
+      record(nb1);
+    }
+    HIPCHK(hipStreamSynchronize(T.stream));
+    *nCounts = k;
+    return B12X_OK;
+  });
+}
+
 // ImplicitBoxTree.hpp:611-784 boxes(depth) and :1273-1283 enumerate()
 int b12x_boxes(b12x_tree t, uint8_t d, double* out, uint64_t* nOut) {
   return guarded([&]() -> int {

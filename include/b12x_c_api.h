@@ -104,6 +104,13 @@ int b12x_set_host_buffer_bytes(b12x_tree t, uint64_t bytes);
 int b12x_size(b12x_tree t, uint32_t* out);
 int b12x_count(b12x_tree t, uint8_t depth, uint32_t* out);
 int b12x_depth(b12x_tree t, uint8_t* out);
+/* The continuation algorithm of continuation.m:13-44 at one depth (no counterpart in the C++
+ * reference; the MATLAB driver runs it through the MEX calls): deleteDepth(0), insertByPoints(points,
+ * depth, INS), then while count(depth) grows {changeFlags(INS, EXPD, depth); insertByIterators(
+ * ByFlags(depth, EXPD), depth, INS, NONE); unsetFlags(EXPD)}. counts receives count(depth) after
+ * the first insertion and after each step (at most max_counts of them); *n_counts is their number. */
+int b12x_continuation(b12x_tree t, b12x_scheme s, uint8_t depth, const double* points, uint64_t n_points,
+                      uint32_t* counts, int max_counts, int* n_counts);
 /* ImplicitBoxTree::boxes(Depth) (ImplicitBoxTree.h:86-91, ImplicitBoxTree.hpp:611-784): *n_out
    boxes of depth `depth`, each 2*dim+1 doubles {centre, radius, flags}, in leaf order; the root
    carries the AND of all leaf flags, a box between root and leaves the AND over its leaves.

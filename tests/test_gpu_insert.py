@@ -109,12 +109,16 @@ def test_continuation_henon_matches_oracle():
     grid, mp = gx.InnerGrid(4), gx.Henon()
     p = henon_fixed_point()
     g, o = pair([0.0, 0.0], [3.0, 3.0])
+    h = gx.ImplicitBoxTree([0.0, 0.0], [3.0, 3.0])
     for d in (8, 14, 20, 24):
         print(f"[insert] continuation henon d={d}", flush=True)
-        cg = gx.continuation(g, grid, mp, [d], p)
+        cg = gx.continuation(g, grid, mp, [d], p)  # the loop on the device (b12x_continuation)
         co = gx.continuation(o, grid, mp, [d], p)
         assert cg == co, d
         check(g, o, d)
+        steps = []  # the loop in Python over the GPU tree's methods
+        assert gx.continuation(h, grid, mp, [d], p, on_step=lambda *x: steps.append(x)) == co
+        assert same_tree(h.download(), o.download()) and len(steps) == len(co[0]) - 1
 
 
 def test_continuation_lorenz_from_the_origin_matches_oracle():
