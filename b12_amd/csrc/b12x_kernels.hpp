@@ -512,12 +512,17 @@ static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
     }
   }
   // one point per lane (and every launch too large for 32-bit indices)
-  // upper levels of the descent: the samples of the leaf keys, staged in LDS
+  // upper levels of the descent in LDS: the samples of the leaf keys when given, or every key when
+  // the tree has at most kDescentSamples leaves (the sample positions are then the identity, so
+  // the search runs in LDS up to the final containment test)
   uint32_t S = 0;
-  if (PPL == 1 && a.samples && !a.depth0 && n > 0) {
-    S = min(kDescentSamples, n);
-    for (uint32_t j = threadIdx.x; j < S; j += BS) sm[j] = a.samples[j];
-    __syncthreads();
+  if (PPL == 1 && !a.depth0 && !a.depthBox && n > 0) {
+    const uint64_t* src = a.samples ? a.samples : (n <= kDescentSamples ? a.key : nullptr);
+    if (src) {
+      S = min(kDescentSamples, n);
+      for (uint32_t j = threadIdx.x; j < S; j += BS) sm[j] = src[j];
+      __syncthreads();
+    }
   }
   if (PPL == 1 && a.pointBase + nPoints <= 0xFFFFFFFFull && uint64_t(gridDim.x) * BS <= 0xFFFFFFFFull) {
     // 32-bit index arithmetic (the common case)
