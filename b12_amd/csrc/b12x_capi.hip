@@ -1213,8 +1213,9 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
                 const DepthMap* M, uint8_t fIns, uint8_t fHit, uint8_t* resHost) {
   const int form = d == kLeafDepth ? 0 : (dc == DC_BEYOND ? 2 : 1);
   const uint32_t n = T.n;
-  uint8_t* ins = dalloc<uint8_t>(P);
-  uint32_t* insU32 = dalloc<uint32_t>(P + 1);
+  DevScope ds;  // every buffer below except the radix sorts' own (they swap and free theirs)
+  uint8_t* ins = ds.alloc<uint8_t>(P);
+  uint32_t* insU32 = ds.alloc<uint32_t>(P + 1);
   hipLaunchKernelGGL(insertMarkKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, pk, sv, P, form,
                      form == 0 ? 0 : int(kMaxDepth) - int(d), (const uint8_t*)T.depth[T.cur], ins, insU32, T.hit,
                      (uint32_t*)nullptr);
@@ -1233,30 +1234,27 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
     HIPCHK(hipGetLastError());
   }
   // the points may outnumber the leaves: the scan over them has scratch of its own
-  uint32_t* ptTotals = dalloc<uint32_t>(uint64_t(ceilDiv(P, kTile)) + 16);
+  uint32_t* ptTotals = ds.alloc<uint32_t>(uint64_t(ceilDiv(P, kTile)) + 16);
   const uint32_t m = scanArray(T.stream, insU32, P, ptTotals, T.scal + 4, T.hscal + 4);
-  dfree(ptTotals);
   if (resHost) HIPCHK(hipMemcpyAsync(resHost, ins, P, hipMemcpyDeviceToHost, T.stream));
   if (m == 0) {
     HIPCHK(hipStreamSynchronize(T.stream));
-    dfree(ins);
-    dfree(insU32);
     return;
   }
   // the new boxes, sorted (comparable key ascending, deeper first on equal keys, stable), then
   // merged into the leaves, which are sorted already
   const uint64_t N = uint64_t(n) + m;
   ensureCap(T, N);
-  uint64_t* newK = dalloc<uint64_t>(m);
-  uint8_t* newD = dalloc<uint8_t>(m);
-  uint8_t* newF = dalloc<uint8_t>(m);
+  uint64_t* newK = ds.alloc<uint64_t>(m);
+  uint8_t* newD = ds.alloc<uint8_t>(m);
+  uint8_t* newF = ds.alloc<uint8_t>(m);
   hipLaunchKernelGGL(insertNewKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, pk,
                      (const uint32_t*)insU32, m, P, form == 0 ? 1 : 0, d,
                      (const uint64_t*)T.key[T.cur], (const uint8_t*)T.depth[T.cur], n, fIns, newK, newD, newF);
   HIPCHK(hipGetLastError());
-  uint64_t* nsK = dalloc<uint64_t>(m);
-  uint8_t* nsD = dalloc<uint8_t>(m);
-  uint8_t* nsF = dalloc<uint8_t>(m);
+  uint64_t* nsK = ds.alloc<uint64_t>(m);
+  uint8_t* nsD = ds.alloc<uint8_t>(m);
+  uint8_t* nsF = ds.alloc<uint8_t>(m);
   if (m <= kSmallSortMax) {
     hipLaunchKernelGGL(smallBoxSortKernel, dim3(1), dim3(kSmallSortThreads), 0, T.stream, (const uint64_t*)newK,
                        (const uint8_t*)newD, (const uint8_t*)newF, m, nsK, nsD, nsF);
@@ -1282,9 +1280,9 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
     dfree(dummy);
   }
   HIPCHK(hipGetLastError());
-  uint64_t* sK = dalloc<uint64_t>(N);
-  uint8_t* sD = dalloc<uint8_t>(N);
-  uint8_t* sF = dalloc<uint8_t>(N);
+  uint64_t* sK = ds.alloc<uint64_t>(N);
+  uint8_t* sD = ds.alloc<uint8_t>(N);
+  uint8_t* sF = ds.alloc<uint8_t>(N);
   hipLaunchKernelGGL(mergeNewKernel, dim3(ceilDiv(m, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)nsK,
                      (const uint8_t*)nsD, (const uint8_t*)nsF, m, (const uint64_t*)T.key[T.cur],
                      (const uint8_t*)T.depth[T.cur], n, sK, sD, sF);
@@ -1293,23 +1291,23 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
                      (const uint64_t*)nsK, (const uint8_t*)nsD, m, sK, sD, sF);
   HIPCHK(hipGetLastError());
   // runs on one path, reduced
-  uint32_t* run = dalloc<uint32_t>(N);
-  uint8_t* runStart = dalloc<uint8_t>(N);
+  uint32_t* run = ds.alloc<uint32_t>(N);
+  uint8_t* runStart = ds.alloc<uint8_t>(N);
   hipLaunchKernelGGL(pathRunKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)sK,
                      (const uint8_t*)sD, uint32_t(N), run, runStart);
   HIPCHK(hipGetLastError());
   const uint32_t R = scanArray(T.stream, run, N, T.totals, T.scal + 6, T.hscal + 6);
   hipLaunchKernelGGL(depthBoxKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, run,
                      (const uint8_t*)runStart, uint32_t(N));
-  unsigned long long* rb = dalloc<unsigned long long>(R);
-  uint32_t* rd = dalloc<uint32_t>(R);
-  uint32_t* rf = dalloc<uint32_t>(R);
+  unsigned long long* rb = ds.alloc<unsigned long long>(R);
+  uint32_t* rd = ds.alloc<uint32_t>(R);
+  uint32_t* rf = ds.alloc<uint32_t>(R);
   HIPCHK(hipMemsetAsync(rb, 0, size_t(R) * 8, T.stream));
   HIPCHK(hipMemsetAsync(rd, 0, size_t(R) * 4, T.stream));
   HIPCHK(hipMemsetAsync(rf, 0, size_t(R) * 4, T.stream));
   hipLaunchKernelGGL(pathReduceKernel, dim3(ceilDiv(N, kBlock)), dim3(kBlock), 0, T.stream, (const uint64_t*)sK,
                      (const uint8_t*)sD, (const uint8_t*)sF, (const uint32_t*)run, uint32_t(N), rb, rd, rf);
-  uint32_t* keep = dalloc<uint32_t>(R);
+  uint32_t* keep = ds.alloc<uint32_t>(R);
   hipLaunchKernelGGL(pathKeepKernel, dim3(ceilDiv(R, kBlock)), dim3(kBlock), 0, T.stream,
                      (const unsigned long long*)rb, (const uint32_t*)rd, R, keep);
   HIPCHK(hipGetLastError());
@@ -1322,10 +1320,6 @@ void insertCore(b12x_tree_s& T, uint64_t P, const uint64_t* pk, const uint32_t* 
   T.cur = 1 - T.cur;
   T.n = K;
   ++T.leafGen;
-  for (void* q : {(void*)ins, (void*)insU32, (void*)newK, (void*)newD, (void*)newF, (void*)nsK, (void*)nsD,
-                  (void*)nsF, (void*)sK, (void*)sD, (void*)sF, (void*)run, (void*)runStart, (void*)rb, (void*)rd,
-                  (void*)rf, (void*)keep})
-    dfree(q);
 }
 
 }  // namespace
@@ -1356,15 +1350,16 @@ int b12x_delete_depth(b12x_tree t, uint8_t d, uint32_t* ret) {
     }
     const uint32_t n = T.n;
     const uint8_t d1 = uint8_t(d - 1);
-    uint32_t* run = dalloc<uint32_t>(n);
-    uint8_t* runStart = dalloc<uint8_t>(n);
+    DevScope ds;
+    uint32_t* run = ds.alloc<uint32_t>(n);
+    uint8_t* runStart = ds.alloc<uint8_t>(n);
     hipLaunchKernelGGL(truncRunKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, T.view(T.cur), n, d1, run,
                        runStart);
     HIPCHK(hipGetLastError());
     const uint32_t m = scanArray(T.stream, run, n, T.totals, T.scal + 4, T.hscal + 4);
     hipLaunchKernelGGL(depthBoxKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream, run,
                        (const uint8_t*)runStart, n);
-    uint32_t* acc = dalloc<uint32_t>(std::max<uint32_t>(m, 1));
+    uint32_t* acc = ds.alloc<uint32_t>(std::max<uint32_t>(m, 1));
     hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(std::max<uint32_t>(m, 1), kBlock)), dim3(kBlock), 0, T.stream, acc,
                        uint64_t(std::max<uint32_t>(m, 1)), 0xFFu);
     hipLaunchKernelGGL(depthCombineKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, T.stream,
@@ -1374,9 +1369,6 @@ int b12x_delete_depth(b12x_tree t, uint8_t d, uint32_t* ret) {
                        (const uint32_t*)acc);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(T.stream));
-    dfree(run);
-    dfree(runStart);
-    dfree(acc);
     T.cur = 1 - T.cur;
     T.n = m;
     ++T.leafGen;
@@ -2389,10 +2381,11 @@ int b12x_coo_make_symmetric(b12x_coo m, int antisymmetric) {
     M.nnz = n2;
     M.rowMajor = M.colMajor = false;
     if (int e = b12x_coo_sort(m, wasRowMajor ? 1 : 0)) return e;
-    double* sum = dalloc<double>(n2);
-    uint32_t* pos = dalloc<uint32_t>(n2);
-    uint32_t* tt = dalloc<uint32_t>(uint64_t(ceilDiv(n2, kTile)) + 16);
-    uint32_t* dTot = dalloc<uint32_t>(1);
+    DevScope ds;
+    double* sum = ds.alloc<double>(n2);
+    uint32_t* pos = ds.alloc<uint32_t>(n2);
+    uint32_t* tt = ds.alloc<uint32_t>(uint64_t(ceilDiv(n2, kTile)) + 16);
+    uint32_t* dTot = ds.alloc<uint32_t>(1);
     uint32_t hTot = 0;
     hipLaunchKernelGGL(cooRunSumKernel, dim3(ceilDiv(n2, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)M.rows,
                        (const uint32_t*)M.cols, (const double*)M.vals, n2, sum, pos);
@@ -2406,8 +2399,7 @@ int b12x_coo_make_symmetric(b12x_coo m, int antisymmetric) {
                        n2, r2, c2, v2);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    for (void* q : {(void*)M.rows, (void*)M.cols, (void*)M.vals, (void*)sum, (void*)pos, (void*)tt, (void*)dTot})
-      dfree(q);
+    for (void* q : {(void*)M.rows, (void*)M.cols, (void*)M.vals}) dfree(q);
     M.rows = r2;
     M.cols = c2;
     M.vals = v2;

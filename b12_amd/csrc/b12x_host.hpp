@@ -236,6 +236,28 @@ inline void dfree(void* p) {
   if (p) DeviceCache::get().release(p);
 }
 
+// Device blocks owned by a scope: returned to the cache when the scope ends, also when an error
+// is thrown part-way (work still queued on them is covered by the cache's fences).
+class DevScope {
+ public:
+  DevScope() = default;
+  DevScope(const DevScope&) = delete;
+  DevScope& operator=(const DevScope&) = delete;
+  ~DevScope() {
+    for (void* p : p_) dfree(p);
+  }
+  template <class T>
+  T* alloc(size_t count) {
+    p_.reserve(p_.size() + 1);  // the push below cannot fail after the block is taken
+    T* q = dalloc<T>(count);
+    p_.push_back(q);
+    return q;
+  }
+
+ private:
+  std::vector<void*> p_;
+};
+
 // a stream of one of the library's objects: registered for the cache's fences on creation,
 // synchronised and retired before the object frees its memory
 inline void streamCreate(hipStream_t* s) {

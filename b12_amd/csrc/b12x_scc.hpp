@@ -222,10 +222,11 @@ inline SccStats sccLabels(hipStream_t st, uint32_t n, const uint32_t* inPtr, con
   SccStats s;
   if (n == 0) return s;
   const uint32_t vb = ceilDiv(n, kBlock);
-  uint32_t* col = dalloc<uint32_t>(n);
-  uint8_t* hasOut = dalloc<uint8_t>(n);
-  uint8_t* hasIn = dalloc<uint8_t>(n);
-  uint8_t* vis = dalloc<uint8_t>(n);
+  DevScope ds;
+  uint32_t* col = ds.alloc<uint32_t>(n);
+  uint8_t* hasOut = ds.alloc<uint8_t>(n);
+  uint8_t* hasIn = ds.alloc<uint8_t>(n);
+  uint8_t* vis = ds.alloc<uint8_t>(n);
   SccCounters cnt;
   uint32_t h[2];
   HIPCHK(hipMemsetAsync(lab, 0xFF, size_t(n) * 4, st));
@@ -308,10 +309,6 @@ inline SccStats sccLabels(hipStream_t st, uint32_t n, const uint32_t* inPtr, con
   cnt.read(st, h);
   s.components = h[0];
   s.nontrivial = h[1];
-  dfree(col);
-  dfree(hasOut);
-  dfree(hasIn);
-  dfree(vis);
   return s;
 }
 
