@@ -87,7 +87,7 @@ def measure_roofline(job):
     n, nnz, iters, ms = job["boxes"], job["nnz"], job["iters"], job["pi_kernel_ms"]
     alg = iters * (12 * nnz + 16 * n)
     achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "latency (one global reduction per iteration); reported against HBM",
+    return {"bound": "latency (the vector crosses XCDs once per iteration: stores, then gathers); reported against HBM",
             "kernel": "piCoopKernel", "achieved": achieved, "peak": PEAK_HBM_BYTES / 1e9, "unit": "GB/s",
             "frac": achieved * 1e9 / PEAK_HBM_BYTES, "traffic": profiled_traffic("piCoopKernel"),
             "launch_ms": ms, "iterations": iters, "us_per_iteration": ms * 1e3 / iters if iters else None,

@@ -329,8 +329,9 @@ def test_power_iteration_fused_and_two_kernel_paths_agree(monkeypatch):
 
 def test_persistent_power_iteration_agrees_with_batched_form(monkeypatch):
     """The single-launch (cooperative) power iteration against the batched step+reduce form and
-    the oracle: a subdivision matrix and an uploaded 300k-column stochastic matrix (more columns
-    than the persistent grid has lanes), iteration limits 1, 2, 3, 50 and a 1e-12 tolerance."""
+    the oracle: a subdivision matrix, an uploaded 300k-column stochastic matrix (more columns
+    than the persistent grid has lanes) and a matrix with columns longer than the entries a lane
+    keeps in registers; iteration limits 1, 2, 3, 50 and a 1e-12 tolerance."""
     print("[pi] persistent vs batched", flush=True)
     g, o = subdivision_run([0.0, 0.0], [1.5, 0.5], gx.InnerGrid(4), gx.Henon(), 13)
     cases = [(g.transitionMatrixForLeaves(gx.InnerGrid(4), gx.Henon()),
@@ -347,6 +348,15 @@ def test_persistent_power_iteration_agrees_with_batched_form(monkeypatch):
     keep[1:] = (rows[1:] != rows[:-1]) | (cols[1:] != cols[:-1])
     rows, cols, vals = rows[keep], cols[keep], vals[keep]
     from oracle_api import OCoo
+    cases.append((gx.CooMatrix.from_arrays(rows, cols, vals, n, n), OCoo.from_arrays(rows, cols, vals, n, n)))
+    # columns of 1 to 37 entries: the longer ones run past the entries a lane keeps in registers
+    n = 4000
+    lens = 1 + (np.arange(n) * 7919) % 37
+    cols = np.repeat(np.arange(n, dtype=np.uint32), lens)
+    rows = np.concatenate([rng.choice(n, k, replace=False) for k in lens]).astype(np.uint32)
+    vals = rng.random(len(rows)) + 0.01
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
     cases.append((gx.CooMatrix.from_arrays(rows, cols, vals, n, n), OCoo.from_arrays(rows, cols, vals, n, n)))
     for mg, mo in cases:
         for mi, tol in ((1, -1.0), (2, -1.0), (3, -1.0), (50, -1.0), (100000, 1e-12)):
