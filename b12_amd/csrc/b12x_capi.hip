@@ -2467,6 +2467,9 @@ struct Csc {
   uint32_t* colPtr = nullptr;  // n + 1
 };
 
+// The caller owns the arrays (adoptCsc hands them to a DevScope). They are allocated here
+// directly: the radix sorts swap their key and payload pointers with buffers of their own and
+// free the originals, so only the final pointers may be owned by anyone.
 Csc buildCsc(b12x_coo_s& M, uint32_t n) {
   hipStream_t st = M.stream;
   std::vector<uint32_t> hT(8);
@@ -2500,18 +2503,27 @@ Csc buildCsc(b12x_coo_s& M, uint32_t n) {
   return C;
 }
 
+Csc adoptCsc(Csc C, DevScope& ds) {
+  ds.adopt(C.keys);
+  ds.adopt(C.rows);
+  ds.adopt(C.vals);
+  ds.adopt(C.colPtr);
+  return C;
+}
+
 template <int M>
 void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* basis) {
   hipStream_t st = mat.stream;
-  Csc C = buildCsc(mat, n);
+  DevScope ds;
+  Csc C = adoptCsc(buildCsc(mat, n), ds);
   const uint32_t nb = ceilDiv(n, kBlock);
   const uint32_t nd = std::min<uint32_t>(nb, kDotBlocks);
-  double* X = dalloc<double>(uint64_t(M) * n);
-  double* Y = dalloc<double>(uint64_t(M) * n);
-  double* part = dalloc<double>(uint64_t(M) * M * nd);
-  double* R = dalloc<double>(uint64_t(M) * M);
-  double* H = dalloc<double>(uint64_t(M) * M);
-  int* bad = dalloc<int>(1);
+  double* X = ds.alloc<double>(uint64_t(M) * n);
+  double* Y = ds.alloc<double>(uint64_t(M) * n);
+  double* part = ds.alloc<double>(uint64_t(M) * M * nd);
+  double* R = ds.alloc<double>(uint64_t(M) * M);
+  double* H = ds.alloc<double>(uint64_t(M) * M);
+  int* bad = ds.alloc<int>(1);
   HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), st));
   auto cholQR = [&](double* Z) {  // in place
     hipLaunchKernelGGL((dotPartialKernel<M, 1>), dim3(nd), dim3(kBlock), 0, st, (const double*)Z, (const double*)Z, n,
@@ -2543,8 +2555,6 @@ void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* b
   HIPCHK(hipMemcpyAsync(ritz, H, uint64_t(M) * M * 8, hipMemcpyDeviceToHost, st));
   if (basis) HIPCHK(hipMemcpyAsync(basis, Y, uint64_t(M) * n * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  dfree(X); dfree(Y); dfree(part); dfree(R); dfree(H); dfree(bad);
-  dfree(C.keys); dfree(C.rows); dfree(C.vals); dfree(C.colPtr);
   if (hbad) throw HipFail(B12X_E_ARG, "subspace iteration: Gram matrix not positive definite");
 }
 
@@ -2581,15 +2591,16 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
     const uint32_t n = m->nRows;
     hipStream_t st = m->stream;
-    Csc C = buildCsc(*m, n);
-    uint32_t *ck = C.keys, *cr = C.rows, *colPtr = C.colPtr;
+    DevScope ds;
+    Csc C = adoptCsc(buildCsc(*m, n), ds);
+    uint32_t *cr = C.rows, *colPtr = C.colPtr;
     double* vals = C.vals;
-    double* mu0 = dalloc<double>(n);
-    double* ybuf = dalloc<double>(uint64_t(n) * 3);
+    double* mu0 = ds.alloc<double>(n);
+    double* ybuf = ds.alloc<double>(uint64_t(n) * 3);
     const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
     const uint32_t nbStep = std::max<uint32_t>(1, ceilDiv(uint64_t(n) * kPiLanes, kBlock));
-    double* partial = dalloc<double>(uint64_t(nbStep) * 2);
-    PiState* pst = dalloc<PiState>(1);
+    double* partial = ds.alloc<double>(uint64_t(nbStep) * 2);
+    PiState* pst = ds.alloc<PiState>(1);
     PiState init{};
     init.t = 1;
     init.resid = -1.0;
@@ -2612,8 +2623,8 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     double* partF = nullptr;
     uint32_t* blockIter = nullptr;
     if (fused) {
-      partF = dalloc<double>(uint64_t(nbFused) * 4);
-      blockIter = dalloc<uint32_t>(nbFused);
+      partF = ds.alloc<double>(uint64_t(nbFused) * 4);
+      blockIter = ds.alloc<uint32_t>(nbFused);
       HIPCHK(hipMemsetAsync(blockIter, 0, nbFused * 4, st));
       HIPCHK(hipMemsetAsync(partF, 0, uint64_t(nbFused) * 4 * 8, st));
     }
@@ -2637,8 +2648,8 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     if (!fused && !init.done && !(pp && pp[0] == '0')) coopBlocks = coopGrid(n);
     if (coopBlocks) {
       const uint64_t nPart = uint64_t(coopBlocks) * 6 + kPiResWords;  // partial slots, then the result slots
-      uint64_t* part = dalloc<uint64_t>(nPart);
-      int* err = dalloc<int>(1);
+      uint64_t* part = ds.alloc<uint64_t>(nPart);
+      int* err = ds.alloc<int>(1);
       hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nPart, kBlock)), dim3(kBlock), 0, st, part, nPart, kCoopEmpty);
       if (n)  // every y slot starts empty
         hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(uint64_t(n) * 3, kBlock)), dim3(kBlock), 0, st,
@@ -2663,7 +2674,6 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       lastPiKernelMs = kms;
-      dfree(part); dfree(err);
       // a wait past its limit (other work holding the CUs): every block has left; recompute in
       // batches (B12X_PI_PERSISTENT=2: report it instead)
       if (herr && pp && pp[0] == '2') throw HipFail(B12X_E_HIP, "persistent power iteration: a wait passed its limit");
@@ -2686,7 +2696,7 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    double* dmu = dalloc<double>(n);
+    double* dmu = ds.alloc<double>(n);
     if (n) hipLaunchKernelGGL(piFinishKernel, dim3(nb), dim3(kBlock), 0, st, ybuf, mu0, n, pst, dmu);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
@@ -2694,8 +2704,6 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     HIPCHK(hipStreamSynchronize(st));
     if (iters) *iters = int(hs.final);
     if (resid) *resid = hs.resid;
-    dfree(mu0); dfree(ybuf); dfree(partial); dfree(pst); dfree(dmu); dfree(colPtr); dfree(partF); dfree(blockIter);
-    dfree(ck); dfree(cr); dfree(vals);
     return B12X_OK;
   });
 }

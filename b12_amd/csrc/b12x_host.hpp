@@ -253,6 +253,16 @@ class DevScope {
     p_.push_back(q);
     return q;
   }
+  // takes ownership of a block from dalloc (nullptr is ignored)
+  void adopt(void* q) {
+    if (!q) return;
+    try {
+      p_.push_back(q);
+    } catch (...) {
+      dfree(q);
+      throw;
+    }
+  }
 
  private:
   std::vector<void*> p_;
