@@ -298,6 +298,8 @@ class CooMatrix:
         return r, c, v
 
     def power_iteration(self, max_iter=200, tol=-1.0, mu=None):
+        if mu is not None and np.shape(mu) != (self.nRows,):
+            raise ValueError(f"mu must hold nRows = {self.nRows} values, got shape {np.shape(mu)}")
         mu_arr = np.zeros(self.nRows, np.float64) if mu is None else np.ascontiguousarray(mu, np.float64).copy()
         it, res = _I(), _D()
         self.api("b12x_power_iteration", self.h, int(max_iter), float(tol), 0 if mu is None else 1, _ptr(mu_arr),

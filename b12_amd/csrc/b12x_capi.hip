@@ -2558,21 +2558,166 @@ void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* b
   if (hbad) throw HipFail(B12X_E_ARG, "subspace iteration: Gram matrix not positive definite");
 }
 
-// Blocks for the persistent power iteration over n columns: one lane per column up to
-// kCoopMaxBlocks blocks, never more than the device holds resident at once, so that on an idle
-// device every block runs from the start (0: use the batched form).
+// Worker blocks for the persistent power iteration over n columns: one wave per group of 64
+// columns up to kFlowMaxWorkers blocks, never more than the device holds resident at once with
+// the aggregator block beside them, so that on an idle device every block runs from the start.
+// 0: use the batched form (no columns, or n >= 2^29, beyond the 32-bit byte offsets of the
+// cached entries).
 thread_local double lastPiKernelMs = 0.0;  // duration of the calling thread's last persistent launch
 
-uint32_t coopGrid(uint32_t n) {
-  int dev = 0, cus = 0, occ = 0;
+uint32_t piFlowWorkers(uint32_t n) {
+  if (n == 0 || n >= (1u << 29)) return 0;
+  int dev = 0, cus = 0, occ = 0, occ2 = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&piCoopKernel<kCoopMaxBlocks / kWave>), kBlock, 0) !=
-          hipSuccess ||
-      occ <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&piFlowKernel<true>), kBlock,
+                                                   0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, reinterpret_cast<const void*>(&piFlowKernel<false>), kBlock,
+                                                   0) != hipSuccess)
     return 0;
-  const uint64_t want = std::max<uint64_t>(1, ceilDiv(uint64_t(n), kBlock));
-  return uint32_t(std::min<uint64_t>({want, uint64_t(kCoopMaxBlocks), uint64_t(occ) * uint64_t(cus)}));
+  occ = std::min(occ, occ2);
+  if (occ <= 0) return 0;
+  const uint64_t resident = uint64_t(occ) * uint64_t(cus);
+  if (resident < 2) return 0;
+  const uint64_t want = ceilDiv(uint64_t(n), kBlock);
+  return uint32_t(std::min<uint64_t>({want, uint64_t(kFlowMaxWorkers), resident - 1}));
+}
+
+// Device buffers of one solve (sizes for n columns; the persistent form's only when workers > 0).
+struct PiWork {
+  double* mu0 = nullptr;      // n
+  double* ybuf = nullptr;     // 3 n: the batched form's z slots
+  double* partial = nullptr;  // 2 ceil(n / kBlock): the batched form's chunk partials
+  PiState* pst = nullptr;
+  uint64_t* ring = nullptr;   // kFlowRing n: the persistent form's z slots
+  uint64_t* part = nullptr;   // kFlowPart ceil(n / 64) 2: its group partials
+  uint64_t* tot = nullptr;    // kFlowTotWords: its totals
+  int* err = nullptr;
+  uint32_t* maxLen = nullptr; // 1: the longest column
+  double* dmu = nullptr;      // n: the result
+};
+
+// The solve on a CSC matrix (colPtr n + 1, rows ascending within each column): the persistent
+// form when workers > 0 and B12X_PI_PERSISTENT is not '0', with the batched form as the fallback
+// when a bounded wait gave up (B12X_PI_PERSISTENT=2: an error instead); the batched form alone
+// otherwise, its iterations launched in batches from a captured HIP graph unless useGraph is
+// false. mu (host, n doubles) is the start vector when muInit, and receives the measure.
+// spinLimit bounds every wait of the persistent form (polls).
+void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const double* vals, uint32_t n,
+             int maxIter, double tol, int muInit, double* mu, int* iters, double* resid, const PiWork& w,
+             uint32_t workers, bool useGraph, uint32_t spinLimit) {
+  lastPiKernelMs = 0.0;
+  const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
+  PiState init{};
+  init.t = 1;
+  init.resid = -1.0;
+  init.done = (maxIter <= 0 || n == 0) ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(w.pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
+  if (muInit && n) HIPCHK(hipMemcpyAsync(w.mu0, mu, size_t(n) * 8, hipMemcpyHostToDevice, st));
+  else if (n) hipLaunchKernelGGL(fillF64Kernel, dim3(nb), dim3(kBlock), 0, st, w.mu0, n, 1.0 / double(n));
+  const uint32_t mi = uint32_t(std::max(maxIter, 0));
+  const char* pp = std::getenv("B12X_PI_PERSISTENT");
+  bool done = init.done != 0;
+  if (!done && workers && !(pp && pp[0] == '0')) {
+    const uint32_t nG = ceilDiv(n, kWave);
+    const uint64_t nRing = uint64_t(kFlowRing) * n, nPart = uint64_t(kFlowPart) * nG * 2;
+    hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nRing, kBlock)), dim3(kBlock), 0, st, w.ring, nRing, kCoopEmpty);
+    hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nPart, kBlock)), dim3(kBlock), 0, st, w.part, nPart, kCoopEmpty);
+    hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(uint64_t(kFlowTotWords), kBlock)), dim3(kBlock), 0, st, w.tot,
+                       uint64_t(kFlowTotWords), kCoopEmpty);
+    hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.err, uint64_t(1), 0u);
+    // the straight-line form when every wave owns one group and every column fits the cache
+    hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, w.maxLen, uint64_t(1), 0u);
+    hipLaunchKernelGGL(colMaxLenKernel, dim3(std::min<uint32_t>(nb, 1024)), dim3(kBlock), 0, st, colPtr, n, w.maxLen);
+    HIPCHK(hipGetLastError());
+    uint32_t maxLen = 0;
+    HIPCHK(hipMemcpyAsync(&maxLen, w.maxLen, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const bool full = maxLen > uint32_t(kCoopCache) || uint64_t(nG) > 4ull * workers;
+    PiFlowArgs fa{colPtr, rows, vals, w.mu0, w.ring, w.part, w.tot, w.pst, w.err, n, nG, workers, mi, tol, spinLimit};
+    hipEvent_t e0 = nullptr, e1 = nullptr;  // the launch's duration, for measurement
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    if (full) hipLaunchKernelGGL(piFlowKernel<true>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
+    else hipLaunchKernelGGL(piFlowKernel<false>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, st));
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float kms = 0.f;
+    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    lastPiKernelMs = kms;
+    // a wait past its limit (other work holding the CUs): every wave has left; recompute in
+    // batches (B12X_PI_PERSISTENT=2: report it instead)
+    if (herr && pp && pp[0] == '2') throw HipFail(B12X_E_HIP, "persistent power iteration: a wait passed its limit");
+    if (!herr) {
+      done = true;
+    } else {
+      lastPiKernelMs = 0.0;
+      HIPCHK(hipMemcpyAsync(w.pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    }
+  }
+  const double* zbase = w.ybuf;
+  if (done && !init.done && workers && !(pp && pp[0] == '0')) zbase = reinterpret_cast<const double*>(w.ring);
+  if (!done) {
+    // batched form: step + reduce per iteration, 32 iterations per launch of a captured graph
+    // (B12X_GRAPHS=0: eager launches); the kernels do nothing once the loop has ended
+    const int batch = 32;
+    const char* ge = std::getenv("B12X_GRAPHS");
+    useGraph = useGraph && !(ge && ge[0] == '0');
+    auto enqueueBatch = [&]() {
+      for (int b = 0; b < batch; ++b) {
+        hipLaunchKernelGGL(piStepKernel, dim3(nb), dim3(kBlock), 0, st, colPtr, rows, vals, (const double*)w.mu0,
+                           w.ybuf, n, w.partial, w.partial + nb, (const PiState*)w.pst);
+        hipLaunchKernelGGL(piReduceKernel, dim3(1), dim3(kWave), 0, st, (const double*)w.partial,
+                           (const double*)(w.partial + nb), nb, w.pst, mi, tol);
+      }
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (useGraph) {
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueueBatch();
+      HIPCHK(hipStreamEndCapture(st, &graph));
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    PiState hs{};
+    for (int64_t it = 0; it <= int64_t(mi) + 2; it += batch) {
+      if (useGraph) HIPCHK(hipGraphLaunch(exec, st));
+      else enqueueBatch();
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&hs, w.pst, sizeof(hs), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (hs.done) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  PiState hs{};
+  if (n) hipLaunchKernelGGL(piFinishKernel, dim3(nb), dim3(kBlock), 0, st, zbase, (const double*)w.mu0, n,
+                           (const PiState*)w.pst, w.dmu);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&hs, w.pst, sizeof(hs), hipMemcpyDeviceToHost, st));
+  if (n) HIPCHK(hipMemcpyAsync(mu, w.dmu, size_t(n) * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (iters) *iters = int(hs.final);
+  if (resid) *resid = hs.resid;
+}
+
+// Spin limit of the persistent form's waits (polls): B12X_PI_SPIN overrides the default, so a
+// test can make every wait give up and take the fallback.
+uint32_t piSpinLimit() {
+  const char* e = std::getenv("B12X_PI_SPIN");
+  if (e && e[0]) {
+    char* end = nullptr;
+    const unsigned long v = std::strtoul(e, &end, 10);
+    if (end && *end == 0 && v < (1ul << 31)) return uint32_t(v);
+  }
+  return 1u << 21;
 }
 }  // namespace
 
@@ -2593,117 +2738,22 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     hipStream_t st = m->stream;
     DevScope ds;
     Csc C = adoptCsc(buildCsc(*m, n), ds);
-    uint32_t *cr = C.rows, *colPtr = C.colPtr;
-    double* vals = C.vals;
-    double* mu0 = ds.alloc<double>(n);
-    double* ybuf = ds.alloc<double>(uint64_t(n) * 3);
-    const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
-    const uint32_t nbStep = std::max<uint32_t>(1, ceilDiv(uint64_t(n) * kPiLanes, kBlock));
-    double* partial = ds.alloc<double>(uint64_t(nbStep) * 2);
-    PiState* pst = ds.alloc<PiState>(1);
-    PiState init{};
-    init.t = 1;
-    init.resid = -1.0;
-    init.done = (maxIter <= 0 || n == 0) ? 1 : 0;
-    HIPCHK(hipMemcpyAsync(pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
-    if (muInit) HIPCHK(hipMemcpyAsync(mu0, mu, size_t(n) * 8, hipMemcpyHostToDevice, st));
-    else if (n) hipLaunchKernelGGL(fillF64Kernel, dim3(nb), dim3(kBlock), 0, st, mu0, n, 1.0 / double(n));
-    PiState hs{};
-    // iterations are launched in batches from a captured HIP graph (B12X_GRAPHS=0: eager launches)
-    const int batch = 32;
-    const char* ge = std::getenv("B12X_GRAPHS");
-    const bool useGraph = !(ge && ge[0] == '0');
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    // small matrices: one fused kernel per iteration; larger ones: step + single-block reduce
-    const uint32_t nbFused = std::max<uint32_t>(1, ceilDiv(uint64_t(n) * kPiLanes, kPiBlock));
-    // the fused form is opt-in (B12X_PI_FUSED=1): on MI355X it measured slower than step+reduce
-    const char* pf = std::getenv("B12X_PI_FUSED");
-    const bool fused = nbFused <= kPiFusedMaxBlocks && pf && pf[0] == '1';
-    double* partF = nullptr;
-    uint32_t* blockIter = nullptr;
-    if (fused) {
-      partF = ds.alloc<double>(uint64_t(nbFused) * 4);
-      blockIter = ds.alloc<uint32_t>(nbFused);
-      HIPCHK(hipMemsetAsync(blockIter, 0, nbFused * 4, st));
-      HIPCHK(hipMemsetAsync(partF, 0, uint64_t(nbFused) * 4 * 8, st));
+    PiWork w;
+    w.mu0 = ds.alloc<double>(n);
+    w.ybuf = ds.alloc<double>(uint64_t(n) * 3);
+    w.partial = ds.alloc<double>(uint64_t(std::max<uint32_t>(1, ceilDiv(n, kBlock))) * 2);
+    w.pst = ds.alloc<PiState>(1);
+    w.dmu = ds.alloc<double>(n);
+    const uint32_t workers = (maxIter > 0) ? piFlowWorkers(n) : 0;
+    if (workers) {
+      w.ring = ds.alloc<uint64_t>(uint64_t(kFlowRing) * n);
+      w.part = ds.alloc<uint64_t>(uint64_t(kFlowPart) * ceilDiv(n, kWave) * 2);
+      w.tot = ds.alloc<uint64_t>(kFlowTotWords);
+      w.err = ds.alloc<int>(1);
+      w.maxLen = ds.alloc<uint32_t>(1);
     }
-    auto enqueueBatch = [&]() {
-      for (int b = 0; b < batch; ++b) {
-        if (fused) {
-          hipLaunchKernelGGL(piFusedKernel, dim3(nbFused), dim3(kPiBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n,
-                             partF, blockIter, pst, uint32_t(std::max(maxIter, 0)), tol);
-        } else {
-          hipLaunchKernelGGL(piStepKernel, dim3(nbStep), dim3(kBlock), 0, st, colPtr, cr, vals, mu0, ybuf, n,
-                             partial, partial + nbStep, (const PiState*)pst);
-          hipLaunchKernelGGL(piReduceKernel, dim3(1), dim3(kBlock), 0, st, (const double*)partial,
-                             (const double*)(partial + nbStep), nbStep, pst, uint32_t(std::max(maxIter, 0)), tol);
-        }
-      }
-    };
-    // default: one persistent launch for the whole loop (B12X_PI_PERSISTENT=0: batches; =2: the
-    // persistent form or an error)
-    const char* pp = std::getenv("B12X_PI_PERSISTENT");
-    uint32_t coopBlocks = 0;
-    if (!fused && !init.done && !(pp && pp[0] == '0')) coopBlocks = coopGrid(n);
-    if (coopBlocks) {
-      const uint64_t nPart = uint64_t(coopBlocks) * 6 + kPiResWords;  // partial slots, then the result slots
-      uint64_t* part = ds.alloc<uint64_t>(nPart);
-      int* err = ds.alloc<int>(1);
-      hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nPart, kBlock)), dim3(kBlock), 0, st, part, nPart, kCoopEmpty);
-      if (n)  // every y slot starts empty
-        hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(uint64_t(n) * 3, kBlock)), dim3(kBlock), 0, st,
-                           reinterpret_cast<uint64_t*>(ybuf), uint64_t(n) * 3, kCoopEmpty);
-      hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)err, uint64_t(1), 0u);
-      HIPCHK(hipGetLastError());
-      PiCoopArgs ca{colPtr, cr, vals, mu0, ybuf, part, part + uint64_t(coopBlocks) * 6, pst, err, n,
-                    uint32_t(std::max(maxIter, 0)), tol, 1u << 21};
-      hipEvent_t e0 = nullptr, e1 = nullptr;  // the launch's duration, for measurement
-      HIPCHK(hipEventCreate(&e0));
-      HIPCHK(hipEventCreate(&e1));
-      HIPCHK(hipEventRecord(e0, st));
-      if (coopBlocks <= 4 * kWave) hipLaunchKernelGGL(piCoopKernel<4>, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
-      else hipLaunchKernelGGL(piCoopKernel<kCoopMaxBlocks / kWave>, dim3(coopBlocks), dim3(kBlock), 0, st, ca);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(e1, st));
-      int herr = 0;
-      HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      float kms = 0.f;
-      HIPCHK(hipEventElapsedTime(&kms, e0, e1));
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      lastPiKernelMs = kms;
-      // a wait past its limit (other work holding the CUs): every block has left; recompute in
-      // batches (B12X_PI_PERSISTENT=2: report it instead)
-      if (herr && pp && pp[0] == '2') throw HipFail(B12X_E_HIP, "persistent power iteration: a wait passed its limit");
-      if (!herr) init.done = 1;
-      else HIPCHK(hipMemcpyAsync(pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
-    }
-    if (useGraph && !init.done) {
-      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueueBatch();
-      HIPCHK(hipStreamEndCapture(st, &graph));
-      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
-    for (int64_t it = 0; !init.done && it <= int64_t(maxIter) + 2; it += batch) {
-      if (useGraph) HIPCHK(hipGraphLaunch(exec, st));
-      else enqueueBatch();
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      if (hs.done) break;
-    }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    double* dmu = ds.alloc<double>(n);
-    if (n) hipLaunchKernelGGL(piFinishKernel, dim3(nb), dim3(kBlock), 0, st, ybuf, mu0, n, pst, dmu);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&hs, pst, sizeof(hs), hipMemcpyDeviceToHost, st));
-    if (n) HIPCHK(hipMemcpyAsync(mu, dmu, size_t(n) * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (iters) *iters = int(hs.final);
-    if (resid) *resid = hs.resid;
+    piSolve(st, C.colPtr, C.rows, C.vals, n, maxIter, tol, muInit, mu, iters, resid, w, workers, true,
+            piSpinLimit());
     return B12X_OK;
   });
 }

@@ -421,7 +421,7 @@ void poolJoin(b12x_forest_s& F, uint32_t m, uint64_t nnz, const uint32_t* colPtr
     pj.hi += n0;
     pj.elo += e0;
     pj.ehi += e0;
-    jd[r] = PoolJobDev{pj.lo, pj.hi, 1u, 0, 0u, b, {0.0, 0.0, 0.0}, -1.0};
+    jd[r] = PoolJobDev{pj.lo, pj.hi, 1u, 0, 0u, b, {0.0, 0.0, 0.0}, -1.0, 0.0};
     b += poolBlocksOf(pj.hi - pj.lo);
     P.jobs.push_back(pj);
   }

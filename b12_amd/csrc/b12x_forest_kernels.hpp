@@ -192,6 +192,7 @@ struct PoolJobDev {
   uint32_t b0;      // first step-kernel block of this job
   double norm[3];
   double resid;
+  double nuLag;     // nu_t of the last t with t % 8 == 0 (the scale c_{t+8} derives from it)
 };
 
 __host__ __device__ inline uint32_t poolBlocksOf(uint32_t len) { return (len + kPoolCols - 1) / kPoolCols; }
@@ -205,10 +206,12 @@ static __global__ __launch_bounds__(kBlock) void poolJobOfBlockKernel(const Pool
   for (uint32_t q = threadIdx.x; q < nb; q += kBlock) jobOfBlock[b0 + q] = r;
 }
 
-// y_t for kPoolCols columns of one job, one lane per column: the lane adds its column's products
-// in ascending-row order (the oracle's order) with kPoolUnroll entries' loads in flight at a time.
-// Also the block's partial sums of |y_t| and of the residual of iteration t-1
-// (sum |mu_{t-1} - mu_{t-2}|), mu_s = y_s / norm_s, mu_0 uniform.
+// y_t for kPoolCols columns of one job, one lane per column: the power iteration of
+// b12x_kernels.hpp (unnormalised iterates y_t = c_t y_{t-1} P, c_t = piScale) over the job's
+// positions. The lane adds its column's products in ascending-row order (the oracle's order) with
+// kPoolUnroll entries' loads in flight at a time. Also the block's partial sums of |y_t| and of
+// the residual terms of iteration t-1, |y_{t-1} / nu_{t-1} - y_{t-2} / nu_{t-2}|, y_0 = mu_0
+// uniform, nu_0 = 1.
 constexpr int kPoolUnroll = 8;
 
 static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* __restrict__ colPtr,
@@ -228,6 +231,7 @@ static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* 
   const double nPrev = t >= 2 ? J.norm[s1] : 1.0;
   const double nPrev2 = t >= 3 ? J.norm[s2] : 1.0;
   const double mu0 = 1.0 / double(J.hi - J.lo);
+  const double c = piScale(t, J.nuLag);
   const uint32_t p = J.lo + (blockIdx.x - J.b0) * kPoolCols + threadIdx.x;
   const double* yPrev = y + uint64_t(s1) * ys;
   const double* yPrev2 = y + uint64_t(s2) * ys;
@@ -251,11 +255,12 @@ static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* 
       for (int u = 0; u < kPoolUnroll; ++u) m[u] = yPrev[i[u]];
 #pragma unroll
       for (int u = 0; u < kPoolUnroll; ++u) {
-        const double mu = (t == 1) ? mu0 : m[u] / nPrev;
-        const double prod = mu * v[u];
+        const double z = (t == 1) ? mu0 : m[u];
+        const double prod = z * v[u];
         s = (e + u < e1) ? s + prod : s;
       }
     }
+    if (c != 1.0) s *= c;
     y[uint64_t(s0) * ys + p] = s;
     sAbs = fabs(s);
     if (t >= 2) {
@@ -312,6 +317,7 @@ static __global__ __launch_bounds__(kWave) void poolReduceKernel(const double* p
   tr = poolLaneTreeSum(tr);
   if (lane == 0) {
     J.norm[t % 3] = ta;
+    if ((t & 7u) == 0) J.nuLag = ta;
     const uint32_t completed = t - 1;
     if (completed >= 1) {
       J.resid = tr;
@@ -327,7 +333,7 @@ static __global__ __launch_bounds__(kWave) void poolReduceKernel(const double* p
 // Persistent form of the pool for pools of at most kPoolCoopMaxBlocks step blocks (every block
 // resident): each job's blocks iterate together, synchronised only with one another through
 // the job's partial slots (buffer t % 3, kCoopEmpty until written, cleared one iteration ahead,
-// as in piCoopKernel), so jobs advance independently. Per launch a job runs until it stops or
+// as the persistent forms of b12x_kernels.hpp use them), so jobs advance independently. Per launch a job runs until it stops or
 // `budget` iterations are done; its first block then stores the job state. A wait past
 // spinLimit polls raises *err; the job's blocks all leave at that iteration, and its first block
 // stores the state as of the last completed iteration (y_{t-1}, y_{t-2} are intact).
@@ -365,6 +371,7 @@ static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) 
   uint32_t t = J->t;
   double nPrev = t >= 2 ? J->norm[(t + 2) % 3] : 1.0;
   double nPrev2 = t >= 3 ? J->norm[(t + 1) % 3] : 1.0;
+  double nuLag = J->nuLag;
   const double mu0 = 1.0 / double(hi - lo);
   const uint32_t p = lo + (b - b0) * kPoolCols + tid;
   const bool valid = p < hi;
@@ -399,6 +406,7 @@ static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) 
     const double* yPrev2 = a.y + uint64_t((t + 1) % 3) * a.ys;
     double* yCur = a.y + uint64_t(t % 3) * a.ys;
     double sAbs = 0.0, sRes = 0.0;
+    const double c = piScale(t, nuLag);
     if (valid) {
       const double yp = ldAgent(yPrev + p), yp2 = ldAgent(yPrev2 + p);
       double mv[kPoolCache];
@@ -414,8 +422,8 @@ static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) 
       double s = 0.0;
 #pragma unroll
       for (int u = 0; u < kPoolCache; ++u) {
-        const double mu = (t == 1) ? mu0 : mv[u] / nPrev;
-        const double prod = mu * cv[u];
+        const double z = (t == 1) ? mu0 : mv[u];
+        const double prod = z * cv[u];
         s = (c0 + u < c1) ? s + prod : s;
       }
       for (uint32_t e = c0 + kPoolCache; e < c1; e += 8) {
@@ -431,11 +439,12 @@ static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) 
         for (int u = 0; u < 8; ++u) mm[u] = ldAgent(yPrev + i[u]);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const double mu = (t == 1) ? mu0 : mm[u] / nPrev;
-          const double prod = mu * v[u];
+          const double z = (t == 1) ? mu0 : mm[u];
+          const double prod = z * v[u];
           s = (e + u < c1) ? s + prod : s;
         }
       }
+      if (c != 1.0) s *= c;
       stAgent(yCur + p, s);
       sAbs = fabs(s);
       if (t >= 2) {
@@ -495,10 +504,12 @@ static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) 
         J->t = t;
         J->norm[(t + 2) % 3] = nPrev;
         J->norm[(t + 1) % 3] = nPrev2;
+        J->nuLag = nuLag;
       }
       return;
     }
     const double ta = sums[0], tr = sums[1];
+    if ((t & 7u) == 0) nuLag = ta;
     const uint32_t completed = t - 1;
     const bool done = completed >= 1 && ((a.tol >= 0.0 && tr <= a.tol) || completed >= a.maxIter);
     if (done || m + 1 >= a.budget) {
@@ -506,6 +517,7 @@ static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) 
         J->norm[t % 3] = ta;
         J->norm[(t + 2) % 3] = nPrev;
         J->norm[(t + 1) % 3] = nPrev2;
+        J->nuLag = nuLag;
         if (completed >= 1) J->resid = tr;
         if (done) {
           J->done = 1;

@@ -7,9 +7,19 @@
 // quoted name, ": " and the value (arrays: "[", each element on its own line indented by four
 // spaces, separated by commas, "]" on its own line indented by two), entries separated by
 // commas, and "\n}" at the end. Doubles are printed as iostreams print them at
-// max_digits10 = 17 significant digits (%.17g); uint8 values as integers. The readers follow
-// the reference's tolerant scan: the value after `"name"` and a colon, an array as the comma-
-// separated tokens between the next '[' and ']' (so the matrix file's trailing comma is fine).
+// max_digits10 = 17 significant digits (%.17g), integers in decimal, and the tree's uint8 arrays
+// (sdScheme, depthVector, flagsVector: Depth, Dimension and Flags are uint8_t, TypeDefinitions.h:
+// 19,37, Flags.h:9) as one raw byte per element, which is what `os << value` inserts for a
+// uint8_t (helpFunctions.hpp:128). The readers follow the reference's tolerant scan: the value
+// after `"name"` and a colon, an array as the comma-separated tokens between the next '[' and
+// ']' (so the matrix file's trailing comma is fine), each token trimmed of whitespace and quotes;
+// a uint8 element is the first character of its token, as `iss >> value` extracts it
+// (helpFunctions.hpp:197-205).
+// That reader loses elements whose byte is whitespace (9-13, 32) or a quote (34), splits at a
+// comma byte (44) and ends the array at a ']' byte (93), so B12 cannot read back such trees
+// (depths 9-13, 32, 34 or 44; flags such as SD|HIT = 9); the reference then indexes its shorter
+// arrays past their end. Here the loader reads the same way and reports the short arrays as an
+// error instead.
 #include <cerrno>
 #include <cstdint>
 #include <cstdio>
@@ -127,6 +137,13 @@ std::vector<uint64_t> u64Array(const std::string& str, const std::string& name) 
   return v;
 }
 
+// a uint8 array: the first character of every non-empty token
+std::vector<uint8_t> byteArray(const std::string& str, const std::string& name) {
+  std::vector<uint8_t> v;
+  for (const auto& t : arrayTokens(str, name)) v.push_back(uint8_t(t[0]));
+  return v;
+}
+
 std::vector<double> doubleArray(const std::string& str, const std::string& name) {
   std::vector<double> v;
   for (const auto& t : arrayTokens(str, name)) v.push_back(parseDouble(t, name));
@@ -182,6 +199,7 @@ int b12x_tree_save_json(b12x_tree t, const char* fileName) {
     std::ofstream ofs(name, std::ofstream::out | std::ofstream::trunc);
     if (!ofs.good()) return b12x::setErr(B12X_E_IO, "file could not be opened: " + name);
     auto asInt = [](auto v) { return std::to_string(uint64_t(v)); };
+    auto asByte = [](uint8_t v) { return std::string(1, char(v)); };
     auto asDouble = [](double v) { return fmtDouble(v); };
     ofs << "{";
     putNumber(ofs, "B12_VERSION_MAJOR", kVersionMajor);
@@ -192,15 +210,15 @@ int b12x_tree_save_json(b12x_tree t, const char* fileName) {
     ofs << ",";
     putVector(ofs, "radius", std::vector<double>(r, r + dim), asDouble);
     ofs << ",";
-    putVector(ofs, "sdScheme", std::vector<uint8_t>(sd, sd + nsd), asInt);
+    putVector(ofs, "sdScheme", std::vector<uint8_t>(sd, sd + nsd), asByte);
     ofs << ",";
     putNumber(ofs, "nLeaves", uint64_t(n));
     ofs << ",";
     putVector(ofs, "bitPatternVector", bits, asInt);
     ofs << ",";
-    putVector(ofs, "depthVector", depth, asInt);
+    putVector(ofs, "depthVector", depth, asByte);
     ofs << ",";
-    putVector(ofs, "flagsVector", flags, asInt);
+    putVector(ofs, "flagsVector", flags, asByte);
     ofs << ",";
     uint64_t hostBytes = kHostBufferBytes;
     check(b12x_get_host_buffer_bytes(t, &hostBytes));
@@ -313,13 +331,17 @@ int b12x_tree_load(const char* fileName, int dim, b12x_tree* out) {
         return b12x::setErr(B12X_E_IO, "centre or radius shorter than the dimension");
       center.resize(dim);
       radius.resize(dim);
-      for (uint64_t v : u64Array(str, "sdScheme")) sd.push_back(uint8_t(v));
+      sd = byteArray(str, "sdScheme");
       const uint64_t n = parseU64(scalarToken(str, "nLeaves"), "nLeaves");
       L.bits = u64Array(str, "bitPatternVector");
-      for (uint64_t v : u64Array(str, "depthVector")) L.depth.push_back(uint8_t(v));
-      for (uint64_t v : u64Array(str, "flagsVector")) L.flags.push_back(uint8_t(v));
-      if (L.bits.size() != n || L.depth.size() != n || L.flags.size() != n)
+      L.depth = byteArray(str, "depthVector");
+      L.flags = byteArray(str, "flagsVector");
+      if (L.bits.size() != n)
         return b12x::setErr(B12X_E_IO, "nLeaves does not match the leaf arrays");
+      if (L.depth.size() != n || L.flags.size() != n)
+        return b12x::setErr(B12X_E_IO,
+                            "depthVector or flagsVector holds fewer entries than nLeaves: the reference's "
+                            "reader drops bytes 9-13, 32 and 34, splits at 44 and stops at 93");
       if (valueStart(str, "nHostBufferBytes") != std::string::npos)
         hostBytes = parseU64(scalarToken(str, "nHostBufferBytes"), "nHostBufferBytes");
       for (uint64_t i = 0; i < n; ++i)

@@ -1241,71 +1241,94 @@ __device__ inline double blockReduceSumF64(double v) {
   return t;
 }
 
-// One launch per iteration t = 1, 2, ...: y_t = mu_{t-1} P, where mu_{t-1} = y_{t-1} / ||y_{t-1}||_1
-// is recomputed on the fly from the previous iterate (the same IEEE quotient the oracle stores),
-// together with the residual ||mu_{t-1} - mu_{t-2}||_1 of the previous iteration; block partials
-// of both sums go to fixed slots for piReduceKernel. y rotates over 3 slots.
+// ---------------------------------------------------------------------------------------------
+// Invariant measure: power iteration on unnormalised iterates (build-defined; the reference leaves
+// this step to MATLAB, subdivisionByNim.m:30-59), row-vector convention:
+//   z_0 = mu_0, nu_0 = 1;  z_t[j] = c_t * sum_{i in column j, ascending} z_{t-1}[i] * P_ij;
+//   nu_t = S(|z_t|);  r_t = S(|z_t[j] / nu_t - z_{t-1}[j] / nu_{t-1}|);
+//   stop at the first t with (tol >= 0 and r_t <= tol) or t = maxIter;  mu = z_t / nu_t.
+// c_t (piScale) is 1 except at t = 16, 24, ... where it is 2^-ilogb(nu_{t-8}) when that exponent
+// leaves [-32, 32]: a power of two, so z_t and nu_t scale exactly and mu_t is that of the
+// normalised iteration with the normalisation deferred. No iterate depends on a norm of the
+// iteration before it, which is what lets the persistent form below run without a grid-wide
+// barrier per iteration.
+// S sums in one fixed order: a chunk of 256 columns is four groups of 64, each group summed by a
+// wave butterfly (x_l += x_{l^o}, o = 32 ... 1; columns past n are 0) and the four group sums
+// added to 0.0 in order; lane l of one wave adds chunks l, l + 64, ... to 0.0 in ascending order
+// and the lanes combine by the butterfly. The persistent and batched forms here, the forest's
+// pools and the oracle all use exactly these operations, so their iterates, norms, stopping
+// decisions and measures agree bit for bit.
+// ---------------------------------------------------------------------------------------------
 struct PiState {
-  double norm[3];
-  double resid;
-  uint32_t t;
-  uint32_t pad;
+  double norm[3];   // batched form: nu_t in slot t % 3
+  double resid;     // residual of the last completed iteration
+  double nuLag;     // batched form: nu_t of the last t with t % 8 == 0 (c_{t+8} derives from it)
+  double nuFinal;   // nu of the final iteration
+  uint32_t t;       // batched form: the next iteration
   int32_t done;
-  uint32_t final;
+  uint32_t final;   // the final iteration (0: mu = mu_0)
+  uint32_t slot;    // ring slot holding z_final
 };
 
-// kPiLanes lanes share a column: they load and multiply its entries in parallel, and the first
-// lane of the group adds the products in ascending-row order (the oracle's order) via shuffles.
-constexpr int kPiLanes = 8;
+__device__ inline double piScale(uint32_t t, double nuLag) {
+  if (t < 16 || (t & 7u) != 0) return 1.0;
+  if (!(nuLag > 0.0) || !isfinite(nuLag)) return 1.0;
+  const int e = ilogb(nuLag);
+  if (e >= -32 && e <= 32) return 1.0;
+  return ldexp(1.0, -max(-1000, min(1000, e)));
+}
+
+__device__ inline double waveButterflySum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Batched form, one launch per iteration t: z_t for kBlock columns per block (one lane per
+// column, kPiUnroll entries' loads in flight), with the chunk sums of |z_t| and of the residual
+// terms of iteration t - 1 (the block is one chunk: blockReduceSumF64 is the chunk order).
+// z rotates over 3 slots (z_t in slot t % 3).
+constexpr int kPiUnroll = 8;
 
 static __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __restrict__ colPtr,
                                                        const uint32_t* __restrict__ rowIdx,
                                                        const double* __restrict__ val, const double* __restrict__ mu0,
-                                                       double* y, uint32_t n, double* partAbs, double* partRes,
+                                                       double* z, uint32_t n, double* partAbs, double* partRes,
                                                        const PiState* st) {
   if (st->done) return;
   const uint32_t t = st->t;
-  const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;  // slots of t, t-1, t-2
-  const double* yPrev = y + uint64_t(s1) * n;
-  const double* yPrev2 = y + uint64_t(s2) * n;
-  double* yCur = y + uint64_t(s0) * n;
-  const double nPrev = st->norm[s1], nPrev2 = st->norm[s2];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int sub = lane % kPiLanes, lead = lane - sub;
-  const uint32_t j = (blockIdx.x * kBlock + threadIdx.x) / kPiLanes;
-  uint32_t e0 = 0, e1 = 0;
-  if (j < n) {
-    e0 = colPtr[j];
-    e1 = colPtr[j + 1];
-  }
-  // trip count uniform across the wave so that every lane takes part in every shuffle
-  uint32_t trips = (e1 - e0 + kPiLanes - 1) / kPiLanes;
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) trips = max(trips, (uint32_t)__shfl_xor(trips, o, kWave));
-  double s = 0.0;
-  for (uint32_t r = 0; r < trips; ++r) {
-    const uint32_t e = e0 + r * kPiLanes + sub;
-    double prod = 0.0;
-    if (e < e1) {
-      const uint32_t i = rowIdx[e];
-      const double m = (t == 1) ? mu0[i] : yPrev[i] / nPrev;
-      prod = m * val[e];
-    }
-#pragma unroll
-    for (int k = 0; k < kPiLanes; ++k) {
-      const double pk = __shfl(prod, lead + k, kWave);
-      if (e0 + r * kPiLanes + k < e1) s += pk;
-    }
-  }
+  const double* zPrev = t == 1 ? mu0 : z + uint64_t((t + 2) % 3) * n;   // z_{t-1}
+  const double* zPrev2 = t == 2 ? mu0 : z + uint64_t((t + 1) % 3) * n;  // z_{t-2} (t >= 2)
+  double* zCur = z + uint64_t(t % 3) * n;
+  const double c = piScale(t, st->nuLag);
+  const double nu1 = t >= 2 ? st->norm[(t + 2) % 3] : 1.0;
+  const double nu2 = t >= 3 ? st->norm[(t + 1) % 3] : 1.0;
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   double sAbs = 0.0, sRes = 0.0;
-  if (j < n && sub == 0) {
-    yCur[j] = s;
-    sAbs = fabs(s);
-    if (t >= 2) {
-      const double a = yPrev[j] / nPrev;
-      const double b = (t == 2) ? mu0[j] : yPrev2[j] / nPrev2;
-      sRes = fabs(a - b);
+  if (j < n) {
+    double s = 0.0;
+    const uint32_t e1 = colPtr[j + 1];
+    for (uint32_t e = colPtr[j]; e < e1; e += kPiUnroll) {
+      uint32_t i[kPiUnroll];
+      double v[kPiUnroll], m[kPiUnroll];
+#pragma unroll
+      for (int u = 0; u < kPiUnroll; ++u) {
+        const uint32_t eu = min(e + u, e1 - 1);
+        i[u] = rowIdx[eu];
+        v[u] = val[eu];
+      }
+#pragma unroll
+      for (int u = 0; u < kPiUnroll; ++u) m[u] = zPrev[i[u]];
+#pragma unroll
+      for (int u = 0; u < kPiUnroll; ++u) {
+        const double prod = m[u] * v[u];
+        s = (e + u < e1) ? s + prod : s;
+      }
     }
+    if (c != 1.0) s *= c;
+    zCur[j] = s;
+    sAbs = fabs(s);
+    if (t >= 2) sRes = fabs(zPrev[j] / nu1 - zPrev2[j] / nu2);
   }
   sAbs = blockReduceSumF64(sAbs);
   sRes = blockReduceSumF64(sRes);
@@ -1315,202 +1338,110 @@ static __global__ __launch_bounds__(kBlock) void piStepKernel(const uint32_t* __
   }
 }
 
-// Sums the block partials of piStepKernel in block order (deterministic) and decides whether
-// iteration t-1 ends the loop. One block; runs after every piStepKernel.
-static __global__ __launch_bounds__(kBlock) void piReduceKernel(const double* partAbs, const double* partRes, uint32_t nb,
-                                                         PiState* st, uint32_t maxIter, double tol) {
+// One wave after every piStepKernel: sums the chunk partials (lane l: chunks l, l + 64, ... in
+// order, then the butterfly), records nu_t, and decides whether iteration t - 1 ends the loop.
+static __global__ __launch_bounds__(kWave) void piReduceKernel(const double* partAbs, const double* partRes, uint32_t nb,
+                                                        PiState* st, uint32_t maxIter, double tol) {
   if (st->done) return;
-  const uint32_t t = st->t;
-  // issue all of a thread's loads before summing them (one memory latency instead of one per
-  // loop trip); the summation order per thread stays fixed
-  constexpr int U = 8;
+  const uint32_t t = st->t, lane = threadIdx.x;
+  constexpr int U = 8;  // loads in flight per lane before they are added (order unchanged)
   double ta = 0.0, tr = 0.0;
-  for (uint32_t b0 = 0; b0 < nb; b0 += kBlock * U) {
+  for (uint32_t q0 = 0; q0 < nb; q0 += kWave * U) {
     double va[U], vr[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t b = b0 + u * kBlock + threadIdx.x;
-      va[u] = b < nb ? partAbs[b] : 0.0;
-      vr[u] = b < nb ? partRes[b] : 0.0;
+      const uint32_t q = q0 + u * kWave + lane;
+      va[u] = q < nb ? partAbs[q] : 0.0;
+      vr[u] = q < nb ? partRes[q] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      ta += va[u];
-      tr += vr[u];
-    }
+    for (int u = 0; u < U; ++u)
+      if (q0 + u * kWave + lane < nb) {
+        ta += va[u];
+        tr += vr[u];
+      }
   }
-  ta = blockReduceSumF64(ta);
-  tr = blockReduceSumF64(tr);
-  if (threadIdx.x == 0) {
+  ta = waveButterflySum(ta);
+  tr = waveButterflySum(tr);
+  if (lane == 0) {
     st->norm[t % 3] = ta;
-    const uint32_t completed = t - 1;
-    if (completed >= 1) {
+    if ((t & 7u) == 0) st->nuLag = ta;
+    if (t >= 2) {
       st->resid = tr;
-      if ((tol >= 0.0 && tr <= tol) || completed >= maxIter) {
+      if ((tol >= 0.0 && tr <= tol) || t - 1 >= maxIter) {
         st->done = 1;
-        st->final = completed;
+        st->final = t - 1;
+        st->nuFinal = st->norm[(t + 2) % 3];
+        st->slot = (t - 1) % 3;
       }
     }
     st->t = t + 1;
   }
 }
 
-// Single-kernel form for grids of at most kPiFusedMaxBlocks blocks of kPiBlock threads: launch t
-// first re-derives norm_{t-1} and the residual of iteration t-2 from the previous launch's block
-// partials (every block sums them in the same order, so all blocks agree), decides whether
-// iteration t-2 ended the loop, then computes y_t and its partials into the other parity slot.
-// Each block counts its own launches (blockIter[b]), so no block waits on or races with another.
-constexpr int kPiBlock = 1024;
-constexpr uint32_t kPiFusedMaxBlocks = 512;
-
-__device__ inline double blockSumF64Big(double v) {
-  __shared__ double ws[kPiBlock / kWave];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  if (lane == 0) ws[wave] = v;
-  __syncthreads();
-  double t = 0.0;
-#pragma unroll
-  for (int w = 0; w < kPiBlock / kWave; ++w) t += ws[w];
-  __syncthreads();
-  return t;
-}
-
-static __global__ __launch_bounds__(kPiBlock) void piFusedKernel(const uint32_t* __restrict__ colPtr,
-                                                          const uint32_t* __restrict__ rowIdx,
-                                                          const double* __restrict__ val,
-                                                          const double* __restrict__ mu0, double* y, uint32_t n,
-                                                          double* part /* [2][2][nb] */, uint32_t* blockIter,
-                                                          PiState* st, uint32_t maxIter, double tol) {
-  const uint32_t nb = gridDim.x;
-  const uint32_t t = blockIter[blockIdx.x] + 1;  // this block's launch index
-  if (st->done) return;
-  double nPrev = 0.0, rPrev = -1.0;
-  if (t >= 2) {
-    const double* pa = part + uint64_t((t - 1) & 1) * 2 * nb;
-    double a = 0.0, r = 0.0;
-    for (uint32_t b = threadIdx.x; b < nb; b += kPiBlock) {
-      a += pa[b];
-      r += pa[nb + b];
-    }
-    nPrev = blockSumF64Big(a);
-    rPrev = blockSumF64Big(r);
-  }
-  const uint32_t completed = t >= 2 ? t - 2 : 0;  // iteration whose residual is rPrev
-  if (completed >= 1 && ((tol >= 0.0 && rPrev <= tol) || completed >= maxIter)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->done = 1;
-      st->final = completed;
-      st->resid = rPrev;
-    }
-    return;
-  }
-  if (threadIdx.x == 0) blockIter[blockIdx.x] = t;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && t >= 2) {
-    st->norm[(t - 1) % 3] = nPrev;
-    if (completed >= 1) st->resid = rPrev;
-  }
-  const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
-  const double* yPrev = y + uint64_t(s1) * n;
-  const double* yPrev2 = y + uint64_t(s2) * n;
-  double* yCur = y + uint64_t(s0) * n;
-  const double nPrev2 = t >= 3 ? st->norm[s2] : 0.0;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int sub = lane % kPiLanes, lead = lane - sub;
-  double sAbs = 0.0, sRes = 0.0;
-  const uint64_t total = uint64_t(n) * kPiLanes;
-  const uint64_t stride = uint64_t(nb) * kPiBlock;
-  for (uint64_t g = uint64_t(blockIdx.x) * kPiBlock + threadIdx.x; g - lane < total; g += stride) {
-    const uint32_t j = uint32_t(g / kPiLanes);
-    uint32_t e0 = 0, e1 = 0;
-    if (g < total) {
-      e0 = colPtr[j];
-      e1 = colPtr[j + 1];
-    }
-    uint32_t trips = (e1 - e0 + kPiLanes - 1) / kPiLanes;
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) trips = max(trips, (uint32_t)__shfl_xor(trips, o, kWave));
-    double s = 0.0;
-    for (uint32_t r = 0; r < trips; ++r) {
-      const uint32_t e = e0 + r * kPiLanes + sub;
-      double prod = 0.0;
-      if (e < e1) {
-        const uint32_t i = rowIdx[e];
-        const double m = (t == 1) ? mu0[i] : yPrev[i] / nPrev;
-        prod = m * val[e];
-      }
-#pragma unroll
-      for (int k = 0; k < kPiLanes; ++k) {
-        const double pk = __shfl(prod, lead + k, kWave);
-        if (e0 + r * kPiLanes + k < e1) s += pk;
-      }
-    }
-    if (g < total && sub == 0) {
-      yCur[j] = s;
-      sAbs += fabs(s);
-      if (t >= 2) {
-        const double a = yPrev[j] / nPrev;
-        const double b = (t == 2) ? mu0[j] : yPrev2[j] / nPrev2;
-        sRes += fabs(a - b);
-      }
-    }
-  }
-  sAbs = blockSumF64Big(sAbs);
-  sRes = blockSumF64Big(sRes);
-  if (threadIdx.x == 0) {
-    double* pc = part + uint64_t(t & 1) * 2 * nb;
-    pc[blockIdx.x] = sAbs;
-    pc[nb + blockIdx.x] = sRes;
-  }
-}
-
-// mu = y_final / ||y_final||_1 (or mu0 when no iteration ran)
-// Persistent form of the loop: one launch whose grid fits the device at once (every block is
-// resident for the whole run on an otherwise idle device).
-// Per iteration each block computes y_t for its columns (one lane per column, products added in
-// ascending-row order as the oracle does) and publishes its partial sums of |y_t| and of the
-// previous residual in its slot of buffer t % 3. A slot holds kCoopEmpty until written. Block 0
-// polls every slot of the buffer until none is empty, sums them in block order and publishes the
-// totals in result slot t % 3 (kPiResCopies copies); the other blocks poll their copy. So all
-// blocks derive the same norm and the same stopping decision. A block clears its partial slot of
-// buffer (t+1) % 3 at the start of iteration t, and block 0 the result slot (t+1) % 3: both were
-// last read in iteration t-2, which every block has finished once the wait of iteration t-1 is
-// over.
-// y rotates over 3 slots (y_t in slot t % 3) and a lane clears its columns of slot (t+1) % 3
-// (y_{t-2}, read for the last time in iteration t-1) to kCoopEmpty in iteration t, before its
-// block publishes. So once a block has seen the totals of iteration t-1, slot t % 3 holds
-// kCoopEmpty or y_t, never an older iterate. That lets each lane gather the y_t entries of its
-// first column right after its block publishes y_t, while the totals are still on their way:
-// the gathers overlap the wait, and a lane re-reads the entries that came back empty (their
-// owners had not stored them yet) until none is left. Everything else is gathered after the
-// wait, when every y_t entry is in place.
-// Every wait is bounded: past spinLimit polls a block raises *err and leaves, and the others
-// leave at their next wait (the host then recomputes with the batched form).
-// Coherence across XCDs: everything written during the run (y, partials, totals) is written and
-// read with agent-scope relaxed atomics (coherent without L2 write-back or invalidation), and
-// each wave waits for its stores to complete before its block publishes. The matrix is read-only
-// and stays cached in L2 across iterations.
-constexpr uint32_t kCoopMaxBlocks = 512;
+// ---------------------------------------------------------------------------------------------
+// Persistent form (piFlowKernel): one launch for the whole solve, its grid resident at once.
+// Worker waves own groups of 64 columns (wave w of block b: groups 4b + w, 4b + w + 4 G, ...
+// for G worker blocks), one lane per column; the last block is the aggregator.
+// Iteration t of a worker wave gathers z_{t-1} from the slots of the other waves, stores z_t and
+// publishes its group's sums of |z_t| and of the residual terms of iteration t - L (L =
+// kFlowLag). It waits only for the entries it gathers (dataflow through the vector: a ring of
+// kFlowRing slots, each entry the empty pattern until its owner stores it) and for the totals
+// of iteration t - L, which the aggregator has published long before in the steady state. So
+// no grid-wide barrier lies on the path of an iteration. In the straight-line form the totals
+// load is issued together with the gathers, and an iteration's group sums are formed and
+// published in the next iteration, while its gathers are in flight.
+// The aggregator sums the group partials of iteration tau in the chunk order and publishes
+// (nu_tau, r_{tau - L}) in totals slot tau % kFlowTot (kFlowCopies copies, one 128-byte line
+// each; wave w of block b polls copy (4 b + w) % kFlowCopies). The stopping decision for
+// iteration f = tau - L is taken once r_f is in the totals: the aggregator records it and
+// leaves, and every worker wave reads the same totals at its iteration f + 2 L and leaves
+// there, before z_f is cleared (iteration f + kFlowClear), so piFinishKernel forms z_f / nu_f.
+// Slot reuse (D = kFlowClear; the static_asserts below hold the bounds):
+//  - z: in iteration t a lane stores z_t in slot t % kFlowRing and clears z_{t-D}: its readers
+//    gathered it in iteration t - D + 1, and having seen the totals of t - L the lane knows every
+//    wave has completed the gathers of iteration t - L >= t - D + 1. A reader gathering z_x from
+//    the slot has seen z_{x-1} of the same lane, which was stored after the lane's store counter
+//    drained past the clear (issued in iteration x - kFlowRing + D <= x - 2), so it finds empty
+//    or z_x.
+//  - partials: the aggregator clears position (tau + L + 1) % kFlowPart (partials of
+//    tau + L + 1 - kFlowPart <= tau - 1, consumed) after publishing tau; a wave writes the
+//    partials of tau + L + 1 only after seeing the totals of tau + 1, published after the clears
+//    of tau drained.
+//  - totals: the aggregator clears slot (tau + 2) % kFlowTot (totals of tau + 2 - kFlowTot,
+//    read by every wave in iteration tau + 2 - kFlowTot + L <= tau, before it published the
+//    partials of tau) after publishing tau; a wave reading the totals of tau + 2 has seen those of
+//    tau + 1, published after the clear drained.
+// Everything written during the run goes through agent-scope relaxed atomics, coherent across
+// XCDs without L2 write-back or invalidation; the matrix is read-only and stays cached. Every
+// wait is bounded: past spinLimit polls a wave raises *err and leaves, the others leave at their
+// next poll, and the host recomputes with the batched form.
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t kFlowMaxWorkers = 512;
+constexpr uint32_t kFlowRing = 16, kFlowLag = 6, kFlowClear = 2 * kFlowLag, kFlowPart = 8, kFlowTot = 8;
+static_assert(kFlowClear >= 2 * kFlowLag, "z_f must outlive the workers");
+static_assert(kFlowClear + 2 <= kFlowRing, "a clear drains before the slot's next value");
+static_assert(kFlowLag + 1 <= kFlowClear, "readers of a cleared value have finished");
+static_assert(kFlowPart >= kFlowLag + 1 && kFlowTot >= kFlowLag + 2, "partial and totals reuse");
+static_assert(kFlowLag <= 8, "c_t derives from nu_{t-8}, which the totals history holds");
+constexpr uint32_t kFlowCopies = 16, kFlowStride = 16;  // u64 words per totals copy: one 128-byte line
+constexpr uint32_t kFlowTotWords = kFlowTot * kFlowCopies * kFlowStride;
 constexpr int kCoopUnroll = 8;
-constexpr int kCoopCache = 3 * kCoopUnroll;  // entries of a lane's first column held in registers
+constexpr int kCoopCache = 3 * kCoopUnroll;  // entries of a lane's first column cached for the whole run
 constexpr uint64_t kCoopEmpty = 0xFFF4B12B12B12B12ull;  // a NaN pattern arithmetic never produces
-// block 0 writes its totals to kPiResCopies copies, one 128-byte line each, and block b polls
-// copy b % kPiResCopies, so that no single line takes every block's polls
-constexpr uint32_t kPiResCopies = 16, kPiResStride = 16;
-constexpr uint32_t kPiResWords = 3 * kPiResCopies * kPiResStride;
+constexpr uint32_t kFlowAggChunks = 4;  // chunks per aggregator lane polled together
 
-struct PiCoopArgs {
+struct PiFlowArgs {
   const uint32_t* colPtr;
   const uint32_t* rowIdx;
   const double* val;
   const double* mu0;
-  double* y;          // 3 * n, all kCoopEmpty at launch
-  uint64_t* part;     // 3 buffers * gridDim.x slots * 2, all kCoopEmpty at launch
-  uint64_t* res;      // 3 slots * kPiResCopies copies * kPiResStride (block 0's totals), all kCoopEmpty
+  uint64_t* z;     // kFlowRing * n, all kCoopEmpty at launch
+  uint64_t* part;  // kFlowPart * nGroups * 2 (|z| sum, residual sum), all kCoopEmpty at launch
+  uint64_t* tot;   // kFlowTotWords, all kCoopEmpty at launch
   PiState* st;
-  int* err;           // zero at launch
-  uint32_t n;
+  int* err;        // zero at launch
+  uint32_t n, nGroups, workers;
   uint32_t maxIter;
   double tol;
   uint32_t spinLimit;
@@ -1524,310 +1455,382 @@ template <class T>
 __device__ inline void stAgent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// y entries as bit patterns (the empty pattern is compared bitwise)
-__device__ inline uint64_t ldBits(const double* p) { return ldAgent(reinterpret_cast<const uint64_t*>(p)); }
-__device__ inline void stBits(double* p, uint64_t v) { stAgent(reinterpret_cast<uint64_t*>(p), v); }
+__device__ inline double asF64(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ inline uint64_t asU64(double v) { return uint64_t(__double_as_longlong(v)); }
 
-// y entry at a byte offset below 2^32 from a uniform base (one 32-bit offset register per gather)
-__device__ inline uint64_t ldAt(const double* base, uint32_t off) {
+// entry at a byte offset below 2^32 from a uniform base (one 32-bit offset register per gather)
+__device__ inline uint64_t ldAt(const uint64_t* base, uint32_t off) {
   return ldAgent(reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(base) + off));
 }
 
-// the cached entries of a lane's first column (byte offsets ci), gathered from src: groups of
-// kCoopUnroll gathers, each issued where some column of the wave reaches it (clamped indices
-// keep the others in bounds), all in flight together
-__device__ inline void piGatherCached(const double* src, const uint32_t (&ci)[kCoopCache], uint32_t c0, uint32_t c1,
-                                      uint64_t (&m)[kCoopCache]) {
+// Issues the gathers m[u] = src[off u] of a lane's entries u < cnt (byte offsets), all in flight
+// together; a lane loads only its own entries (a group of kCoopUnroll entries that no lane of the
+// wave has is skipped by one wave-uniform test). Loads for lanes past their entries measured
+// slower on MI355X whether they repeated a lane's last entry or read a shared or own line: every
+// extra address costs the memory pipeline and the lines they hit are being written.
+template <int U>
+__device__ inline void flowIssue(const uint64_t* src, const uint32_t (&off)[U], uint32_t cnt, uint64_t (&m)[U]) {
 #pragma unroll
-  for (int g = 0; g < kCoopCache; g += kCoopUnroll) {
-    if (__any(c0 + g < c1)) {  // wave-uniform: no wait at the join
+  for (int g = 0; g < U; g += kCoopUnroll) {
+    if (__any(uint32_t(g) < cnt)) {
 #pragma unroll
-      for (int u = g; u < g + kCoopUnroll; ++u) m[u] = ldAt(src, ci[u]);
+      for (int u = g; u < g + kCoopUnroll; ++u) m[u] = uint32_t(u) < cnt ? ldAt(src, off[u]) : 0;
+    } else {
+#pragma unroll
+      for (int u = g; u < g + kCoopUnroll; ++u) m[u] = 0;
     }
   }
 }
 
-// true where a cached entry the column uses came back empty
-__device__ inline bool piMissing(const uint64_t (&m)[kCoopCache], uint32_t c0, uint32_t c1) {
-  bool miss = false;
+// Completes flowIssue: re-reads the entries that came back empty until none is left (check: the
+// source is a ring slot still being filled). false when a wait passed its limit or another wave
+// raised the error flag.
+template <int U>
+__device__ inline bool flowComplete(const uint64_t* src, const uint32_t (&off)[U], uint32_t cnt, bool check,
+                                    uint64_t (&m)[U], const PiFlowArgs& a) {
+  if (!check) return true;
+  for (uint32_t spin = 0;; ++spin) {
+    bool miss = false;
 #pragma unroll
-  for (int u = 0; u < kCoopCache; ++u) miss = miss || (c0 + u < c1 && m[u] == kCoopEmpty);
-  return miss;
+    for (int u = 0; u < U; ++u) miss = miss || (uint32_t(u) < cnt && m[u] == kCoopEmpty);
+    if (!__any(miss)) return true;
+    if (spin >= a.spinLimit || ldAgent(a.err)) return false;
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int g = 0; g < U; g += kCoopUnroll) {
+      bool gm = false;
+#pragma unroll
+      for (int u = g; u < g + kCoopUnroll; ++u) gm = gm || (uint32_t(u) < cnt && m[u] == kCoopEmpty);
+      if (__any(gm)) {
+#pragma unroll
+        for (int u = g; u < g + kCoopUnroll; ++u)
+          if (uint32_t(u) < cnt && m[u] == kCoopEmpty) m[u] = ldAt(src, off[u]);
+      }
+    }
+  }
 }
 
-// re-read the empty entries (all of a lane's re-reads in flight together)
-__device__ inline void piRegather(const double* src, const uint32_t (&ci)[kCoopCache], uint32_t c0, uint32_t c1,
-                                  uint64_t (&m)[kCoopCache]) {
+__device__ inline void flowAbort(const PiFlowArgs& a) { stAgent(a.err, 1); }
+
+__device__ inline void piFlowAggregator(const PiFlowArgs& a) {
+  __shared__ double nuRing[kFlowTot];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nG = a.nGroups, nChunks = (nG + 3) / 4;
+  for (uint32_t tau = 1;; ++tau) {
+    const uint64_t* buf = a.part + uint64_t(tau % kFlowPart) * nG * 2;
+    double ta = 0.0, tr = 0.0;
+    // lane l sums chunks l, l + 64, ... in order; a chunk's sum adds its four group partials to
+    // 0.0 in order. kFlowAggChunks chunks per lane are polled together.
+    for (uint32_t k0 = 0; k0 * kWave < nChunks; k0 += kFlowAggChunks) {
+      uint64_t v[kFlowAggChunks][8];
 #pragma unroll
-  for (int u = 0; u < kCoopCache; ++u)
-    if (c0 + u < c1 && m[u] == kCoopEmpty) m[u] = ldAt(src, ci[u]);
+      for (uint32_t k = 0; k < kFlowAggChunks; ++k)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) v[k][x] = 0;
+      for (uint32_t spin = 0;; ++spin) {
+        bool miss = false;
+#pragma unroll
+        for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
+          const uint32_t q = lane + (k0 + k) * kWave;
+#pragma unroll
+          for (int x = 0; x < 8; ++x) {
+            const uint32_t g = 4 * q + uint32_t(x / 2);
+            if (q < nChunks && g < nG) {
+              if (spin == 0 || v[k][x] == kCoopEmpty) v[k][x] = ldAgent(buf + uint64_t(g) * 2 + (x & 1));
+            }
+          }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
+          const uint32_t q = lane + (k0 + k) * kWave;
+#pragma unroll
+          for (int x = 0; x < 8; ++x)
+            miss = miss || (q < nChunks && 4 * q + uint32_t(x / 2) < nG && v[k][x] == kCoopEmpty);
+        }
+        if (!__any(miss)) break;
+        if (spin >= a.spinLimit || ldAgent(a.err)) {
+          if (lane == 0) flowAbort(a);
+          return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
+        const uint32_t q = lane + (k0 + k) * kWave;
+        if (q < nChunks) {
+          double ca = 0.0, cr = 0.0;
+#pragma unroll
+          for (int w = 0; w < 4; ++w)
+            if (4 * q + uint32_t(w) < nG) {
+              ca += asF64(v[k][2 * w]);
+              cr += asF64(v[k][2 * w + 1]);
+            }
+          ta += ca;
+          tr += cr;
+        }
+      }
+    }
+    ta = waveButterflySum(ta);
+    tr = waveButterflySum(tr);
+    if (lane == 0) nuRing[tau % kFlowTot] = ta;
+    __builtin_amdgcn_s_waitcnt(0);  // the clears of the previous iterations have completed
+    if (lane < kFlowCopies) {
+      uint64_t* tc = a.tot + (uint64_t(tau % kFlowTot) * kFlowCopies + lane) * kFlowStride;
+      stAgent(tc, asU64(ta));
+      stAgent(tc + 1, asU64(tr));
+    }
+    if (tau > kFlowLag) {
+      const uint32_t f = tau - kFlowLag;
+      if ((a.tol >= 0.0 && tr <= a.tol) || f >= a.maxIter) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+          a.st->final = f;
+          a.st->resid = tr;
+          a.st->nuFinal = nuRing[f % kFlowTot];
+          a.st->slot = f % kFlowRing;
+          a.st->done = 1;
+        }
+        return;
+      }
+    }
+    // clear the partials of tau - 3 and the totals of tau - 6 for their next use
+    uint64_t* pc = a.part + uint64_t((tau + kFlowLag + 1) % kFlowPart) * nG * 2;
+    for (uint32_t g = lane; g < nG; g += kWave) {
+      stAgent(pc + uint64_t(g) * 2, kCoopEmpty);
+      stAgent(pc + uint64_t(g) * 2 + 1, kCoopEmpty);
+    }
+    if (lane < kFlowCopies) {
+      uint64_t* tc = a.tot + (uint64_t((tau + 2) % kFlowTot) * kFlowCopies + lane) * kFlowStride;
+      stAgent(tc, kCoopEmpty);
+      stAgent(tc + 1, kCoopEmpty);
+    }
+  }
 }
 
-// SLOTS: partial slots per polling lane of block 0 (grids up to SLOTS * 64 blocks)
-template <uint32_t SLOTS>
-static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void piCoopKernel(PiCoopArgs a) {
-  constexpr uint32_t kSlots = SLOTS;
-  constexpr int kWaves = kBlock / kWave;
-  const uint32_t G = gridDim.x, b = blockIdx.x, tid = threadIdx.x, n = a.n;
-  const uint32_t lane = tid & (kWave - 1), wv = tid / kWave;
-  __shared__ double wsum[2][kWaves];
-  __shared__ double sums[2];
-  __shared__ int leave, failed;
-  if (a.st->done) return;
-  uint32_t t = a.st->t;
-  double nPrev = t >= 2 ? a.st->norm[(t + 2) % 3] : 1.0;  // norm of y_{t-1}
-  double nPrev2 = t >= 3 ? a.st->norm[(t + 1) % 3] : 1.0;  // norm of y_{t-2}
-  // the lane's first column keeps its first kCoopCache entries (row byte offsets and values) in
-  // registers for the whole run (indices clamped into the column; an empty column holds row 0
-  // and adds nothing)
-  const uint32_t p0 = b * kBlock + tid;
+// Reads the totals of iteration t - kFlowLag from copy `copy`: nu_{t-L} shifted into the history
+// nuH (nuH[k] = nu_{t-L-k}), r_{t-2L} into rho. ra/rr: the words loaded already. false on abort.
+__device__ inline bool flowTotals(const PiFlowArgs& a, const uint64_t* tc, uint64_t ra, uint64_t rr,
+                                  double (&nuH)[kFlowLag + 1], double& rho) {
+  for (uint32_t spin = 0; ra == kCoopEmpty || rr == kCoopEmpty; ++spin) {
+    if (spin >= a.spinLimit || ldAgent(a.err)) return false;
+    __builtin_amdgcn_s_sleep(1);
+    ra = ldAgent(tc);
+    rr = ldAgent(tc + 1);
+  }
+#pragma unroll
+  for (uint32_t k = kFlowLag; k > 0; --k) nuH[k] = nuH[k - 1];
+  nuH[0] = asF64(ra);
+  rho = asF64(rr);
+  return true;
+}
+
+__device__ inline bool flowStops(const PiFlowArgs& a, uint32_t t, double rho) {
+  return t > 2 * kFlowLag && ((a.tol >= 0.0 && rho <= a.tol) || t - 2 * kFlowLag >= a.maxIter);
+}
+
+// FULL = false: every wave owns one group and no column has more than kCoopCache entries (the
+// host checks both), so an iteration is straight-line code; FULL = true: waves sweep several
+// groups and long columns read their further entries from the matrix every iteration.
+template <bool FULL>
+static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void piFlowKernel(PiFlowArgs a) {
+  const uint32_t b = blockIdx.x, lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (b == a.workers) {  // the aggregator: one wave
+    if (w == 0) piFlowAggregator(a);
+    return;
+  }
+  const uint32_t n = a.n, nG = a.nGroups, wave = 4 * b + w, waves = 4 * a.workers;
+  if (wave >= nG) return;
+  const uint32_t copy = wave % kFlowCopies;
+  // the first group's column keeps its first kCoopCache entries for the whole run: row byte
+  // offsets in registers, values in LDS (each lane reads only its own slots, so no barrier);
+  // its own z history (z_{t-1}, ..., z_{t-1-kFlowLag}) stays in registers
+  __shared__ double cvL[kCoopCache][kBlock];
+  const uint32_t j0 = 64 * wave + lane;
+  const bool valid0 = j0 < n;
   uint32_t c0 = 0, c1 = 0;
+  if (valid0) {
+    c0 = a.colPtr[j0];
+    c1 = a.colPtr[j0 + 1];
+  }
   uint32_t ci[kCoopCache];
-  double cv[kCoopCache];
 #pragma unroll
   for (int u = 0; u < kCoopCache; ++u) {
     ci[u] = 0;
-    cv[u] = 0.0;
+    cvL[u][threadIdx.x] = 0.0;
   }
-  // the first column's own y_{t-1} and y_{t-2} (mu_0 stands for y_0, with norm 1)
-  double own1 = 0.0, own2 = 0.0;
-  if (p0 < n) {
-    c0 = a.colPtr[p0];
-    c1 = a.colPtr[p0 + 1];
-    if (c1 > c0) {
+  if (c1 > c0) {  // indices clamped into the column
+#pragma unroll
+    for (int u = 0; u < kCoopCache; ++u) {
+      const uint32_t eu = min(c0 + u, c1 - 1);
+      ci[u] = a.rowIdx[eu] * uint32_t(sizeof(double));  // n < 2^29
+      cvL[u][threadIdx.x] = a.val[eu];
+    }
+  }
+  const uint32_t cnt0 = min(c1 - c0, uint32_t(kCoopCache));
+  double own[kFlowLag + 1];
+  const double z00 = valid0 ? a.mu0[j0] : 0.0;
+#pragma unroll
+  for (uint32_t k = 0; k <= kFlowLag; ++k) own[k] = z00;
+  // nu_{t-kFlowLag-k} for k = 0 .. kFlowLag (nu_0 = 1 stands for the start)
+  double nuH[kFlowLag + 1];
+#pragma unroll
+  for (uint32_t k = 0; k <= kFlowLag; ++k) nuH[k] = 1.0;
+  const uint64_t* mu0b = reinterpret_cast<const uint64_t*>(a.mu0);
+  if constexpr (!FULL) {
+    // pending group sums of the previous iteration: |z_{t-1}| and its residual term's inputs
+    double pAbs = 0.0, pZa = 0.0, pZb = 0.0, pNuA = 1.0, pNuB = 1.0;
+    bool pRes = false;
+    for (uint32_t t = 1;; ++t) {
+      const uint64_t* src = t == 1 ? mu0b : a.z + uint64_t((t - 1) % kFlowRing) * n;
+      // totals of t - L and the gathers of z_{t-1}: issued together
+      const uint64_t* tc = a.tot + (uint64_t((t - kFlowLag) % kFlowTot) * kFlowCopies + copy) * kFlowStride;
+      uint64_t ra = 0, rr = 0;
+      if (t > kFlowLag) {
+        ra = ldAgent(tc);
+        rr = ldAgent(tc + 1);
+      }
+      uint64_t m0[kCoopCache];
+      flowIssue<kCoopCache>(src, ci, cnt0, m0);
+      // the previous iteration's group sums, while the loads are in flight
+      if (t > 1) {
+        const double term = (pRes && valid0) ? fabs(pZa / pNuA - pZb / pNuB) : 0.0;
+        const double ga = waveButterflySum(pAbs);
+        const double gr = waveButterflySum(term);
+        if (lane == 0) {
+          uint64_t* pc = a.part + (uint64_t((t - 1) % kFlowPart) * nG + wave) * 2;
+          stAgent(pc, asU64(ga));
+          stAgent(pc + 1, asU64(gr));
+        }
+      }
+      if (t > kFlowLag) {
+        double rho = 0.0;
+        if (!flowTotals(a, tc, ra, rr, nuH, rho)) {
+          if (lane == 0) flowAbort(a);
+          return;
+        }
+        if (flowStops(a, t, rho)) return;
+      }
+      const double c = piScale(t, nuH[8 - kFlowLag]);  // nu_{t-8}
+      if (!flowComplete<kCoopCache>(src, ci, cnt0, t > 1, m0, a)) {
+        flowAbort(a);
+        return;
+      }
+      double s = 0.0;
 #pragma unroll
       for (int u = 0; u < kCoopCache; ++u) {
-        const uint32_t eu = min(c0 + u, c1 - 1);
-        ci[u] = a.rowIdx[eu] * uint32_t(sizeof(double));  // n < 2^29
-        cv[u] = a.val[eu];
+        const double prod = asF64(m0[u]) * cvL[u][threadIdx.x];
+        s = (uint32_t(u) < cnt0) ? s + prod : s;
       }
-    }
-    own1 = t >= 2 ? __longlong_as_double((long long)ldBits(a.y + uint64_t((t + 2) % 3) * n + p0)) : a.mu0[p0];
-    own2 = t >= 3 ? __longlong_as_double((long long)ldBits(a.y + uint64_t((t + 1) % 3) * n + p0))
-                  : (t == 2 ? a.mu0[p0] : 0.0);
-  }
-  // the first gathers: y_{t-1} (mu_0 in iteration 1) is complete at launch
-  uint64_t m[kCoopCache];
+      if (c != 1.0) s *= c;
+      __builtin_amdgcn_s_waitcnt(0);  // earlier stores (clears included) have completed
+      if (valid0) {
+        stAgent(a.z + uint64_t(t % kFlowRing) * n + j0, asU64(s));
+        stAgent(a.z + uint64_t((t + kFlowRing - kFlowClear) % kFlowRing) * n + j0, kCoopEmpty);
+      }
+      // inputs of this iteration's group sums (formed in the next iteration): the residual
+      // term of iteration t - L, |z_{t-L}/nu_{t-L} - z_{t-L-1}/nu_{t-L-1}|
+      pAbs = valid0 ? fabs(s) : 0.0;
+      pRes = t > kFlowLag;
+      pZa = own[kFlowLag - 1];
+      pZb = own[kFlowLag];
+      pNuA = nuH[0];
+      pNuB = nuH[1];
 #pragma unroll
-  for (int u = 0; u < kCoopCache; ++u) m[u] = 0;
-  piGatherCached((t == 1) ? a.mu0 : a.y + uint64_t((t + 2) % 3) * n, ci, c0, c1, m);
-  for (;;) {
-    if (tid == 0) {
-      uint64_t* nextSlot = a.part + (uint64_t((t + 1) % 3) * G + b) * 2;
-      stAgent(nextSlot, kCoopEmpty);
-      stAgent(nextSlot + 1, kCoopEmpty);
-      failed = 0;
+      for (uint32_t k = kFlowLag; k > 0; --k) own[k] = own[k - 1];
+      own[0] = s;
     }
-    if (b == 0 && tid < kPiResCopies) {
-      uint64_t* rn = a.res + (((t + 1) % 3) * kPiResCopies + tid) * kPiResStride;
-      stAgent(rn, kCoopEmpty);
-      stAgent(rn + 1, kCoopEmpty);
-    }
-    const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
-    const double* yPrev = a.y + uint64_t(s1) * n;
-    double* yOld = a.y + uint64_t(s2) * n;  // y_{t-2}: cleared below
-    double* yCur = a.y + uint64_t(s0) * n;
-    // iteration 1 multiplies by mu_0 itself: mu_0[i] / 1.0 is mu_0[i] exactly
-    const double* src = (t == 1) ? a.mu0 : yPrev;
-    const double div = (t == 1) ? 1.0 : nPrev;
-    const double div2 = (t == 2) ? 1.0 : nPrev2;
-    double sAbs = 0.0, sRes = 0.0;
-    for (uint32_t p = p0; p < n; p += G * kBlock) {
-      double s = 0.0, yp, yp2;
-      uint32_t e, e1;
-      if (p == p0) {
-        // mu_{t-1}[i] = y_{t-1}[i] / ||y_{t-1}||_1 as the oracle rounds it, then the product
-#pragma unroll
-        for (int u = 0; u < kCoopCache; ++u) {
-          const double prod = (__longlong_as_double((long long)m[u]) / div) * cv[u];
-          s = (c0 + u < c1) ? s + prod : s;
+  } else {
+    for (uint32_t t = 1;; ++t) {
+      const uint64_t* src = t == 1 ? mu0b : a.z + uint64_t((t - 1) % kFlowRing) * n;
+      uint64_t* dst = a.z + uint64_t(t % kFlowRing) * n;
+      uint64_t* clr = a.z + uint64_t((t + kFlowRing - kFlowClear) % kFlowRing) * n;
+      const bool check = t > 1;
+      if (t > kFlowLag) {
+        const uint64_t* tc = a.tot + (uint64_t((t - kFlowLag) % kFlowTot) * kFlowCopies + copy) * kFlowStride;
+        double rho = 0.0;
+        if (!flowTotals(a, tc, ldAgent(tc), ldAgent(tc + 1), nuH, rho)) {
+          if (lane == 0) flowAbort(a);
+          return;
         }
-        e = c0 + kCoopCache;
-        e1 = c1;
-        yp = own1;
-        yp2 = own2;
-      } else {
-        yp = __longlong_as_double((long long)ldBits(yPrev + p));
-        yp2 = __longlong_as_double((long long)ldBits(((t == 2) ? a.mu0 : yOld) + p));
-        e = a.colPtr[p];
-        e1 = a.colPtr[p + 1];
+        if (flowStops(a, t, rho)) return;
       }
-      // branch-free body (entry indices clamped to the column, sums selected) so that all of a
-      // lane's gathers are in flight at once; y_{t-1} is complete here
-      for (; e < e1; e += kCoopUnroll) {
-        uint32_t i[kCoopUnroll];
-        double v[kCoopUnroll], mm[kCoopUnroll];
-#pragma unroll
-        for (int u = 0; u < kCoopUnroll; ++u) {
-          const uint32_t eu = min(e + u, e1 - 1);
-          i[u] = a.rowIdx[eu];
-          v[u] = a.val[eu];
-        }
-#pragma unroll
-        for (int u = 0; u < kCoopUnroll; ++u) mm[u] = ldAgent(src + i[u]);
-#pragma unroll
-        for (int u = 0; u < kCoopUnroll; ++u) {
-          const double prod = (mm[u] / div) * v[u];
-          s = (e + u < e1) ? s + prod : s;
-        }
-      }
-      stBits(yCur + p, uint64_t(__double_as_longlong(s)));
-      sAbs += fabs(s);
-      if (t >= 2) sRes += fabs(yp / nPrev - yp2 / div2);
-      if (p == p0) {
-        own2 = own1;
-        own1 = s;
-      }
-      stBits(yOld + p, kCoopEmpty);  // after yp2 was read; the next value stored here is y_{t+1}
-    }
-    // block sums: each wave by a fixed shuffle tree, then the waves in order
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-      sAbs += __shfl_xor(sAbs, o, kWave);
-      sRes += __shfl_xor(sRes, o, kWave);
-    }
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's y stores and clears have completed
-    if (lane == 0) {
-      wsum[0][wv] = sAbs;
-      wsum[1][wv] = sRes;
-    }
-    __syncthreads();
-    uint64_t* buf = a.part + uint64_t(t % 3) * G * 2;
-    if (tid == 0) {
-      double A = 0.0, R = 0.0;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        A += wsum[0][w];
-        R += wsum[1][w];
-      }
-      stAgent(buf + 2 * b, uint64_t(__double_as_longlong(A)));
-      stAgent(buf + 2 * b + 1, uint64_t(__double_as_longlong(R)));
-    }
-    // the next iteration's first-column gathers of y_t, overlapping the wait
-    piGatherCached(yCur, ci, c0, c1, m);
-    if (tid < kWave) {
-      // Block 0 polls every partial slot, sums them in block order and publishes the totals in
-      // result slot t % 3; the other blocks poll that one slot. A poll round issues all of its
-      // loads (slots, the error flag and the wave's empty gathers) before testing any, so it
-      // costs one memory round trip.
-      uint64_t* rs = a.res + ((t % 3) * kPiResCopies + b % kPiResCopies) * kPiResStride;
-      double ta = 0.0, tr = 0.0;
-      bool ok = false;
-      if (b == 0) {
-        // lane l holds slots l, l + 64, ... ; sums in that order, then a fixed shuffle tree.
-        // Groups past the grid are skipped by a uniform test; lanes past the last slot re-read
-        // slot G - 1.
-        uint64_t va[kSlots], vr[kSlots];
-#pragma unroll
-        for (uint32_t k = 0; k < kSlots; ++k) va[k] = vr[k] = 0;
-        uint32_t lt = tid;  // opaque: slot offsets are recomputed here, not held across iterations
-        asm volatile("" : "+v"(lt));
-        for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
-          const int e = ldAgent(a.err);
-#pragma unroll
-          for (uint32_t k = 0; k < kSlots; ++k) {
-            if (k * kWave < G) {
-              const uint32_t off = min(lt + k * kWave, G - 1) * 16u;  // byte offset of the slot
-              const char* base = reinterpret_cast<const char*>(buf);
-              va[k] = ldAgent(reinterpret_cast<const uint64_t*>(base + off));
-              vr[k] = ldAgent(reinterpret_cast<const uint64_t*>(base + off + 8));
-            }
-          }
-          bool mine = true;
-#pragma unroll
-          for (uint32_t k = 0; k < kSlots; ++k)
-            if (k * kWave < G) mine = mine && va[k] != kCoopEmpty && vr[k] != kCoopEmpty;
-          ok = __all(mine);
-          if (ok || e) break;
-          __builtin_amdgcn_s_sleep(1);
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kSlots; ++k)
-          if (tid + k * kWave < G) {
-            ta += __longlong_as_double((long long)va[k]);
-            tr += __longlong_as_double((long long)vr[k]);
+      const double c = piScale(t, nuH[8 - kFlowLag]);  // nu_{t-8}
+      const double nuA = nuH[0], nuB = nuH[1];         // nu_{t-L}, nu_{t-L-1}
+      for (uint32_t g = wave, first = 1; g < nG; g += waves, first = 0) {
+        const uint32_t j = 64 * g + lane;
+        const bool valid = j < n;
+        double s = 0.0;
+        uint32_t e, e1;
+        if (first) {
+          uint64_t m0[kCoopCache];
+          flowIssue<kCoopCache>(src, ci, cnt0, m0);
+          if (!flowComplete<kCoopCache>(src, ci, cnt0, check, m0, a)) {
+            flowAbort(a);
+            return;
           }
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-          ta += __shfl_xor(ta, o, kWave);
-          tr += __shfl_xor(tr, o, kWave);
-        }
-        // the clears of result slot (t+1) % 3 (start of this iteration) have completed: the
-        // wave waited for its stores before the block published
-        if (tid < kPiResCopies && ok) {
-          uint64_t* rc = a.res + ((t % 3) * kPiResCopies + tid) * kPiResStride;
-          stAgent(rc, uint64_t(__double_as_longlong(ta)));
-          stAgent(rc + 1, uint64_t(__double_as_longlong(tr)));
-        }
-      }
-      {
-        uint64_t ra = 0, rr = 0;
-        for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
-          const int e = ldAgent(a.err);
-          if (b != 0) {
-            ra = ldAgent(rs);
-            rr = ldAgent(rs + 1);
+          for (int u = 0; u < kCoopCache; ++u) {
+            const double prod = asF64(m0[u]) * cvL[u][threadIdx.x];
+            s = (uint32_t(u) < cnt0) ? s + prod : s;
           }
-          if (spin > 0 && __any(piMissing(m, c0, c1))) piRegather(yCur, ci, c0, c1, m);
-          const bool have = b == 0 ? ok : (ra != kCoopEmpty && rr != kCoopEmpty);
-          const bool done = have && !__any(piMissing(m, c0, c1));
-          if (done || e) {
-            ok = done;
-            break;
+          e = c0 + kCoopCache;
+          e1 = c1;
+        } else {
+          e = valid ? a.colPtr[j] : 0;
+          e1 = valid ? a.colPtr[j + 1] : 0;
+        }
+        // the rest of the column, kCoopUnroll entries at a time (sums selected)
+        for (; __any(e < e1); e += kCoopUnroll) {
+          const uint32_t cnt = e1 > e ? e1 - e : 0;
+          uint32_t off[kCoopUnroll];
+          double v[kCoopUnroll];
+#pragma unroll
+          for (int u = 0; u < kCoopUnroll; ++u) {
+            const uint32_t eu = cnt ? min(e + u, e1 - 1) : 0;
+            off[u] = cnt ? a.rowIdx[eu] * uint32_t(sizeof(double)) : 0;
+            v[u] = cnt ? a.val[eu] : 0.0;
           }
-          __builtin_amdgcn_s_sleep(1);
+          uint64_t m[kCoopUnroll];
+          flowIssue<kCoopUnroll>(src, off, cnt, m);
+          if (!flowComplete<kCoopUnroll>(src, off, cnt, check, m, a)) {
+            flowAbort(a);
+            return;
+          }
+#pragma unroll
+          for (int u = 0; u < kCoopUnroll; ++u) {
+            const double prod = asF64(m[u]) * v[u];
+            s = (uint32_t(u) < cnt) ? s + prod : s;
+          }
         }
-        if (b != 0) {
-          ta = __longlong_as_double((long long)ra);
-          tr = __longlong_as_double((long long)rr);
+        if (c != 1.0) s *= c;
+        if (first) __builtin_amdgcn_s_waitcnt(0);  // earlier stores (clears included) have completed
+        if (valid) {
+          stAgent(dst + j, asU64(s));
+          stAgent(clr + j, kCoopEmpty);
         }
-      }
-      if (tid == 0) {
-        if (!ok) stAgent(a.err, 1);
-        leave = ok ? 0 : 1;
-        sums[0] = ta;
-        sums[1] = tr;
-      }
-    } else {
-      // the other waves complete their gathers of y_t (their owners store them before they
-      // publish, so the entries fill in while the totals arrive)
-      bool ok = false;
-      for (uint32_t spin = 0; spin < a.spinLimit; ++spin) {
-        if (!__any(piMissing(m, c0, c1))) {
-          ok = true;
-          break;
+        // the residual term of iteration t - L: |z_{t-L}/nu_{t-L} - z_{t-L-1}/nu_{t-L-1}|
+        double za = 0.0, zb = 0.0;
+        if (first) {
+          za = own[kFlowLag - 1];
+          zb = own[kFlowLag];
+#pragma unroll
+          for (uint32_t k = kFlowLag; k > 0; --k) own[k] = own[k - 1];
+          own[0] = s;
+        } else if (t > kFlowLag && valid) {
+          za = asF64(ldAgent(a.z + uint64_t((t - kFlowLag) % kFlowRing) * n + j));
+          zb = t - kFlowLag - 1 == 0 ? a.mu0[j]
+                                     : asF64(ldAgent(a.z + uint64_t((t - kFlowLag - 1) % kFlowRing) * n + j));
         }
-        const int e = ldAgent(a.err);
-        piRegather(yCur, ci, c0, c1, m);
-        if (e) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-      if (!ok && lane == 0) {
-        stAgent(a.err, 1);
-        failed = 1;
+        const double term = (t > kFlowLag && valid) ? fabs(za / nuA - zb / nuB) : 0.0;
+        const double ga = waveButterflySum(valid ? fabs(s) : 0.0);
+        const double gr = waveButterflySum(term);
+        if (lane == 0) {
+          uint64_t* pc = a.part + (uint64_t(t % kFlowPart) * nG + g) * 2;
+          stAgent(pc, asU64(ga));
+          stAgent(pc + 1, asU64(gr));
+        }
       }
     }
-    __syncthreads();
-    if (leave || failed) return;
-    const double ta = sums[0], tr = sums[1];
-    const uint32_t completed = t - 1;
-    const bool done = completed >= 1 && ((a.tol >= 0.0 && tr <= a.tol) || completed >= a.maxIter);
-    if (done) {
-      if (b == 0 && tid == 0) {
-        a.st->norm[t % 3] = ta;
-        a.st->norm[(t + 2) % 3] = nPrev;
-        a.st->norm[(t + 1) % 3] = nPrev2;
-        a.st->resid = tr;
-        a.st->final = completed;
-        a.st->t = t + 1;
-        a.st->done = 1;
-      }
-      return;
-    }
-    nPrev2 = nPrev;
-    nPrev = ta;
-    ++t;
-    __syncthreads();  // sums[] is rewritten by the next wait
   }
 }
 
@@ -2023,12 +2026,20 @@ static __global__ __launch_bounds__(kBlock) void copyKernel(const uint4* in, uin
     out[i] = in[i];
 }
 
-static __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* y, const double* mu0, uint32_t n,
+// *out = max(colPtr[j + 1] - colPtr[j]) over the columns (out zero at launch)
+static __global__ __launch_bounds__(kBlock) void colMaxLenKernel(const uint32_t* colPtr, uint32_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) m = max(m, colPtr[j + 1] - colPtr[j]);
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(out, m);
+}
+
+static __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* z, const double* mu0, uint32_t n,
                                                          const PiState* st, double* mu) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
-  const uint32_t f = st->final;
-  mu[j] = f == 0 ? mu0[j] : y[uint64_t(f % 3) * n + j] / st->norm[f % 3];
+  mu[j] = st->final == 0 ? mu0[j] : z[uint64_t(st->slot) * n + j] / st->nuFinal;
 }
 
 }  // namespace b12x

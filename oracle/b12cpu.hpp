@@ -1222,11 +1222,59 @@ class Tree {
 };
 
 // ---------------------------------------------------------------------------------------------
-// Invariant measure (build-defined; the reference leaves it to MATLAB, subdivisionByNim.m:30-59):
-// mu <- mu P / ||mu P||_1 with mu0 = 1/n, P given as COO; gather form over columns in ascending
-// row order; L1 norm summed in index order. Stops after maxIter iterations or when
-// ||mu_{k+1} - mu_k||_1 <= tol (tol < 0 disables the test).
+// Invariant measure (build-defined; the reference leaves it to MATLAB, subdivisionByNim.m:30-59).
+// Power iteration on unnormalised iterates, row-vector convention, P given as COO:
+//   z_0 = mu_0 (1/n or the caller's vector), nu_0 = 1
+//   z_t[j] = c_t * sum_{i in column j, ascending} z_{t-1}[i] * P_ij        (sum from 0.0)
+//   nu_t = S(|z_t|),  r_t = S(|z_t[j] / nu_t - z_{t-1}[j] / nu_{t-1}|)
+//   stop at the first t with (tol >= 0 and r_t <= tol) or t = maxIter; mu = z_t / nu_t.
+// c_t is a power of two that keeps z_t in range: 1, except at t = 16, 24, 32, ... where it is
+// 2^-ilogb(nu_{t-8}) when nu_{t-8} is finite, positive and its exponent lies outside [-32, 32].
+// A power of two scales z_t and nu_t exactly, so mu_t = z_t / nu_t is that of the normalised
+// iteration mu_t = mu_{t-1} P / ||mu_{t-1} P||_1 with the normalisation deferred.
+// S is the sum in a fixed order over chunks of 256 columns: within a chunk, each group of 64
+// columns by a butterfly (x_l += x_{l^o} for o = 32, 16, ..., 1; entries past n are 0), the four
+// group sums added in order to 0.0; over chunks, lane l (0..63) adds chunks l, l+64, ... to 0.0
+// in ascending order, and the 64 lane sums combine by the same butterfly.
 // ---------------------------------------------------------------------------------------------
+inline double butterfly64(double* x) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double y[64];
+    for (int l = 0; l < 64; ++l) y[l] = x[l] + x[l ^ o];
+    for (int l = 0; l < 64; ++l) x[l] = y[l];
+  }
+  return x[0];
+}
+
+template <class F>
+inline double canonicalSum(size_t n, F&& value) {
+  const size_t chunks = (n + 255) / 256;
+  double lane[64];
+  for (int l = 0; l < 64; ++l) lane[l] = 0.0;
+  for (size_t q = 0; q < chunks; ++q) {
+    double c = 0.0;
+    for (int w = 0; w < 4; ++w) {
+      double x[64];
+      for (int l = 0; l < 64; ++l) {
+        const size_t j = q * 256 + size_t(w) * 64 + size_t(l);
+        x[l] = j < n ? value(j) : 0.0;
+      }
+      c += butterfly64(x);
+    }
+    lane[q % 64] += c;
+  }
+  return butterfly64(lane);
+}
+
+// the scale c_t of iteration t from the norm nu_{t-8}
+inline double piScale(uint64_t t, double nuLag) {
+  if (t < 16 || t % 8 != 0) return 1.0;
+  if (!(nuLag > 0.0) || !std::isfinite(nuLag)) return 1.0;
+  const int e = std::ilogb(nuLag);
+  if (e >= -32 && e <= 32) return 1.0;
+  return std::ldexp(1.0, -std::max(-1000, std::min(1000, e)));
+}
+
 inline int powerIteration(const Coo& P, int maxIter, double tol, std::vector<double>& mu, double* residOut) {
   NrBoxes n = P.nRows;
   std::vector<size_t> order(P.rows.size());
@@ -1241,27 +1289,30 @@ inline int powerIteration(const Coo& P, int maxIter, double tol, std::vector<dou
   std::vector<double> va;
   for (size_t e : order) if (P.cols[e] < n) { ri.push_back(P.rows[e]); va.push_back(P.vals[e]); }
   if (mu.size() != n) mu.assign(n, 1.0 / double(n));
-  std::vector<double> nx(n);
+  std::vector<double> z = mu, nz(n);
+  std::vector<double> nuHist;  // nu_t for t = 0, 1, ...
+  nuHist.push_back(1.0);
   int it = 0;
   double resid = -1.0;
   for (it = 0; it < maxIter;) {
+    const uint64_t t = uint64_t(it) + 1;
+    const double c = piScale(t, t >= 8 ? nuHist[t - 8] : 0.0);
     for (NrBoxes j = 0; j < n; ++j) {
       double s = 0.0;
-      for (NrBoxes e = cptr[j]; e < cptr[j + 1]; ++e) s += mu[ri[e]] * va[e];
-      nx[j] = s;
+      for (NrBoxes e = cptr[j]; e < cptr[j + 1]; ++e) s += z[ri[e]] * va[e];
+      nz[j] = c != 1.0 ? s * c : s;
     }
-    double nrm = 0.0;
-    for (NrBoxes j = 0; j < n; ++j) nrm += std::fabs(nx[j]);
-    double r = 0.0;
-    for (NrBoxes j = 0; j < n; ++j) {
-      double v = nx[j] / nrm;
-      r += std::fabs(v - mu[j]);
-      mu[j] = v;
-    }
+    const double nu = canonicalSum(n, [&](size_t j) { return std::fabs(nz[j]); });
+    const double nuPrev = nuHist.back();
+    const double r = canonicalSum(n, [&](size_t j) { return std::fabs(nz[j] / nu - z[j] / nuPrev); });
+    nuHist.push_back(nu);
+    z.swap(nz);
     ++it;
     resid = r;
     if (tol >= 0.0 && r <= tol) break;
   }
+  if (it > 0)
+    for (NrBoxes j = 0; j < n; ++j) mu[j] = z[j] / nuHist.back();
   if (residOut) *residOut = resid;
   return it;
 }

@@ -1,9 +1,12 @@
 """Files of the reference's formats: the tree's JSON file, the legacy binary file of B12a/B12b,
-GAIO's box list and the matrix's JSON file. The expected bytes come from a writer in this file
-that follows the reference's writers (ImplicitBoxTree.hpp:435-569, CooMatrix.hpp:473-534,
-helpFunctions.hpp:100-132) and a reader of the legacy layout (ImplicitBoxTree.hpp:174-257)."""
+GAIO's box list and the matrix's JSON file. The expected tree files come from tests/cpp/
+json_stream, which writes them with C++ iostreams as B12's stream helpers do (helpFunctions.hpp:
+101-216, ImplicitBoxTree.hpp:436-524: the uint8 arrays sdScheme, depthVector and flagsVector as
+one raw byte per element); the matrix file from a writer in this file (CooMatrix.hpp:473-534);
+the legacy layout from ImplicitBoxTree.hpp:174-257."""
 import os
 import struct
+import subprocess
 
 import numpy as np
 import pytest
@@ -13,25 +16,30 @@ from helpers import random_tree, same_tree
 
 pytestmark = pytest.mark.gpu
 
+JSON_STREAM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "json_stream")
+# bytes the reference's reader loses (whitespace, quote) or splits or stops at (comma, ']')
+UNREADABLE = {9, 10, 11, 12, 13, 32, 34, 44, 93}
+
 
 def _vec(name, items):
     return '\n  "%s": [' % name + ",".join("\n    " + x for x in items) + "\n  ]"
 
 
-def ref_tree_json(center, radius, sd, bits, depth, flags):
-    s = "{"
-    s += '\n  "B12_VERSION_MAJOR": 3,'
-    s += '\n  "B12_VERSION_MINOR": 1,'
-    s += _vec("center", ["%.17g" % v for v in center]) + ","
-    s += _vec("radius", ["%.17g" % v for v in radius]) + ","
-    s += _vec("sdScheme", [str(int(v)) for v in sd]) + ","
-    s += '\n  "nLeaves": %d,' % len(bits)
-    s += _vec("bitPatternVector", [str(int(v)) for v in bits]) + ","
-    s += _vec("depthVector", [str(int(v)) for v in depth]) + ","
-    s += _vec("flagsVector", [str(int(v)) for v in flags]) + ","
-    s += '\n  "nHostBufferBytes": 4294967296'
-    s += "\n}"
-    return s
+def ref_tree_json(center, radius, sd, bits, depth, flags, host_bytes=4 << 30):
+    """The tree file as json_stream writes it with iostreams (bytes)."""
+    words = [len(center), *["%.17g" % v for v in center], *["%.17g" % v for v in radius], len(sd), *map(int, sd),
+             len(bits), *map(int, bits), *map(int, depth), *map(int, flags), int(host_bytes)]
+    out = subprocess.run([JSON_STREAM, "write"], input=" ".join(map(str, words)).encode(), capture_output=True,
+                         check=True)
+    return out.stdout
+
+
+def readable_tree(rng, dim, n, max_depth):
+    """A random tree whose depths and flags the reference's reader reads back."""
+    bits, depth, _ = random_tree(rng, dim, n, max_depth=max_depth)
+    keep = np.array([int(d) not in UNREADABLE for d in depth])
+    flags = rng.choice(np.array([0x00, 0x01, 0x08, 0x10, 0x18, 0x11], np.uint8), len(bits))
+    return bits[keep], depth[keep], flags[keep]
 
 
 def ref_coo_json(nr, nc, rows, cols, vals):
@@ -56,7 +64,7 @@ def test_tree_json_bytes_and_round_trip(tmp_path, dim, sd, seed):
     rng = np.random.default_rng(700 + seed)
     c, r = rng.uniform(-3, 3, dim), rng.uniform(0.1, 5, dim)
     g = gx.ImplicitBoxTree(c, r, sd)
-    g.upload(*random_tree(rng, dim, 500, max_depth=20))
+    g.upload(*readable_tree(rng, dim, 500, max_depth=24))
     bits, depth, flags = g.download()
     # the reference keeps 64 scheme entries for its default scheme and 65 for a given one,
     # repeated periodically
@@ -64,8 +72,8 @@ def test_tree_json_bytes_and_round_trip(tmp_path, dim, sd, seed):
     path = str(tmp_path / "tree")
     g.save(path)
     assert os.path.exists(path + ".json")
-    text = open(path + ".json").read()
-    assert text == ref_tree_json(c, r, scheme, bits, depth, flags)
+    raw = open(path + ".json", "rb").read()
+    assert raw == ref_tree_json(c, r, scheme, bits, depth, flags)
     h = gx.ImplicitBoxTree.load(path, dim)  # the name without ".json" finds the file
     assert same_tree(h.download(), (bits, depth, flags))
     c2, r2, sd2 = h.geometry()
@@ -76,19 +84,56 @@ def test_tree_json_bytes_and_round_trip(tmp_path, dim, sd, seed):
     assert np.array_equal(g.searchByPoints(pts), h.searchByPoints(pts))
 
 
+def test_tree_json_bytes_of_every_depth_and_flag_value(tmp_path):
+    """Every depth 1..64 and every flag byte 0..255 is written as its raw byte, as iostreams
+    write it; trees holding a byte the reference's reader loses are rejected on loading, with
+    the reason, while json_stream's reader (the same extraction) returns fewer elements."""
+    print("[files] tree json every byte", flush=True)
+    c, r = [0.0], [1.0]
+    # pattern 0...01 at every depth d = 1..64: the right half of the lowest box of depth d - 1,
+    # so the leaves are disjoint
+    bits, depth = [1] * 64, list(range(1, 65))
+    flags = [int(v) for v in np.arange(64) * 4 % 256]
+    g = gx.ImplicitBoxTree(c, r)
+    g.upload(np.array(bits, np.uint64), np.array(depth, np.uint8), np.array(flags, np.uint8))
+    b, d, f = g.download()
+    path = str(tmp_path / "every.json")
+    g.save(path)
+    raw = open(path, "rb").read()
+    assert raw == ref_tree_json(c, r, default_scheme(1, 64), b, d, f)
+    for name, vals in (("depthVector", d), ("flagsVector", f)):
+        got = subprocess.run([JSON_STREAM, "read", name], stdin=open(path, "rb"), capture_output=True, check=True)
+        back = [int(x) for x in got.stdout.split()]
+        assert len(back) < len(vals)  # bytes 9-13, 32, 34, 44 and 93 are among them
+    with pytest.raises(gx.B12xError, match="fewer entries than nLeaves"):
+        gx.ImplicitBoxTree.load(path, 1)
+    # every flag value on its own: written raw, read back unless the reader loses it
+    for v in range(256):
+        g1 = gx.ImplicitBoxTree(c, r)
+        g1.upload(np.array([0, 1], np.uint64), np.array([1, 1], np.uint8), np.array([v, 0], np.uint8))
+        p1 = str(tmp_path / ("f%d.json" % v))
+        g1.save(p1)
+        assert open(p1, "rb").read() == ref_tree_json(c, r, default_scheme(1, 64), [0, 1], [1, 1], [v, 0])
+        if v in UNREADABLE:
+            with pytest.raises(gx.B12xError):
+                gx.ImplicitBoxTree.load(p1, 1)
+        else:
+            assert list(gx.ImplicitBoxTree.load(p1, 1).download()[2]) == [v, 0]
+
+
 def test_tree_json_merges_equal_boxes_and_reports_missing_files(tmp_path):
     """The loader merges equal consecutive boxes (flags ANDed), as the reference's loader does;
     a missing file is an error."""
     c, r = [0.0, 0.0], [1.0, 1.0]
     bits = [0, 0, 1, 3]
     depth = [1, 1, 2, 2]
-    flags = [0x09, 0x08, 0x01, 0x10]
+    flags = [0x19, 0x18, 0x01, 0x10]
     path = str(tmp_path / "dups.json")
-    with open(path, "w") as f:
+    with open(path, "wb") as f:
         f.write(ref_tree_json(c, r, default_scheme(2, 64), bits, depth, flags))
     h = gx.ImplicitBoxTree.load(path, 2)
     b, d, fl = h.download()
-    assert list(b) == [0, 1, 3] and list(d) == [1, 2, 2] and list(fl) == [0x08, 0x01, 0x10]
+    assert list(b) == [0, 1, 3] and list(d) == [1, 2, 2] and list(fl) == [0x18, 0x01, 0x10]
     with pytest.raises(gx.B12xError):
         gx.ImplicitBoxTree.load(str(tmp_path / "nothing_here"), 2)
 

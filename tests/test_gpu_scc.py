@@ -122,4 +122,4 @@ def test_scc_and_power_iteration_in_any_entry_order():
     assert np.array_equal(b.power_iteration(30)[0], ref)
     o = OCoo.from_arrays(rows[sh], cols[sh], vals[sh], a.nRows, a.nColumns)
     mu_o = o.power_iteration(30)[0]
-    assert np.abs(mu_o - ref).sum() <= 1e-12 * np.abs(ref).sum()  # the measure's bar: 1e-12 relative
+    assert np.array_equal(mu_o, ref)  # bit for bit (the measure's bar: 1e-12 relative)

@@ -252,6 +252,150 @@ def test_subdivide_unsubdivide_roundtrip():
         assert same_tree(o.download(), before)
 
 
+def _unsubdivide_brute_force(bits, depth, flags, f):
+    """unsubdivide(f) restated on bit patterns alone (Functors.hpp:337-377, ImplicitBoxTree.hpp:
+    2023-2058,3413-3461): a leaf of depth > 0 with all of f set merges into its parent iff its
+    sibling (the parent's other half) is a leaf with all of f set, or no leaf lies inside the
+    sibling's region at all; equal parents become one box with the AND of their flags. Returns
+    (#parents formed, the resulting leaf set as {(bits, depth): flags})."""
+    leaves = {(int(b), int(d)): int(fl) for b, d, fl in zip(bits, depth, flags)}
+
+    def inside(region, leaf):  # leaf lies in the region (region is an ancestor of it or it)
+        (rb, rd), (lb, ld) = region, leaf
+        return ld >= rd and (lb >> (ld - rd)) == rb
+
+    out, parents = {}, {}
+    for (b, d), fl in leaves.items():
+        merge = False
+        if d > 0 and (fl & f) == f:
+            sib = (b ^ 1, d)
+            if sib in leaves:
+                merge = (leaves[sib] & f) == f
+            else:
+                merge = not any(inside(sib, l) for l in leaves)
+        if merge:
+            p = (b >> 1, d - 1)
+            parents[p] = parents.get(p, 0xFF) & fl
+        else:
+            out[(b, d)] = fl
+    assert not set(parents) & set(out)  # a formed parent never coincides with a kept leaf
+    out.update(parents)
+    return len(parents), out
+
+
+def test_unsubdivide_merge_rule_brute_force():
+    """unsubdivide on random trees with holes, unflagged siblings, deeper siblings and mixed
+    flags against the bit-pattern restatement above, for several flag masks."""
+    rng = np.random.default_rng(61)
+    cases = 0
+    for trial in range(40):
+        dim = 1 + trial % 4
+        bits, depth, flags = random_tree(rng, dim, int(rng.integers(2, 160)), max_depth=int(rng.integers(1, 9)),
+                                         flag_prob=0.7, flags_pool=(0x08, 0x01, 0x09, 0x18, 0x00))
+        for f in (gx.SD, gx.HIT, gx.SD | gx.HIT, 0):
+            o = OracleTree(np.zeros(dim), np.ones(dim))
+            o.upload(bits, depth, flags)
+            res, expect = _unsubdivide_brute_force(*o.download(), f)
+            assert o.unsubdivide(f) == res
+            b, d, fl = o.download()
+            assert {(int(x), int(y)): int(z) for x, y, z in zip(b, d, fl)} == expect
+            cases += res > 0
+    assert cases > 40  # most cases merge something
+
+
+def _leaf_index_of_points(bits, depth, pts, c, r):
+    """Index of the leaf holding each point by its depth-64 key (coordinateFunctions.hpp:61-93:
+    u = ((p-c)/r + 1)/2, b_j = floor(2^nb_j u), bit i of b_j at position i*dim + (63-j)%dim),
+    or INVALID outside the domain or in a hole; brute force over the leaves."""
+    dim = len(c)
+    nb = [64 // dim + (j < 64 % dim) for j in range(dim)]
+    out = np.full(len(pts), gx.INVALID, np.uint32)
+    for q, p in enumerate(pts):
+        if not np.all((c - r <= p) & (p < c + r)):
+            continue
+        key = 0
+        for j in range(dim):
+            u = ((p[j] - c[j]) / r[j] + 1.0) / 2.0
+            bj = int(np.floor(np.ldexp(u, nb[j])))
+            for i in range(nb[j]):
+                pos = i * dim + (63 - j) % dim
+                if pos < 64 and (bj >> i) & 1:
+                    key |= 1 << pos
+        for k, (b, d) in enumerate(zip(bits, depth)):
+            if (key >> (64 - int(d))) == int(b) if int(d) > 0 else True:
+                out[q] = k
+                break
+    return out
+
+
+def _grid_points_of_leaf(b, d, c, r, n_per_dim):
+    """InnerGrid(n) points of leaf (b, d), default scheme (Grids.hpp:99-134, coordinateFunctions.
+    hpp:128-159): level l bisects coordinate l % dim; centre ((2^-nb idx)*2 - 1)*r + c + r_box."""
+    dim = len(c)
+    idx = [0] * dim
+    nbj = [0] * dim
+    for lev in range(d):
+        j = lev % dim
+        idx[j] = (idx[j] << 1) | ((b >> (d - 1 - lev)) & 1)
+        nbj[j] += 1
+    rb = np.array([np.ldexp(r[j], -nbj[j]) for j in range(dim)])
+    cb = np.array([((np.ldexp(float(idx[j]), -nbj[j]) * 2.0 - 1.0) * r[j] + c[j]) + rb[j] for j in range(dim)])
+    k = n_per_dim ** dim
+    pts = np.empty((k, dim))
+    for i in range(k):
+        for j in range(dim):
+            g = ((float((i // n_per_dim ** j) % n_per_dim) * 2.0 + 1.0) - n_per_dim) / n_per_dim
+            pts[i, j] = cb[j] + g * rb[j]
+    return pts
+
+
+@pytest.mark.parametrize("row_major", [False, True])
+@pytest.mark.parametrize("use_abs", [False, True])
+def test_transition_matrix_invalid_column_label_hand_built_tree(row_major, use_abs):
+    """transitionMatrix with useInvalidNrBoxes on a hand-built Hénon tree with holes, recomputed
+    from the grid formula, the map's closed form and brute-force leaf geometry: with invalid
+    targets the matrix has n + 1 columns and they are labelled n + 1, nColumns_ after its
+    increment (ImplicitBoxTree.hpp:3081-3087 row-major, :3127-3132 column-major); without the
+    flag they are dropped."""
+    c, r = np.array([0.0, 0.0]), np.array([1.5, 0.5])
+    # the depth-4 boxes of the domain, with holes
+    keep = [0, 1, 2, 3, 5, 6, 9, 10, 11, 12, 13, 15]
+    bits = np.array(keep, np.uint64)
+    depth = np.full(len(keep), 4, np.uint8)
+    o = OracleTree(c, r)
+    o.upload(bits, depth, np.zeros(len(keep), np.uint8))
+    b, d, _ = o.download()
+    n, kpd = len(b), 2
+    k = kpd ** 2
+    rows, cols = [], []
+    for i in range(n):
+        x = _grid_points_of_leaf(int(b[i]), int(d[i]), c, r, kpd)
+        y = np.stack([(1.0 - (1.4 * x[:, 0]) * x[:, 0]) + x[:, 1], 0.3 * x[:, 0]], axis=1)
+        tgt = _leaf_index_of_points(b, d, y, c, r)
+        rows += [i] * k
+        cols += list(tgt)
+    rows, cols = np.array(rows, np.int64), np.array(cols, np.int64)
+    assert (cols == gx.INVALID).sum() > 0 and (cols != gx.INVALID).sum() > 0
+    for use_invalid in (False, True):
+        m = o.transitionMatrixForLeaves(gx.InnerGrid(kpd), gx.Henon(), use_abs=use_abs, row_major=row_major,
+                                        use_invalid=use_invalid)
+        pairs = {}
+        for a, z in zip(rows, cols):
+            if z == gx.INVALID and not use_invalid:
+                continue
+            pairs[(a, z)] = pairs.get((a, z), 0) + 1
+        # sort with the invalid label (0xFFFFFFFF) last, as before the relabelling, then relabel
+        order = sorted(pairs, key=(lambda e: (e[0], e[1])) if row_major else (lambda e: (e[1], e[0])))
+        er = np.array([e[0] for e in order], np.uint32)
+        ec = np.array([n + 1 if e[1] == gx.INVALID else e[1] for e in order], np.uint32)
+        ev = np.array([pairs[e] if use_abs else pairs[e] / float(k) for e in order], np.float64)
+        gr, gc, gv = m.download()
+        assert m.nRows == n and m.nColumns == (n + 1 if use_invalid else n)
+        assert np.array_equal(gr, er) and np.array_equal(gc, ec) and bits_equal(gv, ev)
+        if use_invalid:
+            assert np.sum(gc == n + 1) > 0 and np.all(gc != n)
+
+
 def test_remove_keeps_flagged_and_reinserts_root():
     o = OracleTree([0.0, 0.0], [1.0, 1.0])
     o.setFlags(gx.SD, 0)
@@ -366,6 +510,50 @@ def test_power_iteration_matches_numpy_eig():
     v = np.abs(V[:, k].real)
     v /= v.sum()
     assert np.allclose(mu, v, rtol=1e-9, atol=1e-13)
+
+
+def test_power_iteration_exact_on_integer_iterates():
+    """With integer counts (useAbsoluteValues) and an integer start vector every iterate z_t =
+    z_0 P^t is an exact integer below 2^53, whatever the order of the sums, so mu_t = z_t / |z_t|_1
+    is one correctly rounded division per entry: recomputed exactly in Python integers."""
+    rng = np.random.default_rng(12)
+    n = 300
+    nnz = 1200
+    rows = rng.integers(0, n, nnz)
+    cols = rng.integers(0, n, nnz)
+    key = np.unique(rows * n + cols)
+    rows, cols = (key // n).astype(np.uint32), (key % n).astype(np.uint32)
+    vals = rng.integers(1, 4, len(rows)).astype(np.float64)
+    z0 = rng.integers(1, 5, n).astype(np.float64)
+    for iters in (1, 2, 5, 7):
+        mu, it, _ = OCoo.from_arrays(rows, cols, vals, n, n).power_iteration(iters, -1.0, mu=z0)
+        z = [int(v) for v in z0]
+        for _ in range(iters):
+            nz = [0] * n
+            for a, b, v in zip(rows.tolist(), cols.tolist(), vals.tolist()):
+                nz[b] += z[a] * int(v)
+            z = nz
+        assert max(z) < 2 ** 53
+        total = sum(abs(v) for v in z)
+        assert it == iters and bits_equal(mu, np.array([float(v) / float(total) for v in z]))
+
+
+def test_power_iteration_rescales_decaying_iterates():
+    """A matrix whose rows sum to 2^-10: unscaled iterates would fall below the FP64 range after
+    about 110 iterations. The oracle's exact power-of-two rescaling keeps them in range, and its
+    measure after 400 iterations equals numpy's normalised iteration mu <- mu P / |mu P|_1."""
+    rng = np.random.default_rng(13)
+    n = 60
+    A = rng.random((n, n)) * (rng.random((n, n)) < 0.4) + np.eye(n)
+    P = A / A.sum(axis=1, keepdims=True) * 2.0 ** -10
+    r, c = np.nonzero(P)
+    mu, it, res = OCoo.from_arrays(r, c, P[r, c], n, n).power_iteration(400)
+    v = np.full(n, 1.0 / n)
+    for _ in range(400):
+        v = v @ P
+        v /= v.sum()
+    assert it == 400 and np.all(np.isfinite(mu)) and res < 1e-12
+    assert np.allclose(mu, v, rtol=1e-12, atol=0.0)
 
 
 # ------------------------------------------------------------------ depths between root and leaves
@@ -929,3 +1117,23 @@ def test_subdivision_by_nim_cover_inside_the_plain_subdivision():
         kept = set(zip(*[x.tolist() for x in o.download()[:2]]))
         plain = set(zip(*[x.tolist() for x in p.download()[:2]]))
         assert kept and kept <= plain and got[2] == len(kept), d
+
+
+def test_json_stream_helper_writes_raw_uint8_bytes():
+    """tests/cpp/json_stream (the tree file as iostreams write it, used by the file tests) puts
+    one raw byte per uint8 element and reads one character per token."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "json_stream")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/json_stream not built")
+    exp = ('{\n  "B12_VERSION_MAJOR": 3,\n  "B12_VERSION_MINOR": 1,\n  "center": [\n    0.10000000000000001\n  ],'
+           '\n  "radius": [\n    1\n  ],\n  "sdScheme": [\n    \x00\n  ],\n  "nLeaves": 2,'
+           '\n  "bitPatternVector": [\n    0,\n    1\n  ],\n  "depthVector": [\n    \x14,\n    \x14\n  ],'
+           '\n  "flagsVector": [\n    \x08,\n    \t\n  ],\n  "nHostBufferBytes": 4294967296\n}').encode()
+    out = subprocess.run([exe, "write"], input=b"1 0.1 1 1 0 2 0 1 20 20 8 9 4294967296", capture_output=True,
+                         check=True).stdout
+    assert out == exp
+    back = subprocess.run([exe, "read", "flagsVector"], input=out, capture_output=True, check=True).stdout
+    assert back.split() == [b"8"]  # the tab (SD|HIT) is lost as whitespace
+    back = subprocess.run([exe, "read", "depthVector"], input=out, capture_output=True, check=True).stdout
+    assert back.split() == [b"20", b"20"]
