@@ -84,6 +84,7 @@ SIGNATURES = {
     "b12x_coo_make_symmetric": [_P, _I],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_power_iteration_kernel_ms": [_P],
+    "b12x_power_iteration_noalloc": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
     "b12x_coo_scc": [_P, _P, _P, _P],
     "b12x_scc_last_stats": [_P, _P, _P],
@@ -304,6 +305,17 @@ class CooMatrix:
         it, res = _I(), _D()
         self.api("b12x_power_iteration", self.h, int(max_iter), float(tol), 0 if mu is None else 1, _ptr(mu_arr),
                  ctypes.byref(it), ctypes.byref(res))
+        return mu_arr, it.value, res.value
+
+    def power_iteration_noalloc(self, max_iter=200, tol=-1.0, mu=None):
+        """power_iteration through b12x_power_iteration_noalloc: no device allocation (a fixed device
+        scratch array; column-major matrices of at most 131072 columns)."""
+        if mu is not None and np.shape(mu) != (self.nRows,):
+            raise ValueError(f"mu must hold nRows = {self.nRows} values, got shape {np.shape(mu)}")
+        mu_arr = np.zeros(self.nRows, np.float64) if mu is None else np.ascontiguousarray(mu, np.float64).copy()
+        it, res = _I(), _D()
+        self.api("b12x_power_iteration_noalloc", self.h, int(max_iter), float(tol), 0 if mu is None else 1,
+                 _ptr(mu_arr), ctypes.byref(it), ctypes.byref(res))
         return mu_arr, it.value, res.value
 
     def subspace_iteration(self, m=4, iters=100, want_basis=False):

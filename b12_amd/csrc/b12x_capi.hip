@@ -2602,10 +2602,11 @@ struct PiWork {
 // when a bounded wait gave up (B12X_PI_PERSISTENT=2: an error instead); the batched form alone
 // otherwise, its iterations launched in batches from a captured HIP graph unless useGraph is
 // false. mu (host, n doubles) is the start vector when muInit, and receives the measure.
-// spinLimit bounds every wait of the persistent form (polls).
+// spinLimit bounds every wait of the persistent form (polls); timed records the persistent
+// launch's duration with HIP events. With useGraph and timed false nothing is allocated here.
 void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const double* vals, uint32_t n,
              int maxIter, double tol, int muInit, double* mu, int* iters, double* resid, const PiWork& w,
-             uint32_t workers, bool useGraph, uint32_t spinLimit) {
+             uint32_t workers, bool useGraph, uint32_t spinLimit, bool timed) {
   lastPiKernelMs = 0.0;
   const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
   PiState init{};
@@ -2636,21 +2637,25 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
     const bool full = maxLen > uint32_t(kCoopCache) || uint64_t(nG) > 4ull * workers;
     PiFlowArgs fa{colPtr, rows, vals, w.mu0, w.ring, w.part, w.tot, w.pst, w.err, n, nG, workers, mi, tol, spinLimit};
     hipEvent_t e0 = nullptr, e1 = nullptr;  // the launch's duration, for measurement
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, st));
+    if (timed) {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipEventRecord(e0, st));
+    }
     if (full) hipLaunchKernelGGL(piFlowKernel<true>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
     else hipLaunchKernelGGL(piFlowKernel<false>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e1, st));
+    if (timed) HIPCHK(hipEventRecord(e1, st));
     int herr = 0;
     HIPCHK(hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    float kms = 0.f;
-    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    lastPiKernelMs = kms;
+    if (timed) {
+      float kms = 0.f;
+      HIPCHK(hipEventElapsedTime(&kms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      lastPiKernelMs = kms;
+    }
     // a wait past its limit (other work holding the CUs): every wave has left; recompute in
     // batches (B12X_PI_PERSISTENT=2: report it instead)
     if (herr && pp && pp[0] == '2') throw HipFail(B12X_E_HIP, "persistent power iteration: a wait passed its limit");
@@ -2719,6 +2724,68 @@ uint32_t piSpinLimit() {
   }
   return 1u << 21;
 }
+
+// Scratch of b12x_power_iteration_noalloc: one file-scope device array, which every GPU the
+// library runs on holds until the process ends, each at its own address. Host code uses only
+// the address hipGetSymbolAddress returns on the device (the array's name is a host-side stand-in).
+// It holds the buffers of one solve of at most kNoAllocMaxCols columns.
+constexpr uint32_t kNoAllocMaxCols = 1u << 17;
+constexpr uint64_t noAllocAlign(uint64_t b) { return (b + 255) / 256 * 256; }
+constexpr uint64_t kNoAllocBytes =
+    noAllocAlign((uint64_t(kNoAllocMaxCols) + 1) * 4) +                         // colPtr
+    3 * noAllocAlign(uint64_t(kNoAllocMaxCols) * 8) +                           // mu0, dmu, partial
+    noAllocAlign(uint64_t(kNoAllocMaxCols) * 3 * 8) +                           // ybuf
+    noAllocAlign(sizeof(PiState)) +                                             // pst
+    noAllocAlign(uint64_t(kFlowRing) * kNoAllocMaxCols * 8) +                   // ring
+    noAllocAlign(uint64_t(kFlowPart) * (kNoAllocMaxCols / kWave) * 2 * 8) +    // part
+    noAllocAlign(uint64_t(kFlowTotWords) * 8) + 2 * noAllocAlign(4);            // tot, err, maxLen
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kScratchDevices = 16;
+std::mutex g_scratchMutex;
+void* g_scratchPtr[kScratchDevices] = {};
+
+// the scratch's device address on the current device, taken once per device with that device
+// current
+uint8_t* scratchOnDevice() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) throw HipFail(B12X_E_UNSUPPORTED, "device index beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw HipFail(B12X_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratchPtr[dev] = p;
+  }
+  return static_cast<uint8_t*>(g_scratchPtr[dev]);
+}
+
+// the buffers of one solve of n <= kNoAllocMaxCols columns, carved from the scratch
+PiWork carveNoAlloc(uint8_t* base, uint32_t** colPtr) {
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) {
+    uint8_t* p = base + off;
+    off += noAllocAlign(bytes);
+    return p;
+  };
+  const uint64_t N = kNoAllocMaxCols;
+  PiWork w;
+  *colPtr = reinterpret_cast<uint32_t*>(take((N + 1) * 4));
+  w.mu0 = reinterpret_cast<double*>(take(N * 8));
+  w.dmu = reinterpret_cast<double*>(take(N * 8));
+  w.partial = reinterpret_cast<double*>(take(N * 8));
+  w.ybuf = reinterpret_cast<double*>(take(N * 3 * 8));
+  w.pst = reinterpret_cast<PiState*>(take(sizeof(PiState)));
+  w.ring = reinterpret_cast<uint64_t*>(take(uint64_t(kFlowRing) * N * 8));
+  w.part = reinterpret_cast<uint64_t*>(take(uint64_t(kFlowPart) * (N / kWave) * 2 * 8));
+  w.tot = reinterpret_cast<uint64_t*>(take(uint64_t(kFlowTotWords) * 8));
+  w.err = reinterpret_cast<int*>(take(4));
+  w.maxLen = reinterpret_cast<uint32_t*>(take(4));
+  if (off > kNoAllocBytes) throw HipFail(B12X_E_HIP, "scratch layout exceeds the array");
+  return w;
+}
 }  // namespace
 
 extern "C" {
@@ -2753,7 +2820,37 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
       w.maxLen = ds.alloc<uint32_t>(1);
     }
     piSolve(st, C.colPtr, C.rows, C.vals, n, maxIter, tol, muInit, mu, iters, resid, w, workers, true,
-            piSpinLimit());
+            piSpinLimit(), true);
+    return B12X_OK;
+  });
+}
+
+int b12x_power_iteration_noalloc(b12x_coo m, int maxIter, double tol, int muInit, double* mu, int* iters,
+                                 double* resid) {
+  return guarded([&]() -> int {
+    lastPiKernelMs = 0.0;
+    if (!m || !mu) return setErr(B12X_E_ARG, "null argument");
+    if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
+    const uint32_t n = m->nRows;
+    if (n > kNoAllocMaxCols)
+      return setErr(B12X_E_UNSUPPORTED, "b12x_power_iteration_noalloc: " + std::to_string(n) +
+                                            " columns exceed its scratch maximum of " +
+                                            std::to_string(kNoAllocMaxCols) + " (b12x_power_iteration has no limit)");
+    if (!m->colMajor && m->nnz > 0)
+      return setErr(B12X_E_ARG,
+                    "b12x_power_iteration_noalloc: the matrix must be sorted column-major (transitionMatrix's "
+                    "default order, or b12x_coo_sort(m, 0))");
+    hipStream_t st = m->stream;
+    uint32_t* colPtr = nullptr;
+    const PiWork w = carveNoAlloc(scratchOnDevice(), &colPtr);
+    // CSC in place: the column-major entries are the columns' rows in ascending order; entries of
+    // the extra invalid column (column index n or above) lie past colPtr[n]
+    hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, st,
+                       (const uint32_t*)m->cols, m->nnz, n, colPtr);
+    HIPCHK(hipGetLastError());
+    const uint32_t workers = (maxIter > 0) ? piFlowWorkers(n) : 0;
+    piSolve(st, colPtr, m->rows, m->vals, n, maxIter, tol, muInit, mu, iters, resid, w, workers, false,
+            piSpinLimit(), false);
     return B12X_OK;
   });
 }

@@ -1222,11 +1222,9 @@ static __global__ __launch_bounds__(kBlock) void compressKernel(const uint32_t* 
   out[v] = uint32_t(lo);
 }
 
-// ---------------------------------------------------------------------------------------------
-// Power iteration on the CSC form (column j holds the entries P_ij in ascending row order):
-// y_j = sum_i mu_i P_ij summed in ascending row order, then mu <- y / ||y||_1. Block partial
-// sums go to fixed slots and are reduced in a fixed order, so the result is deterministic.
-// ---------------------------------------------------------------------------------------------
+// Block sum in a fixed order: each wave by a butterfly (every lane ends with the same value,
+// since a + b = b + a in IEEE arithmetic), then the wave sums added to 0.0 in wave order. For a
+// block of kBlock lanes this is the sum of one 256-column chunk of the power iteration's order.
 __device__ inline double blockReduceSumF64(double v) {
   __shared__ double waveSums[kBlock / kWave];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;

@@ -250,10 +250,23 @@ int b12x_coo_make_symmetric(b12x_coo m, int antisymmetric);
 int b12x_coo_copy(b12x_coo m, b12x_coo* out);
 /* compressIndexVector(indices, dim) — CooMatrix.h:20-23: sorted index vector -> dim+1 offsets */
 int b12x_compress_index_vector(const uint32_t* indices, uint64_t n, uint32_t dim, uint32_t* out);
-/* Invariant measure: mu <- mu P / ||mu P||_1, mu0 = 1/n (or the given mu if mu_init != 0),
- * until max_iter iterations or ||mu_{k+1} - mu_k||_1 <= tol (tol < 0: fixed count). The
- * reference computes this step in MATLAB (subdivisionByNim.m:30-59); there is no C++ API. */
+/* Invariant measure by power iteration: mu_k = mu_0 P^k / ||mu_0 P^k||_1, mu_0 = 1/n (or the given
+ * mu if mu_init != 0; mu holds n_rows doubles), until max_iter iterations or
+ * ||mu_{k+1} - mu_k||_1 <= tol (tol < 0: fixed count). The iterates are kept unnormalised (scaled
+ * by exact powers of two) and every sum runs in one fixed order, so the result does not depend on
+ * the form that ran it. The reference computes this step in MATLAB (subdivisionByNim.m:30-59);
+ * there is no C++ API. */
 int b12x_power_iteration(b12x_coo m, int max_iter, double tol, int mu_init, double* mu, int* iters, double* resid);
+/* b12x_power_iteration without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, results and kernels as b12x_power_iteration; the matrix must be
+ * sorted column-major (transitionMatrix's default order, or after b12x_coo_sort(m, 0)) and is read
+ * in place. Everything else lives in a file-scope device array that each GPU holds until the process
+ * ends (about 22 MB): matrices of more than 131072 columns are rejected with B12X_E_UNSUPPORTED.
+ * All calls on one GPU share that array, so calls that overlap on one GPU race; each call
+ * initialises what it reads. mu holds n_rows doubles. */
+int b12x_power_iteration_noalloc(b12x_coo m, int max_iter, double tol, int mu_init, double* mu, int* iters,
+                                 double* resid);
 /* Measurement (no counterpart in the reference): the duration of the calling thread's last
  * persistent power-iteration launch, from HIP events on the matrix's stream (0 when the last
  * call ran the batched form). */
