@@ -85,7 +85,7 @@ SIGNATURES = {
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_power_iteration_kernel_ms": [_P],
     "b12x_power_iteration_noalloc": [_P, _I, _D, _I, _P, _P, _P],
-    "b12x_subspace_iteration": [_P, _I, _I, _P, _P],
+    "b12x_subspace_iteration": [_P, _I, _I, _D, _P, _P, _P, _P],
     "b12x_coo_scc": [_P, _P, _P, _P],
     "b12x_scc_last_stats": [_P, _P, _P],
     "b12x_chain_recurrent_steps": [_P, _P, _U8, _I, _P],
@@ -318,13 +318,17 @@ class CooMatrix:
                  _ptr(mu_arr), ctypes.byref(it), ctypes.byref(res))
         return mu_arr, it.value, res.value
 
-    def subspace_iteration(self, m=4, iters=100, want_basis=False):
-        """(Ritz matrix H (m x m), basis X (m x n) or None) of b12x_subspace_iteration; the
-        eigenvalues of H approximate the m leading eigenvalues of the matrix."""
+    def subspace_iteration(self, m=4, iters=100, want_basis=False, tol=-1.0):
+        """(Ritz matrix H (m x m), basis X (m x n) or None, basis updates, residual) of
+        b12x_subspace_iteration: at most `iters` updates, stopping once ||X P - X H||_F / ||H||_F
+        is at most tol (tol < 0: exactly `iters` updates); the eigenvalues of H approximate the m
+        leading eigenvalues of the matrix."""
         H = np.zeros((m, m), np.float64)
         X = np.zeros((m, self.nRows), np.float64) if want_basis else None
-        self.api("b12x_subspace_iteration", self.h, int(m), int(iters), _ptr(H), _ptr(X) if want_basis else None)
-        return H, X
+        it, res = _I(), _D()
+        self.api("b12x_subspace_iteration", self.h, int(m), int(iters), float(tol), _ptr(H),
+                 _ptr(X) if want_basis else None, ctypes.byref(it), ctypes.byref(res))
+        return H, X, it.value, res.value
 
     # ---- ordering and comparison (CooMatrix.h:61-91, CooMatrix.hpp:304-447)
     def isSortedRowMajor(self):

@@ -2511,8 +2511,14 @@ Csc adoptCsc(Csc C, DevScope& ds) {
   return C;
 }
 
+// Subspace iteration (the oracle's definition; every dot product in dotPartialKernel's strided
+// order, which the oracle reproduces): X_0 = cholQR2(start); for k = 0, 1, ...: Y = X_k P, the
+// Ritz matrix H = X_k^T Y and, when a tolerance is set or k = maxIter, the residual
+// rho = ||Y - X_k H||_F / ||H||_F (the two sums on the device, the square roots and the quotient
+// on the host); stop at k = maxIter or rho <= tol, else X_{k+1} = cholQR2(Y).
 template <int M>
-void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* basis) {
+void subspaceRun(b12x_coo_s& mat, uint32_t n, int maxIter, double tol, double* ritz, double* basis, int* itersOut,
+                 double* residOut) {
   hipStream_t st = mat.stream;
   DevScope ds;
   Csc C = adoptCsc(buildCsc(mat, n), ds);
@@ -2522,9 +2528,9 @@ void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* b
   double* Y = ds.alloc<double>(uint64_t(M) * n);
   double* part = ds.alloc<double>(uint64_t(M) * M * nd);
   double* R = ds.alloc<double>(uint64_t(M) * M);
-  double* H = ds.alloc<double>(uint64_t(M) * M);
+  double* H = ds.alloc<double>(uint64_t(M) * M + 1);  // H, then the residual's sum of squares
   int* bad = ds.alloc<int>(1);
-  HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), st));
+  hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)bad, uint64_t(1), 0u);
   auto cholQR = [&](double* Z) {  // in place
     hipLaunchKernelGGL((dotPartialKernel<M, 1>), dim3(nd), dim3(kBlock), 0, st, (const double*)Z, (const double*)Z, n,
                        part);
@@ -2538,24 +2544,42 @@ void subspaceRun(b12x_coo_s& mat, uint32_t n, int iters, double* ritz, double* b
   hipLaunchKernelGGL(subspaceStartKernel, dim3(nb), dim3(kBlock), 0, st, X, n, M);
   cholQR(X);
   cholQR(X);
-  for (int it = 0; it < iters; ++it) {
+  double hh[M * M + 1];
+  int hbad = 0;
+  for (int k = 0;; ++k) {
     spmm(X, Y);
+    hipLaunchKernelGGL((dotPartialKernel<M, 0>), dim3(nd), dim3(kBlock), 0, st, (const double*)X, (const double*)Y,
+                       n, part);
+    hipLaunchKernelGGL(ritzKernel, dim3(1), dim3(kBlock), 0, st, (const double*)part, nd, M, H);
+    if (tol >= 0.0 || k == maxIter) {
+      hipLaunchKernelGGL(residualPartialKernel<M>, dim3(nd), dim3(kBlock), 0, st, (const double*)X,
+                         (const double*)Y, (const double*)H, n, part);
+      hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(kBlock), 0, st, (const double*)part, nd, H + M * M);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(hh, H, sizeof(hh), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (hbad) throw HipFail(B12X_E_ARG, "subspace iteration: Gram matrix not positive definite");
+      double h2 = 0.0;
+      for (int q = 0; q < M * M; ++q) h2 += hh[q] * hh[q];
+      const double rho = std::sqrt(hh[M * M]) / std::sqrt(h2);
+      if (k == maxIter || rho <= tol) {
+        std::memcpy(ritz, hh, sizeof(double) * M * M);
+        if (itersOut) *itersOut = k;
+        if (residOut) *residOut = rho;
+        break;
+      }
+    }
     cholQR(Y);
     cholQR(Y);
     std::swap(X, Y);
   }
-  spmm(X, Y);
-  hipLaunchKernelGGL((dotPartialKernel<M, 0>), dim3(nd), dim3(kBlock), 0, st, (const double*)X, (const double*)Y, n,
-                     part);
-  hipLaunchKernelGGL(ritzKernel, dim3(1), dim3(kBlock), 0, st, (const double*)part, nd, M, H);
-  if (basis) hipLaunchKernelGGL(transposeBasisKernel, dim3(nb), dim3(kBlock), 0, st, (const double*)X, n, M, Y);
-  HIPCHK(hipGetLastError());
-  int hbad = 0;
-  HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(ritz, H, uint64_t(M) * M * 8, hipMemcpyDeviceToHost, st));
-  if (basis) HIPCHK(hipMemcpyAsync(basis, Y, uint64_t(M) * n * 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  if (hbad) throw HipFail(B12X_E_ARG, "subspace iteration: Gram matrix not positive definite");
+  if (basis) {
+    hipLaunchKernelGGL(transposeBasisKernel, dim3(nb), dim3(kBlock), 0, st, (const double*)X, n, M, Y);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(basis, Y, uint64_t(M) * n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
 }
 
 // Worker blocks for the persistent power iteration over n columns: one wave per group of 64
@@ -2855,21 +2879,22 @@ int b12x_power_iteration_noalloc(b12x_coo m, int maxIter, double tol, int muInit
   });
 }
 
-int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, double* basis) {
+int b12x_subspace_iteration(b12x_coo m, int nvec, int max_iter, double tol, double* ritz, double* basis,
+                            int* iters, double* resid) {
   return guarded([&]() -> int {
-    if (!m || !ritz || nvec < 1 || nvec > kSubspaceMax || iters < 0) return setErr(B12X_E_ARG, "bad argument");
+    if (!m || !ritz || nvec < 1 || nvec > kSubspaceMax || max_iter < 0) return setErr(B12X_E_ARG, "bad argument");
     if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
     const uint32_t n = m->nRows;
     if (n < uint32_t(nvec)) return setErr(B12X_E_ARG, "fewer boxes than vectors");
     switch (nvec) {
-      case 1: subspaceRun<1>(*m, n, iters, ritz, basis); break;
-      case 2: subspaceRun<2>(*m, n, iters, ritz, basis); break;
-      case 3: subspaceRun<3>(*m, n, iters, ritz, basis); break;
-      case 4: subspaceRun<4>(*m, n, iters, ritz, basis); break;
-      case 5: subspaceRun<5>(*m, n, iters, ritz, basis); break;
-      case 6: subspaceRun<6>(*m, n, iters, ritz, basis); break;
-      case 7: subspaceRun<7>(*m, n, iters, ritz, basis); break;
-      default: subspaceRun<8>(*m, n, iters, ritz, basis); break;
+      case 1: subspaceRun<1>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      case 2: subspaceRun<2>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      case 3: subspaceRun<3>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      case 4: subspaceRun<4>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      case 5: subspaceRun<5>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      case 6: subspaceRun<6>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      case 7: subspaceRun<7>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      default: subspaceRun<8>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
     }
     return B12X_OK;
   });

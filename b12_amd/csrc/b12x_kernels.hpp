@@ -2002,6 +2002,40 @@ static __global__ __launch_bounds__(kBlock) void applyRinvKernel(double* Y, uint
   for (int c = 0; c < M; ++c) Y[uint64_t(i) * M + c] = x[c];
 }
 
+// Residual of the Ritz pairs: per box sum over c of (y_c - sum_d x_d H_dc)^2 (d, then c in
+// order), summed over the boxes in dotPartialKernel's strided order; the block partials go to
+// part[blockIdx.x]
+template <int M>
+static __global__ __launch_bounds__(kBlock) void residualPartialKernel(const double* X, const double* Y,
+                                                                      const double* H, uint32_t n, double* part) {
+  __shared__ double h[M * M];
+  if (threadIdx.x < M * M) h[threadIdx.x] = H[threadIdx.x];
+  __syncthreads();
+  double acc = 0.0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double x[M];
+#pragma unroll
+    for (int d = 0; d < M; ++d) x[d] = X[uint64_t(i) * M + d];
+    double r = 0.0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      double v = Y[uint64_t(i) * M + c];
+#pragma unroll
+      for (int d = 0; d < M; ++d) v -= x[d] * h[d * M + c];
+      r += v * v;
+    }
+    acc += r;
+  }
+  const double v = blockReduceSumF64(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// *out = the partials summed in sumPartials' order: one block
+static __global__ __launch_bounds__(kBlock) void sumPartialsKernel(const double* part, uint32_t nb, double* out) {
+  const double v = sumPartials(part, nb);
+  if (threadIdx.x == 0) *out = v;
+}
+
 // H_ab = x_a . z_b from dotPartialKernel (SYM = 0): one block
 static __global__ __launch_bounds__(kBlock) void ritzKernel(const double* part, uint32_t nb, int m, double* H) {
   for (int q = 0; q < m * m; ++q) {

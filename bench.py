@@ -105,9 +105,38 @@ def sweep_jobs(n_jobs, seed=0):
     return [(float(rng.uniform(1.0, 1.4)), int(rng.integers(14, 23))) for _ in range(n_jobs)]
 
 
+def plain_henon_job(gx, grid, a, d, pi_iter):
+    t = gx.ImplicitBoxTree(HENON_C, HENON_R)
+    t.setFlags(gx.SD, 0)
+    t.subdivision_steps(grid, gx.Henon(a, 0.3, 1), d)
+    m = t.transitionMatrixForLeaves(grid, gx.Henon(a, 0.3, 1))
+    mu, it, _ = m.power_iteration(pi_iter)
+    return t, m, mu, it
+
+
+def sweep_verify(gx, jobs, grid, pi_iter):
+    """Untimed check before the timed runs: every job of the batched form (one forest) has the
+    cover, matrix, measure and iteration count of its plain single-tree run, bit for bit."""
+    import numpy as np
+    F = gx.Forest(2, grid, 1, pi_max_iter=pi_iter)
+    ids = [F.add(HENON_C, HENON_R, gx.Henon(a, 0.3, 1), d) for a, d in jobs]
+    F.run()
+    boxes = []
+    for j, (a, d) in zip(ids, jobs):
+        t, m, mu, it = plain_henon_job(gx, grid, a, d, pi_iter)
+        same = (F.info(j)["iters"] == it and all(np.array_equal(x, y) for x, y in zip(F.cover(j), t.download()))
+                and all(np.array_equal(x, y) for x, y in zip(F.matrix(j), m.download()))
+                and np.array_equal(F.measure(j), mu))
+        if not same:
+            raise AssertionError(f"sweep job {j} (a={a}, depth {d}): batched result differs from the plain run")
+        boxes.append(t.size())
+    return min(boxes), max(boxes)
+
+
 def sweep_bench(gx, n_jobs, pi_iter=200):
     jobs = sweep_jobs(n_jobs)
     grid = gx.InnerGrid(4)
+    lo, hi = sweep_verify(gx, jobs, grid, pi_iter)
     # plain form: one job per launch sequence, sequentially
     t0 = time.perf_counter()
     for a, d in jobs:
@@ -127,7 +156,8 @@ def sweep_bench(gx, n_jobs, pi_iter=200):
     return {"config": f"{n_jobs} Hénon jobs, a~U[1.0,1.4], b=0.3, depth~U{{14..22}}, k=16, TM + {pi_iter} power "
                       f"iterations each", "plain_jobs_per_s": n_jobs / plain_s,
             "batched_jobs_per_s": n_jobs / batched_s, "ratio": plain_s / batched_s,
-            "plain_s": plain_s, "batched_s": batched_s}
+            "plain_s": plain_s, "batched_s": batched_s, "covers_boxes": [lo, hi],
+            "verified": "every job's cover, matrix, measure and iteration count equal its plain run (bit for bit)"}
 
 
 def serve_requests(n_req, rate=200.0, seed=0):
@@ -212,15 +242,19 @@ def hh_loop(t, grid, mp, depth):
     return imaged
 
 
-def hh_bench(gx, depth, iters, cpu_depth, threads):
+HH_VECTORS = 8                   # subspace size: the 4 leading eigenvalues and 4 guard vectors
+
+
+def hh_bench(gx, depth, iters, tol, cpu_depth, threads):
     """Config 3: Hénon-Heiles, 50 Störmer-Verlet steps of h = 0.02, energy band [0.11, 0.12],
     UniformGrid(64, seed 0), the band-restricted subdivision to `depth`, the transition matrix and
-    `iters` subspace iterations for the 4 leading eigenvalues."""
+    the subspace iteration for the 4 leading eigenvalues (HH_VECTORS vectors), stopped at a
+    residual ||X P - H^T X||_F / ||H||_F of `tol` or after `iters` basis updates."""
     import numpy as np
     grid, mp = gx.UniformGrid(64, 0), gx.HenonHeiles(2, 0.02, 50)
     w = gx.ImplicitBoxTree(HH_C, HH_R)  # warm-up pass (kernel loading, image-kernel tuning)
     hh_loop(w, grid, mp, depth)
-    w.transitionMatrixForLeaves(grid, mp).subspace_iteration(4, 1)
+    w.transitionMatrixForLeaves(grid, mp).subspace_iteration(HH_VECTORS, 1, tol=1.0)
     del w
     t = gx.ImplicitBoxTree(HH_C, HH_R)
     t.image_timing(1)
@@ -229,19 +263,21 @@ def hh_bench(gx, depth, iters, cpu_depth, threads):
     t1 = time.perf_counter()
     m = t.transitionMatrixForLeaves(grid, mp)
     t2 = time.perf_counter()
-    H, _ = m.subspace_iteration(4, iters)
+    H, _, updates, resid = m.subspace_iteration(HH_VECTORS, iters, tol=tol)
     t3 = time.perf_counter()
     ms_img, launches, pts = t.image_timing(0)
     imaged += t.size() * grid.points_per_box(4)
     ev = np.linalg.eigvals(H)
     ev = ev[np.argsort(-np.abs(ev))]
     out = {"config": f"Hénon-Heiles, 50 Verlet steps h=0.02, band E in [0.11, 0.12], UniformGrid(64, 0), depth {depth}, "
-                     f"TM + {iters} subspace iterations (4 vectors)",
-           "boxes": t.size(), "nnz": m.nnz, "images": imaged,
+                     f"TM + subspace iteration ({HH_VECTORS} vectors, the 4 leading Ritz values) to residual "
+                     f"{tol:g} or {iters} updates",
+           "boxes": t.size(), "nnz": m.nnz, "images": imaged, "subspace_updates": updates,
+           "subspace_residual": resid, "subspace_converged": bool(resid <= tol),
            "subdiv_s": t1 - t0, "tm_s": t2 - t1, "subspace_s": t3 - t2,
            "images_per_s": imaged / (t3 - t0),
            "image_kernel_tflops": pts * HH_FLOPS_PER_IMAGE / (ms_img * 1e-3) / 1e12 if ms_img > 0 else None,
-           "ritz_values": [[float(x.real), float(x.imag)] for x in ev]}
+           "ritz_values": [[float(x.real), float(x.imag)] for x in ev[:4]]}
     if cpu_depth > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -368,7 +404,8 @@ def main():
     ap.add_argument("--sweep-jobs", type=int, default=512, help="config 4 jobs (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="config 5 requests (0 to skip)")
     ap.add_argument("--hh-depth", type=int, default=28, help="config 3 depth (0 to skip)")
-    ap.add_argument("--hh-iters", type=int, default=100, help="config 3 subspace iterations")
+    ap.add_argument("--hh-iters", type=int, default=400, help="config 3 subspace iterations at most")
+    ap.add_argument("--hh-tol", type=float, default=1e-8, help="config 3 subspace residual to stop at")
     ap.add_argument("--hh-cpu-depth", type=int, default=18, help="config 3 CPU-baseline sample depth (0 to skip)")
     ap.add_argument("--cont-depth", type=int, default=27, help="continuation depth (0 to skip)")
     ap.add_argument("--cont-cpu-depth", type=int, default=24, help="continuation CPU-baseline depth (0 to skip)")
@@ -448,8 +485,8 @@ def main():
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = min(threads, 16)
     if args.hh_depth > 0:
-        out["hh"] = hh_bench(gx, args.hh_depth, args.hh_iters, 0 if args.no_cpu_baseline else args.hh_cpu_depth,
-                             threads)
+        out["hh"] = hh_bench(gx, args.hh_depth, args.hh_iters, args.hh_tol,
+                             0 if args.no_cpu_baseline else args.hh_cpu_depth, threads)
     if args.cont_depth > 0:
         out["continuation"] = continuation_bench(gx, args.cont_depth,
                                                  0 if args.no_cpu_baseline else args.cont_cpu_depth, threads)

@@ -272,11 +272,15 @@ int b12x_power_iteration_noalloc(b12x_coo m, int max_iter, double tol, int mu_in
  * call ran the batched form). */
 int b12x_power_iteration_kernel_ms(double* ms);
 /* Subspace iteration for the nvec (<= 8) leading eigenvalues (no counterpart in the reference,
- * which leaves eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59): X_0 = [1/n, 3 fixed
- * splitmix64 columns], then `iters` times X <- cholQR2(X P) (row-vector convention, as the power
- * iteration); ritz (nvec x nvec, row-major) = X^T (X P), whose eigenvalues approximate the
- * leading eigenvalues of P. basis (nvec x n, column by column) may be NULL. */
-int b12x_subspace_iteration(b12x_coo m, int nvec, int iters, double* ritz, double* basis);
+ * which leaves eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59): X_0 = cholQR2([1/n, splitmix64
+ * columns]); for k = 0, 1, ...: Y = X_k P (row-vector convention, as the power iteration), the Ritz
+ * matrix H = X_k^T Y and the residual rho = ||Y - X_k H||_F / ||H||_F; it stops at the first k with
+ * rho <= tol (tol >= 0) or k = max_iter, else X_{k+1} = cholQR2(Y). ritz (nvec x nvec, row-major)
+ * receives that H, whose eigenvalues approximate the leading eigenvalues of P; basis (nvec x n,
+ * column by column, may be NULL) X_k; *iters (may be NULL) k, the basis updates made; *resid (may
+ * be NULL) rho. Every sum runs in one fixed order, so results do not depend on timing. */
+int b12x_subspace_iteration(b12x_coo m, int nvec, int max_iter, double tol, double* ritz, double* basis, int* iters,
+                            double* resid);
 /* Strong components of the matrix's graph (an edge row -> column per entry; entries naming a
  * vertex >= n_rows, such as the column of points outside the cover, are left out), as
  * chainRecurrentSet.m:26-39 obtains them from MATLAB's graphconncomp. labels (n_rows entries,

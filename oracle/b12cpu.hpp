@@ -1369,12 +1369,18 @@ inline std::vector<uint8_t> energyBandStencil(const Tree<4>& T, double emin, dou
 // Subspace iteration for the m leading eigenvalues (build-defined; the reference leaves
 // eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59). Row-vector convention as the power
 // iteration: Y = X P with y_j[c] = sum_i x_i[c] P_ij over column j in ascending row order.
-//   X_0: column 0 = 1/n, column c > 0 = u(i, c) = (splitmix64(8 i + c) >> 11) 2^-53 * 2 - 1
-//   orthonormalise by Cholesky QR applied twice (Gram G = Y^T Y, G = R^T R, X = Y R^-1
-//   column by column: x_c = (y_c - sum_{d<c} R_dc x_d) / R_cc, d ascending)
-//   `iters` times: X <- cholQR2(X P); then the Ritz matrix H_ab = x_a . (X P)_b.
-// Dot products here are summed in index order (the GPU sums fixed blocks; results agree within
-// rounding). X is stored column by column (X[c * n + i]).
+//   X_0: column 0 = 1/n, column c > 0 = u(i, c) = (splitmix64(8 i + c) >> 11) 2^-53 * 2 - 1,
+//   orthonormalised by Cholesky QR applied twice (Gram G = Y^T Y, G = R^T R, X = Y R^-1
+//   column by column: x_c = (y_c - sum_{d<c} R_dc x_d) / R_cc, d ascending).
+//   For k = 1, 2, ...: Y = X_{k-1} P, the Ritz matrix H_ab = x_a . y_b and the residual
+//   rho_k = ||Y - X_{k-1} H||_F / ||H||_F; stop when tol >= 0 and rho_k <= tol (the result is
+//   H and X_{k-1}, k - 1 basis updates) or when k - 1 = maxIter; else X_k = cholQR2(Y).
+// Every dot product over the boxes runs in one fixed order: T = 256 min(ceil(n/256), 1024)
+// strided partial sums (partial t adds boxes t, t + T, ... to 0.0), each block of 256 partials
+// reduced by the wave butterfly and the four wave sums added to 0.0 in order, the block sums
+// then strided over 256 lanes (lane l adds blocks l, l + 256, ...) and reduced the same way.
+// ||H||_F^2 adds H_ab^2 in row-major order; the two square roots and the quotient follow.
+// X is stored column by column (X[c * n + i]).
 // ---------------------------------------------------------------------------------------------
 inline uint64_t splitmix64(uint64_t x) {
   x += 0x9e3779b97f4a7c15ull;
@@ -1387,15 +1393,45 @@ inline double subspaceStart(uint64_t i, int c) {
   return double(splitmix64(8 * i + uint64_t(c)) >> 11) * 0x1p-53 * 2.0 - 1.0;
 }
 
+// 256 partial sums: four waves of 64 by the butterfly, the wave sums added to 0.0 in order
+inline double blockSum256(const double* v) {
+  double s = 0.0;
+  for (int w = 0; w < 4; ++w) {
+    double x[64];
+    for (int l = 0; l < 64; ++l) x[l] = v[w * 64 + l];
+    s += butterfly64(x);
+  }
+  return s;
+}
+
+// sum over i < n of term(i) in the fixed strided order above
+template <class F>
+inline double stridedSum(size_t n, F&& term) {
+  const size_t nb = (n + 255) / 256, nd = std::min<size_t>(std::max<size_t>(nb, 1), 1024), T = nd * 256;
+  std::vector<double> part(nd);
+  double lanes[256];
+  for (size_t b = 0; b < nd; ++b) {
+    for (size_t l = 0; l < 256; ++l) {
+      double acc = 0.0;
+      for (size_t i = b * 256 + l; i < n; i += T) acc += term(i);
+      lanes[l] = acc;
+    }
+    part[b] = blockSum256(lanes);
+  }
+  for (size_t l = 0; l < 256; ++l) {
+    double acc = 0.0;
+    for (size_t q = l; q < nd; q += 256) acc += part[q];
+    lanes[l] = acc;
+  }
+  return blockSum256(lanes);
+}
+
 // false if the Gram matrix is not positive definite
 inline bool cholQR(std::vector<double>& X, NrBoxes n, int m) {
   std::vector<double> G(size_t(m) * m), R(size_t(m) * m, 0.0);
   for (int a = 0; a < m; ++a)
-    for (int b = a; b < m; ++b) {
-      double s = 0.0;
-      for (NrBoxes i = 0; i < n; ++i) s += X[size_t(a) * n + i] * X[size_t(b) * n + i];
-      G[a * m + b] = s;
-    }
+    for (int b = a; b < m; ++b)
+      G[a * m + b] = stridedSum(n, [&](size_t i) { return X[size_t(a) * n + i] * X[size_t(b) * n + i]; });
   for (int j = 0; j < m; ++j) {
     double d = G[j * m + j];
     for (int k = 0; k < j; ++k) d -= R[k * m + j] * R[k * m + j];
@@ -1416,7 +1452,26 @@ inline bool cholQR(std::vector<double>& X, NrBoxes n, int m) {
   return true;
 }
 
-inline bool subspaceIteration(const Coo& P, int m, int iters, std::vector<double>& H, std::vector<double>& X) {
+// ||Y - X H||_F / ||H||_F: per box the squares of (y_c - sum_d x_d H_dc) added over c in order
+inline double subspaceResidual(const std::vector<double>& X, const std::vector<double>& Y,
+                               const std::vector<double>& H, NrBoxes n, int m) {
+  const double r2 = stridedSum(n, [&](size_t i) {
+    double r = 0.0;
+    for (int c = 0; c < m; ++c) {
+      double v = Y[size_t(c) * n + i];
+      for (int d = 0; d < m; ++d) v -= X[size_t(d) * n + i] * H[d * m + c];
+      r += v * v;
+    }
+    return r;
+  });
+  double h2 = 0.0;
+  for (int q = 0; q < m * m; ++q) h2 += H[q] * H[q];
+  return std::sqrt(r2) / std::sqrt(h2);
+}
+
+// the number of basis updates, or -1 if a Gram matrix was not positive definite
+inline int subspaceIteration(const Coo& P, int m, int maxIter, double tol, std::vector<double>& H,
+                             std::vector<double>& X, double* residOut) {
   const NrBoxes n = P.nRows;
   std::vector<size_t> order(P.rows.size());
   std::iota(order.begin(), order.end(), 0);
@@ -1438,25 +1493,28 @@ inline bool subspaceIteration(const Coo& P, int m, int iters, std::vector<double
         out[size_t(c) * n + j] = s;
       }
   };
+  auto ritz = [&](const std::vector<double>& A, const std::vector<double>& B) {
+    H.assign(size_t(m) * m, 0.0);
+    for (int a = 0; a < m; ++a)
+      for (int b = 0; b < m; ++b)
+        H[a * m + b] = stridedSum(n, [&](size_t i) { return A[size_t(a) * n + i] * B[size_t(b) * n + i]; });
+  };
   X.assign(size_t(m) * n, 0.0);
   for (NrBoxes i = 0; i < n; ++i)
     for (int c = 0; c < m; ++c) X[size_t(c) * n + i] = c == 0 ? 1.0 / double(n) : subspaceStart(i, c);
-  if (!cholQR(X, n, m) || !cholQR(X, n, m)) return false;
+  if (!cholQR(X, n, m) || !cholQR(X, n, m)) return -1;
   std::vector<double> Y;
-  for (int it = 0; it < iters; ++it) {
+  for (int k = 0;; ++k) {
     spmm(X, Y);
-    if (!cholQR(Y, n, m) || !cholQR(Y, n, m)) return false;
+    ritz(X, Y);
+    if (tol >= 0.0 || k == maxIter) {
+      const double rho = subspaceResidual(X, Y, H, n, m);
+      if (residOut) *residOut = rho;
+      if (k == maxIter || rho <= tol) return k;
+    }
+    if (!cholQR(Y, n, m) || !cholQR(Y, n, m)) return -1;
     X.swap(Y);
   }
-  spmm(X, Y);
-  H.assign(size_t(m) * m, 0.0);
-  for (int a = 0; a < m; ++a)
-    for (int b = 0; b < m; ++b) {
-      double s = 0.0;
-      for (NrBoxes i = 0; i < n; ++i) s += X[size_t(a) * n + i] * Y[size_t(b) * n + i];
-      H[a * m + b] = s;
-    }
-  return true;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -30,7 +30,7 @@ def lib():
         _lib.b12cpu_power_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_double, _P, _P, _P]
         _lib.b12cpu_set_options.argtypes = [_P, ctypes.c_int, ctypes.c_uint64]
         _lib.b12cpu_energy_band_stencil.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P]
-        _lib.b12cpu_subspace_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P]
+        _lib.b12cpu_subspace_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P]
     return _lib
 
 
@@ -121,12 +121,16 @@ class OCoo:
                                           ctypes.byref(res)))
         return m, it.value, res.value
 
-    def subspace_iteration(self, m=4, iters=100, want_basis=False):
-        """(Ritz matrix H (m x m), basis X (m x n) or None) of the build-defined subspace iteration."""
+    def subspace_iteration(self, m=4, iters=100, want_basis=False, tol=-1.0):
+        """(Ritz matrix H (m x m), basis X (m x n) or None, basis updates, residual) of the
+        build-defined subspace iteration: at most `iters` updates, stopping once the residual
+        ||X P - X H||_F / ||H||_F is at most tol (tol < 0: exactly `iters` updates)."""
         H = np.zeros((m, m), np.float64)
         X = np.zeros((m, self.nRows), np.float64) if want_basis else None
-        _chk(lib().b12cpu_subspace_iteration(self.h, int(m), int(iters), _ptr(H), _ptr(X) if want_basis else None))
-        return H, X
+        it, res = ctypes.c_int(), ctypes.c_double()
+        _chk(lib().b12cpu_subspace_iteration(self.h, int(m), int(iters), float(tol), _ptr(H),
+                                             _ptr(X) if want_basis else None, ctypes.byref(it), ctypes.byref(res)))
+        return H, X, it.value, res.value
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -83,3 +83,45 @@ def test_fast_build_target_changes_and_measure_difference():
           f"measure L1 difference {l1:.3g}", flush=True)
     assert share <= 1e-6
     assert l1 <= 1e-6
+
+
+def test_config4_sweep_of_512_jobs_matches_single_trees():
+    """Configuration 4 at its size: the 512 Hénon jobs of the benchmark's sweep (a ~ U[1.0, 1.4],
+    b = 0.3, depth ~ U{14..22}, InnerGrid(4), 200 power iterations) in one forest. Every job's
+    cover, transition matrix and invariant measure equal its single-tree run bit for bit, and the
+    deepest jobs equal the oracle's."""
+    print("[config] 512-job sweep", flush=True)
+    import bench
+    from oracle_api import OracleTree as OT
+    jobs = bench.sweep_jobs(512)
+    grid = gx.InnerGrid(4)
+    F = gx.Forest(2, grid, 1, pi_max_iter=200)
+    ids = [F.add(bench.HENON_C, bench.HENON_R, gx.Henon(a, 0.3, 1), d) for a, d in jobs]
+    F.run()
+    sizes = []
+    for j, (a, d) in zip(ids, jobs):
+        mp = gx.Henon(a, 0.3, 1)
+        t = gx.ImplicitBoxTree(bench.HENON_C, bench.HENON_R)
+        t.setFlags(gx.SD, 0)
+        t.subdivision_steps(grid, mp, d)
+        m = t.transitionMatrixForLeaves(grid, mp)
+        mu, it, _ = m.power_iteration(200)
+        assert F.info(j)["state"] == 2 and F.info(j)["iters"] == it == 200
+        assert same_tree(F.cover(j), t.download())
+        for x, y in zip(F.matrix(j), m.download()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(F.measure(j), mu)
+        sizes.append(t.size())
+    print(f"  covers from {min(sizes)} to {max(sizes)} boxes", flush=True)
+    deepest = sorted(range(512), key=lambda q: -jobs[q][1])[:3]
+    for q in deepest:
+        a, d = jobs[q]
+        o = OT(bench.HENON_C, bench.HENON_R)
+        o.setFlags(gx.SD, 0)
+        o.subdivision_steps(grid, gx.Henon(a, 0.3, 1), d)
+        mo = o.transitionMatrixForLeaves(grid, gx.Henon(a, 0.3, 1))
+        muo, ito, _ = mo.power_iteration(200)
+        assert same_tree(F.cover(ids[q]), o.download())
+        for x, y in zip(F.matrix(ids[q]), mo.download()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(F.measure(ids[q]), muo)

@@ -198,3 +198,52 @@ def test_persistent_and_batched_pool_forms_agree(monkeypatch):
     ref = oracle_job(HENON_C, HENON_R, grid, gx.Henon(a, 0.3, 1), d, 100000, 1e-12)
     assert A.info(j)["iters"] == ref[3]
     assert np.array_equal(A.measure(j), ref[2])
+
+
+def test_config5_served_jobs_at_the_configuration_depths():
+    """Configuration 5's jobs at its own depths, served from a Poisson stream at 200 jobs/s:
+    Lorenz (rho ~ U[24, 32], InnerGrid(3)) at depths 18, 20 and 22, forced Duffing (period map in
+    100 RK4 steps, InnerGrid(4)) at depths 16 and 19, measures to 1e-12. Each served result equals
+    the same job run alone bit for bit, and the Lorenz jobs equal the oracle's (cover, matrix,
+    measure and iteration count)."""
+    print("[serve] configuration 5 depths", flush=True)
+    rng = np.random.default_rng(5)
+    lor_c, lor_r, duf_c, duf_r = [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], [0.0, 0.0], [2.0, 2.0]
+    reqs = [(0, 18), (1, 16), (0, 20), (1, 19), (0, 22)]
+    forest_of, centers, radii, params, depths = [], [], [], [], []
+    for q, d in reqs:
+        forest_of.append(q)
+        centers.append(lor_c if q == 0 else duf_c)
+        radii.append(lor_r if q == 0 else duf_r)
+        params.append(gx.Lorenz(rho=float(rng.uniform(24, 32))).params if q == 0 else gx.Duffing().params)
+        depths.append(d)
+    arrival = np.cumsum(rng.exponential(1 / 200.0, len(reqs)))
+
+    def forests():
+        return [gx.Forest(3, gx.InnerGrid(3), 2, pi_max_iter=1_000_000, pi_tol=1e-12),
+                gx.Forest(2, gx.InnerGrid(4), 4, pi_max_iter=1_000_000, pi_tol=1e-12)]
+
+    fs = forests()
+    lat, comp, ids = gx.serve_run(fs, arrival, forest_of, centers, radii, params, depths)
+    assert np.all(lat > 0) and np.all(comp >= arrival)
+    for i, (q, d) in enumerate(reqs):
+        F = fs[q]
+        grid = gx.InnerGrid(3) if q == 0 else gx.InnerGrid(4)
+        mp = gx.Lorenz(*params[i]) if q == 0 else gx.Duffing(*params[i])
+        alone = forests()[q]
+        ja = alone.add(centers[i], radii[i], mp, d)
+        alone.run()
+        j = int(ids[i])
+        print(f"  request {i}: {'Lorenz' if q == 0 else 'Duffing'} depth {d}, {F.info(j)['n']} boxes, "
+              f"{F.info(j)['iters']} iterations", flush=True)
+        assert same_tree(alone.cover(ja), F.cover(j))
+        for a, b in zip(alone.matrix(ja), F.matrix(j)):
+            assert np.array_equal(a, b)
+        assert alone.info(ja)["iters"] == F.info(j)["iters"]
+        assert np.array_equal(alone.measure(ja), F.measure(j))
+        if q == 0:
+            cover, mat, mu, it = oracle_job(centers[i], radii[i], grid, mp, d, 1_000_000, 1e-12)
+            assert same_tree(F.cover(j), cover)
+            for a, b in zip(F.matrix(j), mat):
+                assert np.array_equal(a, b)
+            assert F.info(j)["iters"] == it and np.array_equal(F.measure(j), mu)

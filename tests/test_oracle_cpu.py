@@ -780,7 +780,8 @@ def test_subspace_iteration_cycle_walk_closed_form():
             rows.append(i); cols.append(j); vals.append(v)
     o = np.lexsort((cols, rows))
     m = OCoo.from_arrays(np.array(rows)[o], np.array(cols)[o], np.array(vals)[o], n, n)
-    H, _ = m.subspace_iteration(4, 400)
+    H, _, it, _ = m.subspace_iteration(4, 400)
+    assert it == 400
     ev = np.sort(np.linalg.eigvals(H).real)[::-1]
     lam = 0.4 + 0.6 * np.cos(2 * np.pi * np.arange(3) / n)
     assert np.allclose(ev, [lam[0], lam[1], lam[1], lam[2]], rtol=0, atol=1e-10)
@@ -795,12 +796,39 @@ def test_subspace_iteration_matches_numpy_eig():
         A[b * 15:(b + 1) * 15, b * 15:(b + 1) * 15] += 2.0 * rng.random((15, 15))
     P = A / A.sum(axis=1, keepdims=True)
     r, c = np.nonzero(P)
-    H, X = OCoo.from_arrays(r, c, P[r, c], n, n).subspace_iteration(4, 300, want_basis=True)
+    H, X, it, _ = OCoo.from_arrays(r, c, P[r, c], n, n).subspace_iteration(4, 300, want_basis=True)
     ev = np.sort(np.linalg.eigvals(H).real)[::-1]
     w = np.linalg.eigvals(P.T)
     ref = np.sort(w[np.argsort(-np.abs(w))[:4]].real)[::-1]
     assert np.allclose(ev, ref, rtol=0, atol=1e-10)
     assert np.allclose(X @ X.T, np.eye(4), atol=1e-12)
+
+
+def test_subspace_iteration_stopping_rule():
+    """With a tolerance the iteration stops at the first basis update k whose residual
+    rho = ||X P - H^T X||_F / ||H||_F (rows of X the basis vectors) is at most tol: rho recomputed
+    with numpy from the returned basis and Ritz matrix agrees, the fixed-count run to k - 1 has a
+    residual above tol, and the Ritz values are those of numpy's eig to the residual's order."""
+    rng = np.random.default_rng(23)
+    n = 80
+    A = rng.random((n, n)) * (rng.random((n, n)) < 0.2)
+    for b in range(4):
+        A[b * 20:(b + 1) * 20, b * 20:(b + 1) * 20] += 2.0 * rng.random((20, 20))
+    P = A / A.sum(axis=1, keepdims=True)
+    r, c = np.nonzero(P)
+    m = OCoo.from_arrays(r, c, P[r, c], n, n)
+    for tol in (1e-6, 1e-10):
+        H, X, k, rho = m.subspace_iteration(4, 10000, want_basis=True, tol=tol)
+        assert 0 < k < 10000 and rho <= tol
+        R = X @ P - H.T @ X
+        assert abs(np.linalg.norm(R) / np.linalg.norm(H) - rho) <= 1e-13
+        _, _, k1, rho1 = m.subspace_iteration(4, k - 1)
+        assert k1 == k - 1 and rho1 > tol
+        _, _, k2, rho2 = m.subspace_iteration(4, k)
+        assert k2 == k and rho2 == rho
+        w = np.linalg.eigvals(P.T)
+        ref = np.sort(w[np.argsort(-np.abs(w))[:4]].real)[::-1]
+        assert np.allclose(np.sort(np.linalg.eigvals(H).real)[::-1], ref, rtol=0, atol=100 * tol)
 
 
 # ------------------------------------------------------------------ HIP library C ABI
