@@ -102,7 +102,7 @@ SIGNATURES = {
     "b12x_forest_footprint": [_P, _P],
     "b12x_forest_stats": [_P, _P, ctypes.c_int],
     "b12x_forest_set_pool_iterations": [_P, ctypes.c_int],
-    "b12x_serve_run": [_P, _I, _U32, _P, _P, _P, _P, _I, _P, _I, _P, _D, _D, _P, _P, _P],
+    "b12x_serve_run": [_P, _I, _U32, _P, _P, _P, _P, _I, _P, _I, _P, _D, _D, _P, _P, _P, _P, _P],
 }
 
 _lib = None
@@ -987,7 +987,9 @@ class Forest:
 
 def serve_run(forests, arrival, forest_of, centers, radii, map_params, depths, budget_bytes=200e9,
               reserve_per_job=64e6):
-    """Continuous-batching request loop (b12x_serve_run); returns (latency, completion, job_ids)."""
+    """Continuous-batching request loop (b12x_serve_run); returns (latency, completion, job_ids,
+    info) with info = {admitted: per-request admission times, peak_footprint: bytes,
+    max_in_flight, deferred: times the budget held a waiting request back}."""
     api = forests[0].api
     n = len(arrival)
     arr = (_P * len(forests))(*[f.h for f in forests])
@@ -1007,6 +1009,10 @@ def serve_run(forests, arrival, forest_of, centers, radii, map_params, depths, b
     lat = np.empty(n, np.float64)
     comp = np.empty(n, np.float64)
     ids = np.empty(n, np.uint32)
+    adm = np.empty(n, np.float64)
+    st = np.zeros(3, np.float64)
     api("b12x_serve_run", arr, len(forests), n, _ptr(arrival), _ptr(forest_of), _ptr(C), _ptr(R), stride, _ptr(MP),
-        max(nmp, 1), _ptr(depths), float(budget_bytes), float(reserve_per_job), _ptr(lat), _ptr(comp), _ptr(ids))
-    return lat, comp, ids
+        max(nmp, 1), _ptr(depths), float(budget_bytes), float(reserve_per_job), _ptr(lat), _ptr(comp), _ptr(ids),
+        _ptr(adm), _ptr(st))
+    return lat, comp, ids, {"admitted": adm, "peak_footprint": float(st[0]), "max_in_flight": int(st[1]),
+                            "deferred": int(st[2])}

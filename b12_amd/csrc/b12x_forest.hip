@@ -1072,35 +1072,68 @@ int b12x_forest_set_pool_iterations(b12x_forest f, int iters) {
 // go to forest forestOf[i]; a request is admitted once it has arrived and the forests'
 // footprint plus reservePerJob stays within budgetBytes. Every pass runs one merged step of
 // each forest with work; completion time = end of the step in which the job's measure converged.
+// Admission's cost model of one job (bytes): per leaf the two leaf buffers of the forest with their
+// doubling slack (2 x 4 x 14 B), the hit byte, the result arena's cover and measure (18 B), the
+// pool's column with its slack (64 B) and the transition matrix's transients (4 B per test point
+// and 12 B per leaf); per matrix entry the arena (16 B), the pool with slack (10 B) and the
+// transients of the batched matrix (48 B).
+static double jobCostBytes(uint64_t leaves, uint64_t nnz, uint64_t k) {
+  return double(leaves) * double(208 + 4 * k) + double(nnz) * 74.0;
+}
+
 int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const double* arrival, const int* forestOf,
                    const double* centers, const double* radii, int stride, const double* mapParams, int nmp,
                    const uint8_t* depths, double budgetBytes, double reservePerJob, double* latency,
-                   double* completion, uint32_t* jobIds) {
+                   double* completion, uint32_t* jobIds, double* admittedAt, double* stats) {
   return guarded([&]() -> int {
     if (!forests || nForests <= 0 ||
         (nReq && (!arrival || !forestOf || !centers || !radii || !depths || !latency || !completion)))
       return setErr(B12X_E_ARG, "null argument");
-    for (uint32_t i = 0; i < nReq; ++i) completion[i] = latency[i] = -1.0;
+    for (uint32_t i = 0; i < nReq; ++i) {
+      completion[i] = latency[i] = -1.0;
+      if (admittedAt) admittedAt[i] = -1.0;
+    }
     for (uint32_t i = 1; i < nReq; ++i)
       if (arrival[i] < arrival[i - 1]) return setErr(B12X_E_ARG, "requests must be sorted by arrival");
-    std::vector<int> reqOfJob;
+    for (uint32_t i = 0; i < nReq; ++i)
+      if (forestOf[i] < 0 || forestOf[i] >= nForests) return setErr(B12X_E_ARG, "bad forest index");
     std::vector<std::vector<int>> reqOf(nForests);
-    std::vector<double> admitted(nReq, -1.0);
-    uint32_t next = 0, completed = 0, inFlight = 0;
+    // learned peak cost per (forest, target depth) of finished jobs
+    std::vector<std::vector<double>> learned(nForests, std::vector<double>(256, -1.0));
+    auto project = [&](int q, int d) {
+      if (learned[q][d] >= 0.0) return learned[q][d];
+      for (int e = d - 1; e >= 0; --e)  // the deepest smaller depth, doubling per level
+        if (learned[q][e] >= 0.0) return learned[q][e] * std::ldexp(1.0, d - e);
+      return reservePerJob;
+    };
+    std::vector<double> projOf(nReq, 0.0);
+    double projInFlight = 0.0, peakUsed = 0.0;
+    uint32_t next = 0, completed = 0, inFlight = 0, maxInFlight = 0, deferred = 0;
+    auto footprint = [&]() {
+      uint64_t used = 0;
+      for (int q = 0; q < nForests; ++q) {
+        uint64_t b = 0;
+        b12x_forest_footprint(forests[q], &b);
+        used += b;
+      }
+      return double(used);
+    };
     const double t0 = nowSeconds();
     while (completed < nReq) {
       double now = nowSeconds() - t0;
-      // admission by arrival and HBM footprint
+      // admission by arrival and projected HBM: the forests' bytes now (unreleased results
+      // included) plus the projected peak of every job in flight and of the new one
+      bool blocked = false;
       while (next < nReq && arrival[next] <= now) {
-        uint64_t used = 0;
-        for (int q = 0; q < nForests; ++q) {
-          uint64_t b = 0;
-          b12x_forest_footprint(forests[q], &b);
-          used += b;
-        }
-        if (inFlight > 0 && double(used) + reservePerJob * (inFlight + 1) > budgetBytes) break;
         const int q = forestOf[next];
-        if (q < 0 || q >= nForests) return setErr(B12X_E_ARG, "bad forest index");
+        const double pj = project(q, depths[next]);
+        const double used = footprint();
+        peakUsed = std::max(peakUsed, used);
+        if (inFlight > 0 && used + projInFlight + pj > budgetBytes) {
+          if (!blocked) ++deferred;
+          blocked = true;
+          break;
+        }
         uint32_t id = 0;
         int rc = b12x_forest_add(forests[q], centers + size_t(next) * stride, radii + size_t(next) * stride,
                                  mapParams ? mapParams + size_t(next) * nmp : nullptr, mapParams ? nmp : 0,
@@ -1109,9 +1142,12 @@ int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const doub
         if (reqOf[q].size() <= id) reqOf[q].resize(id + 1, -1);
         reqOf[q][id] = int(next);
         if (jobIds) jobIds[next] = id;
-        admitted[next] = now;
+        if (admittedAt) admittedAt[next] = now;
+        projOf[next] = pj;
+        projInFlight += pj;
         ++next;
         ++inFlight;
+        maxInFlight = std::max(maxInFlight, inFlight);
       }
       bool work = false;
       for (int q = 0; q < nForests; ++q) {
@@ -1120,6 +1156,7 @@ int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const doub
         work = true;
         forestStep(F, nullptr);
       }
+      peakUsed = std::max(peakUsed, footprint());
       const double tEnd = nowSeconds() - t0;
       for (int q = 0; q < nForests; ++q) {
         b12x_forest_s& F = *forests[q];
@@ -1128,6 +1165,10 @@ int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const doub
           if (r < 0 || completion[r] >= 0.0 || (F.jobs[id].state != 2 && F.jobs[id].state != 3)) continue;
           completion[r] = tEnd;
           latency[r] = tEnd - arrival[r];
+          const Job& J = F.jobs[id];
+          const double cost = jobCostBytes(J.nLeaves, J.nnz, F.grid.k);
+          learned[q][depths[r]] = std::max(learned[q][depths[r]], cost);
+          projInFlight -= projOf[r];
           ++completed;
           --inFlight;
         }
@@ -1136,6 +1177,11 @@ int b12x_serve_run(b12x_forest* forests, int nForests, uint32_t nReq, const doub
         const double wait = arrival[next] - (nowSeconds() - t0);
         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
       }
+    }
+    if (stats) {
+      stats[0] = peakUsed;
+      stats[1] = double(maxInFlight);
+      stats[2] = double(deferred);
     }
     return B12X_OK;
   });

@@ -186,11 +186,13 @@ def serve_bench(gx, n_req, rate=200.0, tol=TOL):
                 gx.Forest(2, gx.InnerGrid(4), 4, pi_max_iter=MAX_ITER, pi_tol=tol)]
     args = ([r[0] for r in req], [r[1] for r in req], [r[2] for r in req], [r[3] for r in req], [r[4] for r in req])
     fs = forests()
-    lat, comp, _ = gx.serve_run(fs, arrival, *args)
+    lat, comp, _, info = gx.serve_run(fs, arrival, *args)
     served = {"jobs_per_s": n_req / comp.max(), "p50_s": float(np.percentile(lat, 50)),
-              "p99_s": float(np.percentile(lat, 99)), "makespan_s": float(comp.max())}
+              "p99_s": float(np.percentile(lat, 99)), "makespan_s": float(comp.max()),
+              "peak_footprint_bytes": info["peak_footprint"], "max_in_flight": info["max_in_flight"],
+              "deferred_by_budget": info["deferred"]}
     fs = forests()
-    lat0, comp0, _ = gx.serve_run(fs, np.zeros(n_req), *args)
+    lat0, comp0, _, _ = gx.serve_run(fs, np.zeros(n_req), *args)
     saturated = {"jobs_per_s": n_req / comp0.max(), "p50_s": float(np.percentile(lat0, 50)),
                  "p99_s": float(np.percentile(lat0, 99))}
     # plain: the same jobs one at a time through the single-tree path, in arrival order
@@ -276,12 +278,13 @@ def hh_bench(gx, depth, iters, tol, cpu_depth, threads):
            "subspace_residual": resid, "subspace_converged": bool(resid <= tol),
            "subdiv_s": t1 - t0, "tm_s": t2 - t1, "subspace_s": t3 - t2,
            "images_per_s": imaged / (t3 - t0),
+           "images_per_s_subdivision_and_matrix": imaged / (t2 - t0),
            "image_kernel_tflops": pts * HH_FLOPS_PER_IMAGE / (ms_img * 1e-3) / 1e12 if ms_img > 0 else None,
            "ritz_values": [[float(x.real), float(x.imag)] for x in ev[:4]]}
     if cpu_depth > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
-        os.environ["OMP_NUM_THREADS"] = str(threads)
-        from oracle_api import OracleTree
+        from oracle_api import OracleTree, set_threads
+        set_threads(threads)
         o = OracleTree(HH_C, HH_R)
         o.set_options(eval_twice=True)
         c0 = time.perf_counter()
@@ -316,8 +319,8 @@ def continuation_bench(gx, depth, cpu_depth, threads):
            "boxes_per_s": c[-1] / (t1 - t0), "images_per_s": c[-1] * K / (t1 - t0)}
     if cpu_depth > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
-        os.environ["OMP_NUM_THREADS"] = str(threads)
-        from oracle_api import OracleTree
+        from oracle_api import OracleTree, set_threads
+        set_threads(threads)
         o = OracleTree(MANIFOLD_C, MANIFOLD_R)
         c0 = time.perf_counter()
         co = gx.continuation(o, grid, mp, [cpu_depth], p)[0]
@@ -358,8 +361,8 @@ def chain_recurrent_bench(gx, steps, cpu, threads):
                    "ms": (s1 - s0) * 1e3, "rounds": rounds, "passes": passes, "host_checks": checks}}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
-        os.environ["OMP_NUM_THREADS"] = str(threads)
-        from oracle_api import OracleTree
+        from oracle_api import OracleTree, set_threads
+        set_threads(threads)
         o = OracleTree(MANIFOLD_C, MANIFOLD_R)
         c0 = time.perf_counter()
         r = gx.chainRecurrentSet(o, grid, mp, steps)
@@ -371,13 +374,31 @@ def chain_recurrent_bench(gx, steps, cpu, threads):
     return out
 
 
+def host_cores():
+    """The cores this process may use (the GPU box allots a share per GPU and sets OMP_NUM_THREADS
+    to it), the machine's logical CPUs and the CPU model."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    cores = int(env) if env.isdigit() and int(env) > 0 else aff
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return min(cores, aff), os.cpu_count(), model
+
+
 def cpu_baseline(sample_depth, threads):
     """B12's CPU path restated (oracle): same algorithm structure as the reference's host
-    Thrust path, including its second image evaluation per search, OpenMP over `threads`."""
+    Thrust path, including its second image evaluation per search, OpenMP over `threads` (B12's
+    OMP backend; one thread is its CPP backend)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     import b12_amd as gx
-    from oracle_api import OracleTree
+    from oracle_api import OracleTree, set_threads
+    set_threads(threads)
 
     o = OracleTree(CENTER, RADIUS)
     o.set_options(eval_twice=True)
@@ -401,6 +422,7 @@ def main():
     ap.add_argument("--flavour", default="parity", choices=["parity", "fast"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-depth", type=int, default=24)
+    ap.add_argument("--cpu1-depth", type=int, default=22, help="depth of the one-thread CPU baseline sample")
     ap.add_argument("--sweep-jobs", type=int, default=512, help="config 4 jobs (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="config 5 requests (0 to skip)")
     ap.add_argument("--hh-depth", type=int, default=28, help="config 3 depth (0 to skip)")
@@ -478,26 +500,40 @@ def main():
         "breakdown_ms": {k: runs[-1][k] for k in ("subdiv_ms", "tm_ms", "measure_ms")},
         "job": {k: runs[-1][k] for k in ("images", "boxes", "nnz", "iters", "resid", "level24_points")},
     }
+    def progress(msg):
+        print(f"[bench] {msg} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    progress(f"headline: {value:.4g} images/s, {ms_per_step:.2f} ms per job")
     if args.sweep_jobs > 0:
         out["sweep"] = sweep_bench(gx, args.sweep_jobs)
+        progress("sweep done")
     if args.serve_requests > 0:
         out["serving"] = serve_bench(gx, args.serve_requests)
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = min(threads, 16)
+        progress("serving done")
+    threads, nproc, cpu_model = host_cores()
     if args.hh_depth > 0:
         out["hh"] = hh_bench(gx, args.hh_depth, args.hh_iters, args.hh_tol,
                              0 if args.no_cpu_baseline else args.hh_cpu_depth, threads)
+        progress("config 3 done")
     if args.cont_depth > 0:
         out["continuation"] = continuation_bench(gx, args.cont_depth,
                                                  0 if args.no_cpu_baseline else args.cont_cpu_depth, threads)
+        progress("continuation done")
     if args.chain_steps > 0:
         out["chain_recurrent"] = chain_recurrent_bench(gx, args.chain_steps, not args.no_cpu_baseline, threads)
+        progress("chain recurrent set done")
     if not args.no_cpu_baseline:
         v, secs, imgs, iters = cpu_baseline(args.cpu_depth, threads)
+        v1, secs1, imgs1, iters1 = cpu_baseline(args.cpu1_depth, 1)
         out["cpu_baseline"] = {"value": v, "unit": "images/s", "cores": threads, "kind": "port",
                                "sample": f"same Lorenz job to depth {args.cpu_depth} + transition matrix "
                                          f"and {iters} power iterations ({imgs} images, {secs:.1f} s), oracle "
-                                         f"with the reference's double image evaluation per search"}
+                                         f"with the reference's double image evaluation per search",
+                               "cpu_model": cpu_model, "nproc": nproc,
+                               "single_thread": {"value": v1, "unit": "images/s", "cores": 1,
+                                                 "sample": f"the same job to depth {args.cpu1_depth} ({imgs1} "
+                                                           f"images, {iters1} power iterations, {secs1:.1f} s) on "
+                                                           f"one thread, as B12's CPP backend runs"}}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -333,14 +333,21 @@ int b12x_forest_stats(b12x_forest f, double* out, int n);
 int b12x_forest_set_pool_iterations(b12x_forest f, int iters);
 /* Continuous-batching request loop over n_forests forests: request i arrives arrival[i] seconds
  * after the start (sorted), goes to forests[forest_of[i]] with centre/radius (stride doubles
- * apart), map_params (n_map_params each, may be NULL) and depth; it is admitted once it has
- * arrived and the forests' footprint plus reserve_per_job per job in flight fits budget_bytes.
- * latency[i] = completion[i] - arrival[i] (seconds); job_ids (may be NULL) gets each request's
- * job id in its forest. */
+ * apart), map_params (n_map_params each, may be NULL) and depth. It is admitted once it has
+ * arrived and the forests' footprint now (b12x_forest_footprint: arrays, pool and unreleased
+ * results) plus the projected peak of every job in flight and of itself fits budget_bytes (a job
+ * alone is always admitted). A job's projected peak is the cost of the largest finished job of
+ * its forest and depth (leaves x (208 + 4k) bytes + matrix entries x 74 bytes: leaf buffers with
+ * their growth slack, pool, result arena and the transition matrix's transients), else of the
+ * deepest smaller depth doubled per level, else reserve_per_job. Every pass runs one merged step
+ * of each forest with work. latency[i] = completion[i] - arrival[i] (seconds); job_ids (may be
+ * NULL) gets each request's job id in its forest, admitted (may be NULL) its admission time;
+ * stats (may be NULL) receives the peak footprint seen (bytes), the most jobs in flight at once
+ * and how many times a waiting request was held back by the budget. */
 int b12x_serve_run(b12x_forest* forests, int n_forests, uint32_t n_req, const double* arrival, const int* forest_of,
                    const double* centers, const double* radii, int stride, const double* map_params,
                    int n_map_params, const uint8_t* depths, double budget_bytes, double reserve_per_job,
-                   double* latency, double* completion, uint32_t* job_ids);
+                   double* latency, double* completion, uint32_t* job_ids, double* admitted, double* stats);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,7 @@
 // C entry points over the b12cpu oracle, for tests and for the CPU-baseline leg of bench.py.
 // TEST INFRASTRUCTURE ONLY: the product library never links this file.
 #include <cstdio>
+#include <omp.h>
 #include <memory>
 
 #include "b12cpu.hpp"
@@ -182,6 +183,14 @@ void b12cpu_tree_destroy(void* h) { delete static_cast<TreeBase*>(h); }
   } catch (const std::exception& e) {                                      \
     return fail(e.what());                                                 \
   }
+
+// OpenMP threads of the oracle's parallel loops (the CPU baseline runs with all of the process's
+// cores and with one, as B12's OMP and CPP host backends do); returns the previous count
+int b12cpu_set_threads(int n) {
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+}
 
 int b12cpu_set_options(void* h, int evalTwice, uint64_t chunkPoints) {
   TREE_CALL(h, { T.evalTwice = evalTwice != 0; T.chunkPoints = chunkPoints ? chunkPoints : ~0ull; return 0; });

@@ -30,6 +30,7 @@ def lib():
         _lib.b12cpu_power_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_double, _P, _P, _P]
         _lib.b12cpu_set_options.argtypes = [_P, ctypes.c_int, ctypes.c_uint64]
         _lib.b12cpu_energy_band_stencil.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P]
+        _lib.b12cpu_set_threads.argtypes = [ctypes.c_int]
         _lib.b12cpu_subspace_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P]
     return _lib
 
@@ -135,6 +136,11 @@ class OCoo:
     def __del__(self):
         if getattr(self, "h", None):
             lib().b12cpu_coo_destroy(self.h)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops; returns the previous count."""
+    return lib().b12cpu_set_threads(int(n))
 
 
 def compressIndexVector(indices, dim):

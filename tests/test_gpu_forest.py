@@ -122,7 +122,7 @@ def test_serve_run_results_match_alone_runs():
         else:
             forest_of.append(1); centers.append([0.0, 0.0]); radii.append([2.0, 2.0])
             params.append(gx.Duffing().params); depths.append(int(rng.integers(5, 9)))
-    lat, comp, ids = gx.serve_run([lor, duf], arrival, forest_of, centers, radii, params, depths)
+    lat, comp, ids, _ = gx.serve_run([lor, duf], arrival, forest_of, centers, radii, params, depths)
     assert np.all(lat > 0) and np.all(comp >= arrival)
     for i in range(n):
         F = lor if forest_of[i] == 0 else duf
@@ -224,7 +224,7 @@ def test_config5_served_jobs_at_the_configuration_depths():
                 gx.Forest(2, gx.InnerGrid(4), 4, pi_max_iter=1_000_000, pi_tol=1e-12)]
 
     fs = forests()
-    lat, comp, ids = gx.serve_run(fs, arrival, forest_of, centers, radii, params, depths)
+    lat, comp, ids, _ = gx.serve_run(fs, arrival, forest_of, centers, radii, params, depths)
     assert np.all(lat > 0) and np.all(comp >= arrival)
     for i, (q, d) in enumerate(reqs):
         F = fs[q]
@@ -247,3 +247,49 @@ def test_config5_served_jobs_at_the_configuration_depths():
             for a, b in zip(F.matrix(j), mat):
                 assert np.array_equal(a, b)
             assert F.info(j)["iters"] == it and np.array_equal(F.measure(j), mu)
+
+
+def test_serve_admission_under_a_binding_budget():
+    """Admission by projected HBM: a budget of the forests' base footprint plus 3.2 times one
+    job's projected cost (leaves x (208 + 4k) + entries x 74 bytes, the rule b12x_serve_run
+    documents) admits at most 3 of 9 simultaneous Lorenz requests at once; the others wait and
+    are admitted as earlier jobs complete; the forests' footprint never exceeds the budget; every
+    served result equals the same job run alone, bit for bit."""
+    print("[serve] binding budget", flush=True)
+    c, r, grid, depth = [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), 14
+    rhos = [24.0 + 0.9 * q for q in range(9)]
+
+    def forest():
+        return gx.Forest(3, grid, 2, pi_max_iter=1_000_000, pi_tol=1e-12)
+
+    alone = []
+    cost = 0.0
+    for rho in rhos:
+        A = forest()
+        ja = A.add(c, r, gx.Lorenz(rho=rho), depth)
+        A.run()
+        info = A.info(ja)
+        cost = max(cost, info["n"] * (208 + 4 * 27) + info["nnz"] * 74.0)
+        alone.append((A, ja))
+    F = forest()
+    base = F.footprint()
+    budget = base + 3.2 * cost
+    n = len(rhos)
+    lat, comp, ids, info = gx.serve_run([F], np.zeros(n), [0] * n, [c] * n, [r] * n,
+                                        [gx.Lorenz(rho=rho).params for rho in rhos], [depth] * n,
+                                        budget_bytes=budget, reserve_per_job=cost)
+    adm = info["admitted"]
+    print(f"  cost {cost / 1e6:.2f} MB budget {budget / 1e6:.2f} MB peak {info['peak_footprint'] / 1e6:.2f} MB "
+          f"max in flight {info['max_in_flight']} deferred {info['deferred']}", flush=True)
+    assert 1 <= info["max_in_flight"] <= 3 and info["deferred"] >= 1
+    assert info["peak_footprint"] <= budget
+    # at every admission after the first three, an earlier request had completed
+    order = np.argsort(adm, kind="stable")
+    for q in order[3:]:
+        assert np.any((comp <= adm[q]) & (np.arange(n) != q))
+    for q, (A, ja) in enumerate(alone):
+        j = int(ids[q])
+        assert same_tree(A.cover(ja), F.cover(j))
+        for a, b in zip(A.matrix(ja), F.matrix(j)):
+            assert np.array_equal(a, b)
+        assert A.info(ja)["iters"] == F.info(j)["iters"] and np.array_equal(A.measure(ja), F.measure(j))
