@@ -84,6 +84,7 @@ SIGNATURES = {
     "b12x_coo_make_symmetric": [_P, _I],
     "b12x_power_iteration": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_power_iteration_kernel_ms": [_P],
+    "b12x_power_iteration_form": [_P],
     "b12x_power_iteration_noalloc": [_P, _I, _D, _I, _P, _P, _P],
     "b12x_subspace_iteration": [_P, _I, _I, _D, _P, _P, _P, _P],
     "b12x_coo_scc": [_P, _P, _P, _P],
@@ -163,6 +164,18 @@ def power_iteration_kernel_ms(flavour="parity"):
     v = _D()
     _Api(flavour)("b12x_power_iteration_kernel_ms", ctypes.byref(v))
     return v.value
+
+
+PI_FORMS = {-1: "batched", 0: "straight", 1: "sweep"}
+
+
+def power_iteration_form(flavour="parity"):
+    """Form of this thread's last power iteration: "straight" (the persistent form, one group of
+    64 columns per wave), "sweep" (the persistent form, waves sweeping several groups) or
+    "batched" (the batched form alone)."""
+    v = ctypes.c_int()
+    _Api(flavour)("b12x_power_iteration_form", ctypes.byref(v))
+    return PI_FORMS[v.value]
 
 
 def scc_last_stats(flavour="parity"):

@@ -2588,6 +2588,7 @@ void subspaceRun(b12x_coo_s& mat, uint32_t n, int maxIter, double tol, double* r
 // 0: use the batched form (no columns, or n >= 2^29, beyond the 32-bit byte offsets of the
 // cached entries).
 thread_local double lastPiKernelMs = 0.0;  // duration of the calling thread's last persistent launch
+thread_local int lastPiForm = -1;          // its form: 0 straight-line, 1 sweeping; -1 batched alone
 
 uint32_t piFlowWorkers(uint32_t n) {
   if (n == 0 || n >= (1u << 29)) return 0;
@@ -2632,6 +2633,7 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
              int maxIter, double tol, int muInit, double* mu, int* iters, double* resid, const PiWork& w,
              uint32_t workers, bool useGraph, uint32_t spinLimit, bool timed) {
   lastPiKernelMs = 0.0;
+  lastPiForm = -1;
   const uint32_t nb = std::max<uint32_t>(1, ceilDiv(n, kBlock));
   PiState init{};
   init.t = 1;
@@ -2659,6 +2661,7 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
     HIPCHK(hipMemcpyAsync(&maxLen, w.maxLen, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const bool full = maxLen > uint32_t(kCoopCache) || uint64_t(nG) > 4ull * workers;
+    lastPiForm = full ? 1 : 0;
     PiFlowArgs fa{colPtr, rows, vals, w.mu0, w.ring, w.part, w.tot, w.pst, w.err, n, nG, workers, mi, tol, spinLimit};
     hipEvent_t e0 = nullptr, e1 = nullptr;  // the launch's duration, for measurement
     if (timed) {
@@ -2687,6 +2690,7 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
       done = true;
     } else {
       lastPiKernelMs = 0.0;
+      lastPiForm = -1;
       HIPCHK(hipMemcpyAsync(w.pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
     }
   }
@@ -2817,6 +2821,12 @@ extern "C" {
 int b12x_power_iteration_kernel_ms(double* ms) {
   if (!ms) return setErr(B12X_E_ARG, "null argument");
   *ms = lastPiKernelMs;
+  return B12X_OK;
+}
+
+int b12x_power_iteration_form(int* form) {
+  if (!form) return setErr(B12X_E_ARG, "null argument");
+  *form = lastPiForm;
   return B12X_OK;
 }
 

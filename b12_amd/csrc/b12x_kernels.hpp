@@ -1510,6 +1510,22 @@ __device__ inline bool flowComplete(const uint64_t* src, const uint32_t (&off)[U
 
 __device__ inline void flowAbort(const PiFlowArgs& a) { stAgent(a.err, 1); }
 
+// sum over u of z_u v_u for the lane's cached entries in ascending order, times c when c != 1:
+// the groups of kCoopUnroll slots beyond every lane's column are skipped (wave-uniform), the
+// others added whole (a slot past the column adds -0.0, which leaves the sum as it is)
+__device__ inline double flowCachedSum(const uint64_t (&m)[kCoopCache], const double (&cv)[kCoopCache][kBlock],
+                                       uint32_t cnt, double c) {
+  double s = 0.0;
+#pragma unroll
+  for (int g = 0; g < kCoopCache; g += kCoopUnroll) {
+    if (__any(uint32_t(g) < cnt)) {
+#pragma unroll
+      for (int u = g; u < g + kCoopUnroll; ++u) s += asF64(m[u]) * cv[u][threadIdx.x];
+    }
+  }
+  return c != 1.0 ? s * c : s;
+}
+
 __device__ inline void piFlowAggregator(const PiFlowArgs& a) {
   __shared__ double nuRing[kFlowTot];
   const uint32_t lane = threadIdx.x;
@@ -1640,8 +1656,10 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2
   if (wave >= nG) return;
   const uint32_t copy = wave % kFlowCopies;
   // the first group's column keeps its first kCoopCache entries for the whole run: row byte
-  // offsets in registers, values in LDS (each lane reads only its own slots, so no barrier);
-  // its own z history (z_{t-1}, ..., z_{t-1-kFlowLag}) stays in registers
+  // offsets in registers, values in LDS (each lane reads only its own slots, so no barrier),
+  // -0.0 past the column's end: the product of such a slot (its entry is never loaded: +0.0) is
+  // -0.0, and x + (-0.0) == x for every x, so the slots past the end add nothing and no select
+  // is needed; its own z history (z_{t-1}, ..., z_{t-1-kFlowLag}) stays in registers
   __shared__ double cvL[kCoopCache][kBlock];
   const uint32_t j0 = 64 * wave + lane;
   const bool valid0 = j0 < n;
@@ -1654,14 +1672,14 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2
 #pragma unroll
   for (int u = 0; u < kCoopCache; ++u) {
     ci[u] = 0;
-    cvL[u][threadIdx.x] = 0.0;
+    cvL[u][threadIdx.x] = -0.0;
   }
   if (c1 > c0) {  // indices clamped into the column
 #pragma unroll
     for (int u = 0; u < kCoopCache; ++u) {
       const uint32_t eu = min(c0 + u, c1 - 1);
       ci[u] = a.rowIdx[eu] * uint32_t(sizeof(double));  // n < 2^29
-      cvL[u][threadIdx.x] = a.val[eu];
+      if (c0 + u < c1) cvL[u][threadIdx.x] = a.val[eu];
     }
   }
   const uint32_t cnt0 = min(c1 - c0, uint32_t(kCoopCache));
@@ -1713,13 +1731,7 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2
         flowAbort(a);
         return;
       }
-      double s = 0.0;
-#pragma unroll
-      for (int u = 0; u < kCoopCache; ++u) {
-        const double prod = asF64(m0[u]) * cvL[u][threadIdx.x];
-        s = (uint32_t(u) < cnt0) ? s + prod : s;
-      }
-      if (c != 1.0) s *= c;
+      const double s = flowCachedSum(m0, cvL, cnt0, c);
       __builtin_amdgcn_s_waitcnt(0);  // earlier stores (clears included) have completed
       if (valid0) {
         stAgent(a.z + uint64_t(t % kFlowRing) * n + j0, asU64(s));
@@ -1766,11 +1778,7 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2
             flowAbort(a);
             return;
           }
-#pragma unroll
-          for (int u = 0; u < kCoopCache; ++u) {
-            const double prod = asF64(m0[u]) * cvL[u][threadIdx.x];
-            s = (uint32_t(u) < cnt0) ? s + prod : s;
-          }
+          s = flowCachedSum(m0, cvL, cnt0, 1.0);
           e = c0 + kCoopCache;
           e1 = c1;
         } else {

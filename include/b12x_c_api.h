@@ -271,6 +271,10 @@ int b12x_power_iteration_noalloc(b12x_coo m, int max_iter, double tol, int mu_in
  * persistent power-iteration launch, from HIP events on the matrix's stream (0 when the last
  * call ran the batched form). */
 int b12x_power_iteration_kernel_ms(double* ms);
+/* Measurement (no counterpart in the reference): the form of the calling thread's last persistent
+ * launch: 0 straight-line (one group of 64 columns per wave, columns of at most 24 entries), 1
+ * waves sweeping several groups; -1 when the last call ran the batched form alone. */
+int b12x_power_iteration_form(int* form);
 /* Subspace iteration for the nvec (<= 8) leading eigenvalues (no counterpart in the reference,
  * which leaves eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59): X_0 = cholQR2([1/n, splitmix64
  * columns]); for k = 0, 1, ...: Y = X_k P (row-vector convention, as the power iteration), the Ritz

@@ -42,7 +42,8 @@ def test_henon_depth20_config_matches_oracle():
 
 def test_lorenz_depth24_config_matches_oracle():
     """24 subdivision steps of Lorenz(10, 28, 8/3; RK4 20 x 0.01) with InnerGrid(3), the
-    column-major matrix and power iteration until ||mu_{k+1} - mu_k||_1 <= 1e-12."""
+    column-major matrix and power iteration until ||mu_{k+1} - mu_k||_1 <= 1e-12 (the persistent
+    straight-line form)."""
     print("[config] lorenz d24", flush=True)
     c, r, grid, mp = [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz()
     g, cg = full_job(c, r, grid, mp, 24)
@@ -56,6 +57,7 @@ def test_lorenz_depth24_config_matches_oracle():
     for a, b in zip(mg.download(), mo.download()):
         assert np.array_equal(a, b)
     mug, itg, rg = mg.power_iteration(1_000_000, 1e-12)
+    assert gx.power_iteration_form() == "straight"
     muo, ito, ro = mo.power_iteration(1_000_000, 1e-12)
     print(f"  boxes {g.size()} nnz {mg.nnz} iterations gpu {itg} oracle {ito}", flush=True)
     assert itg == ito and rg == ro and rg <= 1e-12

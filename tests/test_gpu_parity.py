@@ -341,7 +341,22 @@ def _pi_cases():
     vals2 = vals / (1024.0 * np.bincount(rows, weights=vals, minlength=n)[rows])
     cases.append(("decaying", gx.CooMatrix.from_arrays(rows, cols, vals2, n, n),
                   OCoo.from_arrays(rows, cols, vals2, n, n)))
+    # 20 uniformly spread rows per column, near the 24 entries a lane keeps
+    n = 20_011
+    cols = np.repeat(np.arange(n, dtype=np.uint32), 20)
+    rows = np.concatenate([rng.choice(n, 20, replace=False) for _ in range(n)]).astype(np.uint32)
+    vals = rng.random(len(rows)) + 0.01
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    vals = vals / np.bincount(rows, weights=vals, minlength=n)[rows]
+    cases.append(("spread", gx.CooMatrix.from_arrays(rows, cols, vals, n, n),
+                  OCoo.from_arrays(rows, cols, vals, n, n)))
     return cases
+
+
+# the persistent form of each case
+PI_FORMS = {"henon13": "straight", "random300k": "sweep", "long-columns": "sweep", "decaying": "sweep",
+            "spread": "straight"}
 
 
 def test_persistent_power_iteration_matches_batched_form_and_oracle(monkeypatch):
@@ -355,9 +370,11 @@ def test_persistent_power_iteration_matches_batched_form_and_oracle(monkeypatch)
             monkeypatch.setenv("B12X_PI_PERSISTENT", "2")  # persistent form or an error, no fallback
             a, ia, ra = mg.power_iteration(mi, tol)
             assert gx.power_iteration_kernel_ms() > 0.0
+            assert gx.power_iteration_form() == PI_FORMS[name], name
             a2, ia2, ra2 = mg.power_iteration(mi, tol)
             monkeypatch.setenv("B12X_PI_PERSISTENT", "0")
             b, ib, rb = mg.power_iteration(mi, tol)
+            assert gx.power_iteration_form() == "batched"
             monkeypatch.delenv("B12X_PI_PERSISTENT")
             c, ic, rc = mo.power_iteration(mi, tol)
             assert ia == ia2 == ib == ic and ra == ra2 == rb == rc, (name, mi, ia, ib, ic)
