@@ -6,9 +6,9 @@ binds its C ABI (include/b12x_c_api.h) for Python callers, tests and the benchma
 from .api import (Forest, serve_run, continuation, chainRecurrentSet, fixedBoxes, subdivisionByNim, ALL, EQU, EXPD, EXTR, ENTR, HIT, INS, INVALID, LEAF, NONE, SD, B12xError, CooMatrix, Duffing,
                   Henon, HenonHeiles, Identity, ImplicitBoxTree, InnerGrid, Lorenz, PointIterator, Scheme,
                   UniformGrid, bench_copy, compressIndexVector, device_name, library_path, load,
-                  power_iteration_form, power_iteration_kernel_ms)
+                  power_iteration_form, power_iteration_kernel_ms, tm_last_passes)
 
 __all__ = ["Forest", "serve_run", "continuation", "chainRecurrentSet", "fixedBoxes", "subdivisionByNim", "ALL", "EQU", "EXPD", "EXTR", "ENTR", "HIT", "INS", "INVALID", "LEAF", "NONE", "SD", "B12xError",
            "CooMatrix", "Duffing", "Henon", "HenonHeiles", "Identity", "ImplicitBoxTree", "InnerGrid", "Lorenz",
            "PointIterator", "Scheme", "UniformGrid", "bench_copy", "compressIndexVector", "device_name", "library_path", "load",
-           "power_iteration_form", "power_iteration_kernel_ms"]
+           "power_iteration_form", "power_iteration_kernel_ms", "tm_last_passes"]

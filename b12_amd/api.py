@@ -67,6 +67,10 @@ SIGNATURES = {
     "b12x_search_by_points": [_P, _P, _U64, _U8, _P],
     "b12x_subdivision_steps": [_P, _P, _U8, _I, _P],
     "b12x_tree_image_timing": [_P, _I, _P, _P, _P],
+    "b12x_tree_image_launches": [_P, _P, _P, _U64, _P],
+    "b12x_set_tm_pass_bytes": [_P, _U64],
+    "b12x_get_tm_pass_bytes": [_P, _P],
+    "b12x_tm_last_passes": [_P],
     "b12x_bench_copy": [ctypes.c_uint64, _I, _P],
     "b12x_scheme_create": [_I, _I, _P, _I, _I, _P, _I, _P],
     "b12x_scheme_destroy": [_P],
@@ -86,7 +90,7 @@ SIGNATURES = {
     "b12x_power_iteration_kernel_ms": [_P],
     "b12x_power_iteration_form": [_P],
     "b12x_power_iteration_noalloc": [_P, _I, _D, _I, _P, _P, _P],
-    "b12x_subspace_iteration": [_P, _I, _I, _D, _P, _P, _P, _P],
+    "b12x_subspace_iteration": [_P, _I, _I, _I, _D, _P, _P, _P, _P],
     "b12x_coo_scc": [_P, _P, _P, _P],
     "b12x_scc_last_stats": [_P, _P, _P],
     "b12x_chain_recurrent_steps": [_P, _P, _U8, _I, _P],
@@ -157,6 +161,13 @@ def bench_copy(nbytes, reps=20, flavour="parity"):
     ms = _D()
     _Api(flavour)("b12x_bench_copy", int(nbytes), int(reps), ctypes.byref(ms))
     return ms.value
+
+
+def tm_last_passes(flavour="parity"):
+    """Passes of this thread's last transitionMatrix."""
+    v = _U32()
+    _Api(flavour)("b12x_tm_last_passes", ctypes.byref(v))
+    return v.value
 
 
 def power_iteration_kernel_ms(flavour="parity"):
@@ -331,15 +342,16 @@ class CooMatrix:
                  _ptr(mu_arr), ctypes.byref(it), ctypes.byref(res))
         return mu_arr, it.value, res.value
 
-    def subspace_iteration(self, m=4, iters=100, want_basis=False, tol=-1.0):
+    def subspace_iteration(self, m=4, iters=100, want_basis=False, tol=-1.0, lead=None):
         """(Ritz matrix H (m x m), basis X (m x n) or None, basis updates, residual) of
-        b12x_subspace_iteration: at most `iters` updates, stopping once ||X P - X H||_F / ||H||_F
-        is at most tol (tol < 0: exactly `iters` updates); the eigenvalues of H approximate the m
-        leading eigenvalues of the matrix."""
+        b12x_subspace_iteration: at most `iters` updates, stopping once the residual of the
+        leading `lead` basis columns (all m by default), ||X_L P - X_L H_LL||_F / ||H_LL||_F, is at
+        most tol (tol < 0: exactly `iters` updates); the eigenvalues of H approximate the m leading
+        eigenvalues of the matrix."""
         H = np.zeros((m, m), np.float64)
         X = np.zeros((m, self.nRows), np.float64) if want_basis else None
         it, res = _I(), _D()
-        self.api("b12x_subspace_iteration", self.h, int(m), int(iters), float(tol), _ptr(H),
+        self.api("b12x_subspace_iteration", self.h, int(m), int(lead or m), int(iters), float(tol), _ptr(H),
                  _ptr(X) if want_basis else None, ctypes.byref(it), ctypes.byref(res))
         return H, X, it.value, res.value
 
@@ -542,6 +554,15 @@ class ImplicitBoxTree:
                  ctypes.byref(npt))
         return ms.value, nl.value, npt.value
 
+    def image_launches(self):
+        """[(ms, points)] of each image launch timed since timing was last enabled, in order."""
+        n = _U64()
+        self.api("b12x_tree_image_launches", self.h, None, None, 0, ctypes.byref(n))
+        ms = np.zeros(n.value, np.float64)
+        pts = np.zeros(n.value, np.uint64)
+        self.api("b12x_tree_image_launches", self.h, _ptr(ms), _ptr(pts), n.value, ctypes.byref(n))
+        return list(zip(ms.tolist(), pts.tolist()))
+
     # -- structural operations
     def subdivide(self, flags=SD):
         return self._ret("b12x_subdivide", flags)
@@ -682,6 +703,15 @@ class ImplicitBoxTree:
     def setNHostBufferBytes(self, n):
         self.api("b12x_set_host_buffer_bytes", self.h, int(n))
         return self.getNHostBufferBytes()
+
+    def set_tm_pass_bytes(self, n):
+        """Per-point scratch bound of a transitionMatrix pass (0: a third of the free HBM)."""
+        self.api("b12x_set_tm_pass_bytes", self.h, int(n))
+
+    def get_tm_pass_bytes(self):
+        r = _U64()
+        self.api("b12x_get_tm_pass_bytes", self.h, ctypes.byref(r))
+        return r.value
 
     def getDepthVector(self):
         return self.download()[1]

@@ -65,6 +65,8 @@ struct b12x_tree_s {
   std::vector<hipEvent_t> eventPool;
   double imageMs = 0.0;
   uint64_t imageLaunches = 0, imagePoints = 0;
+  std::vector<double> launchMs;          // each timed image launch's duration, in launch order
+  std::vector<uint64_t> launchPoints;    // and its test points
   uint8_t* rootDepth = nullptr;
 
   // the subdivision step (subdivide, images, keep) captured as a graph, in a small form (one-block
@@ -89,6 +91,7 @@ struct b12x_tree_s {
   uint64_t leafGen = 1, rangeGen = 0;
   uint8_t minDepthC = 0, maxDepthC = 0;
   uint64_t hostBufferBytes = uint64_t(4) << 30;  // nHostBufferBytes_ (ImplicitBoxTree.hpp:38)
+  uint64_t tmPassBytes = 0;  // scratch bound of a transition-matrix pass (0: a third of the free HBM)
 
   TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
   ~b12x_tree_s() {
@@ -729,8 +732,11 @@ void stageLeafCount(b12x_tree_s& T) {
 
 // Image of every point of `src` located on the leaves (or on the root when depth0).
 // mode 0 sets hit bytes, mode 1 writes targets to out.
+// images of the source's test points [pointBase, pointBase + nPoints) (all of them by default);
+// out receives them from index 0
 void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, int mode, uint32_t* out,
-              const DepthMap* dm = nullptr, uint64_t* outKey = nullptr) {
+              const DepthMap* dm = nullptr, uint64_t* outKey = nullptr, uint64_t pointBase = 0,
+              uint64_t nPoints = ~uint64_t(0)) {
   ImageArgs a{};
   a.outKey = outKey;
   if (dm) {  // search among the boxes at depth dm->d
@@ -747,8 +753,8 @@ void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, 
   a.srcKey = src.key;
   a.srcDepth = src.depth;
   a.srcList = src.list;
-  a.nPoints = src.nBoxes * S.grid.k;
-  a.pointBase = 0;
+  a.nPoints = nPoints == ~uint64_t(0) ? src.nBoxes * S.grid.k : nPoints;
+  a.pointBase = pointBase;
   a.mode = mode;
   a.depth0 = depth0 ? 1 : 0;
   a.hit = T.hit;
@@ -773,16 +779,67 @@ void runImage(b12x_tree_s& T, b12x_scheme_s& S, const Source& src, bool depth0, 
     T.pending.push_back({e0, e1});
     T.imageLaunches += 1;
     T.imagePoints += a.nPoints;
+    T.launchPoints.push_back(a.nPoints);
   }
 }
 
 // Fold completed image timings into the totals (called after the stream has synchronised).
+// A device uint32 array that grows by doubling, keeping its contents: the outputs of passes
+// whose sizes become known one pass at a time.
+struct GrowBuf {
+  uint32_t* p = nullptr;
+  uint64_t cap = 0;
+  GrowBuf() = default;
+  GrowBuf(const GrowBuf&) = delete;
+  GrowBuf& operator=(const GrowBuf&) = delete;
+  ~GrowBuf() { dfree(p); }
+  void reserve(uint64_t need, uint64_t used, hipStream_t st) {
+    if (need <= cap) return;
+    const uint64_t nc = std::max(need, 2 * cap);
+    uint32_t* q = dalloc<uint32_t>(std::max<uint64_t>(nc, 1));
+    if (used) HIPCHK(hipMemcpyAsync(q, p, used * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(p);
+    p = q;
+    cap = nc;
+  }
+  // to exactly `used` entries
+  void shrink(uint64_t used, hipStream_t st) {
+    if (cap <= used) return;
+    uint32_t* q = dalloc<uint32_t>(std::max<uint64_t>(used, 1));
+    if (used) HIPCHK(hipMemcpyAsync(q, p, used * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(p);
+    p = q;
+    cap = used;
+  }
+  uint32_t* release() {
+    uint32_t* q = p;
+    p = nullptr;
+    cap = 0;
+    return q;
+  }
+};
+
+// Scratch bound of one transition-matrix pass: the tree's setting, else a third of the device's
+// free memory (the reference sizes its first pass at a third of Memory<DEVICE>::getFreeBytes,
+// ThrustSystem.hpp:64-68, ImplicitBoxTree.hpp:2941-2944)
+uint64_t tmChunkBudget(const b12x_tree_s& T) {
+  if (T.tmPassBytes) return T.tmPassBytes;
+  size_t freeB = 0, totalB = 0;
+  HIPCHK(hipMemGetInfo(&freeB, &totalB));
+  return uint64_t(freeB) / 3;
+}
+
+thread_local uint32_t lastTmPasses = 0;  // passes of the calling thread's last transition matrix
+
 void collectTimings(b12x_tree_s& T) {
   for (auto& p : T.pending) {
     HIPCHK(hipEventSynchronize(p.second));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, p.first, p.second));
     T.imageMs += ms;
+    T.launchMs.push_back(ms);
     T.eventPool.push_back(p.first);
     T.eventPool.push_back(p.second);
   }
@@ -1031,6 +1088,24 @@ int b12x_get_host_buffer_bytes(b12x_tree t, uint64_t* out) {
 int b12x_set_host_buffer_bytes(b12x_tree t, uint64_t v) {
   if (!t) return setErr(B12X_E_ARG, "null tree");
   t->hostBufferBytes = v;
+  return B12X_OK;
+}
+
+int b12x_set_tm_pass_bytes(b12x_tree t, uint64_t bytes) {
+  if (!t) return setErr(B12X_E_ARG, "null tree");
+  t->tmPassBytes = bytes;
+  return B12X_OK;
+}
+
+int b12x_get_tm_pass_bytes(b12x_tree t, uint64_t* bytes) {
+  if (!t || !bytes) return setErr(B12X_E_ARG, "null argument");
+  *bytes = t->tmPassBytes;
+  return B12X_OK;
+}
+
+int b12x_tm_last_passes(uint32_t* passes) {
+  if (!passes) return setErr(B12X_E_ARG, "null argument");
+  *passes = lastTmPasses;
   return B12X_OK;
 }
 
@@ -1856,7 +1931,10 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
         enqueueStep(T, *s, sd, n, nullptr, STEP_EAGER);
         syncScal(T, 4);
         const uint32_t nSub = T.hscal[0], cSub = T.hscal[1], kept = T.hscal[2], cSet = T.hscal[3];
-        if (T.timing) T.imagePoints += uint64_t(nSub) * s->grid.k;
+        if (T.timing) {
+          T.imagePoints += uint64_t(nSub) * s->grid.k;
+          T.launchPoints.push_back(uint64_t(nSub) * s->grid.k);
+        }
         T.n = kept;
         ++T.leafGen;
         insertRootIfEmpty(T);
@@ -1886,8 +1964,25 @@ int b12x_tree_image_timing(b12x_tree t, int enable, double* ms, uint64_t* launch
     if (enable) {
       t->imageMs = 0.0;
       t->imageLaunches = t->imagePoints = 0;
+      t->launchMs.clear();
+      t->launchPoints.clear();
     }
     t->timing = enable != 0;
+    return B12X_OK;
+  });
+}
+
+int b12x_tree_image_launches(b12x_tree t, double* ms, uint64_t* points, uint64_t cap, uint64_t* n) {
+  return guarded([&]() -> int {
+    if (!t || !n) return setErr(B12X_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(t->stream));
+    collectTimings(*t);
+    const uint64_t k = std::min<uint64_t>(t->launchMs.size(), t->launchPoints.size());
+    *n = k;
+    for (uint64_t i = 0; i < std::min(k, cap); ++i) {
+      if (ms) ms[i] = t->launchMs[i];
+      if (points) points[i] = t->launchPoints[i];
+    }
     return B12X_OK;
   });
 }
@@ -1988,8 +2083,6 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
     const DepthClass dc = classifyDepth(T, d);
     const Source src = makeSource(T, dIt, check);
     const uint32_t k = s->grid.k;
-    const uint64_t P = src.nBoxes * k;
-    if (P > 0xFFFFFFFFull) return setErr(B12X_E_UNSUPPORTED, "more than 2^32 test points in one matrix");
     DepthMap DM;
     if (dc == DC_GENERAL) buildDepthMap(T, d, DM);
     const uint32_t nBoxes =  // count(depth)
@@ -2000,79 +2093,98 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
       M->nRows = M->nCols = nBoxes;
       M->rowMajor = rowMajor != 0;
       M->colMajor = rowMajor == 0;
-      uint32_t* col = dalloc<uint32_t>(P);
-      if (dc != DC_BEYOND) runImage(T, *s, src, dc == DC_ROOT, 1, col, dc == DC_GENERAL ? &DM : nullptr);
-      else if (P)
-        hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, col, P, kInvalid);
+      // passes over whole source boxes, each with at most chunkPoints test points of scratch
+      // (ImplicitBoxTree.hpp:2941-3004 sizes its passes from the free memory and carries the open
+      // last row across them; a pass here ends at a box boundary, so its rows are complete and
+      // the passes' entries follow one another in row-major order)
+      const uint64_t bytesPerPoint = k <= 64 ? 4 : 24;  // targets (k > 64: rows, sort buffers)
+      const uint64_t budget = tmChunkBudget(T);
+      const uint64_t boxesPerPass =
+          std::max<uint64_t>(1, std::min<uint64_t>(budget / (bytesPerPoint * k), (uint64_t(1) << 31) / k));
       const uint64_t nRowsSrc = src.nBoxes;
-      uint32_t nnz = 0;
-      uint32_t* rows = nullptr;
-      uint32_t* cols = nullptr;
-      uint32_t* cnt = nullptr;
+      GrowBuf rows, cols, cnt;
+      uint64_t nnz = 0;
+      uint32_t passes = 0;
       zeroScal(T);
-      if (k <= 64) {
-        // k targets of a box sorted inside one wave segment, run-length encoded
-        uint32_t* rowPtr = dalloc<uint32_t>(nRowsSrc + 1);
-        TmRowArgs a{};
-        a.col = col;
-        a.nRows = nRowsSrc;
-        a.k = k;
-        a.keepInvalid = useInvalid ? 1 : 0;
-        a.invalidLabel = nBoxes + 1;  // nColumns_ after its increment (ImplicitBoxTree.hpp:3081-3087,3127-3132)
-        a.rowCount = rowPtr;
-        a.anyInvalid = T.scal + 2;
-        a.pass = 0;
-        tmRows(a, T.stream);
-        HIPCHK(hipMemsetAsync(rowPtr + nRowsSrc, 0, 4, T.stream));
-        nnz = scanArray(T.stream, rowPtr, nRowsSrc + 1, T.totals, T.scal + 3, T.hscal + 3);
-        cols = dalloc<uint32_t>(nnz);
-        cnt = dalloc<uint32_t>(nnz);
-        a.pass = 1;
-        a.rowPtr = rowPtr;
-        a.outCol = cols;
-        a.outCnt = cnt;
-        tmRows(a, T.stream);
-        rows = dalloc<uint32_t>(nnz);
-        if (nRowsSrc)
-          hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nRowsSrc, kBlock)), dim3(kBlock), 0, T.stream, rowPtr,
-                             nRowsSrc, rows);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(T.stream));
-        dfree(rowPtr);
+      for (uint64_t b0 = 0; b0 < nRowsSrc; b0 += boxesPerPass) {
+        const uint64_t nb = std::min(boxesPerPass, nRowsSrc - b0), np = nb * k;
+        ++passes;
+        uint32_t* col = dalloc<uint32_t>(np);
+        if (dc != DC_BEYOND)
+          runImage(T, *s, src, dc == DC_ROOT, 1, col, dc == DC_GENERAL ? &DM : nullptr, nullptr, b0 * k, np);
+        else
+          hipLaunchKernelGGL(fillU32Kernel, dim3(ceilDiv(np, kBlock)), dim3(kBlock), 0, T.stream, col, np, kInvalid);
+        if (k <= 64) {
+          // k targets of a box sorted inside one wave segment, run-length encoded
+          uint32_t* rowPtr = dalloc<uint32_t>(nb + 1);
+          TmRowArgs a{};
+          a.col = col;
+          a.nRows = nb;
+          a.k = k;
+          a.keepInvalid = useInvalid ? 1 : 0;
+          a.invalidLabel = nBoxes + 1;  // nColumns_ after its increment (ImplicitBoxTree.hpp:3081-3087,3127-3132)
+          a.rowCount = rowPtr;
+          a.anyInvalid = T.scal + 2;
+          a.pass = 0;
+          tmRows(a, T.stream);
+          HIPCHK(hipMemsetAsync(rowPtr + nb, 0, 4, T.stream));
+          const uint32_t nc = scanArray(T.stream, rowPtr, nb + 1, T.totals, T.scal + 3, T.hscal + 3);
+          cols.reserve(nnz + nc, nnz, T.stream);
+          cnt.reserve(nnz + nc, nnz, T.stream);
+          rows.reserve(nnz + nc, nnz, T.stream);
+          a.pass = 1;
+          a.rowPtr = rowPtr;
+          a.outCol = cols.p + nnz;
+          a.outCnt = cnt.p + nnz;
+          tmRows(a, T.stream);
+          hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nb, kBlock)), dim3(kBlock), 0, T.stream, rowPtr, nb,
+                             uint32_t(b0), rows.p + nnz);
+          HIPCHK(hipGetLastError());
+          HIPCHK(hipStreamSynchronize(T.stream));
+          dfree(rowPtr);
+          nnz += nc;
+        } else {
+          // any k: stable sort by column, then by row (row-major), then run-length encoding
+          uint32_t* prow = dalloc<uint32_t>(np);
+          uint32_t* dummy = dalloc<uint32_t>(np);
+          hipLaunchKernelGGL(rowOfPointKernel, dim3(ceilDiv(np, kBlock)), dim3(kBlock), 0, T.stream, prow, np, k,
+                             uint32_t(b0));
+          hipLaunchKernelGGL(anyInvalidKernel, dim3(ceilDiv(np, kBlock)), dim3(kBlock), 0, T.stream,
+                             (const uint32_t*)col, np, T.scal + 2);
+          radixSort3(T.stream, col, prow, dummy, np, 32, T.hscal + 4);
+          radixSort3(T.stream, prow, col, dummy, np, bitsFor(nRowsSrc), T.hscal + 4);
+          dfree(dummy);
+          RunLengthOp op{};
+          op.row = prow;
+          op.col = col;
+          op.n = np;
+          op.keepInvalid = useInvalid ? 1 : 0;
+          op.invalidLabel = nBoxes + 1;
+          // count first to size the outputs (tile totals sized for the pass's points)
+          uint32_t* ptot = dalloc<uint32_t>(2 * uint64_t(ceilDiv(np, kTile)) + 16);
+          runScan(T, op, uint32_t(np), ptot, 3, nullptr, -1, /*countOnly=*/true);
+          syncScal(T, 4);
+          const uint32_t nc = T.hscal[3];
+          rows.reserve(nnz + nc, nnz, T.stream);
+          cols.reserve(nnz + nc, nnz, T.stream);
+          cnt.reserve(nnz + nc, nnz, T.stream);
+          op.outRow = rows.p + nnz;
+          op.outCol = cols.p + nnz;
+          op.outCnt = cnt.p + nnz;
+          runScan(T, op, uint32_t(np), ptot, 3);
+          HIPCHK(hipStreamSynchronize(T.stream));
+          dfree(ptot);
+          dfree(prow);
+          nnz += nc;
+        }
         dfree(col);
-      } else {
-        // any k: stable sort by column, then by row (row-major), then run-length encoding
-        uint32_t* prow = dalloc<uint32_t>(P);
-        uint32_t* dummy = dalloc<uint32_t>(P);
-        hipLaunchKernelGGL(rowOfPointKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream, prow, P, k);
-        hipLaunchKernelGGL(anyInvalidKernel, dim3(ceilDiv(P, kBlock)), dim3(kBlock), 0, T.stream,
-                           (const uint32_t*)col, P, T.scal + 2);
-        radixSort3(T.stream, col, prow, dummy, P, 32, T.hscal + 4);
-        radixSort3(T.stream, prow, col, dummy, P, bitsFor(nRowsSrc), T.hscal + 4);
-        dfree(dummy);
-        if (P > 0xFFFFFFFEull) throw HipFail(B12X_E_ARG, "scan too large");
-        RunLengthOp op{};
-        op.row = prow;
-        op.col = col;
-        op.n = P;
-        op.keepInvalid = useInvalid ? 1 : 0;
-        op.invalidLabel = nBoxes + 1;
-        // count first to size the outputs (tile totals sized for P elements, not for the tree)
-        uint32_t* ptot = dalloc<uint32_t>(2 * uint64_t(ceilDiv(P, kTile)) + 16);
-        runScan(T, op, uint32_t(P), ptot, 3, nullptr, -1, /*countOnly=*/true);
-        syncScal(T, 4);
-        nnz = T.hscal[3];
-        rows = dalloc<uint32_t>(nnz);
-        cols = dalloc<uint32_t>(nnz);
-        cnt = dalloc<uint32_t>(nnz);
-        op.outRow = rows;
-        op.outCol = cols;
-        op.outCnt = cnt;
-        runScan(T, op, uint32_t(P), ptot, 3);
-        HIPCHK(hipStreamSynchronize(T.stream));
-        dfree(ptot);
-        dfree(prow);
-        dfree(col);
+      }
+      lastTmPasses = passes;
+      if (nnz > 0xFFFFFFFFull) throw HipFail(B12X_E_UNSUPPORTED, "more than 2^32 - 1 matrix entries");
+      if (passes > 1) {  // the doubling's slack
+        rows.shrink(nnz, T.stream);
+        cols.shrink(nnz, T.stream);
+        cnt.shrink(nnz, T.stream);
       }
       syncScal(T, 3);
       const bool anyInvalid = T.hscal[2] != 0;
@@ -2080,18 +2192,17 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
       if (!rowMajor) {
         // column-major order (CooMatrix.hpp:95-137): stable LSD radix sort by column of the
         // row-major entries leaves rows ascending within each column
-        radixSort3(T.stream, cols, rows, cnt, nnz, bitsFor(uint64_t(nBoxes) + 1), T.hscal + 4);
+        radixSort3(T.stream, cols.p, rows.p, cnt.p, nnz, bitsFor(uint64_t(nBoxes) + 1), T.hscal + 4);
       }
-      M->nnz = nnz;
-      M->rows = rows;
-      M->cols = cols;
+      M->nnz = uint32_t(nnz);
+      M->rows = rows.release();
+      M->cols = cols.release();
       M->vals = dalloc<double>(nnz);
       if (nnz)
-        hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, T.stream, cnt, uint64_t(nnz),
-                           double(k), useAbs ? 1 : 0, M->vals);
+        hipLaunchKernelGGL(valuesKernel, dim3(ceilDiv(nnz, kBlock)), dim3(kBlock), 0, T.stream, (const uint32_t*)cnt.p,
+                           uint64_t(nnz), double(k), useAbs ? 1 : 0, M->vals);
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(T.stream));
-      dfree(cnt);
     } catch (...) {
       delete M;
       throw;
@@ -2517,8 +2628,8 @@ Csc adoptCsc(Csc C, DevScope& ds) {
 // rho = ||Y - X_k H||_F / ||H||_F (the two sums on the device, the square roots and the quotient
 // on the host); stop at k = maxIter or rho <= tol, else X_{k+1} = cholQR2(Y).
 template <int M>
-void subspaceRun(b12x_coo_s& mat, uint32_t n, int maxIter, double tol, double* ritz, double* basis, int* itersOut,
-                 double* residOut) {
+void subspaceRun(b12x_coo_s& mat, uint32_t n, int lead, int maxIter, double tol, double* ritz, double* basis,
+                 int* itersOut, double* residOut) {
   hipStream_t st = mat.stream;
   DevScope ds;
   Csc C = adoptCsc(buildCsc(mat, n), ds);
@@ -2553,15 +2664,16 @@ void subspaceRun(b12x_coo_s& mat, uint32_t n, int maxIter, double tol, double* r
     hipLaunchKernelGGL(ritzKernel, dim3(1), dim3(kBlock), 0, st, (const double*)part, nd, M, H);
     if (tol >= 0.0 || k == maxIter) {
       hipLaunchKernelGGL(residualPartialKernel<M>, dim3(nd), dim3(kBlock), 0, st, (const double*)X,
-                         (const double*)Y, (const double*)H, n, part);
+                         (const double*)Y, (const double*)H, n, lead, part);
       hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(kBlock), 0, st, (const double*)part, nd, H + M * M);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(hh, H, sizeof(hh), hipMemcpyDeviceToHost, st));
       HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       if (hbad) throw HipFail(B12X_E_ARG, "subspace iteration: Gram matrix not positive definite");
-      double h2 = 0.0;
-      for (int q = 0; q < M * M; ++q) h2 += hh[q] * hh[q];
+      double h2 = 0.0;  // ||H_LL||_F^2, row by row
+      for (int a = 0; a < lead; ++a)
+        for (int b = 0; b < lead; ++b) h2 += hh[a * M + b] * hh[a * M + b];
       const double rho = std::sqrt(hh[M * M]) / std::sqrt(h2);
       if (k == maxIter || rho <= tol) {
         std::memcpy(ritz, hh, sizeof(double) * M * M);
@@ -2889,22 +3001,23 @@ int b12x_power_iteration_noalloc(b12x_coo m, int maxIter, double tol, int muInit
   });
 }
 
-int b12x_subspace_iteration(b12x_coo m, int nvec, int max_iter, double tol, double* ritz, double* basis,
+int b12x_subspace_iteration(b12x_coo m, int nvec, int nlead, int max_iter, double tol, double* ritz, double* basis,
                             int* iters, double* resid) {
   return guarded([&]() -> int {
-    if (!m || !ritz || nvec < 1 || nvec > kSubspaceMax || max_iter < 0) return setErr(B12X_E_ARG, "bad argument");
+    if (!m || !ritz || nvec < 1 || nvec > kSubspaceMax || nlead < 1 || nlead > nvec || max_iter < 0)
+      return setErr(B12X_E_ARG, "bad argument");
     if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
     const uint32_t n = m->nRows;
     if (n < uint32_t(nvec)) return setErr(B12X_E_ARG, "fewer boxes than vectors");
     switch (nvec) {
-      case 1: subspaceRun<1>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
-      case 2: subspaceRun<2>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
-      case 3: subspaceRun<3>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
-      case 4: subspaceRun<4>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
-      case 5: subspaceRun<5>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
-      case 6: subspaceRun<6>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
-      case 7: subspaceRun<7>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
-      default: subspaceRun<8>(*m, n, max_iter, tol, ritz, basis, iters, resid); break;
+      case 1: subspaceRun<1>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
+      case 2: subspaceRun<2>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
+      case 3: subspaceRun<3>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
+      case 4: subspaceRun<4>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
+      case 5: subspaceRun<5>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
+      case 6: subspaceRun<6>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
+      case 7: subspaceRun<7>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
+      default: subspaceRun<8>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
     }
     return B12X_OK;
   });

@@ -666,7 +666,7 @@ void retire(b12x_forest_s& F, std::vector<uint32_t> R, const std::vector<uint32_
   tmRows(ta, st);
   if (nList)
     hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nList, kBlock)), dim3(kBlock), 0, st, (const uint32_t*)rowPtr,
-                       uint64_t(nList), erow);
+                       uint64_t(nList), 0u, erow);
   HIPCHK(hipGetLastError());
   std::vector<uint32_t> hRowPtr(uint64_t(nList) + 1);
   HIPCHK(hipMemcpyAsync(hRowPtr.data(), rowPtr, (uint64_t(nList) + 1) * 4, hipMemcpyDeviceToHost, st));

@@ -1080,9 +1080,11 @@ struct RunLengthOp : OpBase {
   }
 };
 
-static __global__ __launch_bounds__(kBlock) void rowOfPointKernel(uint32_t* row, uint64_t n, uint32_t k) {
+// row[i] = rowBase + i / k: the source box of point i of a pass starting at box rowBase
+static __global__ __launch_bounds__(kBlock) void rowOfPointKernel(uint32_t* row, uint64_t n, uint32_t k,
+                                                                  uint32_t rowBase) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i < n) row[i] = uint32_t(i / k);
+  if (i < n) row[i] = rowBase + uint32_t(i / k);
 }
 
 static __global__ __launch_bounds__(kBlock) void anyInvalidKernel(const uint32_t* col, uint64_t n, uint32_t* flag) {
@@ -1091,11 +1093,12 @@ static __global__ __launch_bounds__(kBlock) void anyInvalidKernel(const uint32_t
   if (__ballot(inv) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1u);
 }
 
-// Expand CSR row pointers into the COO row index per entry.
-static __global__ __launch_bounds__(kBlock) void expandRowsKernel(const uint32_t* rowPtr, uint64_t nRows, uint32_t* rows) {
+// Expand CSR row pointers into the COO row index per entry (rowBase + the local row).
+static __global__ __launch_bounds__(kBlock) void expandRowsKernel(const uint32_t* rowPtr, uint64_t nRows,
+                                                                  uint32_t rowBase, uint32_t* rows) {
   const uint64_t r = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (r >= nRows) return;
-  for (uint32_t e = rowPtr[r]; e < rowPtr[r + 1]; ++e) rows[e] = uint32_t(r);
+  for (uint32_t e = rowPtr[r]; e < rowPtr[r + 1]; ++e) rows[e] = rowBase + uint32_t(r);
 }
 
 // values = count / k (or count) as double (ImplicitBoxTree.hpp:3145-3158)
@@ -2010,12 +2013,13 @@ static __global__ __launch_bounds__(kBlock) void applyRinvKernel(double* Y, uint
   for (int c = 0; c < M; ++c) Y[uint64_t(i) * M + c] = x[c];
 }
 
-// Residual of the Ritz pairs: per box sum over c of (y_c - sum_d x_d H_dc)^2 (d, then c in
-// order), summed over the boxes in dotPartialKernel's strided order; the block partials go to
-// part[blockIdx.x]
+// Residual of the leading `lead` basis columns: per box sum over c < lead of (y_c - sum over
+// d < lead of x_d H_dc)^2 (d, then c in order), summed over the boxes in dotPartialKernel's
+// strided order; the block partials go to part[blockIdx.x]
 template <int M>
 static __global__ __launch_bounds__(kBlock) void residualPartialKernel(const double* X, const double* Y,
-                                                                      const double* H, uint32_t n, double* part) {
+                                                                      const double* H, uint32_t n, int lead,
+                                                                      double* part) {
   __shared__ double h[M * M];
   if (threadIdx.x < M * M) h[threadIdx.x] = H[threadIdx.x];
   __syncthreads();
@@ -2027,10 +2031,13 @@ static __global__ __launch_bounds__(kBlock) void residualPartialKernel(const dou
     double r = 0.0;
 #pragma unroll
     for (int c = 0; c < M; ++c) {
-      double v = Y[uint64_t(i) * M + c];
+      if (c < lead) {
+        double v = Y[uint64_t(i) * M + c];
 #pragma unroll
-      for (int d = 0; d < M; ++d) v -= x[d] * h[d * M + c];
-      r += v * v;
+        for (int d = 0; d < M; ++d)
+          if (d < lead) v -= x[d] * h[d * M + c];
+        r += v * v;
+      }
     }
     acc += r;
   }

@@ -97,9 +97,24 @@ int b12x_all_leaves_on_depth(b12x_tree t, uint8_t depth, int* out);
  * a's depth(). */
 int b12x_has_same_boxes(b12x_tree a, b12x_tree b, int* out);
 /* getNHostBufferBytes / setNHostBufferBytes — ImplicitBoxTree.h:568-569: kept and written to the
- * tree's files (default 4 GiB); trees live in HBM whole, so it limits nothing here. */
+ * tree's files (default 4 GiB). The reference consults it only on its host architecture
+ * (Memory<HOST>::getFreeBytes, ThrustSystem.hpp:21-42); its device path sizes passes from the free
+ * device memory (ThrustSystem.hpp:64-68), as this library does, so it limits nothing here. */
 int b12x_get_host_buffer_bytes(b12x_tree t, uint64_t* out);
 int b12x_set_host_buffer_bytes(b12x_tree t, uint64_t bytes);
+/* Pass sizing of transitionMatrix (ImplicitBoxTree.hpp:2941-3004 sizes each pass from
+ * Memory<DEVICE>::getFreeBytes): the test points of the source boxes are imaged and reduced in
+ * passes over whole boxes, each holding at most `bytes` of per-point scratch (4 bytes a point for
+ * k <= 64 test points per box, 24 above; at least one box a pass). 0, the default, bounds a pass
+ * at a third of the device's free memory when it starts, as the reference sizes its first pass.
+ * The matrix does not depend on the passes; its own arrays (16 bytes an entry, 12 more while the
+ * column-major sort runs) are the memory that remains. No C ABI counterpart (the reference has
+ * no setter: it always sizes from free memory). */
+int b12x_set_tm_pass_bytes(b12x_tree t, uint64_t bytes);
+int b12x_get_tm_pass_bytes(b12x_tree t, uint64_t* bytes);
+/* Measurement (no counterpart in the reference): the passes of the calling thread's last
+ * transitionMatrix. */
+int b12x_tm_last_passes(uint32_t* passes);
 /* size(), count(depth), depth() — ImplicitBoxTree.h:110-111,129-130,603 */
 int b12x_size(b12x_tree t, uint32_t* out);
 int b12x_count(b12x_tree t, uint8_t depth, uint32_t* out);
@@ -172,6 +187,10 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd_flag, int n_st
  * and enables HIP-event timing of every image-kernel launch on the tree's stream; the totals
  * since then are returned (milliseconds, launches, test points). enable=0 disables timing. */
 int b12x_tree_image_timing(b12x_tree t, int enable, double* ms, uint64_t* launches, uint64_t* points);
+/* Measurement (no counterpart in the reference): each image launch timed since timing was last
+ * enabled, in launch order: its duration (ms) and test points. *n receives their number; at most
+ * cap of them are written (ms or points may be NULL). */
+int b12x_tree_image_launches(b12x_tree t, double* ms, uint64_t* points, uint64_t cap, uint64_t* n);
 
 /* Measurement hook (no counterpart in the reference): `reps` launches of a plain device copy of
  * `bytes` bytes (16-byte vector loads and stores) between two fresh buffers; the average launch
@@ -278,13 +297,17 @@ int b12x_power_iteration_form(int* form);
 /* Subspace iteration for the nvec (<= 8) leading eigenvalues (no counterpart in the reference,
  * which leaves eigenvalues to MATLAB eigs, subdivisionByNim.m:30-59): X_0 = cholQR2([1/n, splitmix64
  * columns]); for k = 0, 1, ...: Y = X_k P (row-vector convention, as the power iteration), the Ritz
- * matrix H = X_k^T Y and the residual rho = ||Y - X_k H||_F / ||H||_F; it stops at the first k with
- * rho <= tol (tol >= 0) or k = max_iter, else X_{k+1} = cholQR2(Y). ritz (nvec x nvec, row-major)
- * receives that H, whose eigenvalues approximate the leading eigenvalues of P; basis (nvec x n,
- * column by column, may be NULL) X_k; *iters (may be NULL) k, the basis updates made; *resid (may
- * be NULL) rho. Every sum runs in one fixed order, so results do not depend on timing. */
-int b12x_subspace_iteration(b12x_coo m, int nvec, int max_iter, double tol, double* ritz, double* basis, int* iters,
-                            double* resid);
+ * matrix H = X_k^T Y and the residual of the leading nlead (1 <= nlead <= nvec) columns,
+ * rho = ||Y_L - X_L H_LL||_F / ||H_LL||_F (L: those columns; H_LL the leading nlead x nlead block),
+ * zero when their span is invariant (Cholesky QR keeps the spans of leading columns nested, so
+ * they converge to the dominant invariant subspace of that dimension); it stops at the first k
+ * with rho <= tol (tol >= 0) or k = max_iter, else X_{k+1} = cholQR2(Y). ritz (nvec x nvec,
+ * row-major) receives that H, whose eigenvalues approximate the leading eigenvalues of P; basis
+ * (nvec x n, column by column, may be NULL) X_k; *iters (may be NULL) k, the basis updates made;
+ * *resid (may be NULL) rho. Every sum runs in one fixed order, so results do not depend on
+ * timing. */
+int b12x_subspace_iteration(b12x_coo m, int nvec, int nlead, int max_iter, double tol, double* ritz, double* basis,
+                            int* iters, double* resid);
 /* Strong components of the matrix's graph (an edge row -> column per entry; entries naming a
  * vertex >= n_rows, such as the column of points outside the cover, are left out), as
  * chainRecurrentSet.m:26-39 obtains them from MATLAB's graphconncomp. labels (n_rows entries,

@@ -1453,24 +1453,27 @@ inline bool cholQR(std::vector<double>& X, NrBoxes n, int m) {
 }
 
 // ||Y - X H||_F / ||H||_F: per box the squares of (y_c - sum_d x_d H_dc) added over c in order
+// ||Y_L - X_L H_LL||_F / ||H_LL||_F over the leading `lead` columns (L) of the basis and the
+// leading lead x lead block of H: zero when the span of those columns is invariant
 inline double subspaceResidual(const std::vector<double>& X, const std::vector<double>& Y,
-                               const std::vector<double>& H, NrBoxes n, int m) {
+                               const std::vector<double>& H, NrBoxes n, int m, int lead) {
   const double r2 = stridedSum(n, [&](size_t i) {
     double r = 0.0;
-    for (int c = 0; c < m; ++c) {
+    for (int c = 0; c < lead; ++c) {
       double v = Y[size_t(c) * n + i];
-      for (int d = 0; d < m; ++d) v -= X[size_t(d) * n + i] * H[d * m + c];
+      for (int d = 0; d < lead; ++d) v -= X[size_t(d) * n + i] * H[d * m + c];
       r += v * v;
     }
     return r;
   });
   double h2 = 0.0;
-  for (int q = 0; q < m * m; ++q) h2 += H[q] * H[q];
+  for (int a = 0; a < lead; ++a)
+    for (int b = 0; b < lead; ++b) h2 += H[a * m + b] * H[a * m + b];
   return std::sqrt(r2) / std::sqrt(h2);
 }
 
 // the number of basis updates, or -1 if a Gram matrix was not positive definite
-inline int subspaceIteration(const Coo& P, int m, int maxIter, double tol, std::vector<double>& H,
+inline int subspaceIteration(const Coo& P, int m, int lead, int maxIter, double tol, std::vector<double>& H,
                              std::vector<double>& X, double* residOut) {
   const NrBoxes n = P.nRows;
   std::vector<size_t> order(P.rows.size());
@@ -1508,7 +1511,7 @@ inline int subspaceIteration(const Coo& P, int m, int maxIter, double tol, std::
     spmm(X, Y);
     ritz(X, Y);
     if (tol >= 0.0 || k == maxIter) {
-      const double rho = subspaceResidual(X, Y, H, n, m);
+      const double rho = subspaceResidual(X, Y, H, n, m, lead);
       if (residOut) *residOut = rho;
       if (k == maxIter || rho <= tol) return k;
     }

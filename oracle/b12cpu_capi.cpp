@@ -427,14 +427,16 @@ int b12cpu_energy_band_stencil(void* h, double emin, double emax, uint8_t* out) 
   return 0;
 }
 
-// H: m x m Ritz matrix (row-major); X: m x nRows basis (column by column), may be null
-int b12cpu_subspace_iteration(void* c, int m, int maxIter, double tol, double* H, double* X, int* iters,
+// H: m x m Ritz matrix (row-major); X: m x nRows basis (column by column), may be null; the
+// residual is that of the leading `lead` basis columns
+int b12cpu_subspace_iteration(void* c, int m, int lead, int maxIter, double tol, double* H, double* X, int* iters,
                               double* resid) {
   const Coo& P = static_cast<CooH*>(c)->m;
-  if (m < 1 || m > 8 || maxIter < 0) return fail("subspace iteration: 1 <= m <= 8, max_iter >= 0");
+  if (m < 1 || m > 8 || lead < 1 || lead > m || maxIter < 0)
+    return fail("subspace iteration: 1 <= lead <= m <= 8, max_iter >= 0");
   std::vector<double> h, x;
   double r = -1.0;
-  const int k = subspaceIteration(P, m, maxIter, tol, h, x, &r);
+  const int k = subspaceIteration(P, m, lead, maxIter, tol, h, x, &r);
   if (k < 0) return fail("subspace iteration: Gram matrix not positive definite");
   std::memcpy(H, h.data(), h.size() * 8);
   if (X) std::memcpy(X, x.data(), x.size() * 8);

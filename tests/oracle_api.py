@@ -31,7 +31,8 @@ def lib():
         _lib.b12cpu_set_options.argtypes = [_P, ctypes.c_int, ctypes.c_uint64]
         _lib.b12cpu_energy_band_stencil.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P]
         _lib.b12cpu_set_threads.argtypes = [ctypes.c_int]
-        _lib.b12cpu_subspace_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P, _P, _P, _P]
+        _lib.b12cpu_subspace_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P,
+                                                   _P, _P, _P]
     return _lib
 
 
@@ -122,14 +123,15 @@ class OCoo:
                                           ctypes.byref(res)))
         return m, it.value, res.value
 
-    def subspace_iteration(self, m=4, iters=100, want_basis=False, tol=-1.0):
+    def subspace_iteration(self, m=4, iters=100, want_basis=False, tol=-1.0, lead=None):
         """(Ritz matrix H (m x m), basis X (m x n) or None, basis updates, residual) of the
-        build-defined subspace iteration: at most `iters` updates, stopping once the residual
-        ||X P - X H||_F / ||H||_F is at most tol (tol < 0: exactly `iters` updates)."""
+        build-defined subspace iteration: at most `iters` updates, stopping once the residual of
+        the leading `lead` basis columns (all m by default), ||X_L P - X_L H_LL||_F / ||H_LL||_F,
+        is at most tol (tol < 0: exactly `iters` updates)."""
         H = np.zeros((m, m), np.float64)
         X = np.zeros((m, self.nRows), np.float64) if want_basis else None
         it, res = ctypes.c_int(), ctypes.c_double()
-        _chk(lib().b12cpu_subspace_iteration(self.h, int(m), int(iters), float(tol), _ptr(H),
+        _chk(lib().b12cpu_subspace_iteration(self.h, int(m), int(lead or m), int(iters), float(tol), _ptr(H),
                                              _ptr(X) if want_basis else None, ctypes.byref(it), ctypes.byref(res)))
         return H, X, it.value, res.value
 

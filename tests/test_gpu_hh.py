@@ -77,6 +77,10 @@ def test_band_subdivision_matrix_and_subspace_match_oracle():
         assert np.allclose(Xg @ Xg.T, np.eye(4), atol=1e-10)
         assert np.all(np.abs(sorted_eigs(Hg)) <= 1.0 + 1e-12)
         assert kg == iters or rg <= tol
+    # the leading-columns rule: 8 vectors, residual of the leading 4
+    Hg, Xg, kg, rg = mg.subspace_iteration(8, 200, want_basis=True, tol=1e-9, lead=4)
+    Ho, Xo, ko, ro = mo.subspace_iteration(8, 200, want_basis=True, tol=1e-9, lead=4)
+    assert kg == ko and rg == ro and np.array_equal(Hg, Ho) and np.array_equal(Xg, Xo)
 
 
 @pytest.mark.parametrize("nvec", [1, 3, 8])

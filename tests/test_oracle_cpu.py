@@ -831,6 +831,44 @@ def test_subspace_iteration_stopping_rule():
         assert np.allclose(np.sort(np.linalg.eigvals(H).real)[::-1], ref, rtol=0, atol=100 * tol)
 
 
+def test_subspace_iteration_leading_columns_rule():
+    """The residual of the leading `lead` columns: with 8 vectors and lead = 4 the iteration stops
+    once the span of the first 4 basis vectors is invariant. Cholesky QR keeps leading spans
+    nested, so those 4 vectors are the 4-vector iteration's basis (same start columns) to
+    rounding; their span is the dominant 4-dimensional left invariant subspace of P (principal
+    angles against numpy's eigenvectors); the eigenvalues of the leading 4 x 4 block of H are
+    numpy's 4 leading ones; rho recomputed with numpy from X_L and H_LL agrees."""
+    rng = np.random.default_rng(24)
+    n = 90
+    A = rng.random((n, n)) * (rng.random((n, n)) < 0.2)
+    for b in range(4):
+        A[b * 20:(b + 1) * 20, b * 20:(b + 1) * 20] += 3.0 * rng.random((20, 20))
+    P = A / A.sum(axis=1, keepdims=True)
+    r, c = np.nonzero(P)
+    m = OCoo.from_arrays(r, c, P[r, c], n, n)
+    tol = 1e-10
+    H, X, k, rho = m.subspace_iteration(8, 10000, want_basis=True, tol=tol, lead=4)
+    assert 0 < k < 10000 and rho <= tol
+    XL, HL = X[:4], H[:4, :4]
+    assert abs(np.linalg.norm(XL @ P - HL.T @ XL) / np.linalg.norm(HL) - rho) <= 1e-13
+    _, _, k1, rho1 = m.subspace_iteration(8, k - 1, lead=4)
+    assert rho1 > tol
+    # nested spans: the same first 4 vectors as the 4-vector run after k updates
+    _, X4, _, _ = m.subspace_iteration(4, k, want_basis=True)
+    assert np.allclose(XL, X4, rtol=0, atol=1e-9)
+    # the dominant left invariant subspace
+    w, V = np.linalg.eig(P.T)
+    top = np.argsort(-np.abs(w))[:4]
+    Q, _ = np.linalg.qr(np.real_if_close(V[:, top]).real)
+    s = np.linalg.svd(Q.T @ XL.T, compute_uv=False)
+    assert np.all(s >= 1 - 1e-8)  # cosines of the principal angles
+    ref = np.sort(w[top].real)[::-1]
+    assert np.allclose(np.sort(np.linalg.eigvals(HL).real)[::-1], ref, rtol=0, atol=1e-8)
+    # the whole 8-vector block is a different (stricter) rule
+    _, _, k8, rho8 = m.subspace_iteration(8, k)
+    assert k8 == k and rho8 > rho
+
+
 # ------------------------------------------------------------------ HIP library C ABI
 def header_symbols():
     text = open(os.path.join(ROOT, "include", "b12x_c_api.h")).read()
