@@ -72,6 +72,7 @@ SIGNATURES = {
     "b12x_get_tm_pass_bytes": [_P, _P],
     "b12x_tm_last_passes": [_P],
     "b12x_bench_copy": [ctypes.c_uint64, _I, _P],
+    "b12x_bench_fp64_clock": [_P, _P],
     "b12x_scheme_create": [_I, _I, _P, _I, _I, _P, _I, _P],
     "b12x_scheme_destroy": [_P],
     "b12x_scheme_points_per_box": [_P, _P],
@@ -161,6 +162,14 @@ def bench_copy(nbytes, reps=20, flavour="parity"):
     ms = _D()
     _Api(flavour)("b12x_bench_copy", int(nbytes), int(reps), ctypes.byref(ms))
     return ms.value
+
+
+def bench_fp64_clock(flavour="parity"):
+    """(shader clock in GHz under a full FP64 load, fraction of the FP64 issue slots used at that
+    clock) from b12x_bench_fp64_clock."""
+    ghz, frac = _D(), _D()
+    _Api(flavour)("b12x_bench_fp64_clock", ctypes.byref(ghz), ctypes.byref(frac))
+    return ghz.value, frac.value
 
 
 def tm_last_passes(flavour="parity"):

@@ -507,28 +507,40 @@ const uint64_t* stageSamples(b12x_tree_s& T, const uint32_t* nDev) {
   return T.samples;
 }
 
-// Image-kernel variants: test points per lane x block size.
+// Image-kernel variants: test points per lane x block size; sameBox: a lane's points are
+// consecutive points of one box (used when k is a multiple of ppl, else points bs apart).
 struct ImageVariant {
   int ppl, bs;
+  bool sameBox;
 };
-constexpr ImageVariant kImageVariants[] = {{1, 256}, {1, 128}, {1, 64}, {2, 256}, {2, 128},
-                                           {2, 64},  {4, 256}, {4, 128}, {4, 64}};
+constexpr ImageVariant kImageVariants[] = {{1, 256, false}, {1, 128, false}, {1, 64, false}, {2, 256, false},
+                                           {2, 128, false}, {2, 64, false},  {4, 256, false}, {4, 128, false},
+                                           {4, 64, false},  {2, 256, true},  {2, 128, true},  {3, 256, true},
+                                           {3, 128, true},  {4, 256, true},  {4, 128, true}};
 constexpr int kNumImageVariants = int(sizeof(kImageVariants) / sizeof(kImageVariants[0]));
 constexpr uint64_t kTuneSamplePoints = 1ull << 21;
 
-template <int DIM, int MAP, int PPL>
+template <int DIM, int MAP, int PPL, bool SAMEBOX = false>
 void launchImage3(const ImageArgs& a, int gridKind, int bs, hipStream_t st, uint32_t maxBlocks = 1u << 22) {
   const uint64_t per = uint64_t(bs) * PPL;
   const uint32_t blocks = uint32_t(std::min<uint64_t>((a.nPoints + per - 1) / per, maxBlocks));
   if (blocks == 0) return;
   if (gridKind == GRID_INNER)
-    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_INNER, PPL>), dim3(blocks), dim3(bs), 0, st, a);
+    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_INNER, PPL, SAMEBOX>), dim3(blocks), dim3(bs), 0, st, a);
   else
-    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_UNIFORM, PPL>), dim3(blocks), dim3(bs), 0, st, a);
+    hipLaunchKernelGGL((imageKernel<DIM, MAP, GRID_UNIFORM, PPL, SAMEBOX>), dim3(blocks), dim3(bs), 0, st, a);
 }
 
 template <int DIM, int MAP>
 void launchVariant(const ImageArgs& a, int gridKind, ImageVariant v, hipStream_t st) {
+  if (v.sameBox) {
+    switch (v.ppl) {
+      case 4: launchImage3<DIM, MAP, 4, true>(a, gridKind, v.bs, st); break;
+      case 3: launchImage3<DIM, MAP, 3, true>(a, gridKind, v.bs, st); break;
+      default: launchImage3<DIM, MAP, 2, true>(a, gridKind, v.bs, st); break;
+    }
+    return;
+  }
   switch (v.ppl) {
     case 4: launchImage3<DIM, MAP, 4>(a, gridKind, v.bs, st); break;
     case 2: launchImage3<DIM, MAP, 2>(a, gridKind, v.bs, st); break;
@@ -541,9 +553,9 @@ void launchVariant(const ImageArgs& a, int gridKind, ImageVariant v, hipStream_t
 // a prefix of at most kTuneSamplePoints of its points, and later launches use the fastest.
 // Every variant computes the same images and a repeated launch stores the same hit bytes or
 // targets, so tuning never changes a result.
-// B12X_IMAGE_TUNE=0: one point per lane, 256 threads; B12X_IMAGE_PPL / B12X_IMAGE_BLOCK fix a
-// variant. The staged descent and the search among boxes at a general depth always take one
-// point per lane.
+// B12X_IMAGE_TUNE=0: one point per lane, 256 threads; B12X_IMAGE_PPL / B12X_IMAGE_BLOCK /
+// B12X_IMAGE_SAMEBOX fix a variant. The staged descent and the search among boxes at a general
+// depth always take one point per lane.
 std::mutex g_tuneMutex;
 std::unordered_map<uint64_t, int> g_tuned;
 
@@ -553,9 +565,11 @@ int fixedImageVariant(const ImageArgs& a) {
   const char* p = std::getenv("B12X_IMAGE_PPL");
   const char* b = std::getenv("B12X_IMAGE_BLOCK");
   if (!p && !b && !(t && t[0] == '0')) return -1;
+  const char* sb = std::getenv("B12X_IMAGE_SAMEBOX");
   const int ppl = p ? std::atoi(p) : 1, bs = b ? std::atoi(b) : 256;
+  const bool same = sb && sb[0] == '1';
   for (int v = 0; v < kNumImageVariants; ++v)
-    if (kImageVariants[v].ppl == ppl && kImageVariants[v].bs == bs) return v;
+    if (kImageVariants[v].ppl == ppl && kImageVariants[v].bs == bs && kImageVariants[v].sameBox == same) return v;
   return 0;
 }
 
@@ -933,6 +947,46 @@ int b12x_bench_copy(uint64_t bytes, int reps, double* ms) {
     (void)hipEventDestroy(e1);
     dfree(a);
     dfree(b);
+    return B12X_OK;
+  });
+}
+
+int b12x_bench_fp64_clock(double* ghz, double* frac) {
+  return guarded([&]() -> int {
+    if (!ghz) return setErr(B12X_E_ARG, "null output");
+    if (int e = ensureDevice()) return e;
+    constexpr uint32_t kBlocks = 16384;
+    constexpr int kIters = 2000;
+    double* out = dalloc<double>(uint64_t(kBlocks) * kBlock);
+    uint64_t* clk = dalloc<uint64_t>(2 * kBlocks);
+    std::vector<uint64_t> h(2 * kBlocks);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(fp64ClockKernel, dim3(kBlocks), dim3(kBlock), 0, nullptr, out, clk, kIters, 0.999);
+    HIPCHK(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(fp64ClockKernel, dim3(kBlocks), dim3(kBlock), 0, nullptr, out, clk, kIters, 0.999);
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    HIPCHK(hipMemcpy(h.data(), clk, sizeof(uint64_t) * 2 * kBlocks, hipMemcpyDeviceToHost));
+    double cyc = 0.0, real = 0.0;
+    for (uint32_t b = 0; b < kBlocks; ++b) {
+      cyc += double(h[2 * b]);
+      real += double(h[2 * b + 1]);
+    }
+    *ghz = real > 0.0 ? cyc / real * 0.1 : 0.0;  // the real-time counter runs at 100 MHz
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // 16 FP64 wave instructions per iteration and wave, 4 cycles each on a 16-lane SIMD
+    const double instr = double(kBlocks) * (kBlock / kWave) * kIters * 16.0;
+    if (frac) *frac = (t > 0.0f && *ghz > 0.0) ? instr * 4.0 / (4.0 * cus * double(t) * 1e-3 * (*ghz) * 1e9) : 0.0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(out);
+    dfree(clk);
     return B12X_OK;
   });
 }

@@ -433,7 +433,15 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
   if (a.depth0) t = (pk == kInvalidKey) ? kInvalid : 0u;
   else if (a.depthBox) t = locateInDepth(a.key, a.depth, n, pk, a.searchDepth, a.depthIn, a.depthBox);
   else if (S) t = locateStaged(a.key, a.depth, n, pk, sm, S);
-  else t = locate(a.key, a.depth, n, pk);
+  else if (n > 0) {
+    // locate() with a trip count that depends on n only (wave-uniform probes, no branches)
+    const uint64_t pk1[1] = {pk};
+    uint32_t t1[1];
+    locateN<1>(a.key, a.depth, n, pk1, t1);
+    t = t1[0];
+  } else {
+    t = kInvalid;
+  }
   if (a.outKey) a.outKey[i] = pk;
   if (a.mode == 0) {
     if (t != kInvalid) a.hit[t] = 1;
@@ -443,29 +451,52 @@ __device__ inline void imagePoint(const ImageArgs& a, uint64_t i, uint64_t gi, u
 }
 
 // PPL points of one lane at once (local indices idx[q], idx[0] < np): lanes past the end repeat
-// point idx[0] and store nothing. Same arithmetic per point as imagePoint.
-template <int DIM, int MAP, int GRID, int PPL>
+// point idx[0] and store nothing. Same arithmetic per point as imagePoint. SAMEBOX: the PPL
+// points are consecutive points of one box (the caller checks that k and the first global index
+// are multiples of PPL), so the box's geometry and index arithmetic are formed once.
+template <int DIM, int MAP, int GRID, int PPL, bool SAMEBOX = false>
 __device__ inline void imagePointsN(const ImageArgs& a, const uint32_t (&idx)[PPL], uint32_t np, uint32_t n) {
   const uint32_t k = a.grid.k, base = uint32_t(a.pointBase);
   double x[PPL][DIM], y[PPL][DIM];
   bool valid[PPL];
-#pragma unroll
-  for (int q = 0; q < PPL; ++q) {
-    valid[q] = idx[q] < np;
-    const uint32_t gi = base + (valid[q] ? idx[q] : idx[0]);
-    const uint32_t b = gi / k;
+  if constexpr (SAMEBOX) {
+    const uint32_t gi0 = base + idx[0];
+    const uint32_t b = gi0 / k, gl0 = gi0 - b * k;
     const uint32_t src = a.srcList ? a.srcList[b] : b;
-    double c[DIM], rb[DIM], g[DIM];
+    double c[DIM], rb[DIM];
     boxCoords<DIM>(a.dom, a.srcKey[src], a.srcDepth[src], c, rb);
-    if constexpr (GRID == GRID_INNER) {
-      const uint32_t gl = gi - b * k;
 #pragma unroll
-      for (int j = 0; j < DIM; ++j) g[j] = a.grid.table[gl * DIM + j];
-    } else {
-      uniformGridPoint<DIM>(a.grid.seed, gi, g);
+    for (int q = 0; q < PPL; ++q) {
+      valid[q] = idx[q] < np;
+      double g[DIM];
+      if constexpr (GRID == GRID_INNER) {
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) g[j] = a.grid.table[(gl0 + q) * DIM + j];
+      } else {
+        uniformGridPoint<DIM>(a.grid.seed, gi0 + q, g);
+      }
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) x[q][j] = c[j] + g[j] * rb[j];
     }
+  } else {
 #pragma unroll
-    for (int j = 0; j < DIM; ++j) x[q][j] = c[j] + g[j] * rb[j];
+    for (int q = 0; q < PPL; ++q) {
+      valid[q] = idx[q] < np;
+      const uint32_t gi = base + (valid[q] ? idx[q] : idx[0]);
+      const uint32_t b = gi / k;
+      const uint32_t src = a.srcList ? a.srcList[b] : b;
+      double c[DIM], rb[DIM], g[DIM];
+      boxCoords<DIM>(a.dom, a.srcKey[src], a.srcDepth[src], c, rb);
+      if constexpr (GRID == GRID_INNER) {
+        const uint32_t gl = gi - b * k;
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) g[j] = a.grid.table[gl * DIM + j];
+      } else {
+        uniformGridPoint<DIM>(a.grid.seed, gi, g);
+      }
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) x[q][j] = c[j] + g[j] * rb[j];
+    }
   }
   applyMapN<DIM, MAP, PPL>(a.map, x, y);
   uint64_t pk[PPL];
@@ -489,8 +520,10 @@ __device__ inline void imagePointsN(const ImageArgs& a, const uint32_t (&idx)[PP
   }
 }
 
-// Any block size up to kBlock (the launch picks it); PPL points per lane.
-template <int DIM, int MAP, int GRID, int PPL>
+// Any block size up to kBlock (the launch picks it); PPL points per lane. SAMEBOX: a lane's PPL
+// points are consecutive points of one box (when k and the first global index are multiples of
+// PPL; otherwise points BS apart, as without SAMEBOX).
+template <int DIM, int MAP, int GRID, int PPL, bool SAMEBOX = false>
 static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
   const uint32_t BS = blockDim.x;
   __shared__ uint64_t sm[PPL == 1 ? kDescentSamples : 1];
@@ -498,6 +531,20 @@ static __global__ __launch_bounds__(kBlock) void imageKernel(ImageArgs a) {
   const uint64_t nPoints = a.nBoxesDev ? min(uint64_t(*a.nBoxesDev) * a.grid.k, a.nPoints) : a.nPoints;
   const uint32_t n = a.nBoxesDev ? *a.nBoxesDev : a.n;
   if (uint64_t(blockIdx.x) * BS * PPL >= nPoints) return;  // grid sized for a bound: no points here
+  if constexpr (PPL > 1 && SAMEBOX) {
+    if (a.grid.k % PPL == 0 && a.pointBase % PPL == 0 && !a.depth0 &&
+        a.pointBase + nPoints + uint64_t(PPL) * BS <= 0xFFFFFFFFull &&
+        uint64_t(gridDim.x) * BS * PPL <= 0xFFFFFFFFull) {
+      const uint32_t np = uint32_t(nPoints), stride = gridDim.x * BS * PPL;
+      for (uint32_t i0 = (blockIdx.x * BS + threadIdx.x) * PPL; i0 < np; i0 += stride) {
+        uint32_t idx[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) idx[q] = i0 + q;
+        imagePointsN<DIM, MAP, GRID, PPL, true>(a, idx, np, n);
+      }
+      return;
+    }
+  }
   if constexpr (PPL > 1) {
     if (a.pointBase + nPoints + uint64_t(PPL) * BS <= 0xFFFFFFFFull &&
         uint64_t(gridDim.x) * BS * PPL <= 0xFFFFFFFFull) {
@@ -2068,6 +2115,25 @@ static __global__ __launch_bounds__(kBlock) void transposeBasisKernel(const doub
 }
 
 // Plain copy (traffic-counter calibration): every byte read once and written once.
+// Eight independent FP64 multiply-add chains per lane (no memory traffic in the loop); block 0's
+// first lane records the shader-clock and 100 MHz real-time counters around its loop, so the
+// host can read the clock the device runs at under a full FP64 load.
+static __global__ __launch_bounds__(kBlock) void fp64ClockKernel(double* out, uint64_t* clk, int iters, double s) {
+  double a0 = threadIdx.x, a1 = a0 + 1.0, a2 = a0 + 2.0, a3 = a0 + 3.0, a4 = a0 + 4.0, a5 = a0 + 5.0,
+         a6 = a0 + 6.0, a7 = a0 + 7.0;
+  const uint64_t t0 = __builtin_amdgcn_s_memtime(), r0 = __builtin_amdgcn_s_memrealtime();
+  for (int i = 0; i < iters; ++i) {
+    a0 = a0 * s + 1.0; a1 = a1 * s + 1.0; a2 = a2 * s + 1.0; a3 = a3 * s + 1.0;
+    a4 = a4 * s + 1.0; a5 = a5 * s + 1.0; a6 = a6 * s + 1.0; a7 = a7 * s + 1.0;
+  }
+  const uint64_t t1 = __builtin_amdgcn_s_memtime(), r1 = __builtin_amdgcn_s_memrealtime();
+  out[uint64_t(blockIdx.x) * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+  if (threadIdx.x == 0) {
+    clk[2 * blockIdx.x] = t1 - t0;
+    clk[2 * blockIdx.x + 1] = r1 - r0;
+  }
+}
+
 static __global__ __launch_bounds__(kBlock) void copyKernel(const uint4* in, uint4* out, uint64_t n) {
   for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock)
     out[i] = in[i];

@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 vector, FMA counted as 2 flops
 PEAK_FP64_ISSUE_TFLOPS = 39.3    # the same peak when every FP64 op is a separate instruction
+CUS = 256                        # MI355X compute units (4 SIMDs of 16 FP64 lanes each)
 LORENZ_FLOPS_PER_IMAGE = 1590    # 20 RK4 steps x 78 + grid/affine/key 30 (each + - * / once)
 
 PEAK_HBM_BYTES = 8.0e12          # MI355X HBM3E
@@ -472,6 +473,10 @@ def main():
 
     ms_img, launches, pts = prof["image_timing"]
     achieved_tflops = pts * LORENZ_FLOPS_PER_IMAGE / (ms_img * 1e-3) / 1e12 if ms_img > 0 else 0.0
+    # the shader clock under a full FP64 load (below the 2.4 GHz the peaks assume) and the issue
+    # rate at that clock: CUs x 4 SIMDs x 16 lanes x clock
+    clock_ghz, clock_probe_frac = gx.api.bench_fp64_clock(args.flavour)
+    issue_at_clock = CUS * 64 * clock_ghz * 1e9 / 1e12
     # per launch: test points, microseconds and the fraction of the FP64 issue rate
     levels = [[int(p), round(m * 1e3, 2), round(p * LORENZ_FLOPS_PER_IMAGE / (m * 1e-3) / 1e12
                                                 / PEAK_FP64_ISSUE_TFLOPS, 4) if m > 0 else None]
@@ -499,6 +504,11 @@ def main():
         "roofline_image": {"bound": "fp64-valu", "kernel": "imageKernel<3,LORENZ,INNER>", "achieved": achieved_tflops,
                            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
                            "frac_issue_rate": achieved_tflops / PEAK_FP64_ISSUE_TFLOPS,
+                           "clock_ghz_fp64_load": clock_ghz,
+                           "frac_issue_rate_at_clock": achieved_tflops / issue_at_clock if issue_at_clock else None,
+                           "clock_note": "shader clock measured by b12x_bench_fp64_clock (independent FP64 chains "
+                                         f"on every SIMD, which use {clock_probe_frac:.3f} of the issue slots at that "
+                                         "clock); the peaks assume 2.4 GHz",
                            "flops_per_image": LORENZ_FLOPS_PER_IMAGE, "image_launches": launches,
                            "image_points": pts, "image_ms": ms_img, "traffic": profiled_traffic(),
                            "traffic_source": "profiles/lorenz_d24_job_traffic.md (FETCH_SIZE x2.000 + WRITE_SIZE x1.000)",

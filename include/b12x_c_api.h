@@ -197,6 +197,12 @@ int b12x_tree_image_launches(b12x_tree t, double* ms, uint64_t* points, uint64_t
  * time in milliseconds by HIP events. Used to calibrate HBM traffic counters. */
 int b12x_bench_copy(uint64_t bytes, int reps, double* ms);
 
+/* Measurement hook (no counterpart in the reference): a grid of 16384 x 256 lanes, each running
+ * eight independent FP64 multiply-add chains, with the shader-clock and 100 MHz real-time
+ * counters read around the loop. *ghz receives the shader clock under that full FP64 load; *frac
+ * (may be NULL) the fraction of the SIMDs' FP64 issue slots at that clock the launch used. */
+int b12x_bench_fp64_clock(double* ghz, double* frac);
+
 /* --- scheme ----------------------------------------------------------------------------- */
 int b12x_scheme_create(int dim, int grid_kind, const uint64_t* grid_params, int n_grid_params, int map_kind,
                        const double* map_params, int n_map_params, b12x_scheme* out);

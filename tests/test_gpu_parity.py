@@ -444,14 +444,17 @@ def test_transition_matrix_more_than_64_points_per_box(grid, use_invalid, row_ma
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("ppl,block", [(1, 64), (2, 128), (4, 256), (4, 64), (2, 256)])
-def test_image_kernel_variants_are_bit_exact(monkeypatch, ppl, block):
-    """Every (points per lane, block size) variant the autotuner chooses between gives the
-    oracle's covers, matrices and search results: Lorenz with InnerGrid (subdivision + matrix),
-    Hénon-Heiles with UniformGrid on a random tree."""
-    print(f"[variant] ppl={ppl} block={block}", flush=True)
+@pytest.mark.parametrize("ppl,block,same_box", [(1, 64, 0), (2, 128, 0), (4, 256, 0), (4, 64, 0), (2, 256, 0),
+                                                (2, 256, 1), (3, 128, 1), (3, 256, 1), (4, 128, 1)])
+def test_image_kernel_variants_are_bit_exact(monkeypatch, ppl, block, same_box):
+    """Every (points per lane, block size, same-box) variant the autotuner chooses between gives
+    the oracle's covers, matrices and search results: Lorenz with InnerGrid (k = 27: the same-box
+    form for 3 points a lane, points a block apart for 2 and 4), Hénon-Heiles with UniformGrid
+    (k = 7: never the same-box form) on a random tree and with k = 64 on another."""
+    print(f"[variant] ppl={ppl} block={block} same_box={same_box}", flush=True)
     monkeypatch.setenv("B12X_IMAGE_PPL", str(ppl))
     monkeypatch.setenv("B12X_IMAGE_BLOCK", str(block))
+    monkeypatch.setenv("B12X_IMAGE_SAMEBOX", str(same_box))
     g, o = subdivision_run([0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz(), 11)
     for a, b in zip(g.transitionMatrixForLeaves(gx.InnerGrid(3), gx.Lorenz()).download(),
                     o.transitionMatrixForLeaves(gx.InnerGrid(3), gx.Lorenz()).download()):
@@ -465,3 +468,9 @@ def test_image_kernel_variants_are_bit_exact(monkeypatch, ppl, block):
     ig = g2.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
     io = o2.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
     assert np.array_equal(g2.searchByIterators(ig), o2.searchByIterators(io))
+    grid = gx.UniformGrid(64, 0)
+    ig = g2.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
+    io = o2.makePointIteratorPairOverLeaves(grid, gx.HenonHeiles())
+    assert np.array_equal(g2.searchByIterators(ig), o2.searchByIterators(io))
+    assert g2.setFlagsByIterators(ig, 0x20) == o2.setFlagsByIterators(io, 0x20)
+    assert_same_state(g2, o2)
