@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -3073,6 +3074,160 @@ int b12x_subspace_iteration(b12x_coo m, int nvec, int nlead, int max_iter, doubl
       case 7: subspaceRun<7>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
       default: subspaceRun<8>(*m, n, nlead, max_iter, tol, ritz, basis, iters, resid); break;
     }
+    return B12X_OK;
+  });
+}
+
+// Krylov basis of the Krylov-Schur eigensolver: the matrix's columns < n in CSC form, the basis
+// V (ncv + 1 vectors of n, column by column), the next vector w and the dot partials, on a
+// stream of its own (the matrix may be destroyed first).
+struct b12x_krylov_s {
+  uint32_t n = 0;
+  int ncv = 0;
+  uint32_t nd = 0;  // dot-product blocks: min(ceil(n / 256), kDotBlocks)
+  Csc C;
+  double* V = nullptr;
+  double* w = nullptr;
+  double* part = nullptr;  // kKrylovMaxCols * nd
+  double* h = nullptr;     // kKrylovMaxCols + 1 results
+  double* Q = nullptr;     // kKrylovMaxCols^2 rotation
+  hipStream_t stream = nullptr;
+  ~b12x_krylov_s() {
+    streamRetire(stream);
+    dfree(C.keys);
+    dfree(C.rows);
+    dfree(C.vals);
+    dfree(C.colPtr);
+    dfree(V);
+    dfree(w);
+    dfree(part);
+    dfree(h);
+    dfree(Q);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // h[0..cols) = x_c . v over the first cols basis columns (v: n doubles on the device)
+  void dots(uint32_t cols, const double* v, double* out) {
+    for (uint32_t c0 = 0; c0 < cols; c0 += kKrylovDotCols)
+      hipLaunchKernelGGL(colDotsKernel, dim3(nd), dim3(kBlock), 0, stream, (const double*)V, c0,
+                         std::min<uint32_t>(kKrylovDotCols, cols - c0), v, n, part);
+    hipLaunchKernelGGL(colDotsReduceKernel, dim3(cols), dim3(kBlock), 0, stream, (const double*)part, nd, out);
+  }
+};
+
+int b12x_krylov_create(b12x_coo m, int ncv, b12x_krylov* out) {
+  return guarded([&]() -> int {
+    if (!m || !out || ncv < 2 || ncv + 1 > kKrylovMaxCols) return setErr(B12X_E_ARG, "ncv must lie in [2, 63]");
+    if (m->nRows != m->nCols && m->nCols != m->nRows + 1) return setErr(B12X_E_ARG, "matrix must be square");
+    const uint32_t n = m->nRows;
+    if (n < uint32_t(ncv) + 1) return setErr(B12X_E_ARG, "fewer boxes than basis vectors");
+    auto K = std::make_unique<b12x_krylov_s>();
+    K->n = n;
+    K->ncv = ncv;
+    K->nd = std::min<uint32_t>(ceilDiv(n, kBlock), kDotBlocks);
+    K->C = buildCsc(*m, n);  // on the matrix's stream, which the creation waits for below
+    HIPCHK(hipStreamSynchronize(m->stream));
+    streamCreate(&K->stream);
+    K->V = dalloc<double>(uint64_t(ncv + 1) * n);
+    K->w = dalloc<double>(n);
+    K->part = dalloc<double>(uint64_t(kKrylovMaxCols) * K->nd);
+    K->h = dalloc<double>(kKrylovMaxCols + 1);
+    K->Q = dalloc<double>(uint64_t(kKrylovMaxCols) * kKrylovMaxCols);
+    *out = K.release();
+    return B12X_OK;
+  });
+}
+
+int b12x_krylov_destroy(b12x_krylov k) {
+  delete k;
+  return B12X_OK;
+}
+
+int b12x_krylov_start(b12x_krylov k, const double* v0, double* norm) {
+  return guarded([&]() -> int {
+    if (!k) return setErr(B12X_E_ARG, "null basis");
+    b12x_krylov_s& K = *k;
+    const uint32_t nb = ceilDiv(K.n, kBlock);
+    if (v0) HIPCHK(hipMemcpyAsync(K.w, v0, uint64_t(K.n) * 8, hipMemcpyHostToDevice, K.stream));
+    else hipLaunchKernelGGL(krylovStartKernel, dim3(nb), dim3(kBlock), 0, K.stream, K.w, K.n);
+    // ||v0|| in the fixed dot order (V's column 0 stands in for v0 while the dot runs)
+    HIPCHK(hipMemcpyAsync(K.V, K.w, uint64_t(K.n) * 8, hipMemcpyDeviceToDevice, K.stream));
+    K.dots(1, K.w, K.h);
+    double hh = 0.0;
+    HIPCHK(hipMemcpyAsync(&hh, K.h, 8, hipMemcpyDeviceToHost, K.stream));
+    HIPCHK(hipStreamSynchronize(K.stream));
+    const double beta = std::sqrt(hh);
+    if (!(beta > 0.0)) return setErr(B12X_E_ARG, "zero start vector");
+    HIPCHK(hipMemcpyAsync(K.h + kKrylovMaxCols, &beta, 8, hipMemcpyHostToDevice, K.stream));
+    hipLaunchKernelGGL(scaleIntoKernel, dim3(nb), dim3(kBlock), 0, K.stream, (const double*)K.w, K.n,
+                       (const double*)(K.h + kKrylovMaxCols), K.V);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(K.stream));
+    if (norm) *norm = beta;
+    return B12X_OK;
+  });
+}
+
+int b12x_krylov_expand(b12x_krylov k, int p, double* h, double* beta) {
+  return guarded([&]() -> int {
+    if (!k || !h || !beta || p < 0 || p >= k->ncv) return setErr(B12X_E_ARG, "p must lie in [0, ncv)");
+    b12x_krylov_s& K = *k;
+    const uint32_t n = K.n, nb = ceilDiv(n, kBlock), cols = uint32_t(p) + 1;
+    // w = v_p P
+    hipLaunchKernelGGL(spmmKernel<1>, dim3(nb), dim3(kBlock), 0, K.stream, (const uint32_t*)K.C.colPtr,
+                       (const uint32_t*)K.C.rows, (const double*)K.C.vals, (const double*)(K.V + uint64_t(p) * n), n,
+                       K.w);
+    // classical Gram-Schmidt, twice
+    double h1[kKrylovMaxCols], h2[kKrylovMaxCols], nn = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      K.dots(cols, K.w, K.h);
+      hipLaunchKernelGGL(colUpdateKernel, dim3(nb), dim3(kBlock), 0, K.stream, (const double*)K.V, cols,
+                         (const double*)K.h, K.w, n);
+      HIPCHK(hipMemcpyAsync(pass == 0 ? h1 : h2, K.h, cols * 8, hipMemcpyDeviceToHost, K.stream));
+    }
+    // ||w||, then v_{p+1} = w / ||w||
+    hipLaunchKernelGGL(colDotsKernel, dim3(K.nd), dim3(kBlock), 0, K.stream, (const double*)K.w, 0u, 1u,
+                       (const double*)K.w, n, K.part);
+    hipLaunchKernelGGL(colDotsReduceKernel, dim3(1), dim3(kBlock), 0, K.stream, (const double*)K.part, K.nd,
+                       K.h + kKrylovMaxCols);
+    HIPCHK(hipMemcpyAsync(&nn, K.h + kKrylovMaxCols, 8, hipMemcpyDeviceToHost, K.stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(K.stream));
+    const double b = std::sqrt(nn);
+    for (uint32_t c = 0; c < cols; ++c) h[c] = h1[c] + h2[c];
+    *beta = b;
+    if (b > 0.0) {
+      HIPCHK(hipMemcpyAsync(K.h + kKrylovMaxCols, &b, 8, hipMemcpyHostToDevice, K.stream));
+      hipLaunchKernelGGL(scaleIntoKernel, dim3(nb), dim3(kBlock), 0, K.stream, (const double*)K.w, n,
+                         (const double*)(K.h + kKrylovMaxCols), K.V + uint64_t(p + 1) * n);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(K.stream));
+    }
+    return B12X_OK;
+  });
+}
+
+int b12x_krylov_rotate(b12x_krylov k, int m, int kk, const double* q) {
+  return guarded([&]() -> int {
+    if (!k || !q || m < 1 || m > k->ncv || kk < 1 || kk > m) return setErr(B12X_E_ARG, "need 1 <= k <= m <= ncv");
+    b12x_krylov_s& K = *k;
+    const uint32_t n = K.n;
+    HIPCHK(hipMemcpyAsync(K.Q, q, uint64_t(m) * kk * 8, hipMemcpyHostToDevice, K.stream));
+    hipLaunchKernelGGL(krylovRotateKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, K.stream, K.V, n, m, kk,
+                       (const double*)K.Q);
+    HIPCHK(hipMemcpyAsync(K.V + uint64_t(kk) * n, K.V + uint64_t(m) * n, uint64_t(n) * 8, hipMemcpyDeviceToDevice,
+                          K.stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(K.stream));
+    return B12X_OK;
+  });
+}
+
+int b12x_krylov_vector(b12x_krylov k, int c, double* out) {
+  return guarded([&]() -> int {
+    if (!k || !out || c < 0 || c > k->ncv) return setErr(B12X_E_ARG, "column out of range");
+    HIPCHK(hipMemcpyAsync(out, k->V + uint64_t(c) * k->n, uint64_t(k->n) * 8, hipMemcpyDeviceToHost, k->stream));
+    HIPCHK(hipStreamSynchronize(k->stream));
     return B12X_OK;
   });
 }

@@ -2106,6 +2106,89 @@ static __global__ __launch_bounds__(kBlock) void ritzKernel(const double* part, 
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Krylov basis for the Krylov-Schur eigensolver (build-defined: the reference leaves its
+// eigenvalues to MATLAB eigs, subdivisionByNim.m:30-56). The basis is stored column by column
+// (V[c * n + i]); the operator is the row-vector product w = v P (w_j = sum over column j in
+// ascending row order, spmmKernel<1>). Every dot product over the boxes runs in the fixed order of
+// dotPartialKernel (block-strided partials, blockReduceSumF64, sumPartials), so the results
+// equal the oracle's restatement bit for bit.
+// ---------------------------------------------------------------------------------------------
+constexpr int kKrylovDotCols = 8;  // basis columns per dot-product pass
+constexpr int kKrylovMaxCols = 64; // basis vectors at most (ncv + 1)
+
+// block partials of x_c . w for the columns c0 <= c < c0 + cnt (cnt <= kKrylovDotCols) of V:
+// part[c * gridDim.x + block]
+static __global__ __launch_bounds__(kBlock) void colDotsKernel(const double* V, uint32_t c0, uint32_t cnt,
+                                                              const double* w, uint32_t n, double* part) {
+  double acc[kKrylovDotCols];
+#pragma unroll
+  for (int q = 0; q < kKrylovDotCols; ++q) acc[q] = 0.0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double wi = w[i];
+#pragma unroll
+    for (int q = 0; q < kKrylovDotCols; ++q)
+      if (uint32_t(q) < cnt) {
+        const double prod = V[uint64_t(c0 + q) * n + i] * wi;
+        acc[q] += prod;
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < kKrylovDotCols; ++q)
+    if (uint32_t(q) < cnt) {
+      const double v = blockReduceSumF64(acc[q]);
+      if (threadIdx.x == 0) part[uint64_t(c0 + q) * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// h[c] = the partials of column c summed in sumPartials' order: one block per column
+static __global__ __launch_bounds__(kBlock) void colDotsReduceKernel(const double* part, uint32_t nb, double* h) {
+  const double v = sumPartials(part + uint64_t(blockIdx.x) * nb, nb);
+  if (threadIdx.x == 0) h[blockIdx.x] = v;
+}
+
+// w_i = w_i - sum over c < cols of x_c[i] h_c, c ascending (one subtraction per column)
+static __global__ __launch_bounds__(kBlock) void colUpdateKernel(const double* V, uint32_t cols, const double* h,
+                                                                double* w, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double v = w[i];
+  for (uint32_t c = 0; c < cols; ++c) v -= V[uint64_t(c) * n + i] * h[c];
+  w[i] = v;
+}
+
+// out_i = w_i / *beta
+static __global__ __launch_bounds__(kBlock) void scaleIntoKernel(const double* w, uint32_t n, const double* beta,
+                                                                double* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = w[i] / *beta;
+}
+
+// V[:, c] = sum over d < m of V[:, d] Q[d * k + c] (d ascending, from 0.0) for c < k, in place:
+// each lane holds its box's m entries before it writes
+static __global__ __launch_bounds__(kBlock) void krylovRotateKernel(double* V, uint32_t n, int m, int k,
+                                                                   const double* Q) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double x[kKrylovMaxCols];
+#pragma unroll
+  for (int d = 0; d < kKrylovMaxCols; ++d) x[d] = d < m ? V[uint64_t(d) * n + i] : 0.0;
+  for (int c = 0; c < k; ++c) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < kKrylovMaxCols; ++d)
+      if (d < m) s += x[d] * Q[d * k + c];
+    V[uint64_t(c) * n + i] = s;
+  }
+}
+
+// the default start vector: v_i = (splitmix64(8 i + 1) >> 11) 2^-53 * 2 - 1 (the subspace
+// iteration's second start column)
+static __global__ __launch_bounds__(kBlock) void krylovStartKernel(double* v, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = double(splitmix64(8 * uint64_t(i) + 1) >> 11) * 0x1p-53 * 2.0 - 1.0;
+}
+
 // box-by-box basis to column-by-column (the C ABI's layout)
 static __global__ __launch_bounds__(kBlock) void transposeBasisKernel(const double* X, uint32_t n, int m,
                                                                      double* out) {
@@ -2114,10 +2197,9 @@ static __global__ __launch_bounds__(kBlock) void transposeBasisKernel(const doub
   for (int c = 0; c < m; ++c) out[uint64_t(c) * n + i] = X[uint64_t(i) * m + c];
 }
 
-// Plain copy (traffic-counter calibration): every byte read once and written once.
-// Eight independent FP64 multiply-add chains per lane (no memory traffic in the loop); block 0's
-// first lane records the shader-clock and 100 MHz real-time counters around its loop, so the
-// host can read the clock the device runs at under a full FP64 load.
+// Eight independent FP64 multiply-add chains per lane (no memory traffic in the loop); each
+// block's first lane records the shader-clock and 100 MHz real-time counters around its loop, so
+// the host can read the clock the device runs at under a full FP64 load.
 static __global__ __launch_bounds__(kBlock) void fp64ClockKernel(double* out, uint64_t* clk, int iters, double s) {
   double a0 = threadIdx.x, a1 = a0 + 1.0, a2 = a0 + 2.0, a3 = a0 + 3.0, a4 = a0 + 4.0, a5 = a0 + 5.0,
          a6 = a0 + 6.0, a7 = a0 + 7.0;
@@ -2134,6 +2216,7 @@ static __global__ __launch_bounds__(kBlock) void fp64ClockKernel(double* out, ui
   }
 }
 
+// Plain copy (traffic-counter calibration): every byte read once and written once.
 static __global__ __launch_bounds__(kBlock) void copyKernel(const uint4* in, uint4* out, uint64_t n) {
   for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock)
     out[i] = in[i];

@@ -31,6 +31,7 @@ extern "C" {
 typedef struct b12x_tree_s* b12x_tree;
 typedef struct b12x_scheme_s* b12x_scheme;
 typedef struct b12x_coo_s* b12x_coo;
+typedef struct b12x_krylov_s* b12x_krylov;
 
 enum {
   B12X_OK = 0,
@@ -314,6 +315,27 @@ int b12x_power_iteration_form(int* form);
  * timing. */
 int b12x_subspace_iteration(b12x_coo m, int nvec, int nlead, int max_iter, double tol, double* ritz, double* basis,
                             int* iters, double* resid);
+/* Krylov basis for a Krylov-Schur eigensolver (no counterpart in the reference, which leaves its
+ * eigenvalues to MATLAB eigs, subdivisionByNim.m:30-56; a host driver, eigs() in b12_amd/api.py,
+ * does the small dense steps). The operator is the row-vector product w = v P over the matrix's
+ * columns < n_rows (w_j = sum over column j in ascending row order, as the power iteration).
+ * Basis V: ncv + 1 (3 <= ncv + 1 <= 64) device vectors of n_rows doubles; the matrix may be
+ * destroyed after create. Every dot product over the boxes runs in one fixed order (block-strided
+ * partial sums, a fixed reduction tree), so results do not depend on timing.
+ *  - start: V[:, 0] = v0 / ||v0|| (v0: host, n_rows doubles; NULL: v0_i = (splitmix64(8 i + 1)
+ *    >> 11) 2^-53 * 2 - 1); *norm (may be NULL) ||v0||.
+ *  - expand: w = V[:, p] P, then classical Gram-Schmidt against V[:, 0..p] twice (h1 = V^T w,
+ *    w -= V h1 with one subtraction per column in column order; h2 likewise); h[c] = h1[c] + h2[c]
+ *    (p + 1 values), *beta = ||w||, and V[:, p + 1] = w / ||w|| unless ||w|| = 0. 0 <= p < ncv.
+ *  - rotate: V[:, c] = sum over d < m of V[:, d] q[d * k + c] (d ascending) for c < k, then
+ *    V[:, k] = V[:, m]. q: host, m x k row-major; 1 <= k <= m <= ncv.
+ *  - vector: V[:, c] to the host (n_rows doubles), 0 <= c <= ncv. */
+int b12x_krylov_create(b12x_coo m, int ncv, b12x_krylov* out);
+int b12x_krylov_destroy(b12x_krylov k);
+int b12x_krylov_start(b12x_krylov k, const double* v0, double* norm);
+int b12x_krylov_expand(b12x_krylov k, int p, double* h, double* beta);
+int b12x_krylov_rotate(b12x_krylov k, int m, int kk, const double* q);
+int b12x_krylov_vector(b12x_krylov k, int c, double* out);
 /* Strong components of the matrix's graph (an edge row -> column per entry; entries naming a
  * vertex >= n_rows, such as the column of points outside the cover, are left out), as
  * chainRecurrentSet.m:26-39 obtains them from MATLAB's graphconncomp. labels (n_rows entries,

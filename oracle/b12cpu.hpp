@@ -1279,7 +1279,7 @@ inline int powerIteration(const Coo& P, int maxIter, double tol, std::vector<dou
   NrBoxes n = P.nRows;
   std::vector<size_t> order(P.rows.size());
   std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
     return P.cols[a] != P.cols[b] ? P.cols[a] < P.cols[b] : P.rows[a] < P.rows[b];
   });
   std::vector<NrBoxes> cptr(size_t(n) + 1, 0);
@@ -1478,7 +1478,7 @@ inline int subspaceIteration(const Coo& P, int m, int lead, int maxIter, double 
   const NrBoxes n = P.nRows;
   std::vector<size_t> order(P.rows.size());
   std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
     return P.cols[a] != P.cols[b] ? P.cols[a] < P.cols[b] : P.rows[a] < P.rows[b];
   });
   std::vector<NrBoxes> cptr(size_t(n) + 1, 0);
@@ -1519,6 +1519,93 @@ inline int subspaceIteration(const Coo& P, int m, int lead, int maxIter, double 
     X.swap(Y);
   }
 }
+
+// ---------------------------------------------------------------------------------------------
+// Krylov basis of the Krylov-Schur eigensolver (build-defined; the reference leaves eigenvalues
+// to MATLAB eigs, subdivisionByNim.m:30-56). The driver (eigs() in b12_amd/api.py) keeps the
+// small projected matrix; the basis operations over the boxes are defined here:
+//   operator: w = v P, w_j = sum over column j in ascending row order (from 0.0);
+//   start: V_0 = v0 / ||v0||, ||v0|| = sqrt(S(v0_i v0_i)), default v0_i = u(i, 1) (subspaceStart);
+//   expand(p): w = V_p P; twice: h^(q)_c = S(V_c[i] w_i) for c <= p, then w_i -= V_c[i] h^(q)_c
+//     for c = 0..p in order; h_c = h^(1)_c + h^(2)_c; beta = sqrt(S(w_i w_i)); V_{p+1} = w / beta
+//     (left as it was when beta = 0);
+//   rotate(m, k, q): V_c[i] = sum over d < m of V_d[i] q[d k + c] (from 0.0, d ascending) for c < k,
+//     then V_k = V_m.
+// S is stridedSum's fixed order. V is stored column by column (V[c * n + i]).
+// ---------------------------------------------------------------------------------------------
+struct KrylovBasis {
+  NrBoxes n = 0;
+  int ncv = 0;
+  std::vector<NrBoxes> cptr, ri;
+  std::vector<double> va, V, w;
+
+  KrylovBasis(const Coo& P, int ncvIn) : n(P.nRows), ncv(ncvIn) {
+    std::vector<size_t> order(P.rows.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+      return P.cols[a] != P.cols[b] ? P.cols[a] < P.cols[b] : P.rows[a] < P.rows[b];
+    });
+    cptr.assign(size_t(n) + 1, 0);
+    for (size_t e : order) if (P.cols[e] < n) ++cptr[P.cols[e] + 1];
+    for (NrBoxes j = 0; j < n; ++j) cptr[j + 1] += cptr[j];
+    for (size_t e : order)
+      if (P.cols[e] < n) {
+        ri.push_back(P.rows[e]);
+        va.push_back(P.vals[e]);
+      }
+    V.assign(size_t(ncv + 1) * n, 0.0);
+    w.assign(n, 0.0);
+  }
+  double* col(int c) { return V.data() + size_t(c) * n; }
+
+  double start(const double* v0) {
+    for (NrBoxes i = 0; i < n; ++i) w[i] = v0 ? v0[i] : subspaceStart(i, 1);
+    const double beta = std::sqrt(stridedSum(n, [&](size_t i) { return w[i] * w[i]; }));
+    if (beta > 0.0)
+      for (NrBoxes i = 0; i < n; ++i) col(0)[i] = w[i] / beta;
+    return beta;
+  }
+
+  double expand(int p, double* h) {
+    const double* v = col(p);
+    for (NrBoxes j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (NrBoxes e = cptr[j]; e < cptr[j + 1]; ++e) s += v[ri[e]] * va[e];
+      w[j] = s;
+    }
+    std::vector<double> h1(size_t(p) + 1), h2(size_t(p) + 1);
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<double>& hq = pass == 0 ? h1 : h2;
+      for (int c = 0; c <= p; ++c) {
+        const double* x = col(c);
+        hq[c] = stridedSum(n, [&](size_t i) { return x[i] * w[i]; });
+      }
+      for (NrBoxes i = 0; i < n; ++i) {
+        double t = w[i];
+        for (int c = 0; c <= p; ++c) t -= col(c)[i] * hq[c];
+        w[i] = t;
+      }
+    }
+    for (int c = 0; c <= p; ++c) h[c] = h1[c] + h2[c];
+    const double beta = std::sqrt(stridedSum(n, [&](size_t i) { return w[i] * w[i]; }));
+    if (beta > 0.0)
+      for (NrBoxes i = 0; i < n; ++i) col(p + 1)[i] = w[i] / beta;
+    return beta;
+  }
+
+  void rotate(int m, int k, const double* q) {
+    std::vector<double> x(m);
+    for (NrBoxes i = 0; i < n; ++i) {
+      for (int d = 0; d < m; ++d) x[d] = col(d)[i];
+      for (int c = 0; c < k; ++c) {
+        double s = 0.0;
+        for (int d = 0; d < m; ++d) s += x[d] * q[d * k + c];
+        col(c)[i] = s;
+      }
+    }
+    std::copy(col(m), col(m) + n, col(k));
+  }
+};
 
 // ---------------------------------------------------------------------------------------------
 // Strongly connected components of the graph of a matrix (an edge row -> column per entry;

@@ -445,6 +445,42 @@ int b12cpu_subspace_iteration(void* c, int m, int lead, int maxIter, double tol,
   return 0;
 }
 
+// Krylov basis (KrylovBasis in b12cpu.hpp): the same operations as b12x_krylov_*
+int b12cpu_krylov_create(void* c, int ncv, void** out) {
+  const Coo& P = static_cast<CooH*>(c)->m;
+  if (ncv < 2 || ncv + 1 > 64 || P.nRows < NrBoxes(ncv) + 1) return fail("krylov: 2 <= ncv <= 63, ncv < nRows");
+  *out = new KrylovBasis(P, ncv);
+  return 0;
+}
+int b12cpu_krylov_destroy(void* k) {
+  delete static_cast<KrylovBasis*>(k);
+  return 0;
+}
+int b12cpu_krylov_start(void* k, const double* v0, double* norm) {
+  const double b = static_cast<KrylovBasis*>(k)->start(v0);
+  if (!(b > 0.0)) return fail("krylov: zero start vector");
+  if (norm) *norm = b;
+  return 0;
+}
+int b12cpu_krylov_expand(void* k, int p, double* h, double* beta) {
+  auto* K = static_cast<KrylovBasis*>(k);
+  if (p < 0 || p >= K->ncv) return fail("krylov: p must lie in [0, ncv)");
+  *beta = K->expand(p, h);
+  return 0;
+}
+int b12cpu_krylov_rotate(void* k, int m, int kk, const double* q) {
+  auto* K = static_cast<KrylovBasis*>(k);
+  if (m < 1 || m > K->ncv || kk < 1 || kk > m) return fail("krylov: need 1 <= k <= m <= ncv");
+  K->rotate(m, kk, q);
+  return 0;
+}
+int b12cpu_krylov_vector(void* k, int c, double* out) {
+  auto* K = static_cast<KrylovBasis*>(k);
+  if (c < 0 || c > K->ncv) return fail("krylov: column out of range");
+  std::copy(K->col(c), K->col(c) + K->n, out);
+  return 0;
+}
+
 // labels (nRows entries): the largest vertex index of the vertex's strong component; counts of
 // components and of components with more than one vertex
 int b12cpu_coo_scc(void* c, uint32_t* labels, uint32_t* nComp, uint32_t* nBig) {

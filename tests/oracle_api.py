@@ -33,6 +33,12 @@ def lib():
         _lib.b12cpu_set_threads.argtypes = [ctypes.c_int]
         _lib.b12cpu_subspace_iteration.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P,
                                                    _P, _P, _P]
+        _lib.b12cpu_krylov_create.argtypes = [_P, ctypes.c_int, _P]
+        _lib.b12cpu_krylov_destroy.argtypes = [_P]
+        _lib.b12cpu_krylov_start.argtypes = [_P, _P, _P]
+        _lib.b12cpu_krylov_expand.argtypes = [_P, ctypes.c_int, _P, _P]
+        _lib.b12cpu_krylov_rotate.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P]
+        _lib.b12cpu_krylov_vector.argtypes = [_P, ctypes.c_int, _P]
     return _lib
 
 
@@ -135,9 +141,50 @@ class OCoo:
                                              _ptr(X) if want_basis else None, ctypes.byref(it), ctypes.byref(res)))
         return H, X, it.value, res.value
 
+    def krylov(self, ncv):
+        """The oracle's Krylov basis (KrylovBasis) for eigs()."""
+        return OKrylov(self, ncv)
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().b12cpu_coo_destroy(self.h)
+
+
+class OKrylov:
+    """The oracle's Krylov basis: the operations of b12_amd.KrylovBasis, restated on the CPU."""
+
+    def __init__(self, matrix, ncv):
+        self.n, self.ncv = matrix.nRows, int(ncv)
+        h = _P()
+        _chk(lib().b12cpu_krylov_create(matrix.h, self.ncv, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().b12cpu_krylov_destroy(self.h)
+            self.h = None
+
+    def start(self, v0=None):
+        norm = ctypes.c_double()
+        if v0 is not None:
+            v0 = np.ascontiguousarray(v0, np.float64)
+        _chk(lib().b12cpu_krylov_start(self.h, _ptr(v0), ctypes.byref(norm)))
+        return norm.value
+
+    def expand(self, p):
+        h = np.zeros(p + 1, np.float64)
+        beta = ctypes.c_double()
+        _chk(lib().b12cpu_krylov_expand(self.h, int(p), _ptr(h), ctypes.byref(beta)))
+        return h, beta.value
+
+    def rotate(self, m, k, q):
+        q = np.ascontiguousarray(q, np.float64)
+        _chk(lib().b12cpu_krylov_rotate(self.h, int(m), int(k), _ptr(q)))
+
+    def vector(self, c):
+        out = np.empty(self.n, np.float64)
+        _chk(lib().b12cpu_krylov_vector(self.h, int(c), _ptr(out)))
+        return out
 
 
 def set_threads(n):

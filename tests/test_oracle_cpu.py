@@ -1203,3 +1203,95 @@ def test_json_stream_helper_writes_raw_uint8_bytes():
     assert back.split() == [b"8"]  # the tab (SD|HIT) is lost as whitespace
     back = subprocess.run([exe, "read", "depthVector"], input=out, capture_output=True, check=True).stdout
     assert back.split() == [b"20", b"20"]
+
+
+def _substochastic(rng, n, density, blocks=4):
+    """A random sparse row-substochastic matrix with `blocks` weakly coupled diagonal blocks."""
+    A = rng.random((n, n)) * (rng.random((n, n)) < density)
+    s = n // blocks
+    for b in range(blocks):
+        A[b * s:(b + 1) * s, b * s:(b + 1) * s] += 3.0 * rng.random((s, s)) * (rng.random((s, s)) < 4 * density)
+    P = 0.97 * A / np.maximum(A.sum(axis=1, keepdims=True), 1e-300)
+    r, c = np.nonzero(P)
+    return P, r, c
+
+
+def test_krylov_basis_operations():
+    """The oracle's Krylov basis: start normalises v0; each expansion gives the Arnoldi relation
+    V_p^T P ... in row-vector form, w = v_p P = V_{p+1} h + beta v_{p+1}, with V orthonormal to
+    rounding; rotate forms V Q and moves the residual vector. Checked with numpy from the vectors
+    the basis returns (not by restating its sums)."""
+    rng = np.random.default_rng(5)
+    n, ncv = 300, 12
+    P, r, c = _substochastic(rng, n, 0.05)
+    m = OCoo.from_arrays(r, c, P[r, c], n, n)
+    K = m.krylov(ncv)
+    v0 = rng.standard_normal(n)
+    nrm = K.start(v0)
+    assert np.isclose(nrm, np.linalg.norm(v0), rtol=1e-14)
+    assert np.allclose(K.vector(0), v0 / np.linalg.norm(v0), rtol=0, atol=1e-15)
+    for p in range(ncv):
+        vp = K.vector(p)
+        h, beta = K.expand(p)
+        V = np.stack([K.vector(q) for q in range(p + 2)])
+        assert np.allclose(V @ V.T, np.eye(p + 2), atol=1e-13)
+        w = vp @ P
+        assert np.allclose(w, h @ V[:p + 1] + beta * V[p + 1], rtol=0, atol=1e-14)
+        assert np.allclose(h, V[:p + 1] @ w, rtol=0, atol=1e-14)
+    Q, _ = np.linalg.qr(rng.standard_normal((ncv, 5)))
+    V = np.stack([K.vector(q) for q in range(ncv + 1)])
+    K.rotate(ncv, 5, Q)
+    assert np.allclose(np.stack([K.vector(q) for q in range(5)]), Q.T @ V[:ncv], rtol=0, atol=1e-14)
+    assert np.array_equal(K.vector(5), V[ncv])
+    # the default start: the subspace iteration's second start column, normalised
+    K.start(None)
+    u = np.array([((_splitmix64(8 * i + 1) >> 11) * 2.0 ** -53) * 2.0 - 1.0 for i in range(n)])
+    assert np.allclose(K.vector(0), u / np.linalg.norm(u), rtol=0, atol=1e-15)
+
+
+def _splitmix64(x):
+    M = (1 << 64) - 1
+    x = (x + 0x9e3779b97f4a7c15) & M
+    x = ((x ^ (x >> 30)) * 0xbf58476d1ce4e5b9) & M
+    x = ((x ^ (x >> 27)) * 0x94d049bb133111eb) & M
+    return x ^ (x >> 31)
+
+
+def test_eigs_directed_lazy_cycle_closed_form():
+    """Lazy walk around a directed 60-cycle, P = (1 - a) I + a C: eigenvalues (1 - a) + a w^k,
+    w = exp(2 pi i / 60), all simple. The leading by modulus are 1, then the k = +-1 pair, then
+    the k = +-2 pair: nev = 4 cuts that pair, so both of it are returned (5 values)."""
+    n, a = 60, 0.3
+    rows = np.concatenate([np.arange(n), np.arange(n)])
+    cols = np.concatenate([np.arange(n), (np.arange(n) + 1) % n])
+    vals = np.concatenate([np.full(n, 1 - a), np.full(n, a)])
+    o = np.lexsort((cols, rows))
+    m = OCoo.from_arrays(rows[o], cols[o], vals[o], n, n)
+    res = gx.eigs(m, nev=4, ncv=30, tol=1e-12)
+    assert res["converged"]
+    lam = (1 - a) + a * np.exp(2j * np.pi * np.array([0, 1, -1, 2, -2]) / n)
+    got = res["eigenvalues"]
+    assert len(got) == 5
+    for z in lam:
+        assert np.min(np.abs(got - z)) < 1e-11
+    assert res["residual"] <= 1e-12 and np.all(res["pair_residuals"] <= 1e-10)
+
+
+def test_eigs_matches_scipy_on_random_substochastic_matrices():
+    """The four (or five, with a cut pair) leading eigenvalues of random sparse substochastic
+    matrices with almost-invariant blocks equal scipy's eigs (ARPACK) to 1e-9, and the returned
+    Ritz vectors are left eigenvectors to the stated residual."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    for seed, n in ((11, 400), (12, 1500)):
+        rng = np.random.default_rng(seed)
+        P, r, c = _substochastic(rng, n, 8.0 / n)
+        m = OCoo.from_arrays(r, c, P[r, c], n, n)
+        res = gx.eigs(m, nev=4, tol=1e-11, want_vectors=True)
+        assert res["converged"]
+        w = sla.eigs(sp.csr_matrix(P).T.tocsr(), k=8, which="LM", tol=1e-14, maxiter=100000)[0]
+        w = w[np.argsort(-np.abs(w))]
+        for z in res["eigenvalues"]:
+            assert np.min(np.abs(w - z)) <= 1e-9 * abs(z)
+        for z, v in zip(res["eigenvalues"], res["vectors"]):
+            assert np.linalg.norm(v @ P - z * v) <= 1e-10 * abs(z) * np.linalg.norm(v)
