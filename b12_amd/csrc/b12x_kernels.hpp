@@ -1441,7 +1441,7 @@ static __global__ __launch_bounds__(kWave) void piReduceKernel(const double* par
 // published in the next iteration, while its gathers are in flight.
 // The aggregator sums the group partials of iteration tau in the chunk order and publishes
 // (nu_tau, r_{tau - L}) in totals slot tau % kFlowTot (kFlowCopies copies, one 128-byte line
-// each; wave w of block b polls copy (4 b + w) % kFlowCopies). The stopping decision for
+// each; wave w of block b polls copy (4 b + w) % kFlowCopies), two iterations at a time. The stopping decision for
 // iteration f = tau - L is taken once r_f is in the totals: the aggregator records it and
 // leaves, and every worker wave reads the same totals at its iteration f + 2 L and leaves
 // there, before z_f is cleared (iteration f + kFlowClear), so piFinishKernel forms z_f / nu_f.
@@ -1452,25 +1452,26 @@ static __global__ __launch_bounds__(kWave) void piReduceKernel(const double* par
 //    the slot has seen z_{x-1} of the same lane, which was stored after the lane's store counter
 //    drained past the clear (issued in iteration x - kFlowRing + D <= x - 2), so it finds empty
 //    or z_x.
-//  - partials: the aggregator clears position (tau + L + 1) % kFlowPart (partials of
-//    tau + L + 1 - kFlowPart <= tau - 1, consumed) after publishing tau; a wave writes the
-//    partials of tau + L + 1 only after seeing the totals of tau + 1, published after the clears
-//    of tau drained.
-//  - totals: the aggregator clears slot (tau + 2) % kFlowTot (totals of tau + 2 - kFlowTot,
-//    read by every wave in iteration tau + 2 - kFlowTot + L <= tau, before it published the
-//    partials of tau) after publishing tau; a wave reading the totals of tau + 2 has seen those of
-//    tau + 1, published after the clear drained.
+//  - partials: the aggregator takes iterations in pairs (tau, tau + 1) and clears their positions
+//    after publishing the pair's totals; the clears drain before the next pair's totals are
+//    published, and a wave writes the partials of tau + 8 (tau + 9) only after seeing the totals
+//    of tau + 2 (tau + 3).
+//  - totals: after publishing the pair it also clears the slots of the totals of tau - 6 and
+//    tau - 5 (every wave read them before it published the partials of tau and tau + 1); their
+//    next totals, tau + 10 and tau + 11, are published five pairs later, and a wave reading them
+//    has seen those of tau + 9, published after the clears drained.
 // Everything written during the run goes through agent-scope relaxed atomics, coherent across
 // XCDs without L2 write-back or invalidation; the matrix is read-only and stays cached. Every
 // wait is bounded: past spinLimit polls a wave raises *err and leaves, the others leave at their
 // next poll, and the host recomputes with the batched form.
 // ---------------------------------------------------------------------------------------------
 constexpr uint32_t kFlowMaxWorkers = 512;
-constexpr uint32_t kFlowRing = 16, kFlowLag = 6, kFlowClear = 2 * kFlowLag, kFlowPart = 8, kFlowTot = 8;
+constexpr uint32_t kFlowRing = 16, kFlowLag = 6, kFlowClear = 2 * kFlowLag, kFlowPart = 8, kFlowTot = 16;
 static_assert(kFlowClear >= 2 * kFlowLag, "z_f must outlive the workers");
 static_assert(kFlowClear + 2 <= kFlowRing, "a clear drains before the slot's next value");
 static_assert(kFlowLag + 1 <= kFlowClear, "readers of a cleared value have finished");
-static_assert(kFlowPart >= kFlowLag + 1 && kFlowTot >= kFlowLag + 2, "partial and totals reuse");
+static_assert(kFlowPart >= kFlowLag + 2, "partial reuse: positions of a pair are rewritten after the next pair");
+static_assert(kFlowTot >= kFlowLag + 10, "totals reuse: a cleared slot's next totals come five pairs later");
 static_assert(kFlowLag <= 8, "c_t derives from nu_{t-8}, which the totals history holds");
 constexpr uint32_t kFlowCopies = 16, kFlowStride = 16;  // u64 words per totals copy: one 128-byte line
 constexpr uint32_t kFlowTotWords = kFlowTot * kFlowCopies * kFlowStride;
@@ -1576,41 +1577,55 @@ __device__ inline double flowCachedSum(const uint64_t (&m)[kCoopCache], const do
   return c != 1.0 ? s * c : s;
 }
 
+// The aggregator takes iterations two at a time (tau, tau + 1): it polls both iterations' group
+// partials together (one loaded round trip for the pair), sums each in the chunk order, drains its
+// earlier clears, publishes (nu_tau, r_{tau-L}) and then tau + 1's, and only then clears the two
+// consumed partial positions and the slots of the totals of tau - 6 and tau - 5 (the reuse rules
+// above). The clears drain while the next pair's loads are in flight. The stopping decision is
+// taken in iteration order; the totals a worker needs for iteration t (those of t - L) depend on
+// partials of iterations up to t - L + 1, which every worker has published by iteration t.
 __device__ inline void piFlowAggregator(const PiFlowArgs& a) {
   __shared__ double nuRing[kFlowTot];
   const uint32_t lane = threadIdx.x;
   const uint32_t nG = a.nGroups, nChunks = (nG + 3) / 4;
-  for (uint32_t tau = 1;; ++tau) {
-    const uint64_t* buf = a.part + uint64_t(tau % kFlowPart) * nG * 2;
-    double ta = 0.0, tr = 0.0;
+  for (uint32_t tau0 = 1;; tau0 += 2) {
+    double ta[2] = {0.0, 0.0}, tr[2] = {0.0, 0.0};
     // lane l sums chunks l, l + 64, ... in order; a chunk's sum adds its four group partials to
-    // 0.0 in order. kFlowAggChunks chunks per lane are polled together.
+    // 0.0 in order. kFlowAggChunks chunks per lane are polled together, for both iterations.
     for (uint32_t k0 = 0; k0 * kWave < nChunks; k0 += kFlowAggChunks) {
-      uint64_t v[kFlowAggChunks][8];
+      uint64_t v[2][kFlowAggChunks][8];
 #pragma unroll
-      for (uint32_t k = 0; k < kFlowAggChunks; ++k)
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int x = 0; x < 8; ++x) v[k][x] = 0;
+        for (uint32_t k = 0; k < kFlowAggChunks; ++k)
+#pragma unroll
+          for (int x = 0; x < 8; ++x) v[p][k][x] = 0;
       for (uint32_t spin = 0;; ++spin) {
         bool miss = false;
 #pragma unroll
-        for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
-          const uint32_t q = lane + (k0 + k) * kWave;
+        for (int p = 0; p < 2; ++p) {
+          const uint64_t* buf = a.part + uint64_t((tau0 + p) % kFlowPart) * nG * 2;
 #pragma unroll
-          for (int x = 0; x < 8; ++x) {
-            const uint32_t g = 4 * q + uint32_t(x / 2);
-            if (q < nChunks && g < nG) {
-              if (spin == 0 || v[k][x] == kCoopEmpty) v[k][x] = ldAgent(buf + uint64_t(g) * 2 + (x & 1));
+          for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
+            const uint32_t q = lane + (k0 + k) * kWave;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+              const uint32_t g = 4 * q + uint32_t(x / 2);
+              if (q < nChunks && g < nG) {
+                if (spin == 0 || v[p][k][x] == kCoopEmpty) v[p][k][x] = ldAgent(buf + uint64_t(g) * 2 + (x & 1));
+              }
             }
           }
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
-          const uint32_t q = lane + (k0 + k) * kWave;
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int x = 0; x < 8; ++x)
-            miss = miss || (q < nChunks && 4 * q + uint32_t(x / 2) < nG && v[k][x] == kCoopEmpty);
-        }
+          for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
+            const uint32_t q = lane + (k0 + k) * kWave;
+#pragma unroll
+            for (int x = 0; x < 8; ++x)
+              miss = miss || (q < nChunks && 4 * q + uint32_t(x / 2) < nG && v[p][k][x] == kCoopEmpty);
+          }
         if (!__any(miss)) break;
         if (spin >= a.spinLimit || ldAgent(a.err)) {
           if (lane == 0) flowAbort(a);
@@ -1619,54 +1634,67 @@ __device__ inline void piFlowAggregator(const PiFlowArgs& a) {
         __builtin_amdgcn_s_sleep(1);
       }
 #pragma unroll
-      for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
-        const uint32_t q = lane + (k0 + k) * kWave;
-        if (q < nChunks) {
-          double ca = 0.0, cr = 0.0;
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int w = 0; w < 4; ++w)
-            if (4 * q + uint32_t(w) < nG) {
-              ca += asF64(v[k][2 * w]);
-              cr += asF64(v[k][2 * w + 1]);
-            }
-          ta += ca;
-          tr += cr;
+        for (uint32_t k = 0; k < kFlowAggChunks; ++k) {
+          const uint32_t q = lane + (k0 + k) * kWave;
+          if (q < nChunks) {
+            double ca = 0.0, cr = 0.0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+              if (4 * q + uint32_t(w) < nG) {
+                ca += asF64(v[p][k][2 * w]);
+                cr += asF64(v[p][k][2 * w + 1]);
+              }
+            ta[p] += ca;
+            tr[p] += cr;
+          }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      ta[p] = waveButterflySum(ta[p]);
+      tr[p] = waveButterflySum(tr[p]);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the clears of the previous pair have completed
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const uint32_t tau = tau0 + p;
+      if (lane == 0) nuRing[tau % kFlowTot] = ta[p];
+      if (lane < kFlowCopies) {
+        uint64_t* tc = a.tot + (uint64_t(tau % kFlowTot) * kFlowCopies + lane) * kFlowStride;
+        stAgent(tc, asU64(ta[p]));
+        stAgent(tc + 1, asU64(tr[p]));
+      }
+      if (tau > kFlowLag) {
+        const uint32_t f = tau - kFlowLag;
+        if ((a.tol >= 0.0 && tr[p] <= a.tol) || f >= a.maxIter) {
+          __builtin_amdgcn_wave_barrier();
+          if (lane == 0) {
+            a.st->final = f;
+            a.st->resid = tr[p];
+            a.st->nuFinal = nuRing[f % kFlowTot];
+            a.st->slot = f % kFlowRing;
+            a.st->done = 1;
+          }
+          return;
         }
       }
     }
-    ta = waveButterflySum(ta);
-    tr = waveButterflySum(tr);
-    if (lane == 0) nuRing[tau % kFlowTot] = ta;
-    __builtin_amdgcn_s_waitcnt(0);  // the clears of the previous iterations have completed
-    if (lane < kFlowCopies) {
-      uint64_t* tc = a.tot + (uint64_t(tau % kFlowTot) * kFlowCopies + lane) * kFlowStride;
-      stAgent(tc, asU64(ta));
-      stAgent(tc + 1, asU64(tr));
-    }
-    if (tau > kFlowLag) {
-      const uint32_t f = tau - kFlowLag;
-      if ((a.tol >= 0.0 && tr <= a.tol) || f >= a.maxIter) {
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
-          a.st->final = f;
-          a.st->resid = tr;
-          a.st->nuFinal = nuRing[f % kFlowTot];
-          a.st->slot = f % kFlowRing;
-          a.st->done = 1;
-        }
-        return;
+    // clear the consumed partials of tau0 and tau0 + 1 and the totals of tau0 - 6 and tau0 - 5
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      uint64_t* pc = a.part + uint64_t((tau0 + p) % kFlowPart) * nG * 2;
+      for (uint32_t g = lane; g < nG; g += kWave) {
+        stAgent(pc + uint64_t(g) * 2, kCoopEmpty);
+        stAgent(pc + uint64_t(g) * 2 + 1, kCoopEmpty);
       }
-    }
-    // clear the partials of tau - 3 and the totals of tau - 6 for their next use
-    uint64_t* pc = a.part + uint64_t((tau + kFlowLag + 1) % kFlowPart) * nG * 2;
-    for (uint32_t g = lane; g < nG; g += kWave) {
-      stAgent(pc + uint64_t(g) * 2, kCoopEmpty);
-      stAgent(pc + uint64_t(g) * 2 + 1, kCoopEmpty);
-    }
-    if (lane < kFlowCopies) {
-      uint64_t* tc = a.tot + (uint64_t((tau + 2) % kFlowTot) * kFlowCopies + lane) * kFlowStride;
-      stAgent(tc, kCoopEmpty);
-      stAgent(tc + 1, kCoopEmpty);
+      if (lane < kFlowCopies && tau0 + p > kFlowLag) {
+        uint64_t* tc =
+            a.tot + (uint64_t((tau0 + p - kFlowLag) % kFlowTot) * kFlowCopies + lane) * kFlowStride;
+        stAgent(tc, kCoopEmpty);
+        stAgent(tc + 1, kCoopEmpty);
+      }
     }
   }
 }
