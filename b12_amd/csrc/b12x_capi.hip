@@ -3078,20 +3078,36 @@ int b12x_subspace_iteration(b12x_coo m, int nvec, int nlead, int max_iter, doubl
   });
 }
 
+// L(NC) for the basis-pass width of cols columns (1 <= cols <= 64): cols rounded up to 8
+#define B12X_BASIS_WIDTH(cols, L) \
+  do {                            \
+    switch (((cols) + 7) / 8) {   \
+      case 1: L(8); break;        \
+      case 2: L(16); break;       \
+      case 3: L(24); break;       \
+      case 4: L(32); break;       \
+      case 5: L(40); break;       \
+      case 6: L(48); break;       \
+      case 7: L(56); break;       \
+      default: L(64); break;      \
+    }                             \
+  } while (0)
+
 // Krylov basis of the Krylov-Schur eigensolver: the matrix's columns < n in CSC form, the basis
 // V (ncv + 1 vectors of n, column by column), the next vector w and the dot partials, on a
 // stream of its own (the matrix may be destroyed first).
 struct b12x_krylov_s {
   uint32_t n = 0;
   int ncv = 0;
-  uint32_t nd = 0;  // dot-product blocks: min(ceil(n / 256), kDotBlocks)
+  uint32_t nd = 0;  // blocks of the basis passes: min(ceil(n / 256), kDotBlocks)
   Csc C;
   double* V = nullptr;
   double* w = nullptr;
   double* part = nullptr;  // kKrylovMaxCols * nd
-  double* h = nullptr;     // kKrylovMaxCols + 1 results
+  double* h = nullptr;     // kHSize: the two Gram-Schmidt coefficient sets, w . w, ||w||
   double* Q = nullptr;     // kKrylovMaxCols^2 rotation
   hipStream_t stream = nullptr;
+  static constexpr int kH2 = kKrylovMaxCols, kNorm2 = 2 * kKrylovMaxCols, kBeta = kNorm2 + 1, kHSize = kBeta + 1;
   ~b12x_krylov_s() {
     streamRetire(stream);
     dfree(C.keys);
@@ -3106,12 +3122,23 @@ struct b12x_krylov_s {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  // h[0..cols) = x_c . v over the first cols basis columns (v: n doubles on the device)
+  // out[0..cols) = x_c . v over the first cols basis columns (v: n doubles on the device)
   void dots(uint32_t cols, const double* v, double* out) {
-    for (uint32_t c0 = 0; c0 < cols; c0 += kKrylovDotCols)
-      hipLaunchKernelGGL(colDotsKernel, dim3(nd), dim3(kBlock), 0, stream, (const double*)V, c0,
-                         std::min<uint32_t>(kKrylovDotCols, cols - c0), v, n, part);
+#define B12X_DOTS(NC) hipLaunchKernelGGL(colDotsKernel<NC>, dim3(nd), dim3(kBlock), 0, stream, (const double*)V, cols, v, n, part)
+    B12X_BASIS_WIDTH(cols, B12X_DOTS);
+#undef B12X_DOTS
     hipLaunchKernelGGL(colDotsReduceKernel, dim3(cols), dim3(kBlock), 0, stream, (const double*)part, nd, out);
+  }
+
+  // w -= V[:, 0..cols) hc; then out[0..cols) = x_c . w, or out[0] = w . w with norm
+  void update(uint32_t cols, const double* hc, double* out, bool norm) {
+#define B12X_UPDATE(NC) hipLaunchKernelGGL((colUpdateKernel<NC, false>), dim3(nd), dim3(kBlock), 0, stream, (const double*)V, cols, hc, w, n, part)
+#define B12X_UPDATE_NORM(NC) hipLaunchKernelGGL((colUpdateKernel<NC, true>), dim3(nd), dim3(kBlock), 0, stream, (const double*)V, cols, hc, w, n, part)
+    if (norm) B12X_BASIS_WIDTH(cols, B12X_UPDATE_NORM);
+    else B12X_BASIS_WIDTH(cols, B12X_UPDATE);
+#undef B12X_UPDATE
+#undef B12X_UPDATE_NORM
+    hipLaunchKernelGGL(colDotsReduceKernel, dim3(norm ? 1u : cols), dim3(kBlock), 0, stream, (const double*)part, nd, out);
   }
 };
 
@@ -3131,7 +3158,7 @@ int b12x_krylov_create(b12x_coo m, int ncv, b12x_krylov* out) {
     K->V = dalloc<double>(uint64_t(ncv + 1) * n);
     K->w = dalloc<double>(n);
     K->part = dalloc<double>(uint64_t(kKrylovMaxCols) * K->nd);
-    K->h = dalloc<double>(kKrylovMaxCols + 1);
+    K->h = dalloc<double>(b12x_krylov_s::kHSize);
     K->Q = dalloc<double>(uint64_t(kKrylovMaxCols) * kKrylovMaxCols);
     *out = K.release();
     return B12X_OK;
@@ -3158,9 +3185,9 @@ int b12x_krylov_start(b12x_krylov k, const double* v0, double* norm) {
     HIPCHK(hipStreamSynchronize(K.stream));
     const double beta = std::sqrt(hh);
     if (!(beta > 0.0)) return setErr(B12X_E_ARG, "zero start vector");
-    HIPCHK(hipMemcpyAsync(K.h + kKrylovMaxCols, &beta, 8, hipMemcpyHostToDevice, K.stream));
+    HIPCHK(hipMemcpyAsync(K.h + K.kBeta, &beta, 8, hipMemcpyHostToDevice, K.stream));
     hipLaunchKernelGGL(scaleIntoKernel, dim3(nb), dim3(kBlock), 0, K.stream, (const double*)K.w, K.n,
-                       (const double*)(K.h + kKrylovMaxCols), K.V);
+                       (const double*)(K.h + K.kBeta), K.V);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(K.stream));
     if (norm) *norm = beta;
@@ -3177,29 +3204,23 @@ int b12x_krylov_expand(b12x_krylov k, int p, double* h, double* beta) {
     hipLaunchKernelGGL(spmmKernel<1>, dim3(nb), dim3(kBlock), 0, K.stream, (const uint32_t*)K.C.colPtr,
                        (const uint32_t*)K.C.rows, (const double*)K.C.vals, (const double*)(K.V + uint64_t(p) * n), n,
                        K.w);
-    // classical Gram-Schmidt, twice
-    double h1[kKrylovMaxCols], h2[kKrylovMaxCols], nn = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-      K.dots(cols, K.w, K.h);
-      hipLaunchKernelGGL(colUpdateKernel, dim3(nb), dim3(kBlock), 0, K.stream, (const double*)K.V, cols,
-                         (const double*)K.h, K.w, n);
-      HIPCHK(hipMemcpyAsync(pass == 0 ? h1 : h2, K.h, cols * 8, hipMemcpyDeviceToHost, K.stream));
-    }
-    // ||w||, then v_{p+1} = w / ||w||
-    hipLaunchKernelGGL(colDotsKernel, dim3(K.nd), dim3(kBlock), 0, K.stream, (const double*)K.w, 0u, 1u,
-                       (const double*)K.w, n, K.part);
-    hipLaunchKernelGGL(colDotsReduceKernel, dim3(1), dim3(kBlock), 0, K.stream, (const double*)K.part, K.nd,
-                       K.h + kKrylovMaxCols);
-    HIPCHK(hipMemcpyAsync(&nn, K.h + kKrylovMaxCols, 8, hipMemcpyDeviceToHost, K.stream));
+    // classical Gram-Schmidt twice in three passes over V: h1 = V^T w; w -= V h1 with h2 = V^T w
+    // in the same pass; w -= V h2 with w . w in the same pass
+    double hv[b12x_krylov_s::kHSize];
+    K.dots(cols, K.w, K.h);
+    K.update(cols, K.h, K.h + K.kH2, false);
+    K.update(cols, K.h + K.kH2, K.h + K.kNorm2, true);
+    HIPCHK(hipMemcpyAsync(hv, K.h, (K.kNorm2 + 1) * 8, hipMemcpyDeviceToHost, K.stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(K.stream));
-    const double b = std::sqrt(nn);
-    for (uint32_t c = 0; c < cols; ++c) h[c] = h1[c] + h2[c];
+    // v_{p+1} = w / ||w||
+    const double b = std::sqrt(hv[K.kNorm2]);
+    for (uint32_t c = 0; c < cols; ++c) h[c] = hv[c] + hv[K.kH2 + c];
     *beta = b;
     if (b > 0.0) {
-      HIPCHK(hipMemcpyAsync(K.h + kKrylovMaxCols, &b, 8, hipMemcpyHostToDevice, K.stream));
+      HIPCHK(hipMemcpyAsync(K.h + K.kBeta, &b, 8, hipMemcpyHostToDevice, K.stream));
       hipLaunchKernelGGL(scaleIntoKernel, dim3(nb), dim3(kBlock), 0, K.stream, (const double*)K.w, n,
-                         (const double*)(K.h + kKrylovMaxCols), K.V + uint64_t(p + 1) * n);
+                         (const double*)(K.h + K.kBeta), K.V + uint64_t(p + 1) * n);
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(K.stream));
     }
@@ -3213,8 +3234,10 @@ int b12x_krylov_rotate(b12x_krylov k, int m, int kk, const double* q) {
     b12x_krylov_s& K = *k;
     const uint32_t n = K.n;
     HIPCHK(hipMemcpyAsync(K.Q, q, uint64_t(m) * kk * 8, hipMemcpyHostToDevice, K.stream));
-    hipLaunchKernelGGL(krylovRotateKernel, dim3(ceilDiv(n, kBlock)), dim3(kBlock), 0, K.stream, K.V, n, m, kk,
-                       (const double*)K.Q);
+    const uint32_t nb = std::min<uint32_t>(ceilDiv(n, kBlock), kRotateBlocks);
+#define B12X_ROTATE(NC) hipLaunchKernelGGL(krylovRotateKernel<NC>, dim3(nb), dim3(kBlock), 0, K.stream, K.V, n, m, kk, (const double*)K.Q)
+    B12X_BASIS_WIDTH(uint32_t(kk), B12X_ROTATE);
+#undef B12X_ROTATE
     HIPCHK(hipMemcpyAsync(K.V + uint64_t(kk) * n, K.V + uint64_t(m) * n, uint64_t(n) * 8, hipMemcpyDeviceToDevice,
                           K.stream));
     HIPCHK(hipGetLastError());

@@ -2142,30 +2142,86 @@ static __global__ __launch_bounds__(kBlock) void ritzKernel(const double* part, 
 // dotPartialKernel (block-strided partials, blockReduceSumF64, sumPartials), so the results
 // equal the oracle's restatement bit for bit.
 // ---------------------------------------------------------------------------------------------
-constexpr int kKrylovDotCols = 8;  // basis columns per dot-product pass
 constexpr int kKrylovMaxCols = 64; // basis vectors at most (ncv + 1)
 
-// block partials of x_c . w for the columns c0 <= c < c0 + cnt (cnt <= kKrylovDotCols) of V:
-// part[c * gridDim.x + block]
-static __global__ __launch_bounds__(kBlock) void colDotsKernel(const double* V, uint32_t c0, uint32_t cnt,
-                                                              const double* w, uint32_t n, double* part) {
-  double acc[kKrylovDotCols];
+// The basis passes below hold one value per basis column in registers: NC is the column count
+// rounded up to a multiple of 8 (8, 16, ..., 64). Lanes q >= cols load column cols - 1 again
+// (lines the lane has just loaded) and their results are dropped by selects, so no branch
+// depends on the column count. Every lane sums its rows i = t, t + T, ... (T = all lanes of
+// the grid) in order, the block sums its lanes with blockReduceSumF64 and colDotsReduceKernel
+// sums the block partials with sumPartials: the order of dotPartialKernel.
+
+// block partials of x_c . w for every column c < cols (NC - 8 < cols <= NC) in one pass over V
+// and w: part[c * gridDim.x + block]
+template <int NC>
+static __global__ __launch_bounds__(kBlock) void colDotsKernel(const double* __restrict__ V, uint32_t cols,
+                                                              const double* __restrict__ w, uint32_t n,
+                                                              double* __restrict__ part) {
+  __builtin_assume(cols > uint32_t(NC - 8) && cols <= uint32_t(NC));
+  double acc[NC];
 #pragma unroll
-  for (int q = 0; q < kKrylovDotCols; ++q) acc[q] = 0.0;
+  for (int q = 0; q < NC; ++q) acc[q] = 0.0;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double x[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) x[q] = V[uint64_t(min(uint32_t(q), cols - 1)) * n + i];
     const double wi = w[i];
 #pragma unroll
-    for (int q = 0; q < kKrylovDotCols; ++q)
-      if (uint32_t(q) < cnt) {
-        const double prod = V[uint64_t(c0 + q) * n + i] * wi;
-        acc[q] += prod;
-      }
+    for (int q = 0; q < NC; ++q) {
+      const double prod = x[q] * wi;
+      const double a = acc[q] + prod;
+      acc[q] = uint32_t(q) < cols ? a : acc[q];
+    }
   }
 #pragma unroll
-  for (int q = 0; q < kKrylovDotCols; ++q)
-    if (uint32_t(q) < cnt) {
+  for (int q = 0; q < NC; ++q)
+    if (uint32_t(q) < cols) {
       const double v = blockReduceSumF64(acc[q]);
-      if (threadIdx.x == 0) part[uint64_t(c0 + q) * gridDim.x + blockIdx.x] = v;
+      if (threadIdx.x == 0) part[uint64_t(q) * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// Gram-Schmidt step and the next dot products in one pass over V: w_i = w_i - sum over c < cols
+// of x_c[i] h_c (c ascending, one subtraction per column), stored; then, with NORM = false, the
+// block partials of x_c . w for c < cols (part[c * gridDim.x + block]), or, with NORM = true, of
+// w . w (part[block]), both of the updated w.
+template <int NC, bool NORM>
+static __global__ __launch_bounds__(kBlock) void colUpdateKernel(const double* __restrict__ V, uint32_t cols,
+                                                                const double* __restrict__ h, double* __restrict__ w,
+                                                                uint32_t n, double* __restrict__ part) {
+  __builtin_assume(cols > uint32_t(NC - 8) && cols <= uint32_t(NC));
+  constexpr int NA = NORM ? 1 : NC;
+  double acc[NA];
+#pragma unroll
+  for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double x[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) x[q] = V[uint64_t(min(uint32_t(q), cols - 1)) * n + i];
+    double v = w[i];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const double u = v - x[q] * h[min(uint32_t(q), cols - 1)];
+      v = uint32_t(q) < cols ? u : v;
+    }
+    w[i] = v;
+    if (NORM) {
+      const double prod = v * v;
+      acc[0] += prod;
+    } else {
+#pragma unroll
+      for (int q = 0; q < NA; ++q) {
+        const double prod = x[q] * v;
+        const double a = acc[q] + prod;
+        acc[q] = uint32_t(q) < cols ? a : acc[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NA; ++q)
+    if (NORM || uint32_t(q) < cols) {
+      const double v = blockReduceSumF64(acc[q]);
+      if (threadIdx.x == 0) part[uint64_t(q) * gridDim.x + blockIdx.x] = v;
     }
 }
 
@@ -2175,16 +2231,6 @@ static __global__ __launch_bounds__(kBlock) void colDotsReduceKernel(const doubl
   if (threadIdx.x == 0) h[blockIdx.x] = v;
 }
 
-// w_i = w_i - sum over c < cols of x_c[i] h_c, c ascending (one subtraction per column)
-static __global__ __launch_bounds__(kBlock) void colUpdateKernel(const double* V, uint32_t cols, const double* h,
-                                                                double* w, uint32_t n) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double v = w[i];
-  for (uint32_t c = 0; c < cols; ++c) v -= V[uint64_t(c) * n + i] * h[c];
-  w[i] = v;
-}
-
 // out_i = w_i / *beta
 static __global__ __launch_bounds__(kBlock) void scaleIntoKernel(const double* w, uint32_t n, const double* beta,
                                                                 double* out) {
@@ -2192,21 +2238,34 @@ static __global__ __launch_bounds__(kBlock) void scaleIntoKernel(const double* w
   if (i < n) out[i] = w[i] / *beta;
 }
 
-// V[:, c] = sum over d < m of V[:, d] Q[d * k + c] (d ascending, from 0.0) for c < k, in place:
-// each lane holds its box's m entries before it writes
+// V[:, c] = sum over d < m of V[:, d] Q[d * k + c] (d ascending, from 0.0) for c < k, in place
+// (NC - 8 < k <= NC). Q (row-major m x k) is staged in LDS as rows of NC with zeros past k; a
+// lane streams its box's m entries into NC sums (those past k are dropped) and writes the k
+// sums after its last read of the box's row. Lanes stride over the boxes, so at most
+// kRotateBlocks blocks stage Q.
+constexpr uint32_t kRotateBlocks = 4096;
+
+template <int NC>
 static __global__ __launch_bounds__(kBlock) void krylovRotateKernel(double* V, uint32_t n, int m, int k,
-                                                                   const double* Q) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double x[kKrylovMaxCols];
+                                                                   const double* __restrict__ Q) {
+  __shared__ double q[kKrylovMaxCols * NC];
+  for (int t = threadIdx.x; t < m * NC; t += kBlock) {
+    const int d = t / NC, c = t % NC;
+    q[t] = c < k ? Q[d * k + c] : 0.0;
+  }
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double s[NC];
 #pragma unroll
-  for (int d = 0; d < kKrylovMaxCols; ++d) x[d] = d < m ? V[uint64_t(d) * n + i] : 0.0;
-  for (int c = 0; c < k; ++c) {
-    double s = 0.0;
+    for (int c = 0; c < NC; ++c) s[c] = 0.0;
+    for (int d = 0; d < m; ++d) {
+      const double x = V[uint64_t(d) * n + i];
 #pragma unroll
-    for (int d = 0; d < kKrylovMaxCols; ++d)
-      if (d < m) s += x[d] * Q[d * k + c];
-    V[uint64_t(c) * n + i] = s;
+      for (int c = 0; c < NC; ++c) s[c] += x * q[d * NC + c];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (c < k) V[uint64_t(c) * n + i] = s[c];
   }
 }
 

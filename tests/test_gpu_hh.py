@@ -113,13 +113,15 @@ def assert_eigs_match(mg, mo, tol, scipy_check=False, ncv=None, vectors=True):
 
 def test_krylov_basis_matches_oracle():
     """The Krylov basis operations on the GPU (b12x_krylov_*) against the oracle's, bit for bit:
-    the start (given and default), every expansion (h, beta and the new vector) and a rotation,
-    on a random substochastic matrix (one entry per row and column pair, as transition matrices
-    have) uploaded in row-major order."""
+    the start (given and default), every expansion (h, beta and the new vector) and rotations to
+    7, 12, 20 and 33 vectors each followed by the expansions back to 40 (the passes hold 8, 16,
+    32 or 64 values a lane by the column count, so every width runs), on a random substochastic
+    matrix (one entry per row and column pair, as transition matrices have) uploaded in
+    row-major order."""
     print("[krylov] basis operations", flush=True)
     from oracle_api import OCoo
     rng = np.random.default_rng(77)
-    n, ncv = 20000, 16
+    n, ncv = 20000, 40
     rows = np.repeat(np.arange(n), 6)
     cols = (rows + rng.integers(-300, 300, rows.size)) % n
     vals = rng.random(rows.size)
@@ -135,16 +137,22 @@ def test_krylov_basis_matches_oracle():
     for start in (v0, None):
         assert Kg.start(start) == Ko.start(start)
         assert np.array_equal(Kg.vector(0), Ko.vector(0))
-    for p in range(ncv):
-        hg, bg = Kg.expand(p)
-        ho, bo = Ko.expand(p)
-        assert bg == bo and np.array_equal(hg, ho), f"expansion {p}"
-        assert np.array_equal(Kg.vector(p + 1), Ko.vector(p + 1))
-    Q, _ = np.linalg.qr(rng.standard_normal((ncv, 7)))
-    Kg.rotate(ncv, 7, Q)
-    Ko.rotate(ncv, 7, Q)
-    for c in range(8):
-        assert np.array_equal(Kg.vector(c), Ko.vector(c))
+
+    def expand_from(p0):
+        for p in range(p0, ncv):
+            hg, bg = Kg.expand(p)
+            ho, bo = Ko.expand(p)
+            assert bg == bo and np.array_equal(hg, ho), f"expansion {p}"
+            assert np.array_equal(Kg.vector(p + 1), Ko.vector(p + 1))
+
+    expand_from(0)
+    for k in (7, 12, 20, 33):
+        Q, _ = np.linalg.qr(rng.standard_normal((ncv, k)))
+        Kg.rotate(ncv, k, Q)
+        Ko.rotate(ncv, k, Q)
+        for c in range(k + 1):
+            assert np.array_equal(Kg.vector(c), Ko.vector(c)), f"rotation to {k}, vector {c}"
+        expand_from(k)
 
 
 @pytest.mark.parametrize("nvec", [1, 3, 8])
