@@ -1179,7 +1179,7 @@ class Forest:
         return b.value
 
     STAT_KEYS = ("steps", "subdiv_s", "retire_s", "pool_s", "pool_iters", "pool_max_jobs", "pool_max_cols",
-                 "pool_max_nnz")
+                 "pool_max_nnz", "pool_col_iters", "pool_nnz_iters")
 
     def stats(self):
         v = np.zeros(len(self.STAT_KEYS), np.float64)

@@ -145,6 +145,7 @@ struct b12x_forest_s {
   // telemetry: wall seconds per phase, pool iterations, pool high-water marks
   double tSubdiv = 0.0, tRetire = 0.0, tPool = 0.0;
   uint64_t poolIters = 0, poolMaxJobs = 0, poolMaxN = 0, poolMaxNnz = 0;
+  double poolColIters = 0.0, poolNnzIters = 0.0;  // sums of pool columns and entries over iterations
   Pool pool;
   int poolItersPerStep = 64;
 
@@ -333,6 +334,13 @@ uint32_t poolCoopBlocks() {
   return cached;
 }
 
+// Poll limit of the persistent form's waits (B12X_POOL_SPIN overrides it; 0 makes every wait
+// give up at once, which tests use to run the fallback).
+uint32_t poolSpinLimit() {
+  const char* e = std::getenv("B12X_POOL_SPIN");
+  return e ? uint32_t(std::strtoul(e, nullptr, 10)) : (1u << 21);
+}
+
 static __global__ __launch_bounds__(kBlock) void narrowCountKernel(const uint32_t* in, uint64_t n, uint8_t* out) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) out[i] = uint8_t(in[i]);
@@ -448,8 +456,9 @@ void poolLaunch(b12x_forest_s& F, int iters) {
   F.poolMaxJobs = std::max<uint64_t>(F.poolMaxJobs, P.jobs.size());
   F.poolMaxN = std::max<uint64_t>(F.poolMaxN, P.n);
   F.poolMaxNnz = std::max<uint64_t>(F.poolMaxNnz, P.nnz);
+  F.poolColIters += double(P.n) * double(iters);
+  F.poolNnzIters += double(P.nnz) * double(iters);
   hipStream_t st = P.stream;
-  const uint32_t nJ = uint32_t(P.jobs.size());
   // small pools: the persistent form (every block resident; B12X_POOL_PERSISTENT=0: batches).
   // With no subdivision running, a launch may run a longer budget.
   const char* pe = std::getenv("B12X_POOL_PERSISTENT");
@@ -461,10 +470,12 @@ void poolLaunch(b12x_forest_s& F, int iters) {
     HIPCHK(hipMemsetAsync(P.coopErr, 0, sizeof(int), st));
     const uint32_t budget = uint32_t(iters) * (F.active.empty() ? 16u : 1u);
     PoolCoopArgs ca{P.colPtr, P.rowIdx, P.cnt, P.table, P.jobOfBlock, P.dJobs, P.y, P.capN, P.coopPart, P.nB, P.coopErr,
-                    uint32_t(std::max(F.piMaxIter, 0)), F.piTol, budget, 1u << 21};
+                    uint32_t(std::max(F.piMaxIter, 0)), F.piTol, budget, poolSpinLimit()};
     hipLaunchKernelGGL(poolCoopKernel, dim3(P.nB), dim3(kBlock), 0, st, ca);
     HIPCHK(hipGetLastError());
     F.poolIters += budget - uint64_t(iters);
+    F.poolColIters += double(P.n) * double(budget - uint64_t(iters));
+    F.poolNnzIters += double(P.nnz) * double(budget - uint64_t(iters));
     P.coopInFlight = true;
     P.inFlight = true;
     return;
@@ -473,10 +484,8 @@ void poolLaunch(b12x_forest_s& F, int iters) {
     for (int b = 0; b < iters; ++b) {
       hipLaunchKernelGGL(poolStepKernel, dim3(P.nB), dim3(kBlock), 0, st, (const uint32_t*)P.colPtr,
                          (const uint32_t*)P.rowIdx, (const uint8_t*)P.cnt, (const double*)P.table,
-                         (const uint32_t*)P.jobOfBlock,
-                         (const PoolJobDev*)P.dJobs, P.y, P.capN, P.partAbs, P.partRes);
-      hipLaunchKernelGGL(poolReduceKernel, dim3(nJ), dim3(kWave), 0, st, (const double*)P.partAbs,
-                         (const double*)P.partRes, P.dJobs, nJ, uint32_t(std::max(F.piMaxIter, 0)), F.piTol);
+                         (const uint32_t*)P.jobOfBlock, P.dJobs, P.y, P.capN, P.partAbs, P.partRes,
+                         uint32_t(std::max(F.piMaxIter, 0)), F.piTol);
     }
   };
   // the batch runs from a captured HIP graph (B12X_GRAPHS=0: eager launches)
@@ -510,7 +519,21 @@ void poolHarvest(b12x_forest_s& F) {
   int coopErr = 0;
   if (P.coopInFlight) HIPCHK(hipMemcpyAsync(&coopErr, P.coopErr, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (P.coopInFlight && coopErr) P.coopOff = true;  // every block has left; jobs hold consistent states
+  if (P.coopInFlight && coopErr) {
+    // a persistent wait gave up and every block has left: the jobs still running start again
+    // from iteration 1 in the batched form (their iterates do not depend on the form)
+    P.coopOff = true;
+    for (uint32_t r = 0; r < nJ; ++r)
+      if (!hj[r].done) {
+        hj[r].t = 1;
+        hj[r].norm[0] = hj[r].norm[1] = hj[r].norm[2] = 0.0;
+        hj[r].resid = -1.0;
+        hj[r].nuLag = 0.0;
+        hj[r].arrived = 0;
+      }
+    HIPCHK(hipMemcpyAsync(P.dJobs, hj.data(), nJ * sizeof(PoolJobDev), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   P.coopInFlight = false;
   bool anyDone = false;
   for (uint32_t r = 0; r < nJ; ++r) anyDone = anyDone || hj[r].done;
@@ -1053,9 +1076,10 @@ int b12x_forest_stats(b12x_forest f, double* out, int n) {
   return guarded([&]() -> int {
     if (!f || !out) return setErr(B12X_E_ARG, "null argument");
     const b12x_forest_s& F = *f;
-    const double v[8] = {double(F.steps), F.tSubdiv, F.tRetire, F.tPool, double(F.poolIters),
-                         double(F.poolMaxJobs), double(F.poolMaxN), double(F.poolMaxNnz)};
-    for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+    const double v[10] = {double(F.steps),     F.tSubdiv,           F.tRetire,          F.tPool,
+                          double(F.poolIters), double(F.poolMaxJobs), double(F.poolMaxN), double(F.poolMaxNnz),
+                          F.poolColIters,      F.poolNnzIters};
+    for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
     return B12X_OK;
   });
 }

@@ -193,6 +193,7 @@ struct PoolJobDev {
   double norm[3];
   double resid;
   double nuLag;     // nu_t of the last t with t % 8 == 0 (the scale c_{t+8} derives from it)
+  uint32_t arrived; // blocks of the current batched step that have stored their partials (0 at rest)
 };
 
 __host__ __device__ inline uint32_t poolBlocksOf(uint32_t len) { return (len + kPoolCols - 1) / kPoolCols; }
@@ -206,77 +207,6 @@ static __global__ __launch_bounds__(kBlock) void poolJobOfBlockKernel(const Pool
   for (uint32_t q = threadIdx.x; q < nb; q += kBlock) jobOfBlock[b0 + q] = r;
 }
 
-// y_t for kPoolCols columns of one job, one lane per column: the power iteration of
-// b12x_kernels.hpp (unnormalised iterates y_t = c_t y_{t-1} P, c_t = piScale) over the job's
-// positions. The lane adds its column's products in ascending-row order (the oracle's order) with
-// kPoolUnroll entries' loads in flight at a time. Also the block's partial sums of |y_t| and of
-// the residual terms of iteration t-1, |y_{t-1} / nu_{t-1} - y_{t-2} / nu_{t-2}|, y_0 = mu_0
-// uniform, nu_0 = 1.
-constexpr int kPoolUnroll = 8;
-
-static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* __restrict__ colPtr,
-                                                               const uint32_t* __restrict__ rowIdx,
-                                                               const uint8_t* __restrict__ cnt,
-                                                               const double* __restrict__ table,
-                                                               const uint32_t* __restrict__ jobOfBlock,
-                                                               const PoolJobDev* __restrict__ jobs, double* y,
-                                                               uint32_t ys, double* partAbs, double* partRes) {
-  __shared__ double val[kPoolTable];
-  const PoolJobDev& J = jobs[jobOfBlock[blockIdx.x]];
-  if (J.done) return;  // uniform over the block
-  if (threadIdx.x < kPoolTable) val[threadIdx.x] = table[threadIdx.x];
-  __syncthreads();
-  const uint32_t t = J.t;
-  const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
-  const double nPrev = t >= 2 ? J.norm[s1] : 1.0;
-  const double nPrev2 = t >= 3 ? J.norm[s2] : 1.0;
-  const double mu0 = 1.0 / double(J.hi - J.lo);
-  const double c = piScale(t, J.nuLag);
-  const uint32_t p = J.lo + (blockIdx.x - J.b0) * kPoolCols + threadIdx.x;
-  const double* yPrev = y + uint64_t(s1) * ys;
-  const double* yPrev2 = y + uint64_t(s2) * ys;
-  double sAbs = 0.0, sRes = 0.0;
-  if (p < J.hi) {
-    const uint32_t e1 = colPtr[p + 1];
-    double s = 0.0;
-    // branch-free body (entry indices clamped to the column, values and sums selected) so that
-    // all of a lane's gathers are in flight at once; y is always allocated, so the unused read of
-    // iteration 1 stays in bounds
-    for (uint32_t e = colPtr[p]; e < e1; e += kPoolUnroll) {
-      uint32_t i[kPoolUnroll];
-      double v[kPoolUnroll], m[kPoolUnroll];
-#pragma unroll
-      for (int u = 0; u < kPoolUnroll; ++u) {
-        const uint32_t eu = min(e + u, e1 - 1);
-        i[u] = rowIdx[eu];
-        v[u] = val[cnt[eu]];
-      }
-#pragma unroll
-      for (int u = 0; u < kPoolUnroll; ++u) m[u] = yPrev[i[u]];
-#pragma unroll
-      for (int u = 0; u < kPoolUnroll; ++u) {
-        const double z = (t == 1) ? mu0 : m[u];
-        const double prod = z * v[u];
-        s = (e + u < e1) ? s + prod : s;
-      }
-    }
-    if (c != 1.0) s *= c;
-    y[uint64_t(s0) * ys + p] = s;
-    sAbs = fabs(s);
-    if (t >= 2) {
-      const double a = yPrev[p] / nPrev;
-      const double b = (t == 2) ? mu0 : yPrev2[p] / nPrev2;
-      sRes = fabs(a - b);
-    }
-  }
-  sAbs = blockReduceSumF64(sAbs);
-  sRes = blockReduceSumF64(sRes);
-  if (threadIdx.x == 0) {
-    partAbs[blockIdx.x] = sAbs;
-    partRes[blockIdx.x] = sRes;
-  }
-}
-
 // Canonical sum of a job's block partials q = 0..nb-1 (shared by the batched and the persistent
 // forms, so a job's iterates do not depend on which form ran them): lane l of one wave adds
 // q = l, l + 64, ... in ascending order, then the lanes combine by a fixed shuffle tree.
@@ -286,16 +216,12 @@ __device__ inline double poolLaneTreeSum(double v) {
   return v;
 }
 
-// One wave per pool job: sums its block partials, norm of y_t, residual of iteration t-1,
-// stopping decision, t += 1.
-static __global__ __launch_bounds__(kWave) void poolReduceKernel(const double* partAbs, const double* partRes,
-                                                                PoolJobDev* jobs, uint32_t nJobs, uint32_t maxIter,
-                                                                double tol) {
-  const uint32_t r = blockIdx.x;
-  if (r >= nJobs) return;
-  PoolJobDev& J = jobs[r];
-  if (J.done) return;
-  const uint32_t t = J.t, b0 = J.b0, nb = poolBlocksOf(J.hi - J.lo), lane = threadIdx.x;
+// One wave (the lanes 0..63 of the calling block) closes iteration t of job J: sums its block
+// partials (agent-scope loads) in the canonical order (norm of y_t, residual of iteration t-1), stopping decision,
+// t += 1; the arrival count returns to 0.
+__device__ inline void poolReduceJob(const double* partAbs, const double* partRes, PoolJobDev& J, uint32_t t,
+                                     uint32_t maxIter, double tol) {
+  const uint32_t b0 = J.b0, nb = poolBlocksOf(J.hi - J.lo), lane = threadIdx.x & (kWave - 1);
   constexpr int U = 8;  // loads in flight per lane before they are added (order unchanged)
   double ta = 0.0, tr = 0.0;
   for (uint32_t q0 = 0; q0 < nb; q0 += kWave * U) {
@@ -303,8 +229,8 @@ static __global__ __launch_bounds__(kWave) void poolReduceKernel(const double* p
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t q = q0 + u * kWave + lane;
-      va[u] = q < nb ? partAbs[b0 + q] : 0.0;
-      vr[u] = q < nb ? partRes[b0 + q] : 0.0;
+      va[u] = q < nb ? ldAgent(partAbs + b0 + q) : 0.0;
+      vr[u] = q < nb ? ldAgent(partRes + b0 + q) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -327,17 +253,111 @@ static __global__ __launch_bounds__(kWave) void poolReduceKernel(const double* p
       }
     }
     J.t = t + 1;
+    J.arrived = 0;
   }
+}
+
+// y_t for the kPoolCols columns of pool block v (job J, iteration t), one lane per column: the
+// power iteration of b12x_kernels.hpp (unnormalised iterates y_t = c_t y_{t-1} P, c_t = piScale)
+// over the job's positions. The lane adds its column's products in ascending-row order (the
+// oracle's order) with kPoolUnroll entries' loads in flight at a time. Returns the lane's terms
+// of the block's partial sums: |y_t| and the residual term of iteration t-1,
+// |y_{t-1} / nu_{t-1} - y_{t-2} / nu_{t-2}| (y_0 = mu_0 uniform, nu_0 = 1).
+constexpr int kPoolUnroll = 8;
+
+__device__ inline void poolColumnsStep(const uint32_t* __restrict__ colPtr, const uint32_t* __restrict__ rowIdx,
+                                       const uint8_t* __restrict__ cnt, const double* val, const PoolJobDev& J,
+                                       uint32_t v, uint32_t t, double nPrev, double nPrev2, double nuLag, double* y,
+                                       uint32_t ys, double& sAbs, double& sRes) {
+  const uint32_t s0 = t % 3, s1 = (t + 2) % 3, s2 = (t + 1) % 3;
+  const double mu0 = 1.0 / double(J.hi - J.lo);
+  const double c = piScale(t, nuLag);
+  const uint32_t p = J.lo + (v - J.b0) * kPoolCols + threadIdx.x;
+  const double* yPrev = y + uint64_t(s1) * ys;
+  const double* yPrev2 = y + uint64_t(s2) * ys;
+  sAbs = 0.0;
+  sRes = 0.0;
+  if (p >= J.hi) return;
+  const uint32_t e1 = colPtr[p + 1];
+  double s = 0.0;
+  // branch-free body (entry indices clamped to the column, values and sums selected) so that
+  // all of a lane's gathers are in flight at once; y is always allocated, so the unused read of
+  // iteration 1 stays in bounds
+  for (uint32_t e = colPtr[p]; e < e1; e += kPoolUnroll) {
+    uint32_t i[kPoolUnroll];
+    double w[kPoolUnroll], m[kPoolUnroll];
+#pragma unroll
+    for (int u = 0; u < kPoolUnroll; ++u) {
+      const uint32_t eu = min(e + u, e1 - 1);
+      i[u] = rowIdx[eu];
+      w[u] = val[cnt[eu]];
+    }
+#pragma unroll
+    for (int u = 0; u < kPoolUnroll; ++u) m[u] = yPrev[i[u]];
+#pragma unroll
+    for (int u = 0; u < kPoolUnroll; ++u) {
+      const double z = (t == 1) ? mu0 : m[u];
+      const double prod = z * w[u];
+      s = (e + u < e1) ? s + prod : s;
+    }
+  }
+  if (c != 1.0) s *= c;
+  y[uint64_t(s0) * ys + p] = s;
+  sAbs = fabs(s);
+  if (t >= 2) {
+    const double x = yPrev[p] / nPrev;
+    const double z = (t == 2) ? mu0 : yPrev2[p] / nPrev2;
+    sRes = fabs(x - z);
+  }
+}
+
+// Iteration t of every pool job in one launch, one block per pool block (poolColumnsStep). The
+// block stores its partial sums with agent-scope stores, waits for them to complete and counts
+// itself in with an agent-scope atomic; the job's last block to arrive closes the iteration with
+// poolReduceJob, whose agent-scope loads see every block's partials. No block waits for another.
+// (Acquire-release fences at agent scope would write back and invalidate the XCD's L2 per block:
+// four times slower measured.)
+static __global__ __launch_bounds__(kBlock) void poolStepKernel(const uint32_t* __restrict__ colPtr,
+                                                               const uint32_t* __restrict__ rowIdx,
+                                                               const uint8_t* __restrict__ cnt,
+                                                               const double* __restrict__ table,
+                                                               const uint32_t* __restrict__ jobOfBlock,
+                                                               PoolJobDev* jobs, double* y, uint32_t ys,
+                                                               double* partAbs, double* partRes, uint32_t maxIter,
+                                                               double tol) {
+  __shared__ double val[kPoolTable];
+  __shared__ int closes;
+  PoolJobDev& J = jobs[jobOfBlock[blockIdx.x]];
+  if (J.done) return;  // uniform over the block
+  if (threadIdx.x < kPoolTable) val[threadIdx.x] = table[threadIdx.x];
+  __syncthreads();
+  const uint32_t t = J.t;
+  const double nPrev = t >= 2 ? J.norm[(t + 2) % 3] : 1.0;
+  const double nPrev2 = t >= 3 ? J.norm[(t + 1) % 3] : 1.0;
+  const uint32_t nb = poolBlocksOf(J.hi - J.lo);
+  double sAbs, sRes;
+  poolColumnsStep(colPtr, rowIdx, cnt, val, J, blockIdx.x, t, nPrev, nPrev2, J.nuLag, y, ys, sAbs, sRes);
+  sAbs = blockReduceSumF64(sAbs);
+  sRes = blockReduceSumF64(sRes);
+  if (threadIdx.x == 0) {
+    stAgent(partAbs + blockIdx.x, sAbs);
+    stAgent(partRes + blockIdx.x, sRes);
+    __builtin_amdgcn_s_waitcnt(0);  // the partials have completed before the block counts itself in
+    closes = __hip_atomic_fetch_add(&J.arrived, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nb - 1;
+  }
+  __syncthreads();
+  if (closes && threadIdx.x < kWave) poolReduceJob(partAbs, partRes, J, t, maxIter, tol);
 }
 
 // Persistent form of the pool for pools of at most kPoolCoopMaxBlocks step blocks (every block
 // resident): each job's blocks iterate together, synchronised only with one another through
 // the job's partial slots (buffer t % 3, kCoopEmpty until written, cleared one iteration ahead,
-// as the persistent forms of b12x_kernels.hpp use them), so jobs advance independently. Per launch a job runs until it stops or
-// `budget` iterations are done; its first block then stores the job state. A wait past
-// spinLimit polls raises *err; the job's blocks all leave at that iteration, and its first block
-// stores the state as of the last completed iteration (y_{t-1}, y_{t-2} are intact).
-// Same arithmetic and summation orders as poolStepKernel + poolReduceKernel.
+// as the persistent forms of b12x_kernels.hpp use them), so jobs advance independently. Per
+// launch a job runs until it stops or `budget` iterations are done; its first block then stores
+// the job state. A wait past spinLimit polls raises *err and every block leaves at its next
+// wait; the blocks of one job can then stand one iteration apart, so the host restarts the jobs
+// still running from iteration 1 in the batched form (their iterates do not depend on the form).
+// Same arithmetic and summation orders as poolStepKernel (and its poolReduceJob).
 constexpr uint32_t kPoolCoopMaxBlocks = 512;
 constexpr int kPoolCache = 24;  // entries of a lane's column held in registers
 
@@ -499,15 +519,7 @@ static __global__ __launch_bounds__(kBlock) void poolCoopKernel(PoolCoopArgs a) 
       }
     }
     __syncthreads();
-    if (leave) {
-      if (b == b0 && tid == 0) {  // state as of the last completed iteration
-        J->t = t;
-        J->norm[(t + 2) % 3] = nPrev;
-        J->norm[(t + 1) % 3] = nPrev2;
-        J->nuLag = nuLag;
-      }
-      return;
-    }
+    if (leave) return;  // the host restarts the job
     const double ta = sums[0], tr = sums[1];
     if ((t & 7u) == 0) nuLag = ta;
     const uint32_t completed = t - 1;

@@ -382,7 +382,8 @@ int b12x_forest_job_release(b12x_forest f, uint32_t job);
 /* HBM bytes held by the forest's leaf arrays, tables, measure pool and unreleased results */
 int b12x_forest_footprint(b12x_forest f, uint64_t* bytes);
 /* Telemetry, first n of: steps, seconds subdividing, seconds retiring (transition matrices),
- * seconds iterating measures, measure iterations, and the pool's peak jobs, columns, entries. */
+ * seconds iterating measures, measure iterations, the pool's peak jobs, columns, entries, and
+ * the pool's columns and entries summed over the iterations launched (work of the measures). */
 int b12x_forest_stats(b12x_forest f, double* out, int n);
 /* Measure iterations the pool advances per forest step (default 64). */
 int b12x_forest_set_pool_iterations(b12x_forest f, int iters);

@@ -165,30 +165,37 @@ def test_pool_graph_replays_match_eager_launches(monkeypatch):
 def test_persistent_and_batched_pool_forms_agree(monkeypatch):
     """A pool larger than the persistent form takes (batched launches) that shrinks below it as
     jobs leave (persistent launches): every job's measure and iteration count equal the same
-    job run alone (persistent throughout) and the all-batched run, bit for bit."""
+    job run alone (persistent throughout), the all-batched run and a run whose persistent waits
+    give up at once (the jobs still running restart from iteration 1 in the batched form), bit
+    for bit."""
     print("[forest] persistent vs batched pool", flush=True)
     grid = gx.InnerGrid(4)
     rng = np.random.default_rng(9)
     cases = [(float(rng.uniform(1.2, 1.4)), int(rng.integers(20, 22))) for _ in range(14)]
+    names = ("B12X_POOL_PERSISTENT", "B12X_POOL_SPIN")
 
-    def run(env):
-        if env is None:
-            monkeypatch.delenv("B12X_POOL_PERSISTENT", raising=False)
-        else:
-            monkeypatch.setenv("B12X_POOL_PERSISTENT", env)
+    def run(**env):
+        for k in names:
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
         F = gx.Forest(2, grid, 1, pi_max_iter=100000, pi_tol=1e-12)
         F.set_pool_iterations(8)
         ids = [F.add(HENON_C, HENON_R, gx.Henon(a, 0.3, 1), d) for a, d in cases]
         F.run()
         st = F.stats()
+        print(f"  {env}: pool iterations {st['pool_iters']:.0f}", flush=True)
         return [(F.measure(j), F.info(j)["iters"]) for j in ids], st
 
-    mixed, st = run(None)
+    mixed, st = run()
     assert st["pool_max_cols"] > 512 * 256  # the pool started above the persistent form's size
-    batched, _ = run("0")
-    for (ma, ia), (mb, ib) in zip(mixed, batched):
-        assert ia == ib and np.array_equal(ma, mb)
-    monkeypatch.delenv("B12X_POOL_PERSISTENT", raising=False)
+    batched, _ = run(B12X_POOL_PERSISTENT="0")
+    gave_up, _ = run(B12X_POOL_SPIN="0")
+    for other in (batched, gave_up):
+        for (ma, ia), (mb, ib) in zip(mixed, other):
+            assert ia == ib and np.array_equal(ma, mb)
+    for k in names:
+        monkeypatch.delenv(k, raising=False)
     a, d = cases[0]
     A = gx.Forest(2, grid, 1, pi_max_iter=100000, pi_tol=1e-12)
     A.set_pool_iterations(8)
