@@ -186,13 +186,15 @@ def serve_requests(n_req, rate=200.0, seed=0):
     return arrival, req
 
 
-def serve_bench(gx, n_req, rate=200.0, tol=TOL):
+def serve_bench(gx, n_req, rate=200.0, tol=TOL, n_steady=0):
     import numpy as np
     arrival, req = serve_requests(n_req, rate)
     def forests():
         return [gx.Forest(3, gx.InnerGrid(3), 2, pi_max_iter=MAX_ITER, pi_tol=tol),
                 gx.Forest(2, gx.InnerGrid(4), 4, pi_max_iter=MAX_ITER, pi_tol=tol)]
-    args = ([r[0] for r in req], [r[1] for r in req], [r[2] for r in req], [r[3] for r in req], [r[4] for r in req])
+    def request_args(req):
+        return ([r[0] for r in req], [r[1] for r in req], [r[2] for r in req], [r[3] for r in req], [r[4] for r in req])
+    args = request_args(req)
     fs = forests()
     lat, comp, _, info = gx.serve_run(fs, arrival, *args)
     served = {"jobs_per_s": n_req / comp.max(), "p50_s": float(np.percentile(lat, 50)),
@@ -220,13 +222,26 @@ def serve_bench(gx, n_req, rate=200.0, tol=TOL):
         clock = max(clock, a) + dur      # a single server queue
         plain_lat.append(clock - a)
     plain_s = time.perf_counter() - t0
-    return {"config": f"{n_req} requests, Poisson {rate:g} jobs/s, 60% Lorenz rho~U[24,32] depth U{{18..26}} k=27, "
-                      f"40% forced Duffing depth U{{16..24}} k=16; TM + power iteration to {tol:g}; "
-                      f"200 GB admission budget",
-            "served": served, "saturated": saturated,
-            "plain": {"jobs_per_s": n_req / plain_s, "p50_s": float(np.percentile(plain_lat, 50)),
-                      "p99_s": float(np.percentile(plain_lat, 99))},
-            "ratio_saturated_vs_plain": saturated["jobs_per_s"] / (n_req / plain_s)}
+    out = {"config": f"{n_req} requests, Poisson {rate:g} jobs/s, 60% Lorenz rho~U[24,32] depth U{{18..26}} k=27, "
+                     f"40% forced Duffing depth U{{16..24}} k=16; TM + power iteration to {tol:g}; "
+                     f"200 GB admission budget",
+           "served": served, "saturated": saturated,
+           "plain": {"jobs_per_s": n_req / plain_s, "p50_s": float(np.percentile(plain_lat, 50)),
+                     "p99_s": float(np.percentile(plain_lat, 99))},
+           "ratio_saturated_vs_plain": saturated["jobs_per_s"] / (n_req / plain_s)}
+    if n_steady > 0:
+        # a longer stream of the same mix and rate (another seed): the latencies after its first
+        # fifth, and the p99 of each fifth, show whether the queue holds steady
+        arrival, req = serve_requests(n_steady, rate, seed=1)
+        lat, comp, _, info = gx.serve_run(forests(), arrival, *request_args(req))
+        k = n_steady // 5
+        out["steady"] = {"requests": n_steady, "arrival_span_s": float(arrival[-1]),
+                         "jobs_per_s": n_steady / float(comp.max()),
+                         "p50_s": float(np.percentile(lat[k:], 50)), "p99_s": float(np.percentile(lat[k:], 99)),
+                         "p99_per_fifth_s": [float(np.percentile(lat[i:i + k], 99)) for i in range(0, 5 * k, k)],
+                         "max_in_flight": info["max_in_flight"],
+                         "note": "latency percentiles over the requests after the first fifth"}
+    return out
 
 
 HH_C, HH_R = [0.0, 0.0, 0.0, 0.0], [0.6, 0.6, 0.5, 0.5]
@@ -445,6 +460,8 @@ def main():
     ap.add_argument("--cpu1-depth", type=int, default=22, help="depth of the one-thread CPU baseline sample")
     ap.add_argument("--sweep-jobs", type=int, default=512, help="config 4 jobs (0 to skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="config 5 requests (0 to skip)")
+    ap.add_argument("--serve-steady", type=int, default=1000,
+                    help="requests of the longer config-5 stream for steady-state latency (0 to skip)")
     ap.add_argument("--hh-depth", type=int, default=28, help="config 3 depth (0 to skip)")
     ap.add_argument("--hh-iters", type=int, default=400, help="config 3 subspace iterations at most")
     ap.add_argument("--hh-tol", type=float, default=1e-8, help="config 3 subspace residual to stop at")
@@ -541,7 +558,7 @@ def main():
         out["sweep"] = sweep_bench(gx, args.sweep_jobs)
         progress("sweep done")
     if args.serve_requests > 0:
-        out["serving"] = serve_bench(gx, args.serve_requests)
+        out["serving"] = serve_bench(gx, args.serve_requests, n_steady=args.serve_steady)
         progress("serving done")
     threads, nproc, cpu_model = host_cores()
     if args.hh_depth > 0:
