@@ -218,7 +218,7 @@ void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int sca
 template <class Op>
 void runScanSmall(b12x_tree_s& T, const Op& op, uint32_t n, int scalSlot, const uint32_t* nDev, int statSlot,
                   uint32_t* hostOut) {
-  hipLaunchKernelGGL(scanSmallKernel<Op>, dim3(1), dim3(kBlock), 0, T.stream, op, n, nDev, T.scal + scalSlot,
+  hipLaunchKernelGGL(scanSmallKernel<Op>, dim3(1), dim3(kSmallBlock), 0, T.stream, op, n, nDev, T.scal + scalSlot,
                      statSlot >= 0 ? T.scal + statSlot : nullptr, (const uint32_t*)T.scal, hostOut);
   HIPCHK(hipGetLastError());
 }

@@ -36,16 +36,17 @@ __device__ inline uint32_t waveReduceSum(uint32_t v) {
   return v;
 }
 
-// Exclusive scan over the block; returns the exclusive prefix of v and the block total.
+// Exclusive scan over a block of BS threads; returns the exclusive prefix of v and the block total.
+template <int BS = kBlock>
 __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t& total) {
-  __shared__ uint32_t waveSums[kBlock / kWave];
+  __shared__ uint32_t waveSums[BS / kWave];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t inc = waveInclusiveScan(v);
   if (lane == kWave - 1) waveSums[wave] = inc;
   __syncthreads();
   uint32_t before = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < kBlock / kWave; ++w) {
+  for (int w = 0; w < BS / kWave; ++w) {
     const uint32_t s = waveSums[w];
     before += (w < wave) ? s : 0u;
     tot += s;
@@ -55,15 +56,16 @@ __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t& total) {
   return before + inc - v;
 }
 
+template <int BS = kBlock>
 __device__ inline uint32_t blockReduceSum(uint32_t v) {
-  __shared__ uint32_t waveSums[kBlock / kWave];
+  __shared__ uint32_t waveSums[BS / kWave];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   v = waveReduceSum(v);
   if (lane == 0) waveSums[wave] = v;
   __syncthreads();
   uint32_t tot = 0;
 #pragma unroll
-  for (int w = 0; w < kBlock / kWave; ++w) tot += waveSums[w];
+  for (int w = 0; w < BS / kWave; ++w) tot += waveSums[w];
   __syncthreads();
   return tot;
 }
@@ -168,18 +170,21 @@ static __global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32
   }
 }
 
-// The three passes above in one block, for short arrays (a launch instead of three): tiles of
-// kTile elements in order, each scanned in the block and emitted after the carry of the tiles
-// before it. Writes the total to *total and the statistic to *statTotal; with hostOut set, thread
-// 0 then copies scal[0..3] there (a step's counts, read by the host after the stream
-// synchronises).
+// The three passes above in one block of kSmallBlock threads, for short arrays (a launch instead
+// of three): tiles of kSmallTile elements in order, each scanned in the block and emitted after
+// the carry of the tiles before it. Writes the total to *total and the statistic to *statTotal;
+// with hostOut set, thread 0 then copies scal[0..3] there (a step's counts, read by the host
+// after the stream synchronises).
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallTile = kSmallBlock * kItems;
+
 template <class Op>
-static __global__ __launch_bounds__(kBlock) void scanSmallKernel(Op op, uint32_t n, const uint32_t* nDev,
-                                                          uint32_t* total, uint32_t* statTotal,
-                                                          const uint32_t* scal, uint32_t* hostOut) {
+static __global__ __launch_bounds__(kSmallBlock) void scanSmallKernel(Op op, uint32_t n, const uint32_t* nDev,
+                                                               uint32_t* total, uint32_t* statTotal,
+                                                               const uint32_t* scal, uint32_t* hostOut) {
   n = elemCount(n, nDev);
   uint32_t carry = 0, stat = 0;
-  for (uint32_t off = 0; off < n; off += kTile) {
+  for (uint32_t off = 0; off < n; off += kSmallTile) {
     const uint64_t base = uint64_t(off) + uint64_t(threadIdx.x) * kItems;
     uint32_t c[kItems];
     uint32_t s = 0;
@@ -191,7 +196,7 @@ static __global__ __launch_bounds__(kBlock) void scanSmallKernel(Op op, uint32_t
       if (statTotal && i < n) stat += op.stat(uint32_t(i));
     }
     uint32_t tot;
-    uint32_t pos = blockExclusiveScan(s, tot) + carry;
+    uint32_t pos = blockExclusiveScan<kSmallBlock>(s, tot) + carry;
 #pragma unroll
     for (int t = 0; t < kItems; ++t) {
       const uint64_t i = base + t;
@@ -203,7 +208,7 @@ static __global__ __launch_bounds__(kBlock) void scanSmallKernel(Op op, uint32_t
     }
     carry += tot;
   }
-  if (statTotal) stat = blockReduceSum(stat);
+  if (statTotal) stat = blockReduceSum<kSmallBlock>(stat);
   if (threadIdx.x == 0) {
     *total = carry;
     if (statTotal) *statTotal = stat;
