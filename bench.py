@@ -410,8 +410,8 @@ def chain_recurrent_bench(gx, steps, cpu, threads):
 
 
 def host_cores():
-    """The cores this process may use (the GPU box allots a share per GPU and sets OMP_NUM_THREADS
-    to it), the machine's logical CPUs and the CPU model."""
+    """The cores this process may use (OMP_NUM_THREADS when the machine sets the process's share
+    of the host there, else the CPU affinity), the machine's logical CPUs and the CPU model."""
     aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     env = os.environ.get("OMP_NUM_THREADS", "")
     cores = int(env) if env.isdigit() and int(env) > 0 else aff
