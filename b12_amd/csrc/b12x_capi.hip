@@ -2822,7 +2822,7 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
     hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.err, uint64_t(1), 0u);
     // the straight-line form when every wave owns one group and every column fits the cache
     hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, w.maxLen, uint64_t(1), 0u);
-    hipLaunchKernelGGL(colMaxLenKernel, dim3(std::min<uint32_t>(nb, 1024)), dim3(kBlock), 0, st, colPtr, n, w.maxLen);
+    hipLaunchKernelGGL(colMaxLenKernel, dim3(std::min<uint32_t>(nb, 256)), dim3(kBlock), 0, st, colPtr, n, w.maxLen);
     HIPCHK(hipGetLastError());
     uint32_t maxLen = 0;
     HIPCHK(hipMemcpyAsync(&maxLen, w.maxLen, 4, hipMemcpyDeviceToHost, st));

@@ -765,7 +765,10 @@ static __global__ __launch_bounds__(kBlock) void insertMarkKernel(const uint64_t
   ins[i] = in ? 1 : 0;
   insU32[i] = keep ? 1u : 0u;
   if (t != kInvalid && !(form == 0 && in)) hit[t] = 1;
-  if (valid && anyValid) atomicOr(anyValid, 1u);
+  // one atomic per wave with a valid point, none once the flag reads as set
+  if (anyValid && __ballot(valid) && (threadIdx.x & (kWave - 1)) == 0 &&
+      __hip_atomic_load(anyValid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+    atomicOr(anyValid, 1u);
 }
 
 // One-workgroup sort of up to kSmallSortMax new boxes by (comparable key, index): a bitonic
@@ -1059,8 +1062,11 @@ struct TmRowArgs {
 template <int S>
 static __global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
   constexpr int SEGS = kWave / S;
+  __shared__ int blockSawInvalid;
   const int lane = threadIdx.x & (kWave - 1);
   const int seg = lane / S, pos = lane % S;
+  if (threadIdx.x == 0) blockSawInvalid = 0;
+  __syncthreads();
   const uint64_t waveId = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / kWave;
   const uint64_t nWaves = uint64_t(gridDim.x) * kBlock / kWave;
   for (uint64_t grp = waveId; grp * SEGS < a.nRows; grp += nWaves) {
@@ -1094,8 +1100,7 @@ static __global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
       invalidHere = entry && cnt > 0;
       entry = invalidHere && a.keepInvalid;
     }
-    // one atomic per wave that saw an invalid target (same-address atomics serialise chip-wide)
-    if (a.pass == 0 && __ballot(invalidHere) && lane == 0) atomicOr(a.anyInvalid, 1u);
+    if (a.pass == 0 && __ballot(invalidHere) && lane == 0) blockSawInvalid = 1;
     const uint64_t entryMask = __ballot(entry) & segMask;
     if (a.pass == 0) {
       if (pos == 0 && realRow) a.rowCount[row] = uint32_t(__popcll(entryMask));
@@ -1106,6 +1111,12 @@ static __global__ __launch_bounds__(kBlock) void tmRowsKernel(TmRowArgs a) {
       a.outCnt[dst] = cnt;
     }
   }
+  // at most one atomic per block, and none once the flag reads as set (same-address atomics
+  // serialise chip-wide: one per wave took 70 ms for 18 M rows)
+  __syncthreads();
+  if (threadIdx.x == 0 && blockSawInvalid &&
+      __hip_atomic_load(a.anyInvalid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+    atomicOr(a.anyInvalid, 1u);
 }
 
 // General transition-matrix rows for any k: (row, col) pairs sorted row-major by two stable radix
@@ -1142,7 +1153,11 @@ static __global__ __launch_bounds__(kBlock) void rowOfPointKernel(uint32_t* row,
 static __global__ __launch_bounds__(kBlock) void anyInvalidKernel(const uint32_t* col, uint64_t n, uint32_t* flag) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   const bool inv = i < n && col[i] == 0xFFFFFFFFu;
-  if (__ballot(inv) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1u);
+  // one atomic per wave that saw one, none once the flag reads as set (same-address atomics
+  // serialise chip-wide)
+  if (__ballot(inv) && (threadIdx.x & (kWave - 1)) == 0 &&
+      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+    atomicOr(flag, 1u);
 }
 
 // Expand CSR row pointers into the COO row index per entry (rowBase + the local row).
@@ -2314,13 +2329,20 @@ static __global__ __launch_bounds__(kBlock) void copyKernel(const uint4* in, uin
     out[i] = in[i];
 }
 
-// *out = max(colPtr[j + 1] - colPtr[j]) over the columns (out zero at launch)
+// *out = max(colPtr[j + 1] - colPtr[j]) over the columns (out zero at launch); one atomic per block
 static __global__ __launch_bounds__(kBlock) void colMaxLenKernel(const uint32_t* colPtr, uint32_t n, uint32_t* out) {
+  __shared__ uint32_t waveMax[kBlock / kWave];
   uint32_t m = 0;
   for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) m = max(m, colPtr[j + 1] - colPtr[j]);
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(out, m);
+  if ((threadIdx.x & (kWave - 1)) == 0) waveMax[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 1; w < kBlock / kWave; ++w) m = max(m, waveMax[w]);
+    if (m) atomicMax(out, m);
+  }
 }
 
 static __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* z, const double* mu0, uint32_t n,
