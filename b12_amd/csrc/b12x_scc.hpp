@@ -110,7 +110,9 @@ static __global__ __launch_bounds__(kBlock) void sccColorKernel(const uint32_t* 
       grew = true;
     }
   }
-  if (changed && __any(grew) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(changed, 1u);
+  if (changed && __any(grew) && (threadIdx.x & (kWave - 1)) == 0 &&
+      __hip_atomic_load(changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+    atomicOr(changed, 1u);  // none once the flag reads as set (same-address atomics serialise)
 }
 
 // the roots: unassigned vertices whose colour is their own index (frontier state 1)
@@ -138,7 +140,9 @@ static __global__ __launch_bounds__(kBlock) void sccBackKernel(const uint32_t* i
       grew = true;
     }
   }
-  if (changed && __any(grew) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(changed, 1u);
+  if (changed && __any(grew) && (threadIdx.x & (kWave - 1)) == 0 &&
+      __hip_atomic_load(changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+    atomicOr(changed, 1u);  // none once the flag reads as set (same-address atomics serialise)
 }
 
 // the marked vertices take their colour as label (and colour 0); *left counts the vertices
