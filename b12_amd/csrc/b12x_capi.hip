@@ -331,7 +331,8 @@ __global__ __launch_bounds__(kBlock) void leafStatsKernel(const uint8_t* flags, 
 // any difference between the first n leaves of two trees sets *diff
 __global__ void sameLeavesKernel(TreeView a, TreeView b, uint32_t n, uint32_t* diff) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n && (a.key[i] != b.key[i] || a.depth[i] != b.depth[i] || a.flags[i] != b.flags[i])) atomicOr(diff, 1u);
+  const bool d = i < n && (a.key[i] != b.key[i] || a.depth[i] != b.depth[i] || a.flags[i] != b.flags[i]);
+  setFlagOncePerWave(d, diff);
 }
 
 LeafStats leafStats(b12x_tree_s& T) {
@@ -2367,15 +2368,15 @@ __global__ void gatherU32Kernel(const uint32_t* src, const uint32_t* idx, uint64
 // entries out of order for (major, minor) ascending (equal neighbours are in order)
 __global__ void cooOrderKernel(const uint32_t* major, const uint32_t* minor, uint64_t n, uint32_t* bad) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i == 0 || i >= n) return;
-  if (major[i] < major[i - 1] || (major[i] == major[i - 1] && minor[i] < minor[i - 1])) atomicOr(bad, 1u);
+  const bool b = i > 0 && i < n && (major[i] < major[i - 1] || (major[i] == major[i - 1] && minor[i] < minor[i - 1]));
+  setFlagOncePerWave(b, bad);
 }
 
 // strictly ascending (no repeated entry): the order the matrix flags record
 __global__ void cooStrictKernel(const uint32_t* major, const uint32_t* minor, uint64_t n, uint32_t* bad) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i == 0 || i >= n) return;
-  if (!(major[i] > major[i - 1] || (major[i] == major[i - 1] && minor[i] > minor[i - 1]))) atomicOr(bad, 1u);
+  const bool b = i > 0 && i < n && !(major[i] > major[i - 1] || (major[i] == major[i - 1] && minor[i] > minor[i - 1]));
+  setFlagOncePerWave(b, bad);
 }
 
 bool cooStrict(const b12x_coo_s& M, bool rowMajor) {

@@ -56,6 +56,15 @@ __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t& total) {
   return before + inc - v;
 }
 
+// *flag = 1 when some lane of the wave has `set`: one atomic per wave, none once the flag reads as
+// set (same-address atomics serialise chip-wide). Lane 0 of the wave must call it; lanes that have
+// left count as not set.
+__device__ inline void setFlagOncePerWave(bool set, uint32_t* flag) {
+  if (__ballot(set) && (threadIdx.x & (kWave - 1)) == 0 &&
+      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+    atomicOr(flag, 1u);
+}
+
 template <int BS = kBlock>
 __device__ inline uint32_t blockReduceSum(uint32_t v) {
   __shared__ uint32_t waveSums[BS / kWave];
@@ -765,10 +774,8 @@ static __global__ __launch_bounds__(kBlock) void insertMarkKernel(const uint64_t
   ins[i] = in ? 1 : 0;
   insU32[i] = keep ? 1u : 0u;
   if (t != kInvalid && !(form == 0 && in)) hit[t] = 1;
-  // one atomic per wave with a valid point, none once the flag reads as set
-  if (anyValid && __ballot(valid) && (threadIdx.x & (kWave - 1)) == 0 &&
-      __hip_atomic_load(anyValid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
-    atomicOr(anyValid, 1u);
+  // lanes past P have returned; lane 0 holds the wave's smallest index, so it is here if any is
+  if (anyValid) setFlagOncePerWave(valid, anyValid);
 }
 
 // One-workgroup sort of up to kSmallSortMax new boxes by (comparable key, index): a bitonic
@@ -1152,12 +1159,7 @@ static __global__ __launch_bounds__(kBlock) void rowOfPointKernel(uint32_t* row,
 
 static __global__ __launch_bounds__(kBlock) void anyInvalidKernel(const uint32_t* col, uint64_t n, uint32_t* flag) {
   const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  const bool inv = i < n && col[i] == 0xFFFFFFFFu;
-  // one atomic per wave that saw one, none once the flag reads as set (same-address atomics
-  // serialise chip-wide)
-  if (__ballot(inv) && (threadIdx.x & (kWave - 1)) == 0 &&
-      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
-    atomicOr(flag, 1u);
+  setFlagOncePerWave(i < n && col[i] == 0xFFFFFFFFu, flag);
 }
 
 // Expand CSR row pointers into the COO row index per entry (rowBase + the local row).
