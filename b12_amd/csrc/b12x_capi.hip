@@ -93,6 +93,7 @@ struct b12x_tree_s {
   uint8_t minDepthC = 0, maxDepthC = 0;
   uint64_t hostBufferBytes = uint64_t(4) << 30;  // nHostBufferBytes_ (ImplicitBoxTree.hpp:38)
   uint64_t tmPassBytes = 0;  // scratch bound of a transition-matrix pass (0: a third of the free HBM)
+  size_t hscalBytes = 0;     // size of the pinned block hscal
 
   TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
   ~b12x_tree_s() {
@@ -112,10 +113,10 @@ struct b12x_tree_s {
     dfree(samples);
     dfree(srcKey);
     dfree(srcDepth);
-    if (hscal) (void)hipHostFree(hscal);
+    if (hscal) ObjectPool::get().givePinned(hscal, hscalBytes);
     for (auto& p : pending) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
     for (auto e : eventPool) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
+    streamRelease(stream);
   }
 };
 
@@ -133,7 +134,7 @@ struct b12x_coo_s {
     dfree(rows);
     dfree(cols);
     dfree(vals);
-    if (stream) (void)hipStreamDestroy(stream);
+    streamRelease(stream);
   }
 };
 
@@ -1021,14 +1022,16 @@ int b12x_tree_create(int dim, const double* center, const double* radius, const 
     try {
       streamCreate(&T->stream);
       T->scal = dalloc<uint32_t>(kScal);
-      HIPCHK(hipHostMalloc((void**)&T->hscal, kHostScal * sizeof(uint32_t), hipHostMallocDefault));
+      T->hscalBytes = kHostScal * sizeof(uint32_t);
+      T->hscal = static_cast<uint32_t*>(ObjectPool::get().takePinned(T->hscalBytes));
       T->rootKey = dalloc<uint64_t>(1);
       T->rootDepth = dalloc<uint8_t>(1);
       const uint64_t rk = keyOf(0, 0);
       const uint8_t z = 0;
-      HIPCHK(hipMemcpy(T->rootKey, &rk, 8, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(T->rootDepth, &z, 1, hipMemcpyHostToDevice));
-      insertRootIfEmpty(*T);
+      HIPCHK(hipMemcpyAsync(T->rootKey, &rk, 8, hipMemcpyHostToDevice, T->stream));
+      HIPCHK(hipMemcpyAsync(T->rootDepth, &z, 1, hipMemcpyHostToDevice, T->stream));
+      insertRootIfEmpty(*T);  // synchronises the stream: rk and z have been read
+      HIPCHK(hipStreamSynchronize(T->stream));
     } catch (...) {
       delete T;
       throw;
@@ -3120,7 +3123,7 @@ struct b12x_krylov_s {
     dfree(part);
     dfree(h);
     dfree(Q);
-    if (stream) (void)hipStreamDestroy(stream);
+    streamRelease(stream);
   }
 
   // out[0..cols) = x_c . v over the first cols basis columns (v: n doubles on the device)

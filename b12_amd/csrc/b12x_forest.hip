@@ -104,7 +104,7 @@ struct Pool {
     colPtr = jobOfPos = rowIdx = jobOfBlock = nullptr;
     y = partAbs = partRes = nullptr;
     dJobs = nullptr;
-    if (stream) (void)hipStreamDestroy(stream);
+    streamRelease(stream);
     stream = nullptr;
   }
 };
@@ -158,7 +158,7 @@ struct b12x_forest_s {
     dfree(hit); dfree(totals); dfree(scal); dfree(dTable); dfree(dJobs); dfree(dOff); dfree(dEnd); dfree(dRetiring);
     for (auto& j : jobs) dfree(j.dForcing);
     if (hscal) (void)hipHostFree(hscal);
-    if (stream) (void)hipStreamDestroy(stream);
+    streamRelease(stream);
   }
 };
 

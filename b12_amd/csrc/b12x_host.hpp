@@ -268,14 +268,83 @@ class DevScope {
   std::vector<void*> p_;
 };
 
+// Streams and pinned host blocks of released objects, kept per device for the next object:
+// creating and destroying them costs tens of microseconds each, which a tree and a matrix per
+// job would pay every job. A pooled stream is idle (its object retired it); a pooled block is
+// no longer read by the device.
+class ObjectPool {
+ public:
+  static ObjectPool& get() {
+    static ObjectPool* p = new ObjectPool();  // lives to the end of the process
+    return *p;
+  }
+  hipStream_t takeStream() {
+    const int dev = device();
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      for (size_t i = streams_.size(); i-- > 0;)
+        if (streams_[i].first == dev) {
+          const hipStream_t s = streams_[i].second;
+          streams_.erase(streams_.begin() + i);
+          return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreate(&s));
+    return s;
+  }
+  void giveStream(hipStream_t s) {
+    const int dev = device();
+    std::lock_guard<std::mutex> lock(mu_);
+    if (streams_.size() < kKeep) streams_.push_back({dev, s});
+    else (void)hipStreamDestroy(s);
+  }
+  void* takePinned(size_t bytes) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      for (size_t i = pinned_.size(); i-- > 0;)
+        if (pinned_[i].first == bytes) {
+          void* q = pinned_[i].second;
+          pinned_.erase(pinned_.begin() + i);
+          return q;
+        }
+    }
+    void* q = nullptr;
+    HIPCHK(hipHostMalloc(&q, bytes, hipHostMallocDefault));
+    return q;
+  }
+  void givePinned(void* q, size_t bytes) {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (pinned_.size() < kKeep) pinned_.push_back({bytes, q});
+    else (void)hipHostFree(q);
+  }
+
+ private:
+  static constexpr size_t kKeep = 64;
+  static int device() {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) {
+      (void)hipGetLastError();
+      d = 0;
+    }
+    return d;
+  }
+  std::mutex mu_;
+  std::vector<std::pair<int, hipStream_t>> streams_;
+  std::vector<std::pair<size_t, void*>> pinned_;
+};
+
 // a stream of one of the library's objects: registered for the cache's fences on creation,
-// synchronised and retired before the object frees its memory
+// synchronised and retired before the object frees its memory, then released to the pool
 inline void streamCreate(hipStream_t* s) {
-  HIPCHK(hipStreamCreate(s));
+  *s = ObjectPool::get().takeStream();
   DeviceCache::get().addStream(*s);
 }
 inline void streamRetire(hipStream_t s) {
   if (s) DeviceCache::get().retireStream(s);
+}
+inline void streamRelease(hipStream_t s) {
+  if (s) ObjectPool::get().giveStream(s);
 }
 
 inline uint32_t ceilDiv(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
