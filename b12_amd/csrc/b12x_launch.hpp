@@ -12,9 +12,10 @@
 
 namespace b12x {
 
-// Exclusive scan of a device uint32 array in place on a stream; returns the total (synchronises).
+// Exclusive scan of a device uint32 array in place on a stream; returns the total (synchronises),
+// or, with readTotal false, leaves it in *dTotal and returns 0 without synchronising.
 inline uint32_t scanArray(hipStream_t st, uint32_t* v, uint64_t n, uint32_t* scratchTotals, uint32_t* dTotal,
-                   uint32_t* hTotal) {
+                          uint32_t* hTotal, bool readTotal = true) {
   if (n == 0) return 0;
   if (n > 0xFFFFFFFFull) throw HipFail(B12X_E_ARG, "scan too large");
   const uint32_t nb = ceilDiv(n, kTile);
@@ -26,6 +27,7 @@ inline uint32_t scanArray(hipStream_t st, uint32_t* v, uint64_t n, uint32_t* scr
                      dTotal, (const uint32_t*)nullptr, (uint32_t*)nullptr);
   hipLaunchKernelGGL(scanArrayScatterKernel, dim3(nb), dim3(kBlock), 0, st, v, uint32_t(n), scratchTotals);
   HIPCHK(hipGetLastError());
+  if (!readTotal) return 0;
   HIPCHK(hipMemcpyAsync(hTotal, dTotal, 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return *hTotal;
@@ -64,7 +66,7 @@ inline void radixSort3(hipStream_t st, uint32_t*& k, uint32_t*& p0, uint32_t*& p
   uint32_t* dTot = dalloc<uint32_t>(1);
   for (int shift = 0; shift < bits; shift += 8) {
     hipLaunchKernelGGL(radixHistKernel, dim3(nb), dim3(kBlock), 0, st, k, n, shift, hist, nb);
-    scanArray(st, hist, nh, tt, dTot, hTotal);
+    scanArray(st, hist, nh, tt, dTot, hTotal, false);  // the total is n: no read-back
     hipLaunchKernelGGL(radixScatterKernel, dim3(nb), dim3(kBlock), 0, st, k, p0, p1, n, shift, hist, nb, k2, a2, b2);
     HIPCHK(hipGetLastError());
     std::swap(k, k2);
