@@ -2800,6 +2800,8 @@ struct PiWork {
 // false. mu (host, n doubles) is the start vector when muInit, and receives the measure.
 // spinLimit bounds every wait of the persistent form (polls); timed records the persistent
 // launch's duration with HIP events. With useGraph and timed false nothing is allocated here.
+constexpr size_t kPinnedStageMax = size_t(64) << 20;  // measures larger than this copy directly
+
 void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const double* vals, uint32_t n,
              int maxIter, double tol, int muInit, double* mu, int* iters, double* resid, const PiWork& w,
              uint32_t workers, bool useGraph, uint32_t spinLimit, bool timed) {
@@ -2906,8 +2908,25 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
                            (const PiState*)w.pst, w.dmu);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&hs, w.pst, sizeof(hs), hipMemcpyDeviceToHost, st));
-  if (n) HIPCHK(hipMemcpyAsync(mu, w.dmu, size_t(n) * 8, hipMemcpyDeviceToHost, st));
+  // the measure to the caller's (pageable) memory: through a pooled pinned block when the call
+  // may allocate (`timed`, the allocating entry point), one DMA instead of a staged copy
+  const size_t muBytes = size_t(n) * 8;
+  const bool staged = timed && n && muBytes <= kPinnedStageMax;
+  void* pin = nullptr;
+  size_t pinBytes = 0;
+  if (staged) {
+    pinBytes = size_t(1) << 16;
+    while (pinBytes < muBytes) pinBytes <<= 1;
+    pin = ObjectPool::get().takePinned(pinBytes);
+    HIPCHK(hipMemcpyAsync(pin, w.dmu, muBytes, hipMemcpyDeviceToHost, st));
+  } else if (n) {
+    HIPCHK(hipMemcpyAsync(mu, w.dmu, muBytes, hipMemcpyDeviceToHost, st));
+  }
   HIPCHK(hipStreamSynchronize(st));
+  if (staged) {
+    std::memcpy(mu, pin, muBytes);
+    ObjectPool::get().givePinned(pin, pinBytes);
+  }
   if (iters) *iters = int(hs.final);
   if (resid) *resid = hs.resid;
 }
