@@ -2826,29 +2826,37 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
     hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(uint64_t(kFlowTotWords), kBlock)), dim3(kBlock), 0, st, w.tot,
                        uint64_t(kFlowTotWords), kCoopEmpty);
     hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.err, uint64_t(1), 0u);
-    // the straight-line form when every wave owns one group and every column fits the cache
+    // the straight-line form when every wave owns one group and every column fits the cache;
+    // the kernel itself checks the columns (the longest one is computed on the device, so no
+    // host round trip precedes the launch) and leaves at once if one is longer
     hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, w.maxLen, uint64_t(1), 0u);
     hipLaunchKernelGGL(colMaxLenKernel, dim3(std::min<uint32_t>(nb, 256)), dim3(kBlock), 0, st, colPtr, n, w.maxLen);
     HIPCHK(hipGetLastError());
-    uint32_t maxLen = 0;
-    HIPCHK(hipMemcpyAsync(&maxLen, w.maxLen, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    const bool full = maxLen > uint32_t(kCoopCache) || uint64_t(nG) > 4ull * workers;
-    lastPiForm = full ? 1 : 0;
-    PiFlowArgs fa{colPtr, rows, vals, w.mu0, w.ring, w.part, w.tot, w.pst, w.err, n, nG, workers, mi, tol, spinLimit};
+    bool full = uint64_t(nG) > 4ull * workers;
+    PiFlowArgs fa{colPtr, rows, vals, w.mu0, w.ring, w.part, w.tot, w.pst, w.err, n, nG, workers, mi, tol, spinLimit,
+                  w.maxLen};
     hipEvent_t e0 = nullptr, e1 = nullptr;  // the launch's duration, for measurement
     if (timed) {
       HIPCHK(hipEventCreate(&e0));
       HIPCHK(hipEventCreate(&e1));
-      HIPCHK(hipEventRecord(e0, st));
     }
-    if (full) hipLaunchKernelGGL(piFlowKernel<true>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
-    else hipLaunchKernelGGL(piFlowKernel<false>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
-    HIPCHK(hipGetLastError());
-    if (timed) HIPCHK(hipEventRecord(e1, st));
     int herr = 0;
-    HIPCHK(hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if (timed) HIPCHK(hipEventRecord(e0, st));
+      if (full) hipLaunchKernelGGL(piFlowKernel<true>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
+      else hipLaunchKernelGGL(piFlowKernel<false>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
+      HIPCHK(hipGetLastError());
+      if (timed) HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (herr != kFlowNeedsSweep || full) break;
+      // a column longer than the cache: every block left before touching the state; the
+      // sweeping form runs from the same start
+      full = true;
+      hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.err, uint64_t(1), 0u);
+      HIPCHK(hipGetLastError());
+    }
+    lastPiForm = full ? 1 : 0;
     if (timed) {
       float kms = 0.f;
       HIPCHK(hipEventElapsedTime(&kms, e0, e1));

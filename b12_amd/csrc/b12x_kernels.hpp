@@ -1516,7 +1516,10 @@ struct PiFlowArgs {
   uint32_t maxIter;
   double tol;
   uint32_t spinLimit;
+  const uint32_t* maxLen;  // longest column (the straight-line form checks it against its cache)
 };
+
+constexpr int kFlowNeedsSweep = 2;  // *err when a column is longer than the straight-line form caches
 
 template <class T>
 __device__ inline T ldAgent(const T* p) {
@@ -1742,12 +1745,20 @@ __device__ inline bool flowStops(const PiFlowArgs& a, uint32_t t, double rho) {
   return t > 2 * kFlowLag && ((a.tol >= 0.0 && rho <= a.tol) || t - 2 * kFlowLag >= a.maxIter);
 }
 
-// FULL = false: every wave owns one group and no column has more than kCoopCache entries (the
-// host checks both), so an iteration is straight-line code; FULL = true: waves sweep several
-// groups and long columns read their further entries from the matrix every iteration.
+// FULL = false: every wave owns one group (the host checks) and no column has more than
+// kCoopCache entries (every block checks *maxLen first and, if one has, all leave at once with
+// *err = kFlowNeedsSweep, before any wait), so an iteration is straight-line code; FULL = true:
+// waves sweep several groups and long columns read their further entries from the matrix every
+// iteration.
 template <bool FULL>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void piFlowKernel(PiFlowArgs a) {
   const uint32_t b = blockIdx.x, lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if constexpr (!FULL) {
+    if (*a.maxLen > uint32_t(kCoopCache)) {
+      if (b == 0 && threadIdx.x == 0) stAgent(a.err, kFlowNeedsSweep);
+      return;
+    }
+  }
   if (b == a.workers) {  // the aggregator: one wave
     if (w == 0) piFlowAggregator(a);
     return;
