@@ -264,6 +264,9 @@ def Duffing(delta=0.25, alpha=-1.0, beta=1.0, gamma=0.3, omega=1.0, t0=0.0, t_en
     return _Map(MAP_DUFFING, 2, [delta, alpha, beta, gamma, omega, t0, t_end, h])
 
 
+_SCHEMES = {}  # (dim, grid kind, grid params, map kind, map params, flavour) -> Scheme
+
+
 class Scheme:
     """A (Grid, Map) pair on the device."""
 
@@ -480,7 +483,6 @@ class ImplicitBoxTree:
         self.api("b12x_tree_create", self.dim, _ptr(c), _ptr(r), _ptr(sd), 0 if sd is None else len(sd),
                  ctypes.byref(h))
         self.h = h
-        self._schemes = {}
 
     @classmethod
     def load(cls, file_name, dim, flavour="parity"):
@@ -490,7 +492,6 @@ class ImplicitBoxTree:
         obj.api = _Api(flavour)
         obj.flavour = flavour
         obj.dim = int(dim)
-        obj._schemes = {}
         h = _P()
         obj.api("b12x_tree_load", os.fsencode(file_name), obj.dim, ctypes.byref(h))
         obj.h = h
@@ -749,10 +750,13 @@ class ImplicitBoxTree:
 
     # -- point iterators
     def scheme(self, grid, mp):
-        key = (grid.kind, tuple(grid.params), mp.kind, tuple(mp.params))
-        if key not in self._schemes:
-            self._schemes[key] = Scheme(self.dim, grid, mp, self.flavour)
-        return self._schemes[key]
+        # schemes are immutable device tables: one per (dimension, grid, map, library) for the
+        # process, so a tree made per job does not rebuild them
+        key = (self.dim, grid.kind, tuple(grid.params), mp.kind, tuple(mp.params), self.flavour)
+        sch = _SCHEMES.get(key)
+        if sch is None:
+            sch = _SCHEMES[key] = Scheme(self.dim, grid, mp, self.flavour)
+        return sch
 
     def makePointIteratorPairOverLeaves(self, grid, mp):
         return PointIterator(self.scheme(grid, mp), LEAF)
