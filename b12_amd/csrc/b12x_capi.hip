@@ -40,6 +40,17 @@ struct b12x_scheme_s {
   }
 };
 
+// tags of pooled graph executables (ObjectPool): a tree's large and small step graphs
+constexpr int kGraphKindStep = 0;
+// B12X_GRAPH_REUSE=0: every tree instantiates its step graphs anew
+inline bool graphReuse() {
+  static const bool on = [] {
+    const char* e = std::getenv("B12X_GRAPH_REUSE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 struct b12x_tree_s {
   int dim = 0;
   int sdEntries = 0;  // entries of the reference's sdScheme_ (64 default, 65 when given)
@@ -98,7 +109,15 @@ struct b12x_tree_s {
   TreeView view(int b) { return {key[b], depth[b], flags[b]}; }
   ~b12x_tree_s() {
     streamRetire(stream);
-    for (auto& g : step) g.drop();
+    // the step graphs' executables go to the pool (the stream has retired: none is running),
+    // where the next tree updates them in place
+    for (int f = 0; f < 2; ++f) {
+      if (step[f].exec && graphReuse()) {
+        ObjectPool::get().giveExec(kGraphKindStep + f, step[f].exec);
+        step[f].exec = nullptr;
+      }
+      step[f].drop();
+    }
     for (int b = 0; b < 2; ++b) {
       dfree(key[b]);
       dfree(depth[b]);
@@ -1960,7 +1979,17 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
             throw;
           }
           HIPCHK(hipStreamEndCapture(T.stream, &G.graph));
-          HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+          G.exec = graphReuse() ? ObjectPool::get().takeExec(kGraphKindStep + (form == STEP_SMALL ? 1 : 0)) : nullptr;
+          if (G.exec) {  // a released tree's executable: updated in place when the topology matches
+            hipGraphNode_t errNode = nullptr;
+            hipGraphExecUpdateResult r = hipGraphExecUpdateError;
+            if (hipGraphExecUpdate(G.exec, G.graph, &errNode, &r) != hipSuccess || r != hipGraphExecUpdateSuccess) {
+              (void)hipGetLastError();
+              (void)hipGraphExecDestroy(G.exec);
+              G.exec = nullptr;
+            }
+          }
+          if (!G.exec) HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
           G.scheme = s->id;
           G.sd = sd;
           G.gen = T.bufGen;

@@ -268,7 +268,8 @@ class DevScope {
   std::vector<void*> p_;
 };
 
-// Streams and pinned host blocks of released objects, kept per device for the next object:
+// Streams, pinned host blocks and instantiated graphs of released objects, kept per device for
+// the next object:
 // creating and destroying them costs tens of microseconds each, which a tree and a matrix per
 // job would pay every job. A pooled stream is idle (its object retired it); a pooled block is
 // no longer read by the device.
@@ -318,6 +319,25 @@ class ObjectPool {
     if (pinned_.size() < kKeep) pinned_.push_back({bytes, q});
     else (void)hipHostFree(q);
   }
+  // instantiated graphs of a kind (the caller's tag) released with their object, for the next
+  // object to update in place (hipGraphExecUpdate) instead of instantiating anew
+  hipGraphExec_t takeExec(int kind) {
+    const int dev = device();
+    std::lock_guard<std::mutex> lock(mu_);
+    for (size_t i = execs_.size(); i-- > 0;)
+      if (execs_[i].dev == dev && execs_[i].kind == kind) {
+        const hipGraphExec_t e = execs_[i].exec;
+        execs_.erase(execs_.begin() + i);
+        return e;
+      }
+    return nullptr;
+  }
+  void giveExec(int kind, hipGraphExec_t e) {
+    const int dev = device();
+    std::lock_guard<std::mutex> lock(mu_);
+    if (execs_.size() < kKeep) execs_.push_back({dev, kind, e});
+    else (void)hipGraphExecDestroy(e);
+  }
 
  private:
   static constexpr size_t kKeep = 64;
@@ -329,9 +349,14 @@ class ObjectPool {
     }
     return d;
   }
+  struct PooledExec {
+    int dev, kind;
+    hipGraphExec_t exec;
+  };
   std::mutex mu_;
   std::vector<std::pair<int, hipStream_t>> streams_;
   std::vector<std::pair<size_t, void*>> pinned_;
+  std::vector<PooledExec> execs_;
 };
 
 // a stream of one of the library's objects: registered for the cache's fences on creation,
