@@ -237,9 +237,9 @@ void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int sca
 // runScan in one block (short arrays); hostOut: where the block copies scal[0..3] at the end
 template <class Op>
 void runScanSmall(b12x_tree_s& T, const Op& op, uint32_t n, int scalSlot, const uint32_t* nDev, int statSlot,
-                  uint32_t* hostOut) {
+                  uint32_t* hostOut, const StepLogArgs& log = StepLogArgs{}) {
   hipLaunchKernelGGL(scanSmallKernel<Op>, dim3(1), dim3(kSmallBlock), 0, T.stream, op, n, nDev, T.scal + scalSlot,
-                     statSlot >= 0 ? T.scal + statSlot : nullptr, (const uint32_t*)T.scal, hostOut);
+                     statSlot >= 0 ? T.scal + statSlot : nullptr, (const uint32_t*)T.scal, hostOut, log);
   HIPCHK(hipGetLastError());
 }
 
@@ -696,8 +696,11 @@ bool stepGraphsOn(const b12x_tree_s& T) {
   return !(ds && ds[0] == '1');
 }
 
+// the end of a captured step whose keep ran the three-pass scan
+__global__ void stepLogKernel(StepLogArgs g, uint32_t* scal) { logStep(g, scal, scal + 2); }
+
 void enqueueStep(b12x_tree_s& T, b12x_scheme_s& S, uint8_t sd, uint32_t boundN, const uint32_t* countDev,
-                 StepForm form) {
+                 StepForm form, bool logStep = false) {
   SubdivideOp sop{};
   sop.in = T.view(T.cur);
   sop.out = T.view(1 - T.cur);
@@ -740,24 +743,19 @@ void enqueueStep(b12x_tree_s& T, b12x_scheme_s& S, uint8_t sd, uint32_t boundN, 
   kop.hit = T.hit;
   kop.mask = kop.hitFlag = kop.clearMask = uint8_t(F_HIT);
   kop.clearHit = 1;
-  if (form == STEP_SMALL) runScanSmall(T, kop, boundN * 2, 2, T.scal + 0, 3, T.hscal);
-  else runScan(T, kop, boundN * 2 > 0 ? boundN * 2 : 1, T.totals, 2, T.scal + 0, 3);
-  T.cur = 1 - T.cur;
-}
-
-// The end of a captured step: appends the step's counts (scal[0..3]) to the pinned log and, when
-// the step kept no leaf, writes the surrounding box as the only leaf (ImplicitBoxTree.hpp:
-// 1959-1970) and makes the next step read a count of 1.
-__global__ void stepLogKernel(uint32_t* scal, uint32_t* hostLog, TreeView v, uint64_t rootKey) {
-  const uint32_t i = scal[kScalStepLog];
-  scal[kScalStepLog] = i + 1;
-  for (int j = 0; j < 4; ++j) hostLog[4 * i + j] = scal[j];
-  if (scal[2] == 0) {
-    v.key[0] = rootKey;
-    v.depth[0] = 0;
-    v.flags[0] = 0;
-    scal[2] = 1;
+  // the log of a captured step (logStep): folded into the keep's one-block scan, or a launch of
+  // its own after the three-pass scan
+  const StepLogArgs lg{T.scal + kScalStepLog, T.hscal + kScal, kop.out.key, kop.out.depth, kop.out.flags, keyOf(0, 0)};
+  if (form == STEP_SMALL) {
+    runScanSmall(T, kop, boundN * 2, 2, T.scal + 0, 3, T.hscal, logStep ? lg : StepLogArgs{});
+  } else {
+    runScan(T, kop, boundN * 2 > 0 ? boundN * 2 : 1, T.totals, 2, T.scal + 0, 3);
+    if (logStep) {
+      hipLaunchKernelGGL(stepLogKernel, dim3(1), dim3(1), 0, T.stream, lg, T.scal);
+      HIPCHK(hipGetLastError());
+    }
   }
+  T.cur = 1 - T.cur;
 }
 
 // the leaf count where a captured step reads it (the keep of every step leaves it there too)
@@ -1968,10 +1966,7 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
           const uint32_t gb = uint32_t(bound / 2);
           HIPCHK(hipStreamBeginCapture(T.stream, hipStreamCaptureModeThreadLocal));
           try {
-            enqueueStep(T, *s, sd, gb, T.scal + 2, form);
-            hipLaunchKernelGGL(stepLogKernel, dim3(1), dim3(1), 0, T.stream, T.scal, T.hscal + kScal,
-                               T.view(T.cur), keyOf(0, 0));
-            HIPCHK(hipGetLastError());
+            enqueueStep(T, *s, sd, gb, T.scal + 2, form, /*logStep=*/true);
           } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(T.stream, &g);

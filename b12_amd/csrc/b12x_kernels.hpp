@@ -179,18 +179,46 @@ static __global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32
   }
 }
 
+// The end of a subdivision step replayed from a graph: its counts scal[0..3] are appended to a
+// pinned host log (entry *index, which advances), and when the step kept no leaf the surrounding
+// box is written as the only leaf (ImplicitBoxTree.hpp:1959-1970) and *kept becomes 1, so the
+// next step reads a count of 1. index == nullptr: no log.
+struct StepLogArgs {
+  uint32_t* index;
+  uint32_t* hostLog;
+  uint64_t* key;  // the leaf arrays the step wrote
+  uint8_t* depth;
+  uint8_t* flags;
+  uint64_t rootKey;
+};
+
+__device__ inline void logStep(const StepLogArgs& g, const uint32_t* scal, uint32_t* kept) {
+  const uint32_t i = *g.index;
+  *g.index = i + 1;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) g.hostLog[4 * i + j] = scal[j];
+  if (*kept == 0) {
+    g.key[0] = g.rootKey;
+    g.depth[0] = 0;
+    g.flags[0] = 0;
+    *kept = 1;
+  }
+}
+
 // The three passes above in one block of kSmallBlock threads, for short arrays (a launch instead
 // of three): tiles of kSmallTile elements in order, each scanned in the block and emitted after
 // the carry of the tiles before it. Writes the total to *total and the statistic to *statTotal;
 // with hostOut set, thread 0 then copies scal[0..3] there (a step's counts, read by the host
-// after the stream synchronises).
+// after the stream synchronises), and with log.index set it ends the step as logStep does
+// (total is then the step's kept count, scal + 2).
 constexpr int kSmallBlock = 1024;
 constexpr int kSmallTile = kSmallBlock * kItems;
 
 template <class Op>
 static __global__ __launch_bounds__(kSmallBlock) void scanSmallKernel(Op op, uint32_t n, const uint32_t* nDev,
                                                                uint32_t* total, uint32_t* statTotal,
-                                                               const uint32_t* scal, uint32_t* hostOut) {
+                                                               const uint32_t* scal, uint32_t* hostOut,
+                                                               StepLogArgs log) {
   n = elemCount(n, nDev);
   uint32_t carry = 0, stat = 0;
   for (uint32_t off = 0; off < n; off += kSmallTile) {
@@ -225,6 +253,7 @@ static __global__ __launch_bounds__(kSmallBlock) void scanSmallKernel(Op op, uin
 #pragma unroll
       for (int q = 0; q < 4; ++q) hostOut[q] = scal[q];
     }
+    if (log.index) logStep(log, scal, total);
   }
 }
 
