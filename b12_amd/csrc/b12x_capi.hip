@@ -2188,6 +2188,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
       GrowBuf rows, cols, cnt;
       uint64_t nnz = 0;
       uint32_t passes = 0;
+      bool anyInvalid = false;  // scal[2], set by the passes, read with each pass's count
       zeroScal(T);
       for (uint64_t b0 = 0; b0 < nRowsSrc; b0 += boxesPerPass) {
         const uint64_t nb = std::min(boxesPerPass, nRowsSrc - b0), np = nb * k;
@@ -2211,7 +2212,10 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
           a.pass = 0;
           tmRows(a, T.stream);
           HIPCHK(hipMemsetAsync(rowPtr + nb, 0, 4, T.stream));
-          const uint32_t nc = scanArray(T.stream, rowPtr, nb + 1, T.totals, T.scal + 3, T.hscal + 3);
+          scanArray(T.stream, rowPtr, nb + 1, T.totals, T.scal + 3, nullptr, false);
+          syncScal(T, 4);
+          const uint32_t nc = T.hscal[3];
+          anyInvalid = T.hscal[2] != 0;
           cols.reserve(nnz + nc, nnz, T.stream);
           cnt.reserve(nnz + nc, nnz, T.stream);
           rows.reserve(nnz + nc, nnz, T.stream);
@@ -2223,8 +2227,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
           hipLaunchKernelGGL(expandRowsKernel, dim3(ceilDiv(nb, kBlock)), dim3(kBlock), 0, T.stream, rowPtr, nb,
                              uint32_t(b0), rows.p + nnz);
           HIPCHK(hipGetLastError());
-          HIPCHK(hipStreamSynchronize(T.stream));
-          dfree(rowPtr);
+          dfree(rowPtr);  // the cache fences its reuse behind the queued kernels
           nnz += nc;
         } else {
           // any k: stable sort by column, then by row (row-major), then run-length encoding
@@ -2234,8 +2237,8 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
                              uint32_t(b0));
           hipLaunchKernelGGL(anyInvalidKernel, dim3(ceilDiv(np, kBlock)), dim3(kBlock), 0, T.stream,
                              (const uint32_t*)col, np, T.scal + 2);
-          radixSort3(T.stream, col, prow, dummy, np, 32, T.hscal + 4);
-          radixSort3(T.stream, prow, col, dummy, np, bitsFor(nRowsSrc), T.hscal + 4);
+          radixSort3(T.stream, col, prow, dummy, np, 32, T.hscal + 4, false);
+          radixSort3(T.stream, prow, col, dummy, np, bitsFor(nRowsSrc), T.hscal + 4, false);
           dfree(dummy);
           RunLengthOp op{};
           op.row = prow;
@@ -2248,6 +2251,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
           runScan(T, op, uint32_t(np), ptot, 3, nullptr, -1, /*countOnly=*/true);
           syncScal(T, 4);
           const uint32_t nc = T.hscal[3];
+          anyInvalid = T.hscal[2] != 0;
           rows.reserve(nnz + nc, nnz, T.stream);
           cols.reserve(nnz + nc, nnz, T.stream);
           cnt.reserve(nnz + nc, nnz, T.stream);
@@ -2255,8 +2259,7 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
           op.outCol = cols.p + nnz;
           op.outCnt = cnt.p + nnz;
           runScan(T, op, uint32_t(np), ptot, 3);
-          HIPCHK(hipStreamSynchronize(T.stream));
-          dfree(ptot);
+          dfree(ptot);  // the cache fences its reuse behind the queued kernels
           dfree(prow);
           nnz += nc;
         }
@@ -2269,13 +2272,11 @@ int b12x_transition_matrix(b12x_tree t, b12x_scheme s, uint8_t dIt, uint8_t chec
         cols.shrink(nnz, T.stream);
         cnt.shrink(nnz, T.stream);
       }
-      syncScal(T, 3);
-      const bool anyInvalid = T.hscal[2] != 0;
       if (useInvalid && anyInvalid) M->nCols = nBoxes + 1;
       if (!rowMajor) {
         // column-major order (CooMatrix.hpp:95-137): stable LSD radix sort by column of the
         // row-major entries leaves rows ascending within each column
-        radixSort3(T.stream, cols.p, rows.p, cnt.p, nnz, bitsFor(uint64_t(nBoxes) + 1), T.hscal + 4);
+        radixSort3(T.stream, cols.p, rows.p, cnt.p, nnz, bitsFor(uint64_t(nBoxes) + 1), T.hscal + 4, false);
       }
       M->nnz = uint32_t(nnz);
       M->rows = rows.release();
@@ -2642,16 +2643,6 @@ int b12x_coo_copy(b12x_coo m, b12x_coo* out) {
 
 namespace {
 
-__global__ void fillF64Kernel(double* p, uint32_t n, double v) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-__global__ void fillU64Kernel(uint64_t* p, uint64_t n, uint64_t v) {
-  const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 // CSC over the first n columns, rows ascending within each column (a column-major copy of the
 // entries); entries in column n (the useInvalid label) fall past colPtr[n].
 struct Csc {
@@ -2836,24 +2827,27 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
   init.t = 1;
   init.resid = -1.0;
   init.done = (maxIter <= 0 || n == 0) ? 1 : 0;
-  HIPCHK(hipMemcpyAsync(w.pst, &init, sizeof(init), hipMemcpyHostToDevice, st));
   if (muInit && n) HIPCHK(hipMemcpyAsync(w.mu0, mu, size_t(n) * 8, hipMemcpyHostToDevice, st));
-  else if (n) hipLaunchKernelGGL(fillF64Kernel, dim3(nb), dim3(kBlock), 0, st, w.mu0, n, 1.0 / double(n));
   const uint32_t mi = uint32_t(std::max(maxIter, 0));
   const char* pp = std::getenv("B12X_PI_PERSISTENT");
   bool done = init.done != 0;
-  if (!done && workers && !(pp && pp[0] == '0')) {
-    const uint32_t nG = ceilDiv(n, kWave);
-    const uint64_t nRing = uint64_t(kFlowRing) * n, nPart = uint64_t(kFlowPart) * nG * 2;
-    hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nRing, kBlock)), dim3(kBlock), 0, st, w.ring, nRing, kCoopEmpty);
-    hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(nPart, kBlock)), dim3(kBlock), 0, st, w.part, nPart, kCoopEmpty);
-    hipLaunchKernelGGL(fillU64Kernel, dim3(ceilDiv(uint64_t(kFlowTotWords), kBlock)), dim3(kBlock), 0, st, w.tot,
-                       uint64_t(kFlowTotWords), kCoopEmpty);
-    hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.err, uint64_t(1), 0u);
+  const bool flow = !done && workers && !(pp && pp[0] == '0');
+  const uint32_t nG = ceilDiv(n, kWave);
+  const uint64_t nRing = uint64_t(kFlowRing) * n, nPart = uint64_t(kFlowPart) * nG * 2;
+  {
+    // the state, mu_0 and the persistent form's buffers in one launch
+    const PiInitArgs ia{w.pst,  init,  w.mu0, n,     (!muInit && n) ? 1 : 0, flow ? w.ring : nullptr, nRing,
+                        w.part, nPart, w.tot, w.err, w.maxLen};
+    const uint64_t words = std::max<uint64_t>({uint64_t(n), flow ? nRing : 0, flow ? nPart : 0,
+                                               flow ? uint64_t(kFlowTotWords) : 0, 1});
+    hipLaunchKernelGGL(piInitKernel, dim3(uint32_t(std::min<uint64_t>(ceilDiv(words, kBlock), 65536))), dim3(kBlock),
+                       0, st, ia);
+    HIPCHK(hipGetLastError());
+  }
+  if (flow) {
     // the straight-line form when every wave owns one group and every column fits the cache;
     // the kernel itself checks the columns (the longest one is computed on the device, so no
     // host round trip precedes the launch) and leaves at once if one is longer
-    hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, st, w.maxLen, uint64_t(1), 0u);
     hipLaunchKernelGGL(colMaxLenKernel, dim3(std::min<uint32_t>(nb, 256)), dim3(kBlock), 0, st, colPtr, n, w.maxLen);
     HIPCHK(hipGetLastError());
     bool full = uint64_t(nG) > 4ull * workers;

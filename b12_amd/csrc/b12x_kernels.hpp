@@ -2387,6 +2387,44 @@ static __global__ __launch_bounds__(kBlock) void colMaxLenKernel(const uint32_t*
   }
 }
 
+// The start of a solve in one launch: the state, mu_0 = 1/n unless the caller gave it, and for
+// the persistent form (ring != nullptr) its ring, partials and totals at the empty pattern and
+// its error flag and longest-column slot at zero.
+struct PiInitArgs {
+  PiState* st;
+  PiState init;
+  double* mu0;
+  uint32_t n;
+  int fillMu;
+  uint64_t* ring;
+  uint64_t nRing;
+  uint64_t* part;
+  uint64_t nPart;
+  uint64_t* tot;
+  int* err;
+  uint32_t* maxLen;
+};
+
+static __global__ __launch_bounds__(kBlock) void piInitKernel(PiInitArgs a) {
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  const uint64_t first = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (first == 0) {
+    *a.st = a.init;
+    if (a.ring) {
+      *a.err = 0;
+      *a.maxLen = 0;
+    }
+  }
+  if (a.fillMu) {
+    const double v = 1.0 / double(a.n);
+    for (uint64_t i = first; i < a.n; i += stride) a.mu0[i] = v;
+  }
+  if (!a.ring) return;
+  for (uint64_t i = first; i < a.nRing; i += stride) a.ring[i] = kCoopEmpty;
+  for (uint64_t i = first; i < a.nPart; i += stride) a.part[i] = kCoopEmpty;
+  for (uint64_t i = first; i < kFlowTotWords; i += stride) a.tot[i] = kCoopEmpty;
+}
+
 static __global__ __launch_bounds__(kBlock) void piFinishKernel(const double* z, const double* mu0, uint32_t n,
                                                          const PiState* st, double* mu) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;

@@ -52,9 +52,10 @@ inline void tmRows(const TmRowArgs& a, hipStream_t st) {
   HIPCHK(hipGetLastError());
 }
 
-// Stable sort of (key, p0, p1) triples by key using ceil(bits/8) LSD passes.
+// Stable sort of (key, p0, p1) triples by key using ceil(bits/8) LSD passes; with sync false the
+// sort is only enqueued on st (its scratch returns to the cache, which fences reuse).
 inline void radixSort3(hipStream_t st, uint32_t*& k, uint32_t*& p0, uint32_t*& p1, uint64_t n, int bits,
-                uint32_t* hTotal) {
+                uint32_t* hTotal, bool sync = true) {
   if (n <= 1) return;
   uint32_t* k2 = dalloc<uint32_t>(n);
   uint32_t* a2 = dalloc<uint32_t>(n);
@@ -73,7 +74,7 @@ inline void radixSort3(hipStream_t st, uint32_t*& k, uint32_t*& p0, uint32_t*& p
     std::swap(p0, a2);
     std::swap(p1, b2);
   }
-  HIPCHK(hipStreamSynchronize(st));
+  if (sync) HIPCHK(hipStreamSynchronize(st));
   dfree(k2);
   dfree(a2);
   dfree(b2);
