@@ -355,14 +355,23 @@ __global__ void sameLeavesKernel(TreeView a, TreeView b, uint32_t n, uint32_t* d
   setFlagOncePerWave(d, diff);
 }
 
-LeafStats leafStats(b12x_tree_s& T) {
+// the leaves' statistics into scal[8..11] (enqueued only)
+void enqueueLeafStats(b12x_tree_s& T) {
   hipLaunchKernelGGL(statsInitKernel, dim3(1), dim3(1), 0, T.stream, T.scal + 8);
   const uint32_t nb = std::max<uint32_t>(1, std::min<uint32_t>(ceilDiv(T.n, 4 * kBlock), kStatsBlocks));
   hipLaunchKernelGGL(leafStatsKernel, dim3(nb), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.depth[T.cur], T.n,
                      T.scal + 8);
   HIPCHK(hipGetLastError());
-  syncScal(T, 12);
+}
+
+LeafStats leafStatsRead(const b12x_tree_s& T) {
   return {uint8_t(T.hscal[8]), uint8_t(T.hscal[9]), uint8_t(T.hscal[10]), uint8_t(T.hscal[11])};
+}
+
+LeafStats leafStats(b12x_tree_s& T) {
+  enqueueLeafStats(T);
+  syncScal(T, 12);
+  return leafStatsRead(T);
 }
 
 // Depth classes of the B12 API: 0 = root, 255 = leaves; a depth on which all leaves lie
@@ -442,17 +451,24 @@ uint32_t depthFlagsOp(b12x_tree_s& T, const DepthMap& M, const uint8_t* sel, int
   return T.hscal[kScalDepthFlags];
 }
 
+// the surrounding box (key k, depth 0, no flags) as entry 0 of the given arrays (flags optional)
+__global__ void putRootKernel(uint64_t* key, uint8_t* depth, uint8_t* flags, uint64_t k) {
+  key[0] = k;
+  depth[0] = 0;
+  if (flags) flags[0] = 0;
+}
+
 void insertRootIfEmpty(b12x_tree_s& T) {  // ImplicitBoxTree.hpp:1959-1970
   if (T.n) return;
   ensureCap(T, 1);
-  const uint64_t k = keyOf(0, 0);
-  const uint8_t z = 0;
-  HIPCHK(hipMemcpyAsync(T.key[T.cur], &k, 8, hipMemcpyHostToDevice, T.stream));
-  HIPCHK(hipMemcpyAsync(T.depth[T.cur], &z, 1, hipMemcpyHostToDevice, T.stream));
-  HIPCHK(hipMemcpyAsync(T.flags[T.cur], &z, 1, hipMemcpyHostToDevice, T.stream));
-  HIPCHK(hipStreamSynchronize(T.stream));
+  hipLaunchKernelGGL(putRootKernel, dim3(1), dim3(1), 0, T.stream, T.key[T.cur], T.depth[T.cur], T.flags[T.cur],
+                     keyOf(0, 0));
+  HIPCHK(hipGetLastError());
   T.n = 1;
   ++T.leafGen;
+  // the only leaf lies at depth 0
+  T.minDepthC = T.maxDepthC = 0;
+  T.rangeGen = T.leafGen;
 }
 
 // -- point iterators --------------------------------------------------------------------------
@@ -886,11 +902,18 @@ void checkScheme(b12x_tree_s& T, b12x_scheme_s* S) {
 }
 
 // Flag operation over a selector (nullptr = all leaves); returns the reference's count.
-uint32_t applyFlags(b12x_tree_s& T, uint8_t* sel, bool clearSel, int op, uint8_t a, uint8_t b) {
+// the flag operation on the leaves (selected by sel when set); its count lands in scal[0].
+// withStats: the leaves' statistics before the operation land in scal[8..11] too.
+void enqueueFlags(b12x_tree_s& T, uint8_t* sel, bool clearSel, int op, uint8_t a, uint8_t b, bool withStats) {
   zeroScal(T);
+  if (withStats) enqueueLeafStats(T);
   hipLaunchKernelGGL(flagsKernel, dim3(ceilDiv(T.n, kBlock)), dim3(kBlock), 0, T.stream, T.flags[T.cur], T.n, sel,
                      clearSel ? 1 : 0, op, a, b, T.scal);
   HIPCHK(hipGetLastError());
+}
+
+uint32_t applyFlags(b12x_tree_s& T, uint8_t* sel, bool clearSel, int op, uint8_t a, uint8_t b) {
+  enqueueFlags(T, sel, clearSel, op, a, b, false);
   syncScal(T, 1);
   return T.hscal[0];
 }
@@ -913,12 +936,11 @@ uint32_t stencilOp(b12x_tree_s& T, int op, const uint8_t* hostStencil, uint8_t a
     dfree(sel);
     return r;
   }
-  if (dc == DC_ROOT) {
-    const bool on = hostStencil ? hostStencil[0] != 0 : true;
-    const LeafStats s = leafStats(T);
-    const uint32_t r = (on && rootPred(op, s, a)) ? 1u : 0u;
-    if (on) applyFlags(T, nullptr, false, op, a, b);
-    return r;
+  if (dc == DC_ROOT) {  // the count is the root's, from the statistics before the operation
+    if (hostStencil && hostStencil[0] == 0) return 0;
+    enqueueFlags(T, nullptr, false, op, a, b, /*withStats=*/true);
+    syncScal(T, 12);
+    return rootPred(op, leafStatsRead(T), a) ? 1u : 0u;
   }
   uint8_t* sel = nullptr;
   if (hostStencil) {
@@ -1043,11 +1065,10 @@ int b12x_tree_create(int dim, const double* center, const double* radius, const 
       T->hscal = static_cast<uint32_t*>(ObjectPool::get().takePinned(T->hscalBytes));
       T->rootKey = dalloc<uint64_t>(1);
       T->rootDepth = dalloc<uint8_t>(1);
-      const uint64_t rk = keyOf(0, 0);
-      const uint8_t z = 0;
-      HIPCHK(hipMemcpyAsync(T->rootKey, &rk, 8, hipMemcpyHostToDevice, T->stream));
-      HIPCHK(hipMemcpyAsync(T->rootDepth, &z, 1, hipMemcpyHostToDevice, T->stream));
-      insertRootIfEmpty(*T);  // synchronises the stream: rk and z have been read
+      hipLaunchKernelGGL(putRootKernel, dim3(1), dim3(1), 0, T->stream, T->rootKey, T->rootDepth, (uint8_t*)nullptr,
+                         keyOf(0, 0));
+      HIPCHK(hipGetLastError());
+      insertRootIfEmpty(*T);
       HIPCHK(hipStreamSynchronize(T->stream));
     } catch (...) {
       delete T;
