@@ -164,7 +164,8 @@ constexpr int kScal = 32;
 // subdivision steps (4 counts per step)
 constexpr int kStepBatch = 16;
 constexpr int kHostScal = kScal + 4 * kStepBatch;
-constexpr int kScalStepLog = 25;  // device slot: steps logged in the current batch
+constexpr int kScalStepLog = 25;    // device slot: steps logged in the current batch
+constexpr int kScalStepsLeft = 27;  // device slot: subdivides left in a small-form batch
 constexpr int kMaxScanTiles = 1 << 24;
 
 void syncScal(b12x_tree_s& T, int count = kScal) {
@@ -235,14 +236,6 @@ void runScan(b12x_tree_s& T, const Op& op, uint32_t n, uint32_t* totals, int sca
 }
 
 // runScan in one block (short arrays); hostOut: where the block copies scal[0..3] at the end
-template <class Op>
-void runScanSmall(b12x_tree_s& T, const Op& op, uint32_t n, int scalSlot, const uint32_t* nDev, int statSlot,
-                  uint32_t* hostOut, const StepLogArgs& log = StepLogArgs{}) {
-  hipLaunchKernelGGL(scanSmallKernel<Op>, dim3(1), dim3(kSmallBlock), 0, T.stream, op, n, nDev, T.scal + scalSlot,
-                     statSlot >= 0 ? T.scal + statSlot : nullptr, (const uint32_t*)T.scal, hostOut, log);
-  HIPCHK(hipGetLastError());
-}
-
 struct IndexOp : OpBase {  // indices i with sel(i) — flag-filtered source lists and searchCompacted
   const uint8_t* flags;
   const uint8_t* hit;
@@ -696,8 +689,8 @@ void launchImage(int dim, const ImageArgs& a, int gridKind, hipStream_t st) {
 // remove(HIT) and unsetFlags(HIT, -1) as one stable compaction. boundN bounds the leaves before
 // the step; countDev holds their count on the device (nullptr: boundN is the count).
 // The captured forms hold a fixed image variant with a capped grid (a tree that times its images
-// launches eagerly); the small form scans in one block and its last kernel copies the step's
-// counts to the pinned scalars, the large form copies them with a memcpy node.
+// launches eagerly); the small form keeps and subdivides in one block, in place, and logs the
+// step's counts there; the large form scans in tiles and logs them with a one-thread kernel.
 enum StepForm { STEP_EAGER, STEP_LARGE, STEP_SMALL };
 constexpr uint32_t kStepGraphLeaves = 1u << 17;  // bound of the large form's subdivided count
 constexpr uint32_t kStepSmallLeaves = 1u << 14;  // bound of the small form's subdivided count
@@ -721,8 +714,7 @@ void enqueueStep(b12x_tree_s& T, b12x_scheme_s& S, uint8_t sd, uint32_t boundN, 
   sop.in = T.view(T.cur);
   sop.out = T.view(1 - T.cur);
   sop.mask = sd;
-  if (form == STEP_SMALL) runScanSmall(T, sop, boundN, 0, countDev, 1, nullptr);
-  else runScan(T, sop, boundN, T.totals, 0, countDev, 1);
+  runScan(T, sop, boundN, T.totals, 0, countDev, 1);
   T.cur = 1 - T.cur;
   ImageArgs a{};
   a.dom = T.dom;
@@ -759,19 +751,42 @@ void enqueueStep(b12x_tree_s& T, b12x_scheme_s& S, uint8_t sd, uint32_t boundN, 
   kop.hit = T.hit;
   kop.mask = kop.hitFlag = kop.clearMask = uint8_t(F_HIT);
   kop.clearHit = 1;
-  // the log of a captured step (logStep): folded into the keep's one-block scan, or a launch of
-  // its own after the three-pass scan
-  const StepLogArgs lg{T.scal + kScalStepLog, T.hscal + kScal, kop.out.key, kop.out.depth, kop.out.flags, keyOf(0, 0)};
-  if (form == STEP_SMALL) {
-    runScanSmall(T, kop, boundN * 2, 2, T.scal + 0, 3, T.hscal, logStep ? lg : StepLogArgs{});
-  } else {
-    runScan(T, kop, boundN * 2 > 0 ? boundN * 2 : 1, T.totals, 2, T.scal + 0, 3);
-    if (logStep) {
-      hipLaunchKernelGGL(stepLogKernel, dim3(1), dim3(1), 0, T.stream, lg, T.scal);
-      HIPCHK(hipGetLastError());
-    }
+  runScan(T, kop, boundN * 2 > 0 ? boundN * 2 : 1, T.totals, 2, T.scal + 0, 3);
+  if (logStep) {  // the end of a captured step
+    const StepLogArgs lg{T.scal + kScalStepLog, T.hscal + kScal, kop.out.key, kop.out.depth, kop.out.flags,
+                         keyOf(0, 0)};
+    hipLaunchKernelGGL(stepLogKernel, dim3(1), dim3(1), 0, T.stream, lg, T.scal);
+    HIPCHK(hipGetLastError());
   }
   T.cur = 1 - T.cur;
+}
+
+// The one-block keep of a small step (keepSubdivideKernel) on the current half, in place; with
+// hit == nullptr the batch's first subdivide, which also sets the batch's step count and log.
+void launchKeepSubdivide(b12x_tree_s& T, uint8_t sd, uint8_t* hit, uint32_t stepsInit) {
+  const KeepSubArgs k{T.view(T.cur), hit, T.scal, T.scal + kScalStepsLeft, stepsInit, T.scal + kScalStepLog,
+                      T.hscal + kScal, sd, keyOf(0, 0)};
+  hipLaunchKernelGGL(keepSubdivideKernel, dim3(1), dim3(kSmallBlock), 0, T.stream, k);
+  HIPCHK(hipGetLastError());
+}
+
+// The captured small step: the image of the subdivided leaves (the current half, their count in
+// scal[0]) and the keep that subdivides for the next step of the batch, both in place, so the
+// step ends in the half it started in and replays with no host input.
+void enqueueSmallStep(b12x_tree_s& T, b12x_scheme_s& S, uint8_t sd) {
+  static_assert(kStepSmallLeaves <= kKeepSubMax, "a small step's leaves fit one block's keep");
+  ImageArgs a{};
+  a.dom = T.dom;
+  a.map = S.map;
+  a.grid = S.grid;
+  a.key = a.srcKey = T.key[T.cur];
+  a.depth = a.srcDepth = T.depth[T.cur];
+  a.nPoints = uint64_t(kStepSmallLeaves) * S.grid.k;  // grid bound; the kernel reads the count
+  a.nBoxesDev = T.scal + 0;
+  a.mode = 0;
+  a.hit = T.hit;
+  launchImageFixed(T.dim, a, S.grid.kind, T.stream, kStepGraphBlocks);
+  launchKeepSubdivide(T, sd, T.hit, 0);
 }
 
 // the leaf count where a captured step reads it (the keep of every step leaves it there too)
@@ -1987,7 +2002,8 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
           const uint32_t gb = uint32_t(bound / 2);
           HIPCHK(hipStreamBeginCapture(T.stream, hipStreamCaptureModeThreadLocal));
           try {
-            enqueueStep(T, *s, sd, gb, T.scal + 2, form, /*logStep=*/true);
+            if (form == STEP_SMALL) enqueueSmallStep(T, *s, sd);
+            else enqueueStep(T, *s, sd, gb, T.scal + 2, form, /*logStep=*/true);
           } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(T.stream, &g);
@@ -2011,8 +2027,12 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
           G.gen = T.bufGen;
           G.cur = T.cur;
         }
-        hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, T.stream, T.scal + kScalStepLog, uint64_t(1), 0u);
-        HIPCHK(hipGetLastError());
+        if (form == STEP_SMALL) {
+          launchKeepSubdivide(T, sd, nullptr, uint32_t(K - 1));  // the first step's subdivide
+        } else {
+          hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, T.stream, T.scal + kScalStepLog, uint64_t(1), 0u);
+          HIPCHK(hipGetLastError());
+        }
         for (int k = 0; k < K; ++k) HIPCHK(hipGraphLaunch(G.exec, T.stream));
         HIPCHK(hipStreamSynchronize(T.stream));
         uint32_t kept = 0;

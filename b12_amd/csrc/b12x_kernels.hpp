@@ -182,7 +182,7 @@ static __global__ __launch_bounds__(kBlock) void scanScatterKernel(Op op, uint32
 // The end of a subdivision step replayed from a graph: its counts scal[0..3] are appended to a
 // pinned host log (entry *index, which advances), and when the step kept no leaf the surrounding
 // box is written as the only leaf (ImplicitBoxTree.hpp:1959-1970) and *kept becomes 1, so the
-// next step reads a count of 1. index == nullptr: no log.
+// next step reads a count of 1.
 struct StepLogArgs {
   uint32_t* index;
   uint32_t* hostLog;
@@ -205,57 +205,7 @@ __device__ inline void logStep(const StepLogArgs& g, const uint32_t* scal, uint3
   }
 }
 
-// The three passes above in one block of kSmallBlock threads, for short arrays (a launch instead
-// of three): tiles of kSmallTile elements in order, each scanned in the block and emitted after
-// the carry of the tiles before it. Writes the total to *total and the statistic to *statTotal;
-// with hostOut set, thread 0 then copies scal[0..3] there (a step's counts, read by the host
-// after the stream synchronises), and with log.index set it ends the step as logStep does
-// (total is then the step's kept count, scal + 2).
-constexpr int kSmallBlock = 1024;
-constexpr int kSmallTile = kSmallBlock * kItems;
-
-template <class Op>
-static __global__ __launch_bounds__(kSmallBlock) void scanSmallKernel(Op op, uint32_t n, const uint32_t* nDev,
-                                                               uint32_t* total, uint32_t* statTotal,
-                                                               const uint32_t* scal, uint32_t* hostOut,
-                                                               StepLogArgs log) {
-  n = elemCount(n, nDev);
-  uint32_t carry = 0, stat = 0;
-  for (uint32_t off = 0; off < n; off += kSmallTile) {
-    const uint64_t base = uint64_t(off) + uint64_t(threadIdx.x) * kItems;
-    uint32_t c[kItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int t = 0; t < kItems; ++t) {
-      const uint64_t i = base + t;
-      c[t] = i < n ? op.count(uint32_t(i)) : 0u;
-      s += c[t];
-      if (statTotal && i < n) stat += op.stat(uint32_t(i));
-    }
-    uint32_t tot;
-    uint32_t pos = blockExclusiveScan<kSmallBlock>(s, tot) + carry;
-#pragma unroll
-    for (int t = 0; t < kItems; ++t) {
-      const uint64_t i = base + t;
-      if (i < n) {
-        if (c[t]) op.emit(uint32_t(i), c[t], pos);
-        op.after(uint32_t(i));
-      }
-      pos += c[t];
-    }
-    carry += tot;
-  }
-  if (statTotal) stat = blockReduceSum<kSmallBlock>(stat);
-  if (threadIdx.x == 0) {
-    *total = carry;
-    if (statTotal) *statTotal = stat;
-    if (hostOut) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) hostOut[q] = scal[q];
-    }
-    if (log.index) logStep(log, scal, total);
-  }
-}
+constexpr int kSmallBlock = 1024;  // threads of the one-block step kernels
 
 struct TreeView {
   uint64_t* key;
@@ -317,6 +267,127 @@ struct KeepOp : OpBase {
     out.flags[pos] = uint8_t(flagsAt(i) & ~clearMask);
   }
 };
+
+// The end of a captured small subdivision step in one block, in place: the keep of step t
+// (remove(HIT) with the hit bytes counting as HIT, then unsetFlags(HIT): ImplicitBoxTree.hpp:
+// 2343-2371, 3334-3411) and, while the batch has a step left, the subdivide(sd) of step t + 1
+// (:2883-2930) in the same pass: a kept leaf emits its two halves, or itself when it lacks sd or
+// lies at the maximum depth, at its scan position. With hit == nullptr it is the batch's first
+// subdivide alone (every leaf stays with its flags). Every thread reads all its leaves before the
+// block's scan barriers and writes only after them, so the output overwrites the input safely.
+// At most kKeepSubMax leaves in and out. Counts: scal[0] leaves after the subdivide, scal[1]
+// leaves with sd, scal[2] leaves kept, scal[3] leaves newly hit. A keep appends its step's entry
+// (scal[0], scal[1], kept, newly hit) to the pinned log before scal[0..1] move on to the next
+// step, and when it kept no leaf the surrounding box takes their place (ImplicitBoxTree.hpp:
+// 1959-1970), subdivided like any kept leaf.
+constexpr int kKeepSubItems = 16;
+constexpr uint32_t kKeepSubMax = uint32_t(kSmallBlock) * kKeepSubItems;
+
+struct KeepSubArgs {
+  TreeView v;           // the leaves, read and rewritten in place
+  uint8_t* hit;         // the step's hit bytes (cleared here); nullptr: the batch's first subdivide
+  uint32_t* scal;
+  uint32_t* stepsLeft;  // subdivides left in the batch: a keep takes one; the first subdivide sets it
+  uint32_t stepsInit;
+  uint32_t* logIndex;   // entries in the pinned log: a keep appends one; the first subdivide resets it
+  uint32_t* hostLog;
+  uint8_t sd;
+  uint64_t rootKey;
+};
+
+// the leaf (k, d, f) as c = 1 or 2 leaves at pos (two: its halves)
+__device__ inline void emitLeaf(const TreeView& v, uint32_t pos, uint64_t k, int d, uint8_t f, uint32_t c) {
+  if (c == 1) {
+    v.key[pos] = k;
+    v.depth[pos] = uint8_t(d);
+    v.flags[pos] = f;
+  } else {
+    const uint64_t b = bitsOf(k, d) << 1;
+    v.key[pos] = keyOf(b, d + 1);
+    v.key[pos + 1] = keyOf(b | 1ull, d + 1);
+    v.depth[pos] = v.depth[pos + 1] = uint8_t(d + 1);
+    v.flags[pos] = v.flags[pos + 1] = f;
+  }
+}
+
+static __global__ __launch_bounds__(kSmallBlock) void keepSubdivideKernel(KeepSubArgs a) {
+  const bool keepPass = a.hit != nullptr;
+  const uint32_t n = min(keepPass ? a.scal[0] : a.scal[2], kKeepSubMax);
+  const bool doSub = keepPass ? (*a.stepsLeft > 0) : true;
+  const uint32_t base = threadIdx.x * uint32_t(kKeepSubItems);
+  uint64_t key[kKeepSubItems];
+  uint32_t packed[kKeepSubItems];  // depth | flags << 8 | count << 16
+  uint32_t s = 0, kept = 0, fresh = 0, withSd = 0;
+#pragma unroll
+  for (int t = 0; t < kKeepSubItems; ++t) {
+    const uint32_t i = base + uint32_t(t);
+    key[t] = 0;
+    packed[t] = 0;
+    if (i < n) {
+      key[t] = a.v.key[i];
+      const uint32_t d = a.v.depth[i];
+      uint8_t f = a.v.flags[i];
+      bool keep = true;
+      if (keepPass) {
+        const bool h = a.hit[i] != 0;
+        fresh += (h && !areFlagsSet(f, F_HIT)) ? 1u : 0u;
+        keep = h || areFlagsSet(f, F_HIT);
+        f = uint8_t(f & ~uint8_t(F_HIT));
+      }
+      uint32_t c = 0;
+      if (keep) {
+        ++kept;
+        const bool sdSet = doSub && areFlagsSet(f, a.sd);
+        withSd += sdSet ? 1u : 0u;
+        c = (sdSet && d < uint32_t(kMaxDepth)) ? 2u : 1u;
+      }
+      s += c;
+      packed[t] = d | (uint32_t(f) << 8) | (c << 16);
+    }
+  }
+  uint32_t nOut = 0;
+  uint32_t pos = blockExclusiveScan<kSmallBlock>(s, nOut);
+  kept = blockReduceSum<kSmallBlock>(kept);
+  fresh = blockReduceSum<kSmallBlock>(fresh);
+  withSd = blockReduceSum<kSmallBlock>(withSd);
+  // every leaf of the block has been read (the barriers above): write in place
+#pragma unroll
+  for (int t = 0; t < kKeepSubItems; ++t) {
+    const uint32_t c = packed[t] >> 16;
+    if (c) emitLeaf(a.v, pos, key[t], int(packed[t] & 0xFFu), uint8_t((packed[t] >> 8) & 0xFFu), c);
+    pos += c;
+    if (keepPass && base + uint32_t(t) < n) a.hit[base + t] = 0;
+  }
+  if (threadIdx.x == 0) {
+    if (kept == 0) {  // only a keep can empty the tree: the surrounding box, with no flags
+      const bool sdSet = doSub && areFlagsSet(uint8_t(0), a.sd);
+      const uint32_t c = sdSet ? 2u : 1u;
+      emitLeaf(a.v, 0, a.rootKey, 0, 0, c);
+      nOut = c;
+      withSd = sdSet ? 1u : 0u;
+    }
+    if (keepPass) {
+      const uint32_t li = *a.logIndex;
+      *a.logIndex = li + 1;
+      a.hostLog[4 * li + 0] = a.scal[0];
+      a.hostLog[4 * li + 1] = a.scal[1];
+      a.hostLog[4 * li + 2] = kept;
+      a.hostLog[4 * li + 3] = fresh;
+      a.scal[2] = kept ? kept : 1u;
+      a.scal[3] = fresh;
+      if (doSub) {
+        a.scal[0] = nOut;
+        a.scal[1] = withSd;
+        *a.stepsLeft -= 1;
+      }
+    } else {
+      a.scal[0] = nOut;
+      a.scal[1] = withSd;
+      *a.stepsLeft = a.stepsInit;
+      *a.logIndex = 0;
+    }
+  }
+}
 
 // unsubdivide (ImplicitBoxTree.hpp:3413-3461): first pass marks and lifts unsubdividable leaves
 // (Functors.hpp:337-377 over the predecessor/successor stencil of ImplicitBoxTree.hpp:2023-2058);
