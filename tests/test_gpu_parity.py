@@ -172,6 +172,37 @@ def test_subdivision_step_graph_matches_eager_launches(monkeypatch, name, center
         assert res["1"][0][-1][3] > 65536
 
 
+@pytest.mark.parametrize("sd_flag", [gx.SD, 0])
+def test_subdivision_steps_that_empty_the_tree(monkeypatch, sd_flag):
+    """Steps whose images all leave the domain keep no leaf, so the surrounding box comes back as
+    the only leaf (ImplicitBoxTree.hpp:1959-1970) and the next step starts from it. With the SD
+    flag the box, which carries no flags, stays whole, and the images outside the domain land
+    on it (a tree whose only leaf is the surrounding box reports them there), so it is kept;
+    with no flag (0) every leaf, the box included, is subdivided again and nothing is kept. The
+    captured steps (several in one batch), the eager launches and the oracle agree on every
+    step's counts and on the leaves."""
+    print(f"[subdivision] emptied tree, sd flag {sd_flag}", flush=True)
+    center, radius = [10.0, 10.0], [1.0, 1.0]  # the Hénon map sends the whole box outside it
+    grid, mp, steps = gx.InnerGrid(4), gx.Henon(), 6
+    res = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("B12X_GRAPHS", graphs)
+        g = gx.ImplicitBoxTree(center, radius)
+        g.setFlags(gx.SD, 0)
+        res[graphs] = (g.subdivision_steps(grid, mp, steps, sd_flag), g.download())
+    o = OracleTree(center, radius)
+    o.setFlags(gx.SD, 0)
+    co = o.subdivision_steps(grid, mp, steps, sd_flag)
+    assert co[0, 3] == 0  # the first step keeps no leaf
+    assert co[0, 0] == 1 and (co[1:, 0] == (1 if sd_flag == 0 else 0)).all()
+    if sd_flag == 0:
+        assert (co[:, 3] == 0).all()
+    ref = o.download()
+    for graphs in ("1", "0"):
+        assert np.array_equal(res[graphs][0], co), (graphs, res[graphs][0], co)
+        assert same_tree(res[graphs][1], ref), graphs
+
+
 def test_stepwise_api_matches_fused_step():
     """The four reference calls one by one give the same cover as the fused step."""
     g1, o = both([0.0, 0.0], [1.5, 0.5])
