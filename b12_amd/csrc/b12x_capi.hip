@@ -2737,6 +2737,22 @@ Csc adoptCsc(Csc C, DevScope& ds) {
   return C;
 }
 
+// The CSC of a matrix for reading only: a column-major matrix already holds the rows and values
+// in CSC order, so only the column offsets are made (owned by ds); other matrices are sorted into
+// a copy (buildCsc).
+Csc readCsc(b12x_coo_s& M, uint32_t n, DevScope& ds) {
+  if (!M.colMajor) return adoptCsc(buildCsc(M, n), ds);
+  Csc C;
+  C.keys = M.cols;
+  C.rows = M.rows;
+  C.vals = M.vals;
+  C.colPtr = ds.alloc<uint32_t>(uint64_t(n) + 1);
+  hipLaunchKernelGGL(compressKernel, dim3(ceilDiv(uint64_t(n) + 1, kBlock)), dim3(kBlock), 0, M.stream,
+                     (const uint32_t*)M.cols, uint64_t(M.nnz), n, C.colPtr);
+  HIPCHK(hipGetLastError());
+  return C;
+}
+
 // Subspace iteration (the oracle's definition; every dot product in dotPartialKernel's strided
 // order, which the oracle reproduces): X_0 = cholQR2(start); for k = 0, 1, ...: Y = X_k P, the
 // Ritz matrix H = X_k^T Y and, when a tolerance is set or k = maxIter, the residual
@@ -3095,7 +3111,7 @@ int b12x_power_iteration(b12x_coo m, int maxIter, double tol, int muInit, double
     const uint32_t n = m->nRows;
     hipStream_t st = m->stream;
     DevScope ds;
-    Csc C = adoptCsc(buildCsc(*m, n), ds);
+    const Csc C = readCsc(*m, n, ds);
     PiWork w;
     w.mu0 = ds.alloc<double>(n);
     w.ybuf = ds.alloc<double>(uint64_t(n) * 3);
