@@ -2901,6 +2901,25 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
                        0, st, ia);
     HIPCHK(hipGetLastError());
   }
+  const size_t muBytes = size_t(n) * 8;
+  constexpr size_t kEarlyState = 64, kEarlyHead = 256;  // pinned block: error word, state, measure
+  static_assert(kEarlyState + sizeof(PiState) <= kEarlyHead, "the state fits before the measure");
+  // the one-synchronisation read-back of the persistent form (timed calls): a pooled pinned
+  // block, returned when the solve ends (after the stream has drained, also on an error)
+  struct PinnedLease {
+    hipStream_t st = nullptr;
+    char* p = nullptr;
+    size_t bytes = 0;
+    ~PinnedLease() {
+      if (!p) return;
+      (void)hipStreamSynchronize(st);
+      ObjectPool::get().givePinned(p, bytes);
+    }
+  } lease;
+  lease.st = st;
+  char*& early = lease.p;
+  size_t& earlyBytes = lease.bytes;
+  bool earlyDone = false;  // the measure, state and error word arrived with the launch's synchronisation
   if (flow) {
     // the straight-line form when every wave owns one group and every column fits the cache;
     // the kernel itself checks the columns (the longest one is computed on the device, so no
@@ -2915,6 +2934,15 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
       HIPCHK(hipEventCreate(&e0));
       HIPCHK(hipEventCreate(&e1));
     }
+    // The allocating entry point (`timed`) reads the outcome, the state and the measure back in
+    // one synchronisation: the measure is formed and copied right behind the launch into a pooled
+    // pinned block (error word, state, measure) and kept when no wait gave up; otherwise the
+    // paths below recompute it.
+    if (timed && n && kEarlyHead + muBytes <= kPinnedStageMax) {
+      earlyBytes = size_t(1) << 16;
+      while (earlyBytes < kEarlyHead + muBytes) earlyBytes <<= 1;
+      early = static_cast<char*>(ObjectPool::get().takePinned(earlyBytes));
+    }
     int herr = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
       if (timed) HIPCHK(hipEventRecord(e0, st));
@@ -2922,8 +2950,19 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
       else hipLaunchKernelGGL(piFlowKernel<false>, dim3(workers + 1), dim3(kBlock), 0, st, fa);
       HIPCHK(hipGetLastError());
       if (timed) HIPCHK(hipEventRecord(e1, st));
-      HIPCHK(hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
+      if (early) {
+        hipLaunchKernelGGL(piFinishKernel, dim3(nb), dim3(kBlock), 0, st, (const double*)w.ring,
+                           (const double*)w.mu0, n, (const PiState*)w.pst, w.dmu);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(early, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(early + kEarlyState, w.pst, sizeof(PiState), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(early + kEarlyHead, w.dmu, muBytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::memcpy(&herr, early, sizeof(int));
+      } else {
+        HIPCHK(hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+      }
       if (herr != kFlowNeedsSweep || full) break;
       // a column longer than the cache: every block left before touching the state; the
       // sweeping form runs from the same start
@@ -2944,6 +2983,7 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
     if (herr && pp && pp[0] == '2') throw HipFail(B12X_E_HIP, "persistent power iteration: a wait passed its limit");
     if (!herr) {
       done = true;
+      earlyDone = early != nullptr;
     } else {
       lastPiKernelMs = 0.0;
       lastPiForm = -1;
@@ -2986,6 +3026,14 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
   }
+  if (earlyDone) {
+    PiState hs{};
+    std::memcpy(&hs, early + kEarlyState, sizeof(PiState));
+    std::memcpy(mu, early + kEarlyHead, muBytes);
+    if (iters) *iters = int(hs.final);
+    if (resid) *resid = hs.resid;
+    return;
+  }
   PiState hs{};
   if (n) hipLaunchKernelGGL(piFinishKernel, dim3(nb), dim3(kBlock), 0, st, zbase, (const double*)w.mu0, n,
                            (const PiState*)w.pst, w.dmu);
@@ -2993,7 +3041,6 @@ void piSolve(hipStream_t st, const uint32_t* colPtr, const uint32_t* rows, const
   HIPCHK(hipMemcpyAsync(&hs, w.pst, sizeof(hs), hipMemcpyDeviceToHost, st));
   // the measure to the caller's (pageable) memory: through a pooled pinned block when the call
   // may allocate (`timed`, the allocating entry point), one DMA instead of a staged copy
-  const size_t muBytes = size_t(n) * 8;
   const bool staged = timed && n && muBytes <= kPinnedStageMax;
   void* pin = nullptr;
   size_t pinBytes = 0;
