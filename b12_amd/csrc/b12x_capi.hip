@@ -697,6 +697,14 @@ constexpr uint32_t kStepSmallLeaves = 1u << 14;  // bound of the small form's su
 constexpr uint32_t kStepGraphBlocks = 8192;      // image blocks of a captured step (they stride)
 constexpr int kScalStepIn = 20;                  // pinned slot staging the leaf count
 
+// Batched steps are launched on the stream (back to back, the device idle only while a kernel
+// drains); B12X_STEP_LAUNCH=graph replays a captured step instead, which measured slower: the
+// device idles about 8 µs between two graph launches.
+bool stepLaunchStream() {
+  const char* e = std::getenv("B12X_STEP_LAUNCH");
+  return !(e && std::strcmp(e, "graph") == 0);
+}
+
 bool stepGraphsOn(const b12x_tree_s& T) {
   if (T.timing) return false;  // image timing brackets each eager image launch with events
   const char* ge = std::getenv("B12X_GRAPHS");
@@ -1988,16 +1996,17 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
                             : sub <= kStepGraphLeaves ? STEP_LARGE
                                                       : STEP_EAGER;
       if (form != STEP_EAGER) {
-        // replay the captured step K times back to back: the leaf count chains through scal[2]
-        // on the device and the buffers end in the same half after a step, so the steps need no
-        // host input; the leaves can at most double per step, so K keeps every step's
-        // subdivided count within the form's bound
+        // K steps back to back, launched on the stream or replayed from a captured step: the
+        // leaf count chains through device scalars and the buffers end in the same half after a
+        // step, so the steps need no host input; the leaves can at most double per step, so K
+        // keeps every step's subdivided count within the form's bound
         const uint64_t bound = form == STEP_SMALL ? kStepSmallLeaves : kStepGraphLeaves;
         int K = 1;
         while (K < kStepBatch && step + K < nSteps && (uint64_t(n) << (K + 1)) <= bound) ++K;
         ensureCap(T, kStepGraphLeaves);
+        const bool viaStream = stepLaunchStream();
         b12x_tree_s::StepGraph& G = T.step[form == STEP_SMALL ? 1 : 0];
-        if (!G.exec || G.scheme != s->id || G.sd != sd || G.gen != T.bufGen || G.cur != T.cur) {
+        if (!viaStream && (!G.exec || G.scheme != s->id || G.sd != sd || G.gen != T.bufGen || G.cur != T.cur)) {
           G.drop();
           const uint32_t gb = uint32_t(bound / 2);
           HIPCHK(hipStreamBeginCapture(T.stream, hipStreamCaptureModeThreadLocal));
@@ -2033,7 +2042,11 @@ int b12x_subdivision_steps(b12x_tree t, b12x_scheme s, uint8_t sd, int nSteps, u
           hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(kBlock), 0, T.stream, T.scal + kScalStepLog, uint64_t(1), 0u);
           HIPCHK(hipGetLastError());
         }
-        for (int k = 0; k < K; ++k) HIPCHK(hipGraphLaunch(G.exec, T.stream));
+        for (int k = 0; k < K; ++k) {
+          if (!viaStream) HIPCHK(hipGraphLaunch(G.exec, T.stream));
+          else if (form == STEP_SMALL) enqueueSmallStep(T, *s, sd);
+          else enqueueStep(T, *s, sd, uint32_t(bound / 2), T.scal + 2, form, /*logStep=*/true);
+        }
         HIPCHK(hipStreamSynchronize(T.stream));
         uint32_t kept = 0;
         for (int k = 0; k < K; ++k) {

@@ -156,20 +156,23 @@ def test_subdivision_duffing_cover():
     ("lorenz", [0.0, 0.0, 27.0], [30.0, 30.0, 30.0], gx.InnerGrid(3), gx.Lorenz(), 26),
 ])
 def test_subdivision_step_graph_matches_eager_launches(monkeypatch, name, center, radius, grid, mp, steps):
-    """The captured subdivision step (levels whose subdivided count fits the graph bound) and
-    the eager launches give the same per-step counts and covers; the Lorenz run crosses the
-    bound (more than 65536 leaves from depth 25 on), so both forms alternate within it."""
-    print(f"[subdivision graph] {name} d{steps}", flush=True)
+    """The batched subdivision steps (levels whose subdivided count fits a batch form's bound),
+    launched on the stream or replayed from a captured graph, and the eager launches with a host
+    synchronisation per step give the same per-step counts and covers; the Lorenz run crosses
+    the bound (more than 65536 leaves from depth 25 on), so the forms alternate within it."""
+    print(f"[subdivision batches] {name} d{steps}", flush=True)
     res = {}
-    for graphs in ("1", "0"):
+    for mode, graphs, launch in (("stream", "1", "stream"), ("graph", "1", "graph"), ("eager", "0", "stream")):
         monkeypatch.setenv("B12X_GRAPHS", graphs)
+        monkeypatch.setenv("B12X_STEP_LAUNCH", launch)
         g = gx.ImplicitBoxTree(center, radius)
         g.setFlags(gx.SD, 0)
-        res[graphs] = (g.subdivision_steps(grid, mp, steps), g.download())
-    assert np.array_equal(res["1"][0], res["0"][0])
-    assert same_tree(res["1"][1], res["0"][1])
+        res[mode] = (g.subdivision_steps(grid, mp, steps), g.download())
+    for mode in ("graph", "eager"):
+        assert np.array_equal(res["stream"][0], res[mode][0]), mode
+        assert same_tree(res["stream"][1], res[mode][1]), mode
     if name == "lorenz":
-        assert res["1"][0][-1][3] > 65536
+        assert res["stream"][0][-1][3] > 65536
 
 
 @pytest.mark.parametrize("sd_flag", [gx.SD, 0])
@@ -179,17 +182,18 @@ def test_subdivision_steps_that_empty_the_tree(monkeypatch, sd_flag):
     flag the box, which carries no flags, stays whole, and the images outside the domain land
     on it (a tree whose only leaf is the surrounding box reports them there), so it is kept;
     with no flag (0) every leaf, the box included, is subdivided again and nothing is kept. The
-    captured steps (several in one batch), the eager launches and the oracle agree on every
-    step's counts and on the leaves."""
+    batched steps (several in one batch, on the stream or from a captured graph), the eager
+    launches and the oracle agree on every step's counts and on the leaves."""
     print(f"[subdivision] emptied tree, sd flag {sd_flag}", flush=True)
     center, radius = [10.0, 10.0], [1.0, 1.0]  # the Hénon map sends the whole box outside it
     grid, mp, steps = gx.InnerGrid(4), gx.Henon(), 6
     res = {}
-    for graphs in ("1", "0"):
+    for mode, graphs, launch in (("stream", "1", "stream"), ("graph", "1", "graph"), ("eager", "0", "stream")):
         monkeypatch.setenv("B12X_GRAPHS", graphs)
+        monkeypatch.setenv("B12X_STEP_LAUNCH", launch)
         g = gx.ImplicitBoxTree(center, radius)
         g.setFlags(gx.SD, 0)
-        res[graphs] = (g.subdivision_steps(grid, mp, steps, sd_flag), g.download())
+        res[mode] = (g.subdivision_steps(grid, mp, steps, sd_flag), g.download())
     o = OracleTree(center, radius)
     o.setFlags(gx.SD, 0)
     co = o.subdivision_steps(grid, mp, steps, sd_flag)
@@ -198,9 +202,9 @@ def test_subdivision_steps_that_empty_the_tree(monkeypatch, sd_flag):
     if sd_flag == 0:
         assert (co[:, 3] == 0).all()
     ref = o.download()
-    for graphs in ("1", "0"):
-        assert np.array_equal(res[graphs][0], co), (graphs, res[graphs][0], co)
-        assert same_tree(res[graphs][1], ref), graphs
+    for mode in res:
+        assert np.array_equal(res[mode][0], co), (mode, res[mode][0], co)
+        assert same_tree(res[mode][1], ref), mode
 
 
 def test_stepwise_api_matches_fused_step():
